@@ -1,0 +1,121 @@
+// gsa_common.h — shared host/device helpers of libgsa_hip.so (gfx950 only).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include <type_traits>
+#include "../../include/gsa_hip.h"
+
+namespace gsa {
+
+// ---------------------------------------------------------------- errors ----
+void set_error(int code, const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+#define GSA_HIP(call)                                                        \
+    do {                                                                     \
+        hipError_t _e = (call);                                              \
+        if (_e != hipSuccess) return ::gsa::hip_fail(_e, #call);             \
+    } while (0)
+
+#define GSA_CHECK(cond, code, ...)                                           \
+    do {                                                                     \
+        if (!(cond)) { ::gsa::set_error((code), __VA_ARGS__); return (code); } \
+    } while (0)
+
+#define GSA_TRY(expr)                                                        \
+    do { int _r = (expr); if (_r != GSA_OK) return _r; } while (0)
+
+int ensure_init();
+
+// Stream-ordered scratch memory from the device's default pool (kept cached by
+// gsa_init, so repeated calls do not return memory to the driver).
+int ws_alloc(void** p, size_t bytes, hipStream_t s);
+void ws_free(void* p, hipStream_t s);
+
+// RAII scratch buffer; frees on the stream it was taken on.
+struct Scratch {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    Scratch() = default;
+    explicit Scratch(hipStream_t st) : s(st) {}
+    int alloc(size_t bytes) { return ws_alloc(&p, bytes, s); }
+    template <typename U> U* as() const { return reinterpret_cast<U*>(p); }
+    ~Scratch() { if (p) ws_free(p, s); }
+    Scratch(const Scratch&) = delete;
+    Scratch& operator=(const Scratch&) = delete;
+};
+
+inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// Grid size for grid-stride element loops: enough waves to fill 256 CUs.
+inline unsigned grid_for(int64_t n, int block) {
+    int64_t g = cdiv(n > 0 ? n : 1, block);
+    if (g > 256 * 32) g = 256 * 32;
+    return (unsigned)g;
+}
+
+// ------------------------------------------------------- type dispatch ----
+template <typename F>
+int dispatch_v(int vtype, F&& f) {
+    switch (vtype) {
+        case GSA_F64: return f(double{});
+        case GSA_F32: return f(float{});
+        default: set_error(GSA_ERR_UNSUPPORTED, "unsupported value type %d", vtype);
+                 return GSA_ERR_UNSUPPORTED;
+    }
+}
+template <typename F>
+int dispatch_i(int itype, F&& f) {
+    switch (itype) {
+        case GSA_I64: return f(int64_t{});
+        case GSA_I32: return f(int32_t{});
+        default: set_error(GSA_ERR_UNSUPPORTED, "unsupported index type %d", itype);
+                 return GSA_ERR_UNSUPPORTED;
+    }
+}
+template <typename F>
+int dispatch_vi(int vtype, int itype, F&& f) {
+    return dispatch_v(vtype, [&](auto tv) {
+        return dispatch_i(itype, [&](auto ti) { return f(tv, ti); });
+    });
+}
+
+// ------------------------------------------------------- plan object ----
+// Row-block ownership: block b owns the rows whose first stored entry (0-based)
+// lies in [b*NB, (b+1)*NB).  blk_row[b] is the first owned row, blk_row[nblocks]
+// = m.  Built on the device in O(m).
+constexpr int SPMV_NB = 2048;
+
+struct CsrPlanData {
+    int itype = 0;
+    int64_t m = 0, nnz = 0;
+    int64_t nblocks = 0;
+    int64_t* blk_row = nullptr;   // device, nblocks+1
+    const void* ptr = nullptr;    // the row pointer the plan was built on
+};
+
+int build_csr_plan(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr,
+                   int64_t* blk_row, int64_t nblocks);
+
+}  // namespace gsa
+
+namespace gsa { void batch_plan_release(gsa_plan_s* plan); }
+
+struct gsa_plan_s {
+    int kind = 0;  // 0 = CSR row blocks, 1 = batch schedule
+    gsa::CsrPlanData csr;
+    void* batch = nullptr;  // gsa::BatchPlan*
+};
+
+struct gsa_ctx_s {
+    int kind = 0;
+    hipStream_t stream = nullptr;
+    int vtype = 0, itype = 0;
+    int64_t m = 0, n = 0, nnz_out = 0;
+    void* buf[8] = {nullptr};
+    int64_t aux[8] = {0};
+};

@@ -1,0 +1,182 @@
+// gsa_convert.hip — CSR/CSC <-> COO and CSR <-> CSC on gfx950.
+//
+// CSR -> COO expands the row pointer (kernel_csr_to_coo!,
+// src/conversions/conversion_kernels.jl:20-35) with a row-block plan so long
+// rows are written by a whole workgroup.  COO -> CSR is the reference's
+// keyed stable sort (conversions.jl:282-323): 64-bit keys (row-1)*n + (col-1)
+// — the same order as the reference's Ti keys rowind*n + colind, without their
+// Int32 overflow — radix-sorted with the entry position as payload, then the
+// index/value gathers and a row pointer from per-row counts (equal to
+// searchsortedfirst over the sorted rows).  CSR -> CSC is a stable sort on the
+// column alone: CSR storage is already row-ordered, so the result equals the
+// reference's CSR -> COO -> CSC route (conversions.jl:85-86) bit for bit.
+#include "gsa_scan.h"
+#include "gsa_rows.h"
+
+namespace gsa {
+
+template <typename I>
+__global__ __launch_bounds__(ROW_WG) void k_expand_rows(int64_t m, int64_t nnz,
+                                                        const I* __restrict__ ptr,
+                                                        const int64_t* __restrict__ blk_row,
+                                                        I* __restrict__ out) {
+    visit_owned_rows<I>(
+        m, nnz, ptr, blk_row,
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s; j < e; ++j) out[j] = (I)(r + 1);
+        },
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) out[j] = (I)(r + 1);
+        });
+}
+
+int expand_rows(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr, void* out) {
+    if (m == 0 || nnz == 0) return GSA_OK;
+    const int64_t nblocks = nnz / SPMV_NB + 1;
+    Scratch plan(s);
+    GSA_TRY(plan.alloc(sizeof(int64_t) * (nblocks + 1)));
+    GSA_TRY(build_csr_plan(s, itype, m, nnz, ptr, plan.as<int64_t>(), nblocks));
+    return dispatch_i(itype, [&](auto ti) {
+        using I = decltype(ti);
+        hipLaunchKernelGGL(k_expand_rows<I>, dim3((unsigned)nblocks), dim3(ROW_WG), 0, s, m, nnz,
+                           (const I*)ptr, (const int64_t*)plan.as<int64_t>(), (I*)out);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    });
+}
+
+// keys = (major-1)*minor_dim + (minor-1), payload = position
+template <typename I, typename V>
+__global__ void k_make_keys(int64_t nnz, const I* __restrict__ major, const I* __restrict__ minor,
+                            int64_t minor_dim, int64_t major_dim, uint64_t* __restrict__ keys,
+                            V* __restrict__ pos) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t a = dclamp((int64_t)major[i] - 1, 0, major_dim - 1);
+        int64_t b = minor ? dclamp((int64_t)minor[i] - 1, 0, minor_dim - 1) : 0;
+        keys[i] = (uint64_t)a * (uint64_t)minor_dim + (uint64_t)b;
+        pos[i] = (V)i;
+    }
+}
+
+// out_idx[i] = idx[perm[i]], out_val[i] = val[perm[i]]; counts[major[perm[i]]] += 1
+template <typename T, typename I, typename V>
+__global__ void k_gather_count(int64_t nnz, const V* __restrict__ perm,
+                               const I* __restrict__ major, const I* __restrict__ minor,
+                               const T* __restrict__ val, int64_t major_dim,
+                               I* __restrict__ out_minor, T* __restrict__ out_val,
+                               I* __restrict__ counts /* ptr+1 */) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t p = (int64_t)perm[i];
+        out_minor[i] = minor[p];
+        out_val[i] = val[p];
+        int64_t r = dclamp((int64_t)major[p] - 1, 0, major_dim - 1);
+        atomic_inc(&counts[r]);
+    }
+}
+
+// Sort the entries by (major, minor) [minor may be null: by major alone] and
+// write minor indices, values and the major pointer.
+template <typename T, typename I>
+static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_dim, int64_t nnz,
+                              const I* major, const I* minor_key, const I* minor_out,
+                              const T* val, I* ptr_out, I* idx_out, T* val_out) {
+    GSA_CHECK(nnz == 0 || (major_dim > 0 && (minor_key == nullptr || minor_dim > 0)),
+              GSA_ERR_INVALID, "stored entries in a matrix with a zero dimension");
+    GSA_HIP(hipMemsetAsync(ptr_out, 0, sizeof(I) * (major_dim + 1), s));
+    if (nnz > 0) {
+        const uint64_t maxkey = minor_key
+            ? (uint64_t)(major_dim > 0 ? major_dim - 1 : 0) * (uint64_t)(minor_dim > 0 ? minor_dim : 1) +
+                  (uint64_t)(minor_dim > 0 ? minor_dim - 1 : 0)
+            : (uint64_t)(major_dim > 0 ? major_dim - 1 : 0);
+        const int bits = bits_for(maxkey);
+        Scratch keys(s), perm(s);
+        GSA_TRY(keys.alloc(sizeof(uint64_t) * nnz));
+        const int64_t kdim = minor_key ? (minor_dim > 0 ? minor_dim : 1) : 1;
+        if (nnz < (int64_t)UINT32_MAX) {
+            GSA_TRY(perm.alloc(sizeof(uint32_t) * nnz));
+            hipLaunchKernelGGL((k_make_keys<I, uint32_t>), dim3(grid_for(nnz, 256)), dim3(256), 0, s,
+                               nnz, major, minor_key, kdim, major_dim, keys.as<uint64_t>(),
+                               perm.as<uint32_t>());
+            GSA_TRY(radix_sort_pairs_u64_u32(s, keys.as<uint64_t>(), perm.as<uint32_t>(), nnz, bits));
+            hipLaunchKernelGGL((k_gather_count<T, I, uint32_t>), dim3(grid_for(nnz, 256)), dim3(256),
+                               0, s, nnz, (const uint32_t*)perm.as<uint32_t>(), major, minor_out,
+                               val, major_dim, idx_out, val_out, ptr_out + 1);
+        } else {
+            GSA_TRY(perm.alloc(sizeof(uint64_t) * nnz));
+            hipLaunchKernelGGL((k_make_keys<I, uint64_t>), dim3(grid_for(nnz, 256)), dim3(256), 0, s,
+                               nnz, major, minor_key, kdim, major_dim, keys.as<uint64_t>(),
+                               perm.as<uint64_t>());
+            GSA_TRY(radix_sort_pairs_u64(s, keys.as<uint64_t>(), perm.as<uint64_t>(), nnz, bits));
+            hipLaunchKernelGGL((k_gather_count<T, I, uint64_t>), dim3(grid_for(nnz, 256)), dim3(256),
+                               0, s, nnz, (const uint64_t*)perm.as<uint64_t>(), major, minor_out,
+                               val, major_dim, idx_out, val_out, ptr_out + 1);
+        }
+        GSA_HIP(hipGetLastError());
+    }
+    return counts_to_ptr<I>(s, ptr_out, major_dim, nullptr);
+}
+
+}  // namespace gsa
+
+using namespace gsa;
+
+extern "C" {
+
+int gsa_csr_to_coo(void* stream, int vtype, int itype, int64_t m, int64_t nnz, const void* rowptr,
+                   const void* colval, const void* nzval, void* rowind, void* colind,
+                   void* nzval_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && nnz >= 0, GSA_ERR_INVALID, "gsa_csr_to_coo: negative size");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        if (nnz == 0) return (int)GSA_OK;
+        GSA_TRY(expand_rows(s, itype, m, nnz, rowptr, rowind));
+        GSA_HIP(hipMemcpyAsync(colind, colval, sizeof(I) * nnz, hipMemcpyDeviceToDevice, s));
+        GSA_HIP(hipMemcpyAsync(nzval_out, nzval, sizeof(T) * nnz, hipMemcpyDeviceToDevice, s));
+        return (int)GSA_OK;
+    });
+}
+
+int gsa_coo_to_csr(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnz,
+                   const void* rowind, const void* colind, const void* nzval, void* rowptr,
+                   void* colval, void* nzval_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && n >= 0 && nnz >= 0, GSA_ERR_INVALID, "gsa_coo_to_csr: negative size");
+    GSA_CHECK(m == 0 || n == 0 || (uint64_t)m <= UINT64_MAX / (uint64_t)n, GSA_ERR_INVALID,
+              "gsa_coo_to_csr: m*n overflows 64-bit keys");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        return sort_to_compressed<T, I>(s, m, n, nnz, (const I*)rowind, (const I*)colind,
+                                        (const I*)colind, (const T*)nzval, (I*)rowptr, (I*)colval,
+                                        (T*)nzval_out);
+    });
+}
+
+int gsa_csr_to_csc(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnz,
+                   const void* rowptr, const void* colval, const void* nzval, void* colptr,
+                   void* rowval, void* nzval_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && n >= 0 && nnz >= 0, GSA_ERR_INVALID, "gsa_csr_to_csc: negative size");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        Scratch rows(s);
+        if (nnz > 0) {
+            GSA_TRY(rows.alloc(sizeof(I) * nnz));
+            GSA_TRY(expand_rows(s, itype, m, nnz, rowptr, rows.p));
+        }
+        // stable sort by column only; payload gathers the (row, value) pairs
+        return sort_to_compressed<T, I>(s, n, m, nnz, (const I*)colval, nullptr,
+                                        (const I*)rows.p, (const T*)nzval, (I*)colptr,
+                                        (I*)rowval, (T*)nzval_out);
+    });
+}
+
+}  // extern "C"

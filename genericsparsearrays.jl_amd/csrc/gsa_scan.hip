@@ -1,0 +1,132 @@
+// gsa_scan.hip — device-wide inclusive prefix sums (the _cumsum_AK primitive,
+// src/helpers.jl:3) used to turn per-row counts into one-based pointers and
+// masks into write positions.  Reduce-then-scan over 4096-element tiles:
+// tile sums, one-block scan of the tile sums, then a per-tile scan with its
+// offset.  Deterministic (integer).
+#include "gsa_scan.h"
+
+namespace gsa {
+
+constexpr int SCAN_WG = 256;
+constexpr int SCAN_IPT = 16;
+constexpr int SCAN_TILE = SCAN_WG * SCAN_IPT;
+
+template <typename T>
+__device__ __forceinline__ T block_exclusive_sum(T v, T* lds, T* total) {
+    // wave inclusive scan with shuffles, then across the 4 waves
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        T y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) lds[wave] = x;
+    __syncthreads();
+    T wpre = 0, all = 0;
+    for (int w = 0; w < SCAN_WG / 64; ++w) {
+        T s = lds[w];
+        if (w < wave) wpre += s;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return wpre + x - v;
+}
+
+template <typename T>
+__global__ __launch_bounds__(SCAN_WG) void k_tile_sums(const T* __restrict__ in, int64_t n,
+                                                       T* __restrict__ sums) {
+    __shared__ T lds[SCAN_WG / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE;
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) {
+        int64_t idx = base + (int64_t)i * SCAN_WG + threadIdx.x;
+        if (idx < n) s += in[idx];
+    }
+    T total;
+    (void)block_exclusive_sum(s, lds, &total);
+    if (threadIdx.x == 0) sums[blockIdx.x] = total;
+}
+
+// exclusive scan of `count` tile sums in place, one block, looping
+template <typename T>
+__global__ __launch_bounds__(SCAN_WG) void k_scan_sums(T* __restrict__ sums, int64_t count) {
+    __shared__ T lds[SCAN_WG / 64];
+    T carry = 0;
+    for (int64_t base = 0; base < count; base += SCAN_WG) {
+        int64_t idx = base + threadIdx.x;
+        T v = idx < count ? sums[idx] : T(0);
+        T total;
+        T ex = block_exclusive_sum(v, lds, &total);
+        if (idx < count) sums[idx] = carry + ex;
+        carry += total;
+    }
+}
+
+// inclusive scan of each tile plus its offset; out may alias in
+template <typename T>
+__global__ __launch_bounds__(SCAN_WG) void k_scan_tiles(const T* in, int64_t n,
+                                                        const T* __restrict__ offs, T* out,
+                                                        T add) {
+    __shared__ T lds[SCAN_WG / 64];
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+    T v[SCAN_IPT];
+    T s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) {
+        int64_t idx = base + i;
+        v[i] = idx < n ? in[idx] : T(0);
+        s += v[i];
+    }
+    T total;
+    T ex = block_exclusive_sum(s, lds, &total);
+    T run = offs[blockIdx.x] + ex + add;
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) {
+        int64_t idx = base + i;
+        run += v[i];
+        if (idx < n) out[idx] = run;
+    }
+}
+
+template <typename T>
+int inclusive_scan(hipStream_t s, const T* in, T* out, int64_t n, T add) {
+    if (n <= 0) return GSA_OK;
+    const int64_t tiles = cdiv(n, SCAN_TILE);
+    Scratch sums(s);
+    GSA_TRY(sums.alloc(sizeof(T) * tiles));
+    hipLaunchKernelGGL(k_tile_sums<T>, dim3((unsigned)tiles), dim3(SCAN_WG), 0, s, in, n,
+                       sums.as<T>());
+    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_WG), 0, s, sums.as<T>(), tiles);
+    hipLaunchKernelGGL(k_scan_tiles<T>, dim3((unsigned)tiles), dim3(SCAN_WG), 0, s, in, n,
+                       (const T*)sums.as<T>(), out, add);
+    GSA_HIP(hipGetLastError());
+    return GSA_OK;
+}
+
+template int inclusive_scan<int64_t>(hipStream_t, const int64_t*, int64_t*, int64_t, int64_t);
+template int inclusive_scan<int32_t>(hipStream_t, const int32_t*, int32_t*, int64_t, int32_t);
+
+template <typename I>
+__global__ void k_set_one(I* p) { *p = 1; }
+
+template <typename I>
+int counts_to_ptr(hipStream_t s, I* ptr, int64_t m, int64_t* total) {
+    // ptr[1..m] hold counts; make ptr one-based pointers (ptr[0] = 1).
+    hipLaunchKernelGGL(k_set_one<I>, dim3(1), dim3(1), 0, s, ptr);
+    if (m > 0) GSA_TRY(inclusive_scan<I>(s, ptr + 1, ptr + 1, m, (I)1));
+    if (total) {
+        I last = 1;
+        GSA_HIP(hipMemcpyAsync(&last, ptr + m, sizeof(I), hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        *total = (int64_t)last - 1;
+    }
+    return GSA_OK;
+}
+
+template int counts_to_ptr<int64_t>(hipStream_t, int64_t*, int64_t, int64_t*);
+template int counts_to_ptr<int32_t>(hipStream_t, int32_t*, int64_t, int64_t*);
+
+}  // namespace gsa

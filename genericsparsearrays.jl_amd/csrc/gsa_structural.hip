@@ -1,0 +1,756 @@
+// gsa_structural.hip — dropzeros, sparse +, kron and SpGEMM on gfx950.
+//
+// Variable-size results follow the reference's two-phase shape (count, prefix
+// sum, host reads the total, caller allocates, fill):
+//  * CSR dropzeros: per-row count of nonzeros, pointer scan, order-preserving
+//    copy (kernel_count_nonzeros_csr!/kernel_dropzeros_csr!,
+//    src/matrix_csr/matrix_csr_kernels.jl:410-443).  Long rows are compacted by
+//    a whole workgroup with a prefix sum of keep flags.
+//  * CSR +: the two-pointer merge per row of kernel_count_nnz_per_row_csr! and
+//    kernel_merge_csr! (matrix_csr_kernels.jl:148-248) with the following
+//    dropzeros folded into the same two passes: an entry is kept when its
+//    (summed) value is nonzero.
+//  * COO +: concatenation, stable radix sort on column-major keys, left-to-right
+//    sums of equal-key runs, zero sums dropped (src/matrix_coo/matrix_coo.jl:300-384).
+//  * kron COO: entry (a, b) -> a*nnzB + b, as kernel_kron_coo!
+//    (src/matrix_coo/matrix_coo_kernels.jl:144-183), one workgroup per A entry.
+//  * kron CSR: for canonical operands (strictly ascending columns per row) the
+//    canonical result is written directly: row (iA, iB) starts at
+//    (rowptrA[iA]-1)*nnzB + lenA(iA)*(rowptrB[iB]-1) and lists jA-major, jB-minor,
+//    which is exactly the reference's sorted COO -> CSR result.  Other operands
+//    take the reference's route: kron COO then the keyed sort.
+//  * SpGEMM: expand-sort-compress.  Every product a_ik*b_kj is generated in the
+//    reference's order (A-row order, then B-row order), keyed (row, col), stably
+//    radix-sorted, and each key's run summed left to right starting from +0.0 —
+//    the accumulation order and arithmetic of kernel_spgemm_csr!
+//    (src/matrix_csr/matrix_csr_kernels.jl:293-343), with cancellation zeros kept.
+#include "gsa_scan.h"
+#include "gsa_rows.h"
+
+namespace gsa {
+
+int expand_rows(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr, void* out);
+
+// ------------------------------------------------------------ dropzeros ----
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_dz_count(int64_t m, int64_t nnz,
+                                                     const I* __restrict__ ptr,
+                                                     const T* __restrict__ val,
+                                                     const int64_t* __restrict__ blk_row,
+                                                     I* __restrict__ counts) {
+    __shared__ int64_t lds4[ROW_WG / 64];
+    visit_owned_rows<I>(
+        m, nnz, ptr, blk_row,
+        [&](int64_t r, int64_t s, int64_t e) {
+            int64_t c = 0;
+            for (int64_t j = s; j < e; ++j) c += (val[j] != T(0));
+            counts[r] = (I)c;
+        },
+        [&](int64_t r, int64_t s, int64_t e) {
+            int64_t c = 0;
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) c += (val[j] != T(0));
+            int64_t total;
+            (void)wg_exclusive_sum<int64_t>(c, lds4, &total);
+            if (threadIdx.x == 0) counts[r] = (I)total;
+        });
+}
+
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_dz_fill(int64_t m, int64_t nnz,
+                                                    const I* __restrict__ ptr,
+                                                    const I* __restrict__ idx,
+                                                    const T* __restrict__ val,
+                                                    const int64_t* __restrict__ blk_row,
+                                                    const I* __restrict__ ptr_out,
+                                                    I* __restrict__ idx_out,
+                                                    T* __restrict__ val_out) {
+    __shared__ int64_t lds4[ROW_WG / 64];
+    visit_owned_rows<I>(
+        m, nnz, ptr, blk_row,
+        [&](int64_t r, int64_t s, int64_t e) {
+            int64_t w = (int64_t)ptr_out[r] - 1;
+            for (int64_t j = s; j < e; ++j)
+                if (val[j] != T(0)) { idx_out[w] = idx[j]; val_out[w] = val[j]; ++w; }
+        },
+        [&](int64_t r, int64_t s, int64_t e) {
+            int64_t w0 = (int64_t)ptr_out[r] - 1;
+            for (int64_t base = s; base < e; base += ROW_WG) {
+                int64_t j = base + threadIdx.x;
+                bool keep = j < e && val[j] != T(0);
+                int64_t total;
+                int64_t off = wg_exclusive_sum<int64_t>(keep ? 1 : 0, lds4, &total);
+                if (keep) { idx_out[w0 + off] = idx[j]; val_out[w0 + off] = val[j]; }
+                w0 += total;
+            }
+        });
+}
+
+// ------------------------------------------------------------ CSR + --------
+template <typename T, typename I, bool FILL>
+__global__ void k_add_rows(int64_t m, const I* __restrict__ pA, const I* __restrict__ cA,
+                           const T* __restrict__ vA, const I* __restrict__ pB,
+                           const I* __restrict__ cB, const T* __restrict__ vB,
+                           I* __restrict__ ptrC /* counts at +1 when !FILL */,
+                           I* __restrict__ cC, T* __restrict__ vC) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t a = (int64_t)pA[r] - 1, ea = (int64_t)pA[r + 1] - 1;
+        int64_t b = (int64_t)pB[r] - 1, eb = (int64_t)pB[r + 1] - 1;
+        int64_t w = FILL ? (int64_t)ptrC[r] - 1 : 0;
+        int64_t cnt = 0;
+        while (a < ea || b < eb) {
+            I c;
+            T v;
+            if (a < ea && (b >= eb || cA[a] < cB[b])) { c = cA[a]; v = vA[a]; ++a; }
+            else if (b < eb && (a >= ea || cA[a] > cB[b])) { c = cB[b]; v = vB[b]; ++b; }
+            else { c = cA[a]; v = vA[a] + vB[b]; ++a; ++b; }
+            if (v != T(0)) {
+                if (FILL) { cC[w] = c; vC[w] = v; ++w; }
+                ++cnt;
+            }
+        }
+        if (!FILL) ptrC[r + 1] = (I)cnt;
+    }
+}
+
+// ------------------------------------------------------------ COO helpers --
+template <typename T>
+__global__ void k_nonzero_flags(int64_t n, const T* __restrict__ v, int64_t* __restrict__ f) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        f[i] = v[i] != T(0);
+}
+
+template <typename T, typename I>
+__global__ void k_compact(int64_t n, const int64_t* __restrict__ incl, const I* __restrict__ r,
+                          const I* __restrict__ c, const T* __restrict__ v, I* __restrict__ ro,
+                          I* __restrict__ co, T* __restrict__ vo) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t prev = i ? incl[i - 1] : 0;
+        if (incl[i] != prev) { ro[prev] = r[i]; co[prev] = c[i]; vo[prev] = v[i]; }
+    }
+}
+
+// concatenated (A then B) column-major keys with positions
+template <typename I, typename V>
+__global__ void k_concat_keys(int64_t nA, const I* __restrict__ rA, const I* __restrict__ cA,
+                              int64_t nB, const I* __restrict__ rB, const I* __restrict__ cB,
+                              int64_t m, int64_t n, uint64_t* __restrict__ keys,
+                              V* __restrict__ pos) {
+    const int64_t tot = nA + nB;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t r = i < nA ? (int64_t)rA[i] : (int64_t)rB[i - nA];
+        int64_t c = i < nA ? (int64_t)cA[i] : (int64_t)cB[i - nA];
+        r = dclamp(r - 1, 0, m - 1);
+        c = dclamp(c - 1, 0, n - 1);
+        keys[i] = (uint64_t)c * (uint64_t)m + (uint64_t)r;
+        pos[i] = (V)i;
+    }
+}
+
+template <typename T, typename I, typename V>
+__global__ void k_concat_gather(int64_t tot, const V* __restrict__ perm, int64_t nA,
+                                const I* __restrict__ rA, const I* __restrict__ cA,
+                                const T* __restrict__ vA, const I* __restrict__ rB,
+                                const I* __restrict__ cB, const T* __restrict__ vB,
+                                I* __restrict__ ro, I* __restrict__ co, T* __restrict__ vo) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t p = (int64_t)perm[i];
+        if (p < nA) { ro[i] = rA[p]; co[i] = cA[p]; vo[i] = vA[p]; }
+        else { ro[i] = rB[p - nA]; co[i] = cB[p - nA]; vo[i] = vB[p - nA]; }
+    }
+}
+
+// Run heads of sorted keys: sum each run left to right starting from its first
+// value (kernel_compact_coo!), flag it kept when the sum is nonzero.
+template <typename T>
+__global__ void k_run_sums(int64_t n, const uint64_t* __restrict__ keys, const T* __restrict__ v,
+                           T* __restrict__ sums, int64_t* __restrict__ keep) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        bool head = (i == 0) || keys[i] != keys[i - 1];
+        int64_t k = 0;
+        if (head) {
+            T s = v[i];
+            int64_t j = i + 1;
+            while (j < n && keys[j] == keys[i]) { s = s + v[j]; ++j; }
+            sums[i] = s;
+            k = s != T(0);
+        }
+        keep[i] = k;
+    }
+}
+
+// ------------------------------------------------------------ kron --------
+template <typename T, typename I>
+__global__ __launch_bounds__(256) void k_kron_coo(int64_t nnzA, int64_t nnzB, const I* __restrict__ rA,
+                                                  const I* __restrict__ cA, const T* __restrict__ vA,
+                                                  const I* __restrict__ rB, const I* __restrict__ cB,
+                                                  const T* __restrict__ vB, int64_t mB, int64_t nB,
+                                                  I* __restrict__ rC, I* __restrict__ cC,
+                                                  T* __restrict__ vC) {
+    for (int64_t a = blockIdx.x; a < nnzA; a += gridDim.x) {
+        const int64_t ra = (int64_t)rA[a] - 1, ca = (int64_t)cA[a] - 1;
+        const T va = vA[a];
+        const int64_t base = a * nnzB;
+        for (int64_t b = threadIdx.x; b < nnzB; b += blockDim.x) {
+            rC[base + b] = (I)(ra * mB + (int64_t)rB[b]);
+            cC[base + b] = (I)(ca * nB + (int64_t)cB[b]);
+            vC[base + b] = va * vB[b];
+        }
+    }
+}
+
+// flag[0] = 1 if some row has a non-increasing column pair
+template <typename I>
+__global__ void k_check_canonical(int64_t m, const I* __restrict__ p, const I* __restrict__ c,
+                                  int* __restrict__ flag) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        for (int64_t j = (int64_t)p[r]; j < (int64_t)p[r + 1] - 1; ++j)
+            if (c[j] <= c[j - 1]) { *flag = 1; break; }
+    }
+}
+
+template <typename I>
+__global__ void k_kron_rowptr(int64_t mA, int64_t mB, int64_t nnzB, const I* __restrict__ pA,
+                              const I* __restrict__ pB, I* __restrict__ pC) {
+    const int64_t total = mA * mB;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= total;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i == total) {
+            pC[i] = (I)((int64_t)(pA[mA] - 1) * nnzB + 1);
+            continue;
+        }
+        int64_t iA = i / mB, iB = i - iA * mB;
+        int64_t lenA = (int64_t)pA[iA + 1] - pA[iA];
+        pC[i] = (I)(((int64_t)pA[iA] - 1) * nnzB + lenA * ((int64_t)pB[iB] - 1) + 1);
+    }
+}
+
+// One workgroup per A row: its C region [(pA[iA]-1)*nnzB, +lenA*nnzB) is
+// ordered (iB, a, b); each offset finds iB by binary search on B's pointer.
+template <typename T, typename I>
+__global__ __launch_bounds__(256) void k_kron_csr_fill(
+    int64_t mA, int64_t mB, int64_t nB, int64_t nnzB, const I* __restrict__ pA,
+    const I* __restrict__ cA, const T* __restrict__ vA, const I* __restrict__ pB,
+    const I* __restrict__ cB, const T* __restrict__ vB, I* __restrict__ cC, T* __restrict__ vC) {
+    for (int64_t iA = blockIdx.x; iA < mA; iA += gridDim.x) {
+        const int64_t a0 = (int64_t)pA[iA] - 1;
+        const int64_t lenA = (int64_t)pA[iA + 1] - 1 - a0;
+        if (lenA <= 0) continue;
+        const int64_t base = a0 * nnzB;
+        const int64_t cnt = lenA * nnzB;
+        for (int64_t o = threadIdx.x; o < cnt; o += blockDim.x) {
+            // largest iB with lenA*(pB[iB]-1) <= o, skipping empty B rows
+            int64_t lo = 0, hi = mB - 1;
+            while (lo < hi) {
+                int64_t mid = (lo + hi + 1) >> 1;
+                if (lenA * ((int64_t)pB[mid] - 1) <= o) lo = mid; else hi = mid - 1;
+            }
+            const int64_t b0 = (int64_t)pB[lo] - 1;
+            const int64_t lenB = (int64_t)pB[lo + 1] - 1 - b0;
+            const int64_t rr = o - lenA * b0;
+            const int64_t aL = rr / lenB, bL = rr - aL * lenB;
+            const int64_t a = a0 + aL, b = b0 + bL;
+            cC[base + o] = (I)(((int64_t)cA[a] - 1) * nB + (int64_t)cB[b]);
+            vC[base + o] = vA[a] * vB[b];
+        }
+    }
+}
+
+// ------------------------------------------------------------ SpGEMM ------
+template <typename I>
+__global__ void k_sg_count(int64_t m, const I* __restrict__ pA, const I* __restrict__ cA,
+                           const I* __restrict__ pB, int64_t kdim, int64_t* __restrict__ cnt) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t c = 0;
+        for (int64_t a = (int64_t)pA[r] - 1; a < (int64_t)pA[r + 1] - 1; ++a) {
+            int64_t k = dclamp((int64_t)cA[a] - 1, 0, kdim - 1);
+            c += (int64_t)pB[k + 1] - pB[k];
+        }
+        cnt[r + 1] = c;
+    }
+}
+
+template <typename T, typename I, typename V>
+__global__ void k_sg_expand(int64_t m, int64_t n, int64_t kdim, const I* __restrict__ pA,
+                            const I* __restrict__ cA, const T* __restrict__ vA,
+                            const I* __restrict__ pB, const I* __restrict__ cB,
+                            const T* __restrict__ vB, const int64_t* __restrict__ pptr,
+                            uint64_t* __restrict__ keys, T* __restrict__ prods,
+                            V* __restrict__ pos) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        int64_t w = pptr[r] - 1;
+        for (int64_t a = (int64_t)pA[r] - 1; a < (int64_t)pA[r + 1] - 1; ++a) {
+            int64_t k = dclamp((int64_t)cA[a] - 1, 0, kdim - 1);
+            T va = vA[a];
+            for (int64_t b = (int64_t)pB[k] - 1; b < (int64_t)pB[k + 1] - 1; ++b) {
+                int64_t c = dclamp((int64_t)cB[b] - 1, 0, n - 1);
+                keys[w] = (uint64_t)r * (uint64_t)n + (uint64_t)c;
+                prods[w] = va * vB[b];
+                pos[w] = (V)w;
+                ++w;
+            }
+        }
+    }
+}
+
+template <typename T, typename V>
+__global__ void k_gather_vals(int64_t n, const V* __restrict__ perm, const T* __restrict__ in,
+                              T* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[(int64_t)perm[i]];
+}
+
+template <typename T>
+__global__ void k_sg_heads(int64_t P, const uint64_t* __restrict__ keys, int64_t* __restrict__ f) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P;
+         i += (int64_t)gridDim.x * blockDim.x)
+        f[i] = (i == 0) || keys[i] != keys[i - 1];
+}
+
+// per output entry: sum its run from +0.0 in generation order; per-row counts
+template <typename T, typename I>
+__global__ void k_sg_sum(int64_t P, int64_t n, const uint64_t* __restrict__ keys,
+                         const T* __restrict__ v, const int64_t* __restrict__ incl,
+                         uint64_t* __restrict__ okeys, T* __restrict__ ovals,
+                         I* __restrict__ rowcnt /* ptr+1 */) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < P;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i != 0 && keys[i] == keys[i - 1]) continue;
+        T s = T(0);
+        int64_t j = i;
+        while (j < P && keys[j] == keys[i]) { s = s + v[j]; ++j; }
+        int64_t o = incl[i] - 1;
+        okeys[o] = keys[i];
+        ovals[o] = s;
+        atomic_inc(&rowcnt[keys[i] / (uint64_t)n]);
+    }
+}
+
+template <typename T, typename I>
+__global__ void k_sg_write(int64_t nnzC, int64_t n, const uint64_t* __restrict__ okeys,
+                           const T* __restrict__ ovals, I* __restrict__ cC, T* __restrict__ vC) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnzC;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t k = okeys[i];
+        cC[i] = (I)(k % (uint64_t)n + 1);
+        vC[i] = ovals[i];
+    }
+}
+
+template <typename T, typename I>
+static int run_kron_coo(hipStream_t s, int64_t mB, int64_t nB, int64_t nnzA, const I* rA,
+                        const I* cA, const T* vA, int64_t nnzB, const I* rB, const I* cB,
+                        const T* vB, I* rC, I* cC, T* vC) {
+    if (nnzA == 0 || nnzB == 0) return GSA_OK;
+    unsigned g = (unsigned)(nnzA < 65536 ? nnzA : 65536);
+    hipLaunchKernelGGL((k_kron_coo<T, I>), dim3(g), dim3(256), 0, s, nnzA, nnzB, rA, cA, vA, rB, cB,
+                       vB, mB, nB, rC, cC, vC);
+    GSA_HIP(hipGetLastError());
+    return GSA_OK;
+}
+
+}  // namespace gsa
+
+using namespace gsa;
+
+extern "C" int gsa_coo_to_csr(void*, int, int, int64_t, int64_t, int64_t, const void*, const void*,
+                              const void*, void*, void*, void*);
+
+extern "C" {
+
+int gsa_csr_dropzeros_count(void* stream, int vtype, int itype, int64_t m, const void* rowptr,
+                            const void* nzval, void* rowptr_out, int64_t* nnz_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && nnz_out, GSA_ERR_INVALID, "gsa_csr_dropzeros_count: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        I last = 1;
+        if (m > 0) {
+            GSA_HIP(hipMemcpyAsync(&last, (const I*)rowptr + m, sizeof(I), hipMemcpyDeviceToHost, s));
+            GSA_HIP(hipStreamSynchronize(s));
+        }
+        const int64_t nnz = (int64_t)last - 1;
+        GSA_HIP(hipMemsetAsync(rowptr_out, 0, sizeof(I) * (m + 1), s));
+        if (m > 0 && nnz > 0) {
+            const int64_t nblocks = nnz / SPMV_NB + 1;
+            Scratch plan(s);
+            GSA_TRY(plan.alloc(sizeof(int64_t) * (nblocks + 1)));
+            GSA_TRY(build_csr_plan(s, itype, m, nnz, rowptr, plan.as<int64_t>(), nblocks));
+            hipLaunchKernelGGL((k_dz_count<T, I>), dim3((unsigned)nblocks), dim3(ROW_WG), 0, s, m, nnz,
+                               (const I*)rowptr, (const T*)nzval,
+                               (const int64_t*)plan.as<int64_t>(), (I*)rowptr_out + 1);
+            GSA_HIP(hipGetLastError());
+        }
+        return counts_to_ptr<I>(s, (I*)rowptr_out, m, nnz_out);
+    });
+}
+
+int gsa_csr_dropzeros_fill(void* stream, int vtype, int itype, int64_t m, const void* rowptr,
+                           const void* colval, const void* nzval, const void* rowptr_out,
+                           void* colval_out, void* nzval_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0, GSA_ERR_INVALID, "gsa_csr_dropzeros_fill: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        if (m == 0) return (int)GSA_OK;
+        I last = 1;
+        GSA_HIP(hipMemcpyAsync(&last, (const I*)rowptr + m, sizeof(I), hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        const int64_t nnz = (int64_t)last - 1;
+        if (nnz <= 0) return (int)GSA_OK;
+        const int64_t nblocks = nnz / SPMV_NB + 1;
+        Scratch plan(s);
+        GSA_TRY(plan.alloc(sizeof(int64_t) * (nblocks + 1)));
+        GSA_TRY(build_csr_plan(s, itype, m, nnz, rowptr, plan.as<int64_t>(), nblocks));
+        hipLaunchKernelGGL((k_dz_fill<T, I>), dim3((unsigned)nblocks), dim3(ROW_WG), 0, s, m, nnz,
+                           (const I*)rowptr, (const I*)colval, (const T*)nzval,
+                           (const int64_t*)plan.as<int64_t>(), (const I*)rowptr_out, (I*)colval_out,
+                           (T*)nzval_out);
+        GSA_HIP(hipGetLastError());
+        return (int)GSA_OK;
+    });
+}
+
+int gsa_coo_dropzeros_count(void* stream, int vtype, int64_t nnz, const void* nzval,
+                            int64_t* nnz_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(nnz >= 0 && nnz_out, GSA_ERR_INVALID, "gsa_coo_dropzeros_count: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_v(vtype, [&](auto tv) {
+        using T = decltype(tv);
+        *nnz_out = 0;
+        if (nnz == 0) return (int)GSA_OK;
+        Scratch f(s);
+        GSA_TRY(f.alloc(sizeof(int64_t) * nnz));
+        hipLaunchKernelGGL(k_nonzero_flags<T>, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,
+                           (const T*)nzval, f.as<int64_t>());
+        GSA_TRY(inclusive_scan<int64_t>(s, f.as<int64_t>(), f.as<int64_t>(), nnz, 0));
+        int64_t tot = 0;
+        GSA_HIP(hipMemcpyAsync(&tot, f.as<int64_t>() + nnz - 1, sizeof(int64_t),
+                               hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        *nnz_out = tot;
+        return (int)GSA_OK;
+    });
+}
+
+int gsa_coo_dropzeros_fill(void* stream, int vtype, int itype, int64_t nnz, const void* rowind,
+                           const void* colind, const void* nzval, void* rowind_out,
+                           void* colind_out, void* nzval_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(nnz >= 0, GSA_ERR_INVALID, "gsa_coo_dropzeros_fill: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        if (nnz == 0) return (int)GSA_OK;
+        Scratch f(s);
+        GSA_TRY(f.alloc(sizeof(int64_t) * nnz));
+        hipLaunchKernelGGL(k_nonzero_flags<T>, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,
+                           (const T*)nzval, f.as<int64_t>());
+        GSA_TRY(inclusive_scan<int64_t>(s, f.as<int64_t>(), f.as<int64_t>(), nnz, 0));
+        hipLaunchKernelGGL((k_compact<T, I>), dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,
+                           (const int64_t*)f.as<int64_t>(), (const I*)rowind, (const I*)colind,
+                           (const T*)nzval, (I*)rowind_out, (I*)colind_out, (T*)nzval_out);
+        GSA_HIP(hipGetLastError());
+        return (int)GSA_OK;
+    });
+}
+
+int gsa_csr_add_count(void* stream, int vtype, int itype, int64_t m, const void* rowptrA,
+                      const void* colA, const void* valA, const void* rowptrB, const void* colB,
+                      const void* valB, void* rowptrC, int64_t* nnzC) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && nnzC, GSA_ERR_INVALID, "gsa_csr_add_count: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        if (m > 0)
+            hipLaunchKernelGGL((k_add_rows<T, I, false>), dim3(grid_for(m, 256)), dim3(256), 0, s, m,
+                               (const I*)rowptrA, (const I*)colA, (const T*)valA, (const I*)rowptrB,
+                               (const I*)colB, (const T*)valB, (I*)rowptrC, (I*)nullptr,
+                               (T*)nullptr);
+        GSA_HIP(hipGetLastError());
+        return counts_to_ptr<I>(s, (I*)rowptrC, m, nnzC);
+    });
+}
+
+int gsa_csr_add_fill(void* stream, int vtype, int itype, int64_t m, const void* rowptrA,
+                     const void* colA, const void* valA, const void* rowptrB, const void* colB,
+                     const void* valB, const void* rowptrC, void* colC, void* valC) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0, GSA_ERR_INVALID, "gsa_csr_add_fill: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        if (m == 0) return (int)GSA_OK;
+        hipLaunchKernelGGL((k_add_rows<T, I, true>), dim3(grid_for(m, 256)), dim3(256), 0, s, m,
+                           (const I*)rowptrA, (const I*)colA, (const T*)valA, (const I*)rowptrB,
+                           (const I*)colB, (const T*)valB, (I*)rowptrC, (I*)colC, (T*)valC);
+        GSA_HIP(hipGetLastError());
+        return (int)GSA_OK;
+    });
+}
+
+int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnzA,
+                      const void* rowA, const void* colA, const void* valA, int64_t nnzB,
+                      const void* rowB, const void* colB, const void* valB, gsa_ctx* ctx,
+                      int64_t* nnzC) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(ctx && nnzC && m >= 0 && n >= 0 && nnzA >= 0 && nnzB >= 0, GSA_ERR_INVALID,
+              "gsa_coo_add_begin: bad arguments");
+    const int64_t tot = nnzA + nnzB;
+    GSA_CHECK(tot == 0 || (m > 0 && n > 0), GSA_ERR_INVALID,
+              "stored entries in a matrix with a zero dimension");
+    GSA_CHECK(m == 0 || n == 0 || (uint64_t)m <= UINT64_MAX / (uint64_t)n, GSA_ERR_INVALID,
+              "gsa_coo_add_begin: m*n overflows 64-bit keys");
+    hipStream_t s = as_stream(stream);
+    *ctx = nullptr;
+    *nnzC = 0;
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        auto* c = new gsa_ctx_s();
+        c->kind = 1; c->stream = s; c->vtype = vtype; c->itype = itype; c->m = m; c->n = n;
+        c->aux[0] = tot;
+        auto fail = [&](int r) { gsa_ctx_destroy(c); return r; };
+        if (tot > 0) {
+            Scratch keys(s), perm(s);
+            int r;
+            if ((r = keys.alloc(sizeof(uint64_t) * tot)) != GSA_OK) return fail(r);
+            if ((r = perm.alloc(sizeof(uint64_t) * tot)) != GSA_OK) return fail(r);
+            hipLaunchKernelGGL((k_concat_keys<I, uint64_t>), dim3(grid_for(tot, 256)), dim3(256), 0, s,
+                               nnzA, (const I*)rowA, (const I*)colA, nnzB, (const I*)rowB,
+                               (const I*)colB, m, n, keys.as<uint64_t>(), perm.as<uint64_t>());
+            const int bits = bits_for((uint64_t)(n - 1) * (uint64_t)m + (uint64_t)(m - 1));
+            if ((r = radix_sort_pairs_u64(s, keys.as<uint64_t>(), perm.as<uint64_t>(), tot, bits)) != GSA_OK)
+                return fail(r);
+            for (int b = 0; b < 4; ++b) {
+                size_t sz = b < 2 ? sizeof(I) * tot : (b == 2 ? sizeof(T) * tot : sizeof(T) * tot);
+                if ((r = ws_alloc(&c->buf[b], sz, s)) != GSA_OK) return fail(r);
+            }
+            if ((r = ws_alloc(&c->buf[4], sizeof(int64_t) * tot, s)) != GSA_OK) return fail(r);
+            I* ro = (I*)c->buf[0]; I* co = (I*)c->buf[1]; T* vo = (T*)c->buf[2];
+            T* sums = (T*)c->buf[3]; int64_t* keep = (int64_t*)c->buf[4];
+            hipLaunchKernelGGL((k_concat_gather<T, I, uint64_t>), dim3(grid_for(tot, 256)), dim3(256), 0,
+                               s, tot, (const uint64_t*)perm.as<uint64_t>(), nnzA, (const I*)rowA,
+                               (const I*)colA, (const T*)valA, (const I*)rowB, (const I*)colB,
+                               (const T*)valB, ro, co, vo);
+            hipLaunchKernelGGL(k_run_sums<T>, dim3(grid_for(tot, 256)), dim3(256), 0, s, tot,
+                               (const uint64_t*)keys.as<uint64_t>(), (const T*)vo, sums, keep);
+            if ((r = inclusive_scan<int64_t>(s, keep, keep, tot, 0)) != GSA_OK) return fail(r);
+            hipError_t e = hipMemcpyAsync(&c->nnz_out, keep + tot - 1, sizeof(int64_t),
+                                          hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return fail(hip_fail(e, "gsa_coo_add_begin"));
+        }
+        *nnzC = c->nnz_out;
+        *ctx = c;
+        return (int)GSA_OK;
+    });
+}
+
+int gsa_coo_add_finish(gsa_ctx ctx, void* rowC, void* colC, void* valC) {
+    GSA_CHECK(ctx && ctx->kind == 1, GSA_ERR_INVALID, "gsa_coo_add_finish: bad context");
+    hipStream_t s = ctx->stream;
+    const int64_t tot = ctx->aux[0];
+    int r = dispatch_vi(ctx->vtype, ctx->itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        if (tot == 0) return (int)GSA_OK;
+        hipLaunchKernelGGL((k_compact<T, I>), dim3(grid_for(tot, 256)), dim3(256), 0, s, tot,
+                           (const int64_t*)ctx->buf[4], (const I*)ctx->buf[0],
+                           (const I*)ctx->buf[1], (const T*)ctx->buf[3], (I*)rowC, (I*)colC,
+                           (T*)valC);
+        GSA_HIP(hipGetLastError());
+        return (int)GSA_OK;
+    });
+    gsa_ctx_destroy(ctx);
+    return r;
+}
+
+int gsa_kron_coo(void* stream, int vtype, int itype, int64_t mB, int64_t nB, int64_t nnzA,
+                 const void* rowA, const void* colA, const void* valA, int64_t nnzB,
+                 const void* rowB, const void* colB, const void* valB, void* rowC, void* colC,
+                 void* valC) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(mB >= 0 && nB >= 0 && nnzA >= 0 && nnzB >= 0, GSA_ERR_INVALID,
+              "gsa_kron_coo: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        return run_kron_coo<T, I>(s, mB, nB, nnzA, (const I*)rowA, (const I*)colA, (const T*)valA,
+                                  nnzB, (const I*)rowB, (const I*)colB, (const T*)valB, (I*)rowC,
+                                  (I*)colC, (T*)valC);
+    });
+}
+
+int gsa_kron_csr(void* stream, int vtype, int itype, int64_t mA, int64_t nA, int64_t mB,
+                 int64_t nB, const void* rowptrA, const void* colA, const void* valA, int64_t nnzA,
+                 const void* rowptrB, const void* colB, const void* valB, int64_t nnzB,
+                 void* rowptrC, void* colC, void* valC) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(mA >= 0 && nA >= 0 && mB >= 0 && nB >= 0 && nnzA >= 0 && nnzB >= 0, GSA_ERR_INVALID,
+              "gsa_kron_csr: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        const int64_t mC = mA * mB, nnzC = nnzA * nnzB;
+        // canonical operands?
+        Scratch flag(s);
+        GSA_TRY(flag.alloc(sizeof(int)));
+        GSA_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+        if (mA > 0)
+            hipLaunchKernelGGL(k_check_canonical<I>, dim3(grid_for(mA, 256)), dim3(256), 0, s, mA,
+                               (const I*)rowptrA, (const I*)colA, flag.as<int>());
+        if (mB > 0)
+            hipLaunchKernelGGL(k_check_canonical<I>, dim3(grid_for(mB, 256)), dim3(256), 0, s, mB,
+                               (const I*)rowptrB, (const I*)colB, flag.as<int>());
+        int bad = 0;
+        GSA_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        if (!bad) {
+            hipLaunchKernelGGL(k_kron_rowptr<I>, dim3(grid_for(mC + 1, 256)), dim3(256), 0, s, mA, mB,
+                               nnzB, (const I*)rowptrA, (const I*)rowptrB, (I*)rowptrC);
+            if (nnzC > 0 && mB > 0) {
+                unsigned g = (unsigned)(mA < 65536 ? mA : 65536);
+                hipLaunchKernelGGL((k_kron_csr_fill<T, I>), dim3(g), dim3(256), 0, s, mA, mB, nB, nnzB,
+                                   (const I*)rowptrA, (const I*)colA, (const T*)valA,
+                                   (const I*)rowptrB, (const I*)colB, (const T*)valB, (I*)colC,
+                                   (T*)valC);
+            }
+            GSA_HIP(hipGetLastError());
+            return (int)GSA_OK;
+        }
+        // reference route: CSR -> COO, kron COO, COO -> CSR (keyed sort)
+        Scratch rA(s), rB(s), rC(s), cCc(s), vCc(s);
+        GSA_TRY(rA.alloc(sizeof(I) * (nnzA ? nnzA : 1)));
+        GSA_TRY(rB.alloc(sizeof(I) * (nnzB ? nnzB : 1)));
+        GSA_TRY(expand_rows(s, itype, mA, nnzA, rowptrA, rA.p));
+        GSA_TRY(expand_rows(s, itype, mB, nnzB, rowptrB, rB.p));
+        GSA_TRY(rC.alloc(sizeof(I) * (nnzC ? nnzC : 1)));
+        GSA_TRY(cCc.alloc(sizeof(I) * (nnzC ? nnzC : 1)));
+        GSA_TRY(vCc.alloc(sizeof(T) * (nnzC ? nnzC : 1)));
+        GSA_TRY((run_kron_coo<T, I>(s, mB, nB, nnzA, rA.as<I>(), (const I*)colA, (const T*)valA, nnzB,
+                                    rB.as<I>(), (const I*)colB, (const T*)valB, rC.as<I>(),
+                                    cCc.as<I>(), vCc.as<T>())));
+        return gsa_coo_to_csr(s, vtype, itype, mC, nA * nB, nnzC, rC.p, cCc.p, vCc.p, rowptrC, colC,
+                              valC);
+    });
+}
+
+int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t kdim, int64_t n,
+                         const void* rowptrA, const void* colA, const void* valA, int64_t nnzA,
+                         const void* rowptrB, const void* colB, const void* valB, int64_t nnzB,
+                         gsa_ctx* ctx, int64_t* nnzC) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(ctx && nnzC && m >= 0 && kdim >= 0 && n >= 0 && nnzA >= 0 && nnzB >= 0,
+              GSA_ERR_INVALID, "gsa_spgemm_csr_begin: bad arguments");
+    GSA_CHECK(m == 0 || n == 0 || (uint64_t)m <= UINT64_MAX / (uint64_t)n, GSA_ERR_INVALID,
+              "gsa_spgemm_csr_begin: m*n overflows 64-bit keys");
+    hipStream_t s = as_stream(stream);
+    *ctx = nullptr;
+    *nnzC = 0;
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        auto* c = new gsa_ctx_s();
+        c->kind = 2; c->stream = s; c->vtype = vtype; c->itype = itype; c->m = m; c->n = n;
+        auto fail = [&](int r) { gsa_ctx_destroy(c); return r; };
+        int r;
+        // per-row counts (ctx buf[0]: I[m+1]) always allocated
+        if ((r = ws_alloc(&c->buf[0], sizeof(I) * (m + 1), s)) != GSA_OK) return fail(r);
+        hipError_t e = hipMemsetAsync(c->buf[0], 0, sizeof(I) * (m + 1), s);
+        if (e != hipSuccess) return fail(hip_fail(e, "hipMemsetAsync"));
+        if (m == 0 || n == 0 || nnzA == 0 || nnzB == 0 || kdim == 0) {
+            *ctx = c;
+            return (int)GSA_OK;
+        }
+        Scratch pptr(s);
+        if ((r = pptr.alloc(sizeof(int64_t) * (m + 1))) != GSA_OK) return fail(r);
+        hipLaunchKernelGGL(k_sg_count<I>, dim3(grid_for(m, 256)), dim3(256), 0, s, m, (const I*)rowptrA,
+                           (const I*)colA, (const I*)rowptrB, kdim, pptr.as<int64_t>());
+        int64_t P = 0;
+        if ((r = counts_to_ptr<int64_t>(s, pptr.as<int64_t>(), m, &P)) != GSA_OK) return fail(r);
+        c->aux[0] = P;
+        if (P == 0) { *ctx = c; return (int)GSA_OK; }
+        Scratch keys(s), prods(s), perm(s), sorted(s), flags(s);
+        if ((r = keys.alloc(sizeof(uint64_t) * P)) != GSA_OK) return fail(r);
+        if ((r = prods.alloc(sizeof(T) * P)) != GSA_OK) return fail(r);
+        if ((r = perm.alloc(sizeof(uint64_t) * P)) != GSA_OK) return fail(r);
+        hipLaunchKernelGGL((k_sg_expand<T, I, uint64_t>), dim3(grid_for(m, 256)), dim3(256), 0, s, m, n,
+                           kdim, (const I*)rowptrA, (const I*)colA, (const T*)valA,
+                           (const I*)rowptrB, (const I*)colB, (const T*)valB,
+                           (const int64_t*)pptr.as<int64_t>(), keys.as<uint64_t>(), prods.as<T>(),
+                           perm.as<uint64_t>());
+        const int bits = bits_for((uint64_t)(m - 1) * (uint64_t)n + (uint64_t)(n - 1));
+        if ((r = radix_sort_pairs_u64(s, keys.as<uint64_t>(), perm.as<uint64_t>(), P, bits)) != GSA_OK)
+            return fail(r);
+        if ((r = sorted.alloc(sizeof(T) * P)) != GSA_OK) return fail(r);
+        hipLaunchKernelGGL((k_gather_vals<T, uint64_t>), dim3(grid_for(P, 256)), dim3(256), 0, s, P,
+                           (const uint64_t*)perm.as<uint64_t>(), (const T*)prods.as<T>(),
+                           sorted.as<T>());
+        if ((r = flags.alloc(sizeof(int64_t) * P)) != GSA_OK) return fail(r);
+        hipLaunchKernelGGL(k_sg_heads<T>, dim3(grid_for(P, 256)), dim3(256), 0, s, P,
+                           (const uint64_t*)keys.as<uint64_t>(), flags.as<int64_t>());
+        if ((r = inclusive_scan<int64_t>(s, flags.as<int64_t>(), flags.as<int64_t>(), P, 0)) != GSA_OK)
+            return fail(r);
+        int64_t total = 0;
+        e = hipMemcpyAsync(&total, flags.as<int64_t>() + P - 1, sizeof(int64_t),
+                           hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return fail(hip_fail(e, "gsa_spgemm_csr_begin"));
+        c->nnz_out = total;
+        if ((r = ws_alloc(&c->buf[1], sizeof(uint64_t) * total, s)) != GSA_OK) return fail(r);
+        if ((r = ws_alloc(&c->buf[2], sizeof(T) * total, s)) != GSA_OK) return fail(r);
+        hipLaunchKernelGGL((k_sg_sum<T, I>), dim3(grid_for(P, 256)), dim3(256), 0, s, P, n,
+                           (const uint64_t*)keys.as<uint64_t>(), (const T*)sorted.as<T>(),
+                           (const int64_t*)flags.as<int64_t>(), (uint64_t*)c->buf[1],
+                           (T*)c->buf[2], (I*)c->buf[0] + 1);
+        e = hipGetLastError();
+        if (e != hipSuccess) return fail(hip_fail(e, "k_sg_sum"));
+        *nnzC = total;
+        *ctx = c;
+        return (int)GSA_OK;
+    });
+}
+
+int gsa_spgemm_csr_finish(gsa_ctx ctx, void* rowptrC, void* colC, void* valC) {
+    GSA_CHECK(ctx && ctx->kind == 2, GSA_ERR_INVALID, "gsa_spgemm_csr_finish: bad context");
+    hipStream_t s = ctx->stream;
+    int r = dispatch_vi(ctx->vtype, ctx->itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        const int64_t m = ctx->m;
+        GSA_HIP(hipMemcpyAsync(rowptrC, ctx->buf[0], sizeof(I) * (m + 1), hipMemcpyDeviceToDevice, s));
+        GSA_TRY(counts_to_ptr<I>(s, (I*)rowptrC, m, nullptr));
+        if (ctx->nnz_out > 0) {
+            hipLaunchKernelGGL((k_sg_write<T, I>), dim3(grid_for(ctx->nnz_out, 256)), dim3(256), 0, s,
+                               ctx->nnz_out, ctx->n, (const uint64_t*)ctx->buf[1],
+                               (const T*)ctx->buf[2], (I*)colC, (T*)valC);
+            GSA_HIP(hipGetLastError());
+        }
+        return (int)GSA_OK;
+    });
+    gsa_ctx_destroy(ctx);
+    return r;
+}
+
+}  // extern "C"

@@ -1,0 +1,213 @@
+"""Format conversions (src/conversions/conversions.jl) over libgsa_hip.so.
+
+Device-to-device conversions are HIP kernels; host ingest from a
+scipy.sparse matrix (the SparseMatrixCSC interop of conversions.jl:7-79) builds
+the one-based arrays on the host once and copies them to the device.
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import ArgumentError
+from .matrix import (GenericSparseMatrixCSR, GenericSparseMatrixCSC, GenericSparseMatrixCOO,
+                     Transpose, Adjoint, itype_of, vtype_of, ptr, stream_of, require_device)
+
+
+def _is_scipy(A) -> bool:
+    try:
+        import scipy.sparse as sp
+        return sp.issparse(A)
+    except Exception:  # pragma: no cover
+        return False
+
+
+def _dev(device):
+    return torch.device(device if device is not None else "cuda")
+
+
+def _t(a, dtype, device):
+    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype=dtype, device=device)
+
+
+def _conj(v: torch.Tensor, conj: bool) -> torch.Tensor:
+    return v.conj().resolve_conj() if conj and v.is_complex() else v
+
+
+# ------------------------------------------------------------------ to CSR --
+def csr_from_scipy(A, device=None, index_dtype=torch.int64, dtype=None):
+    """GenericSparseMatrixCSR(::SparseMatrixCSC) (conversions.jl:25-30): canonical CSR."""
+    B = A.tocsr(copy=True)
+    B.sum_duplicates()
+    B.sort_indices()
+    dev = _dev(device)
+    vals = B.data if dtype is None else B.data.astype(dtype)
+    return GenericSparseMatrixCSR(B.shape[0], B.shape[1],
+                                  _t(B.indptr.astype(np.int64) + 1, index_dtype, dev),
+                                  _t(B.indices.astype(np.int64) + 1, index_dtype, dev),
+                                  torch.as_tensor(np.ascontiguousarray(vals)).to(dev),
+                                  copy=False)
+
+
+def coo_to_csr(A: GenericSparseMatrixCOO) -> GenericSparseMatrixCSR:
+    """GenericSparseMatrixCSR(::COO) (conversions.jl:282-323)."""
+    dev = require_device(A.rowind, A.colind, A.nzval)
+    nnz = A.nnz()
+    rowptr = torch.empty(A.m + 1, dtype=A.rowind.dtype, device=dev)
+    colval = torch.empty(nnz, dtype=A.rowind.dtype, device=dev)
+    nzval = torch.empty(nnz, dtype=A.nzval.dtype, device=dev)
+    L.call("gsa_coo_to_csr", stream_of(dev), vtype_of(A.nzval), itype_of(A.rowind), A.m, A.n,
+           nnz, ptr(A.rowind), ptr(A.colind), ptr(A.nzval), ptr(rowptr), ptr(colval), ptr(nzval))
+    return GenericSparseMatrixCSR(A.m, A.n, rowptr, colval, nzval, copy=False)
+
+
+def csr_to_csc(A: GenericSparseMatrixCSR) -> GenericSparseMatrixCSC:
+    """GenericSparseMatrixCSC(::CSR) (conversions.jl:85-86) by a stable column sort."""
+    dev = require_device(A.rowptr, A.colval, A.nzval)
+    nnz = A.nnz()
+    colptr = torch.empty(A.n + 1, dtype=A.rowptr.dtype, device=dev)
+    rowval = torch.empty(nnz, dtype=A.rowptr.dtype, device=dev)
+    nzval = torch.empty(nnz, dtype=A.nzval.dtype, device=dev)
+    L.call("gsa_csr_to_csc", stream_of(dev), vtype_of(A.nzval), itype_of(A.rowptr), A.m, A.n,
+           nnz, ptr(A.rowptr), ptr(A.colval), ptr(A.nzval), ptr(colptr), ptr(rowval), ptr(nzval))
+    return GenericSparseMatrixCSC(A.m, A.n, colptr, rowval, nzval, copy=False)
+
+
+def csc_to_csr(A: GenericSparseMatrixCSC) -> GenericSparseMatrixCSR:
+    """GenericSparseMatrixCSR(::CSC) (conversions.jl:104-105): the same sort on Aᵀ's arrays."""
+    dev = require_device(A.colptr, A.rowval, A.nzval)
+    nnz = A.nnz()
+    rowptr = torch.empty(A.m + 1, dtype=A.colptr.dtype, device=dev)
+    colval = torch.empty(nnz, dtype=A.colptr.dtype, device=dev)
+    nzval = torch.empty(nnz, dtype=A.nzval.dtype, device=dev)
+    L.call("gsa_csr_to_csc", stream_of(dev), vtype_of(A.nzval), itype_of(A.colptr), A.n, A.m,
+           nnz, ptr(A.colptr), ptr(A.rowval), ptr(A.nzval), ptr(rowptr), ptr(colval), ptr(nzval))
+    return GenericSparseMatrixCSR(A.m, A.n, rowptr, colval, nzval, copy=False)
+
+
+def to_csr(src, device=None, index_dtype=torch.int64) -> GenericSparseMatrixCSR:
+    if isinstance(src, GenericSparseMatrixCSR):
+        return src.copy()
+    if isinstance(src, GenericSparseMatrixCOO):
+        return coo_to_csr(src)
+    if isinstance(src, GenericSparseMatrixCSC):
+        return csc_to_csr(src)
+    if isinstance(src, Transpose):
+        p = src.parent
+        if isinstance(p, GenericSparseMatrixCSC):  # reinterpretation (conversions.jl:106-123)
+            return GenericSparseMatrixCSR(p.n, p.m, p.colptr, p.rowval, _conj(p.nzval, src.conj))
+        if isinstance(p, GenericSparseMatrixCOO):
+            return coo_to_csr(to_coo(src))
+        if isinstance(p, GenericSparseMatrixCSR):
+            return csc_to_csr(to_csc(src))
+    if _is_scipy(src):
+        return csr_from_scipy(src, device=device, index_dtype=index_dtype)
+    raise ArgumentError(f"cannot convert {type(src).__name__} to GenericSparseMatrixCSR")
+
+
+# ------------------------------------------------------------------ to CSC --
+def csc_from_scipy(A, device=None, index_dtype=torch.int64, dtype=None):
+    """GenericSparseMatrixCSC(::SparseMatrixCSC) (conversions.jl:7-8)."""
+    B = A.tocsc(copy=True)
+    B.sum_duplicates()
+    B.sort_indices()
+    dev = _dev(device)
+    vals = B.data if dtype is None else B.data.astype(dtype)
+    return GenericSparseMatrixCSC(B.shape[0], B.shape[1],
+                                  _t(B.indptr.astype(np.int64) + 1, index_dtype, dev),
+                                  _t(B.indices.astype(np.int64) + 1, index_dtype, dev),
+                                  torch.as_tensor(np.ascontiguousarray(vals)).to(dev),
+                                  copy=False)
+
+
+def coo_to_csc(A: GenericSparseMatrixCOO) -> GenericSparseMatrixCSC:
+    """GenericSparseMatrixCSC(::COO) (conversions.jl:171-212): keys colind*m + rowind."""
+    dev = require_device(A.rowind, A.colind, A.nzval)
+    nnz = A.nnz()
+    colptr = torch.empty(A.n + 1, dtype=A.rowind.dtype, device=dev)
+    rowval = torch.empty(nnz, dtype=A.rowind.dtype, device=dev)
+    nzval = torch.empty(nnz, dtype=A.nzval.dtype, device=dev)
+    L.call("gsa_coo_to_csr", stream_of(dev), vtype_of(A.nzval), itype_of(A.rowind), A.n, A.m,
+           nnz, ptr(A.colind), ptr(A.rowind), ptr(A.nzval), ptr(colptr), ptr(rowval), ptr(nzval))
+    return GenericSparseMatrixCSC(A.m, A.n, colptr, rowval, nzval, copy=False)
+
+
+def to_csc(src, device=None, index_dtype=torch.int64) -> GenericSparseMatrixCSC:
+    if isinstance(src, GenericSparseMatrixCSC):
+        return src.copy()
+    if isinstance(src, GenericSparseMatrixCSR):
+        return csr_to_csc(src)
+    if isinstance(src, GenericSparseMatrixCOO):
+        return coo_to_csc(src)
+    if isinstance(src, Transpose):
+        p = src.parent
+        if isinstance(p, GenericSparseMatrixCSR):  # reinterpretation (conversions.jl:87-103)
+            return GenericSparseMatrixCSC(p.n, p.m, p.rowptr, p.colval, _conj(p.nzval, src.conj))
+        if isinstance(p, GenericSparseMatrixCOO):  # conversions.jl:215-236
+            return coo_to_csc(to_coo(src))
+        if isinstance(p, GenericSparseMatrixCSC):
+            return csr_to_csc(to_csr(src))
+    if _is_scipy(src):
+        return csc_from_scipy(src, device=device, index_dtype=index_dtype)
+    raise ArgumentError(f"cannot convert {type(src).__name__} to GenericSparseMatrixCSC")
+
+
+# ------------------------------------------------------------------ to COO --
+def csr_to_coo(A: GenericSparseMatrixCSR) -> GenericSparseMatrixCOO:
+    """GenericSparseMatrixCOO(::CSR) (conversions.jl:264-280): row-major order."""
+    dev = require_device(A.rowptr, A.colval, A.nzval)
+    nnz = A.nnz()
+    r = torch.empty(nnz, dtype=A.rowptr.dtype, device=dev)
+    c = torch.empty(nnz, dtype=A.rowptr.dtype, device=dev)
+    v = torch.empty(nnz, dtype=A.nzval.dtype, device=dev)
+    L.call("gsa_csr_to_coo", stream_of(dev), vtype_of(A.nzval), itype_of(A.rowptr), A.m, nnz,
+           ptr(A.rowptr), ptr(A.colval), ptr(A.nzval), ptr(r), ptr(c), ptr(v))
+    return GenericSparseMatrixCOO(A.m, A.n, r, c, v, copy=False)
+
+
+def csc_to_coo(A: GenericSparseMatrixCSC) -> GenericSparseMatrixCOO:
+    """GenericSparseMatrixCOO(::CSC) (conversions.jl:153-169): column-major order."""
+    dev = require_device(A.colptr, A.rowval, A.nzval)
+    nnz = A.nnz()
+    r = torch.empty(nnz, dtype=A.colptr.dtype, device=dev)
+    c = torch.empty(nnz, dtype=A.colptr.dtype, device=dev)
+    v = torch.empty(nnz, dtype=A.nzval.dtype, device=dev)
+    L.call("gsa_csr_to_coo", stream_of(dev), vtype_of(A.nzval), itype_of(A.colptr), A.n, nnz,
+           ptr(A.colptr), ptr(A.rowval), ptr(A.nzval), ptr(c), ptr(r), ptr(v))
+    return GenericSparseMatrixCOO(A.m, A.n, r, c, v, copy=False)
+
+
+def coo_from_scipy(A, device=None, index_dtype=torch.int64, dtype=None):
+    """GenericSparseMatrixCOO(::SparseMatrixCSC) (conversions.jl:52-56): findnz order,
+    i.e. column-major."""
+    B = A.tocsc(copy=True)
+    B.sum_duplicates()
+    B.sort_indices()
+    C = B.tocoo()
+    order = np.lexsort((C.row, C.col))
+    dev = _dev(device)
+    vals = C.data[order] if dtype is None else C.data[order].astype(dtype)
+    return GenericSparseMatrixCOO(B.shape[0], B.shape[1],
+                                  _t(C.row[order].astype(np.int64) + 1, index_dtype, dev),
+                                  _t(C.col[order].astype(np.int64) + 1, index_dtype, dev),
+                                  torch.as_tensor(np.ascontiguousarray(vals)).to(dev),
+                                  copy=False)
+
+
+def to_coo(src, device=None, index_dtype=torch.int64) -> GenericSparseMatrixCOO:
+    if isinstance(src, GenericSparseMatrixCOO):
+        return src  # GenericSparseMatrixCOO(A::COO) = A (conversions.jl:128)
+    if isinstance(src, GenericSparseMatrixCSR):
+        return csr_to_coo(src)
+    if isinstance(src, GenericSparseMatrixCSC):
+        return csc_to_coo(src)
+    if isinstance(src, Transpose):
+        p = src.parent
+        base = p if isinstance(p, GenericSparseMatrixCOO) else to_coo(p)
+        # conversions.jl:130-147, 238-258, 326-346: swap the index vectors
+        return GenericSparseMatrixCOO(p.n, p.m, base.colind, base.rowind,
+                                      _conj(base.nzval, src.conj))
+    if _is_scipy(src):
+        return coo_from_scipy(src, device=device, index_dtype=index_dtype)
+    raise ArgumentError(f"cannot convert {type(src).__name__} to GenericSparseMatrixCOO")

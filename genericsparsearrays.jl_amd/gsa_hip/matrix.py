@@ -1,0 +1,466 @@
+"""Sparse matrix types: GenericSparseMatrixCSR / CSC / COO on HIP device storage.
+
+Mirrors the reference's type layer (src/matrix_csr/matrix_csr.jl:17-63,
+src/matrix_csc/matrix_csc.jl:17-63, src/matrix_coo/matrix_coo.jl:17-61):
+dimensions m, n and three storage vectors with ONE-BASED indices, the
+constructor's checks and its copy of the buffers.  Storage is torch tensors
+(used only as device memory); every operation runs in libgsa_hip.so.
+
+Dense operands follow Julia's column-major layout: a dense m x k matrix is a
+torch tensor of shape (m, k) whose stride(0) == 1 (see `dense`), and a
+row-major tensor is read as the Transpose of a column-major one.
+"""
+from __future__ import annotations
+
+import ctypes
+import weakref
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from ._lib import ArgumentError, DimensionMismatch
+
+VTYPES = {torch.float64: L.GSA_F64, torch.float32: L.GSA_F32}
+ITYPES = {torch.int32: L.GSA_I32, torch.int64: L.GSA_I64}
+
+
+def vtype_of(t: torch.Tensor) -> int:
+    try:
+        return VTYPES[t.dtype]
+    except KeyError:
+        raise ArgumentError(f"element type {t.dtype} is not provided by the HIP path "
+                            "(Float64 and Float32 are)") from None
+
+
+def itype_of(t: torch.Tensor) -> int:
+    try:
+        return ITYPES[t.dtype]
+    except KeyError:
+        raise ArgumentError(f"index type {t.dtype} is not provided (Int32 and Int64 are)") from None
+
+
+def ptr(t: torch.Tensor | None):
+    return ctypes.c_void_p(t.data_ptr() if t is not None and t.numel() > 0 else 0)
+
+
+def stream_of(device: torch.device):
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def require_device(*tensors: torch.Tensor) -> torch.device:
+    """All storage on one HIP device (the reference's backend-equality check,
+    src/matrix_csr/matrix_csr.jl:185-190)."""
+    devs = {t.device for t in tensors if t is not None}
+    if len(devs) != 1:
+        raise ArgumentError("All arrays must have the same backend")
+    dev = devs.pop()
+    if dev.type != "cuda":
+        raise ArgumentError(
+            f"storage is on {dev}; the HIP kernel layer needs device storage "
+            "(use adapt('cuda', A)) — there is no CPU path")
+    return dev
+
+
+def dense(a, device="cuda", dtype=None) -> torch.Tensor:
+    """A column-major (Julia layout) device copy of a host/numpy/torch 1-D or 2-D array."""
+    t = torch.as_tensor(np.asarray(a) if not isinstance(a, torch.Tensor) else a)
+    if dtype is not None:
+        t = t.to(dtype)
+    if t.dim() == 1:
+        return t.to(device).contiguous()
+    out = torch.empty((t.shape[1], t.shape[0]), dtype=t.dtype, device=device).t()
+    out.copy_(t)
+    return out
+
+
+def similar_dense(rows: int, k: int | None, dtype, device) -> torch.Tensor:
+    if k is None:
+        return torch.empty(rows, dtype=dtype, device=device)
+    return torch.empty((k, rows), dtype=dtype, device=device).t()
+
+
+class AbstractGenericSparseMatrix:
+    """Common interface (src/core.jl:11-54)."""
+
+    m: int
+    n: int
+
+    @property
+    def shape(self):
+        return (self.m, self.n)
+
+    def size(self, dim=None):
+        return self.shape if dim is None else self.shape[dim - 1]
+
+    def __len__(self):
+        return self.m * self.n
+
+    def nnz(self) -> int:
+        return int(self.nzval.numel())
+
+    def nonzeros(self) -> torch.Tensor:
+        return self.nzval
+
+    @property
+    def dtype(self):
+        return self.nzval.dtype
+
+    @property
+    def index_dtype(self):
+        return self._idx().dtype
+
+    @property
+    def device(self):
+        return self.nzval.device
+
+    def _idx(self):
+        raise NotImplementedError
+
+    # ---- wrappers / arithmetic sugar (Julia operators) ----
+    @property
+    def T(self):
+        return Transpose(self)
+
+    def transpose(self):
+        return Transpose(self)
+
+    def adjoint(self):
+        return Adjoint(self)
+
+    def __matmul__(self, other):
+        from . import ops
+        return ops.times(self, other)
+
+    def __mul__(self, other):
+        from . import ops
+        if isinstance(other, (int, float)):
+            return ops.scale(self, other)
+        return ops.times(self, other)
+
+    def __rmul__(self, other):
+        from . import ops
+        if isinstance(other, (int, float)):
+            return ops.scale(self, other)
+        return NotImplemented
+
+    def __truediv__(self, other):
+        from . import ops
+        return ops.scale(self, 1.0 / other)
+
+    def __add__(self, other):
+        from . import ops
+        return ops.add(self, other)
+
+    def __sub__(self, other):
+        from . import ops
+        return ops.add(self, ops.negate(other))
+
+    def __neg__(self):
+        from . import ops
+        return ops.negate(self)
+
+    def __pos__(self):
+        return self.copy()
+
+    def to_scipy(self):
+        raise NotImplementedError
+
+    def collect(self) -> np.ndarray:
+        """Dense host copy (Base.collect)."""
+        return np.asarray(self.to_scipy().toarray())
+
+
+def _check_common(m, n, tensors, itype_tensor):
+    if m < 0:
+        raise ArgumentError("m must be non-negative")
+    if n < 0:
+        raise ArgumentError("n must be non-negative")
+    devs = {t.device for t in tensors}
+    if len(devs) != 1:
+        raise ArgumentError("All storage vectors must be on the same device/backend.")
+    # SparseArrays.sparse_check_Ti: the dimensions must fit the index type
+    if itype_tensor.dtype == torch.int32 and (m >= 2 ** 31 or n >= 2 ** 31):
+        raise ArgumentError("number of rows/columns does not fit the index type Int32")
+    if itype_tensor.dtype not in ITYPES:
+        raise ArgumentError(f"index type {itype_tensor.dtype} is not supported")
+
+
+class _PlanCache:
+    """Per-matrix cached row-block analysis (gsa_plan) of one pointer array."""
+
+    def __init__(self):
+        self.handle = None
+        self.key = None
+
+    def get(self, ptr_t: torch.Tensor, m: int, nnz: int):
+        key = (ptr_t.data_ptr(), m, nnz, ptr_t.dtype, ptr_t._version)
+        if self.handle is not None and self.key == key:
+            return self.handle
+        self.release()
+        h = ctypes.c_void_p()
+        L.call("gsa_plan_create", stream_of(ptr_t.device), itype_of(ptr_t), m, nnz,
+               ptr(ptr_t), ctypes.byref(h))
+        self.handle, self.key = h, key
+        return h
+
+    def release(self):
+        if self.handle is not None:
+            try:
+                L.lib().gsa_plan_destroy(self.handle)
+            except Exception:
+                pass
+            self.handle = None
+
+    def __del__(self):
+        self.release()
+
+
+class GenericSparseMatrixCSR(AbstractGenericSparseMatrix):
+    """CSR matrix: rowptr (m+1), colval (nnz), nzval (nnz), one-based."""
+
+    def __init__(self, *args, copy: bool = True, device=None, index_dtype=torch.int64):
+        if len(args) == 5:
+            m, n, rowptr, colval, nzval = args
+            m, n = int(m), int(n)
+            _check_common(m, n, (rowptr, colval, nzval), rowptr)
+            if rowptr.numel() != m + 1:
+                raise ArgumentError("rowptr length must be m+1")
+            if colval.numel() != nzval.numel():
+                raise ArgumentError("colval and nzval must have same length")
+            if colval.dtype != rowptr.dtype:
+                raise ArgumentError("rowptr and colval must share the index type")
+            self.m, self.n = m, n
+            self.rowptr = rowptr.clone() if copy else rowptr
+            self.colval = colval.clone() if copy else colval
+            self.nzval = nzval.clone() if copy else nzval
+            self._plan = _PlanCache()
+            return
+        if len(args) == 1:
+            from . import convert
+            src = args[0]
+            out = convert.to_csr(src, device=device, index_dtype=index_dtype)
+            self.m, self.n = out.m, out.n
+            self.rowptr, self.colval, self.nzval = out.rowptr, out.colval, out.nzval
+            self._plan = _PlanCache()
+            return
+        raise ArgumentError("GenericSparseMatrixCSR(m, n, rowptr, colval, nzval) or (A)")
+
+    def _idx(self):
+        return self.colval
+
+    def getrowptr(self):
+        return self.rowptr
+
+    def colvals(self):
+        return self.colval
+
+    def plan(self):
+        return self._plan.get(self.rowptr, self.m, self.nnz())
+
+    def copy(self):
+        return GenericSparseMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        rp = self.rowptr.cpu().numpy().astype(np.int64) - 1
+        cv = self.colval.cpu().numpy().astype(np.int64) - 1
+        return sp.csr_matrix((self.nzval.cpu().numpy(), cv, rp), shape=(self.m, self.n))
+
+    def __repr__(self):
+        return (f"{self.m}×{self.n} GenericSparseMatrixCSR{{{self.dtype}, {self.index_dtype}}} "
+                f"with {self.nnz()} stored entries on {self.device}")
+
+
+class GenericSparseMatrixCSC(AbstractGenericSparseMatrix):
+    """CSC matrix: colptr (n+1), rowval (nnz), nzval (nnz), one-based."""
+
+    def __init__(self, *args, copy: bool = True, device=None, index_dtype=torch.int64):
+        if len(args) == 5:
+            m, n, colptr, rowval, nzval = args
+            m, n = int(m), int(n)
+            _check_common(m, n, (colptr, rowval, nzval), colptr)
+            if colptr.numel() != n + 1:
+                raise ArgumentError("colptr length must be n+1")
+            if rowval.numel() != nzval.numel():
+                raise ArgumentError("rowval and nzval must have same length")
+            if rowval.dtype != colptr.dtype:
+                raise ArgumentError("colptr and rowval must share the index type")
+            self.m, self.n = m, n
+            self.colptr = colptr.clone() if copy else colptr
+            self.rowval = rowval.clone() if copy else rowval
+            self.nzval = nzval.clone() if copy else nzval
+            self._plan = _PlanCache()
+            return
+        if len(args) == 1:
+            from . import convert
+            out = convert.to_csc(args[0], device=device, index_dtype=index_dtype)
+            self.m, self.n = out.m, out.n
+            self.colptr, self.rowval, self.nzval = out.colptr, out.rowval, out.nzval
+            self._plan = _PlanCache()
+            return
+        raise ArgumentError("GenericSparseMatrixCSC(m, n, colptr, rowval, nzval) or (A)")
+
+    def _idx(self):
+        return self.rowval
+
+    def getcolptr(self):
+        return self.colptr
+
+    def rowvals(self):
+        return self.rowval
+
+    def plan(self):
+        return self._plan.get(self.colptr, self.n, self.nnz())
+
+    def copy(self):
+        return GenericSparseMatrixCSC(self.m, self.n, self.colptr, self.rowval, self.nzval)
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        cp = self.colptr.cpu().numpy().astype(np.int64) - 1
+        rv = self.rowval.cpu().numpy().astype(np.int64) - 1
+        return sp.csc_matrix((self.nzval.cpu().numpy(), rv, cp), shape=(self.m, self.n))
+
+    def __repr__(self):
+        return (f"{self.m}×{self.n} GenericSparseMatrixCSC{{{self.dtype}, {self.index_dtype}}} "
+                f"with {self.nnz()} stored entries on {self.device}")
+
+
+class GenericSparseMatrixCOO(AbstractGenericSparseMatrix):
+    """COO matrix: rowind, colind, nzval (nnz each), one-based, any order."""
+
+    def __init__(self, *args, copy: bool = True, device=None, index_dtype=torch.int64):
+        if len(args) == 5:
+            m, n, rowind, colind, nzval = args
+            m, n = int(m), int(n)
+            _check_common(m, n, (rowind, colind, nzval), rowind)
+            if not (rowind.numel() == colind.numel() == nzval.numel()):
+                raise ArgumentError("rowind, colind, and nzval must have same length")
+            if rowind.dtype != colind.dtype:
+                raise ArgumentError("rowind and colind must share the index type")
+            self.m, self.n = m, n
+            self.rowind = rowind.clone() if copy else rowind
+            self.colind = colind.clone() if copy else colind
+            self.nzval = nzval.clone() if copy else nzval
+            return
+        if len(args) == 1:
+            from . import convert
+            out = convert.to_coo(args[0], device=device, index_dtype=index_dtype)
+            self.m, self.n = out.m, out.n
+            self.rowind, self.colind, self.nzval = out.rowind, out.colind, out.nzval
+            return
+        raise ArgumentError("GenericSparseMatrixCOO(m, n, rowind, colind, nzval) or (A)")
+
+    def _idx(self):
+        return self.rowind
+
+    def getrowind(self):
+        return self.rowind
+
+    def getcolind(self):
+        return self.colind
+
+    def copy(self):
+        return GenericSparseMatrixCOO(self.m, self.n, self.rowind, self.colind, self.nzval)
+
+    def to_scipy(self):
+        """SparseMatrixCSC(::COO) sums duplicates (conversions.jl:59-66)."""
+        import scipy.sparse as sp
+        r = self.rowind.cpu().numpy().astype(np.int64) - 1
+        c = self.colind.cpu().numpy().astype(np.int64) - 1
+        return sp.coo_matrix((self.nzval.cpu().numpy(), (r, c)), shape=(self.m, self.n)).tocsc()
+
+    def __repr__(self):
+        return (f"{self.m}×{self.n} GenericSparseMatrixCOO{{{self.dtype}, {self.index_dtype}}} "
+                f"with {self.nnz()} stored entries on {self.device}")
+
+
+SPARSE_TYPES = (GenericSparseMatrixCSR, GenericSparseMatrixCSC, GenericSparseMatrixCOO)
+
+
+class Transpose:
+    """LinearAlgebra.Transpose of a sparse matrix (a lazy wrapper)."""
+
+    conj = False
+
+    def __init__(self, parent):
+        self.parent = parent
+
+    @property
+    def shape(self):
+        return (self.parent.n, self.parent.m)
+
+    def size(self, dim=None):
+        return self.shape if dim is None else self.shape[dim - 1]
+
+    @property
+    def T(self):
+        return self.parent
+
+    def __matmul__(self, other):
+        from . import ops
+        return ops.times(self, other)
+
+    __mul__ = __matmul__
+
+    def __add__(self, other):
+        from . import ops
+        return ops.add(self, other)
+
+    def __sub__(self, other):
+        from . import ops
+        return ops.add(self, ops.negate(other))
+
+    def __neg__(self):
+        from . import ops
+        return type(self)(ops.negate(self.parent))
+
+    def collect(self):
+        return self.parent.collect().T
+
+
+class Adjoint(Transpose):
+    """LinearAlgebra.Adjoint; equal to Transpose for the real element types here."""
+
+    conj = True
+
+
+def transpose(A):
+    return A.parent if isinstance(A, Transpose) and not isinstance(A, Adjoint) else Transpose(A)
+
+
+def adjoint(A):
+    return A.parent if isinstance(A, Adjoint) else Adjoint(A)
+
+
+def adapt(device, A):
+    """Adapt.adapt_structure(to, A) (src/matrix_csr/matrix_csr.jl:65-71): re-home the buffers."""
+    dev = torch.device(device)
+    if isinstance(A, GenericSparseMatrixCSR):
+        return GenericSparseMatrixCSR(A.m, A.n, A.rowptr.to(dev), A.colval.to(dev),
+                                      A.nzval.to(dev))
+    if isinstance(A, GenericSparseMatrixCSC):
+        return GenericSparseMatrixCSC(A.m, A.n, A.colptr.to(dev), A.rowval.to(dev),
+                                      A.nzval.to(dev))
+    if isinstance(A, GenericSparseMatrixCOO):
+        return GenericSparseMatrixCOO(A.m, A.n, A.rowind.to(dev), A.colind.to(dev),
+                                      A.nzval.to(dev))
+    raise ArgumentError(f"cannot adapt {type(A)}")
+
+
+def nnz(A) -> int:
+    return A.nnz()
+
+
+def nonzeros(A):
+    return A.nonzeros()
+
+
+__all__ = [
+    "GenericSparseMatrixCSR", "GenericSparseMatrixCSC", "GenericSparseMatrixCOO",
+    "Transpose", "Adjoint", "transpose", "adjoint", "adapt", "nnz", "nonzeros", "dense",
+]
+
+# keep a module-level registry of live plans so interpreter shutdown frees them first
+_live = weakref.WeakSet()

@@ -1,0 +1,394 @@
+"""Operators of the sparse kernel layer: mul!, *, +, -, kron, dropzeros.
+
+Each function keeps the reference's argument checks, messages and result
+types, then calls libgsa_hip.so.  References are to GenericSparseArrays.jl
+v0.1.2.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import _lib as L
+from ._lib import ArgumentError, DimensionMismatch
+from .matrix import (GenericSparseMatrixCSR, GenericSparseMatrixCSC, GenericSparseMatrixCOO,
+                     SPARSE_TYPES, Transpose, Adjoint, itype_of, vtype_of, ptr, stream_of,
+                     require_device, similar_dense)
+from . import convert
+
+_PROMOTE_OK = {
+    (torch.float32, torch.float32), (torch.float32, torch.float64),
+    (torch.float64, torch.float64),
+}
+
+
+def _unwrap(A):
+    """(parent, opA) for identity / Transpose / Adjoint (src/core.jl:102-113)."""
+    if isinstance(A, Adjoint):
+        return A.parent, L.GSA_OP_C
+    if isinstance(A, Transpose):
+        return A.parent, L.GSA_OP_T
+    return A, L.GSA_OP_N
+
+
+def _dense_operand(B: torch.Tensor):
+    """(tensor, k, ld, opB) for a dense vector or matrix in any stride order."""
+    if B.dim() == 1:
+        if B.stride(0) != 1:
+            B = B.contiguous()
+        return B, 1, max(B.shape[0], 1), L.GSA_OP_N
+    if B.dim() != 2:
+        raise ArgumentError("dense operand must be a vector or a matrix")
+    rows, k = B.shape
+    if B.stride(0) == 1 and (k == 1 or B.stride(1) >= max(rows, 1)):
+        return B, k, max(B.stride(1), rows, 1), L.GSA_OP_N
+    if B.stride(1) == 1 and (rows == 1 or B.stride(0) >= max(k, 1)):
+        return B, k, max(B.stride(0), k, 1), L.GSA_OP_T
+    Bc = B.contiguous()
+    return Bc, k, max(k, 1), L.GSA_OP_T
+
+
+def mul_(C: torch.Tensor, A, B: torch.Tensor, alpha=1.0, beta=0.0):
+    """LinearAlgebra.mul!(C, A, B, α, β) for a sparse A (maybe wrapped) and dense B, C.
+
+    Reference: src/matrix_csr/matrix_csr.jl:145-211, src/matrix_csc/matrix_csc.jl:144-213,
+    src/matrix_coo/matrix_coo.jl:155-221 (same checks and messages)."""
+    Ap, opA = _unwrap(A)
+    if not isinstance(Ap, SPARSE_TYPES):
+        raise ArgumentError(f"mul!: unsupported sparse operand {type(A).__name__}")
+    if not isinstance(B, torch.Tensor) or not isinstance(C, torch.Tensor):
+        raise ArgumentError("mul!: B and C must be device tensors")
+    mA, nA = (Ap.m, Ap.n) if opA == L.GSA_OP_N else (Ap.n, Ap.m)
+    b_rows = B.shape[0]
+    b_cols = 1 if B.dim() == 1 else B.shape[1]
+    c_rows = C.shape[0]
+    c_cols = 1 if C.dim() == 1 else C.shape[1]
+    if nA != b_rows:
+        raise DimensionMismatch(f"second dimension of A, {nA}, does not match the first "
+                                f"dimension of B, {b_rows}")
+    if mA != c_rows:
+        raise DimensionMismatch(f"first dimension of A, {mA}, does not match the first "
+                                f"dimension of C, {c_rows}")
+    if b_cols != c_cols:
+        raise DimensionMismatch(f"second dimension of B, {b_cols}, does not match the second "
+                                f"dimension of C, {c_cols}")
+    T3 = C.dtype
+    for src in (Ap.nzval.dtype, B.dtype):
+        if (src, T3) not in _PROMOTE_OK:
+            raise ArgumentError("element types of A, B, α, and β must be promotable to the "
+                                "element type of C")
+    dev = require_device(C, Ap.nzval, B)
+    vals = Ap.nzval if Ap.nzval.dtype == T3 else Ap.nzval.to(T3)
+    Bt = B if B.dtype == T3 else B.to(T3)
+    Bt, k, ldb, opB = _dense_operand(Bt)
+    # C must be column-major for the kernels; compute into a temp otherwise
+    Cw = C
+    if C.dim() == 2 and not (C.stride(0) == 1 and (c_cols == 1 or C.stride(1) >= max(c_rows, 1))):
+        Cw = similar_dense(c_rows, c_cols, T3, dev)
+        if beta != 0:
+            Cw.copy_(C)
+    ldc = max(c_rows, 1) if Cw.dim() == 1 else max(Cw.stride(1), c_rows, 1)
+    s = stream_of(dev)
+    vt = vtype_of(Cw)
+    if isinstance(Ap, GenericSparseMatrixCSR):
+        it = itype_of(Ap.rowptr)
+        L.call("gsa_spmm_csr", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowptr), ptr(Ap.colval),
+               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
+               Ap.plan() if Ap.nnz() > 0 else None)
+    elif isinstance(Ap, GenericSparseMatrixCSC):
+        it = itype_of(Ap.colptr)
+        L.call("gsa_spmm_csc", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.colptr), ptr(Ap.rowval),
+               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
+               Ap.plan() if Ap.nnz() > 0 else None)
+    else:
+        it = itype_of(Ap.rowind)
+        L.call("gsa_spmm_coo", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowind), ptr(Ap.colind),
+               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta))
+    if Cw is not C:
+        C.copy_(Cw)
+    return C
+
+
+def _result_dtype(*dts):
+    return torch.float64 if torch.float64 in dts else torch.float32
+
+
+def times(A, B):
+    """Base.:*(A, B): sparse x dense allocates and calls mul!; sparse x sparse is SpGEMM."""
+    if isinstance(B, torch.Tensor):
+        Ap, opA = _unwrap(A)
+        rows = Ap.m if opA == L.GSA_OP_N else Ap.n
+        T = _result_dtype(Ap.nzval.dtype, B.dtype)
+        C = similar_dense(rows, None if B.dim() == 1 else B.shape[1], T, B.device)
+        return mul_(C, A, B, 1.0, 0.0)
+    if isinstance(B, (int, float)):
+        return scale(A, B)
+    return spgemm(A, B)
+
+
+# ------------------------------------------------------------ scalar ops ----
+def scale(A, alpha):
+    """α*A / A*α / A/α on the stored values (src/core.jl:26-47)."""
+    if isinstance(A, Transpose):
+        return type(A)(scale(A.parent, alpha))
+    B = A.copy()
+    B.nzval.mul_(alpha)
+    return B
+
+
+def negate(A):
+    """Base.:-(A) (src/matrix_csr/matrix_csr.jl:88-90)."""
+    if isinstance(A, Transpose):
+        return type(A)(negate(A.parent))
+    B = A.copy()
+    B.nzval.neg_()
+    return B
+
+
+# ------------------------------------------------------------ dropzeros -----
+def dropzeros(A):
+    """SparseArrays.dropzeros (src/matrix_csr/matrix_csr.jl:723-745;
+    src/matrix_csc/matrix_csc.jl:728-750; src/matrix_coo/matrix_coo.jl:909-935)."""
+    if isinstance(A, GenericSparseMatrixCOO):
+        dev = require_device(A.rowind, A.colind, A.nzval)
+        s = stream_of(dev)
+        cnt = ctypes.c_int64()
+        L.call("gsa_coo_dropzeros_count", s, vtype_of(A.nzval), A.nnz(), ptr(A.nzval),
+               ctypes.byref(cnt))
+        if cnt.value == A.nnz():
+            return A.copy()
+        r = torch.empty(cnt.value, dtype=A.rowind.dtype, device=dev)
+        c = torch.empty(cnt.value, dtype=A.rowind.dtype, device=dev)
+        v = torch.empty(cnt.value, dtype=A.nzval.dtype, device=dev)
+        L.call("gsa_coo_dropzeros_fill", s, vtype_of(A.nzval), itype_of(A.rowind), A.nnz(),
+               ptr(A.rowind), ptr(A.colind), ptr(A.nzval), ptr(r), ptr(c), ptr(v))
+        return GenericSparseMatrixCOO(A.m, A.n, r, c, v, copy=False)
+    if isinstance(A, GenericSparseMatrixCSR):
+        major, pt, ix = A.m, A.rowptr, A.colval
+    elif isinstance(A, GenericSparseMatrixCSC):
+        major, pt, ix = A.n, A.colptr, A.rowval
+    else:
+        raise ArgumentError(f"dropzeros: unsupported {type(A).__name__}")
+    dev = require_device(pt, ix, A.nzval)
+    s = stream_of(dev)
+    newptr = torch.empty(major + 1, dtype=pt.dtype, device=dev)
+    cnt = ctypes.c_int64()
+    L.call("gsa_csr_dropzeros_count", s, vtype_of(A.nzval), itype_of(pt), major, ptr(pt),
+           ptr(A.nzval), ptr(newptr), ctypes.byref(cnt))
+    newix = torch.empty(cnt.value, dtype=pt.dtype, device=dev)
+    newv = torch.empty(cnt.value, dtype=A.nzval.dtype, device=dev)
+    L.call("gsa_csr_dropzeros_fill", s, vtype_of(A.nzval), itype_of(pt), major, ptr(pt), ptr(ix),
+           ptr(A.nzval), ptr(newptr), ptr(newix), ptr(newv))
+    return type(A)(A.m, A.n, newptr, newix, newv, copy=False)
+
+
+def dropzeros_(A):
+    """SparseArrays.dropzeros! — keeps stored zeros when every entry is zero
+    (src/matrix_csr/matrix_csr.jl:685-721)."""
+    B = dropzeros(A)
+    if B.nnz() == 0 and A.nnz() > 0:
+        return A
+    if isinstance(A, GenericSparseMatrixCSR):
+        A.rowptr, A.colval = B.rowptr, B.colval
+    elif isinstance(A, GenericSparseMatrixCSC):
+        A.colptr, A.rowval = B.colptr, B.rowval
+    else:
+        A.rowind, A.colind = B.rowind, B.colind
+    A.nzval = B.nzval
+    return A
+
+
+# ------------------------------------------------------------ sparse + ------
+def _csr_like_add(A, B, cls, major, ptrA, ixA, ptrB, ixB):
+    dev = require_device(ptrA, ptrB, A.nzval, B.nzval)
+    T = _result_dtype(A.nzval.dtype, B.nzval.dtype)
+    if ptrA.dtype != ptrB.dtype:
+        raise ArgumentError("operands must share the index type")
+    vA = A.nzval.to(T)
+    vB = B.nzval.to(T)
+    s = stream_of(dev)
+    vt, it = vtype_of(vA), itype_of(ptrA)
+    ptrC = torch.empty(major + 1, dtype=ptrA.dtype, device=dev)
+    cnt = ctypes.c_int64()
+    L.call("gsa_csr_add_count", s, vt, it, major, ptr(ptrA), ptr(ixA), ptr(vA), ptr(ptrB), ptr(ixB),
+           ptr(vB), ptr(ptrC), ctypes.byref(cnt))
+    ixC = torch.empty(cnt.value, dtype=ptrA.dtype, device=dev)
+    vC = torch.empty(cnt.value, dtype=T, device=dev)
+    L.call("gsa_csr_add_fill", s, vt, it, major, ptr(ptrA), ptr(ixA), ptr(vA), ptr(ptrB), ptr(ixB),
+           ptr(vB), ptr(ptrC), ptr(ixC), ptr(vC))
+    return cls(A.m, A.n, ptrC, ixC, vC, copy=False)
+
+
+def _coo_add(A, B):
+    dev = require_device(A.rowind, B.rowind, A.nzval, B.nzval)
+    T = _result_dtype(A.nzval.dtype, B.nzval.dtype)
+    if A.rowind.dtype != B.rowind.dtype:
+        raise ArgumentError("operands must share the index type")
+    vA, vB = A.nzval.to(T), B.nzval.to(T)
+    s = stream_of(dev)
+    ctx = ctypes.c_void_p()
+    cnt = ctypes.c_int64()
+    L.call("gsa_coo_add_begin", s, vtype_of(vA), itype_of(A.rowind), A.m, A.n, A.nnz(),
+           ptr(A.rowind), ptr(A.colind), ptr(vA), B.nnz(), ptr(B.rowind), ptr(B.colind), ptr(vB),
+           ctypes.byref(ctx), ctypes.byref(cnt))
+    try:
+        r = torch.empty(cnt.value, dtype=A.rowind.dtype, device=dev)
+        c = torch.empty(cnt.value, dtype=A.rowind.dtype, device=dev)
+        v = torch.empty(cnt.value, dtype=T, device=dev)
+    except BaseException:
+        L.lib().gsa_ctx_destroy(ctx)
+        raise
+    L.call("gsa_coo_add_finish", ctx, ptr(r), ptr(c), ptr(v))
+    return GenericSparseMatrixCOO(A.m, A.n, r, c, v, copy=False)
+
+
+def add(A, B):
+    """Base.:+(A, B) for sparse operands (src/matrix_csr/matrix_csr.jl:290-355,409-440;
+    src/matrix_csc/matrix_csc.jl:292-357,411-442; src/matrix_coo/matrix_coo.jl:300-384,461-582)."""
+    if tuple(A.shape) != tuple(B.shape):
+        raise DimensionMismatch(f"dimensions must match: A has dims {tuple(A.shape)}, "
+                                f"B has dims {tuple(B.shape)}")
+    Ap, opA = _unwrap(A)
+    Bp, opB = _unwrap(B)
+    if type(Ap) is not type(Bp):
+        raise ArgumentError(f"+: mixed formats {type(Ap).__name__} and {type(Bp).__name__}")
+    if isinstance(Ap, GenericSparseMatrixCOO):
+        if opA == L.GSA_OP_N and opB == L.GSA_OP_N:
+            return _coo_add(Ap, Bp)
+        return _coo_add(convert.to_coo(A), convert.to_coo(B))
+    if isinstance(Ap, GenericSparseMatrixCSR):
+        if opA == L.GSA_OP_N and opB == L.GSA_OP_N:
+            return _csr_like_add(Ap, Bp, GenericSparseMatrixCSR, Ap.m, Ap.rowptr, Ap.colval,
+                                 Bp.rowptr, Bp.colval)
+        # transposed operands: through CSC and back (matrix_csr.jl:409-440)
+        return convert.to_csr(add(convert.to_csc(A), convert.to_csc(B)))
+    if isinstance(Ap, GenericSparseMatrixCSC):
+        if opA == L.GSA_OP_N and opB == L.GSA_OP_N:
+            return _csr_like_add(Ap, Bp, GenericSparseMatrixCSC, Ap.n, Ap.colptr, Ap.rowval,
+                                 Bp.colptr, Bp.rowval)
+        return convert.to_csc(add(convert.to_csr(A), convert.to_csr(B)))
+    raise ArgumentError(f"+: unsupported operand {type(A).__name__}")
+
+
+# ------------------------------------------------------------ kron ---------
+def kron(A, B):
+    """LinearAlgebra.kron for sparse operands (src/matrix_coo/matrix_coo.jl:641-699;
+    CSR/CSC route through COO, src/matrix_csr/matrix_csr.jl:442-458)."""
+    Ap, opA = _unwrap(A)
+    Bp, opB = _unwrap(B)
+    if type(Ap) is not type(Bp):
+        raise ArgumentError(f"kron: mixed formats {type(Ap).__name__} and {type(Bp).__name__}")
+    if isinstance(Ap, GenericSparseMatrixCOO):
+        if opA != L.GSA_OP_N or opB != L.GSA_OP_N:
+            return kron(convert.to_coo(A), convert.to_coo(B))
+        return _kron_coo(Ap, Bp)
+    if isinstance(Ap, GenericSparseMatrixCSR):
+        if opA != L.GSA_OP_N or opB != L.GSA_OP_N:
+            return convert.coo_to_csr(_kron_coo(convert.to_coo(A), convert.to_coo(B)))
+        return _kron_ptr(Ap, Bp, GenericSparseMatrixCSR)
+    if isinstance(Ap, GenericSparseMatrixCSC):
+        if opA != L.GSA_OP_N or opB != L.GSA_OP_N:
+            return convert.coo_to_csc(_kron_coo(convert.to_coo(A), convert.to_coo(B)))
+        return _kron_ptr(Ap, Bp, GenericSparseMatrixCSC)
+    raise ArgumentError(f"kron: unsupported operand {type(A).__name__}")
+
+
+def _kron_coo(A, B):
+    dev = require_device(A.rowind, B.rowind, A.nzval, B.nzval)
+    T = _result_dtype(A.nzval.dtype, B.nzval.dtype)
+    it = torch.int64 if torch.int64 in (A.rowind.dtype, B.rowind.dtype) else torch.int32
+    mC, nC = A.m * B.m, A.n * B.n
+    if it == torch.int32 and (mC >= 2 ** 31 or nC >= 2 ** 31):
+        raise ArgumentError("kron result dimensions do not fit Int32 indices")
+    rA, cA, rB, cB = (x.to(it) for x in (A.rowind, A.colind, B.rowind, B.colind))
+    vA, vB = A.nzval.to(T), B.nzval.to(T)
+    nnzC = A.nnz() * B.nnz()
+    r = torch.empty(nnzC, dtype=it, device=dev)
+    c = torch.empty(nnzC, dtype=it, device=dev)
+    v = torch.empty(nnzC, dtype=T, device=dev)
+    L.call("gsa_kron_coo", stream_of(dev), vtype_of(vA), itype_of(r), B.m, B.n, A.nnz(), ptr(rA),
+           ptr(cA), ptr(vA), B.nnz(), ptr(rB), ptr(cB), ptr(vB), ptr(r), ptr(c), ptr(v))
+    return GenericSparseMatrixCOO(mC, nC, r, c, v, copy=False)
+
+
+def _kron_ptr(A, B, cls):
+    csr = cls is GenericSparseMatrixCSR
+    pA, iA = (A.rowptr, A.colval) if csr else (A.colptr, A.rowval)
+    pB, iB = (B.rowptr, B.colval) if csr else (B.colptr, B.rowval)
+    dev = require_device(pA, pB, A.nzval, B.nzval)
+    T = _result_dtype(A.nzval.dtype, B.nzval.dtype)
+    it = torch.int64 if torch.int64 in (pA.dtype, pB.dtype) else torch.int32
+    mC, nC = A.m * B.m, A.n * B.n
+    if it == torch.int32 and (mC >= 2 ** 31 or nC >= 2 ** 31 or A.nnz() * B.nnz() >= 2 ** 31):
+        raise ArgumentError("kron result does not fit Int32 indices")
+    pA, iA, pB, iB = (x.to(it) for x in (pA, iA, pB, iB))
+    vA, vB = A.nzval.to(T), B.nzval.to(T)
+    major = mC if csr else nC
+    nnzC = A.nnz() * B.nnz()
+    pC = torch.empty(major + 1, dtype=it, device=dev)
+    iC = torch.empty(nnzC, dtype=it, device=dev)
+    vC = torch.empty(nnzC, dtype=T, device=dev)
+    if csr:
+        dims = (A.m, A.n, B.m, B.n)
+    else:  # CSC arrays are the CSR arrays of the transpose
+        dims = (A.n, A.m, B.n, B.m)
+    L.call("gsa_kron_csr", stream_of(dev), vtype_of(vA), itype_of(pC), *dims, ptr(pA), ptr(iA),
+           ptr(vA), A.nnz(), ptr(pB), ptr(iB), ptr(vB), B.nnz(), ptr(pC), ptr(iC), ptr(vC))
+    return cls(mC, nC, pC, iC, vC, copy=False)
+
+
+# ------------------------------------------------------------ SpGEMM -------
+def _spgemm_arrays(dev, T, m, kdim, n, pA, iA, vA, nnzA, pB, iB, vB, nnzB):
+    s = stream_of(dev)
+    ctx = ctypes.c_void_p()
+    cnt = ctypes.c_int64()
+    L.call("gsa_spgemm_csr_begin", s, vtype_of(vA), itype_of(pA), m, kdim, n, ptr(pA), ptr(iA),
+           ptr(vA), nnzA, ptr(pB), ptr(iB), ptr(vB), nnzB, ctypes.byref(ctx), ctypes.byref(cnt))
+    try:
+        pC = torch.empty(m + 1, dtype=pA.dtype, device=dev)
+        iC = torch.empty(cnt.value, dtype=pA.dtype, device=dev)
+        vC = torch.empty(cnt.value, dtype=T, device=dev)
+    except BaseException:
+        L.lib().gsa_ctx_destroy(ctx)
+        raise
+    L.call("gsa_spgemm_csr_finish", ctx, ptr(pC), ptr(iC), ptr(vC))
+    return pC, iC, vC
+
+
+def spgemm(A, B):
+    """Base.:*(A, B) for sparse operands (src/matrix_csr/matrix_csr.jl:498-609;
+    src/matrix_csc/matrix_csc.jl:500-611; src/matrix_coo/matrix_coo.jl:584-639)."""
+    if A.size(2) != B.size(1):
+        raise DimensionMismatch(f"second dimension of A, {A.size(2)}, does not match first "
+                                f"dimension of B, {B.size(1)}")
+    Ap, opA = _unwrap(A)
+    Bp, opB = _unwrap(B)
+    if type(Ap) is not type(Bp):
+        raise ArgumentError(f"*: mixed formats {type(Ap).__name__} and {type(Bp).__name__}")
+    if isinstance(Ap, GenericSparseMatrixCOO):
+        return convert.to_coo(spgemm(convert.to_csc(A), convert.to_csc(B)))
+    if isinstance(Ap, GenericSparseMatrixCSR):
+        if opA != L.GSA_OP_N or opB != L.GSA_OP_N:
+            return convert.to_csr(spgemm(convert.to_csc(A), convert.to_csc(B)))
+        dev = require_device(Ap.rowptr, Bp.rowptr, Ap.nzval, Bp.nzval)
+        if Ap.rowptr.dtype != Bp.rowptr.dtype:
+            raise ArgumentError("operands must share the index type")
+        T = _result_dtype(Ap.nzval.dtype, Bp.nzval.dtype)
+        pC, iC, vC = _spgemm_arrays(dev, T, Ap.m, Ap.n, Bp.n, Ap.rowptr, Ap.colval,
+                                    Ap.nzval.to(T), Ap.nnz(), Bp.rowptr, Bp.colval,
+                                    Bp.nzval.to(T), Bp.nnz())
+        return GenericSparseMatrixCSR(Ap.m, Bp.n, pC, iC, vC, copy=False)
+    if isinstance(Ap, GenericSparseMatrixCSC):
+        if opA != L.GSA_OP_N or opB != L.GSA_OP_N:
+            return convert.to_csc(spgemm(convert.to_csr(A), convert.to_csr(B)))
+        dev = require_device(Ap.colptr, Bp.colptr, Ap.nzval, Bp.nzval)
+        if Ap.colptr.dtype != Bp.colptr.dtype:
+            raise ArgumentError("operands must share the index type")
+        T = _result_dtype(Ap.nzval.dtype, Bp.nzval.dtype)
+        # Cᵀ = Bᵀ Aᵀ: the CSC arrays of B and A are CSR arrays of Bᵀ and Aᵀ.
+        pC, iC, vC = _spgemm_arrays(dev, T, Bp.n, Bp.m, Ap.m, Bp.colptr, Bp.rowval,
+                                    Bp.nzval.to(T), Bp.nnz(), Ap.colptr, Ap.rowval,
+                                    Ap.nzval.to(T), Ap.nnz())
+        return GenericSparseMatrixCSC(Ap.m, Bp.n, pC, iC, vC, copy=False)
+    raise ArgumentError(f"*: unsupported operand {type(A).__name__}")

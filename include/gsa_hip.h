@@ -1,0 +1,261 @@
+/*
+ * gsa_hip.h — C ABI of libgsa_hip.so, the gfx950 (MI355X) kernel layer behind
+ * GenericSparseArrays.jl's sparse operators.
+ *
+ * Every entry point takes plain device pointers, sizes and enums and returns an
+ * int status (GSA_OK = 0, negative on error; the message is in gsa_last_error()).
+ * Nothing throws across the ABI.  Index arrays are ONE-BASED, exactly as the
+ * reference stores them (Julia), so buffers are shared without copies.
+ * Dense operands are column-major with a leading dimension, as Julia's
+ * DenseVecOrMat.  Every operation is enqueued on the caller's `stream`
+ * (hipStream_t, NULL = legacy default stream) and does not synchronise, except
+ * the "count"/"begin" calls that return a size to the host, which synchronise
+ * the stream the way the reference's `@allowscalar` read-backs of nnz totals do
+ * (src/matrix_csr/matrix_csr.jl:332,543).
+ *
+ * Reference: GenericSparseArrays.jl v0.1.2 (Project.toml:3).  Each declaration
+ * names the reference method it replaces.
+ */
+#ifndef GSA_HIP_H
+#define GSA_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define GSA_OK               0
+#define GSA_ERR_INVALID     -1  /* bad argument: maps to ArgumentError           */
+#define GSA_ERR_DIM         -2  /* dimension mismatch: maps to DimensionMismatch */
+#define GSA_ERR_HIP         -3  /* HIP runtime error                             */
+#define GSA_ERR_OOM         -4  /* device allocation failed                      */
+#define GSA_ERR_UNSUPPORTED -5  /* type/op combination not provided              */
+
+/* ---- element and index types (Tv, Ti of GenericSparseMatrix*{Tv,Ti}) -------- */
+#define GSA_F64  0   /* Float64                                                 */
+#define GSA_F32  1   /* Float32                                                 */
+#define GSA_BF16 2   /* BFloat16 storage with one Float32 scale per row (serving)*/
+#define GSA_I32  0   /* Int32                                                   */
+#define GSA_I64  1   /* Int64                                                   */
+
+/* ---- operand wrappers: identity / Transpose / Adjoint ----------------------
+ * Mirrors trans_adj_wrappers (src/core.jl:102-113).  For the real element types
+ * provided here Adjoint == Transpose. */
+#define GSA_OP_N 0
+#define GSA_OP_T 1
+#define GSA_OP_C 2
+
+typedef struct gsa_plan_s* gsa_plan;   /* per-matrix cached analysis (row blocks) */
+typedef struct gsa_ctx_s*  gsa_ctx;    /* state carried between two-phase calls   */
+
+/* ========================== runtime ========================================= */
+/* One-time device set-up (memory-pool policy).  Called implicitly by every op. */
+int         gsa_init(void);
+/* Message of the last failing call on this host thread ("" if none). */
+const char* gsa_last_error(void);
+int         gsa_version(void);   /* 10000*major + 100*minor + patch */
+
+/* Device memory/stream helpers for hosts with no array library of their own
+ * (the Julia HipVector storage type): hipMalloc/hipFree/hipMemcpyAsync. */
+int gsa_malloc(void** ptr, int64_t bytes);
+int gsa_free(void* ptr);
+/* kind: 0 = host->device, 1 = device->host, 2 = device->device (async on stream) */
+int gsa_memcpy(void* dst, const void* src, int64_t bytes, int kind, void* stream);
+int gsa_memset(void* dst, int value, int64_t bytes, void* stream);
+int gsa_stream_create(void** stream);
+int gsa_stream_destroy(void* stream);
+int gsa_stream_sync(void* stream);
+
+/* ========================== SpMV / SpMM (mul!) ============================== */
+/* Row-block analysis of a CSR (or, on its column pointer, CSC) matrix: the
+ * owner block of every row, computed once and reused by every mul! on the same
+ * structure.  `ptr` has length `m+1`.  Stream-ordered; the plan is usable on
+ * `stream` immediately. */
+int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz,
+                    const void* ptr, gsa_plan* plan);
+int gsa_plan_destroy(gsa_plan plan);
+
+/* C = beta*C + alpha*opA(A)*opB(B) with A in CSR (m x n, `nnz` stored entries).
+ * C is size(opA(A),1) x k column-major (ldc); opB(B) is size(opA(A),2) x k:
+ * opB = N reads B[i + j*ldb], opB = T reads B[j + i*ldb].
+ * beta == 0 zero-fills C first, beta == 1 leaves it, else scales it.
+ * opA = N is the row gather of kernel_spmatmul_csr_N! and opA = T/C the scatter
+ * of kernel_spmatmul_csr_T!.  `plan` may be NULL (analysis done inline).
+ * Replaces LinearAlgebra.mul!(C, ::W{GenericSparseMatrixCSR}, ::W{DenseVecOrMat}, α, β)
+ * — src/matrix_csr/matrix_csr.jl:145-211, kernels src/matrix_csr/matrix_csr_kernels.jl:1-45. */
+int gsa_spmm_csr(void* stream, int vtype, int itype, int opA,
+                 int64_t m, int64_t n, int64_t k,
+                 const void* rowptr, const void* colval, const void* nzval, int64_t nnz,
+                 const void* B, int64_t ldb, int opB,
+                 void* C, int64_t ldc,
+                 double alpha, double beta, gsa_plan plan);
+
+/* Same for A in CSC (colptr length n+1): opA = N scatters (kernel_spmatmul_csc_N!),
+ * opA = T/C gathers.  `plan` (over colptr) serves the gather direction.
+ * Replaces mul! for GenericSparseMatrixCSC — src/matrix_csc/matrix_csc.jl:144-213,
+ * kernels src/matrix_csc/matrix_csc_kernels.jl:1-45. */
+int gsa_spmm_csc(void* stream, int vtype, int itype, int opA,
+                 int64_t m, int64_t n, int64_t k,
+                 const void* colptr, const void* rowval, const void* nzval, int64_t nnz,
+                 const void* B, int64_t ldb, int opB,
+                 void* C, int64_t ldc,
+                 double alpha, double beta, gsa_plan plan);
+
+/* Same for A in COO (any entry order, duplicates summed).
+ * Replaces mul! for GenericSparseMatrixCOO — src/matrix_coo/matrix_coo.jl:155-221,
+ * kernels src/matrix_coo/matrix_coo_kernels.jl:1-43. */
+int gsa_spmm_coo(void* stream, int vtype, int itype, int opA,
+                 int64_t m, int64_t n, int64_t k,
+                 const void* rowind, const void* colind, const void* nzval, int64_t nnz,
+                 const void* B, int64_t ldb, int opB,
+                 void* C, int64_t ldc,
+                 double alpha, double beta);
+
+/* BF16-stored CSR SpMV/SpMM with a Float32 scale per row, Float32 x/y and
+ * Float32 accumulation (TwoSum-compensated on rows over 256 entries):
+ * y = beta*y + alpha * diag(row_scale) * A_bf16 * x.  Serving precision
+ * variant; no counterpart in the reference (whose Tv is a Julia number type). */
+int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
+                      const void* rowptr, const void* colval, const uint16_t* nzval_bf16,
+                      const float* row_scale, int64_t nnz,
+                      const float* B, int64_t ldb, float* C, int64_t ldc,
+                      float alpha, float beta, gsa_plan plan);
+
+/* ========================== format conversion =============================== */
+/* CSR -> COO (row-major entry order).  Replaces GenericSparseMatrixCOO(::CSR),
+ * src/conversions/conversions.jl:264-280, kernel_csr_to_coo!
+ * src/conversions/conversion_kernels.jl:20-35.  For CSC -> COO
+ * (conversions.jl:153-169, kernel_csc_to_coo! conversion_kernels.jl:2-17) pass
+ * (n, colptr, rowval) and swap the two index outputs. */
+int gsa_csr_to_coo(void* stream, int vtype, int itype, int64_t m, int64_t nnz,
+                   const void* rowptr, const void* colval, const void* nzval,
+                   void* rowind, void* colind, void* nzval_out);
+
+/* COO -> CSR: stable sort by key (row, col), gather, row pointer by
+ * searchsortedfirst; duplicates are kept in input order.  Replaces
+ * GenericSparseMatrixCSR(::COO), src/conversions/conversions.jl:282-323
+ * (keys kernel_make_csr_keys! conversion_kernels.jl:49-57).  For COO -> CSC
+ * (conversions.jl:171-212) pass (n, m, colind, rowind) and read the outputs as
+ * (colptr, rowval). */
+int gsa_coo_to_csr(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnz,
+                   const void* rowind, const void* colind, const void* nzval,
+                   void* rowptr, void* colval, void* nzval_out);
+
+/* CSR (m x n) -> CSC (m x n): storage transpose by a stable sort on the column
+ * index.  Bit-identical to the reference's CSR -> COO -> CSC route,
+ * src/conversions/conversions.jl:85-86.  CSC -> CSR (conversions.jl:104-105)
+ * is the same call on (n, m, colptr, rowval). */
+int gsa_csr_to_csc(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnz,
+                   const void* rowptr, const void* colval, const void* nzval,
+                   void* colptr, void* rowval, void* nzval_out);
+
+/* ========================== sparse + sparse ================================= */
+/* CSR + CSR with exact zeros dropped (merge then dropzeros).  Phase 1 writes the
+ * result row pointer (m+1) and returns nnz(C) (synchronises); phase 2 fills.
+ * Replaces Base.:+(::CSR, ::CSR) src/matrix_csr/matrix_csr.jl:290-355 with
+ * kernels matrix_csr_kernels.jl:148-248 and dropzeros matrix_csr.jl:723-745.
+ * CSC + CSC (src/matrix_csc/matrix_csc.jl:292-357) is the same call on the
+ * column pointers. */
+int gsa_csr_add_count(void* stream, int vtype, int itype, int64_t m,
+                      const void* rowptrA, const void* colA, const void* valA,
+                      const void* rowptrB, const void* colB, const void* valB,
+                      void* rowptrC, int64_t* nnzC);
+int gsa_csr_add_fill(void* stream, int vtype, int itype, int64_t m,
+                     const void* rowptrA, const void* colA, const void* valA,
+                     const void* rowptrB, const void* colB, const void* valB,
+                     const void* rowptrC, void* colC, void* valC);
+
+/* COO + COO: concatenate, stable sort by column-major key, sum duplicate runs in
+ * order, drop exact zeros.  Output is column-major sorted COO.
+ * Replaces Base.:+(::COO, ::COO) src/matrix_coo/matrix_coo.jl:300-384
+ * (kernel_mark_unique_coo!/kernel_compact_coo! matrix_coo_kernels.jl:260-310). */
+int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n,
+                      int64_t nnzA, const void* rowA, const void* colA, const void* valA,
+                      int64_t nnzB, const void* rowB, const void* colB, const void* valB,
+                      gsa_ctx* ctx, int64_t* nnzC);
+int gsa_coo_add_finish(gsa_ctx ctx, void* rowC, void* colC, void* valC);
+
+/* ========================== dropzeros ====================================== */
+/* Replaces SparseArrays.dropzeros(::CSR) src/matrix_csr/matrix_csr.jl:650-683,723-745
+ * (kernels matrix_csr_kernels.jl:410-443); CSC is the same call on colptr. */
+int gsa_csr_dropzeros_count(void* stream, int vtype, int itype, int64_t m,
+                            const void* rowptr, const void* nzval,
+                            void* rowptr_out, int64_t* nnz_out);
+int gsa_csr_dropzeros_fill(void* stream, int vtype, int itype, int64_t m,
+                           const void* rowptr, const void* colval, const void* nzval,
+                           const void* rowptr_out, void* colval_out, void* nzval_out);
+/* Replaces SparseArrays.dropzeros(::COO) src/matrix_coo/matrix_coo.jl:824-854,909-935. */
+int gsa_coo_dropzeros_count(void* stream, int vtype, int64_t nnz, const void* nzval,
+                            int64_t* nnz_out);
+int gsa_coo_dropzeros_fill(void* stream, int vtype, int itype, int64_t nnz,
+                           const void* rowind, const void* colind, const void* nzval,
+                           void* rowind_out, void* colind_out, void* nzval_out);
+
+/* ========================== kron =========================================== */
+/* kron(COO, COO): entry (iA, iB) -> ((iA-1)*mB+iB, (jA-1)*nB+jB, vA*vB), A-major /
+ * B-minor entry order.  Replaces LinearAlgebra.kron(::COO, ::COO)
+ * src/matrix_coo/matrix_coo.jl:641-682, kernel_kron_coo! matrix_coo_kernels.jl:144-183. */
+int gsa_kron_coo(void* stream, int vtype, int itype,
+                 int64_t mB, int64_t nB,
+                 int64_t nnzA, const void* rowA, const void* colA, const void* valA,
+                 int64_t nnzB, const void* rowB, const void* colB, const void* valB,
+                 void* rowC, void* colC, void* valC);
+/* kron(CSR, CSR) -> CSR in canonical order, built directly (no sort) when both
+ * inputs have strictly ascending columns per row, else through kron(COO,COO)
+ * and the COO -> CSR sort, as the reference does (src/matrix_csr/matrix_csr.jl:442-458).
+ * rowptrC has mA*mB+1 entries; nnz(C) = nnzA*nnzB.  CSC is the same call on the
+ * column pointers with (nA, nB) in place of (mA, mB). */
+int gsa_kron_csr(void* stream, int vtype, int itype,
+                 int64_t mA, int64_t nA, int64_t mB, int64_t nB,
+                 const void* rowptrA, const void* colA, const void* valA, int64_t nnzA,
+                 const void* rowptrB, const void* colB, const void* valB, int64_t nnzB,
+                 void* rowptrC, void* colC, void* valC);
+
+/* ========================== SpGEMM ========================================= */
+/* C = A*B, A (m x kdim) and B (kdim x n) in CSR.  Row-wise Gustavson semantics of
+ * kernel_count_nnz_spgemm_csr!/kernel_spgemm_csr! (src/matrix_csr/matrix_csr_kernels.jl:254-343):
+ * per output entry the products are summed in (A-row order, B-row order),
+ * columns ascending, cancellation zeros kept.  begin returns nnz(C)
+ * (synchronises); finish fills (m+1, nnzC, nnzC) and releases ctx.
+ * Replaces Base.:*(::CSR, ::CSR) src/matrix_csr/matrix_csr.jl:498-572.
+ * CSC * CSC (src/matrix_csc/matrix_csc.jl:500-574) is this call with the
+ * operands swapped (B's column arrays first). */
+int gsa_spgemm_csr_begin(void* stream, int vtype, int itype,
+                         int64_t m, int64_t kdim, int64_t n,
+                         const void* rowptrA, const void* colA, const void* valA, int64_t nnzA,
+                         const void* rowptrB, const void* colB, const void* valB, int64_t nnzB,
+                         gsa_ctx* ctx, int64_t* nnzC);
+int gsa_spgemm_csr_finish(gsa_ctx ctx, void* rowptrC, void* colC, void* valC);
+/* Release a ctx without finishing (error paths). */
+int gsa_ctx_destroy(gsa_ctx ctx);
+
+/* ========================== batched uneven CSR SpMM ======================== */
+/* Many independent CSR matrices in ONE persistent launch.  Each instance i has
+ * its own (m_i, n_i, k_i) and pointers; the descriptor table lives in device
+ * memory (gsa_batch_desc, 64 bytes each).  Row work is split into tiles by a
+ * per-batch schedule built on the device (gsa_batch_plan_create), so the
+ * launch is graph-capturable (no host work, no allocation at run time).
+ * No single counterpart in the reference: it replaces a loop of
+ * mul!(C_i, A_i, B_i) calls (src/matrix_csr/matrix_csr.jl:145-211). */
+typedef struct {
+    const void* rowptr;   /* m+1 one-based                     */
+    const void* colval;   /* nnz one-based                     */
+    const void* nzval;    /* nnz values (vtype)                */
+    const void* row_scale;/* BF16 only: m scales, else NULL    */
+    const void* B;        /* n x k column-major, ld = ldb      */
+    void*       C;        /* m x k column-major, ld = ldc      */
+    int32_t m, n, k, ldb, ldc, pad;
+} gsa_batch_desc;
+
+int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
+                          const gsa_batch_desc* host_desc, int rows_per_tile,
+                          gsa_plan* plan);
+/* y_i = alpha*A_i*B_i + beta*C_i for all i; one launch on `stream`. */
+int gsa_batch_spmm(void* stream, gsa_plan plan, float alpha, float beta);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GSA_HIP_H */

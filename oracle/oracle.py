@@ -1,0 +1,253 @@
+"""Python face of the CPU oracle (oracle/gsa_oracle.c).
+
+TEST INFRASTRUCTURE ONLY — the parity checker.  Imported by tests/, by the
+smoke check and by bench.py's CPU-baseline leg; never by the product package.
+
+All functions take and return numpy arrays with ONE-BASED int64 indices (the
+reference's storage), and follow the reference's semantics as documented in
+gsa_oracle.c (each C function cites the reference file:line it restates).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_SO = os.path.join(_HERE, "libgsa_oracle.so")
+_lib = None
+
+_I = np.int64
+_P = ctypes.c_void_p
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_SO):
+            build()
+        _lib = ctypes.CDLL(_SO)
+    return _lib
+
+
+def _p(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
+
+
+def _sfx(dtype):
+    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+
+
+def _ctv(dtype):
+    return ctypes.c_double if np.dtype(dtype) == np.float64 else ctypes.c_float
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=_I)
+
+
+def max_threads() -> int:
+    return int(lib().ora_max_threads())
+
+
+# ---------------------------------------------------------------- products --
+def spmm_csr_N(m, rowptr, colval, nzval, B, C=None, alpha=1.0, beta=0.0, threads=1):
+    """mul!(C, A::CSR, B, α, β) — bit-exact restatement (matrix_csr_kernels.jl:1-22).
+    B, C are numpy arrays (vector or 2-D, any layout); returns C."""
+    T = nzval.dtype
+    B2 = np.asarray(B, dtype=T)
+    vec = B2.ndim == 1
+    Bf = np.asfortranarray(B2.reshape(-1, 1) if vec else B2)
+    k = Bf.shape[1]
+    Cf = np.zeros((m, k), dtype=T, order="F") if C is None else np.array(
+        np.asarray(C, dtype=T).reshape(m, k), order="F", copy=True)
+    fn = getattr(lib(), "ora_spmm_csr_N_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_int64, ctypes.c_int,
+                   _P, ctypes.c_int64, _ctv(T), _ctv(T), ctypes.c_int]
+    rp, cv, nz = _i(rowptr), _i(colval), np.ascontiguousarray(nzval)
+    fn(m, k, _p(rp), _p(cv), _p(nz), _p(Bf), max(Bf.shape[0], 1), 0, _p(Cf), max(m, 1),
+       alpha, beta, int(threads))
+    return Cf[:, 0].copy() if vec else Cf
+
+
+def spmm_csr_T(m, n, rowptr, colval, nzval, B, C=None, alpha=1.0, beta=0.0):
+    """mul!(C, transpose(A::CSR), B, α, β): scatter in storage order
+    (matrix_csr_kernels.jl:24-45); also CSC-N on (colptr, rowval)."""
+    T = nzval.dtype
+    B2 = np.asarray(B, dtype=T)
+    vec = B2.ndim == 1
+    Bf = np.asfortranarray(B2.reshape(-1, 1) if vec else B2)
+    k = Bf.shape[1]
+    Cf = np.zeros((n, k), dtype=T, order="F") if C is None else np.array(
+        np.asarray(C, dtype=T).reshape(n, k), order="F", copy=True)
+    fn = getattr(lib(), "ora_spmm_csr_T_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int64] * 3 + [_P] * 4 + [ctypes.c_int64, ctypes.c_int, _P,
+                                                     ctypes.c_int64, _ctv(T), _ctv(T)]
+    rp, cv, nz = _i(rowptr), _i(colval), np.ascontiguousarray(nzval)
+    fn(m, n, k, _p(rp), _p(cv), _p(nz), _p(Bf), max(Bf.shape[0], 1), 0, _p(Cf), max(n, 1),
+       alpha, beta)
+    return Cf[:, 0].copy() if vec else Cf
+
+
+def spmm_coo(trans, rows_out, rowind, colind, nzval, B, C=None, alpha=1.0, beta=0.0):
+    """mul! for COO (matrix_coo_kernels.jl:1-43), entries in storage order."""
+    T = nzval.dtype
+    B2 = np.asarray(B, dtype=T)
+    vec = B2.ndim == 1
+    Bf = np.asfortranarray(B2.reshape(-1, 1) if vec else B2)
+    k = Bf.shape[1]
+    Cf = np.zeros((rows_out, k), dtype=T, order="F") if C is None else np.array(
+        np.asarray(C, dtype=T).reshape(rows_out, k), order="F", copy=True)
+    fn = getattr(lib(), "ora_spmm_coo_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P,
+                   ctypes.c_int64, ctypes.c_int, _P, ctypes.c_int64, _ctv(T), _ctv(T)]
+    r, c, v = _i(rowind), _i(colind), np.ascontiguousarray(nzval)
+    fn(int(trans), rows_out, k, v.size, _p(r), _p(c), _p(v), _p(Bf), max(Bf.shape[0], 1), 0,
+       _p(Cf), max(rows_out, 1), alpha, beta)
+    return Cf[:, 0].copy() if vec else Cf
+
+
+# ---------------------------------------------------------------- conversions
+def csr_to_coo(m, rowptr, colval, nzval):
+    nnz = colval.size
+    r = np.empty(nnz, _I); c = np.empty(nnz, _I); v = np.empty(nnz, nzval.dtype)
+    fn = getattr(lib(), "ora_csr_to_coo_" + _sfx(nzval.dtype))
+    fn.argtypes = [ctypes.c_int64] + [_P] * 6
+    rp, cv, nz = _i(rowptr), _i(colval), np.ascontiguousarray(nzval)
+    fn(m, _p(rp), _p(cv), _p(nz), _p(r), _p(c), _p(v))
+    return r, c, v
+
+
+def coo_to_csr(m, n, rowind, colind, nzval):
+    """GenericSparseMatrixCSR(::COO) (conversions.jl:282-323)."""
+    nnz = nzval.size
+    rp = np.empty(m + 1, _I); cv = np.empty(nnz, _I); v = np.empty(nnz, nzval.dtype)
+    fn = getattr(lib(), "ora_coo_to_csr_" + _sfx(nzval.dtype))
+    fn.argtypes = [ctypes.c_int64] * 3 + [_P] * 6
+    r, c, nz = _i(rowind), _i(colind), np.ascontiguousarray(nzval)
+    fn(m, n, nnz, _p(r), _p(c), _p(nz), _p(rp), _p(cv), _p(v))
+    return rp, cv, v
+
+
+def coo_to_csc(m, n, rowind, colind, nzval):
+    """GenericSparseMatrixCSC(::COO) (conversions.jl:171-212): keys colind*m + rowind."""
+    return coo_to_csr(n, m, colind, rowind, nzval)
+
+
+def csr_to_csc(m, n, rowptr, colval, nzval):
+    """GenericSparseMatrixCSC(::CSR) = CSC(COO(A)) (conversions.jl:85-86)."""
+    r, c, v = csr_to_coo(m, rowptr, colval, nzval)
+    return coo_to_csc(m, n, r, c, v)
+
+
+def csc_to_csr(m, n, colptr, rowval, nzval):
+    """GenericSparseMatrixCSR(::CSC) = CSR(COO(A)) (conversions.jl:104-105); CSC -> COO
+    is column-major (conversions.jl:153-169)."""
+    c, r, v = csr_to_coo(n, colptr, rowval, nzval)
+    return coo_to_csr(m, n, r, c, v)
+
+
+# ---------------------------------------------------------------- structural
+def csr_add(m, A, B):
+    """A + B for CSR tuples (rowptr, colval, nzval) (matrix_csr.jl:290-355)."""
+    T = A[2].dtype
+    fn = getattr(lib(), "ora_csr_add_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int64] + [_P] * 9
+    fn.restype = ctypes.c_int64
+    a = [_i(A[0]), _i(A[1]), np.ascontiguousarray(A[2], T)]
+    b = [_i(B[0]), _i(B[1]), np.ascontiguousarray(B[2], T)]
+    cnt = fn(m, *map(_p, a), *map(_p, b), _p(None), _p(None), _p(None))
+    rp = np.empty(m + 1, _I); cv = np.empty(cnt, _I); v = np.empty(cnt, T)
+    fn(m, *map(_p, a), *map(_p, b), _p(rp), _p(cv), _p(v))
+    return rp, cv, v
+
+
+def csr_dropzeros(m, rowptr, colval, nzval):
+    T = nzval.dtype
+    fn = getattr(lib(), "ora_csr_dropzeros_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int64] + [_P] * 6
+    fn.restype = ctypes.c_int64
+    a = [_i(rowptr), _i(colval), np.ascontiguousarray(nzval)]
+    cnt = fn(m, *map(_p, a), _p(None), _p(None), _p(None))
+    rp = np.empty(m + 1, _I); cv = np.empty(cnt, _I); v = np.empty(cnt, T)
+    fn(m, *map(_p, a), _p(rp), _p(cv), _p(v))
+    return rp, cv, v
+
+
+def coo_add(m, A, B):
+    """A + B for COO tuples (rowind, colind, nzval) (matrix_coo.jl:300-384)."""
+    T = A[2].dtype
+    fn = getattr(lib(), "ora_coo_add_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, ctypes.c_int64, _P, _P, _P,
+                   _P, _P, _P]
+    fn.restype = ctypes.c_int64
+    a = [_i(A[0]), _i(A[1]), np.ascontiguousarray(A[2], T)]
+    b = [_i(B[0]), _i(B[1]), np.ascontiguousarray(B[2], T)]
+    cnt = fn(m, a[2].size, *map(_p, a), b[2].size, *map(_p, b), _p(None), _p(None), _p(None))
+    r = np.empty(cnt, _I); c = np.empty(cnt, _I); v = np.empty(cnt, T)
+    fn(m, a[2].size, *map(_p, a), b[2].size, *map(_p, b), _p(r), _p(c), _p(v))
+    return r, c, v
+
+
+def coo_dropzeros(rowind, colind, nzval):
+    keep = nzval != 0
+    return _i(rowind)[keep], _i(colind)[keep], nzval[keep]
+
+
+def kron_coo(mB, nB, A, B):
+    """kron(COO, COO) (matrix_coo_kernels.jl:144-183): A-major / B-minor."""
+    T = A[2].dtype
+    nA_, nB_ = A[2].size, B[2].size
+    r = np.empty(nA_ * nB_, _I); c = np.empty(nA_ * nB_, _I); v = np.empty(nA_ * nB_, T)
+    fn = getattr(lib(), "ora_kron_coo_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _P, _P, _P, ctypes.c_int64,
+                   _P, _P, _P, _P, _P, _P]
+    a = [_i(A[0]), _i(A[1]), np.ascontiguousarray(A[2], T)]
+    b = [_i(B[0]), _i(B[1]), np.ascontiguousarray(B[2], T)]
+    fn(mB, nB, nA_, *map(_p, a), nB_, *map(_p, b), _p(r), _p(c), _p(v))
+    return r, c, v
+
+
+def kron_csr(mA, nA, mB, nB, A, B):
+    """kron(CSR, CSR) = CSR(kron(COO(A), COO(B))) (matrix_csr.jl:442-458)."""
+    ca = csr_to_coo(mA, *A)
+    cb = csr_to_coo(mB, *B)
+    r, c, v = kron_coo(mB, nB, ca, cb)
+    return coo_to_csr(mA * mB, nA * nB, r, c, v)
+
+
+def spgemm_csr(m, n, A, B):
+    """A * B for CSR tuples (matrix_csr.jl:498-572, kernels :254-343)."""
+    T = A[2].dtype
+    fn = getattr(lib(), "ora_spgemm_csr_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int64, ctypes.c_int64] + [_P] * 9
+    fn.restype = ctypes.c_int64
+    a = [_i(A[0]), _i(A[1]), np.ascontiguousarray(A[2], T)]
+    b = [_i(B[0]), _i(B[1]), np.ascontiguousarray(B[2], T)]
+    cnt = fn(m, n, *map(_p, a), *map(_p, b), _p(None), _p(None), _p(None))
+    rp = np.empty(m + 1, _I); cv = np.empty(cnt, _I); v = np.empty(cnt, T)
+    fn(m, n, *map(_p, a), *map(_p, b), _p(rp), _p(cv), _p(v))
+    return rp, cv, v
+
+
+def sortperm(keys):
+    k = _i(keys)
+    p = np.empty(k.size, _I)
+    fn = lib().ora_sortperm
+    fn.argtypes = [ctypes.c_int64, _P, _P]
+    fn(k.size, _p(k), _p(p))
+    return p + 1
+
+
+def searchsortedfirst(v, x):
+    fn = lib().ora_searchsortedfirst
+    fn.argtypes = [_P, ctypes.c_int64, ctypes.c_int64]
+    fn.restype = ctypes.c_int64
+    a = _i(v)
+    return int(fn(_p(a), a.size, int(x)))

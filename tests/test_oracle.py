@@ -1,0 +1,246 @@
+"""Pins the CPU oracle (oracle/gsa_oracle.c) before it is trusted as the checker.
+
+Each oracle function is checked against something that does not restate it:
+the reference's literal known answers (golden fixtures), scipy.sparse on
+seeded inputs, exact dense arithmetic on integer-valued matrices, Python's
+stable sort on tiny inputs, and closed forms.
+"""
+import os
+
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+import oracle as O
+from conftest import GOLDEN
+
+
+def g(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def to_sp(m, n, rp, cv, nz):
+    return sp.csr_matrix((nz, cv - 1, rp - 1), shape=(m, n))
+
+
+def int_csr(m, n, density, seed, lo=-4, hi=5):
+    rng = np.random.default_rng(seed)
+    A = sp.random(m, n, density=density, random_state=rng, format="csr")
+    A.data = rng.integers(lo, hi, size=A.data.size).astype(np.float64)
+    A.data[A.data == 0] = 2.0
+    A.sort_indices()
+    return A
+
+
+def arrs(A):
+    A = sp.csr_matrix(A)
+    return A.indptr.astype(np.int64) + 1, A.indices.astype(np.int64) + 1, A.data.copy()
+
+
+# ------------------------------------------------------------- products ----
+def test_doc_example_spmv():
+    d = g("doc_spmv.npz")
+    y = O.spmm_csr_N(3, d["rowptr"], d["colval"], d["nzval"], d["x"])
+    assert np.array_equal(y, d["y"])  # [10, 3, 15], docs/src/index.md:47-61
+
+
+@pytest.mark.parametrize("k", [1, 3])
+def test_spmm_N_integer_exact(k):
+    A = int_csr(60, 45, 0.2, 1)
+    rng = np.random.default_rng(2)
+    B = rng.integers(-3, 4, size=(45, k)).astype(np.float64)
+    C0 = rng.integers(-3, 4, size=(60, k)).astype(np.float64)
+    exp = A.toarray() @ B
+    got = O.spmm_csr_N(60, *arrs(A), B)
+    assert np.array_equal(got, exp)
+    # β scaling and α: C = β C0 + α A B, exact on small integers
+    got2 = O.spmm_csr_N(60, *arrs(A), B, C=C0, alpha=2.0, beta=-3.0)
+    assert np.array_equal(got2, -3.0 * C0 + 2.0 * exp)
+    got3 = O.spmm_csr_N(60, *arrs(A), B, C=C0, alpha=1.0, beta=1.0)
+    assert np.array_equal(got3, C0 + exp)
+
+
+def test_spmm_beta_zero_clears_nan():
+    A = int_csr(20, 10, 0.3, 3)
+    C0 = np.full((20, 1), np.nan)
+    got = O.spmm_csr_N(20, *arrs(A), np.ones(10), C=C0, beta=0.0)
+    assert np.all(np.isfinite(got))
+
+
+def test_spmm_T_integer_exact():
+    A = int_csr(50, 70, 0.15, 4)
+    rng = np.random.default_rng(5)
+    x = rng.integers(-3, 4, size=50).astype(np.float64)
+    got = O.spmm_csr_T(50, 70, *arrs(A), x)
+    assert np.array_equal(got, A.toarray().T @ x)
+
+
+def test_spmm_coo_duplicates_exact():
+    rng = np.random.default_rng(6)
+    r = rng.integers(1, 21, size=200)
+    c = rng.integers(1, 16, size=200)
+    v = rng.integers(-3, 4, size=200).astype(np.float64)
+    D = np.zeros((20, 15))
+    np.add.at(D, (r - 1, c - 1), v)
+    x = rng.integers(-2, 3, size=15).astype(np.float64)
+    assert np.array_equal(O.spmm_coo(0, 20, r, c, v, x), D @ x)
+    xt = rng.integers(-2, 3, size=20).astype(np.float64)
+    assert np.array_equal(O.spmm_coo(1, 15, r, c, v, xt), D.T @ xt)
+
+
+def test_spmm_float_close_to_dense():
+    A = sp.random(200, 150, density=0.05, random_state=np.random.default_rng(7), format="csr")
+    x = np.random.default_rng(8).standard_normal(150)
+    got = O.spmm_csr_N(200, *arrs(A), x)
+    np.testing.assert_allclose(got, A @ x, rtol=1e-13, atol=1e-13)
+
+
+# ------------------------------------------------------------- conversions --
+def test_conversions_random_vs_scipy():
+    d = g("conv_random.npz")
+    m, n = int(d["m"]), int(d["n"])
+    csr = (d["csr_0"], d["csr_1"], d["csr_2"])
+    csc = (d["csc_0"], d["csc_1"], d["csc_2"])
+    rowmaj = (d["coo_rowmajor_0"], d["coo_rowmajor_1"], d["coo_rowmajor_2"])
+    colmaj = (d["coo_colmajor_0"], d["coo_colmajor_1"], d["coo_colmajor_2"])
+    # CSR -> COO is row-major
+    for a, b in zip(O.csr_to_coo(m, *csr), rowmaj):
+        assert np.array_equal(a, b)
+    # CSC -> COO is column-major (pass (n, colptr, rowval) and swap)
+    c, r, v = O.csr_to_coo(n, *csc)
+    for a, b in zip((r, c, v), colmaj):
+        assert np.array_equal(a, b)
+    # COO (either order) -> CSR / CSC canonical
+    for coo in (rowmaj, colmaj):
+        for a, b in zip(O.coo_to_csr(m, n, *coo), csr):
+            assert np.array_equal(a, b)
+        for a, b in zip(O.coo_to_csc(m, n, *coo), csc):
+            assert np.array_equal(a, b)
+    for a, b in zip(O.csr_to_csc(m, n, *csr), csc):
+        assert np.array_equal(a, b)
+    for a, b in zip(O.csc_to_csr(m, n, *csc), csr):
+        assert np.array_equal(a, b)
+
+
+def test_conversion_edges():
+    d = g("conv_edges.npz")
+    m, n = 3, 4
+    for name in ("single", "fullrow", "fullcol"):
+        csr = tuple(d[f"{name}_csr_{i}"] for i in range(3))
+        csc = tuple(d[f"{name}_csc_{i}"] for i in range(3))
+        for a, b in zip(O.csr_to_csc(m, n, *csr), csc):
+            assert np.array_equal(a, b)
+        for a, b in zip(O.csc_to_csr(m, n, *csc), csr):
+            assert np.array_equal(a, b)
+        assert np.array_equal(to_sp(m, n, *csr).toarray(), d[f"{name}_dense"])
+
+
+def test_coo_to_csr_empty_gives_all_ones():
+    rp, cv, v = O.coo_to_csr(4, 3, np.zeros(0, np.int64), np.zeros(0, np.int64), np.zeros(0))
+    assert np.array_equal(rp, np.ones(5, np.int64)) and cv.size == 0
+
+
+def test_coo_to_csr_duplicates_keep_input_order():
+    # brute force on a tiny input: Python's sorted() is stable
+    rng = np.random.default_rng(9)
+    r = rng.integers(1, 4, size=40)
+    c = rng.integers(1, 3, size=40)
+    v = np.arange(40, dtype=np.float64)
+    order = sorted(range(40), key=lambda i: (r[i], c[i]))
+    rp, cv, vv = O.coo_to_csr(3, 2, r, c, v)
+    assert np.array_equal(vv, v[order]) and np.array_equal(cv, c[order])
+    expected_rp = np.r_[np.searchsorted(r[order], np.arange(1, 4), side="left") + 1, 41]
+    assert np.array_equal(rp, expected_rp)
+
+
+def test_sortperm_and_search_primitives():
+    rng = np.random.default_rng(10)
+    keys = rng.integers(0, 7, size=500)
+    assert np.array_equal(O.sortperm(keys), np.argsort(keys, kind="stable") + 1)
+    v = np.sort(rng.integers(0, 50, size=100))
+    for x in (-1, 0, 13, 49, 50, 60):
+        assert O.searchsortedfirst(v, x) == int(np.searchsorted(v, x, side="left")) + 1
+
+
+# ------------------------------------------------------------- + and dropzeros
+def test_add_random_vs_scipy():
+    d = g("add_random.npz")
+    m = int(d["m"])
+    A = (d["A_rowptr"], d["A_colval"], d["A_nzval"])
+    B = (d["B_rowptr"], d["B_colval"], d["B_nzval"])
+    C = O.csr_add(m, A, B)
+    for a, b in zip(C, (d["C_rowptr"], d["C_colval"], d["C_nzval"])):
+        assert np.array_equal(a, b)
+    # COO +: column-major sorted result
+    Acoo = tuple(d[f"Acoo_{i}"] for i in range(3))
+    Bcoo = tuple(d[f"Bcoo_{i}"] for i in range(3))
+    for a, b in zip(O.coo_add(m, Acoo, Bcoo), tuple(d[f"Ccoo_{i}"] for i in range(3))):
+        assert np.array_equal(a, b)
+
+
+def test_add_overlap_fixture_and_self_cancel():
+    d = g("overlap_add.npz")
+    m = int(d["m"])
+    A = (d["A_rowptr"], d["A_colval"], d["A_nzval"])
+    B = (d["B_rowptr"], d["B_colval"], d["B_nzval"])
+    for a, b in zip(O.csr_add(m, A, B), (d["C_rowptr"], d["C_colval"], d["C_nzval"])):
+        assert np.array_equal(a, b)
+    negA = (A[0], A[1], -A[2])
+    rp, cv, v = O.csr_add(m, A, negA)  # A - A has nnz 0 (test/shared/matrix_csr.jl:379-381)
+    assert v.size == 0 and np.array_equal(rp, np.ones(m + 1, np.int64))
+
+
+def test_dropzeros_vs_scipy():
+    A = int_csr(40, 30, 0.2, 15, lo=-1, hi=2)
+    A.data[::3] = 0.0  # stored zeros
+    rp, cv, v = O.csr_dropzeros(40, *arrs(A))
+    B = A.copy()
+    B.eliminate_zeros()
+    for a, b in zip((rp, cv, v), arrs(B)):
+        assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------- kron ---------
+def test_kron_vs_scipy():
+    d = g("kron_random.npz")
+    mA, nA, mB, nB = (int(d[k]) for k in ("mA", "nA", "mB", "nB"))
+    A = (d["A_rowptr"], d["A_colval"], d["A_nzval"])
+    B = (d["B_rowptr"], d["B_colval"], d["B_nzval"])
+    # COO kron is A-major / B-minor (scipy.sparse.kron(coo, coo) has the same order)
+    got = O.kron_coo(mB, nB, O.csr_to_coo(mA, *A), O.csr_to_coo(mB, *B))
+    for a, b in zip(got, (d["Kcoo_0"], d["Kcoo_1"], d["Kcoo_2"])):
+        assert np.array_equal(a, b)
+    for a, b in zip(O.kron_csr(mA, nA, mB, nB, A, B), (d["C_rowptr"], d["C_colval"], d["C_nzval"])):
+        assert np.array_equal(a, b)
+
+
+# ------------------------------------------------------------- SpGEMM -------
+def test_spgemm_vs_scipy():
+    d = g("spgemm_random.npz")
+    m, n = int(d["m"]), int(d["n"])
+    A = (d["A_rowptr"], d["A_colval"], d["A_nzval"])
+    B = (d["B_rowptr"], d["B_colval"], d["B_nzval"])
+    got = O.spgemm_csr(m, n, A, B)
+    for a, b in zip(got, (d["C_rowptr"], d["C_colval"], d["C_nzval"])):
+        assert np.array_equal(a, b)
+
+
+def test_spgemm_keeps_cancellation_zero():
+    # A = [1 1], B = [1; -1]: C = [0] stored explicitly (matrix_csr_kernels.jl:329-341)
+    A = (np.array([1, 3]), np.array([1, 2]), np.array([1.0, 1.0]))
+    B = (np.array([1, 2, 3]), np.array([1, 1]), np.array([1.0, -1.0]))
+    rp, cv, v = O.spgemm_csr(1, 1, A, B)
+    assert np.array_equal(rp, [1, 2]) and np.array_equal(cv, [1]) and v[0] == 0.0
+
+
+def test_spgemm_integer_exact_vs_dense():
+    A = int_csr(30, 25, 0.2, 16)
+    B = int_csr(25, 20, 0.2, 17)
+    rp, cv, v = O.spgemm_csr(30, 20, arrs(A), arrs(B))
+    D = to_sp(30, 20, rp, cv, v).toarray()
+    assert np.array_equal(D, A.toarray() @ B.toarray())
+    # pattern = structural product (every (i,j) with some A_ik B_kj stored)
+    S = (sp.csr_matrix((np.ones_like(A.data), A.indices, A.indptr), shape=A.shape)
+         @ sp.csr_matrix((np.ones_like(B.data), B.indices, B.indptr), shape=B.shape)).tocsr()
+    S.sort_indices()
+    assert np.array_equal(rp - 1, S.indptr) and np.array_equal(cv - 1, S.indices)
