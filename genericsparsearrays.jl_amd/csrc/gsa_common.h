@@ -89,17 +89,34 @@ int dispatch_vi(int vtype, int itype, F&& f) {
 // lies in [b*NB, (b+1)*NB).  blk_row[b] is the first owned row, blk_row[nblocks]
 // = m.  Built on the device in O(m).
 constexpr int SPMV_NB = 2048;
+constexpr int SPMV_CAP = SPMV_NB + 256;  // entries a workgroup stages in LDS
+
+// Per-window staging summary: P0 (first owned entry), Ps (staged end), Pend
+// (end of the last owned row) and a spill flag (last row read from HBM).
+constexpr int PLAN_POS = 4;
+
+// Wavefront windows of the SpMV kernel: each wavefront owns the rows that
+// start in its SPMV_WNB-entry window and stages up to SPMV_WCAP entries.
+constexpr int SPMV_WNB = 512;
+constexpr int SPMV_WCAP = SPMV_WNB + 64;
 
 struct CsrPlanData {
     int itype = 0;
     int64_t m = 0, nnz = 0;
-    int64_t nblocks = 0;
+    int64_t nblocks = 0;          // workgroup windows (SPMV_NB)
     int64_t* blk_row = nullptr;   // device, nblocks+1
+    int64_t* blk_pos = nullptr;   // device, PLAN_POS*nblocks
+    int64_t nwin = 0;             // wavefront windows (SPMV_WNB)
+    int64_t* win_row = nullptr;   // device, nwin+1
+    int64_t* win_pos = nullptr;   // device, PLAN_POS*nwin
     const void* ptr = nullptr;    // the row pointer the plan was built on
 };
 
+// Row ownership (and, if blk_pos is given, the staging summary) of windows of
+// `nb` entries with staging capacity `cap`.
 int build_csr_plan(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr,
-                   int64_t* blk_row, int64_t nblocks);
+                   int64_t* blk_row, int64_t nblocks, int64_t* blk_pos = nullptr,
+                   int nb = SPMV_NB, int cap = SPMV_CAP);
 
 }  // namespace gsa
 

@@ -29,12 +29,16 @@
 namespace gsa {
 
 constexpr int WG = 256;
-constexpr int SPMV_CAP = SPMV_NB + 256;  // LDS staging capacity (entries)
-constexpr int LONG_T = 48;               // rows longer than this: one wavefront each
-constexpr int MAX_LONG = SPMV_CAP / (LONG_T + 1) + 1;
 
 __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
     return v < lo ? lo : (v > hi ? hi : v);
+}
+
+// Streamed-once operands (indices, values, row pointers) are read with
+// non-temporal loads so they do not evict the gathered x from L2.
+template <typename U>
+__device__ __forceinline__ U ldnt(const U* p) {
+    return __builtin_nontemporal_load(p);
 }
 
 template <typename T>
@@ -58,64 +62,245 @@ __device__ __forceinline__ void store_out(T* __restrict__ y, int64_t idx, T acc,
     y[idx] = base + t;
 }
 
-// ------------------------------------------------------------ plan kernel --
+// ------------------------------------------------------------ plan kernels --
 template <typename I>
 __global__ void k_csr_plan(int64_t m, int64_t nnz, const I* __restrict__ ptr,
-                           int64_t* __restrict__ blk_row, int64_t nblocks) {
+                           int64_t* __restrict__ blk_row, int64_t nblocks, int nb) {
     int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r > m) return;
     int64_t cur = (r == m) ? nnz : clamp64((int64_t)ptr[r] - 1, 0, nnz);
     int64_t bstart = 0;
-    if (r > 0) bstart = clamp64((int64_t)ptr[r - 1] - 1, 0, nnz) / SPMV_NB + 1;
-    int64_t bend = cur / SPMV_NB;
+    if (r > 0) bstart = clamp64((int64_t)ptr[r - 1] - 1, 0, nnz) / nb + 1;
+    int64_t bend = cur / nb;
     for (int64_t b = bstart; b <= bend && b < nblocks; ++b) blk_row[b] = r;
     if (r == m) blk_row[nblocks] = m;
 }
 
+// Per window: P0, Ps, Pend, spill (see PLAN_POS).  The last owned row is
+// "spilled" (reduced from HBM by the whole workgroup) when the window's rows
+// do not fit the LDS staging capacity.
+template <typename I>
+__global__ void k_csr_plan_pos(int64_t m, int64_t nnz, const I* __restrict__ ptr,
+                               const int64_t* __restrict__ blk_row, int64_t nblocks,
+                               int64_t* __restrict__ pos, int cap) {
+    int64_t b = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (b >= nblocks) return;
+    int64_t r0 = clamp64(blk_row[b], 0, m), r1 = clamp64(blk_row[b + 1], r0, m);
+    int64_t p0 = (r0 < m) ? clamp64((int64_t)ptr[r0] - 1, 0, nnz) : nnz;
+    int64_t pend = clamp64((int64_t)ptr[r1] - 1, p0, nnz);
+    int64_t ps = pend, spill = 0;
+    if (r1 > r0 && pend - p0 > cap) {
+        ps = clamp64((int64_t)ptr[r1 - 1] - 1, p0, p0 + cap);
+        spill = 1;
+    }
+    pos[PLAN_POS * b + 0] = p0;
+    pos[PLAN_POS * b + 1] = ps;
+    pos[PLAN_POS * b + 2] = pend;
+    pos[PLAN_POS * b + 3] = spill;
+}
+
 int build_csr_plan(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr,
-                   int64_t* blk_row, int64_t nblocks) {
+                   int64_t* blk_row, int64_t nblocks, int64_t* blk_pos, int nb, int cap) {
     GSA_HIP(hipMemsetAsync(blk_row, 0, sizeof(int64_t) * (nblocks + 1), s));
     return dispatch_i(itype, [&](auto ti) {
         using I = decltype(ti);
         unsigned g = (unsigned)cdiv(m + 1, 256);
         hipLaunchKernelGGL(k_csr_plan<I>, dim3(g), dim3(256), 0, s, m, nnz,
-                           (const I*)ptr, blk_row, nblocks);
+                           (const I*)ptr, blk_row, nblocks, nb);
+        if (blk_pos)
+            hipLaunchKernelGGL(k_csr_plan_pos<I>, dim3((unsigned)cdiv(nblocks, 256)), dim3(256), 0, s,
+                               m, nnz, (const I*)ptr, (const int64_t*)blk_row, nblocks, blk_pos, cap);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
     });
 }
 
+// Window of block b under the plan.
+struct Window {
+    int64_t r0, r1, P0, Ps, Pend, rs_end;
+    bool spill;
+};
+
+__device__ __forceinline__ Window load_window(int64_t b, int64_t m, int64_t nnz,
+                                              const int64_t* __restrict__ blk_row,
+                                              const int64_t* __restrict__ pos,
+                                              int cap = SPMV_CAP) {
+    Window w;
+    w.r0 = clamp64(blk_row[b], 0, m);
+    w.r1 = clamp64(blk_row[b + 1], w.r0, m);
+    w.P0 = clamp64(pos[PLAN_POS * b + 0], 0, nnz);
+    w.Pend = clamp64(pos[PLAN_POS * b + 2], w.P0, nnz);
+    w.Ps = clamp64(pos[PLAN_POS * b + 1], w.P0, w.P0 + cap);
+    if (w.Ps > w.Pend) w.Ps = w.Pend;
+    w.spill = pos[PLAN_POS * b + 3] != 0 && w.r1 > w.r0;
+    w.rs_end = w.spill ? w.r1 - 1 : w.r1;
+    return w;
+}
+
 // ------------------------------------------------------------ SpMV kernel --
+// One wavefront per SPMV_WNB-entry window; the WAVES wavefronts of a
+// workgroup are independent (no workgroup barrier), so while some wait on
+// their x gathers others stream indices and values.
+constexpr int WAVES = 4;
+constexpr int WU = SPMV_WCAP / 64;       // staged entries per lane
+constexpr int WRP = 256;                 // row offsets staged per wavefront
+constexpr int WRPL = WRP / 64;
+constexpr int W_LONG = 32;               // staged rows longer than this: the whole wavefront
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 template <typename T, typename I>
-__global__ __launch_bounds__(WG) void k_csr_spmv(
+__global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
     int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowptr,
     const I* __restrict__ colval, const T* __restrict__ nzval, const T* __restrict__ x,
-    T* __restrict__ y, T alpha, T beta, int mode, const int64_t* __restrict__ blk_row) {
-    __shared__ T prod[SPMV_CAP];
-    __shared__ int64_t long_rows[MAX_LONG];
+    T* __restrict__ y, T alpha, T beta, int mode, const int64_t* __restrict__ win_row,
+    const int64_t* __restrict__ win_pos, int64_t nwin) {
+    __shared__ T s_prod[WAVES][SPMV_WCAP];
+    __shared__ int s_off[WAVES][WRP];
+
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int64_t w = (int64_t)blockIdx.x * WAVES + wv;
+    if (w >= nwin) return;
+    const Window win = load_window(w, m, nnz, win_row, win_pos, SPMV_WCAP);
+    if (win.r0 >= win.r1) return;
+    const int64_t P0 = win.P0, Ps = win.Ps, Pend = win.Pend, r0 = win.r0, r1 = win.r1;
+    const int64_t rs_end = win.rs_end;
+    const int cnt = (int)(Ps - P0);
+    const int nst = (int)(rs_end - r0 < WRP - 1 ? rs_end - r0 : WRP - 1);
+    const bool rp_in_lds = rs_end - r0 + 1 <= WRP;
+    T* prod = s_prod[wv];
+    int* off = s_off[wv];
+
+    // 1. row offsets, indices and values in flight together, then the gathers
+    I rpv[WRPL];
+#pragma unroll
+    for (int k = 0; k < WRPL; ++k) {
+        int i = lane + 64 * k;
+        i = i <= nst ? i : nst;
+        rpv[k] = ldnt(rowptr + r0 + i);
+    }
+    if (cnt > 0) {
+        I ci[WU];
+        T vv[WU], xv[WU];
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+            int i = u * 64 + lane;
+            i = i < cnt ? i : cnt - 1;
+            ci[u] = ldnt(colval + P0 + i);
+            vv[u] = ldnt(nzval + P0 + i);
+        }
+#pragma unroll
+        for (int u = 0; u < WU; ++u) xv[u] = x[clamp64((int64_t)ci[u] - 1, 0, n - 1)];
+#pragma unroll
+        for (int u = 0; u < WU; ++u) {
+            int i = u * 64 + lane;
+            if (i < cnt) prod[i] = vv[u] * xv[u];
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < WRPL; ++k) {
+        int i = lane + 64 * k;
+        if (i <= nst) off[i] = (int)(clamp64((int64_t)rpv[k] - 1, P0, Ps) - P0);
+    }
+    wave_lds_sync();
+
+    // 2. one lane per short row (left-to-right sum); longer staged rows are
+    //    reduced by the whole wavefront, one at a time
+    for (int64_t base = r0; base < rs_end; base += 64) {
+        const int64_t r = base + lane;
+        const bool act = r < rs_end;
+        int s = 0, e = 0;
+        if (act) {
+            if (rp_in_lds) {
+                s = off[r - r0];
+                e = off[r - r0 + 1];
+            } else {
+                s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
+                e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
+            }
+        }
+        const bool lng = act && (e - s > W_LONG);
+        if (act && !lng) {
+            T acc = T(0);
+            for (int j = s; j < e; ++j) acc = acc + prod[j];
+            store_out(y, r, acc, alpha, beta, mode);
+        }
+        uint64_t lm = __ballot(lng);
+        while (lm) {
+            const int src = __ffsll((long long)lm) - 1;
+            lm &= lm - 1;
+            const int ls = __shfl(s, src, 64), le = __shfl(e, src, 64);
+            T acc = T(0);
+            for (int j = ls + lane; j < le; j += 64) acc = acc + prod[j];
+            acc = wave_sum(acc);
+            if (lane == 0) store_out(y, base + src, acc, alpha, beta, mode);
+        }
+    }
+
+    // 3. spill row: the wavefront reduces it from HBM
+    if (rs_end < r1) {
+        const int64_t r = r1 - 1;
+        T acc = T(0);
+        int64_t j = Ps + lane;
+        for (; j + 3 * 64 < Pend; j += 4 * 64) {
+            I c0 = ldnt(colval + j), c1 = ldnt(colval + j + 64);
+            I c2 = ldnt(colval + j + 128), c3 = ldnt(colval + j + 192);
+            T v0 = ldnt(nzval + j), v1 = ldnt(nzval + j + 64);
+            T v2 = ldnt(nzval + j + 128), v3 = ldnt(nzval + j + 192);
+            T p0 = v0 * x[clamp64((int64_t)c0 - 1, 0, n - 1)];
+            T p1 = v1 * x[clamp64((int64_t)c1 - 1, 0, n - 1)];
+            T p2 = v2 * x[clamp64((int64_t)c2 - 1, 0, n - 1)];
+            T p3 = v3 * x[clamp64((int64_t)c3 - 1, 0, n - 1)];
+            acc = acc + p0; acc = acc + p1; acc = acc + p2; acc = acc + p3;
+        }
+        for (; j < Pend; j += 64)
+            acc = acc + ldnt(nzval + j) * x[clamp64((int64_t)ldnt(colval + j) - 1, 0, n - 1)];
+        acc = wave_sum(acc);
+        if (lane == 0) store_out(y, r, acc, alpha, beta, mode);
+    }
+}
+
+// ------------------------------------------------------------ SpMM kernel --
+constexpr int SPMM_LONG = 64;  // staged rows longer than this are split over the workgroup
+constexpr int SPMM_MAX_LONG = SPMV_CAP / (SPMM_LONG + 1) + 1;
+constexpr int SPMM_RP = 1024;  // row offsets staged in LDS per workgroup
+
+// Dense operand read as BT[col*ldbt + kk] (row-major).  LPR lanes per row.
+template <typename T, typename I, typename CI, int LPR>
+__global__ __launch_bounds__(WG) void k_csr_spmm(
+    int64_t m, int64_t n, int64_t k, int64_t nnz, const I* __restrict__ rowptr,
+    const I* __restrict__ colval, const T* __restrict__ nzval, const T* __restrict__ BT,
+    int64_t ldbt, T* __restrict__ C, int64_t ldc, T alpha, T beta, int mode,
+    const int64_t* __restrict__ blk_row, const int64_t* __restrict__ blk_pos) {
+    __shared__ CI s_col[SPMV_CAP];
+    __shared__ T s_val[SPMV_CAP];
+    __shared__ T red[WG];
+    __shared__ int s_off[SPMM_RP];
+    __shared__ int64_t long_rows[SPMM_MAX_LONG];
     __shared__ int nlong;
-    __shared__ T red[WG / 64];
 
     const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
-    const int64_t r0 = clamp64(blk_row[b], 0, m);
-    const int64_t r1 = clamp64(blk_row[b + 1], r0, m);
-    if (r0 >= r1) return;
-
-    const int64_t P0 = clamp64((int64_t)rowptr[r0] - 1, 0, nnz);
-    const int64_t Pend = clamp64((int64_t)rowptr[r1] - 1, P0, nnz);
-    int64_t rs_end, Ps;
-    if (Pend - P0 <= SPMV_CAP) {
-        rs_end = r1;
-        Ps = Pend;
-    } else {
-        rs_end = r1 - 1;
-        Ps = clamp64((int64_t)rowptr[r1 - 1] - 1, P0, P0 + SPMV_CAP);
-    }
+    const Window w = load_window(blockIdx.x, m, nnz, blk_row, blk_pos);
+    if (w.r0 >= w.r1) return;
+    const int64_t P0 = w.P0, Ps = w.Ps, Pend = w.Pend, r0 = w.r0, r1 = w.r1;
+    const int64_t rs_end = w.rs_end;
     const int cnt = (int)(Ps - P0);
-    if (tid == 0) nlong = 0;
+    const int nst = (int)(rs_end - r0 < SPMM_RP - 1 ? rs_end - r0 : SPMM_RP - 1);
+    const bool rp_in_lds = rs_end - r0 + 1 <= SPMM_RP;
 
-    // 1. stage products (coalesced index/value loads, 8 in flight per lane)
+    // stage row offsets, column indices and values (one round trip)
+    I rpv[SPMM_RP / WG];
+#pragma unroll
+    for (int q = 0; q < SPMM_RP / WG; ++q) {
+        int i = tid + q * WG;
+        i = i <= nst ? i : nst;
+        rpv[q] = ldnt(rowptr + r0 + i);
+    }
     constexpr int U = 8;
     for (int base = 0; base < cnt; base += WG * U) {
         I ci[U];
@@ -123,143 +308,141 @@ __global__ __launch_bounds__(WG) void k_csr_spmv(
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             int i = base + u * WG + tid;
-            if (i < cnt) { ci[u] = colval[P0 + i]; vv[u] = nzval[P0 + i]; }
+            i = i < cnt ? i : cnt - 1;
+            ci[u] = ldnt(colval + P0 + i);
+            vv[u] = ldnt(nzval + P0 + i);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             int i = base + u * WG + tid;
             if (i < cnt) {
-                int64_t c = clamp64((int64_t)ci[u] - 1, 0, n - 1);
-                prod[i] = vv[u] * x[c];
+                s_col[i] = (CI)clamp64((int64_t)ci[u] - 1, 0, n - 1);
+                s_val[i] = vv[u];
             }
         }
     }
-    __syncthreads();
-
-    // 2. one lane per short row, left-to-right sum
-    for (int64_t r = r0 + tid; r < rs_end; r += WG) {
-        int s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
-        int e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
-        if (e - s <= LONG_T) {
-            T acc = T(0);
-            for (int j = s; j < e; ++j) acc = acc + prod[j];
-            store_out(y, r, acc, alpha, beta, mode);
-        } else {
-            int slot = atomicAdd(&nlong, 1);
-            if (slot < MAX_LONG) long_rows[slot] = r;
-        }
+#pragma unroll
+    for (int q = 0; q < SPMM_RP / WG; ++q) {
+        int i = tid + q * WG;
+        if (i <= nst) s_off[i] = (int)(clamp64((int64_t)rpv[q] - 1, P0, Ps) - P0);
     }
-    __syncthreads();
-
-    // 3. one wavefront per long staged row
-    const int wave = tid >> 6, lane = tid & 63;
-    const int nl = nlong < MAX_LONG ? nlong : MAX_LONG;
-    for (int q = wave; q < nl; q += WG / 64) {
-        int64_t r = long_rows[q];
-        int s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
-        int e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
-        T acc = T(0);
-        for (int j = s + lane; j < e; j += 64) acc = acc + prod[j];
-        acc = wave_sum(acc);
-        if (lane == 0) store_out(y, r, acc, alpha, beta, mode);
-    }
-
-    // 4. spill row: whole workgroup from HBM
-    if (rs_end < r1) {
-        const int64_t r = r1 - 1;
-        T acc = T(0);
-        int64_t j = Ps + tid;
-        for (; j + 3 * WG < Pend; j += 4 * WG) {
-            I c0 = colval[j], c1 = colval[j + WG], c2 = colval[j + 2 * WG], c3 = colval[j + 3 * WG];
-            T v0 = nzval[j], v1 = nzval[j + WG], v2 = nzval[j + 2 * WG], v3 = nzval[j + 3 * WG];
-            T p0 = v0 * x[clamp64((int64_t)c0 - 1, 0, n - 1)];
-            T p1 = v1 * x[clamp64((int64_t)c1 - 1, 0, n - 1)];
-            T p2 = v2 * x[clamp64((int64_t)c2 - 1, 0, n - 1)];
-            T p3 = v3 * x[clamp64((int64_t)c3 - 1, 0, n - 1)];
-            acc = acc + p0; acc = acc + p1; acc = acc + p2; acc = acc + p3;
-        }
-        for (; j < Pend; j += WG) acc = acc + nzval[j] * x[clamp64((int64_t)colval[j] - 1, 0, n - 1)];
-        acc = wave_sum(acc);
-        if (lane == 0) red[wave] = acc;
-        __syncthreads();
-        if (tid == 0) {
-            T s = red[0];
-            for (int w = 1; w < WG / 64; ++w) s = s + red[w];
-            store_out(y, r, s, alpha, beta, mode);
-        }
-    }
-}
-
-// ------------------------------------------------------------ SpMM kernel --
-// Dense operand read as BT[col*ldbt + kk] (row-major).  LPR lanes per row.
-template <typename T, typename I, int LPR>
-__global__ __launch_bounds__(WG) void k_csr_spmm(
-    int64_t m, int64_t n, int64_t k, int64_t nnz, const I* __restrict__ rowptr,
-    const I* __restrict__ colval, const T* __restrict__ nzval, const T* __restrict__ BT,
-    int64_t ldbt, T* __restrict__ C, int64_t ldc, T alpha, T beta, int mode,
-    const int64_t* __restrict__ blk_row) {
-    __shared__ I s_col[SPMV_CAP];
-    __shared__ T s_val[SPMV_CAP];
-    __shared__ T red[WG];
-
-    const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
-    const int64_t r0 = clamp64(blk_row[b], 0, m);
-    const int64_t r1 = clamp64(blk_row[b + 1], r0, m);
-    if (r0 >= r1) return;
-
-    const int64_t P0 = clamp64((int64_t)rowptr[r0] - 1, 0, nnz);
-    const int64_t Pend = clamp64((int64_t)rowptr[r1] - 1, P0, nnz);
-    int64_t rs_end, Ps;
-    if (Pend - P0 <= SPMV_CAP) { rs_end = r1; Ps = Pend; }
-    else { rs_end = r1 - 1; Ps = clamp64((int64_t)rowptr[r1 - 1] - 1, P0, P0 + SPMV_CAP); }
-    const int cnt = (int)(Ps - P0);
-
-    for (int i = tid; i < cnt; i += WG) {
-        s_col[i] = (I)clamp64((int64_t)colval[P0 + i] - 1, 0, n - 1);
-        s_val[i] = nzval[P0 + i];
-    }
+    if (tid == 0) nlong = 0;
     __syncthreads();
 
     constexpr int GROUPS = WG / LPR;
     const int g = tid / LPR, lane = tid % LPR;
+    auto row_range = [&](int64_t r, int& s, int& e) {
+        if (rp_in_lds) {
+            s = s_off[r - r0];
+            e = s_off[r - r0 + 1];
+        } else {
+            s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
+            e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
+        }
+    };
+    // rows up to SPMM_LONG entries: one group of LPR lanes per row, each lane
+    // one (row, kk) output summed left to right with 8 gathers in flight
     for (int64_t r = r0 + g; r < rs_end; r += GROUPS) {
-        int s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
-        int e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
+        int s, e;
+        row_range(r, s, e);
+        if (e - s > SPMM_LONG) {
+            if (lane == 0) {
+                int q = atomicAdd(&nlong, 1);
+                if (q < SPMM_MAX_LONG) long_rows[q] = r;
+            }
+            continue;
+        }
         for (int64_t kk = lane; kk < k; kk += LPR) {
             T acc = T(0);
-            int j = s;
-            for (; j + 1 < e; j += 2) {
-                T b0 = BT[(int64_t)s_col[j] * ldbt + kk];
-                T b1 = BT[(int64_t)s_col[j + 1] * ldbt + kk];
-                T p0 = s_val[j] * b0;
-                T p1 = s_val[j + 1] * b1;
-                acc = acc + p0;
-                acc = acc + p1;
+            for (int j = s; j < e; j += 8) {
+                T av[8], bv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int jj = j + u < e ? j + u : e - 1;
+                    av[u] = s_val[jj];
+                    bv[u] = BT[(int64_t)s_col[jj] * ldbt + kk];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (j + u < e) acc = acc + av[u] * bv[u];
             }
-            if (j < e) acc = acc + s_val[j] * BT[(int64_t)s_col[j] * ldbt + kk];
             store_out(C, r + kk * ldc, acc, alpha, beta, mode);
         }
     }
-
-    if (rs_end < r1) {  // spill row: groups take strided entries, LDS combine
-        const int64_t r = r1 - 1;
+    __syncthreads();
+    // longer staged rows: every group takes a contiguous share, LDS combine
+    const int nl = nlong < SPMM_MAX_LONG ? nlong : SPMM_MAX_LONG;
+    for (int q = 0; q < nl; ++q) {
+        const int64_t r = long_rows[q];
+        const int s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
+        const int e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
+        const int per = (e - s + GROUPS - 1) / GROUPS;
+        const int js = s + g * per, je = (js + per < e) ? js + per : e;
         for (int64_t k0 = 0; k0 < k; k0 += LPR) {
             const int64_t kk = k0 + lane;
             T acc = T(0);
             if (kk < k) {
-                for (int64_t j = Ps + g; j < Pend; j += GROUPS) {
-                    int64_t c = clamp64((int64_t)colval[j] - 1, 0, n - 1);
-                    acc = acc + nzval[j] * BT[c * ldbt + kk];
+                for (int j = js; j < je; j += 8) {
+                    T av[8], bv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int jj = j + u < je ? j + u : je - 1;
+                        av[u] = s_val[jj];
+                        bv[u] = BT[(int64_t)s_col[jj] * ldbt + kk];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (j + u < je) acc = acc + av[u] * bv[u];
+                }
+            }
+            red[tid] = acc;
+            __syncthreads();
+            if (g == 0 && kk < k) {
+                T sum = red[lane];
+                for (int qq = 1; qq < GROUPS; ++qq) sum = sum + red[qq * LPR + lane];
+                store_out(C, r + kk * ldc, sum, alpha, beta, mode);
+            }
+            __syncthreads();
+        }
+    }
+
+    if (rs_end < r1) {  // spill row: staged in LDS chunk by chunk, groups split each chunk
+        const int64_t r = r1 - 1;
+        for (int64_t k0 = 0; k0 < k; k0 += LPR) {
+            const int64_t kk = k0 + lane;
+            T acc = T(0);
+            for (int64_t c0 = Ps; c0 < Pend; c0 += SPMV_CAP) {
+                const int cn = (int)(Pend - c0 < SPMV_CAP ? Pend - c0 : SPMV_CAP);
+                __syncthreads();  // previous users of s_col/s_val are done
+                for (int i = tid; i < cn; i += WG) {
+                    s_col[i] = (CI)clamp64((int64_t)ldnt(colval + c0 + i) - 1, 0, n - 1);
+                    s_val[i] = ldnt(nzval + c0 + i);
+                }
+                __syncthreads();
+                if (kk < k) {
+                    const int per = (cn + GROUPS - 1) / GROUPS;
+                    const int js = g * per, je = (js + per < cn) ? js + per : cn;
+                    for (int j = js; j < je; j += 8) {
+                        T av[8], bv[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int jj = j + u < je ? j + u : je - 1;
+                            av[u] = s_val[jj];
+                            bv[u] = BT[(int64_t)s_col[jj] * ldbt + kk];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (j + u < je) acc = acc + av[u] * bv[u];
+                    }
                 }
             }
             __syncthreads();
             red[tid] = acc;
             __syncthreads();
             if (g == 0 && kk < k) {
-                T s = red[lane];
-                for (int q = 1; q < GROUPS; ++q) s = s + red[q * LPR + lane];
-                store_out(C, r + kk * ldc, s, alpha, beta, mode);
+                T sum = red[lane];
+                for (int q = 1; q < GROUPS; ++q) sum = sum + red[q * LPR + lane];
+                store_out(C, r + kk * ldc, sum, alpha, beta, mode);
             }
         }
     }
@@ -384,20 +567,35 @@ static int apply_beta(hipStream_t s, int64_t rows, int64_t k, T* C, int64_t ldc,
 // a temporary one in scratch.
 struct PlanUse {
     const int64_t* blk_row = nullptr;
+    const int64_t* blk_pos = nullptr;
     int64_t nblocks = 0;
+    const int64_t* win_row = nullptr;
+    const int64_t* win_pos = nullptr;
+    int64_t nwin = 0;
     Scratch tmp;
     explicit PlanUse(hipStream_t s) : tmp(s) {}
-    int get(hipStream_t s, gsa_plan plan, int itype, int64_t m, int64_t nnz, const void* ptr) {
+    // wave = true: the wavefront windows (SpMV); else the workgroup windows.
+    int get(hipStream_t s, gsa_plan plan, int itype, int64_t m, int64_t nnz, const void* ptr,
+            bool wave) {
         if (plan && plan->kind == 0 && plan->csr.m == m && plan->csr.nnz == nnz &&
             plan->csr.itype == itype && plan->csr.ptr == ptr) {
             blk_row = plan->csr.blk_row;
+            blk_pos = plan->csr.blk_pos;
             nblocks = plan->csr.nblocks;
+            win_row = plan->csr.win_row;
+            win_pos = plan->csr.win_pos;
+            nwin = plan->csr.nwin;
             return GSA_OK;
         }
-        nblocks = nnz / SPMV_NB + 1;
-        GSA_TRY(tmp.alloc(sizeof(int64_t) * (nblocks + 1)));
-        GSA_TRY(build_csr_plan(s, itype, m, nnz, ptr, tmp.as<int64_t>(), nblocks));
-        blk_row = tmp.as<int64_t>();
+        const int nb = wave ? SPMV_WNB : SPMV_NB;
+        const int cap = wave ? SPMV_WCAP : SPMV_CAP;
+        const int64_t cnt = nnz / nb + 1;
+        GSA_TRY(tmp.alloc(sizeof(int64_t) * ((cnt + 1) + PLAN_POS * cnt)));
+        int64_t* br = tmp.as<int64_t>();
+        int64_t* bp = br + (cnt + 1);
+        GSA_TRY(build_csr_plan(s, itype, m, nnz, ptr, br, cnt, bp, nb, cap));
+        if (wave) { win_row = br; win_pos = bp; nwin = cnt; }
+        else { blk_row = br; blk_pos = bp; nblocks = cnt; }
         return GSA_OK;
     }
 };
@@ -413,7 +611,7 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         return apply_beta<T>(s, m, k, C, ldc, beta);
     }
     PlanUse pu(s);
-    GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr));
+    GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr, k == 1));
     const int mode = beta_mode(beta);
     const unsigned grid = (unsigned)pu.nblocks;
     if (k == 1) {
@@ -425,8 +623,9 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
                                      hipMemcpyDeviceToDevice, s));
             x = xs.as<T>();
         }
-        hipLaunchKernelGGL((k_csr_spmv<T, I>), dim3(grid), dim3(WG), 0, s, m, n, nnz, rowptr,
-                           colval, nzval, x, C, (T)alpha, (T)beta, mode, pu.blk_row);
+        hipLaunchKernelGGL((k_csr_spmv<T, I>), dim3((unsigned)cdiv(pu.nwin, WAVES)), dim3(64 * WAVES),
+                           0, s, m, n, nnz, rowptr, colval, nzval, x, C, (T)alpha, (T)beta, mode,
+                           pu.win_row, pu.win_pos, pu.nwin);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
     }
@@ -442,8 +641,16 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         ldbt = k;
     }
 #define GSA_SPMM_LAUNCH(L)                                                                  \
-    hipLaunchKernelGGL((k_csr_spmm<T, I, L>), dim3(grid), dim3(WG), 0, s, m, n, k, nnz, rowptr, \
-                       colval, nzval, BT, ldbt, C, ldc, (T)alpha, (T)beta, mode, pu.blk_row)
+    do {                                                                                    \
+        if (n <= INT32_MAX)                                                                 \
+            hipLaunchKernelGGL((k_csr_spmm<T, I, int32_t, L>), dim3(grid), dim3(WG), 0, s, m, n, k, \
+                               nnz, rowptr, colval, nzval, BT, ldbt, C, ldc, (T)alpha, (T)beta, \
+                               mode, pu.blk_row, pu.blk_pos);                               \
+        else                                                                                \
+            hipLaunchKernelGGL((k_csr_spmm<T, I, I, L>), dim3(grid), dim3(WG), 0, s, m, n, k, nnz, \
+                               rowptr, colval, nzval, BT, ldbt, C, ldc, (T)alpha, (T)beta, mode, \
+                               pu.blk_row, pu.blk_pos);                                     \
+    } while (0)
     if (k <= 2) GSA_SPMM_LAUNCH(2);
     else if (k <= 4) GSA_SPMM_LAUNCH(4);
     else if (k <= 8) GSA_SPMM_LAUNCH(8);
@@ -465,7 +672,7 @@ static int scatter_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* 
     GSA_TRY(apply_beta<T>(s, n, k, C, ldc, beta));
     if (m == 0 || nnz == 0) return GSA_OK;
     PlanUse pu(s);
-    GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr));
+    GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr, false));
     hipLaunchKernelGGL((k_csr_scatter<T, I>), dim3((unsigned)pu.nblocks), dim3(WG), 0, s, m, n, k,
                        nnz, rowptr, colval, nzval, B, ldb, opB, C, ldc, (T)alpha, pu.blk_row);
     GSA_HIP(hipGetLastError());
@@ -513,9 +720,18 @@ int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz, const void*
     p->csr.nnz = nnz;
     p->csr.ptr = ptr;
     p->csr.nblocks = nnz / SPMV_NB + 1;
-    hipError_t e = hipMalloc(&p->csr.blk_row, sizeof(int64_t) * (p->csr.nblocks + 1));
+    p->csr.nwin = nnz / SPMV_WNB + 1;
+    const int64_t words = (p->csr.nblocks + 1) + PLAN_POS * p->csr.nblocks +
+                          (p->csr.nwin + 1) + PLAN_POS * p->csr.nwin;
+    hipError_t e = hipMalloc(&p->csr.blk_row, sizeof(int64_t) * words);
     if (e != hipSuccess) { delete p; return hip_fail(e, "hipMalloc(plan)"); }
-    int r = build_csr_plan(s, itype, m, nnz, ptr, p->csr.blk_row, p->csr.nblocks);
+    p->csr.blk_pos = p->csr.blk_row + (p->csr.nblocks + 1);
+    p->csr.win_row = p->csr.blk_pos + PLAN_POS * p->csr.nblocks;
+    p->csr.win_pos = p->csr.win_row + (p->csr.nwin + 1);
+    int r = build_csr_plan(s, itype, m, nnz, ptr, p->csr.blk_row, p->csr.nblocks, p->csr.blk_pos);
+    if (r == GSA_OK)
+        r = build_csr_plan(s, itype, m, nnz, ptr, p->csr.win_row, p->csr.nwin, p->csr.win_pos,
+                           SPMV_WNB, SPMV_WCAP);
     if (r != GSA_OK) { (void)hipFree(p->csr.blk_row); delete p; return r; }
     *plan = p;
     return GSA_OK;

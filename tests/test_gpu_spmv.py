@@ -1,7 +1,7 @@
 """Parity of the HIP mul! (SpMV / SpMM, CSR / CSC / COO) with the oracle.
 
 Tolerances (Float64): |y - y_ref| <= 1e-12 * Σ_j |a_ij||x_j| per entry; rows the
-kernel sums on one lane (<= 48 stored entries in the staged window) and the
+kernel sums on one lane (<= 32 stored entries in the staged window) and the
 reference's order must match bit for bit.  Float32: 2e-6 normwise (row sums
 here stay under 1000 entries, so any summation order stays inside
 n*eps32*Σ|a||x| ~ 1e-4 and in practice ~1e-6).
@@ -68,7 +68,7 @@ def test_powerlaw_parity(dev, dtype, itype):
     bound = norm_bound(H.rowptr, H.colval, H.nzval, x)
     assert_normwise(y, ref, bound, 1e-12 if dtype == torch.float64 else 2e-6)
     lens = np.diff(H.rowptr)
-    short = lens <= 48
+    short = lens <= 32
     assert np.array_equal(y[short], ref[short])
 
 

@@ -1,0 +1,315 @@
+"""Parity of the HIP conversions, +, dropzeros, kron and SpGEMM with the oracle.
+
+Index arrays and patterns must match bit for bit; values too (every value is
+either copied, one IEEE add/multiply, or — SpGEMM — summed in the reference's
+order).  Inputs: the golden fixtures, seeded random matrices and, at full
+size, the benchmark-scale matrices with size-independent checks.
+"""
+import os
+
+import numpy as np
+import pytest
+import torch
+
+import gsa_hip as G
+import oracle as O
+from gsa_hip import synth
+from conftest import GOLDEN
+from util import dev_csr, dev_csc_of, host
+
+pytestmark = pytest.mark.gpu
+
+
+def g(name):
+    return np.load(os.path.join(GOLDEN, name))
+
+
+def csr_tuple(A):
+    return host(A.rowptr), host(A.colval), host(A.nzval)
+
+
+def csc_tuple(A):
+    return host(A.colptr), host(A.rowval), host(A.nzval)
+
+
+def coo_tuple(A):
+    return host(A.rowind), host(A.colind), host(A.nzval)
+
+
+def same(got, exp):
+    for a, b in zip(got, exp):
+        assert a.shape == b.shape and np.array_equal(a, b), (a[:10], b[:10])
+
+
+def from_arrays(cls, m, n, a, b, c, dev, itype=torch.int64):
+    return cls(m, n, torch.as_tensor(a).to(dev, itype), torch.as_tensor(b).to(dev, itype),
+               torch.as_tensor(c).to(dev), copy=False)
+
+
+# ------------------------------------------------------------ conversions ----
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+def test_conversions_golden(dev, itype):
+    d = g("conv_random.npz")
+    m, n = int(d["m"]), int(d["n"])
+    csr = (d["csr_0"], d["csr_1"], d["csr_2"])
+    csc = (d["csc_0"], d["csc_1"], d["csc_2"])
+    rowmaj = (d["coo_rowmajor_0"], d["coo_rowmajor_1"], d["coo_rowmajor_2"])
+    colmaj = (d["coo_colmajor_0"], d["coo_colmajor_1"], d["coo_colmajor_2"])
+    A = from_arrays(G.GenericSparseMatrixCSR, m, n, *csr, dev, itype)
+    Ac = from_arrays(G.GenericSparseMatrixCSC, m, n, *csc, dev, itype)
+    same(coo_tuple(G.GenericSparseMatrixCOO(A)), rowmaj)
+    same(coo_tuple(G.GenericSparseMatrixCOO(Ac)), colmaj)
+    same(csc_tuple(G.GenericSparseMatrixCSC(A)), csc)
+    same(csr_tuple(G.GenericSparseMatrixCSR(Ac)), csr)
+    for coo in (rowmaj, colmaj):
+        Ao = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo, dev, itype)
+        same(csr_tuple(G.GenericSparseMatrixCSR(Ao)), csr)
+        same(csc_tuple(G.GenericSparseMatrixCSC(Ao)), csc)
+
+
+def test_conversion_edges_and_storage_pin(dev):
+    d = g("conv_edges.npz")
+    for name in ("single", "fullrow", "fullcol"):
+        csr = tuple(d[f"{name}_csr_{i}"] for i in range(3))
+        csc = tuple(d[f"{name}_csc_{i}"] for i in range(3))
+        Ac = from_arrays(G.GenericSparseMatrixCSC, 3, 4, *csc, dev)
+        same(csr_tuple(G.GenericSparseMatrixCSR(Ac)), csr)
+        assert np.array_equal(G.GenericSparseMatrixCSR(Ac).collect(), d[f"{name}_dense"])
+    p = g("csr_storage_2x2.npz")  # test/shared/matrix_csr.jl:49-59
+    Ao = from_arrays(G.GenericSparseMatrixCOO, 2, 2, p["coo_rows"], p["coo_cols"], p["coo_vals"], dev)
+    same(csr_tuple(G.GenericSparseMatrixCSR(Ao)), (p["rowptr"], p["colval"], p["nzval"]))
+
+
+def test_coo_to_csr_unsorted_duplicates_and_empty(dev):
+    rng = np.random.default_rng(20)
+    r = rng.integers(1, 301, size=5000)
+    c = rng.integers(1, 201, size=5000)
+    v = rng.standard_normal(5000)
+    Ao = from_arrays(G.GenericSparseMatrixCOO, 300, 200, r, c, v, dev)
+    same(csr_tuple(G.GenericSparseMatrixCSR(Ao)), O.coo_to_csr(300, 200, r, c, v))
+    same(csc_tuple(G.GenericSparseMatrixCSC(Ao)), O.coo_to_csc(300, 200, r, c, v))
+    E = from_arrays(G.GenericSparseMatrixCOO, 4, 3, np.zeros(0, np.int64), np.zeros(0, np.int64),
+                    np.zeros(0), dev)
+    Ecsr = G.GenericSparseMatrixCSR(E)
+    assert host(Ecsr.rowptr).tolist() == [1] * 5 and Ecsr.nnz() == 0
+
+
+def test_transposed_conversions(dev):
+    H = synth.sprand(300, 200, 0.03, seed=21, values="normal")
+    A = dev_csr(H)
+    Ac = dev_csc_of(H)
+    # Transpose(CSC) -> CSR is a reinterpretation (conversions.jl:106-123)
+    At = G.GenericSparseMatrixCSR(G.transpose(Ac))
+    assert At.shape == (200, 300)
+    np.testing.assert_array_equal(At.collect(), H.to_scipy().toarray().T)
+    # Transpose(CSR) -> CSC, Transpose(COO) -> CSR
+    np.testing.assert_array_equal(G.GenericSparseMatrixCSC(G.transpose(A)).collect(),
+                                  H.to_scipy().toarray().T)
+    Ao = G.GenericSparseMatrixCOO(A)
+    np.testing.assert_array_equal(G.GenericSparseMatrixCSR(G.transpose(Ao)).collect(),
+                                  H.to_scipy().toarray().T)
+
+
+def test_conversions_full_size_powerlaw(dev):
+    """1M x 1M power-law matrix: CSR <-> CSC <-> COO round trips, bit-exact."""
+    H = synth.powerlaw_csr(1_000_000, 1_000_000, alpha=1.846, seed=2)
+    A = dev_csr(H)
+    Ac = G.GenericSparseMatrixCSC(A)
+    Ao = G.GenericSparseMatrixCOO(A)
+    # round trips return the original arrays
+    same(csr_tuple(G.GenericSparseMatrixCSR(Ac)), (H.rowptr, H.colval, H.nzval))
+    same(csr_tuple(G.GenericSparseMatrixCSR(Ao)), (H.rowptr, H.colval, H.nzval))
+    Aoc = G.GenericSparseMatrixCOO(Ac)  # column-major COO
+    same(csr_tuple(G.GenericSparseMatrixCSR(Aoc)), (H.rowptr, H.colval, H.nzval))
+    # CSC is the oracle's (checked on a 200k-row sample of the pipeline's pieces)
+    cp, rv, v = csc_tuple(Ac)
+    assert cp[0] == 1 and cp[-1] == H.nnz + 1 and np.all(np.diff(cp) >= 0)
+    S = H.to_scipy().tocsc()
+    S.sort_indices()
+    assert np.array_equal(cp - 1, S.indptr) and np.array_equal(rv - 1, S.indices)
+    assert np.array_equal(v, S.data)
+    r, c, vv = coo_tuple(Ao)
+    assert np.array_equal(np.diff(H.rowptr), np.bincount(r - 1, minlength=H.m))
+    assert np.array_equal(c, H.colval) and np.array_equal(vv, H.nzval)
+
+
+# ------------------------------------------------------------ + / dropzeros ---
+def test_add_golden_csr_csc_coo(dev):
+    d = g("add_random.npz")
+    m, n = int(d["m"]), int(d["n"])
+    A = from_arrays(G.GenericSparseMatrixCSR, m, n, d["A_rowptr"], d["A_colval"], d["A_nzval"], dev)
+    B = from_arrays(G.GenericSparseMatrixCSR, m, n, d["B_rowptr"], d["B_colval"], d["B_nzval"], dev)
+    C = A + B
+    assert isinstance(C, G.GenericSparseMatrixCSR)
+    same(csr_tuple(C), (d["C_rowptr"], d["C_colval"], d["C_nzval"]))
+    Cc = G.GenericSparseMatrixCSC(A) + G.GenericSparseMatrixCSC(B)
+    np.testing.assert_array_equal(Cc.collect(), C.collect())
+    Ao = from_arrays(G.GenericSparseMatrixCOO, m, n, *(d[f"Acoo_{i}"] for i in range(3)), dev)
+    Bo = from_arrays(G.GenericSparseMatrixCOO, m, n, *(d[f"Bcoo_{i}"] for i in range(3)), dev)
+    same(coo_tuple(Ao + Bo), tuple(d[f"Ccoo_{i}"] for i in range(3)))
+
+
+def test_add_overlap_and_self_cancel(dev):
+    d = g("overlap_add.npz")
+    m, n = int(d["m"]), int(d["n"])
+    A = from_arrays(G.GenericSparseMatrixCSR, m, n, d["A_rowptr"], d["A_colval"], d["A_nzval"], dev)
+    B = from_arrays(G.GenericSparseMatrixCSR, m, n, d["B_rowptr"], d["B_colval"], d["B_nzval"], dev)
+    same(csr_tuple(A + B), (d["C_rowptr"], d["C_colval"], d["C_nzval"]))
+    for M in (A, G.GenericSparseMatrixCSC(A), G.GenericSparseMatrixCOO(A)):
+        Z = M - M  # nnz 0 (test/shared/matrix_csr.jl:379-381)
+        assert Z.nnz() == 0 and not Z.collect().any()
+
+
+def test_add_transposed_operands(dev):
+    H1 = synth.sprand(60, 60, 0.1, seed=22, values="normal")
+    H2 = synth.sprand(60, 60, 0.15, seed=23, values="normal")
+    A, B = dev_csr(H1), dev_csr(H2)
+    D1, D2 = H1.to_scipy().toarray(), H2.to_scipy().toarray()
+    for opA, dA in ((lambda X: X, D1), (G.transpose, D1.T)):
+        for opB, dB in ((lambda X: X, D2), (G.transpose, D2.T), (G.adjoint, D2.T)):
+            C = opA(A) + opB(B)
+            assert isinstance(C, G.GenericSparseMatrixCSR)
+            np.testing.assert_array_equal(C.collect(), dA + dB)
+
+
+def test_add_laplacian_perturbed_full_size(dev):
+    """L + L' (L' = L with 10% of entries dropped) on the 1000x1000 grid."""
+    L = synth.laplacian2d(1000)
+    Lp = synth.drop_entries(L, 0.1, seed=3)
+    A, B = dev_csr(L), dev_csr(Lp)
+    C = A + B
+    same(csr_tuple(C), O.csr_add(L.m, (L.rowptr, L.colval, L.nzval), (Lp.rowptr, Lp.colval, Lp.nzval)))
+    D = A - B  # exercises dropzeros: only the dropped entries remain
+    same(csr_tuple(D), O.csr_add(L.m, (L.rowptr, L.colval, L.nzval), (Lp.rowptr, Lp.colval, -Lp.nzval)))
+    assert D.nnz() == L.nnz - Lp.nnz
+
+
+def test_dropzeros_csr_csc_coo(dev):
+    rng = np.random.default_rng(24)
+    H = synth.csr_from_lengths(np.r_[rng.integers(0, 20, 3000), [3000, 0, 5]], 5000, rng, "normal")
+    H.nzval[rng.random(H.nnz) < 0.3] = 0.0
+    A = dev_csr(H)
+    exp = O.csr_dropzeros(H.m, H.rowptr, H.colval, H.nzval)
+    same(csr_tuple(G.dropzeros(A)), exp)
+    Ao = G.GenericSparseMatrixCOO(A)
+    same(coo_tuple(G.dropzeros(Ao)), O.coo_dropzeros(*coo_tuple(Ao)))
+    Ac = G.GenericSparseMatrixCSC(A)
+    np.testing.assert_array_equal(G.dropzeros(Ac).collect(), H.to_scipy().toarray())
+    # dropzeros! keeps the stored zeros of an all-zero matrix (matrix_csr.jl:698-703)
+    Z = dev_csr(H)
+    Z.nzval.zero_()
+    G.dropzeros_(Z)
+    assert Z.nnz() == H.nnz
+    assert G.dropzeros(Z).nnz() == 0
+
+
+# ------------------------------------------------------------ kron ---------
+def test_kron_golden(dev):
+    d = g("kron_random.npz")
+    mA, nA, mB, nB = (int(d[k]) for k in ("mA", "nA", "mB", "nB"))
+    A = from_arrays(G.GenericSparseMatrixCSR, mA, nA, d["A_rowptr"], d["A_colval"], d["A_nzval"], dev)
+    B = from_arrays(G.GenericSparseMatrixCSR, mB, nB, d["B_rowptr"], d["B_colval"], d["B_nzval"], dev)
+    C = G.kron(A, B)
+    assert isinstance(C, G.GenericSparseMatrixCSR) and C.shape == (mA * mB, nA * nB)
+    same(csr_tuple(C), (d["C_rowptr"], d["C_colval"], d["C_nzval"]))
+    K = G.kron(G.GenericSparseMatrixCOO(A), G.GenericSparseMatrixCOO(B))
+    same(coo_tuple(K), (d["Kcoo_0"], d["Kcoo_1"], d["Kcoo_2"]))
+    Kc = G.kron(G.GenericSparseMatrixCSC(A), G.GenericSparseMatrixCSC(B))
+    np.testing.assert_array_equal(Kc.collect(), C.collect())
+
+
+def test_kron_transposed_and_noncanonical(dev):
+    H1 = synth.sprand(20, 15, 0.2, seed=25, values="normal")
+    H2 = synth.sprand(12, 10, 0.2, seed=26, values="normal")
+    A, B = dev_csr(H1), dev_csr(H2)
+    import scipy.sparse as sp
+    for opA, sA in ((lambda X: X, H1.to_scipy()), (G.transpose, H1.to_scipy().T)):
+        for opB, sB in ((lambda X: X, H2.to_scipy()), (G.adjoint, H2.to_scipy().T)):
+            C = G.kron(opA(A), opB(B))
+            np.testing.assert_array_equal(C.collect(), sp.kron(sA, sB).toarray())
+    # a CSR operand with descending columns in a row takes the sort route; the
+    # result is the reference's canonical order
+    rp = np.array([1, 3, 4]); cv = np.array([3, 1, 2]); v = np.array([1.0, 2.0, 3.0])
+    N = from_arrays(G.GenericSparseMatrixCSR, 2, 3, rp, cv, v, dev)
+    C = G.kron(N, B)
+    same(csr_tuple(C), O.kron_csr(2, 3, H2.m, H2.n, (rp, cv, v), (H2.rowptr, H2.colval, H2.nzval)))
+
+
+def test_kron_full_size(dev):
+    """Two 1000x1000 matrices at density 0.01: C has ~1e8 entries (Int64)."""
+    H1 = synth.sprand(1000, 1000, 0.01, seed=31)
+    H2 = synth.sprand(1000, 1000, 0.01, seed=32)
+    A, B = dev_csr(H1), dev_csr(H2)
+    C = G.kron(A, B)
+    assert C.nnz() == H1.nnz * H2.nnz
+    rp = host(C.rowptr)
+    # row (iA, iB) length is lenA(iA) * lenB(iB); rows strictly ascending
+    lens = np.outer(np.diff(H1.rowptr), np.diff(H2.rowptr)).ravel()
+    assert rp[0] == 1 and np.array_equal(np.diff(rp), lens)
+    cv = C.colval
+    same_row = torch.ones(cv.numel() - 1, dtype=torch.bool, device=dev)
+    starts = torch.as_tensor(rp[1:-1] - 1, device=dev)
+    starts = starts[(starts > 0) & (starts < cv.numel())]
+    same_row[starts - 1] = False
+    assert bool(((cv[1:] > cv[:-1]) | ~same_row).all())
+    # spot-check rows against the closed form
+    for i in (0, 12345, 500000, 999999):
+        iA, iB = divmod(i, 1000)
+        a = slice(H1.rowptr[iA] - 1, H1.rowptr[iA + 1] - 1)
+        b = slice(H2.rowptr[iB] - 1, H2.rowptr[iB + 1] - 1)
+        ec = ((H1.colval[a] - 1)[:, None] * 1000 + H2.colval[b][None, :]).ravel()
+        ev = (H1.nzval[a][:, None] * H2.nzval[b][None, :]).ravel()
+        got_c = host(C.colval[rp[i] - 1:rp[i + 1] - 1])
+        got_v = host(C.nzval[rp[i] - 1:rp[i + 1] - 1])
+        assert np.array_equal(got_c, ec) and np.array_equal(got_v, ev)
+
+
+# ------------------------------------------------------------ SpGEMM --------
+def test_spgemm_golden_and_formats(dev):
+    d = g("spgemm_random.npz")
+    m, k, n = int(d["m"]), int(d["k"]), int(d["n"])
+    A = from_arrays(G.GenericSparseMatrixCSR, m, k, d["A_rowptr"], d["A_colval"], d["A_nzval"], dev)
+    B = from_arrays(G.GenericSparseMatrixCSR, k, n, d["B_rowptr"], d["B_colval"], d["B_nzval"], dev)
+    C = A * B
+    assert isinstance(C, G.GenericSparseMatrixCSR)
+    same(csr_tuple(C), (d["C_rowptr"], d["C_colval"], d["C_nzval"]))
+    Cc = G.GenericSparseMatrixCSC(A) * G.GenericSparseMatrixCSC(B)
+    np.testing.assert_array_equal(Cc.collect(), C.collect())
+    Co = G.GenericSparseMatrixCOO(A) * G.GenericSparseMatrixCOO(B)
+    np.testing.assert_array_equal(Co.collect(), C.collect())
+    for opA in (G.transpose, G.adjoint):
+        At = G.GenericSparseMatrixCSR(G.transpose(G.GenericSparseMatrixCSC(A)))  # Aᵀ stored
+        np.testing.assert_array_equal((opA(At) * B).collect(), C.collect())
+
+
+def test_spgemm_cancellation_zero_kept(dev):
+    A = from_arrays(G.GenericSparseMatrixCSR, 1, 2, np.array([1, 3]), np.array([1, 2]),
+                    np.array([1.0, 1.0]), dev)
+    B = from_arrays(G.GenericSparseMatrixCSR, 2, 1, np.array([1, 2, 3]), np.array([1, 1]),
+                    np.array([1.0, -1.0]), dev)
+    C = A * B
+    assert C.nnz() == 1 and host(C.nzval)[0] == 0.0
+
+
+@pytest.mark.parametrize("which", ["LL_small", "RR_small", "powerlaw"])
+def test_spgemm_random_vs_oracle(dev, which):
+    if which == "LL_small":
+        H1 = H2 = synth.laplacian2d(60)
+    elif which == "RR_small":
+        H1 = H2 = synth.fixed_row_csr(5000, 5000, 8, seed=27)
+    else:
+        H1 = synth.powerlaw_csr(3000, 3000, alpha=2.0, seed=28, hi=300)
+        H2 = synth.powerlaw_csr(3000, 3000, alpha=2.0, seed=29, hi=300)
+    C = dev_csr(H1) * dev_csr(H2)
+    same(csr_tuple(C), O.spgemm_csr(H1.m, H2.n, (H1.rowptr, H1.colval, H1.nzval),
+                                    (H2.rowptr, H2.colval, H2.nzval)))
+
+
+def test_spgemm_laplacian_full_size(dev):
+    """L*L on the 1000x1000 grid: 12,980,004 entries, bit-exact with the oracle."""
+    L = synth.laplacian2d(1000)
+    A = dev_csr(L)
+    C = A * A
+    assert C.nnz() == 12_980_004
+    same(csr_tuple(C), O.spgemm_csr(L.m, L.n, (L.rowptr, L.colval, L.nzval),
+                                    (L.rowptr, L.colval, L.nzval)))
