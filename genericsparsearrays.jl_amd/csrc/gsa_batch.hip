@@ -1,33 +1,439 @@
-// gsa_batch.hip — batched uneven CSR SpMM (placeholder until the grouped kernel lands).
+// gsa_batch.hip — many uneven CSR SpMM instances in one persistent launch,
+// the BF16-with-row-scale value format, and HIP-graph capture of a batch.
+//
+// Work unit: a tile = (instance, [r0, r1)) from a device table built once per
+// batch.  A persistent grid of workgroups takes tiles from an atomic counter
+// (reset by a one-thread kernel launched just before, so the pair is
+// graph-capturable: no allocation, no memset).  Inside a tile each row gets
+// LPR = next_pow2(k) lanes (one (row, kk) output per lane, summed left to
+// right); rows over BATCH_LONG entries are reduced by the whole workgroup.
+//
+// Precision: Float32 and BF16 storage accumulate in Float32; rows over 256
+// entries accumulate with TwoSum (an error-free sum and a running
+// compensation, combined pairwise in the reduction).  Float64 accumulates in
+// Float64.  BF16 values carry one Float32 scale per row (a power of two, so
+// scaling is exact): a_ij = scale_i * bf16_ij.
 #include "gsa_common.h"
+#include <vector>
 
 namespace gsa {
-void batch_plan_release(gsa_plan_s* plan) { (void)plan; }
+
+constexpr int BWG = 256;
+constexpr int BATCH_LONG = 256;   // rows above this: workgroup-cooperative, compensated
+constexpr int BATCH_MAX_TILE_ROWS = 1024;
+
+struct BatchPlan {
+    int vtype = 0, itype = 0;
+    int64_t count = 0;
+    gsa_batch_desc* d_desc = nullptr;
+    int32_t* d_tiles = nullptr;   // 3 per tile: instance, r0, r1
+    int64_t ntiles = 0;
+    int* d_counter = nullptr;
+    int grid = 0;
+};
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t v) {
+    return __uint_as_float(((uint32_t)v) << 16);
+}
+
+// round-to-nearest-even Float32 -> BF16 (NaN kept quiet)
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu)) return (uint16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// Value access by storage type: V = double / float / uint16_t (BF16).
+template <typename V> struct Acc;
+template <> struct Acc<double> { using T = double; };
+template <> struct Acc<float> { using T = float; };
+template <> struct Acc<uint16_t> { using T = float; };
+
+template <typename V>
+__device__ __forceinline__ typename Acc<V>::T vload(const V* p) {
+    if constexpr (sizeof(V) == 2) return bf16_to_f32(*p);
+    else return *p;
+}
+
+// Knuth's TwoSum: s + e == a + b exactly (no FMA contraction in this file).
+template <typename T>
+__device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
+    s = a + b;
+    T bp = s - a;
+    T ap = s - bp;
+    e = (a - ap) + (b - bp);
+}
+
+__global__ void k_reset_counter(int* c) { *c = 0; }
+
+// Shared-memory workspace of one workgroup processing tiles.
+template <typename T>
+struct TileLds {
+    int nlong;
+    int long_rows[BATCH_MAX_TILE_ROWS];
+    T red[BWG];
+    T cmp[BWG];
+};
+
+// One tile: rows [r0, r1) of instance d.  All BWG lanes call it together.
+template <typename V, typename I>
+__device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0, int64_t r1,
+                                             typename Acc<V>::T alpha, typename Acc<V>::T beta,
+                                             int mode, TileLds<typename Acc<V>::T>& sh) {
+    using T = typename Acc<V>::T;
+    const int tid = threadIdx.x;
+    const I* __restrict__ rowptr = (const I*)d.rowptr;
+    const I* __restrict__ colval = (const I*)d.colval;
+    const V* __restrict__ nzval = (const V*)d.nzval;
+    const float* __restrict__ scale = (const float*)d.row_scale;
+    const T* __restrict__ B = (const T*)d.B;
+    T* __restrict__ C = (T*)d.C;
+    const int k = d.k;
+    const int64_t n = d.n > 0 ? d.n : 1;  // clamp bound for column indices
+    const int64_t ldb = d.ldb, ldc = d.ldc;
+    if (tid == 0) sh.nlong = 0;
+    __syncthreads();
+    int lpr = 1;
+    while (lpr < k && lpr < 64) lpr <<= 1;
+    const int groups = BWG / lpr;
+    const int g = tid / lpr, lane = tid % lpr;
+    for (int64_t r = r0 + g; r < r1; r += groups) {
+        const int64_t s = (int64_t)rowptr[r] - 1, e = (int64_t)rowptr[r + 1] - 1;
+        if (e - s > BATCH_LONG) {
+            if (lane == 0) {
+                int q = atomicAdd(&sh.nlong, 1);
+                if (q < BATCH_MAX_TILE_ROWS) sh.long_rows[q] = (int)(r - r0);
+            }
+            continue;
+        }
+        for (int kk = lane; kk < k; kk += lpr) {
+            T acc = T(0);
+            int64_t j = s;
+            for (; j + 3 < e; j += 4) {
+                int64_t c0 = colval[j] - 1, c1 = colval[j + 1] - 1;
+                int64_t c2 = colval[j + 2] - 1, c3 = colval[j + 3] - 1;
+                c0 = c0 < 0 ? 0 : (c0 >= n ? n - 1 : c0);
+                c1 = c1 < 0 ? 0 : (c1 >= n ? n - 1 : c1);
+                c2 = c2 < 0 ? 0 : (c2 >= n ? n - 1 : c2);
+                c3 = c3 < 0 ? 0 : (c3 >= n ? n - 1 : c3);
+                T p0 = vload(nzval + j) * B[c0 + kk * ldb];
+                T p1 = vload(nzval + j + 1) * B[c1 + kk * ldb];
+                T p2 = vload(nzval + j + 2) * B[c2 + kk * ldb];
+                T p3 = vload(nzval + j + 3) * B[c3 + kk * ldb];
+                acc = acc + p0; acc = acc + p1; acc = acc + p2; acc = acc + p3;
+            }
+            for (; j < e; ++j) {
+                int64_t c = colval[j] - 1;
+                c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+                acc = acc + vload(nzval + j) * B[c + kk * ldb];
+            }
+            if (scale) acc = acc * (T)scale[r];
+            T* out = C + r + kk * ldc;
+            T tt = acc * alpha;
+            T base = mode == 0 ? T(0) : (mode == 1 ? *out : *out * beta);
+            *out = base + tt;
+        }
+    }
+    __syncthreads();
+    // long rows: the whole workgroup, TwoSum-compensated
+    const int nl = sh.nlong < BATCH_MAX_TILE_ROWS ? sh.nlong : BATCH_MAX_TILE_ROWS;
+    for (int q = 0; q < nl; ++q) {
+        const int64_t r = r0 + sh.long_rows[q];
+        const int64_t s = (int64_t)rowptr[r] - 1, e = (int64_t)rowptr[r + 1] - 1;
+        for (int kk = 0; kk < k; ++kk) {
+            T sum = T(0), cmp = T(0);
+            for (int64_t j = s + tid; j < e; j += BWG) {
+                int64_t c = colval[j] - 1;
+                c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+                T p = vload(nzval + j) * B[c + kk * ldb];
+                T ns, er;
+                two_sum(sum, p, ns, er);
+                sum = ns;
+                cmp = cmp + er;
+            }
+            sh.red[tid] = sum;
+            sh.cmp[tid] = cmp;
+            __syncthreads();
+            for (int h = BWG / 2; h > 0; h >>= 1) {
+                if (tid < h) {
+                    T ns, er;
+                    two_sum(sh.red[tid], sh.red[tid + h], ns, er);
+                    sh.red[tid] = ns;
+                    sh.cmp[tid] = sh.cmp[tid] + sh.cmp[tid + h] + er;
+                }
+                __syncthreads();
+            }
+            if (tid == 0) {
+                T acc = sh.red[0] + sh.cmp[0];
+                if (scale) acc = acc * (T)scale[r];
+                T* out = C + r + kk * ldc;
+                T tt = acc * alpha;
+                T base = mode == 0 ? T(0) : (mode == 1 ? *out : *out * beta);
+                *out = base + tt;
+            }
+            __syncthreads();
+        }
+    }
+}
+
+// Persistent: tiles from the table, taken through the atomic counter.
+template <typename V, typename I>
+__global__ __launch_bounds__(BWG) void k_batch_spmm(const gsa_batch_desc* __restrict__ desc,
+                                                    const int32_t* __restrict__ tiles,
+                                                    int64_t ntiles, int* __restrict__ counter,
+                                                    typename Acc<V>::T alpha,
+                                                    typename Acc<V>::T beta, int mode) {
+    using T = typename Acc<V>::T;
+    __shared__ int s_tile;
+    __shared__ TileLds<T> sh;
+    for (;;) {
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        __syncthreads();
+        const int64_t t = s_tile;
+        __syncthreads();  // every lane has read s_tile before lane 0 takes the next
+        if (t >= ntiles) break;
+        const gsa_batch_desc d = desc[tiles[3 * t]];
+        process_tile<V, I>(d, tiles[3 * t + 1], tiles[3 * t + 2], alpha, beta, mode, sh);
+    }
+}
+
+// One instance, descriptor by value, tile = blockIdx.x * rows.
+template <typename V, typename I>
+__global__ __launch_bounds__(BWG) void k_single_spmm(gsa_batch_desc d, int rows,
+                                                     typename Acc<V>::T alpha,
+                                                     typename Acc<V>::T beta, int mode) {
+    using T = typename Acc<V>::T;
+    __shared__ TileLds<T> sh;
+    const int64_t r0 = (int64_t)blockIdx.x * rows;
+    const int64_t r1 = r0 + rows < d.m ? r0 + rows : d.m;
+    process_tile<V, I>(d, r0, r1, alpha, beta, mode, sh);
+}
+
+template <typename I>
+__global__ void k_row_bf16(int64_t m, const I* __restrict__ rowptr, const void* __restrict__ nz,
+                           int is_f64, uint16_t* __restrict__ out, float* __restrict__ scale) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
+         r += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t s = (int64_t)rowptr[r] - 1, e = (int64_t)rowptr[r + 1] - 1;
+        double mx = 0;
+        for (int64_t j = s; j < e; ++j) {
+            double v = is_f64 ? ((const double*)nz)[j] : (double)((const float*)nz)[j];
+            v = v < 0 ? -v : v;
+            mx = v > mx ? v : mx;
+        }
+        // power-of-two scale >= max |a|: the division below is exact
+        int ex = 0;
+        if (mx > 0) (void)frexp(mx, &ex);
+        const double sc = mx > 0 ? ldexp(1.0, ex) : 1.0;
+        scale[r] = (float)sc;
+        for (int64_t j = s; j < e; ++j) {
+            double v = is_f64 ? ((const double*)nz)[j] : (double)((const float*)nz)[j];
+            out[j] = f32_to_bf16((float)(v / sc));
+        }
+    }
+}
+
+static int launch_batch(hipStream_t s, int vtype, int itype, const gsa_batch_desc* d_desc,
+                        const int32_t* d_tiles, int64_t ntiles, int* d_counter, double alpha,
+                        double beta, int grid) {
+    const int mode = beta == 0.0 ? 0 : (beta == 1.0 ? 1 : 2);
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
+    if (ntiles == 0) return GSA_OK;
+    if (grid <= 0) grid = 256 * 8;
+    auto go = [&](auto vz, auto iz) {
+        using V = decltype(vz);
+        using I = decltype(iz);
+        using T = typename Acc<V>::T;
+        hipLaunchKernelGGL((k_batch_spmm<V, I>), dim3(grid), dim3(BWG), 0, s, d_desc, d_tiles, ntiles,
+                           d_counter, (T)alpha, (T)beta, mode);
+        return (int)GSA_OK;
+    };
+    int r;
+    if (vtype == GSA_BF16) r = dispatch_i(itype, [&](auto iz) { return go(uint16_t{}, iz); });
+    else r = dispatch_vi(vtype, itype, [&](auto vz, auto iz) { return go(vz, iz); });
+    if (r != GSA_OK) return r;
+    GSA_HIP(hipGetLastError());
+    return GSA_OK;
+}
+
+// Tiles for `count` instances: tile rows = max(16, rows_per_tile / next_pow2(k)).
+static void make_tiles(const gsa_batch_desc* h, int64_t count, int rows_per_tile,
+                       std::vector<int32_t>& tiles) {
+    for (int64_t i = 0; i < count; ++i) {
+        int lpr = 1;
+        while (lpr < h[i].k && lpr < 64) lpr <<= 1;
+        int64_t rows = rows_per_tile / lpr;
+        if (rows < 16) rows = 16;
+        if (rows > BATCH_MAX_TILE_ROWS) rows = BATCH_MAX_TILE_ROWS;
+        for (int64_t r = 0; r < h[i].m; r += rows) {
+            tiles.push_back((int32_t)i);
+            tiles.push_back((int32_t)r);
+            tiles.push_back((int32_t)(r + rows < h[i].m ? r + rows : h[i].m));
+        }
+    }
+}
+
+void batch_plan_release(gsa_plan_s* plan) {
+    auto* bp = (BatchPlan*)plan->batch;
+    if (!bp) return;
+    (void)hipDeviceSynchronize();
+    (void)hipFree(bp->d_desc);
+    (void)hipFree(bp->d_tiles);
+    (void)hipFree(bp->d_counter);
+    delete bp;
+    plan->batch = nullptr;
+}
+
 }  // namespace gsa
 
 using namespace gsa;
 
 extern "C" {
+
 int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
                           const gsa_batch_desc* host_desc, int rows_per_tile, gsa_plan* plan) {
-    (void)stream; (void)vtype; (void)itype; (void)count; (void)host_desc; (void)rows_per_tile;
-    (void)plan;
-    set_error(GSA_ERR_UNSUPPORTED, "gsa_batch_plan_create: not built yet");
-    return GSA_ERR_UNSUPPORTED;
+    GSA_TRY(ensure_init());
+    GSA_CHECK(plan && count >= 0 && (host_desc || count == 0), GSA_ERR_INVALID,
+              "gsa_batch_plan_create: bad arguments");
+    GSA_CHECK(vtype == GSA_F64 || vtype == GSA_F32 || vtype == GSA_BF16, GSA_ERR_UNSUPPORTED,
+              "bad vtype %d", vtype);
+    GSA_CHECK(itype == GSA_I32 || itype == GSA_I64, GSA_ERR_UNSUPPORTED, "bad itype %d", itype);
+    if (rows_per_tile <= 0) rows_per_tile = 512;
+    for (int64_t i = 0; i < count; ++i) {
+        const gsa_batch_desc& d = host_desc[i];
+        GSA_CHECK(d.m >= 0 && d.n >= 0 && d.k >= 0, GSA_ERR_INVALID, "instance %lld: negative size",
+                  (long long)i);
+        GSA_CHECK(d.m == 0 || (d.rowptr && d.C), GSA_ERR_INVALID, "instance %lld: null pointer",
+                  (long long)i);
+        GSA_CHECK(d.k <= 1 || (d.ldb >= d.n && d.ldc >= d.m), GSA_ERR_INVALID,
+                  "instance %lld: leading dimension too small", (long long)i);
+        GSA_CHECK(vtype != GSA_BF16 || d.row_scale || d.m == 0, GSA_ERR_INVALID,
+                  "instance %lld: BF16 values need row_scale", (long long)i);
+    }
+    hipStream_t s = as_stream(stream);
+    std::vector<int32_t> tiles;
+    make_tiles(host_desc, count, rows_per_tile, tiles);
+    auto* bp = new BatchPlan();
+    bp->vtype = vtype; bp->itype = itype; bp->count = count;
+    bp->ntiles = (int64_t)tiles.size() / 3;
+    int dev = 0, cus = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    bp->grid = cus * 8;
+    hipError_t e = hipMalloc(&bp->d_desc, sizeof(gsa_batch_desc) * (count > 0 ? count : 1));
+    if (e == hipSuccess) e = hipMalloc(&bp->d_tiles, sizeof(int32_t) * (tiles.size() + 3));
+    if (e == hipSuccess) e = hipMalloc(&bp->d_counter, sizeof(int));
+    if (e == hipSuccess && count > 0)
+        e = hipMemcpyAsync(bp->d_desc, host_desc, sizeof(gsa_batch_desc) * count,
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && !tiles.empty())
+        e = hipMemcpyAsync(bp->d_tiles, tiles.data(), sizeof(int32_t) * tiles.size(),
+                           hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);  // host vectors go out of scope
+    if (e != hipSuccess) {
+        (void)hipFree(bp->d_desc); (void)hipFree(bp->d_tiles); (void)hipFree(bp->d_counter);
+        delete bp;
+        return hip_fail(e, "gsa_batch_plan_create");
+    }
+    auto* p = new gsa_plan_s();
+    p->kind = 1;
+    p->batch = bp;
+    *plan = p;
+    return GSA_OK;
 }
+
+int gsa_batch_spmm(void* stream, gsa_plan plan, float alpha, float beta) {
+    GSA_CHECK(plan && plan->kind == 1 && plan->batch, GSA_ERR_INVALID, "gsa_batch_spmm: not a batch plan");
+    auto* bp = (BatchPlan*)plan->batch;
+    return launch_batch(as_stream(stream), bp->vtype, bp->itype, bp->d_desc, bp->d_tiles, bp->ntiles,
+                        bp->d_counter, alpha, beta, bp->grid);
+}
+
+int gsa_batch_spmm_tables(void* stream, int vtype, int itype, const gsa_batch_desc* d_desc,
+                          const int32_t* d_tiles, int64_t ntiles, int* d_counter, float alpha,
+                          float beta, int grid) {
+    GSA_CHECK(ntiles >= 0 && d_counter && (ntiles == 0 || (d_desc && d_tiles)), GSA_ERR_INVALID,
+              "gsa_batch_spmm_tables: bad arguments");
+    return launch_batch(as_stream(stream), vtype, itype, d_desc, d_tiles, ntiles, d_counter, alpha,
+                        beta, grid);
+}
+
+int gsa_batch_graph_create(void* stream, gsa_plan plan, float alpha, float beta, void** graph_exec) {
+    GSA_CHECK(plan && plan->kind == 1 && graph_exec, GSA_ERR_INVALID, "gsa_batch_graph_create: bad arguments");
+    hipStream_t s = as_stream(stream);
+    hipGraph_t g = nullptr;
+    GSA_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+    int r = gsa_batch_spmm(stream, plan, alpha, beta);
+    hipError_t e = hipStreamEndCapture(s, &g);
+    if (r != GSA_OK) { if (g) (void)hipGraphDestroy(g); return r; }
+    if (e != hipSuccess) return hip_fail(e, "hipStreamEndCapture");
+    hipGraphExec_t ex = nullptr;
+    e = hipGraphInstantiate(&ex, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) return hip_fail(e, "hipGraphInstantiate");
+    *graph_exec = ex;
+    return GSA_OK;
+}
+
+int gsa_graph_launch(void* graph_exec, void* stream) {
+    GSA_CHECK(graph_exec, GSA_ERR_INVALID, "gsa_graph_launch: null graph");
+    GSA_HIP(hipGraphLaunch((hipGraphExec_t)graph_exec, as_stream(stream)));
+    return GSA_OK;
+}
+
+int gsa_graph_destroy(void* graph_exec) {
+    if (graph_exec) GSA_HIP(hipGraphExecDestroy((hipGraphExec_t)graph_exec));
+    return GSA_OK;
+}
+
+int gsa_csr_to_bf16(void* stream, int vtype, int itype, int64_t m, const void* rowptr,
+                    const void* nzval, uint16_t* nzval_bf16, float* row_scale) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && (vtype == GSA_F64 || vtype == GSA_F32), GSA_ERR_INVALID,
+              "gsa_csr_to_bf16: bad arguments");
+    if (m == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_i(itype, [&](auto iz) {
+        using I = decltype(iz);
+        hipLaunchKernelGGL(k_row_bf16<I>, dim3(grid_for(m, 256)), dim3(256), 0, s, m, (const I*)rowptr,
+                           nzval, vtype == GSA_F64 ? 1 : 0, nzval_bf16, row_scale);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    });
+}
+
 int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
                       const void* rowptr, const void* colval, const uint16_t* nzval_bf16,
                       const float* row_scale, int64_t nnz, const float* B, int64_t ldb, float* C,
                       int64_t ldc, float alpha, float beta, gsa_plan plan) {
-    (void)stream; (void)itype; (void)m; (void)n; (void)k; (void)rowptr; (void)colval;
-    (void)nzval_bf16; (void)row_scale; (void)nnz; (void)B; (void)ldb; (void)C; (void)ldc;
-    (void)alpha; (void)beta; (void)plan;
-    set_error(GSA_ERR_UNSUPPORTED, "gsa_spmm_csr_bf16: not built yet");
-    return GSA_ERR_UNSUPPORTED;
+    (void)nnz;
+    (void)plan;
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && n >= 0 && k >= 0 && m <= INT32_MAX && n <= INT32_MAX && k <= INT32_MAX,
+              GSA_ERR_INVALID, "gsa_spmm_csr_bf16: bad sizes");
+    GSA_CHECK(k <= 1 || (ldb >= n && ldc >= m), GSA_ERR_INVALID,
+              "gsa_spmm_csr_bf16: leading dimension too small");
+    GSA_CHECK(m == 0 || row_scale, GSA_ERR_INVALID, "gsa_spmm_csr_bf16: row_scale is null");
+    if (m == 0 || k == 0) return GSA_OK;
+    gsa_batch_desc d{};
+    d.rowptr = rowptr; d.colval = colval; d.nzval = nzval_bf16; d.row_scale = row_scale;
+    d.B = B; d.C = C; d.m = (int32_t)m; d.n = (int32_t)n; d.k = (int32_t)k;
+    d.ldb = (int32_t)(k <= 1 ? (n > 0 ? n : 1) : ldb);
+    d.ldc = (int32_t)(k <= 1 ? m : ldc);
+    int lpr = 1;
+    while (lpr < k && lpr < 64) lpr <<= 1;
+    int rows = 512 / lpr;
+    if (rows < 16) rows = 16;
+    const int mode = beta == 0.0f ? 0 : (beta == 1.0f ? 1 : 2);
+    hipStream_t s = as_stream(stream);
+    return dispatch_i(itype, [&](auto iz) {
+        using I = decltype(iz);
+        hipLaunchKernelGGL((k_single_spmm<uint16_t, I>), dim3((unsigned)cdiv(m, rows)), dim3(BWG), 0, s,
+                           d, rows, alpha, beta, mode);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    });
 }
-int gsa_batch_spmm(void* stream, gsa_plan plan, float alpha, float beta) {
-    (void)stream; (void)plan; (void)alpha; (void)beta;
-    set_error(GSA_ERR_UNSUPPORTED, "gsa_batch_spmm: not built yet");
-    return GSA_ERR_UNSUPPORTED;
-}
-}
+
+}  // extern "C"

@@ -48,6 +48,49 @@ __device__ __forceinline__ T wave_sum(T v) {
     return v;
 }
 
+// Knuth's TwoSum: s + e == a + b exactly (this library is built without FMA
+// contraction).
+template <typename T>
+__device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
+    s = a + b;
+    T bp = s - a;
+    T ap = s - bp;
+    e = (a - ap) + (b - bp);
+}
+
+// Float32 rows reduced across lanes are summed with TwoSum: each lane keeps a
+// (sum, compensation) pair, pairs are combined error-free in the butterfly.
+// Float64 keeps the plain sum.
+template <typename T>
+struct CompSum {
+    T s = T(0), c = T(0);
+    __device__ __forceinline__ void add(T p) {
+        if constexpr (sizeof(T) == 4) {
+            T ns, er;
+            two_sum(s, p, ns, er);
+            s = ns;
+            c = c + er;
+        } else {
+            s = s + p;
+        }
+    }
+    __device__ __forceinline__ T wave_total() {
+        if constexpr (sizeof(T) == 4) {
+#pragma unroll
+            for (int off = 32; off >= 1; off >>= 1) {
+                T os = __shfl_xor(s, off, 64), oc = __shfl_xor(c, off, 64);
+                T ns, er;
+                two_sum(s, os, ns, er);
+                s = ns;
+                c = c + oc + er;
+            }
+            return s + c;
+        } else {
+            return wave_sum(s);
+        }
+    }
+};
+
 // out = (β==0 ? 0 : β==1 ? y : y*β) + acc*α — the two roundings of
 // `_rmul_or_fill!(C, β)` followed by `C[row,k] += tmp*α` (matrix_csr.jl:192,
 // matrix_csr_kernels.jl:21).
@@ -235,9 +278,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
             const int src = __ffsll((long long)lm) - 1;
             lm &= lm - 1;
             const int ls = __shfl(s, src, 64), le = __shfl(e, src, 64);
-            T acc = T(0);
-            for (int j = ls + lane; j < le; j += 64) acc = acc + prod[j];
-            acc = wave_sum(acc);
+            CompSum<T> cs;
+            for (int j = ls + lane; j < le; j += 64) cs.add(prod[j]);
+            const T acc = cs.wave_total();
             if (lane == 0) store_out(y, base + src, acc, alpha, beta, mode);
         }
     }
@@ -245,22 +288,24 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
     // 3. spill row: the wavefront reduces it from HBM
     if (rs_end < r1) {
         const int64_t r = r1 - 1;
-        T acc = T(0);
+        CompSum<T> cs;
         int64_t j = Ps + lane;
-        for (; j + 3 * 64 < Pend; j += 4 * 64) {
-            I c0 = ldnt(colval + j), c1 = ldnt(colval + j + 64);
-            I c2 = ldnt(colval + j + 128), c3 = ldnt(colval + j + 192);
-            T v0 = ldnt(nzval + j), v1 = ldnt(nzval + j + 64);
-            T v2 = ldnt(nzval + j + 128), v3 = ldnt(nzval + j + 192);
-            T p0 = v0 * x[clamp64((int64_t)c0 - 1, 0, n - 1)];
-            T p1 = v1 * x[clamp64((int64_t)c1 - 1, 0, n - 1)];
-            T p2 = v2 * x[clamp64((int64_t)c2 - 1, 0, n - 1)];
-            T p3 = v3 * x[clamp64((int64_t)c3 - 1, 0, n - 1)];
-            acc = acc + p0; acc = acc + p1; acc = acc + p2; acc = acc + p3;
+        for (; j + 7 * 64 < Pend; j += 8 * 64) {
+            I cc[8];
+            T vv[8], xv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                cc[u] = ldnt(colval + j + u * 64);
+                vv[u] = ldnt(nzval + j + u * 64);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) xv[u] = x[clamp64((int64_t)cc[u] - 1, 0, n - 1)];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) cs.add(vv[u] * xv[u]);
         }
         for (; j < Pend; j += 64)
-            acc = acc + ldnt(nzval + j) * x[clamp64((int64_t)ldnt(colval + j) - 1, 0, n - 1)];
-        acc = wave_sum(acc);
+            cs.add(ldnt(nzval + j) * x[clamp64((int64_t)ldnt(colval + j) - 1, 0, n - 1)]);
+        const T acc = cs.wave_total();
         if (lane == 0) store_out(y, r, acc, alpha, beta, mode);
     }
 }

@@ -45,7 +45,7 @@ _pi64 = ctypes.POINTER(ctypes.c_int64)
 
 
 class BatchDesc(ctypes.Structure):
-    """gsa_batch_desc (64 bytes)."""
+    """gsa_batch_desc (72 bytes)."""
     _fields_ = [
         ("rowptr", _vp), ("colval", _vp), ("nzval", _vp), ("row_scale", _vp),
         ("B", _vp), ("C", _vp),
@@ -98,6 +98,11 @@ _SIGS = {
     "gsa_batch_plan_create": [_vp, _i32, _i32, _i64, ctypes.POINTER(BatchDesc), _i32,
                               ctypes.POINTER(_vp)],
     "gsa_batch_spmm": [_vp, _vp, _f32, _f32],
+    "gsa_batch_spmm_tables": [_vp, _i32, _i32, _vp, _vp, _i64, _vp, _f32, _f32, _i32],
+    "gsa_batch_graph_create": [_vp, _vp, _f32, _f32, ctypes.POINTER(_vp)],
+    "gsa_graph_launch": [_vp, _vp],
+    "gsa_graph_destroy": [_vp],
+    "gsa_csr_to_bf16": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {"gsa_last_error": ctypes.c_char_p}
 

@@ -234,7 +234,7 @@ int gsa_ctx_destroy(gsa_ctx ctx);
 /* ========================== batched uneven CSR SpMM ======================== */
 /* Many independent CSR matrices in ONE persistent launch.  Each instance i has
  * its own (m_i, n_i, k_i) and pointers; the descriptor table lives in device
- * memory (gsa_batch_desc, 64 bytes each).  Row work is split into tiles by a
+ * memory (gsa_batch_desc, 72 bytes each).  Row work is split into tiles by a
  * per-batch schedule built on the device (gsa_batch_plan_create), so the
  * launch is graph-capturable (no host work, no allocation at run time).
  * No single counterpart in the reference: it replaces a loop of
@@ -249,11 +249,33 @@ typedef struct {
     int32_t m, n, k, ldb, ldc, pad;
 } gsa_batch_desc;
 
+/* Tiles: instance i is cut into row tiles of max(16, min(1024,
+ * rows_per_tile / next_pow2(k_i))) rows.  vtype GSA_BF16 needs row_scale. */
 int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
                           const gsa_batch_desc* host_desc, int rows_per_tile,
                           gsa_plan* plan);
-/* y_i = alpha*A_i*B_i + beta*C_i for all i; one launch on `stream`. */
+/* C_i = alpha*A_i*B_i + beta*C_i for all i: a counter-reset kernel and one
+ * persistent launch on `stream`; allocates nothing, so it can be captured. */
 int gsa_batch_spmm(void* stream, gsa_plan plan, float alpha, float beta);
+/* The same launch on caller-built device tables: d_tiles holds 3 int32 per
+ * tile (instance, first row, end row), d_counter one int of scratch.  For
+ * serving loops that regroup instances per launch.  grid <= 0: 8 per CU. */
+int gsa_batch_spmm_tables(void* stream, int vtype, int itype, const gsa_batch_desc* d_desc,
+                          const int32_t* d_tiles, int64_t ntiles, int* d_counter,
+                          float alpha, float beta, int grid);
+/* Capture gsa_batch_spmm into an executable HIP graph; replay with
+ * gsa_graph_launch, release with gsa_graph_destroy. */
+int gsa_batch_graph_create(void* stream, gsa_plan plan, float alpha, float beta,
+                           void** graph_exec);
+int gsa_graph_launch(void* graph_exec, void* stream);
+int gsa_graph_destroy(void* graph_exec);
+
+/* BF16 storage of CSR values with one power-of-two Float32 scale per row
+ * (scale_i >= max_j |a_ij|, so a_ij / scale_i is exact before rounding to
+ * BF16).  Input Float64 or Float32.  Serving precision variant; no
+ * counterpart in the reference. */
+int gsa_csr_to_bf16(void* stream, int vtype, int itype, int64_t m, const void* rowptr,
+                    const void* nzval, uint16_t* nzval_bf16, float* row_scale);
 
 #ifdef __cplusplus
 }
