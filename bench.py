@@ -112,6 +112,228 @@ def cpu_baseline(H, x, seconds=10.0):
                       f"({el:.1f} s, {per * 1e3:.2f} ms each)"}
 
 
+def _sync_time(fn, reps=3):
+    """Median wall time (s) of fn() with device syncs around it (ops that read a
+    size back to the host synchronise anyway)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    ts = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        fn()
+        torch.cuda.synchronize()
+        ts.append(time.perf_counter() - t0)
+    return sorted(ts)[len(ts) // 2]
+
+
+def bench_structural(H, dev):
+    """Structural ops: CSR<->CSC<->COO on the 1M power-law matrix; L + L', L*L,
+    R*R and kron on the Laplacian / random / density-0.01 operands."""
+    import torch
+    import gsa_hip as G
+    from gsa_hip import synth
+
+    def dcsr(M, itype=torch.int64):
+        return G.GenericSparseMatrixCSR(M.m, M.n, torch.as_tensor(M.rowptr).to(dev, itype),
+                                        torch.as_tensor(M.colval).to(dev, itype),
+                                        torch.as_tensor(M.nzval).to(dev), copy=False)
+    out = {}
+    A = dcsr(H)
+    nnz, m, n = H.nnz, H.m, H.n
+    Ac = G.GenericSparseMatrixCSC(A)
+    Ao = G.GenericSparseMatrixCOO(A)
+    for name, fn, byts in (
+            ("csr_to_csc", lambda: G.GenericSparseMatrixCSC(A),
+             nnz * 16 + (m + 1) * 8 + nnz * 16 + (n + 1) * 8),
+            ("csc_to_csr", lambda: G.GenericSparseMatrixCSR(Ac),
+             nnz * 16 + (n + 1) * 8 + nnz * 16 + (m + 1) * 8),
+            ("csr_to_coo", lambda: G.GenericSparseMatrixCOO(A),
+             nnz * 16 + (m + 1) * 8 + nnz * 24),
+            ("coo_to_csr", lambda: G.GenericSparseMatrixCSR(Ao),
+             nnz * 24 + nnz * 16 + (m + 1) * 8)):
+        t = _sync_time(fn)
+        out[name] = {"ms": round(t * 1e3, 3), "GBs": round(byts / t / 1e9, 1)}
+    del Ac, Ao, A
+    L = synth.laplacian2d(1000)
+    Lp = synth.drop_entries(L, 0.1, seed=3)
+    R = synth.fixed_row_csr(200_000, 200_000, 8, seed=3)
+    K1 = synth.sprand(1000, 1000, 0.01, seed=31)
+    K2 = synth.sprand(1000, 1000, 0.01, seed=32)
+    dL, dLp, dR, dK1, dK2 = (dcsr(M) for M in (L, Lp, R, K1, K2))
+    C = dL + dLp
+    t = _sync_time(lambda: dL + dLp)
+    out["add_L_Lp"] = {"ms": round(t * 1e3, 3), "nnzC": C.nnz(),
+                       "GBs": round(((L.nnz + Lp.nnz) * 16 + 2 * (L.m + 1) * 8 + C.nnz() * 16
+                                     + (L.m + 1) * 8) / t / 1e9, 1)}
+    for name, X in (("spgemm_LL", dL), ("spgemm_RR", dR)):
+        Cx = X * X
+        t = _sync_time(lambda: X * X)
+        out[name] = {"ms": round(t * 1e3, 3), "nnzC": Cx.nnz(),
+                     "GBs_lower_bound": round((2 * X.nnz() + Cx.nnz()) * 16 / t / 1e9, 1)}
+        del Cx
+    Kc = G.kron(dK1, dK2)
+    t = _sync_time(lambda: G.kron(dK1, dK2))
+    out["kron_csr_1000x1000_d0.01"] = {"ms": round(t * 1e3, 3), "nnzC": Kc.nnz(),
+                                       "GBs": round((Kc.nnz() * 16 + (Kc.m + 1) * 8) / t / 1e9, 1)}
+    del Kc
+    torch.cuda.empty_cache()
+    return out
+
+
+def bench_batched(dev, count=4096):
+    """4096 uneven Float32 CSR instances (rows log-uniform in [64, 65536],
+    Poisson(8)+1 per row, k in {1,2,4,8,16}) in one graph-captured persistent
+    launch, against 4096 plain mul! calls on the same operands."""
+    import numpy as np
+    import torch
+    import gsa_hip as G
+    from gsa_hip import synth
+    from gsa_hip.autotune import BatchAutotuner
+    from gsa_hip.batched import BatchedSpMM
+
+    t0 = time.time()
+    insts, kk = synth.uneven_batch_device(count, seed=4, device=dev)
+    items = []
+    nnz_tot = rows_tot = bytes_tot = flops = 0
+    plain_out = []
+    for (m, rp, cv, nz), k in zip(insts, kk):
+        A = G.GenericSparseMatrixCSR(m, m, rp, cv, nz, copy=False)
+        # row-major n x k blocks (Julia: transpose(Bt)): the k values an entry
+        # needs share a cache line; mul! takes them without a copy as well
+        B = torch.randn((m, k), dtype=torch.float32, device=dev)
+        C = torch.empty((m, k), dtype=torch.float32, device=dev)
+        items.append((A, B if k > 1 else B[:, 0].contiguous(), C if k > 1 else C[:, 0].contiguous()))
+        plain_out.append(torch.empty((k, m), dtype=torch.float32, device=dev).t() if k > 1
+                         else torch.empty(m, dtype=torch.float32, device=dev))
+        nnz_tot += nz.numel()
+        rows_tot += m
+        bytes_tot += nz.numel() * 8 + (m + 1) * 4 + 2 * m * k * 4
+        flops += 2 * nz.numel() * k
+    print(f"[bench] batch of {count}: {rows_tot} rows, {nnz_tot} nnz built in "
+          f"{time.time() - t0:.1f}s", file=sys.stderr, flush=True)
+    tuner = BatchAutotuner(candidates=(256, 512, 1024, 2048, 4096), reps=3)
+    tuned = tuner.tune(items)
+    batch = BatchedSpMM(items, rows_per_tile=tuned["rows_per_tile"])
+    g = batch.graph()
+    for _ in range(3):
+        g.replay()
+    torch.cuda.synchronize()
+    reps = 20
+    a = torch.cuda.Event(enable_timing=True)
+    b = torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        g.replay()
+    b.record()
+    b.synchronize()
+    ms_batch = a.elapsed_time(b) / reps
+    # plain: one mul! per instance (plans built beforehand)
+    for A, _, _ in items:
+        A.plan()
+    torch.cuda.synchronize()
+
+    def plain():
+        for (A, B, _), C in zip(items, plain_out):
+            G.mul_(C, A, B, 1.0, 0.0)
+    t_plain = _sync_time(plain, reps=2)
+    # accuracy of the batched results against the Float64 product of a sample
+    worst = 0.0
+    batch.run()
+    for i in range(0, count, max(1, count // 16)):
+        A, B, C = items[i]
+        A64 = G.GenericSparseMatrixCSR(A.m, A.n, A.rowptr, A.colval, A.nzval.double(), copy=False)
+        ref = G.times(A64, B.double())
+        Aabs = G.GenericSparseMatrixCSR(A.m, A.n, A.rowptr, A.colval, A.nzval.double().abs(),
+                                        copy=False)
+        bound = G.times(Aabs, B.double().abs())
+        err = ((C.double() - ref).abs() / bound.clamp_min(1e-300)).max().item()
+        worst = max(worst, err)
+    return {"instances": count, "rows": rows_tot, "nnz": nnz_tot,
+            "batched_graph_ms": round(ms_batch, 3),
+            "batched_req_per_s": round(count / (ms_batch * 1e-3), 1),
+            "plain_mul_ms": round(t_plain * 1e3, 3),
+            "plain_req_per_s": round(count / t_plain, 1),
+            "speedup": round(t_plain * 1e3 / ms_batch, 2),
+            "batched_GBs": round(bytes_tot / (ms_batch * 1e-3) / 1e9, 1),
+            "batched_GFLOPs": round(flops / (ms_batch * 1e-3) / 1e9, 1),
+            "rows_per_tile": tuned["rows_per_tile"], "autotune_ms": tuned["timings"],
+            "max_rel_err_vs_f64": worst}
+
+
+def bench_serving(dev, rates=(2000, 10000, 50000), small=512, big=8):
+    """Served stream: 512 uneven instances + 8 of the 1M-row power-law matrices,
+    Float32 and BF16 storage, Poisson arrivals, k in {1, 4, 16}; against one
+    plain mul! per request."""
+    import numpy as np
+    import torch
+    import gsa_hip as G
+    from gsa_hip import synth
+    from gsa_hip.precision import BF16CSR
+    from gsa_hip.serving import MatrixRegistry, Server, PlainServer, Requests, output_view
+
+    insts, _ = synth.uneven_batch_device(small, seed=5, device=dev)
+    mats = [G.GenericSparseMatrixCSR(m, m, rp, cv, nz, copy=False) for m, rp, cv, nz in insts]
+    for b in range(big):
+        Hb = synth.powerlaw_csr(1_000_000, 1_000_000, alpha=1.846, seed=100 + b, dtype=np.float32)
+        mats.append(G.GenericSparseMatrixCSR(Hb.m, Hb.n, torch.as_tensor(Hb.rowptr).to(dev, torch.int32),
+                                             torch.as_tensor(Hb.colval).to(dev, torch.int32),
+                                             torch.as_tensor(Hb.nzval).to(dev), copy=False))
+    xs = [torch.randn((A.n, 16), dtype=torch.float32, device=dev) for A in mats]
+    out = {"matrices": len(mats)}
+    for fmt in ("f32", "bf16"):
+        reg = MatrixRegistry(dev, rows_per_tile=1024, max_k=16)
+        for A, X in zip(mats, xs):
+            reg.register(BF16CSR(A) if fmt == "bf16" else A, X)
+        for A in mats:
+            A.plan()
+        res = {}
+        for rate in rates:
+            nreq = int(min(rate * 1.0, 20000))
+            req = Requests.poisson(nreq, rate, len(mats), ks=(1, 4, 16), seed=5)
+            srv = Server(reg, hbm_budget=2 << 30, slots=4)
+            sv = srv.run(req, keep_outputs=True)
+            pl = PlainServer(reg).run(req)
+            worst = 0.0
+            for i in range(0, nreq, max(1, nreq // 12)):
+                mid, k = int(req.mid[i]), int(req.k[i])
+                A = mats[mid]
+                Xk = xs[mid][:, :k]
+                # error against the Float64 product of the ORIGINAL Float32 values
+                A64 = G.GenericSparseMatrixCSR(A.m, A.n, A.rowptr, A.colval, A.nzval.double(),
+                                               copy=False)
+                ref = G.times(A64, Xk.double())
+                Aabs = G.GenericSparseMatrixCSR(A.m, A.n, A.rowptr, A.colval,
+                                                A.nzval.double().abs(), copy=False)
+                bound = G.times(Aabs, Xk.double().abs())
+                got = output_view(sv["outputs"][i]).double()
+                err = ((got - ref).abs() / bound.clamp_min(1e-300)).max().item()
+                worst = max(worst, err)
+            res[str(rate)] = {"requests": nreq,
+                              "served_req_per_s": round(sv["req_per_s"], 1),
+                              "served_p50_ms": round(sv["p50_ms"], 3),
+                              "served_p99_ms": round(sv["p99_ms"], 3),
+                              "launches": sv["launches"], "mean_group": round(sv["mean_group"], 1),
+                              "plain_req_per_s": round(pl["req_per_s"], 1),
+                              "plain_p50_ms": round(pl["p50_ms"], 3),
+                              "plain_p99_ms": round(pl["p99_ms"], 3),
+                              "max_rel_err_vs_f64": worst}
+            print(f"[bench] serving {fmt} {rate}/s: served {sv['req_per_s']:.0f} req/s p99 "
+                  f"{sv['p99_ms']:.2f} ms; plain {pl['req_per_s']:.0f} req/s p99 "
+                  f"{pl['p99_ms']:.2f} ms; err {worst:.2e}", file=sys.stderr, flush=True)
+        # capacity: every request available at t=0
+        nreq = 20000
+        r6 = Requests.poisson(nreq, 1e9, len(mats), ks=(1, 4, 16), seed=6)
+        req = Requests(np.zeros(nreq), r6.mid, r6.k)
+        sv = Server(reg, hbm_budget=2 << 30, slots=4).run(req)
+        pl = PlainServer(reg).run(req)
+        res["capacity"] = {"requests": nreq, "served_req_per_s": round(sv["req_per_s"], 1),
+                           "plain_req_per_s": round(pl["req_per_s"], 1),
+                           "ratio": round(sv["req_per_s"] / pl["req_per_s"], 2)}
+        out[fmt] = res
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,6 +344,9 @@ def main():
     ap.add_argument("--copies", type=int, default=3)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
+    ap.add_argument("--no-structural", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the headline SpMV steps (for rocprofv3)")
     args = ap.parse_args()
@@ -207,6 +432,15 @@ def main():
         print(f"[bench] SpMM k=32 {ms3 * 1e3:.0f} us", file=sys.stderr, flush=True)
         del A, B, C
         torch.cuda.empty_cache()
+
+    if not args.no_extra and not args.no_structural:
+        extra["structural"] = bench_structural(H, dev)
+        print(f"[bench] structural {extra['structural']}", file=sys.stderr, flush=True)
+    if not args.no_extra and not args.no_batched:
+        extra["batched"] = bench_batched(dev)
+        print(f"[bench] batched {extra['batched']}", file=sys.stderr, flush=True)
+    if not args.no_extra and not args.no_serving:
+        extra["serving"] = bench_serving(dev)
 
     cpu = None
     if not args.no_cpu:

@@ -21,6 +21,7 @@ namespace gsa {
 constexpr int BWG = 256;
 constexpr int BATCH_LONG = 256;   // rows above this: workgroup-cooperative, compensated
 constexpr int BATCH_MAX_TILE_ROWS = 1024;
+constexpr int BATCH_TCAP = 3072;   // entries of a tile staged in LDS
 
 struct BatchPlan {
     int vtype = 0, itype = 0;
@@ -72,11 +73,16 @@ template <typename T>
 struct TileLds {
     int nlong;
     int long_rows[BATCH_MAX_TILE_ROWS];
+    int off[BATCH_MAX_TILE_ROWS + 1];   // staged row offsets (relative to P0)
+    int col[BATCH_TCAP];                // staged column indices (0-based, clamped)
+    T val[BATCH_TCAP];                  // staged values (BF16 widened)
     T red[BWG];
     T cmp[BWG];
 };
 
 // One tile: rows [r0, r1) of instance d.  All BWG lanes call it together.
+// The tile's row offsets and its first BATCH_TCAP entries are staged in LDS
+// with coalesced loads; rows beyond the staged extent read HBM directly.
 template <typename V, typename I>
 __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0, int64_t r1,
                                              typename Acc<V>::T alpha, typename Acc<V>::T beta,
@@ -91,83 +97,118 @@ __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0
     T* __restrict__ C = (T*)d.C;
     const int k = d.k;
     const int64_t n = d.n > 0 ? d.n : 1;  // clamp bound for column indices
-    const int64_t ldb = d.ldb, ldc = d.ldc;
+    // layout flags: B[i,kk] at i + kk*ldb (column-major) or i*ldb + kk
+    // (row-major); the same for C with ldc
+    const bool brow = (d.flags & GSA_BATCH_B_ROWMAJOR) != 0;
+    const bool crow = (d.flags & GSA_BATCH_C_ROWMAJOR) != 0;
+    const int64_t bsr = brow ? d.ldb : 1, bsc = brow ? 1 : d.ldb;
+    const int64_t csr = crow ? d.ldc : 1, csc = crow ? 1 : d.ldc;
+    const int nrows = (int)(r1 - r0);
+    const int64_t P0 = (int64_t)rowptr[r0] - 1;
+    const int64_t P1 = (int64_t)rowptr[r1] - 1;
+    const int cnt = (int)(P1 - P0 < BATCH_TCAP ? P1 - P0 : BATCH_TCAP);
     if (tid == 0) sh.nlong = 0;
+    for (int i = tid; i <= nrows; i += BWG) sh.off[i] = (int)((int64_t)rowptr[r0 + i] - 1 - P0);
+    for (int i = tid; i < cnt; i += BWG) {
+        int64_t c = (int64_t)colval[P0 + i] - 1;
+        sh.col[i] = (int)(c < 0 ? 0 : (c >= n ? n - 1 : c));
+        sh.val[i] = vload(nzval + P0 + i);
+    }
     __syncthreads();
     int lpr = 1;
     while (lpr < k && lpr < 64) lpr <<= 1;
     const int groups = BWG / lpr;
     const int g = tid / lpr, lane = tid % lpr;
-    for (int64_t r = r0 + g; r < r1; r += groups) {
-        const int64_t s = (int64_t)rowptr[r] - 1, e = (int64_t)rowptr[r + 1] - 1;
+    for (int rr = g; rr < nrows; rr += groups) {
+        const int64_t r = r0 + rr;
+        const int s = sh.off[rr], e = sh.off[rr + 1];
         if (e - s > BATCH_LONG) {
             if (lane == 0) {
                 int q = atomicAdd(&sh.nlong, 1);
-                if (q < BATCH_MAX_TILE_ROWS) sh.long_rows[q] = (int)(r - r0);
+                if (q < BATCH_MAX_TILE_ROWS) sh.long_rows[q] = rr;
             }
             continue;
         }
+        const bool staged = e <= cnt;
         for (int kk = lane; kk < k; kk += lpr) {
+            // 8 gathers in flight per lane; the sum stays left to right
             T acc = T(0);
-            int64_t j = s;
-            for (; j + 3 < e; j += 4) {
-                int64_t c0 = colval[j] - 1, c1 = colval[j + 1] - 1;
-                int64_t c2 = colval[j + 2] - 1, c3 = colval[j + 3] - 1;
-                c0 = c0 < 0 ? 0 : (c0 >= n ? n - 1 : c0);
-                c1 = c1 < 0 ? 0 : (c1 >= n ? n - 1 : c1);
-                c2 = c2 < 0 ? 0 : (c2 >= n ? n - 1 : c2);
-                c3 = c3 < 0 ? 0 : (c3 >= n ? n - 1 : c3);
-                T p0 = vload(nzval + j) * B[c0 + kk * ldb];
-                T p1 = vload(nzval + j + 1) * B[c1 + kk * ldb];
-                T p2 = vload(nzval + j + 2) * B[c2 + kk * ldb];
-                T p3 = vload(nzval + j + 3) * B[c3 + kk * ldb];
-                acc = acc + p0; acc = acc + p1; acc = acc + p2; acc = acc + p3;
-            }
-            for (; j < e; ++j) {
-                int64_t c = colval[j] - 1;
-                c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-                acc = acc + vload(nzval + j) * B[c + kk * ldb];
+            for (int j = s; j < e; j += 8) {
+                T pv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int jj = j + u < e ? j + u : e - 1;
+                    int64_t c;
+                    T v;
+                    if (staged) {
+                        c = sh.col[jj];
+                        v = sh.val[jj];
+                    } else {
+                        c = (int64_t)colval[P0 + jj] - 1;
+                        c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+                        v = vload(nzval + P0 + jj);
+                    }
+                    pv[u] = v * B[c * bsr + kk * bsc];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (j + u < e) acc = acc + pv[u];
             }
             if (scale) acc = acc * (T)scale[r];
-            T* out = C + r + kk * ldc;
+            T* out = C + r * csr + kk * csc;
             T tt = acc * alpha;
             T base = mode == 0 ? T(0) : (mode == 1 ? *out : *out * beta);
             *out = base + tt;
         }
     }
     __syncthreads();
-    // long rows: the whole workgroup, TwoSum-compensated
+    // long rows: the groups stride over the entries, each lane owns one kk
+    // (all k columns in one pass), TwoSum per lane, then an error-free
+    // combine of the groups' partial sums through LDS
     const int nl = sh.nlong < BATCH_MAX_TILE_ROWS ? sh.nlong : BATCH_MAX_TILE_ROWS;
     for (int q = 0; q < nl; ++q) {
         const int64_t r = r0 + sh.long_rows[q];
-        const int64_t s = (int64_t)rowptr[r] - 1, e = (int64_t)rowptr[r + 1] - 1;
-        for (int kk = 0; kk < k; ++kk) {
+        const int64_t s = P0 + sh.off[sh.long_rows[q]], e = P0 + sh.off[sh.long_rows[q] + 1];
+        for (int k0 = 0; k0 < k; k0 += lpr) {
+            const int kk = k0 + lane;
+            const bool act = kk < k;
+            const int kc = act ? kk : 0;
             T sum = T(0), cmp = T(0);
-            for (int64_t j = s + tid; j < e; j += BWG) {
-                int64_t c = colval[j] - 1;
-                c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-                T p = vload(nzval + j) * B[c + kk * ldb];
-                T ns, er;
-                two_sum(sum, p, ns, er);
-                sum = ns;
-                cmp = cmp + er;
+            for (int64_t j = s + g; j < e; j += 4 * (int64_t)groups) {
+                T pv[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int64_t j2 = j + u * (int64_t)groups;
+                    const int64_t jj = j2 < e ? j2 : e - 1;
+                    int64_t c = (int64_t)colval[jj] - 1;
+                    c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+                    pv[u] = (j2 < e) ? vload(nzval + jj) * B[c * bsr + kc * bsc] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    T ns, er;
+                    two_sum(sum, pv[u], ns, er);
+                    sum = ns;
+                    cmp = cmp + er;
+                }
             }
             sh.red[tid] = sum;
             sh.cmp[tid] = cmp;
             __syncthreads();
-            for (int h = BWG / 2; h > 0; h >>= 1) {
-                if (tid < h) {
+            for (int h = groups / 2; h > 0; h >>= 1) {
+                if (g < h) {
+                    const int o = tid + h * lpr;
                     T ns, er;
-                    two_sum(sh.red[tid], sh.red[tid + h], ns, er);
+                    two_sum(sh.red[tid], sh.red[o], ns, er);
                     sh.red[tid] = ns;
-                    sh.cmp[tid] = sh.cmp[tid] + sh.cmp[tid + h] + er;
+                    sh.cmp[tid] = sh.cmp[tid] + sh.cmp[o] + er;
                 }
                 __syncthreads();
             }
-            if (tid == 0) {
-                T acc = sh.red[0] + sh.cmp[0];
+            if (g == 0 && act) {
+                T acc = sh.red[tid] + sh.cmp[tid];
                 if (scale) acc = acc * (T)scale[r];
-                T* out = C + r + kk * ldc;
+                T* out = C + r * csr + kk * csc;
                 T tt = acc * alpha;
                 T base = mode == 0 ? T(0) : (mode == 1 ? *out : *out * beta);
                 *out = base + tt;
@@ -175,6 +216,7 @@ __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0
             __syncthreads();
         }
     }
+    __syncthreads();  // the LDS staging is reused by the next tile
 }
 
 // Persistent: tiles from the table, taken through the atomic counter.
@@ -196,18 +238,6 @@ __global__ __launch_bounds__(BWG) void k_batch_spmm(const gsa_batch_desc* __rest
         const gsa_batch_desc d = desc[tiles[3 * t]];
         process_tile<V, I>(d, tiles[3 * t + 1], tiles[3 * t + 2], alpha, beta, mode, sh);
     }
-}
-
-// One instance, descriptor by value, tile = blockIdx.x * rows.
-template <typename V, typename I>
-__global__ __launch_bounds__(BWG) void k_single_spmm(gsa_batch_desc d, int rows,
-                                                     typename Acc<V>::T alpha,
-                                                     typename Acc<V>::T beta, int mode) {
-    using T = typename Acc<V>::T;
-    __shared__ TileLds<T> sh;
-    const int64_t r0 = (int64_t)blockIdx.x * rows;
-    const int64_t r1 = r0 + rows < d.m ? r0 + rows : d.m;
-    process_tile<V, I>(d, r0, r1, alpha, beta, mode, sh);
 }
 
 template <typename I>
@@ -306,8 +336,11 @@ int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
                   (long long)i);
         GSA_CHECK(d.m == 0 || (d.rowptr && d.C), GSA_ERR_INVALID, "instance %lld: null pointer",
                   (long long)i);
-        GSA_CHECK(d.k <= 1 || (d.ldb >= d.n && d.ldc >= d.m), GSA_ERR_INVALID,
-                  "instance %lld: leading dimension too small", (long long)i);
+        const bool brow = (d.flags & GSA_BATCH_B_ROWMAJOR) != 0;
+        const bool crow = (d.flags & GSA_BATCH_C_ROWMAJOR) != 0;
+        GSA_CHECK(d.k < 1 || ((brow ? d.ldb >= d.k : (d.k == 1 || d.ldb >= d.n)) &&
+                              (crow ? d.ldc >= d.k : (d.k == 1 || d.ldc >= d.m))),
+                  GSA_ERR_INVALID, "instance %lld: leading dimension too small", (long long)i);
         GSA_CHECK(vtype != GSA_BF16 || d.row_scale || d.m == 0, GSA_ERR_INVALID,
                   "instance %lld: BF16 values need row_scale", (long long)i);
     }
@@ -398,39 +431,6 @@ int gsa_csr_to_bf16(void* stream, int vtype, int itype, int64_t m, const void* r
         using I = decltype(iz);
         hipLaunchKernelGGL(k_row_bf16<I>, dim3(grid_for(m, 256)), dim3(256), 0, s, m, (const I*)rowptr,
                            nzval, vtype == GSA_F64 ? 1 : 0, nzval_bf16, row_scale);
-        GSA_HIP(hipGetLastError());
-        return GSA_OK;
-    });
-}
-
-int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
-                      const void* rowptr, const void* colval, const uint16_t* nzval_bf16,
-                      const float* row_scale, int64_t nnz, const float* B, int64_t ldb, float* C,
-                      int64_t ldc, float alpha, float beta, gsa_plan plan) {
-    (void)nnz;
-    (void)plan;
-    GSA_TRY(ensure_init());
-    GSA_CHECK(m >= 0 && n >= 0 && k >= 0 && m <= INT32_MAX && n <= INT32_MAX && k <= INT32_MAX,
-              GSA_ERR_INVALID, "gsa_spmm_csr_bf16: bad sizes");
-    GSA_CHECK(k <= 1 || (ldb >= n && ldc >= m), GSA_ERR_INVALID,
-              "gsa_spmm_csr_bf16: leading dimension too small");
-    GSA_CHECK(m == 0 || row_scale, GSA_ERR_INVALID, "gsa_spmm_csr_bf16: row_scale is null");
-    if (m == 0 || k == 0) return GSA_OK;
-    gsa_batch_desc d{};
-    d.rowptr = rowptr; d.colval = colval; d.nzval = nzval_bf16; d.row_scale = row_scale;
-    d.B = B; d.C = C; d.m = (int32_t)m; d.n = (int32_t)n; d.k = (int32_t)k;
-    d.ldb = (int32_t)(k <= 1 ? (n > 0 ? n : 1) : ldb);
-    d.ldc = (int32_t)(k <= 1 ? m : ldc);
-    int lpr = 1;
-    while (lpr < k && lpr < 64) lpr <<= 1;
-    int rows = 512 / lpr;
-    if (rows < 16) rows = 16;
-    const int mode = beta == 0.0f ? 0 : (beta == 1.0f ? 1 : 2);
-    hipStream_t s = as_stream(stream);
-    return dispatch_i(itype, [&](auto iz) {
-        using I = decltype(iz);
-        hipLaunchKernelGGL((k_single_spmm<uint16_t, I>), dim3((unsigned)cdiv(m, rows)), dim3(BWG), 0, s,
-                           d, rows, alpha, beta, mode);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
     });

@@ -41,6 +41,19 @@ __device__ __forceinline__ U ldnt(const U* p) {
     return __builtin_nontemporal_load(p);
 }
 
+// Stored value -> arithmetic type: identity, or BF16 bits -> Float32.
+template <typename T, typename V>
+__device__ __forceinline__ T widen(V v) {
+    if constexpr (sizeof(V) == 2) return __uint_as_float(((uint32_t)v) << 16);
+    else return (T)v;
+}
+
+// Row sum times the row's scale (BF16 storage), or unchanged.
+template <typename T>
+__device__ __forceinline__ T row_scaled(T acc, const float* __restrict__ rscale, int64_t r) {
+    return rscale ? acc * (T)rscale[r] : acc;
+}
+
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
@@ -196,12 +209,12 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T, typename I>
+template <typename T, typename I, typename V = T>
 __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
     int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowptr,
-    const I* __restrict__ colval, const T* __restrict__ nzval, const T* __restrict__ x,
-    T* __restrict__ y, T alpha, T beta, int mode, const int64_t* __restrict__ win_row,
-    const int64_t* __restrict__ win_pos, int64_t nwin) {
+    const I* __restrict__ colval, const V* __restrict__ nzval, const float* __restrict__ rscale,
+    const T* __restrict__ x, T* __restrict__ y, T alpha, T beta, int mode,
+    const int64_t* __restrict__ win_row, const int64_t* __restrict__ win_pos, int64_t nwin) {
     __shared__ T s_prod[WAVES][SPMV_WCAP];
     __shared__ int s_off[WAVES][WRP];
 
@@ -235,7 +248,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
             int i = u * 64 + lane;
             i = i < cnt ? i : cnt - 1;
             ci[u] = ldnt(colval + P0 + i);
-            vv[u] = ldnt(nzval + P0 + i);
+            vv[u] = widen<T>(ldnt(nzval + P0 + i));
         }
 #pragma unroll
         for (int u = 0; u < WU; ++u) xv[u] = x[clamp64((int64_t)ci[u] - 1, 0, n - 1)];
@@ -271,7 +284,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
         if (act && !lng) {
             T acc = T(0);
             for (int j = s; j < e; ++j) acc = acc + prod[j];
-            store_out(y, r, acc, alpha, beta, mode);
+            store_out(y, r, row_scaled(acc, rscale, r), alpha, beta, mode);
         }
         uint64_t lm = __ballot(lng);
         while (lm) {
@@ -281,7 +294,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
             CompSum<T> cs;
             for (int j = ls + lane; j < le; j += 64) cs.add(prod[j]);
             const T acc = cs.wave_total();
-            if (lane == 0) store_out(y, base + src, acc, alpha, beta, mode);
+            if (lane == 0) store_out(y, base + src, row_scaled(acc, rscale, base + src), alpha, beta, mode);
         }
     }
 
@@ -296,7 +309,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
 #pragma unroll
             for (int u = 0; u < 8; ++u) {
                 cc[u] = ldnt(colval + j + u * 64);
-                vv[u] = ldnt(nzval + j + u * 64);
+                vv[u] = widen<T>(ldnt(nzval + j + u * 64));
             }
 #pragma unroll
             for (int u = 0; u < 8; ++u) xv[u] = x[clamp64((int64_t)cc[u] - 1, 0, n - 1)];
@@ -304,9 +317,9 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
             for (int u = 0; u < 8; ++u) cs.add(vv[u] * xv[u]);
         }
         for (; j < Pend; j += 64)
-            cs.add(ldnt(nzval + j) * x[clamp64((int64_t)ldnt(colval + j) - 1, 0, n - 1)]);
+            cs.add(widen<T>(ldnt(nzval + j)) * x[clamp64((int64_t)ldnt(colval + j) - 1, 0, n - 1)]);
         const T acc = cs.wave_total();
-        if (lane == 0) store_out(y, r, acc, alpha, beta, mode);
+        if (lane == 0) store_out(y, r, row_scaled(acc, rscale, r), alpha, beta, mode);
     }
 }
 
@@ -316,10 +329,11 @@ constexpr int SPMM_MAX_LONG = SPMV_CAP / (SPMM_LONG + 1) + 1;
 constexpr int SPMM_RP = 1024;  // row offsets staged in LDS per workgroup
 
 // Dense operand read as BT[col*ldbt + kk] (row-major).  LPR lanes per row.
-template <typename T, typename I, typename CI, int LPR>
+template <typename T, typename I, typename CI, int LPR, typename V = T>
 __global__ __launch_bounds__(WG) void k_csr_spmm(
     int64_t m, int64_t n, int64_t k, int64_t nnz, const I* __restrict__ rowptr,
-    const I* __restrict__ colval, const T* __restrict__ nzval, const T* __restrict__ BT,
+    const I* __restrict__ colval, const V* __restrict__ nzval, const float* __restrict__ rscale,
+    const T* __restrict__ BT,
     int64_t ldbt, T* __restrict__ C, int64_t ldc, T alpha, T beta, int mode,
     const int64_t* __restrict__ blk_row, const int64_t* __restrict__ blk_pos) {
     __shared__ CI s_col[SPMV_CAP];
@@ -355,7 +369,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             int i = base + u * WG + tid;
             i = i < cnt ? i : cnt - 1;
             ci[u] = ldnt(colval + P0 + i);
-            vv[u] = ldnt(nzval + P0 + i);
+            vv[u] = widen<T>(ldnt(nzval + P0 + i));
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -411,7 +425,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                 for (int u = 0; u < 8; ++u)
                     if (j + u < e) acc = acc + av[u] * bv[u];
             }
-            store_out(C, r + kk * ldc, acc, alpha, beta, mode);
+            store_out(C, r + kk * ldc, row_scaled(acc, rscale, r), alpha, beta, mode);
         }
     }
     __syncthreads();
@@ -445,7 +459,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             if (g == 0 && kk < k) {
                 T sum = red[lane];
                 for (int qq = 1; qq < GROUPS; ++qq) sum = sum + red[qq * LPR + lane];
-                store_out(C, r + kk * ldc, sum, alpha, beta, mode);
+                store_out(C, r + kk * ldc, row_scaled(sum, rscale, r), alpha, beta, mode);
             }
             __syncthreads();
         }
@@ -461,7 +475,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                 __syncthreads();  // previous users of s_col/s_val are done
                 for (int i = tid; i < cn; i += WG) {
                     s_col[i] = (CI)clamp64((int64_t)ldnt(colval + c0 + i) - 1, 0, n - 1);
-                    s_val[i] = ldnt(nzval + c0 + i);
+                    s_val[i] = widen<T>(ldnt(nzval + c0 + i));
                 }
                 __syncthreads();
                 if (kk < k) {
@@ -487,7 +501,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             if (g == 0 && kk < k) {
                 T sum = red[lane];
                 for (int q = 1; q < GROUPS; ++q) sum = sum + red[q * LPR + lane];
-                store_out(C, r + kk * ldc, sum, alpha, beta, mode);
+                store_out(C, r + kk * ldc, row_scaled(sum, rscale, r), alpha, beta, mode);
             }
         }
     }
@@ -646,11 +660,11 @@ struct PlanUse {
 };
 
 // Gather-shaped product on CSR arrays: C (rows m) = β C + α A B, B has n rows.
-template <typename T, typename I>
+template <typename T, typename I, typename V = T>
 static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* rowptr,
-                      const I* colval, const T* nzval, int64_t nnz, const T* B, int64_t ldb,
+                      const I* colval, const V* nzval, int64_t nnz, const T* B, int64_t ldb,
                       int opB, T* C, int64_t ldc, double alpha, double beta, gsa_plan plan,
-                      int itype) {
+                      int itype, const float* rscale = nullptr) {
     if (m == 0 || k == 0) return GSA_OK;
     if (n == 0 || nnz == 0) {  // A*B contributes nothing: only β applies
         return apply_beta<T>(s, m, k, C, ldc, beta);
@@ -668,9 +682,9 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
                                      hipMemcpyDeviceToDevice, s));
             x = xs.as<T>();
         }
-        hipLaunchKernelGGL((k_csr_spmv<T, I>), dim3((unsigned)cdiv(pu.nwin, WAVES)), dim3(64 * WAVES),
-                           0, s, m, n, nnz, rowptr, colval, nzval, x, C, (T)alpha, (T)beta, mode,
-                           pu.win_row, pu.win_pos, pu.nwin);
+        hipLaunchKernelGGL((k_csr_spmv<T, I, V>), dim3((unsigned)cdiv(pu.nwin, WAVES)), dim3(64 * WAVES),
+                           0, s, m, n, nnz, rowptr, colval, nzval, rscale, x, C,
+                           (T)alpha, (T)beta, mode, pu.win_row, pu.win_pos, pu.nwin);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
     }
@@ -688,13 +702,13 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
 #define GSA_SPMM_LAUNCH(L)                                                                  \
     do {                                                                                    \
         if (n <= INT32_MAX)                                                                 \
-            hipLaunchKernelGGL((k_csr_spmm<T, I, int32_t, L>), dim3(grid), dim3(WG), 0, s, m, n, k, \
-                               nnz, rowptr, colval, nzval, BT, ldbt, C, ldc, (T)alpha, (T)beta, \
-                               mode, pu.blk_row, pu.blk_pos);                               \
+            hipLaunchKernelGGL((k_csr_spmm<T, I, int32_t, L, V>), dim3(grid), dim3(WG), 0, s, m, n, k, \
+                               nnz, rowptr, colval, nzval, rscale, BT, ldbt, C, ldc,        \
+                               (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos);            \
         else                                                                                \
-            hipLaunchKernelGGL((k_csr_spmm<T, I, I, L>), dim3(grid), dim3(WG), 0, s, m, n, k, nnz, \
-                               rowptr, colval, nzval, BT, ldbt, C, ldc, (T)alpha, (T)beta, mode, \
-                               pu.blk_row, pu.blk_pos);                                     \
+            hipLaunchKernelGGL((k_csr_spmm<T, I, I, L, V>), dim3(grid), dim3(WG), 0, s, m, n, k, nnz, \
+                               rowptr, colval, nzval, rscale, BT, ldbt, C, ldc,             \
+                               (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos);            \
     } while (0)
     if (k <= 2) GSA_SPMM_LAUNCH(2);
     else if (k <= 4) GSA_SPMM_LAUNCH(4);
@@ -832,6 +846,22 @@ int gsa_spmm_csc(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
         return gather_csr<T, I>(s, n, m, k, (const I*)colptr, (const I*)rowval, (const T*)nzval,
                                 nnz, (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
                                 plan, itype);
+    });
+}
+
+int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
+                      const void* rowptr, const void* colval, const uint16_t* nzval_bf16,
+                      const float* row_scale, int64_t nnz, const float* B, int64_t ldb, int opB,
+                      float* C, int64_t ldc, float alpha, float beta, gsa_plan plan) {
+    GSA_TRY(ensure_init());
+    GSA_TRY(check_common(GSA_OP_N, opB, m, n, k, nnz, ldb, ldc));
+    GSA_CHECK(m == 0 || row_scale, GSA_ERR_INVALID, "gsa_spmm_csr_bf16: row_scale is null");
+    hipStream_t s = as_stream(stream);
+    return dispatch_i(itype, [&](auto ti) {
+        using I = decltype(ti);
+        return gather_csr<float, I, uint16_t>(s, m, n, k, (const I*)rowptr, (const I*)colval,
+                                              nzval_bf16, nnz, B, ldb, opB != GSA_OP_N, C, ldc,
+                                              alpha, beta, plan, itype, row_scale);
     });
 }
 

@@ -22,6 +22,7 @@ GSA_ERR_UNSUPPORTED = -5
 GSA_F64, GSA_F32, GSA_BF16 = 0, 1, 2
 GSA_I32, GSA_I64 = 0, 1
 GSA_OP_N, GSA_OP_T, GSA_OP_C = 0, 1, 2
+GSA_BATCH_B_ROWMAJOR, GSA_BATCH_C_ROWMAJOR = 1, 2
 
 
 class ArgumentError(ValueError):
@@ -50,7 +51,7 @@ class BatchDesc(ctypes.Structure):
         ("rowptr", _vp), ("colval", _vp), ("nzval", _vp), ("row_scale", _vp),
         ("B", _vp), ("C", _vp),
         ("m", ctypes.c_int32), ("n", ctypes.c_int32), ("k", ctypes.c_int32),
-        ("ldb", ctypes.c_int32), ("ldc", ctypes.c_int32), ("pad", ctypes.c_int32),
+        ("ldb", ctypes.c_int32), ("ldc", ctypes.c_int32), ("flags", ctypes.c_int32),
     ]
 
 
@@ -74,7 +75,7 @@ _SIGS = {
     "gsa_spmm_coo": [_vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,
                      _vp, _i64, _i32, _vp, _i64, _f64, _f64],
     "gsa_spmm_csr_bf16": [_vp, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64,
-                          _vp, _i64, _vp, _i64, _f32, _f32, _vp],
+                          _vp, _i64, _i32, _vp, _i64, _f32, _f32, _vp],
     "gsa_csr_to_coo": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "gsa_coo_to_csr": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "gsa_csr_to_csc": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],

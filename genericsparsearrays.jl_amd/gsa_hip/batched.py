@@ -6,7 +6,9 @@
     g.replay()
 
 Each A is a GenericSparseMatrixCSR (Float32 or Float64) or a BF16CSR; B and C
-are column-major (Julia layout) device tensors, vectors or matrices.  All
+are device vectors or matrices, column-major (Julia's layout) or row-major
+(Julia's transpose(Bt)); row-major B keeps the k values one stored entry needs
+in one cache line.  All
 instances of one batch share the value type and the index type.  This
 replaces a loop of mul!(C_i, A_i, B_i, α, β) calls
 (src/matrix_csr/matrix_csr.jl:145-211) with one launch.
@@ -33,12 +35,17 @@ def _vtype(A) -> int:
     raise ArgumentError(f"unsupported value type {A.nzval.dtype}")
 
 
-def _ld(t: torch.Tensor, rows: int) -> int:
+def _layout(t: torch.Tensor, rows: int, k: int):
+    """(ld, rowmajor) of a dense vector or matrix operand."""
     if t.dim() == 1:
-        return max(rows, 1)
-    if t.stride(0) != 1:
-        raise ArgumentError("dense operands of a batch must be column-major")
-    return max(t.stride(1), rows, 1)
+        if t.stride(0) != 1:
+            raise ArgumentError("vector operands must be contiguous")
+        return max(rows, 1), False
+    if t.stride(0) == 1 and (k == 1 or t.stride(1) >= max(rows, 1)):
+        return max(t.stride(1), rows, 1), False
+    if t.stride(1) == 1 and (rows <= 1 or t.stride(0) >= k):
+        return max(t.stride(0), k, 1), True
+    raise ArgumentError("dense operands of a batch must be column- or row-major")
 
 
 def describe(items):
@@ -68,7 +75,10 @@ def describe(items):
         d.row_scale = A.row_scale.data_ptr() if isinstance(A, BF16CSR) else None
         d.B, d.C = B.data_ptr(), C.data_ptr()
         d.m, d.n, d.k = m, n, k
-        d.ldb, d.ldc = _ld(B, n), _ld(C, m)
+        ldb, brow = _layout(B, n, k)
+        ldc, crow = _layout(C, m, k)
+        d.ldb, d.ldc = ldb, ldc
+        d.flags = (L.GSA_BATCH_B_ROWMAJOR if brow else 0) | (L.GSA_BATCH_C_ROWMAJOR if crow else 0)
     return vt, it, arr
 
 

@@ -17,7 +17,9 @@ import torch
 
 from . import _lib as L
 from ._lib import ArgumentError, DimensionMismatch
-from .matrix import GenericSparseMatrixCSR, itype_of, vtype_of, ptr, stream_of, require_device
+from .matrix import (GenericSparseMatrixCSR, itype_of, vtype_of, ptr, stream_of, require_device,
+                     _PlanCache)
+from .ops import _dense_operand
 
 
 class BF16CSR:
@@ -33,6 +35,10 @@ class BF16CSR:
         self.row_scale = torch.empty(A.m, dtype=torch.float32, device=dev)
         L.call("gsa_csr_to_bf16", stream_of(dev), vtype_of(A.nzval), itype_of(A.rowptr), A.m,
                ptr(A.rowptr), ptr(A.nzval), ptr(self.nzval), ptr(self.row_scale))
+        self._plan = _PlanCache()
+
+    def plan(self):
+        return self._plan.get(self.rowptr, self.m, self.nnz())
 
     @property
     def shape(self):
@@ -54,7 +60,8 @@ class BF16CSR:
 
 
 def mul_bf16(C: torch.Tensor, A: BF16CSR, B: torch.Tensor, alpha=1.0, beta=0.0):
-    """C = beta*C + alpha * A_bf16 * B with Float32 B and C (column-major 2-D or vectors)."""
+    """C = beta*C + alpha * A_bf16 * B with Float32 B and C.  B may be a vector,
+    column-major or row-major (Julia's transpose(Bt)); C a vector or column-major."""
     if B.dtype != torch.float32 or C.dtype != torch.float32:
         raise ArgumentError("BF16 products take Float32 dense operands")
     k = 1 if B.dim() == 1 else B.shape[1]
@@ -64,13 +71,11 @@ def mul_bf16(C: torch.Tensor, A: BF16CSR, B: torch.Tensor, alpha=1.0, beta=0.0):
     if C.shape[0] != A.m or (1 if C.dim() == 1 else C.shape[1]) != k:
         raise DimensionMismatch("C does not match A*B")
     dev = require_device(B, C, A.nzval)
-    if B.dim() == 2 and B.stride(0) != 1:
-        B = B.t().contiguous().t()
-    if C.dim() == 2 and C.stride(0) != 1:
+    Bt, k, ldb, opB = _dense_operand(B)
+    if C.dim() == 2 and not (C.stride(0) == 1 and (k == 1 or C.stride(1) >= max(A.m, 1))):
         raise ArgumentError("C must be column-major")
-    ldb = A.n if B.dim() == 1 else max(B.stride(1), A.n)
-    ldc = A.m if C.dim() == 1 else max(C.stride(1), A.m)
+    ldc = max(A.m, 1) if C.dim() == 1 else max(C.stride(1), A.m, 1)
     L.call("gsa_spmm_csr_bf16", stream_of(dev), itype_of(A.rowptr), A.m, A.n, k, ptr(A.rowptr),
-           ptr(A.colval), ptr(A.nzval), ptr(A.row_scale), A.nnz(), ptr(B), ldb, ptr(C), ldc,
-           float(alpha), float(beta), None)
+           ptr(A.colval), ptr(A.nzval), ptr(A.row_scale), A.nnz(), ptr(Bt), ldb, opB, ptr(C), ldc,
+           float(alpha), float(beta), A.plan() if A.nnz() > 0 else None)
     return C

@@ -5,7 +5,9 @@
     srv = Server(reg, hbm_budget=2 << 30, slots=4)
     stats = srv.run(requests)                # requests: arrays of arrival time, matrix, k
 
-Every launch takes all requests that have arrived and fit the HBM budget
+Inputs and outputs are row-major blocks (n x k and m x k, Julia's
+transpose of a k x n matrix): the k values one stored entry needs sit in one
+cache line.  Every launch takes all requests that have arrived and fit the HBM budget
 (their inputs and outputs), up to `max_group`, and runs them as one
 gsa_batch_spmm_tables launch on one of `slots` streams; requests that arrive
 while launches are on the device wait for the next free slot, so work from
@@ -19,6 +21,7 @@ order, on one stream.
 from __future__ import annotations
 
 import ctypes
+import gc
 import time
 from collections import deque
 from dataclasses import dataclass
@@ -34,7 +37,7 @@ from .ops import mul_
 
 DESC_DTYPE = np.dtype([("rowptr", "u8"), ("colval", "u8"), ("nzval", "u8"), ("row_scale", "u8"),
                        ("B", "u8"), ("C", "u8"), ("m", "i4"), ("n", "i4"), ("k", "i4"),
-                       ("ldb", "i4"), ("ldc", "i4"), ("pad", "i4")])
+                       ("ldb", "i4"), ("ldc", "i4"), ("flags", "i4")])
 assert DESC_DTYPE.itemsize == 72  # sizeof(gsa_batch_desc)
 
 
@@ -48,10 +51,13 @@ def _lpr(k: int) -> int:
 class MatrixRegistry:
     """Matrices kept resident in HBM for serving, with their per-k row tiles."""
 
-    def __init__(self, device="cuda", rows_per_tile: int = 512, max_k: int = 16):
+    def __init__(self, device="cuda", rows_per_tile: int = 512, max_k: int = 16,
+                 big_rows: int = 100_000):
         self.device = torch.device(device)
         self.rows_per_tile = rows_per_tile
         self.max_k = max_k
+        self.big_rows = big_rows   # matrices with at least this many rows use the dedicated kernels
+        self.big = []              # per matrix: (plan handle, contiguous column 0) or None
         self.mats = []
         self.inputs = []     # per matrix: n x max_k column-major Float32 input block
         self.vtype = None
@@ -60,6 +66,8 @@ class MatrixRegistry:
         self._tiles = {}     # (mid, lpr) -> int32 [T, 2]
 
     def register(self, A, x_block: torch.Tensor | None = None) -> int:
+        """Keep A resident; x_block is its row-major n x max_k Float32 input block
+        (a request with k right-hand sides reads its first k columns)."""
         vt = L.GSA_BF16 if isinstance(A, BF16CSR) else (
             L.GSA_F32 if A.nzval.dtype == torch.float32 else None)
         if vt is None:
@@ -71,20 +79,24 @@ class MatrixRegistry:
             raise ArgumentError("one registry serves one value/index type")
         m, n = A.shape
         if x_block is None:
-            x_block = torch.zeros((self.max_k, n), dtype=torch.float32, device=self.device).t()
-        if (tuple(x_block.shape) != (n, self.max_k) or x_block.stride(0) != 1
-                or (self.max_k > 1 and x_block.stride(1) != max(n, 1))
+            x_block = torch.zeros((n, self.max_k), dtype=torch.float32, device=self.device)
+        if (tuple(x_block.shape) != (n, self.max_k) or not x_block.is_contiguous()
                 or x_block.dtype != torch.float32):
-            raise ArgumentError("x_block must be a dense column-major n x max_k Float32 block")
+            raise ArgumentError("x_block must be a contiguous row-major n x max_k Float32 block")
         mid = len(self.mats)
         self.mats.append(A)
         self.inputs.append(x_block)
+        if m >= self.big_rows:
+            self.big.append((A.plan() if A.nnz() > 0 else None, x_block[:, 0].contiguous()))
+        else:
+            self.big.append(None)
         d = np.zeros(1, DESC_DTYPE)
         d["rowptr"], d["colval"], d["nzval"] = (A.rowptr.data_ptr(), A.colval.data_ptr(),
                                                 A.nzval.data_ptr())
         d["row_scale"] = A.row_scale.data_ptr() if isinstance(A, BF16CSR) else 0
         d["B"] = x_block.data_ptr()
-        d["m"], d["n"], d["ldb"], d["ldc"] = m, n, max(n, 1), max(m, 1)
+        d["m"], d["n"], d["ldb"], d["ldc"] = m, n, self.max_k, 1
+        d["flags"] = L.GSA_BATCH_B_ROWMAJOR | L.GSA_BATCH_C_ROWMAJOR
         self._desc.append(d[0])
         return mid
 
@@ -121,13 +133,17 @@ class Requests:
 
 
 class _Slot:
-    def __init__(self, device, max_group, max_tiles):
+    def __init__(self, device, max_group, max_tiles, arena_bytes):
         self.stream = torch.cuda.Stream(device=device)
+        # outputs of one launch are carved from this arena: no allocation in
+        # the serving loop (an allocation can synchronise the device)
+        self.arena = torch.empty(max(arena_bytes // 4, 1), dtype=torch.float32, device=device)
         self.h_desc = torch.empty(max_group * DESC_DTYPE.itemsize, dtype=torch.uint8, pin_memory=True)
         self.h_tiles = torch.empty(max_tiles * 3, dtype=torch.int32, pin_memory=True)
         self.d_desc = torch.empty(max_group * DESC_DTYPE.itemsize, dtype=torch.uint8, device=device)
         self.d_tiles = torch.empty(max_tiles * 3, dtype=torch.int32, device=device)
         self.d_counter = torch.zeros(1, dtype=torch.int32, device=device)
+        self.done = torch.cuda.Event(enable_timing=True)   # reused for every launch
         self.event = None
         self.req = None
         self.bytes = 0
@@ -136,80 +152,142 @@ class _Slot:
 
 class Server:
     def __init__(self, registry: MatrixRegistry, hbm_budget: int = 2 << 30, slots: int = 4,
-                 max_group: int = 2048, max_tiles: int = 1 << 20, grid: int = 0):
+                 max_group: int = 4096, max_tiles: int = 1 << 21, grid: int = 0):
+        """hbm_budget: bytes of result buffers in flight, split evenly over the
+        slots (each launch's results must fit its slot's share)."""
         self.reg = registry
         self.budget = int(hbm_budget)
         self.max_group = max_group
         self.max_tiles = max_tiles
         self.grid = grid
         dev = registry.device
-        self.slots = [_Slot(dev, max_group, max_tiles) for _ in range(slots)]
+        self.slot_bytes = self.budget // slots
+        self.slots = [_Slot(dev, max_group, max_tiles, self.slot_bytes) for _ in range(slots)]
         self.desc = registry.desc_table()
         m = self.desc["m"].astype(np.int64)
         n = self.desc["n"].astype(np.int64)
         self._m, self._n = m, n
         self.in_ptr = self.desc["B"].copy()
+        self.is_big = np.array([b is not None for b in registry.big], dtype=bool)
 
     def request_bytes(self, mids, ks):
-        return (self._m[mids] + self._n[mids]) * ks * 4
+        """HBM bytes a request holds while in flight: its m x k Float32 result."""
+        return self._m[mids] * ks * 4
 
     def _launch(self, slot: _Slot, idx: np.ndarray, mids, ks, out_list):
+        """One group: the small-matrix requests in one batched launch, then each
+        big-matrix request through the dedicated SpMV/SpMM kernels, all on the
+        slot's stream, followed by the slot's completion event."""
         reg = self.reg
         ms = self._m[mids]
         sizes = ms * ks
         offs = np.concatenate([[0], np.cumsum(sizes)])
-        out = torch.empty(int(offs[-1]) if offs[-1] > 0 else 1, dtype=torch.float32,
-                          device=reg.device)
-        d = self.desc[mids].copy()
-        d["B"] = self.in_ptr[mids]
-        d["C"] = np.uint64(out.data_ptr()) + (offs[:-1] * 4).astype(np.uint64)
-        d["k"] = ks
-        tl, inst = [], []
-        for j, (mid, k) in enumerate(zip(mids.tolist(), ks.tolist())):
-            t = reg.tiles(mid, k)
-            tl.append(t)
-            inst.append(np.full(t.shape[0], j, dtype=np.int32))
-        tiles = np.concatenate(tl)
-        ntiles = tiles.shape[0]
-        if ntiles > self.max_tiles:
-            raise ArgumentError("group exceeds the tile table; raise max_tiles")
-        flat = np.empty((ntiles, 3), dtype=np.int32)
-        flat[:, 0] = np.concatenate(inst)
-        flat[:, 1:] = tiles
-        nb = d.nbytes
-        slot.h_desc[:nb].numpy()[:] = d.view(np.uint8)
-        slot.h_tiles[:ntiles * 3].numpy()[:] = flat.ravel()
+        if offs[-1] > slot.arena.numel():
+            raise ArgumentError("a request's result is larger than a slot's share of the budget")
+        out = slot.arena
+        isbig = self.is_big[mids]
+        sm = np.flatnonzero(~isbig)
+        st = ctypes.c_void_p(slot.stream.cuda_stream)
         with torch.cuda.stream(slot.stream):
-            slot.d_desc[:nb].copy_(slot.h_desc[:nb], non_blocking=True)
-            slot.d_tiles[:ntiles * 3].copy_(slot.h_tiles[:ntiles * 3], non_blocking=True)
-            L.call("gsa_batch_spmm_tables", ctypes.c_void_p(slot.stream.cuda_stream), reg.vtype,
-                   reg.itype, ctypes.c_void_p(slot.d_desc.data_ptr()),
-                   ctypes.c_void_p(slot.d_tiles.data_ptr()), ntiles,
-                   ctypes.c_void_p(slot.d_counter.data_ptr()), 1.0, 0.0, self.grid)
-            ev = torch.cuda.Event(enable_timing=True)
-            ev.record(slot.stream)
-        slot.event, slot.req, slot.out = ev, idx, out
+            if sm.size:
+                smids, sks, sms = mids[sm], ks[sm], ms[sm]
+                d = self.desc[smids].copy()
+                d["B"] = self.in_ptr[smids]
+                d["C"] = np.uint64(out.data_ptr()) + (offs[sm] * 4).astype(np.uint64)
+                d["k"] = sks
+                d["ldc"] = np.maximum(sks, 1)
+                # tiles of every request, vectorised: request j has T_j tiles of
+                # rows_j = max(16, min(1024, rows_per_tile / lpr(k_j))) rows
+                lpr = np.ones_like(sks)
+                while True:
+                    grow = (lpr < sks) & (lpr < 64)
+                    if not grow.any():
+                        break
+                    lpr[grow] <<= 1
+                rows = np.clip(reg.rows_per_tile // lpr, 16, 1024)
+                T = (sms + rows - 1) // rows
+                ntiles = int(T.sum())
+                if ntiles > self.max_tiles:
+                    raise ArgumentError("group exceeds the tile table; raise max_tiles")
+                inst = np.repeat(np.arange(sm.size, dtype=np.int64), T)
+                first = np.repeat(np.cumsum(T) - T, T)
+                r0 = (np.arange(ntiles, dtype=np.int64) - first) * rows[inst]
+                flat = np.empty((ntiles, 3), dtype=np.int32)
+                flat[:, 0] = inst
+                flat[:, 1] = r0
+                flat[:, 2] = np.minimum(r0 + rows[inst], sms[inst])
+                nb = d.nbytes
+                slot.h_desc[:nb].numpy()[:] = d.view(np.uint8)
+                slot.h_tiles[:ntiles * 3].numpy()[:] = flat.ravel()
+                slot.d_desc[:nb].copy_(slot.h_desc[:nb], non_blocking=True)
+                slot.d_tiles[:ntiles * 3].copy_(slot.h_tiles[:ntiles * 3], non_blocking=True)
+                L.call("gsa_batch_spmm_tables", st, reg.vtype, reg.itype,
+                       ctypes.c_void_p(slot.d_desc.data_ptr()),
+                       ctypes.c_void_p(slot.d_tiles.data_ptr()), ntiles,
+                       ctypes.c_void_p(slot.d_counter.data_ptr()), 1.0, 0.0, self.grid)
+            for j in np.flatnonzero(isbig).tolist():
+                mid, k, m = int(mids[j]), int(ks[j]), int(ms[j])
+                A = reg.mats[mid]
+                plan, x0 = reg.big[mid]
+                X = reg.inputs[mid]
+                if k == 1:
+                    bptr, ldb, opB = x0.data_ptr(), max(A.n, 1), L.GSA_OP_N
+                else:
+                    bptr, ldb, opB = X.data_ptr(), reg.max_k, L.GSA_OP_T
+                cptr = ctypes.c_void_p(out.data_ptr() + int(offs[j]) * 4)
+                if isinstance(A, BF16CSR):
+                    L.call("gsa_spmm_csr_bf16", st, reg.itype, m, A.n, k,
+                           ctypes.c_void_p(A.rowptr.data_ptr()), ctypes.c_void_p(A.colval.data_ptr()),
+                           ctypes.c_void_p(A.nzval.data_ptr()),
+                           ctypes.c_void_p(A.row_scale.data_ptr()), A.nnz(), ctypes.c_void_p(bptr),
+                           ldb, opB, cptr, max(m, 1), 1.0, 0.0, plan)
+                else:
+                    L.call("gsa_spmm_csr", st, L.GSA_F32, reg.itype, L.GSA_OP_N, m, A.n, k,
+                           ctypes.c_void_p(A.rowptr.data_ptr()), ctypes.c_void_p(A.colval.data_ptr()),
+                           ctypes.c_void_p(A.nzval.data_ptr()), A.nnz(), ctypes.c_void_p(bptr),
+                           ldb, opB, cptr, max(m, 1), 1.0, 0.0, plan)
+            slot.done.record(slot.stream)
+        slot.event, slot.req, slot.out = slot.done, idx, out
+        slot.used_words = int(offs[-1])
         if out_list is not None:
             for j, i in enumerate(idx.tolist()):
-                out_list[i] = (out, int(offs[j]), int(ms[j]), int(ks[j]))
+                out_list[i] = (out, int(offs[j]), int(ms[j]), int(ks[j]),
+                               "col" if isbig[j] else "row")
 
     def run(self, req: Requests, keep_outputs: bool = False, spin_sleep: float = 0.0):
-        """Serve `req` in real time; returns per-request latency (s) and summary stats."""
+        """Serve `req` in real time; returns per-request latency (s) and summary stats.
+
+        Latency of a request = completion of its launch on the GPU clock (an
+        event recorded after the launch) minus its arrival time."""
         N = len(req)
-        done_ms = np.full(N, np.nan)
-        out_list = [None] * N if keep_outputs else None
-        pend = deque()
-        free = list(range(len(self.slots)))
-        busy = {}
-        used = 0
-        rbytes = self.request_bytes(req.mid, req.k)
-        nxt = 0
-        ndone = 0
-        launches = 0
-        group_sizes = []
+        st = {"done_ms": np.full(N, np.nan), "out": [None] * N if keep_outputs else None,
+              "launches": 0, "groups": []}
         torch.cuda.synchronize()
         start = torch.cuda.Event(enable_timing=True)
         start.record(self.slots[0].stream)
+        gc_was = gc.isenabled()
+        gc.disable()  # no collector pauses inside the real-time loop
+        try:
+            self._loop(req, start, st, keep_outputs, spin_sleep)
+        finally:
+            if gc_was:
+                gc.enable()
+        torch.cuda.synchronize()
+        lat = st["done_ms"] / 1e3 - req.arrival
+        span = np.nanmax(st["done_ms"]) / 1e3
+        return {"latency_s": lat, "p50_ms": float(np.percentile(lat, 50) * 1e3),
+                "p99_ms": float(np.percentile(lat, 99) * 1e3), "req_per_s": N / span,
+                "launches": st["launches"], "mean_group": float(np.mean(st["groups"])),
+                "outputs": st["out"]}
+
+    def _loop(self, req: Requests, start, st, keep_outputs, spin_sleep):
+        N = len(req)
+        done_ms, out_list = st["done_ms"], st["out"]
+        rbytes = self.request_bytes(req.mid, req.k)
+        pend = deque()
+        free = list(range(len(self.slots)))
+        busy = set()
+        nxt = ndone = 0
         t0 = time.perf_counter()
         while ndone < N:
             now = time.perf_counter() - t0
@@ -220,13 +298,16 @@ class Server:
             for si in list(busy):
                 s = self.slots[si]
                 if s.event.query():
-                    ms = start.elapsed_time(s.event)
-                    done_ms[s.req] = ms
+                    done_ms[s.req] = start.elapsed_time(s.event)
                     ndone += s.req.size
-                    used -= s.bytes
-                    if not keep_outputs:
-                        s.out = None
-                    del busy[si]
+                    if keep_outputs:  # the arena is reused: keep a copy of the results,
+                        # taken on the slot's stream so the next launch there waits for it
+                        with torch.cuda.stream(s.stream):
+                            copy = s.arena[:s.used_words].clone()
+                        for i in s.req.tolist():
+                            out_list[i] = (copy,) + out_list[i][1:]
+                    s.out = None
+                    busy.discard(si)
                     free.append(si)
                     progressed = True
             while pend and free:
@@ -234,36 +315,28 @@ class Server:
                 b = 0
                 while pend and len(take) < self.max_group:
                     i = pend[0]
-                    if used + b + rbytes[i] > self.budget and (take or used > 0):
-                        break  # admission by HBM bytes
+                    if b + rbytes[i] > self.slot_bytes:
+                        break  # admission by HBM bytes (the slot's share of the budget)
                     take.append(pend.popleft())
                     b += int(rbytes[i])
                 if not take:
-                    break
+                    raise ArgumentError("a request's result is larger than a slot's share "
+                                        "of the HBM budget")
                 si = free.pop()
-                s = self.slots[si]
                 idx = np.array(take, dtype=np.int64)
-                s.bytes = b
-                used += b
-                self._launch(s, idx, req.mid[idx], req.k[idx], out_list)
-                busy[si] = True
-                launches += 1
-                group_sizes.append(len(take))
+                self._launch(self.slots[si], idx, req.mid[idx], req.k[idx], out_list)
+                busy.add(si)
+                st["launches"] += 1
+                st["groups"].append(len(take))
                 progressed = True
-            if not progressed:
-                if spin_sleep:
-                    time.sleep(spin_sleep)
-        torch.cuda.synchronize()
-        lat = done_ms / 1e3 - req.arrival
-        span = np.nanmax(done_ms) / 1e3 - 0.0
-        return {"latency_s": lat, "p50_ms": float(np.percentile(lat, 50) * 1e3),
-                "p99_ms": float(np.percentile(lat, 99) * 1e3), "req_per_s": N / span,
-                "launches": launches, "mean_group": float(np.mean(group_sizes)),
-                "outputs": out_list}
+            if not progressed and spin_sleep:
+                time.sleep(spin_sleep)
 
 
 class PlainServer:
-    """One drop-in mul! per request, in arrival order, on the current stream."""
+    """One drop-in mul! per request, in arrival order, on the current stream.
+    It reads the same row-major input blocks (as Julia's transpose(Xt), which
+    mul! takes without a copy) and writes column-major results."""
 
     def __init__(self, registry: MatrixRegistry):
         self.reg = registry
@@ -273,17 +346,20 @@ class PlainServer:
         N = len(req)
         outs = [None] * N if keep_outputs else None
         evs = [None] * N
+        x0 = [X[:, 0].contiguous() for X in reg.inputs]  # k == 1: a contiguous vector
         torch.cuda.synchronize()
         stream = torch.cuda.current_stream(reg.device)
         start = torch.cuda.Event(enable_timing=True)
         start.record(stream)
+        gc_was = gc.isenabled()
+        gc.disable()
         t0 = time.perf_counter()
         for i in range(N):
             while time.perf_counter() - t0 < req.arrival[i]:
                 pass
             mid, k = int(req.mid[i]), int(req.k[i])
             A = reg.mats[mid]
-            X = reg.inputs[mid][:, :k] if k > 1 else reg.inputs[mid][:, 0]
+            X = reg.inputs[mid][:, :k] if k > 1 else x0[mid]
             m = A.shape[0]
             C = torch.empty((k, m), dtype=torch.float32, device=reg.device).t() if k > 1 else \
                 torch.empty(m, dtype=torch.float32, device=reg.device)
@@ -296,6 +372,8 @@ class PlainServer:
             evs[i] = ev
             if keep_outputs:
                 outs[i] = C
+        if gc_was:
+            gc.enable()
         torch.cuda.synchronize()
         done_ms = np.array([start.elapsed_time(e) for e in evs])
         lat = done_ms / 1e3 - req.arrival
@@ -305,6 +383,8 @@ class PlainServer:
 
 
 def output_view(entry):
-    """The (m x k) column-major result of a served request."""
-    out, off, m, k = entry
-    return out[off:off + m * k].view(k, m).t()
+    """The (m x k) result of a served request (row-major from the batched
+    launch, column-major from the dedicated kernels)."""
+    out, off, m, k, layout = entry
+    seg = out[off:off + m * k]
+    return seg.view(m, k) if layout == "row" else seg.view(k, m).t()

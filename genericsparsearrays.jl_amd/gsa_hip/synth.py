@@ -152,3 +152,53 @@ def uneven_batch(count: int, seed: int, lo: int = 64, hi: int = 65536, mean_extr
         mats.append(csr_from_lengths(lengths, int(s), rng, "normal", dtype))
         kk.append(int(rng.choice(ks)))
     return mats, kk
+
+
+def uneven_batch_device(count: int, seed: int, device="cuda", lo: int = 64, hi: int = 65536,
+                        mean_extra: float = 8.0, ks=(1, 2, 4, 8, 16), itype=None, vdtype=None):
+    """The uneven batch built directly in device memory (for batches too large to
+    build on the host quickly): same size/row-length/k laws as `uneven_batch`,
+    distinct uniform columns per row, N(0,1) values.  Returns a list of
+    (m, rowptr, colval, nzval) device tensors and the list of k.  The instances
+    share one column/value buffer (views)."""
+    import torch
+    itype = itype or torch.int32
+    vdtype = vdtype or torch.float32
+    rng = np.random.default_rng(seed)
+    sizes = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi), size=count))).astype(np.int64)
+    kk = [int(rng.choice(ks)) for _ in range(count)]
+    g = torch.Generator(device=device)
+    g.manual_seed(seed)
+    dev = torch.device(device)
+    total_rows = int(sizes.sum())
+    row_n = torch.as_tensor(np.repeat(sizes, sizes), device=dev)          # n of each row's matrix
+    lens = torch.poisson(torch.full((total_rows,), mean_extra, device=dev), generator=g)
+    lens = torch.minimum(lens.to(torch.int64) + 1, row_n)
+    rows = torch.repeat_interleave(torch.arange(total_rows, device=dev), lens)
+    n_of = row_n[rows]
+    cols = (torch.rand(rows.numel(), generator=g, device=dev) * n_of).to(torch.int64)
+    cols = torch.minimum(cols, n_of - 1)
+    while True:  # distinct columns per row: sort (row, col), redraw repeats
+        key = rows * 65536 + cols
+        key, order = torch.sort(key)
+        cols = cols[order]
+        dup = torch.zeros_like(key, dtype=torch.bool)
+        dup[1:] = key[1:] == key[:-1]
+        nd = int(dup.sum())
+        if nd == 0:
+            break
+        redraw = (torch.rand(nd, generator=g, device=dev) * n_of[dup]).to(torch.int64)
+        cols[dup] = torch.minimum(redraw, n_of[dup] - 1)
+    vals = torch.randn(rows.numel(), generator=g, device=dev).to(vdtype)
+    cols1 = (cols + 1).to(itype)
+    gptr = torch.zeros(total_rows + 1, dtype=torch.int64, device=dev)
+    gptr[1:] = torch.cumsum(lens, 0)
+    row_start = np.concatenate([[0], np.cumsum(sizes)])
+    out = []
+    starts = gptr[torch.as_tensor(row_start, device=dev)].cpu().numpy()
+    for i in range(count):
+        r0, r1 = int(row_start[i]), int(row_start[i + 1])
+        p0, p1 = int(starts[i]), int(starts[i + 1])
+        rp = (gptr[r0:r1 + 1] - p0 + 1).to(itype)
+        out.append((int(sizes[i]), rp, cols1[p0:p1], vals[p0:p1]))
+    return out, kk

@@ -113,14 +113,16 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA,
                  void* C, int64_t ldc,
                  double alpha, double beta);
 
-/* BF16-stored CSR SpMV/SpMM with a Float32 scale per row, Float32 x/y and
- * Float32 accumulation (TwoSum-compensated on rows over 256 entries):
- * y = beta*y + alpha * diag(row_scale) * A_bf16 * x.  Serving precision
- * variant; no counterpart in the reference (whose Tv is a Julia number type). */
+/* BF16-stored CSR SpMV/SpMM with a Float32 scale per row, Float32 B/C and
+ * Float32 accumulation (TwoSum-compensated on rows reduced across lanes,
+ * which include every row over 256 entries):
+ * C = beta*C + alpha * diag(row_scale) * A_bf16 * opB(B), same kernels and
+ * layouts as gsa_spmm_csr with opA = N.  Serving precision variant; no
+ * counterpart in the reference (whose Tv is a Julia number type). */
 int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
                       const void* rowptr, const void* colval, const uint16_t* nzval_bf16,
                       const float* row_scale, int64_t nnz,
-                      const float* B, int64_t ldb, float* C, int64_t ldc,
+                      const float* B, int64_t ldb, int opB, float* C, int64_t ldc,
                       float alpha, float beta, gsa_plan plan);
 
 /* ========================== format conversion =============================== */
@@ -240,14 +242,17 @@ int gsa_ctx_destroy(gsa_ctx ctx);
  * No single counterpart in the reference: it replaces a loop of
  * mul!(C_i, A_i, B_i) calls (src/matrix_csr/matrix_csr.jl:145-211). */
 typedef struct {
-    const void* rowptr;   /* m+1 one-based                     */
-    const void* colval;   /* nnz one-based                     */
-    const void* nzval;    /* nnz values (vtype)                */
-    const void* row_scale;/* BF16 only: m scales, else NULL    */
-    const void* B;        /* n x k column-major, ld = ldb      */
-    void*       C;        /* m x k column-major, ld = ldc      */
-    int32_t m, n, k, ldb, ldc, pad;
+    const void* rowptr;   /* m+1 one-based                                  */
+    const void* colval;   /* nnz one-based                                  */
+    const void* nzval;    /* nnz values (vtype)                             */
+    const void* row_scale;/* BF16 only: m scales, else NULL                 */
+    const void* B;        /* n x k, column-major (ld = ldb) unless flagged   */
+    void*       C;        /* m x k, column-major (ld = ldc) unless flagged   */
+    int32_t m, n, k, ldb, ldc;
+    int32_t flags;        /* GSA_BATCH_B_ROWMAJOR | GSA_BATCH_C_ROWMAJOR      */
 } gsa_batch_desc;
+#define GSA_BATCH_B_ROWMAJOR 1  /* B[i,j] at B[i*ldb + j] (Julia: transpose(Bt)) */
+#define GSA_BATCH_C_ROWMAJOR 2  /* C[i,j] at C[i*ldc + j]                        */
 
 /* Tiles: instance i is cut into row tiles of max(16, min(1024,
  * rows_per_tile / next_pow2(k_i))) rows.  vtype GSA_BF16 needs row_scale. */

@@ -41,14 +41,19 @@ def small_batch(count, seed, dtype=np.float32, lo=64, hi=4096, ks=(1, 2, 4, 8, 1
 
 @pytest.mark.parametrize("dtype", [np.float32, np.float64])
 @pytest.mark.parametrize("itype", [torch.int32, torch.int64])
-def test_batched_matches_oracle(dev, dtype, itype):
+@pytest.mark.parametrize("layout", ["colmajor", "rowmajor"])
+def test_batched_matches_oracle(dev, dtype, itype, layout):
     mats, kk, Bs = small_batch(60, 40, dtype=dtype)
     tdt = torch.float32 if dtype == np.float32 else torch.float64
     items = []
     for H, k, B in zip(mats, kk, Bs):
         A = dev_csr(H, dtype=tdt, itype=itype)
-        Bd = G.dense(B, device=dev)
-        Cd = torch.empty((k, H.m), dtype=tdt, device=dev).t()
+        if layout == "colmajor":
+            Bd = G.dense(B, device=dev)
+            Cd = torch.empty((k, H.m), dtype=tdt, device=dev).t()
+        else:
+            Bd = torch.as_tensor(B, device=dev)
+            Cd = torch.empty((H.m, k), dtype=tdt, device=dev)
         items.append((A, Bd, Cd))
     batch = BatchedSpMM(items, rows_per_tile=256)
     batch.run()
@@ -162,7 +167,7 @@ def test_serving_same_alone_and_in_mix(dev, fmt):
             A = BF16CSR(A)
         X = rng.standard_normal((H.n, 16)).astype(np.float32)
         blocks.append(X)
-        reg.register(A, G.dense(X, device=dev))
+        reg.register(A, torch.as_tensor(X, device=dev))
     req = Requests.poisson(400, rate=20_000, n_mats=len(hosts), ks=(1, 4, 16), seed=51)
     srv = Server(reg, hbm_budget=64 << 20, slots=3)
     mixed = srv.run(req, keep_outputs=True)
@@ -195,4 +200,5 @@ def test_serving_same_alone_and_in_mix(dev, fmt):
         H = hosts[int(req.mid[i])]
         a = host(plain["outputs"][i]).reshape(H.m, -1)
         b = host(output_view(mixed["outputs"][i])).reshape(H.m, -1)
+        assert a.shape == b.shape
         np.testing.assert_allclose(a, b, rtol=1e-4, atol=1e-4)
