@@ -25,6 +25,20 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(HERE, "genericsparsearrays.jl_amd"))
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_SUMMARY = os.path.join(HERE, "profiles", "spmv_f64_i64_powerlaw1M_pmc.md")
+
+
+def measured_traffic():
+    """HBM bytes per launch of the SpMV kernel from the committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE scaled by a plain copy), or None."""
+    try:
+        with open(TRAFFIC_SUMMARY) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
 
 
 def spmv_bytes(m, n, nnz, sv, si):
@@ -349,6 +363,8 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the headline SpMV steps (for rocprofv3)")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="also run plain 256 MiB device copies (the scale for PMC byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build runs every kernel on a single "
@@ -402,6 +418,14 @@ def main():
     ms, gbs, lat, ms_eager, nsteps = spmv_run(torch.float64, torch.int64, args.steps, args.warmup)
     print(f"[bench] f64/i64 SpMV {ms * 1e3:.1f} us/step (graph), {ms_eager * 1e3:.1f} us eager, "
           f"{gbs:.0f} GB/s", file=sys.stderr, flush=True)
+    if args.calibrate:
+        src = torch.empty(32 * 1024 * 1024, dtype=torch.float64, device=dev).fill_(1.0)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(f"[bench] calibration copies: 5 x {src.numel() * 8} bytes", file=sys.stderr,
+              flush=True)
     if args.profile_only:
         return
 
@@ -469,7 +493,7 @@ def main():
                    "copies_rotated": args.copies, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
-                     "traffic": None, "algorithmic_bytes": algo,
+                     "traffic": measured_traffic(), "algorithmic_bytes": algo,
                      "kernel": "k_csr_spmv<double,long>"},
         "cpu_baseline": cpu,
         "extra": extra,
