@@ -95,10 +95,12 @@ constexpr int SPMV_CAP = SPMV_NB + 256;  // entries a workgroup stages in LDS
 // (end of the last owned row) and a spill flag (last row read from HBM).
 constexpr int PLAN_POS = 4;
 
-// Wavefront windows of the SpMV kernel: each wavefront owns the rows that
-// start in its SPMV_WNB-entry window and stages up to SPMV_WCAP entries.
-constexpr int SPMV_WNB = 512;
-constexpr int SPMV_WCAP = SPMV_WNB + 64;
+// SpMV windows: wavefront w reduces entries [w*SPMV_WIN, (w+1)*SPMV_WIN) and
+// owns the rows that start there (owner table built like the row blocks).
+constexpr int SPMV_WIN = 576;
+// Entries staged past a window's end, so a short row (at most SPMV_WX
+// entries) that crosses it is finished, left to right, by its owner.
+constexpr int SPMV_WX = 32;
 
 struct CsrPlanData {
     int itype = 0;
@@ -106,9 +108,12 @@ struct CsrPlanData {
     int64_t nblocks = 0;          // workgroup windows (SPMV_NB)
     int64_t* blk_row = nullptr;   // device, nblocks+1
     int64_t* blk_pos = nullptr;   // device, PLAN_POS*nblocks
-    int64_t nwin = 0;             // wavefront windows (SPMV_WNB)
-    int64_t* win_row = nullptr;   // device, nwin+1
-    int64_t* win_pos = nullptr;   // device, PLAN_POS*nwin
+    int64_t nwin = 0;             // SpMV windows (SPMV_WIN entries each)
+    int64_t* win_row = nullptr;   // device, nwin+1: first row starting in each window
+    // Partial sums of the SpMV in flight: the heads and tails of rows that
+    // cross window ends, 2*nwin values of up to 8 bytes.  A plan serves one
+    // SpMV at a time.
+    void* part = nullptr;
     const void* ptr = nullptr;    // the row pointer the plan was built on
 };
 

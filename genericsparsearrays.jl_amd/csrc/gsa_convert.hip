@@ -84,7 +84,7 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
                               const T* val, I* ptr_out, I* idx_out, T* val_out) {
     GSA_CHECK(nnz == 0 || (major_dim > 0 && (minor_key == nullptr || minor_dim > 0)),
               GSA_ERR_INVALID, "stored entries in a matrix with a zero dimension");
-    GSA_HIP(hipMemsetAsync(ptr_out, 0, sizeof(I) * (major_dim + 1), s));
+    GSA_TRY(zero_fill(s, ptr_out, sizeof(I) * (major_dim + 1)));
     if (nnz > 0) {
         const uint64_t maxkey = minor_key
             ? (uint64_t)(major_dim > 0 ? major_dim - 1 : 0) * (uint64_t)(minor_dim > 0 ? minor_dim : 1) +

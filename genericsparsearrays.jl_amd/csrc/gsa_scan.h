@@ -8,6 +8,10 @@ namespace gsa {
 template <typename T>
 int inclusive_scan(hipStream_t s, const T* in, T* out, int64_t n, T add);
 
+// Zero a device buffer with a fill kernel (graph-capture safe, unlike a
+// memset node).
+int zero_fill(hipStream_t s, void* p, size_t bytes);
+
 // ptr[1..m] hold per-row counts on entry; on exit ptr is a one-based pointer
 // array (ptr[0] = 1, ptr[m] = total + 1).  If total != nullptr the stream is
 // synchronised and the total returned.

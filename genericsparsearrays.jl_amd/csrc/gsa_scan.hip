@@ -109,6 +109,31 @@ int inclusive_scan(hipStream_t s, const T* in, T* out, int64_t n, T add) {
 template int inclusive_scan<int64_t>(hipStream_t, const int64_t*, int64_t*, int64_t, int64_t);
 template int inclusive_scan<int32_t>(hipStream_t, const int32_t*, int32_t*, int64_t, int32_t);
 
+// Zero `bytes` bytes with a kernel: the unaligned head bytes, the 8-byte
+// words, then the tail bytes.  An ordinary kernel node under graph capture,
+// unlike a memset node.
+__global__ void k_zero_fill(uint8_t* __restrict__ p, int64_t head, int64_t words, int64_t tail) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    uint64_t* w = reinterpret_cast<uint64_t*>(p + head);
+    for (int64_t i = t; i < words; i += stride) w[i] = 0;
+    if (t < head) p[t] = 0;
+    if (t < tail) p[head + words * 8 + t] = 0;
+}
+
+int zero_fill(hipStream_t s, void* p, size_t bytes) {
+    if (bytes == 0) return GSA_OK;
+    const int64_t n = (int64_t)bytes;
+    int64_t head = (int64_t)((8 - ((uintptr_t)p & 7)) & 7);
+    if (head > n) head = n;
+    const int64_t words = (n - head) / 8;
+    const int64_t tail = (n - head) - words * 8;
+    hipLaunchKernelGGL(k_zero_fill, dim3(grid_for(words > 0 ? words : 1, 256)), dim3(256), 0, s,
+                       reinterpret_cast<uint8_t*>(p), head, words, tail);
+    GSA_HIP(hipGetLastError());
+    return GSA_OK;
+}
+
 template <typename I>
 __global__ void k_set_one(I* p) { *p = 1; }
 

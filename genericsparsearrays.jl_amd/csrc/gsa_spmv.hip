@@ -25,6 +25,11 @@
 // atomics, as the reference's @atomic kernels do (matrix_csr_kernels.jl:24-45,
 // src/matrix_csc/matrix_csc_kernels.jl:1-22, src/matrix_coo/matrix_coo_kernels.jl:1-43).
 #include "gsa_common.h"
+#include "gsa_scan.h"
+
+#include <algorithm>
+#include <cstdlib>
+#include <vector>
 
 namespace gsa {
 
@@ -86,6 +91,11 @@ struct CompSum {
         } else {
             s = s + p;
         }
+    }
+    // this lane's compensated total
+    __device__ __forceinline__ T total() const {
+        if constexpr (sizeof(T) == 4) return s + c;
+        else return s;
     }
     __device__ __forceinline__ T wave_total() {
         if constexpr (sizeof(T) == 4) {
@@ -157,7 +167,7 @@ __global__ void k_csr_plan_pos(int64_t m, int64_t nnz, const I* __restrict__ ptr
 
 int build_csr_plan(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr,
                    int64_t* blk_row, int64_t nblocks, int64_t* blk_pos, int nb, int cap) {
-    GSA_HIP(hipMemsetAsync(blk_row, 0, sizeof(int64_t) * (nblocks + 1), s));
+    GSA_TRY(zero_fill(s, blk_row, sizeof(int64_t) * (nblocks + 1)));
     return dispatch_i(itype, [&](auto ti) {
         using I = decltype(ti);
         unsigned g = (unsigned)cdiv(m + 1, 256);
@@ -194,14 +204,27 @@ __device__ __forceinline__ Window load_window(int64_t b, int64_t m, int64_t nnz,
 }
 
 // ------------------------------------------------------------ SpMV kernel --
-// One wavefront per SPMV_WNB-entry window; the WAVES wavefronts of a
-// workgroup are independent (no workgroup barrier), so while some wait on
-// their x gathers others stream indices and values.
+// Flat windows: wavefront w reduces exactly the entries [w*SPMV_WIN,
+// (w+1)*SPMV_WIN), so every wavefront streams and gathers a full window
+// whatever the row lengths (the gathers of x are the bound; a half-empty
+// wavefront costs nearly as much as a full one).  The wavefronts of a
+// workgroup are independent (no workgroup barrier).
+//
+// Window w owns the rows that start in it.  A row is summed where it lies:
+//   * an owned row that ends inside the window (or a short row, at most
+//     SPMV_WX entries, that ends within the SPMV_WX entries staged past the
+//     window) is written directly.  Rows of at most W_LONG entries are summed
+//     left to right on one lane, the order of the reference loop, so they are
+//     bit-identical to it; longer ones by the whole wavefront;
+//   * a longer row that runs past the window leaves partial sums: its owner's
+//     part ("tail") and each later window's part ("head").  k_spmv_fold adds
+//     them in entry order and writes the row.
 constexpr int WAVES = 4;
-constexpr int WU = SPMV_WCAP / 64;       // staged entries per lane
+constexpr int WU = SPMV_WIN / 64;        // window entries per lane
+constexpr int WS = SPMV_WIN + SPMV_WX;   // staged entries per window
 constexpr int WRP = 256;                 // row offsets staged per wavefront
 constexpr int WRPL = WRP / 64;
-constexpr int W_LONG = 32;               // staged rows longer than this: the whole wavefront
+constexpr int W_LONG = SPMV_WX;          // rows longer than this: the whole wavefront
 
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -209,28 +232,58 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Zero-based [start, end) of row r, clamped to [0, nnz].
+template <typename I>
+__device__ __forceinline__ void row_span(const I* __restrict__ ptr, int64_t r, int64_t nnz,
+                                         int64_t& a, int64_t& b) {
+    a = clamp64((int64_t)ptr[r] - 1, 0, nnz);
+    b = clamp64((int64_t)ptr[r + 1] - 1, a, nnz);
+}
+
+// Does window w's last owned row (r1-1) leave a tail: longer than W_LONG and
+// running past the window?  Shared by the SpMV kernel and the fold.
+template <typename I>
+__device__ __forceinline__ bool window_tail(const I* __restrict__ ptr, int64_t r0, int64_t r1,
+                                            int64_t e1, int64_t nnz, int64_t& end) {
+    if (r1 <= r0) return false;
+    int64_t a;
+    row_span(ptr, r1 - 1, nnz, a, end);
+    return end > e1 && end - a > W_LONG;
+}
+
 template <typename T, typename I, typename V = T>
 __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
     int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowptr,
     const I* __restrict__ colval, const V* __restrict__ nzval, const float* __restrict__ rscale,
     const T* __restrict__ x, T* __restrict__ y, T alpha, T beta, int mode,
-    const int64_t* __restrict__ win_row, const int64_t* __restrict__ win_pos, int64_t nwin) {
-    __shared__ T s_prod[WAVES][SPMV_WCAP];
+    const int64_t* __restrict__ win_row, int64_t nwin, T* __restrict__ head,
+    T* __restrict__ tail) {
+    __shared__ T s_prod[WAVES][WS];
     __shared__ int s_off[WAVES][WRP];
 
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
     const int64_t w = (int64_t)blockIdx.x * WAVES + wv;
     if (w >= nwin) return;
-    const Window win = load_window(w, m, nnz, win_row, win_pos, SPMV_WCAP);
-    if (win.r0 >= win.r1) return;
-    const int64_t P0 = win.P0, Ps = win.Ps, Pend = win.Pend, r0 = win.r0, r1 = win.r1;
-    const int64_t rs_end = win.rs_end;
-    const int cnt = (int)(Ps - P0);
-    const int nst = (int)(rs_end - r0 < WRP - 1 ? rs_end - r0 : WRP - 1);
-    const bool rp_in_lds = rs_end - r0 + 1 <= WRP;
+    auto emit = [&](int64_t r, T acc) {
+        store_out(y, r, row_scaled(acc, rscale, r), alpha, beta, mode);
+    };
+    const int64_t E0 = w * SPMV_WIN < nnz ? w * SPMV_WIN : nnz;
+    const int64_t E1 = E0 + SPMV_WIN < nnz ? E0 + SPMV_WIN : nnz;
+    const int cnt = (int)(E1 - E0);
+    const int64_t r0 = clamp64(win_row[w], 0, m);
+    const int64_t r1 = clamp64(win_row[w + 1], r0, m);
+    const int64_t nrows = r1 - r0;
+    const int nst = (int)(nrows < WRP - 1 ? nrows : WRP - 1);  // offsets r0..r0+nst staged
+    const bool rp_in_lds = nrows + 1 <= WRP;
     T* prod = s_prod[wv];
     int* off = s_off[wv];
+
+    // the last owned row: a tail, or a short row finished from the extension
+    int64_t last_end = 0;
+    const bool has_tail = window_tail(rowptr, r0, r1, E1, nnz, last_end);
+    const int need = (nrows > 0 && !has_tail && last_end > E1) ? (int)(last_end - E1) : 0;
+    const int staged = cnt + need;  // <= SPMV_WIN + SPMV_WX
 
     // 1. row offsets, indices and values in flight together, then the gathers
     I rpv[WRPL];
@@ -240,51 +293,68 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
         i = i <= nst ? i : nst;
         rpv[k] = ldnt(rowptr + r0 + i);
     }
-    if (cnt > 0) {
-        I ci[WU];
-        T vv[WU], xv[WU];
+    if (staged > 0) {
+        I ci[WU + 1];
+        T vv[WU + 1], xv[WU + 1];
 #pragma unroll
-        for (int u = 0; u < WU; ++u) {
+        for (int u = 0; u <= WU; ++u) {
             int i = u * 64 + lane;
-            i = i < cnt ? i : cnt - 1;
-            ci[u] = ldnt(colval + P0 + i);
-            vv[u] = widen<T>(ldnt(nzval + P0 + i));
+            i = i < staged ? i : staged - 1;
+            ci[u] = ldnt(colval + E0 + i);
+            vv[u] = widen<T>(ldnt(nzval + E0 + i));
         }
 #pragma unroll
-        for (int u = 0; u < WU; ++u) xv[u] = x[clamp64((int64_t)ci[u] - 1, 0, n - 1)];
+        for (int u = 0; u <= WU; ++u) xv[u] = x[clamp64((int64_t)ci[u] - 1, 0, n - 1)];
 #pragma unroll
-        for (int u = 0; u < WU; ++u) {
+        for (int u = 0; u <= WU; ++u) {
             int i = u * 64 + lane;
-            if (i < cnt) prod[i] = vv[u] * xv[u];
+            if (i < staged) prod[i] = vv[u] * xv[u];
         }
     }
+    const int64_t EX = E0 + staged;
 #pragma unroll
     for (int k = 0; k < WRPL; ++k) {
         int i = lane + 64 * k;
-        if (i <= nst) off[i] = (int)(clamp64((int64_t)rpv[k] - 1, P0, Ps) - P0);
+        if (i <= nst) off[i] = (int)(clamp64((int64_t)rpv[k] - 1, E0, EX) - E0);
     }
     wave_lds_sync();
 
-    // 2. one lane per short row (left-to-right sum); longer staged rows are
-    //    reduced by the whole wavefront, one at a time
-    for (int64_t base = r0; base < rs_end; base += 64) {
+    // 2. head: entries [0, hend) belong to a row started in an earlier window;
+    //    short ones were finished by their owner
+    const int hend = off[0] < cnt ? off[0] : cnt;
+    if (hend > 0 && r0 > 0) {
+        int64_t a, b;
+        row_span(rowptr, r0 - 1, nnz, a, b);
+        if (b - a > W_LONG) {
+            CompSum<T> cs;
+            for (int j = lane; j < hend; j += 64) cs.add(prod[j]);
+            const T acc = cs.wave_total();
+            if (lane == 0) head[w] = acc;
+        }
+    }
+
+    // 3. owned rows: one lane per short row (left to right); longer rows and
+    //    the tail by the whole wavefront, one at a time
+    for (int64_t base = r0; base < r1; base += 64) {
         const int64_t r = base + lane;
-        const bool act = r < rs_end;
+        const bool act = r < r1;
         int s = 0, e = 0;
         if (act) {
             if (rp_in_lds) {
                 s = off[r - r0];
                 e = off[r - r0 + 1];
             } else {
-                s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
-                e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
+                s = (int)(clamp64((int64_t)rowptr[r] - 1, E0, EX) - E0);
+                e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, E0, EX) - E0);
             }
         }
-        const bool lng = act && (e - s > W_LONG);
+        const bool is_tail = act && has_tail && r == r1 - 1;
+        if (is_tail) e = cnt;  // this window's part only
+        const bool lng = act && (is_tail || e - s > W_LONG);
         if (act && !lng) {
             T acc = T(0);
             for (int j = s; j < e; ++j) acc = acc + prod[j];
-            store_out(y, r, row_scaled(acc, rscale, r), alpha, beta, mode);
+            emit(r, acc);
         }
         uint64_t lm = __ballot(lng);
         while (lm) {
@@ -294,33 +364,38 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
             CompSum<T> cs;
             for (int j = ls + lane; j < le; j += 64) cs.add(prod[j]);
             const T acc = cs.wave_total();
-            if (lane == 0) store_out(y, base + src, row_scaled(acc, rscale, base + src), alpha, beta, mode);
-        }
-    }
-
-    // 3. spill row: the wavefront reduces it from HBM
-    if (rs_end < r1) {
-        const int64_t r = r1 - 1;
-        CompSum<T> cs;
-        int64_t j = Ps + lane;
-        for (; j + 7 * 64 < Pend; j += 8 * 64) {
-            I cc[8];
-            T vv[8], xv[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                cc[u] = ldnt(colval + j + u * 64);
-                vv[u] = widen<T>(ldnt(nzval + j + u * 64));
+            if (lane == 0) {
+                if (has_tail && base + src == r1 - 1) tail[w] = acc;
+                else emit(base + src, acc);
             }
-#pragma unroll
-            for (int u = 0; u < 8; ++u) xv[u] = x[clamp64((int64_t)cc[u] - 1, 0, n - 1)];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) cs.add(vv[u] * xv[u]);
         }
-        for (; j < Pend; j += 64)
-            cs.add(widen<T>(ldnt(nzval + j)) * x[clamp64((int64_t)ldnt(colval + j) - 1, 0, n - 1)]);
-        const T acc = cs.wave_total();
-        if (lane == 0) store_out(y, r, row_scaled(acc, rscale, r), alpha, beta, mode);
     }
+}
+
+// Rows that ran past their window: the owner's tail then every later
+// window's head, in entry order (compensated for Float32).  One thread per
+// window.
+template <typename T, typename I>
+__global__ void k_spmv_fold(int64_t m, int64_t nnz, const I* __restrict__ rowptr,
+                            const int64_t* __restrict__ win_row, int64_t nwin,
+                            const T* __restrict__ head, const T* __restrict__ tail,
+                            const float* __restrict__ rscale, T* __restrict__ y, T alpha, T beta,
+                            int mode) {
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (w >= nwin) return;
+    const int64_t E0 = w * SPMV_WIN < nnz ? w * SPMV_WIN : nnz;
+    const int64_t E1 = E0 + SPMV_WIN < nnz ? E0 + SPMV_WIN : nnz;
+    const int64_t r0 = clamp64(win_row[w], 0, m);
+    const int64_t r1 = clamp64(win_row[w + 1], r0, m);
+    int64_t end = 0;
+    if (!window_tail(rowptr, r0, r1, E1, nnz, end)) return;
+    int64_t we = (end - 1) / SPMV_WIN;
+    we = we < nwin - 1 ? we : nwin - 1;
+    CompSum<T> cs;
+    cs.add(tail[w]);
+    for (int64_t v = w + 1; v <= we; ++v) cs.add(head[v]);
+    const int64_t r = r1 - 1;
+    store_out(y, r, row_scaled(cs.total(), rscale, r), alpha, beta, mode);
 }
 
 // ------------------------------------------------------------ SpMM kernel --
@@ -612,10 +687,6 @@ template <typename T>
 static int apply_beta(hipStream_t s, int64_t rows, int64_t k, T* C, int64_t ldc, double beta) {
     int mode = beta_mode(beta);
     if (mode == 1 || rows == 0 || k == 0) return GSA_OK;
-    if (ldc == rows && mode == 0) {
-        GSA_HIP(hipMemsetAsync(C, 0, sizeof(T) * rows * k, s));
-        return GSA_OK;
-    }
     hipLaunchKernelGGL(k_beta<T>, dim3(grid_for(rows * k, 256)), dim3(256), 0, s, rows, k, C,
                        ldc, (T)beta, mode);
     GSA_HIP(hipGetLastError());
@@ -623,38 +694,54 @@ static int apply_beta(hipStream_t s, int64_t rows, int64_t k, T* C, int64_t ldc,
 }
 
 // Row-block plan for one call: use the caller's plan when it matches, else build
-// a temporary one in scratch.
+// a temporary one in scratch (device work only, no synchronisation).
 struct PlanUse {
     const int64_t* blk_row = nullptr;
     const int64_t* blk_pos = nullptr;
     int64_t nblocks = 0;
     const int64_t* win_row = nullptr;
-    const int64_t* win_pos = nullptr;
     int64_t nwin = 0;
-    Scratch tmp;
-    explicit PlanUse(hipStream_t s) : tmp(s) {}
-    // wave = true: the wavefront windows (SpMV); else the workgroup windows.
+    CsrPlanData* owner = nullptr;
+    Scratch tmp, tmp_part;
+    explicit PlanUse(hipStream_t s) : tmp(s), tmp_part(s) {}
+    // wave = true: the SpMV windows; else the workgroup row blocks.
     int get(hipStream_t s, gsa_plan plan, int itype, int64_t m, int64_t nnz, const void* ptr,
             bool wave) {
         if (plan && plan->kind == 0 && plan->csr.m == m && plan->csr.nnz == nnz &&
             plan->csr.itype == itype && plan->csr.ptr == ptr) {
-            blk_row = plan->csr.blk_row;
-            blk_pos = plan->csr.blk_pos;
-            nblocks = plan->csr.nblocks;
-            win_row = plan->csr.win_row;
-            win_pos = plan->csr.win_pos;
-            nwin = plan->csr.nwin;
+            owner = &plan->csr;
+            blk_row = owner->blk_row;
+            blk_pos = owner->blk_pos;
+            nblocks = owner->nblocks;
+            win_row = owner->win_row;
+            nwin = owner->nwin;
             return GSA_OK;
         }
-        const int nb = wave ? SPMV_WNB : SPMV_NB;
-        const int cap = wave ? SPMV_WCAP : SPMV_CAP;
-        const int64_t cnt = nnz / nb + 1;
+        if (wave) {
+            const int64_t cnt = nnz / SPMV_WIN + 1;
+            GSA_TRY(tmp.alloc(sizeof(int64_t) * (cnt + 1)));
+            GSA_TRY(build_csr_plan(s, itype, m, nnz, ptr, tmp.as<int64_t>(), cnt, nullptr,
+                                   SPMV_WIN));
+            win_row = tmp.as<int64_t>();
+            nwin = cnt;
+            return GSA_OK;
+        }
+        const int64_t cnt = nnz / SPMV_NB + 1;
         GSA_TRY(tmp.alloc(sizeof(int64_t) * ((cnt + 1) + PLAN_POS * cnt)));
         int64_t* br = tmp.as<int64_t>();
         int64_t* bp = br + (cnt + 1);
-        GSA_TRY(build_csr_plan(s, itype, m, nnz, ptr, br, cnt, bp, nb, cap));
-        if (wave) { win_row = br; win_pos = bp; nwin = cnt; }
-        else { blk_row = br; blk_pos = bp; nblocks = cnt; }
+        GSA_TRY(build_csr_plan(s, itype, m, nnz, ptr, br, cnt, bp, SPMV_NB, SPMV_CAP));
+        blk_row = br; blk_pos = bp; nblocks = cnt;
+        return GSA_OK;
+    }
+    // Head/tail partial sums of the SpMV windows: the plan's, or call scratch.
+    int part_buffer(void** out) {
+        if (owner) {
+            *out = owner->part;
+            return GSA_OK;
+        }
+        GSA_TRY(tmp_part.alloc(sizeof(double) * 2 * nwin));
+        *out = tmp_part.p;
         return GSA_OK;
     }
 };
@@ -682,9 +769,18 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
                                      hipMemcpyDeviceToDevice, s));
             x = xs.as<T>();
         }
-        hipLaunchKernelGGL((k_csr_spmv<T, I, V>), dim3((unsigned)cdiv(pu.nwin, WAVES)), dim3(64 * WAVES),
-                           0, s, m, n, nnz, rowptr, colval, nzval, rscale, x, C,
-                           (T)alpha, (T)beta, mode, pu.win_row, pu.win_pos, pu.nwin);
+        void* buf = nullptr;
+        GSA_TRY(pu.part_buffer(&buf));
+        T* head = static_cast<T*>(buf);
+        T* tail = head + pu.nwin;
+        const int64_t grid1 = cdiv(pu.nwin, WAVES);
+        GSA_CHECK(grid1 <= (int64_t)UINT32_MAX, GSA_ERR_INVALID, "SpMV grid too large");
+        hipLaunchKernelGGL((k_csr_spmv<T, I, V>), dim3((unsigned)grid1), dim3(64 * WAVES), 0, s, m,
+                           n, nnz, rowptr, colval, nzval, rscale, x, C, (T)alpha, (T)beta, mode,
+                           pu.win_row, pu.nwin, head, tail);
+        hipLaunchKernelGGL((k_spmv_fold<T, I>), dim3((unsigned)cdiv(pu.nwin, 256)), dim3(256), 0, s,
+                           m, nnz, rowptr, pu.win_row, pu.nwin, (const T*)head, (const T*)tail,
+                           rscale, C, (T)alpha, (T)beta, mode);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
     }
@@ -759,6 +855,16 @@ static int check_common(int opA, int opB, int64_t m, int64_t n, int64_t k, int64
     return GSA_OK;
 }
 
+static int plan_release(CsrPlanData& d) {
+    hipError_t e = hipSuccess, f;
+    for (void* q : {(void*)d.blk_row, d.part})
+        if (q && (f = hipFree(q)) != hipSuccess && e == hipSuccess) e = f;
+    d.blk_row = d.blk_pos = d.win_row = nullptr;
+    d.part = nullptr;
+    if (e != hipSuccess) return hip_fail(e, "hipFree(plan)");
+    return GSA_OK;
+}
+
 }  // namespace gsa
 
 using namespace gsa;
@@ -768,8 +874,8 @@ extern "C" {
 int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz, const void* ptr,
                     gsa_plan* plan) {
     GSA_TRY(ensure_init());
-    GSA_CHECK(plan && m >= 0 && nnz >= 0 && (ptr || m == 0), GSA_ERR_INVALID,
-              "gsa_plan_create: bad arguments");
+    GSA_CHECK(plan && m >= 0 && nnz >= 0 && ptr, GSA_ERR_INVALID,
+              "gsa_plan_create: bad arguments (the row pointer has m+1 >= 1 entries)");
     GSA_CHECK(itype == GSA_I32 || itype == GSA_I64, GSA_ERR_UNSUPPORTED, "bad itype %d", itype);
     hipStream_t s = as_stream(stream);
     auto* p = new gsa_plan_s();
@@ -779,29 +885,31 @@ int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz, const void*
     p->csr.nnz = nnz;
     p->csr.ptr = ptr;
     p->csr.nblocks = nnz / SPMV_NB + 1;
-    p->csr.nwin = nnz / SPMV_WNB + 1;
-    const int64_t words = (p->csr.nblocks + 1) + PLAN_POS * p->csr.nblocks +
-                          (p->csr.nwin + 1) + PLAN_POS * p->csr.nwin;
+    p->csr.nwin = nnz / SPMV_WIN + 1;
+    const int64_t words = (p->csr.nblocks + 1) + PLAN_POS * p->csr.nblocks + (p->csr.nwin + 1);
     hipError_t e = hipMalloc(&p->csr.blk_row, sizeof(int64_t) * words);
-    if (e != hipSuccess) { delete p; return hip_fail(e, "hipMalloc(plan)"); }
+    if (e == hipSuccess) e = hipMalloc(&p->csr.part, sizeof(double) * 2 * p->csr.nwin);
+    if (e != hipSuccess) {
+        (void)plan_release(p->csr);
+        delete p;
+        return hip_fail(e, "hipMalloc(plan)");
+    }
     p->csr.blk_pos = p->csr.blk_row + (p->csr.nblocks + 1);
     p->csr.win_row = p->csr.blk_pos + PLAN_POS * p->csr.nblocks;
-    p->csr.win_pos = p->csr.win_row + (p->csr.nwin + 1);
     int r = build_csr_plan(s, itype, m, nnz, ptr, p->csr.blk_row, p->csr.nblocks, p->csr.blk_pos);
     if (r == GSA_OK)
-        r = build_csr_plan(s, itype, m, nnz, ptr, p->csr.win_row, p->csr.nwin, p->csr.win_pos,
-                           SPMV_WNB, SPMV_WCAP);
-    if (r != GSA_OK) { (void)hipFree(p->csr.blk_row); delete p; return r; }
+        r = build_csr_plan(s, itype, m, nnz, ptr, p->csr.win_row, p->csr.nwin, nullptr, SPMV_WIN);
+    if (r != GSA_OK) { (void)plan_release(p->csr); delete p; return r; }
     *plan = p;
     return GSA_OK;
 }
 
 int gsa_plan_destroy(gsa_plan plan) {
     if (!plan) return GSA_OK;
-    if (plan->kind == 0 && plan->csr.blk_row) {
+    if (plan->kind == 0) {
         // the plan may still be in use by queued work on any stream
         GSA_HIP(hipDeviceSynchronize());
-        GSA_HIP(hipFree(plan->csr.blk_row));
+        GSA_TRY(plan_release(plan->csr));
     }
     if (plan->kind == 1) batch_plan_release(plan);
     delete plan;

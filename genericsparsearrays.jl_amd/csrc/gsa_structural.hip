@@ -381,7 +381,7 @@ int gsa_csr_dropzeros_count(void* stream, int vtype, int itype, int64_t m, const
             GSA_HIP(hipStreamSynchronize(s));
         }
         const int64_t nnz = (int64_t)last - 1;
-        GSA_HIP(hipMemsetAsync(rowptr_out, 0, sizeof(I) * (m + 1), s));
+        GSA_TRY(zero_fill(s, rowptr_out, sizeof(I) * (m + 1)));
         if (m > 0 && nnz > 0) {
             const int64_t nblocks = nnz / SPMV_NB + 1;
             Scratch plan(s);
@@ -616,7 +616,7 @@ int gsa_kron_csr(void* stream, int vtype, int itype, int64_t mA, int64_t nA, int
         // canonical operands?
         Scratch flag(s);
         GSA_TRY(flag.alloc(sizeof(int)));
-        GSA_HIP(hipMemsetAsync(flag.p, 0, sizeof(int), s));
+        GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
         if (mA > 0)
             hipLaunchKernelGGL(k_check_canonical<I>, dim3(grid_for(mA, 256)), dim3(256), 0, s, mA,
                                (const I*)rowptrA, (const I*)colA, flag.as<int>());
@@ -677,8 +677,7 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
         int r;
         // per-row counts (ctx buf[0]: I[m+1]) always allocated
         if ((r = ws_alloc(&c->buf[0], sizeof(I) * (m + 1), s)) != GSA_OK) return fail(r);
-        hipError_t e = hipMemsetAsync(c->buf[0], 0, sizeof(I) * (m + 1), s);
-        if (e != hipSuccess) return fail(hip_fail(e, "hipMemsetAsync"));
+        if ((r = zero_fill(s, c->buf[0], sizeof(I) * (m + 1))) != GSA_OK) return fail(r);
         if (m == 0 || n == 0 || nnzA == 0 || nnzB == 0 || kdim == 0) {
             *ctx = c;
             return (int)GSA_OK;
@@ -713,7 +712,7 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
         if ((r = inclusive_scan<int64_t>(s, flags.as<int64_t>(), flags.as<int64_t>(), P, 0)) != GSA_OK)
             return fail(r);
         int64_t total = 0;
-        e = hipMemcpyAsync(&total, flags.as<int64_t>() + P - 1, sizeof(int64_t),
+        hipError_t e = hipMemcpyAsync(&total, flags.as<int64_t>() + P - 1, sizeof(int64_t),
                            hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (e != hipSuccess) return fail(hip_fail(e, "gsa_spgemm_csr_begin"));

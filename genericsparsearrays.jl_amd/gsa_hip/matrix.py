@@ -187,7 +187,10 @@ def _check_common(m, n, tensors, itype_tensor):
 
 
 class _PlanCache:
-    """Per-matrix cached row-block analysis (gsa_plan) of one pointer array."""
+    """Per-matrix cached analysis (gsa_plan) of one pointer array: the row
+    blocks and SpMV windows.  A plan also holds the partial sums of the SpMV in
+    flight, so products on different streams at the same time need different
+    plans."""
 
     def __init__(self):
         self.handle = None

@@ -31,7 +31,7 @@ import torch
 
 from . import _lib as L
 from ._lib import ArgumentError
-from .matrix import GenericSparseMatrixCSR, itype_of, stream_of
+from .matrix import GenericSparseMatrixCSR, _PlanCache, itype_of, stream_of
 from .precision import BF16CSR, mul_bf16
 from .ops import mul_
 
@@ -57,7 +57,8 @@ class MatrixRegistry:
         self.rows_per_tile = rows_per_tile
         self.max_k = max_k
         self.big_rows = big_rows   # matrices with at least this many rows use the dedicated kernels
-        self.big = []              # per matrix: (plan handle, contiguous column 0) or None
+        self.big = []              # per matrix: its contiguous input column 0, or None
+        self._slot_plans = {}      # (matrix, slot) -> plan of a big matrix owned by one slot
         self.mats = []
         self.inputs = []     # per matrix: n x max_k column-major Float32 input block
         self.vtype = None
@@ -87,7 +88,7 @@ class MatrixRegistry:
         self.mats.append(A)
         self.inputs.append(x_block)
         if m >= self.big_rows:
-            self.big.append((A.plan() if A.nnz() > 0 else None, x_block[:, 0].contiguous()))
+            self.big.append(x_block[:, 0].contiguous())
         else:
             self.big.append(None)
         d = np.zeros(1, DESC_DTYPE)
@@ -99,6 +100,19 @@ class MatrixRegistry:
         d["flags"] = L.GSA_BATCH_B_ROWMAJOR | L.GSA_BATCH_C_ROWMAJOR
         self._desc.append(d[0])
         return mid
+
+    def slot_plan(self, mid: int, slot: int):
+        """Plan handle of big matrix `mid` owned by serving slot `slot`, or None
+        for an empty matrix.  A plan carries the partial sums of the call in
+        flight, so calls that can overlap (different slots) use different plans."""
+        A = self.mats[mid]
+        if A.nnz() == 0:
+            return None
+        pc = self._slot_plans.get((mid, slot))
+        if pc is None:
+            pc = _PlanCache()
+            self._slot_plans[(mid, slot)] = pc
+        return pc.get(A.rowptr, A.shape[0], A.nnz())
 
     def desc_table(self) -> np.ndarray:
         return np.array(self._desc, dtype=DESC_DTYPE)
@@ -144,6 +158,7 @@ class _Slot:
         self.d_tiles = torch.empty(max_tiles * 3, dtype=torch.int32, device=device)
         self.d_counter = torch.zeros(1, dtype=torch.int32, device=device)
         self.done = torch.cuda.Event(enable_timing=True)   # reused for every launch
+        self.index = 0
         self.event = None
         self.req = None
         self.bytes = 0
@@ -169,10 +184,44 @@ class Server:
         self._m, self._n = m, n
         self.in_ptr = self.desc["B"].copy()
         self.is_big = np.array([b is not None for b in registry.big], dtype=bool)
+        # one plan per (big matrix, slot), each run once so its partial-sum
+        # buffer exists before the loop (no allocation while serving)
+        for si, slot in enumerate(self.slots):
+            slot.index = si
+            for mid in np.flatnonzero(self.is_big).tolist():
+                self._big_call(slot, int(mid), 1, slot.arena.data_ptr())
+        torch.cuda.synchronize(dev)
 
     def request_bytes(self, mids, ks):
         """HBM bytes a request holds while in flight: its m x k Float32 result."""
         return self._m[mids] * ks * 4
+
+    def _big_call(self, slot: _Slot, mid: int, k: int, cptr: int):
+        """One big-matrix request (k right-hand sides, column-major m x k result at
+        device address cptr) through the dedicated SpMV/SpMM kernels on the
+        slot's stream."""
+        reg = self.reg
+        A = reg.mats[mid]
+        m = A.shape[0]
+        if m > slot.arena.numel():
+            raise ArgumentError("a request's result is larger than a slot's share of the budget")
+        plan = reg.slot_plan(mid, slot.index)
+        st = ctypes.c_void_p(slot.stream.cuda_stream)
+        if k == 1:
+            bptr, ldb, opB = reg.big[mid].data_ptr(), max(A.n, 1), L.GSA_OP_N
+        else:
+            bptr, ldb, opB = reg.inputs[mid].data_ptr(), reg.max_k, L.GSA_OP_T
+        c = ctypes.c_void_p(cptr)
+        if isinstance(A, BF16CSR):
+            L.call("gsa_spmm_csr_bf16", st, reg.itype, m, A.n, k,
+                   ctypes.c_void_p(A.rowptr.data_ptr()), ctypes.c_void_p(A.colval.data_ptr()),
+                   ctypes.c_void_p(A.nzval.data_ptr()), ctypes.c_void_p(A.row_scale.data_ptr()),
+                   A.nnz(), ctypes.c_void_p(bptr), ldb, opB, c, max(m, 1), 1.0, 0.0, plan)
+        else:
+            L.call("gsa_spmm_csr", st, L.GSA_F32, reg.itype, L.GSA_OP_N, m, A.n, k,
+                   ctypes.c_void_p(A.rowptr.data_ptr()), ctypes.c_void_p(A.colval.data_ptr()),
+                   ctypes.c_void_p(A.nzval.data_ptr()), A.nnz(), ctypes.c_void_p(bptr), ldb, opB,
+                   c, max(m, 1), 1.0, 0.0, plan)
 
     def _launch(self, slot: _Slot, idx: np.ndarray, mids, ks, out_list):
         """One group: the small-matrix requests in one batched launch, then each
@@ -226,26 +275,7 @@ class Server:
                        ctypes.c_void_p(slot.d_tiles.data_ptr()), ntiles,
                        ctypes.c_void_p(slot.d_counter.data_ptr()), 1.0, 0.0, self.grid)
             for j in np.flatnonzero(isbig).tolist():
-                mid, k, m = int(mids[j]), int(ks[j]), int(ms[j])
-                A = reg.mats[mid]
-                plan, x0 = reg.big[mid]
-                X = reg.inputs[mid]
-                if k == 1:
-                    bptr, ldb, opB = x0.data_ptr(), max(A.n, 1), L.GSA_OP_N
-                else:
-                    bptr, ldb, opB = X.data_ptr(), reg.max_k, L.GSA_OP_T
-                cptr = ctypes.c_void_p(out.data_ptr() + int(offs[j]) * 4)
-                if isinstance(A, BF16CSR):
-                    L.call("gsa_spmm_csr_bf16", st, reg.itype, m, A.n, k,
-                           ctypes.c_void_p(A.rowptr.data_ptr()), ctypes.c_void_p(A.colval.data_ptr()),
-                           ctypes.c_void_p(A.nzval.data_ptr()),
-                           ctypes.c_void_p(A.row_scale.data_ptr()), A.nnz(), ctypes.c_void_p(bptr),
-                           ldb, opB, cptr, max(m, 1), 1.0, 0.0, plan)
-                else:
-                    L.call("gsa_spmm_csr", st, L.GSA_F32, reg.itype, L.GSA_OP_N, m, A.n, k,
-                           ctypes.c_void_p(A.rowptr.data_ptr()), ctypes.c_void_p(A.colval.data_ptr()),
-                           ctypes.c_void_p(A.nzval.data_ptr()), A.nnz(), ctypes.c_void_p(bptr),
-                           ldb, opB, cptr, max(m, 1), 1.0, 0.0, plan)
+                self._big_call(slot, int(mids[j]), int(ks[j]), out.data_ptr() + int(offs[j]) * 4)
             slot.done.record(slot.stream)
         slot.event, slot.req, slot.out = slot.done, idx, out
         slot.used_words = int(offs[-1])

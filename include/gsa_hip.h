@@ -69,10 +69,14 @@ int gsa_stream_destroy(void* stream);
 int gsa_stream_sync(void* stream);
 
 /* ========================== SpMV / SpMM (mul!) ============================== */
-/* Row-block analysis of a CSR (or, on its column pointer, CSC) matrix: the
- * owner block of every row, computed once and reused by every mul! on the same
- * structure.  `ptr` has length `m+1`.  Stream-ordered; the plan is usable on
- * `stream` immediately. */
+/* Row-block analysis of a CSR (or, on its column pointer, CSC) matrix, computed
+ * once and reused by every mul! on the same structure: the first row starting
+ * in each block of entries, for the SpMV's 576-entry windows and the SpMM's
+ * 2048-entry blocks.  `ptr` has length `m+1`.  Stream-ordered device work; the
+ * plan is usable on `stream` immediately.  A plan also holds the partial sums
+ * of the SpMV in flight (rows that cross window ends): calls that may run at
+ * the same time, on different streams, need different plans.  No counterpart
+ * in the reference, whose mul! analyses nothing (src/matrix_csr/matrix_csr.jl:145-211). */
 int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz,
                     const void* ptr, gsa_plan* plan);
 int gsa_plan_destroy(gsa_plan plan);

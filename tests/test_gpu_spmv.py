@@ -180,3 +180,113 @@ def test_int_overflowing_dims_rejected():
     with pytest.raises(G.ArgumentError):
         G.GenericSparseMatrixCSR(2 ** 31, 1, torch.ones(1, dtype=torch.int32),
                                  torch.zeros(0, dtype=torch.int32), torch.zeros(0))
+
+
+# ---- rows across SpMV windows -----------------------------------------------
+# The SpMV cuts the entries into fixed 576-entry windows; a row longer than 32
+# entries that runs past its window is summed from partial sums (its owner's
+# tail and each later window's head); a shorter one is finished by its owner
+# from 32 entries staged past the window, so it stays bit-identical.
+
+def _spmv(dev, A, x, planned):
+    """y = A x through gsa_spmm_csr, with a plan (gsa_plan_create) or without."""
+    import ctypes
+    from gsa_hip import _lib as L
+    from gsa_hip.matrix import stream_of
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    h = ctypes.c_void_p()
+    st = stream_of(dev)
+    it = L.GSA_I64 if A.rowptr.dtype == torch.int64 else L.GSA_I32
+    if planned:
+        L.call("gsa_plan_create", st, it, A.m, A.nnz(), p(A.rowptr), ctypes.byref(h))
+    try:
+        y = torch.empty(A.m, dtype=torch.float64, device=dev)
+        L.call("gsa_spmm_csr", st, L.GSA_F64, it, L.GSA_OP_N, A.m, A.n, 1, p(A.rowptr),
+               p(A.colval), p(A.nzval), A.nnz(), p(x), max(A.n, 1), L.GSA_OP_N, p(y), max(A.m, 1),
+               1.0, 0.0, h if planned else None)
+        return host(y)
+    finally:
+        if planned:
+            L.lib().gsa_plan_destroy(h)
+
+
+@pytest.mark.parametrize("lengths", [
+    [5000] + [1] * 3000,
+    [2048, 1, 2303, 2304, 2305, 0, 4097, 3],
+    [0] * 5000 + [7] + [0] * 5000,
+    [100_000],
+    [575, 1, 576, 33, 1152, 0, 32, 31, 1153, 577],  # lengths around the window and W_LONG
+    [600] * 300,                                    # every row crosses a window end
+    [32] * 4000,                                    # short rows crossing: finished by owners
+    [33] * 4000,                                    # just long enough for head/tail sums
+])
+@pytest.mark.parametrize("planned", [True, False])
+def test_rows_across_windows(dev, lengths, planned):
+    rng = np.random.default_rng(25)
+    H = synth.csr_from_lengths(np.array(lengths), 120_000, rng, "normal")
+    x = rng.standard_normal(H.n)
+    A = dev_csr(H)
+    y = _spmv(dev, A, torch.as_tensor(x, device=dev), planned)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, x)
+    assert_normwise(y, ref, norm_bound(H.rowptr, H.colval, H.nzval, x), 1e-12)
+    short = np.diff(H.rowptr) <= 32
+    assert np.array_equal(y[short], ref[short])
+
+
+def test_unsorted_columns(dev):
+    """Column order inside a row is not required (the reference does not sort)."""
+    rng = np.random.default_rng(27)
+    H = synth.csr_from_lengths(np.array([3000, 5, 2500, 0, 900, 40] * 3), 80_000, rng, "normal")
+    for r in range(H.m):  # shuffle every row's entries (columns and values together)
+        a, b = H.rowptr[r] - 1, H.rowptr[r + 1] - 1
+        perm = rng.permutation(b - a)
+        H.colval[a:b] = H.colval[a:b][perm]
+        H.nzval[a:b] = H.nzval[a:b][perm]
+    x = rng.standard_normal(H.n)
+    A = dev_csr(H)
+    y = _spmv(dev, A, torch.as_tensor(x, device=dev), True)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, x)
+    assert_normwise(y, ref, norm_bound(H.rowptr, H.colval, H.nzval, x), 1e-12)
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.5, 0.0), (1.0, 1.0), (0.5, 2.5)])
+def test_rows_across_windows_alpha_beta(dev, alpha, beta):
+    rng = np.random.default_rng(24)
+    H = synth.csr_from_lengths(np.r_[rng.integers(0, 40, 3000), [4000, 700, 513]], 20_000, rng,
+                               "normal")
+    x = rng.standard_normal(H.n)
+    c0 = rng.standard_normal(H.m)
+    if beta == 0.0:
+        c0[::5] = np.nan  # cleared by β == 0 on every path, folded rows included
+    A = dev_csr(H)
+    C = torch.as_tensor(c0.copy(), device=dev)
+    G.mul_(C, A, torch.as_tensor(x, device=dev), alpha, beta)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, x, C=c0, alpha=alpha, beta=beta)
+    # β C contributes |β c0| (0 where β == 0 clears a NaN)
+    bound = abs(alpha) * norm_bound(H.rowptr, H.colval, H.nzval, x) + np.nan_to_num(np.abs(beta * c0))
+    got = host(C)
+    assert not np.isnan(got).any()
+    assert_normwise(got, ref, bound, 1e-12)
+    short = np.diff(H.rowptr) <= 32
+    assert np.array_equal(got[short], ref[short])
+
+
+def test_bf16_f32_rows_across_windows(dev):
+    """Float32 and BF16-stored values through the same windows (TwoSum on the
+    wavefront-reduced rows, per-row scale for BF16)."""
+    from gsa_hip.precision import BF16CSR, mul_bf16
+    rng = np.random.default_rng(28)
+    H = synth.csr_from_lengths(np.r_[rng.integers(0, 30, 5000), [3000, 800]], 30_000, rng,
+                               "normal", dtype=np.float32)
+    x = rng.standard_normal(H.n).astype(np.float32)
+    A = dev_csr(H, dtype=torch.float32)
+    y = host(A @ torch.as_tensor(x, device=dev))
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval.astype(np.float64), x.astype(np.float64))
+    bound = norm_bound(H.rowptr, H.colval, H.nzval, x)
+    assert_normwise(y, ref, bound, 2e-6)
+    B = BF16CSR(A)
+    yb = torch.empty(H.m, dtype=torch.float32, device=dev)
+    mul_bf16(yb, B, torch.as_tensor(x, device=dev))
+    deq = B.dequantized_values()
+    refb = O.spmm_csr_N(H.m, H.rowptr, H.colval, deq, x.astype(np.float64))
+    assert_normwise(host(yb), refb, norm_bound(H.rowptr, H.colval, deq, x), 2e-6)
