@@ -440,6 +440,42 @@ def main():
             extra[tag + "_GBs"] = round(gbs2, 1)
             extra[tag + "_us"] = round(ms2 * 1e3, 2)
             print(f"[bench] {tag} {ms2 * 1e3:.1f} us, {gbs2:.0f} GB/s", file=sys.stderr, flush=True)
+        # the same matrix as CSC (y = A x scatters) and COO (row-major from CSR, and
+        # column-major from CSC), f64/i64; bytes as the survey's formulas
+        A = G.GenericSparseMatrixCSR(H.m, H.n, torch.as_tensor(H.rowptr).to(dev),
+                                     torch.as_tensor(H.colval).to(dev),
+                                     torch.as_tensor(H.nzval).to(dev), copy=False)
+        xd = torch.as_tensor(x_h).to(dev)
+        yd = torch.empty(H.m, dtype=torch.float64, device=dev)
+        Ac = G.GenericSparseMatrixCSC(A)
+        for tag, M, byts in (
+                ("spmv_csc_f64_i64", Ac, spmv_bytes(H.m, H.n, H.nnz, 8, 8) - (H.m + 1) * 8 + (H.n + 1) * 8),
+                ("spmv_coo_rowmajor_f64_i64", G.GenericSparseMatrixCOO(A), H.nnz * 24 + (H.n + H.m) * 8),
+                ("spmv_coo_colmajor_f64_i64", G.GenericSparseMatrixCOO(Ac), H.nnz * 24 + (H.n + H.m) * 8)):
+            if hasattr(M, "plan"):
+                M.plan()
+            ms4 = time_steps(lambda i: G.mul_(yd, M, xd, 1.0, 0.0), 20, 3, stream)
+            extra[tag + "_us"] = round(ms4 * 1e3, 1)
+            extra[tag + "_GBs"] = round(byts / (ms4 * 1e-3) / 1e9, 1)
+            print(f"[bench] {tag} {ms4 * 1e3:.1f} us", file=sys.stderr, flush=True)
+            del M
+        del A, Ac, xd, yd
+        torch.cuda.empty_cache()
+        # secondary point: Zipf alpha = 2.0 (mean 5.53 entries per row)
+        H2 = synth.powerlaw_csr(args.rows, args.rows, alpha=2.0, seed=2)
+        A2 = G.GenericSparseMatrixCSR(H2.m, H2.n, torch.as_tensor(H2.rowptr).to(dev),
+                                      torch.as_tensor(H2.colval).to(dev),
+                                      torch.as_tensor(H2.nzval).to(dev), copy=False)
+        x2 = torch.as_tensor(x_h).to(dev)
+        y2 = torch.empty(H2.m, dtype=torch.float64, device=dev)
+        A2.plan()
+        ms5 = time_steps(lambda i: G.mul_(y2, A2, x2, 1.0, 0.0), 50, 5, stream)
+        extra["spmv_alpha2_f64_i64_nnz"] = H2.nnz
+        extra["spmv_alpha2_f64_i64_us"] = round(ms5 * 1e3, 1)
+        extra["spmv_alpha2_f64_i64_GBs"] = round(spmv_bytes(H2.m, H2.n, H2.nnz, 8, 8) / (ms5 * 1e-3) / 1e9, 1)
+        print(f"[bench] alpha=2.0 SpMV {ms5 * 1e3:.1f} us", file=sys.stderr, flush=True)
+        del A2, x2, y2, H2
+        torch.cuda.empty_cache()
         # SpMM with a dense 1M x 32 right-hand side (column-major, as Julia's Matrix)
         k = 32
         A = G.GenericSparseMatrixCSR(H.m, H.n, torch.as_tensor(H.rowptr).to(dev),

@@ -46,33 +46,46 @@ int expand_rows(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* pt
 }
 
 // keys = (major-1)*minor_dim + (minor-1), payload = position
-template <typename I, typename V>
+template <typename I, typename K, typename V>
 __global__ void k_make_keys(int64_t nnz, const I* __restrict__ major, const I* __restrict__ minor,
-                            int64_t minor_dim, int64_t major_dim, uint64_t* __restrict__ keys,
+                            int64_t minor_dim, int64_t major_dim, K* __restrict__ keys,
                             V* __restrict__ pos) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
          i += (int64_t)gridDim.x * blockDim.x) {
         int64_t a = dclamp((int64_t)major[i] - 1, 0, major_dim - 1);
         int64_t b = minor ? dclamp((int64_t)minor[i] - 1, 0, minor_dim - 1) : 0;
-        keys[i] = (uint64_t)a * (uint64_t)minor_dim + (uint64_t)b;
+        keys[i] = (K)((uint64_t)a * (uint64_t)minor_dim + (uint64_t)b);
         pos[i] = (V)i;
     }
 }
 
-// out_idx[i] = idx[perm[i]], out_val[i] = val[perm[i]]; counts[major[perm[i]]] += 1
+// out_idx[i] = idx[perm[i]], out_val[i] = val[perm[i]]
 template <typename T, typename I, typename V>
-__global__ void k_gather_count(int64_t nnz, const V* __restrict__ perm,
-                               const I* __restrict__ major, const I* __restrict__ minor,
-                               const T* __restrict__ val, int64_t major_dim,
-                               I* __restrict__ out_minor, T* __restrict__ out_val,
-                               I* __restrict__ counts /* ptr+1 */) {
+__global__ void k_gather(int64_t nnz, const V* __restrict__ perm, const I* __restrict__ minor,
+                         const T* __restrict__ val, I* __restrict__ out_minor,
+                         T* __restrict__ out_val) {
     for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
          i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t p = (int64_t)perm[i];
+        const int64_t p = (int64_t)perm[i];
         out_minor[i] = minor[p];
         out_val[i] = val[p];
-        int64_t r = dclamp((int64_t)major[p] - 1, 0, major_dim - 1);
-        atomic_inc(&counts[r]);
+    }
+}
+
+// One-based major pointer from the sorted keys (major = key / kdim): ptr[r]
+// is one plus the first position whose major is >= r — searchsortedfirst over
+// the sorted majors (conversions.jl:194-199).  Position i writes the pointers
+// of the majors in (major[i-1], major[i]], so every entry of ptr[0..major_dim]
+// is written once, with no atomics.
+template <typename I, typename K>
+__global__ void k_ptr_from_keys(int64_t nnz, const K* __restrict__ keys, uint64_t kdim,
+                                int64_t major_dim, I* __restrict__ ptr) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t prev = -1, cur = major_dim;
+        if (i > 0) prev = dclamp((int64_t)((uint64_t)keys[i - 1] / kdim), 0, major_dim - 1);
+        if (i < nnz) cur = dclamp((int64_t)((uint64_t)keys[i] / kdim), 0, major_dim - 1);
+        for (int64_t r = prev + 1; r <= cur; ++r) ptr[r] = (I)(i + 1);
     }
 }
 
@@ -84,38 +97,43 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
                               const T* val, I* ptr_out, I* idx_out, T* val_out) {
     GSA_CHECK(nnz == 0 || (major_dim > 0 && (minor_key == nullptr || minor_dim > 0)),
               GSA_ERR_INVALID, "stored entries in a matrix with a zero dimension");
-    GSA_TRY(zero_fill(s, ptr_out, sizeof(I) * (major_dim + 1)));
-    if (nnz > 0) {
-        const uint64_t maxkey = minor_key
-            ? (uint64_t)(major_dim > 0 ? major_dim - 1 : 0) * (uint64_t)(minor_dim > 0 ? minor_dim : 1) +
-                  (uint64_t)(minor_dim > 0 ? minor_dim - 1 : 0)
-            : (uint64_t)(major_dim > 0 ? major_dim - 1 : 0);
-        const int bits = bits_for(maxkey);
-        Scratch keys(s), perm(s);
-        GSA_TRY(keys.alloc(sizeof(uint64_t) * nnz));
-        const int64_t kdim = minor_key ? (minor_dim > 0 ? minor_dim : 1) : 1;
-        if (nnz < (int64_t)UINT32_MAX) {
-            GSA_TRY(perm.alloc(sizeof(uint32_t) * nnz));
-            hipLaunchKernelGGL((k_make_keys<I, uint32_t>), dim3(grid_for(nnz, 256)), dim3(256), 0, s,
-                               nnz, major, minor_key, kdim, major_dim, keys.as<uint64_t>(),
-                               perm.as<uint32_t>());
-            GSA_TRY(radix_sort_pairs_u64_u32(s, keys.as<uint64_t>(), perm.as<uint32_t>(), nnz, bits));
-            hipLaunchKernelGGL((k_gather_count<T, I, uint32_t>), dim3(grid_for(nnz, 256)), dim3(256),
-                               0, s, nnz, (const uint32_t*)perm.as<uint32_t>(), major, minor_out,
-                               val, major_dim, idx_out, val_out, ptr_out + 1);
-        } else {
-            GSA_TRY(perm.alloc(sizeof(uint64_t) * nnz));
-            hipLaunchKernelGGL((k_make_keys<I, uint64_t>), dim3(grid_for(nnz, 256)), dim3(256), 0, s,
-                               nnz, major, minor_key, kdim, major_dim, keys.as<uint64_t>(),
-                               perm.as<uint64_t>());
-            GSA_TRY(radix_sort_pairs_u64(s, keys.as<uint64_t>(), perm.as<uint64_t>(), nnz, bits));
-            hipLaunchKernelGGL((k_gather_count<T, I, uint64_t>), dim3(grid_for(nnz, 256)), dim3(256),
-                               0, s, nnz, (const uint64_t*)perm.as<uint64_t>(), major, minor_out,
-                               val, major_dim, idx_out, val_out, ptr_out + 1);
-        }
-        GSA_HIP(hipGetLastError());
+    if (nnz == 0) {  // every pointer is 1 (conversions.jl:178-184, 289-295)
+        GSA_TRY(zero_fill(s, ptr_out, sizeof(I) * (major_dim + 1)));
+        return counts_to_ptr<I>(s, ptr_out, major_dim, nullptr);
     }
-    return counts_to_ptr<I>(s, ptr_out, major_dim, nullptr);
+    const uint64_t maxkey = minor_key
+        ? (uint64_t)(major_dim - 1) * (uint64_t)minor_dim + (uint64_t)(minor_dim - 1)
+        : (uint64_t)(major_dim - 1);
+    const int bits = bits_for(maxkey);
+    const uint64_t kdim = minor_key ? (uint64_t)minor_dim : 1;
+    const unsigned g = grid_for(nnz, 256);
+    Scratch keys(s), perm(s);
+    auto run = [&](auto key_t, auto pos_t, auto sorter) -> int {
+        using K = decltype(key_t);
+        using V = decltype(pos_t);
+        GSA_TRY(keys.alloc(sizeof(K) * nnz));
+        GSA_TRY(perm.alloc(sizeof(V) * nnz));
+        hipLaunchKernelGGL((k_make_keys<I, K, V>), dim3(g), dim3(256), 0, s, nnz, major, minor_key,
+                           (int64_t)kdim, major_dim, keys.as<K>(), perm.as<V>());
+        GSA_TRY(sorter(keys.as<K>(), perm.as<V>()));
+        hipLaunchKernelGGL((k_gather<T, I, V>), dim3(g), dim3(256), 0, s, nnz,
+                           (const V*)perm.as<V>(), minor_out, val, idx_out, val_out);
+        hipLaunchKernelGGL((k_ptr_from_keys<I, K>), dim3(grid_for(nnz + 1, 256)), dim3(256), 0, s,
+                           nnz, (const K*)keys.as<K>(), kdim, major_dim, ptr_out);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    };
+    if (bits <= 32 && nnz < (int64_t)UINT32_MAX)  // 32-bit keys and positions
+        return run(uint32_t{}, uint32_t{}, [&](uint32_t* k, uint32_t* p) {
+            return radix_sort_pairs_u32_u32(s, k, p, nnz, bits);
+        });
+    if (nnz < (int64_t)UINT32_MAX)
+        return run(uint64_t{}, uint32_t{}, [&](uint64_t* k, uint32_t* p) {
+            return radix_sort_pairs_u64_u32(s, k, p, nnz, bits);
+        });
+    return run(uint64_t{}, uint64_t{}, [&](uint64_t* k, uint64_t* p) {
+        return radix_sort_pairs_u64(s, k, p, nnz, bits);
+    });
 }
 
 }  // namespace gsa

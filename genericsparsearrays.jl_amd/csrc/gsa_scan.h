@@ -19,10 +19,11 @@ template <typename I>
 int counts_to_ptr(hipStream_t s, I* ptr, int64_t m, int64_t* total);
 
 // Stable LSD radix sort of (key, value) pairs on the low `bits` bits of the
-// unsigned 64-bit keys; values are 32- or 64-bit payloads.  keys/vals are
+// unsigned 32- or 64-bit keys; values are 32- or 64-bit payloads.  keys/vals are
 // sorted in place (double-buffered through scratch).
 int radix_sort_pairs_u64(hipStream_t s, uint64_t* keys, uint64_t* vals, int64_t n, int bits);
 int radix_sort_pairs_u64_u32(hipStream_t s, uint64_t* keys, uint32_t* vals, int64_t n, int bits);
+int radix_sort_pairs_u32_u32(hipStream_t s, uint32_t* keys, uint32_t* vals, int64_t n, int bits);
 
 inline int bits_for(uint64_t maxval) {
     int b = 0;

@@ -624,59 +624,114 @@ __device__ __forceinline__ void atomic_add(T* p, T v) {
 
 // Scatter: for every stored (row, col, v) of the CSR arrays,
 // C[col, kk] += v * (B[row, kk] * α)  (kernel_spmatmul_csr_T!).
+// Scatter on CSR arrays (CSR-T / Adjoint, CSC-N): C[col, kk] += v * (B[row, kk] * α)
+// with hardware float atomics, as the reference's @atomic kernels
+// (matrix_csr_kernels.jl:24-45, src/matrix_csc/matrix_csc_kernels.jl:1-22).
+// It runs on the SpMV windows: wavefront w streams the entries [576w, 576(w+1))
+// with coalesced loads, and each lane finds its entry's row by a binary search
+// over the window's row offsets (in LDS): the row of the entries before the
+// first owned row is the one started in an earlier window.
 template <typename T, typename I>
-__global__ __launch_bounds__(WG) void k_csr_scatter(
+__global__ __launch_bounds__(64 * WAVES) void k_csr_scatter(
     int64_t m, int64_t n, int64_t k, int64_t nnz, const I* __restrict__ rowptr,
     const I* __restrict__ colval, const T* __restrict__ nzval, const T* __restrict__ B,
     int64_t ldb, int opB, T* __restrict__ C, int64_t ldc, T alpha,
-    const int64_t* __restrict__ blk_row) {
-    const int tid = threadIdx.x;
-    const int64_t b = blockIdx.x;
-    const int64_t r0 = clamp64(blk_row[b], 0, m);
-    const int64_t r1 = clamp64(blk_row[b + 1], r0, m);
-    if (r0 >= r1) return;
-    // rows [r0, r1-1): one lane per row; last row: whole workgroup
-    for (int64_t kk = 0; kk < k; ++kk) {
-        for (int64_t r = r0 + tid; r < r1 - 1; r += WG) {
-            T xb = opB ? B[kk + r * ldb] : B[r + kk * ldb];
-            T axj = xb * alpha;
-            int64_t s = clamp64((int64_t)rowptr[r] - 1, 0, nnz);
-            int64_t e = clamp64((int64_t)rowptr[r + 1] - 1, s, nnz);
-            for (int64_t j = s; j < e; ++j) {
-                int64_t c = clamp64((int64_t)colval[j] - 1, 0, n - 1);
-                atomic_add(&C[c + kk * ldc], nzval[j] * axj);
-            }
+    const int64_t* __restrict__ win_row, int64_t nwin) {
+    __shared__ int s_off[WAVES][WRP];
+    const int lane = threadIdx.x & 63;
+    const int wv = threadIdx.x >> 6;
+    const int64_t w = (int64_t)blockIdx.x * WAVES + wv;
+    if (w >= nwin) return;
+    const int64_t E0 = w * SPMV_WIN < nnz ? w * SPMV_WIN : nnz;
+    const int64_t E1 = E0 + SPMV_WIN < nnz ? E0 + SPMV_WIN : nnz;
+    const int cnt = (int)(E1 - E0);
+    if (cnt <= 0) return;
+    const int64_t r0 = clamp64(win_row[w], 0, m);
+    const int64_t r1 = clamp64(win_row[w + 1], r0, m);
+    const int64_t nrows = r1 - r0;
+    const bool rp_in_lds = nrows + 1 <= WRP;
+    int* off = s_off[wv];
+    if (rp_in_lds) {
+        for (int i = lane; i <= (int)nrows; i += 64)
+            off[i] = (int)(clamp64((int64_t)rowptr[r0 + i] - 1, E0, E1) - E0);
+    }
+    wave_lds_sync();
+    // zero-based row of window entry i (E0 + i)
+    auto row_of = [&](int i) -> int64_t {
+        int64_t lo = 0, hi = nrows;  // largest t in [0, nrows] with start(r0 + t) <= E0 + i
+        const int64_t first = rp_in_lds ? off[0]
+                                        : clamp64((int64_t)rowptr[r0] - 1, E0, E1) - E0;
+        if (nrows == 0 || i < first) return r0 - 1;  // started in an earlier window
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            const int64_t st = rp_in_lds ? off[mid]
+                                         : clamp64((int64_t)rowptr[r0 + mid] - 1, E0, E1) - E0;
+            if (st <= i) lo = mid;
+            else hi = mid - 1;
         }
-        const int64_t r = r1 - 1;
-        T xb = opB ? B[kk + r * ldb] : B[r + kk * ldb];
-        T axj = xb * alpha;
-        int64_t s = clamp64((int64_t)rowptr[r] - 1, 0, nnz);
-        int64_t e = clamp64((int64_t)rowptr[r + 1] - 1, s, nnz);
-        for (int64_t j = s + tid; j < e; j += WG) {
-            int64_t c = clamp64((int64_t)colval[j] - 1, 0, n - 1);
-            atomic_add(&C[c + kk * ldc], nzval[j] * axj);
+        return r0 + lo;
+    };
+#pragma unroll
+    for (int u = 0; u < WU; ++u) {
+        const int i = u * 64 + lane;
+        if (i >= cnt) continue;
+        const int64_t j = E0 + i;
+        const int64_t c = clamp64((int64_t)ldnt(colval + j) - 1, 0, n - 1);
+        const T v = ldnt(nzval + j);
+        const int64_t r = clamp64(row_of(i), 0, m - 1);
+        for (int64_t kk = 0; kk < k; ++kk) {
+            const T xb = opB ? B[kk + r * ldb] : B[r + kk * ldb];
+            atomic_add(&C[c + kk * ldc], v * (xb * alpha));
         }
     }
 }
 
 // COO scatter: C[r, kk] += v * (B[c, kk] * α); trans swaps the roles of r and c.
+// Consecutive lanes adding into the same output (a row-major COO gives runs of
+// equal rows) are first summed across the wavefront by a segmented scan, so
+// one atomic per run reaches memory instead of one per entry.
 template <typename T, typename I>
 __global__ void k_coo_scatter(int64_t rows_out, int64_t cols_in, int64_t k, int64_t nnz,
                               const I* __restrict__ rowind, const I* __restrict__ colind,
                               const T* __restrict__ nzval, int trans,
                               const T* __restrict__ B, int64_t ldb, int opB,
                               T* __restrict__ C, int64_t ldc, T alpha) {
-    int64_t total = nnz * k;
-    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < total;
-         t += (int64_t)gridDim.x * blockDim.x) {
-        int64_t i = t % nnz, kk = t / nnz;
-        int64_t r = (int64_t)(trans ? colind[i] : rowind[i]) - 1;
-        int64_t c = (int64_t)(trans ? rowind[i] : colind[i]) - 1;
-        r = clamp64(r, 0, rows_out - 1);
-        c = clamp64(c, 0, cols_in - 1);
-        T xb = opB ? B[kk + c * ldb] : B[c + kk * ldb];
-        T axj = xb * alpha;
-        atomic_add(&C[r + kk * ldc], nzval[i] * axj);
+    const int lane = threadIdx.x & 63;
+    const int64_t total = nnz * k;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    // whole wavefronts iterate together (the scan needs every lane)
+    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + (threadIdx.x & ~63);
+    for (int64_t tb = t0; tb < total; tb += stride) {
+        const int64_t t = tb + lane;
+        const bool valid = t < total;
+        int64_t key = -1 - lane;  // distinct for invalid lanes: never joins a run
+        T val = T(0);
+        if (valid) {
+            const int64_t i = t % nnz, kk = t / nnz;
+            int64_t r = (int64_t)(trans ? colind[i] : rowind[i]) - 1;
+            int64_t c = (int64_t)(trans ? rowind[i] : colind[i]) - 1;
+            r = clamp64(r, 0, rows_out - 1);
+            c = clamp64(c, 0, cols_in - 1);
+            const T xb = opB ? B[kk + c * ldb] : B[c + kk * ldb];
+            val = nzval[i] * (xb * alpha);
+            key = r + kk * ldc;
+        }
+        // segmented inclusive scan over runs of equal keys (head = key differs
+        // from the previous lane's)
+        const int64_t kprev = __shfl_up(key, 1, 64);
+        int head = (lane == 0 || kprev != key) ? 1 : 0;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const T vu = __shfl_up(val, o, 64);
+            const int hu = __shfl_up(head, o, 64);
+            if (lane >= o && !head) {
+                val = val + vu;
+                head = hu;
+            }
+        }
+        const int64_t knext = __shfl_down(key, 1, 64);
+        const bool tail = lane == 63 || knext != key;
+        if (valid && tail) atomic_add(&C[key], val);
     }
 }
 
@@ -827,9 +882,10 @@ static int scatter_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* 
     GSA_TRY(apply_beta<T>(s, n, k, C, ldc, beta));
     if (m == 0 || nnz == 0) return GSA_OK;
     PlanUse pu(s);
-    GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr, false));
-    hipLaunchKernelGGL((k_csr_scatter<T, I>), dim3((unsigned)pu.nblocks), dim3(WG), 0, s, m, n, k,
-                       nnz, rowptr, colval, nzval, B, ldb, opB, C, ldc, (T)alpha, pu.blk_row);
+    GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr, true));
+    hipLaunchKernelGGL((k_csr_scatter<T, I>), dim3((unsigned)cdiv(pu.nwin, WAVES)), dim3(64 * WAVES),
+                       0, s, m, n, k, nnz, rowptr, colval, nzval, B, ldb, opB, C, ldc, (T)alpha,
+                       pu.win_row, pu.nwin);
     GSA_HIP(hipGetLastError());
     return GSA_OK;
 }

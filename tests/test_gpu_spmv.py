@@ -290,3 +290,65 @@ def test_bf16_f32_rows_across_windows(dev):
     deq = B.dequantized_values()
     refb = O.spmm_csr_N(H.m, H.rowptr, H.colval, deq, x.astype(np.float64))
     assert_normwise(host(yb), refb, norm_bound(H.rowptr, H.colval, deq, x), 2e-6)
+
+
+# ---- scatter-shaped products (CSR-T, CSC-N, COO) ------------------------------
+# The reference adds with @atomic, so its order is undefined: tolerance only.
+
+@pytest.mark.parametrize("lengths", [
+    [5000] + [1] * 3000,
+    [1] * 3000,                        # windows owning more than 256 rows (global search)
+    [0] * 5000 + [7] + [0] * 5000,
+    [575, 1, 576, 33, 1152, 0, 32, 31, 1153, 577],
+    [100_000],
+])
+@pytest.mark.parametrize("k", [1, 3])
+def test_transposed_scatter_windows(dev, lengths, k):
+    rng = np.random.default_rng(31)
+    H = synth.csr_from_lengths(np.array(lengths), 7_000, rng, "normal")
+    vec = lambda a: a[:, 0] if k == 1 else a  # keeps a length-1 vector a vector
+    xT = vec(rng.standard_normal((H.m, k)))
+    A = dev_csr(H)
+    Ac = dev_csc_of(H)
+    d = torch.as_tensor(xT, device=dev) if k == 1 else G.dense(xT, device=dev)
+    ref = O.spmm_csr_T(H.m, H.n, H.rowptr, H.colval, H.nzval, xT)
+    bound = O.spmm_csr_T(H.m, H.n, H.rowptr, H.colval, np.abs(H.nzval), np.abs(xT))
+    assert_normwise(host(G.transpose(A) @ d), ref, bound, 1e-12)
+    xN = vec(rng.standard_normal((H.n, k)))
+    dN = torch.as_tensor(xN, device=dev) if k == 1 else G.dense(xN, device=dev)
+    refN = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, xN)
+    bN = O.spmm_csr_N(H.m, H.rowptr, H.colval, np.abs(H.nzval), np.abs(xN))
+    assert_normwise(host(Ac @ dN), refN, bN, 1e-12)  # CSC-N scatters on the column arrays
+
+
+@pytest.mark.parametrize("order", ["rowmajor", "shuffled"])
+@pytest.mark.parametrize("k", [1, 4])
+def test_coo_runs_duplicates_and_order(dev, order, k):
+    """COO with long runs of one row, duplicate (row, col) pairs and any order:
+    duplicates are summed (matrix_coo_kernels.jl:1-43)."""
+    rng = np.random.default_rng(32)
+    m, n = 3000, 2000
+    lens = np.r_[[3000, 700], rng.integers(0, 12, m - 2)]
+    rows = np.repeat(np.arange(1, m + 1), lens)
+    cols = rng.integers(1, n + 1, rows.size)
+    dup = rng.choice(rows.size, rows.size // 10, replace=False)
+    rows = np.r_[rows, rows[dup]]
+    cols = np.r_[cols, cols[dup]]
+    if order == "rowmajor":
+        o = np.argsort(rows, kind="stable")
+    else:
+        o = rng.permutation(rows.size)
+    rows, cols = rows[o].astype(np.int64), cols[o].astype(np.int64)
+    vals = rng.standard_normal(rows.size)
+    A = G.GenericSparseMatrixCOO(m, n, torch.as_tensor(rows, device=dev),
+                                 torch.as_tensor(cols, device=dev), torch.as_tensor(vals, device=dev))
+    x = rng.standard_normal((n, k)).squeeze()
+    d = torch.as_tensor(x, device=dev) if k == 1 else G.dense(x, device=dev)
+    ref = O.spmm_coo(0, m, rows, cols, vals, x)
+    bound = O.spmm_coo(0, m, rows, cols, np.abs(vals), np.abs(x))
+    assert_normwise(host(A @ d), ref, bound, 1e-12)
+    xt = rng.standard_normal((m, k)).squeeze()
+    dt = torch.as_tensor(xt, device=dev) if k == 1 else G.dense(xt, device=dev)
+    refT = O.spmm_coo(1, n, rows, cols, vals, xt)
+    bT = O.spmm_coo(1, n, rows, cols, np.abs(vals), np.abs(xt))
+    assert_normwise(host(G.transpose(A) @ dt), refT, bT, 1e-12)
