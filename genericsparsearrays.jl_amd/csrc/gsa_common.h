@@ -138,6 +138,7 @@ struct gsa_ctx_s {
     hipStream_t stream = nullptr;
     int vtype = 0, itype = 0;
     int64_t m = 0, n = 0, nnz_out = 0;
-    void* buf[8] = {nullptr};
+    void* buf[8] = {nullptr};     // scratch owned by the context (freed with it)
     int64_t aux[8] = {0};
+    const void* in[8] = {nullptr}; // caller arrays read again by the second phase
 };

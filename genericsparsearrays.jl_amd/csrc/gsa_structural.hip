@@ -346,6 +346,260 @@ __global__ void k_sg_write(int64_t nnzC, int64_t n, const uint64_t* __restrict__
     }
 }
 
+// ---- row-local SpGEMM ------------------------------------------------------
+// Two phases over the same per-row work.  Output row r's products are
+// generated in the reference's order (A-row entries in order, each B row in
+// order: kernel_spgemm_csr!, matrix_csr_kernels.jl:293-343) and numbered p;
+// the key (column << 24 | p) is sorted inside a wavefront (rows of at most
+// SG_WAVE_P products, bitonic across lanes) or a workgroup (at most
+// SG_BLOCK_P, bitonic in LDS).  Equal columns then lie together in generation
+// order, so each output entry is the left-to-right sum from +0.0 of its
+// products, cancellation zeros kept.  The symbolic phase counts the distinct
+// columns of every row; the numeric phase writes columns and values at the row
+// pointer.  A product with a row over SG_BLOCK_P products uses the global
+// expand-sort-compress path below instead.
+constexpr int SG_WAVE_P = 64;
+constexpr int SG_BLOCK_P = 2048;
+constexpr int SG_WAVES = 4;
+constexpr uint64_t SG_NONE = ~0ull;
+
+__device__ __forceinline__ uint64_t sg_key(int64_t col, int p) {
+    return ((uint64_t)col << 24) | (uint64_t)p;
+}
+__device__ __forceinline__ int64_t sg_col(uint64_t k) { return (int64_t)(k >> 24); }
+
+// Products of output row r: one-based exclusive prefix pp (m+1 entries).
+__device__ __forceinline__ int64_t sg_products(const int64_t* __restrict__ pp, int64_t r) {
+    return pp[r + 1] - pp[r];
+}
+
+// Rows with at most G products (and more than G/2, or any for the smallest
+// group): groups of G lanes, 64/G rows per wavefront.  Scans, the bitonic sort
+// and the run sums stay inside a group.
+template <bool NUM, int G, typename T, typename I>
+__global__ __launch_bounds__(64 * SG_WAVES) void k_sg_row_wave(
+    int64_t m, int64_t n, int64_t kdim, int64_t nnzA, int64_t nnzB, const I* __restrict__ pA,
+    const I* __restrict__ cA, const T* __restrict__ vA, const I* __restrict__ pB,
+    const I* __restrict__ cB, const T* __restrict__ vB, const int64_t* __restrict__ pp,
+    int min_p, I* __restrict__ rowcnt, const I* __restrict__ rowptrC, I* __restrict__ cC,
+    T* __restrict__ vC) {
+    constexpr int RPW = 64 / G;  // rows per wavefront
+    __shared__ T s_val[SG_WAVES][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int gl = lane & (G - 1);        // lane inside the group
+    const int gbase = lane & ~(G - 1);    // first lane of the group
+    const int64_t r = ((int64_t)blockIdx.x * SG_WAVES + wv) * RPW + (lane / G);
+    // a group with no row of its size class sits out the collective steps
+    int64_t P = 0;
+    if (r < m) P = sg_products(pp, r);
+    const bool mine = r < m && P > min_p && P <= G;
+    if (__ballot(mine) == 0) return;
+    uint64_t key = SG_NONE;
+    T val = T(0);
+    if (mine) {
+        const int64_t a0 = dclamp((int64_t)pA[r] - 1, 0, nnzA);
+        const int64_t a1 = dclamp((int64_t)pA[r + 1] - 1, a0, nnzA);
+        int base = 0;  // products generated by earlier chunks of A entries
+        for (int64_t ac = a0; ac < a1 && base < P; ac += G) {
+            const int64_t a = ac + gl;
+            const bool va = a < a1;
+            int64_t bs = 0, bl = 0;
+            if (va) {
+                const int64_t k = dclamp((int64_t)cA[a] - 1, 0, kdim - 1);
+                bs = dclamp((int64_t)pB[k] - 1, 0, nnzB);
+                bl = dclamp((int64_t)pB[k + 1] - 1, bs, nnzB) - bs;
+            }
+            int incl = (int)bl;
+#pragma unroll
+            for (int o = 1; o < G; o <<= 1) {
+                const int y = __shfl_up(incl, o, G);
+                if (gl >= o) incl += y;
+            }
+            const int chunkP = __shfl(incl, G - 1, G);
+            const int q = gl - base;  // this lane's product inside the chunk
+            int lo = 0;
+#pragma unroll
+            for (int step = G / 2; step >= 1; step >>= 1) {
+                const int im = __shfl(incl, lo + step - 1, G);
+                if (im <= q) lo += step;
+            }
+            const int e = lo < G - 1 ? lo : G - 1;
+            const int64_t ea = __shfl(a, e, G);
+            const int64_t ebs = __shfl(bs, e, G);
+            const int eex = __shfl(incl, e, G) - (int)__shfl(bl, e, G);
+            if (q >= 0 && q < chunkP) {
+                const int64_t b = dclamp(ebs + (q - eex), 0, nnzB - 1);
+                const int64_t col = dclamp((int64_t)cB[b] - 1, 0, n - 1);
+                key = sg_key(col, gl);
+                if (NUM) val = vA[dclamp(ea, 0, nnzA - 1)] * vB[b];
+            }
+            base += chunkP;
+        }
+    }
+    // bitonic sort of (key, val) inside each group, ascending
+#pragma unroll
+    for (int k = 2; k <= G; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const uint64_t ok = __shfl_xor(key, j, G);
+            const T ov = __shfl_xor(val, j, G);
+            const bool up = (gl & k) == 0;
+            const bool lower = (gl & j) == 0;
+            const bool take = (lower == up) ? (ok < key) : (ok > key);
+            if (take) { key = ok; val = ov; }
+        }
+    }
+    const bool valid = mine && gl < P;
+    const int64_t col = sg_col(key);
+    const int64_t pcol = __shfl_up(col, 1, G);
+    const bool head = valid && (gl == 0 || pcol != col);
+    const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << gbase);
+    const uint64_t heads = __ballot(head) & gmask;
+    if (!NUM) {
+        if (mine && gl == 0) rowcnt[r + 1] = (I)__popcll(heads);
+        return;
+    }
+    s_val[wv][lane] = val;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    if (head) {
+        const uint64_t later = lane == 63 ? 0ull : (heads & (~0ull << (lane + 1)));
+        const int stop = later ? (__ffsll((long long)later) - 1) : gbase + (int)P;
+        T acc = T(0);
+        for (int j = lane; j < stop; ++j) acc = acc + s_val[wv][j];
+        const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+        const int64_t o = (int64_t)rowptrC[r] - 1 + __popcll(heads & lt);
+        cC[o] = (I)(col + 1);
+        vC[o] = acc;
+    }
+}
+
+// Launch the grouped row kernels for the size classes (0,16], (16,32], (32,64].
+template <bool NUM, typename T, typename I>
+static void sg_rows_small(hipStream_t s, int64_t m, int64_t n, int64_t kdim, int64_t nnzA,
+                          int64_t nnzB, const I* pA, const I* cA, const T* vA, const I* pB,
+                          const I* cB, const T* vB, const int64_t* pp, I* rowcnt,
+                          const I* rowptrC, I* cC, T* vC) {
+    hipLaunchKernelGGL((k_sg_row_wave<NUM, 16, T, I>), dim3((unsigned)cdiv(m, SG_WAVES * 4)),
+                       dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
+                       pp, 0, rowcnt, rowptrC, cC, vC);
+    hipLaunchKernelGGL((k_sg_row_wave<NUM, 32, T, I>), dim3((unsigned)cdiv(m, SG_WAVES * 2)),
+                       dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
+                       pp, 16, rowcnt, rowptrC, cC, vC);
+    hipLaunchKernelGGL((k_sg_row_wave<NUM, 64, T, I>), dim3((unsigned)cdiv(m, SG_WAVES)),
+                       dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
+                       pp, 32, rowcnt, rowptrC, cC, vC);
+}
+
+template <bool NUM, typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_sg_row_block(
+    int64_t n, int64_t kdim, int64_t nnzA, int64_t nnzB, const I* __restrict__ pA,
+    const I* __restrict__ cA, const T* __restrict__ vA, const I* __restrict__ pB,
+    const I* __restrict__ cB, const T* __restrict__ vB, const int64_t* __restrict__ pp,
+    const int64_t* __restrict__ rows, I* __restrict__ rowcnt, const I* __restrict__ rowptrC,
+    I* __restrict__ cC, T* __restrict__ vC) {
+    __shared__ uint64_t sk[SG_BLOCK_P];
+    __shared__ T sv[SG_BLOCK_P];
+    __shared__ int64_t lds4[ROW_WG / 64];
+    const int tid = threadIdx.x;
+    const int64_t r = rows[blockIdx.x];
+    const int P = (int)sg_products(pp, r);  // 64 < P <= SG_BLOCK_P (the row list holds these)
+    int n2 = 128;
+    while (n2 < P) n2 <<= 1;
+    const int64_t a0 = dclamp((int64_t)pA[r] - 1, 0, nnzA);
+    const int64_t a1 = dclamp((int64_t)pA[r + 1] - 1, a0, nnzA);
+    // 1. products in generation order: A entries in chunks of ROW_WG threads
+    int64_t base = 0;
+    for (int64_t ac = a0; ac < a1; ac += ROW_WG) {
+        const int64_t a = ac + tid;
+        int64_t bs = 0, bl = 0;
+        if (a < a1) {
+            const int64_t k = dclamp((int64_t)cA[a] - 1, 0, kdim - 1);
+            bs = dclamp((int64_t)pB[k] - 1, 0, nnzB);
+            bl = dclamp((int64_t)pB[k + 1] - 1, bs, nnzB) - bs;
+        }
+        int64_t chunk = 0;
+        const int64_t ex = wg_exclusive_sum<int64_t>(bl, lds4, &chunk);
+        if (a < a1) {
+            const T va = NUM ? vA[a] : T(0);
+            for (int64_t j = 0; j < bl; ++j) {
+                const int64_t p = base + ex + j;
+                if (p >= P) break;
+                const int64_t col = dclamp((int64_t)cB[bs + j] - 1, 0, n - 1);
+                sk[p] = sg_key(col, (int)p);
+                if (NUM) sv[p] = va * vB[bs + j];
+            }
+        }
+        base += chunk;
+    }
+    for (int p = P + tid; p < n2; p += ROW_WG) sk[p] = SG_NONE;
+    __syncthreads();
+    // 2. bitonic sort in LDS
+    for (int k = 2; k <= n2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int t = tid; t < n2 / 2; t += ROW_WG) {
+                const int i = 2 * t - (t & (j - 1));
+                const int l = i + j;
+                const bool asc = (i & k) == 0;
+                const uint64_t ki = sk[i], kl = sk[l];
+                if ((ki > kl) == asc) {
+                    sk[i] = kl; sk[l] = ki;
+                    if (NUM) { const T t2 = sv[i]; sv[i] = sv[l]; sv[l] = t2; }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // 3. heads: thread tid owns the contiguous elements [tid*per, (tid+1)*per)
+    const int per = SG_BLOCK_P / ROW_WG;
+    int64_t mine = 0;
+    for (int q = 0; q < per; ++q) {
+        const int i = tid * per + q;
+        if (i < P && (i == 0 || sg_col(sk[i]) != sg_col(sk[i - 1]))) ++mine;
+    }
+    int64_t total = 0;
+    int64_t rank = wg_exclusive_sum<int64_t>(mine, lds4, &total);
+    if (!NUM) {
+        if (tid == 0) rowcnt[r + 1] = (I)total;
+        return;
+    }
+    const int64_t off = (int64_t)rowptrC[r] - 1;
+    for (int q = 0; q < per; ++q) {
+        const int i = tid * per + q;
+        if (i >= P) break;
+        const int64_t col = sg_col(sk[i]);
+        if (i != 0 && col == sg_col(sk[i - 1])) continue;
+        T acc = T(0);
+        for (int j = i; j < P && sg_col(sk[j]) == col; ++j) acc = acc + sv[j];
+        cC[off + rank] = (I)(col + 1);
+        vC[off + rank] = acc;
+        ++rank;
+    }
+}
+
+// Products over SG_WAVE_P per row -> the row list of the workgroup kernel.
+__global__ void k_sg_flags(int64_t m, const int64_t* __restrict__ pp, int64_t* __restrict__ f,
+                           unsigned long long* __restrict__ maxp) {
+    __shared__ unsigned long long bmax;
+    if (threadIdx.x == 0) bmax = 0;
+    __syncthreads();
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r < m) {
+        const int64_t P = sg_products(pp, r);
+        f[r] = P > SG_WAVE_P;
+        atomicMax(&bmax, (unsigned long long)P);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(maxp, bmax);
+}
+
+__global__ void k_sg_list(int64_t m, const int64_t* __restrict__ pp,
+                          const int64_t* __restrict__ incl, int64_t* __restrict__ rows) {
+    const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (r < m && sg_products(pp, r) > SG_WAVE_P) rows[incl[r] - 1] = r;
+}
+
 template <typename T, typename I>
 static int run_kron_coo(hipStream_t s, int64_t mB, int64_t nB, int64_t nnzA, const I* rA,
                         const I* cA, const T* vA, int64_t nnzB, const I* rB, const I* cB,
@@ -690,6 +944,65 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
         if ((r = counts_to_ptr<int64_t>(s, pptr.as<int64_t>(), m, &P)) != GSA_OK) return fail(r);
         c->aux[0] = P;
         if (P == 0) { *ctx = c; return (int)GSA_OK; }
+        // row-local path when every row's products fit one workgroup's sort
+        {
+            Scratch fl(s), mx(s);
+            if ((r = fl.alloc(sizeof(int64_t) * m)) != GSA_OK) return fail(r);
+            if ((r = mx.alloc(sizeof(unsigned long long))) != GSA_OK) return fail(r);
+            if ((r = zero_fill(s, mx.p, sizeof(unsigned long long))) != GSA_OK) return fail(r);
+            hipLaunchKernelGGL(k_sg_flags, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m,
+                               (const int64_t*)pptr.as<int64_t>(), fl.as<int64_t>(),
+                               (unsigned long long*)mx.p);
+            if ((r = inclusive_scan<int64_t>(s, fl.as<int64_t>(), fl.as<int64_t>(), m, 0)) != GSA_OK)
+                return fail(r);
+            int64_t h[2] = {0, 0};
+            hipError_t e = hipMemcpyAsync(&h[0], mx.p, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&h[1], fl.as<int64_t>() + (m - 1), sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess) e = hipStreamSynchronize(s);
+            if (e != hipSuccess) return fail(hip_fail(e, "gsa_spgemm_csr_begin"));
+            const int64_t maxP = h[0], nbig = h[1];
+            if (maxP <= SG_BLOCK_P && n < (int64_t)1 << 40) {
+                // keep the product pointer and the long-row list for the numeric phase
+                if ((r = ws_alloc(&c->buf[3], sizeof(int64_t) * (m + 1), s)) != GSA_OK) return fail(r);
+                GSA_HIP(hipMemcpyAsync(c->buf[3], pptr.p, sizeof(int64_t) * (m + 1),
+                                       hipMemcpyDeviceToDevice, s));
+                if (nbig > 0) {
+                    if ((r = ws_alloc(&c->buf[4], sizeof(int64_t) * nbig, s)) != GSA_OK) return fail(r);
+                    hipLaunchKernelGGL(k_sg_list, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m,
+                                       (const int64_t*)pptr.as<int64_t>(),
+                                       (const int64_t*)fl.as<int64_t>(), (int64_t*)c->buf[4]);
+                }
+                c->aux[1] = 1;  // row-local path
+                c->aux[2] = nbig;
+                c->aux[3] = kdim;
+                c->aux[4] = nnzA;
+                c->aux[5] = nnzB;
+                c->in[0] = rowptrA; c->in[1] = colA; c->in[2] = valA;
+                c->in[3] = rowptrB; c->in[4] = colB; c->in[5] = valB;
+                I* cnt = (I*)c->buf[0];
+                sg_rows_small<false, T, I>(s, m, n, kdim, nnzA, nnzB, (const I*)rowptrA,
+                                           (const I*)colA, (const T*)valA, (const I*)rowptrB,
+                                           (const I*)colB, (const T*)valB,
+                                           (const int64_t*)c->buf[3], cnt, (const I*)nullptr,
+                                           (I*)nullptr, (T*)nullptr);
+                if (nbig > 0)
+                    hipLaunchKernelGGL((k_sg_row_block<false, T, I>), dim3((unsigned)nbig), dim3(ROW_WG),
+                                       0, s, n, kdim, nnzA, nnzB, (const I*)rowptrA, (const I*)colA,
+                                       (const T*)valA, (const I*)rowptrB, (const I*)colB,
+                                       (const T*)valB, (const int64_t*)c->buf[3],
+                                       (const int64_t*)c->buf[4], cnt, (const I*)nullptr,
+                                       (I*)nullptr, (T*)nullptr);
+                if ((e = hipGetLastError()) != hipSuccess) return fail(hip_fail(e, "k_sg_row"));
+                int64_t total = 0;
+                if ((r = counts_to_ptr<I>(s, cnt, m, &total)) != GSA_OK) return fail(r);
+                c->nnz_out = total;
+                *nnzC = total;
+                *ctx = c;
+                return (int)GSA_OK;
+            }
+        }
         Scratch keys(s), prods(s), perm(s), sorted(s), flags(s);
         if ((r = keys.alloc(sizeof(uint64_t) * P)) != GSA_OK) return fail(r);
         if ((r = prods.alloc(sizeof(T) * P)) != GSA_OK) return fail(r);
@@ -739,6 +1052,28 @@ int gsa_spgemm_csr_finish(gsa_ctx ctx, void* rowptrC, void* colC, void* valC) {
         using I = decltype(ti);
         const int64_t m = ctx->m;
         GSA_HIP(hipMemcpyAsync(rowptrC, ctx->buf[0], sizeof(I) * (m + 1), hipMemcpyDeviceToDevice, s));
+        if (ctx->aux[1] == 1) {  // row-local: the pointer is final; write the rows
+            if (ctx->nnz_out > 0) {
+                const int64_t n = ctx->n, kdim = ctx->aux[3], nnzA = ctx->aux[4], nnzB = ctx->aux[5];
+                const I* pA = (const I*)ctx->in[0];
+                const I* cA = (const I*)ctx->in[1];
+                const T* vA = (const T*)ctx->in[2];
+                const I* pB = (const I*)ctx->in[3];
+                const I* cB = (const I*)ctx->in[4];
+                const T* vB = (const T*)ctx->in[5];
+                sg_rows_small<true, T, I>(s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
+                                          (const int64_t*)ctx->buf[3], (I*)nullptr,
+                                          (const I*)rowptrC, (I*)colC, (T*)valC);
+                if (ctx->aux[2] > 0)
+                    hipLaunchKernelGGL((k_sg_row_block<true, T, I>), dim3((unsigned)ctx->aux[2]),
+                                       dim3(ROW_WG), 0, s, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB,
+                                       vB, (const int64_t*)ctx->buf[3],
+                                       (const int64_t*)ctx->buf[4], (I*)nullptr, (const I*)rowptrC,
+                                       (I*)colC, (T*)valC);
+                GSA_HIP(hipGetLastError());
+            }
+            return (int)GSA_OK;
+        }
         GSA_TRY(counts_to_ptr<I>(s, (I*)rowptrC, m, nullptr));
         if (ctx->nnz_out > 0) {
             hipLaunchKernelGGL((k_sg_write<T, I>), dim3(grid_for(ctx->nnz_out, 256)), dim3(256), 0, s,

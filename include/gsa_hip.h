@@ -222,9 +222,13 @@ int gsa_kron_csr(void* stream, int vtype, int itype,
 /* ========================== SpGEMM ========================================= */
 /* C = A*B, A (m x kdim) and B (kdim x n) in CSR.  Row-wise Gustavson semantics of
  * kernel_count_nnz_spgemm_csr!/kernel_spgemm_csr! (src/matrix_csr/matrix_csr_kernels.jl:254-343):
- * per output entry the products are summed in (A-row order, B-row order),
- * columns ascending, cancellation zeros kept.  begin returns nnz(C)
- * (synchronises); finish fills (m+1, nnzC, nnzC) and releases ctx.
+ * per output entry the products are summed in (A-row order, B-row order) from
+ * +0.0, columns ascending, cancellation zeros kept.  Two phases: begin counts
+ * the distinct columns of every output row (symbolic) and returns nnz(C)
+ * (synchronises); finish writes the row pointer (m+1), columns and values
+ * (numeric) and releases ctx.  Each output row is sorted by column inside a
+ * wavefront or workgroup; a product with a row over 2048 partial products
+ * uses a global sort instead.  A and B must stay unchanged until finish.
  * Replaces Base.:*(::CSR, ::CSR) src/matrix_csr/matrix_csr.jl:498-572.
  * CSC * CSC (src/matrix_csc/matrix_csc.jl:500-574) is this call with the
  * operands swapped (B's column arrays first). */

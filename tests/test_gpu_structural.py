@@ -313,3 +313,29 @@ def test_spgemm_laplacian_full_size(dev):
     assert C.nnz() == 12_980_004
     same(csr_tuple(C), O.spgemm_csr(L.m, L.n, (L.rowptr, L.colval, L.nzval),
                                     (L.rowptr, L.colval, L.nzval)))
+
+
+@pytest.mark.parametrize("case", ["wave_and_block", "fallback_over_2048", "long_A_rows_empty_B",
+                                  "empty_rows"])
+def test_spgemm_row_paths(dev, case):
+    """Each output row is sorted in a wavefront (<= 64 products), a workgroup
+    (<= 2048), or the global path (any row over 2048): all bit-exact with the
+    oracle's Gustavson order."""
+    rng = np.random.default_rng(41)
+    if case == "wave_and_block":
+        H1 = synth.csr_from_lengths(np.r_[rng.integers(0, 6, 400), [30, 60, 64, 65, 90]], 300, rng, "normal")
+        H2 = synth.csr_from_lengths(rng.integers(0, 25, 300), 500, rng, "normal")
+    elif case == "fallback_over_2048":
+        H1 = synth.csr_from_lengths(np.r_[rng.integers(0, 5, 200), [150]], 300, rng, "normal")
+        H2 = synth.csr_from_lengths(rng.integers(10, 40, 300), 700, rng, "normal")
+    elif case == "long_A_rows_empty_B":
+        H1 = synth.csr_from_lengths(np.r_[[250, 200, 3], rng.integers(0, 4, 50)], 300, rng, "normal")
+        lens = np.zeros(300, dtype=np.int64)
+        lens[::37] = rng.integers(1, 9, lens[::37].size)
+        H2 = synth.csr_from_lengths(lens, 400, rng, "normal")
+    else:
+        H1 = synth.csr_from_lengths(np.r_[[0] * 50, [3, 0, 5], [0] * 50], 40, rng, "normal")
+        H2 = synth.csr_from_lengths(np.r_[[0] * 20, rng.integers(0, 6, 20)], 30, rng, "normal")
+    C = dev_csr(H1) * dev_csr(H2)
+    same(csr_tuple(C), O.spgemm_csr(H1.m, H2.n, (H1.rowptr, H1.colval, H1.nzval),
+                                    (H2.rowptr, H2.colval, H2.nzval)))
