@@ -125,12 +125,38 @@ int build_csr_plan(hipStream_t s, int itype, int64_t m, int64_t nnz, const void*
 
 }  // namespace gsa
 
-namespace gsa { void batch_plan_release(gsa_plan_s* plan); }
+namespace gsa {
+void batch_plan_release(gsa_plan_s* plan);
+
+// Row-ordered view of a COO or CSC matrix (the sorted-key form of its mul!):
+// the entries in row order as a CSR row pointer, and, unless the storage is
+// already in row order, each entry's column and storage position.
+struct RowView {
+    int itype = 0;
+    int64_t m = 0, nnz = 0;
+    void* rowptr = nullptr;    // I[m+1], one-based
+    void* col = nullptr;       // I[nnz] one-based columns, or null: storage is row-ordered
+    void* perm = nullptr;      // storage position of each entry (u32 or u64), or null
+    bool perm64 = false;
+    gsa_plan_s* win = nullptr; // SpMV windows over rowptr
+    const void* src[2] = {nullptr, nullptr};  // the arrays it was built from (checked on use)
+};
+
+// Build v (m rows, nnz entries) from the one-based row and column of every
+// stored entry, in storage order.  `col_is_temp`: colidx does not outlive the
+// call, so a row-ordered storage still gets its own column array.
+// Synchronises the stream.
+int build_row_view(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* rowidx,
+                   const void* colidx, bool col_is_temp, RowView& v);
+void release_row_view(RowView* v);
+int expand_rows(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr, void* out);
+}  // namespace gsa
 
 struct gsa_plan_s {
-    int kind = 0;  // 0 = CSR row blocks, 1 = batch schedule
+    int kind = 0;  // 0 = CSR row blocks, 1 = batch schedule, 2 = COO row view only
     gsa::CsrPlanData csr;
-    void* batch = nullptr;  // gsa::BatchPlan*
+    void* batch = nullptr;           // gsa::BatchPlan*
+    gsa::RowView* view = nullptr;    // row view (CSC: beside the column blocks)
 };
 
 struct gsa_ctx_s {

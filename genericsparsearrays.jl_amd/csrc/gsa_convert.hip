@@ -136,6 +136,110 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
     });
 }
 
+// ---- row views of COO / CSC storage (sorted-key SpMV) -------------------------
+template <typename I>
+__global__ void k_unsorted(int64_t nnz, const I* __restrict__ major, int* __restrict__ flag) {
+    for (int64_t i = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (major[i] < major[i - 1]) *flag = 1;
+}
+
+// One-based row pointer from non-decreasing one-based rows (as k_ptr_from_keys).
+template <typename I>
+__global__ void k_ptr_from_major(int64_t nnz, const I* __restrict__ major, int64_t major_dim,
+                                 I* __restrict__ ptr) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t prev = -1, cur = major_dim;
+        if (i > 0) prev = dclamp((int64_t)major[i - 1] - 1, 0, major_dim - 1);
+        if (i < nnz) cur = dclamp((int64_t)major[i] - 1, 0, major_dim - 1);
+        for (int64_t r = prev + 1; r <= cur; ++r) ptr[r] = (I)(i + 1);
+    }
+}
+
+template <typename I, typename V>
+__global__ void k_gather_idx(int64_t nnz, const V* __restrict__ perm, const I* __restrict__ in,
+                             I* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = in[(int64_t)perm[i]];
+}
+
+void release_row_view(RowView* v) {
+    if (!v) return;
+    if (v->win) gsa_plan_destroy(v->win);
+    for (void* q : {v->rowptr, v->col, v->perm})
+        if (q) (void)hipFree(q);
+    delete v;
+}
+
+int build_row_view(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* rowidx,
+                   const void* colidx, bool col_is_temp, RowView& v) {
+    v.itype = itype;
+    v.m = m;
+    v.nnz = nnz;
+    return dispatch_i(itype, [&](auto ti) -> int {
+        using I = decltype(ti);
+        GSA_HIP(hipMalloc(&v.rowptr, sizeof(I) * (m + 1)));
+        if (nnz == 0) {
+            GSA_TRY(zero_fill(s, v.rowptr, sizeof(I) * (m + 1)));
+            return counts_to_ptr<I>(s, (I*)v.rowptr, m, nullptr);
+        }
+        GSA_CHECK(m > 0, GSA_ERR_INVALID, "stored entries in a matrix with no rows");
+        const unsigned g = grid_for(nnz, 256);
+        Scratch flag(s);
+        GSA_TRY(flag.alloc(sizeof(int)));
+        GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
+        hipLaunchKernelGGL(k_unsorted<I>, dim3(g), dim3(256), 0, s, nnz, (const I*)rowidx,
+                           flag.as<int>());
+        int unsorted = 0;
+        GSA_HIP(hipMemcpyAsync(&unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        if (!unsorted) {  // storage already in row order: pointer only
+            hipLaunchKernelGGL(k_ptr_from_major<I>, dim3(grid_for(nnz + 1, 256)), dim3(256), 0, s,
+                               nnz, (const I*)rowidx, m, (I*)v.rowptr);
+            if (col_is_temp) {
+                GSA_HIP(hipMalloc(&v.col, sizeof(I) * nnz));
+                GSA_HIP(hipMemcpyAsync(v.col, colidx, sizeof(I) * nnz, hipMemcpyDeviceToDevice, s));
+            }
+            GSA_HIP(hipGetLastError());
+            return GSA_OK;
+        }
+        // stable sort of the storage positions by row
+        const int bits = bits_for((uint64_t)(m - 1));
+        auto sorted = [&](auto key_t, auto pos_t, auto sorter) -> int {
+            using K = decltype(key_t);
+            using V = decltype(pos_t);
+            Scratch keys(s);
+            GSA_TRY(keys.alloc(sizeof(K) * nnz));
+            GSA_HIP(hipMalloc(&v.perm, sizeof(V) * nnz));
+            v.perm64 = sizeof(V) == 8;
+            hipLaunchKernelGGL((k_make_keys<I, K, V>), dim3(g), dim3(256), 0, s, nnz, (const I*)rowidx,
+                               (const I*)nullptr, (int64_t)1, m, keys.as<K>(), (V*)v.perm);
+            GSA_TRY(sorter(keys.as<K>(), (V*)v.perm));
+            GSA_HIP(hipMalloc(&v.col, sizeof(I) * nnz));
+            hipLaunchKernelGGL((k_gather_idx<I, V>), dim3(g), dim3(256), 0, s, nnz,
+                               (const V*)v.perm, (const I*)colidx, (I*)v.col);
+            hipLaunchKernelGGL((k_ptr_from_keys<I, K>), dim3(grid_for(nnz + 1, 256)), dim3(256), 0,
+                               s, nnz, (const K*)keys.as<K>(), (uint64_t)1, m, (I*)v.rowptr);
+            GSA_HIP(hipGetLastError());
+            GSA_HIP(hipStreamSynchronize(s));  // the keys scratch is released on return
+            return GSA_OK;
+        };
+        if (bits <= 32 && nnz < (int64_t)UINT32_MAX)
+            return sorted(uint32_t{}, uint32_t{}, [&](uint32_t* k, uint32_t* p) {
+                return radix_sort_pairs_u32_u32(s, k, p, nnz, bits);
+            });
+        if (nnz < (int64_t)UINT32_MAX)
+            return sorted(uint64_t{}, uint32_t{}, [&](uint64_t* k, uint32_t* p) {
+                return radix_sort_pairs_u64_u32(s, k, p, nnz, bits);
+            });
+        return sorted(uint64_t{}, uint64_t{}, [&](uint64_t* k, uint64_t* p) {
+            return radix_sort_pairs_u64(s, k, p, nnz, bits);
+        });
+    });
+}
+
 }  // namespace gsa
 
 using namespace gsa;

@@ -53,6 +53,15 @@ __device__ __forceinline__ T widen(V v) {
     else return (T)v;
 }
 
+// Stored value j, or, through a row view of a COO/CSC matrix, the value at
+// storage position perm[j].
+template <typename PI, typename V>
+__device__ __forceinline__ V load_val(const V* __restrict__ nz, const PI* __restrict__ perm,
+                                      int64_t j) {
+    if constexpr (std::is_void_v<PI>) return __builtin_nontemporal_load(nz + j);
+    else return nz[(int64_t)__builtin_nontemporal_load(perm + j)];
+}
+
 // Row sum times the row's scale (BF16 storage), or unchanged.
 template <typename T>
 __device__ __forceinline__ T row_scaled(T acc, const float* __restrict__ rscale, int64_t r) {
@@ -251,12 +260,12 @@ __device__ __forceinline__ bool window_tail(const I* __restrict__ ptr, int64_t r
     return end > e1 && end - a > W_LONG;
 }
 
-template <typename T, typename I, typename V = T>
+template <typename T, typename I, typename V = T, typename PI = void>
 __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
     int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowptr,
-    const I* __restrict__ colval, const V* __restrict__ nzval, const float* __restrict__ rscale,
-    const T* __restrict__ x, T* __restrict__ y, T alpha, T beta, int mode,
-    const int64_t* __restrict__ win_row, int64_t nwin, T* __restrict__ head,
+    const I* __restrict__ colval, const V* __restrict__ nzval, const PI* __restrict__ perm,
+    const float* __restrict__ rscale, const T* __restrict__ x, T* __restrict__ y, T alpha,
+    T beta, int mode, const int64_t* __restrict__ win_row, int64_t nwin, T* __restrict__ head,
     T* __restrict__ tail) {
     __shared__ T s_prod[WAVES][WS];
     __shared__ int s_off[WAVES][WRP];
@@ -301,7 +310,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
             int i = u * 64 + lane;
             i = i < staged ? i : staged - 1;
             ci[u] = ldnt(colval + E0 + i);
-            vv[u] = widen<T>(ldnt(nzval + E0 + i));
+            vv[u] = widen<T>(load_val(nzval, perm, E0 + i));
         }
 #pragma unroll
         for (int u = 0; u <= WU; ++u) xv[u] = x[clamp64((int64_t)ci[u] - 1, 0, n - 1)];
@@ -802,15 +811,17 @@ struct PlanUse {
 };
 
 // Gather-shaped product on CSR arrays: C (rows m) = β C + α A B, B has n rows.
-template <typename T, typename I, typename V = T>
+// With `perm` (SpMV through a row view), value j is nzval[perm[j]].
+template <typename T, typename I, typename V = T, typename PI = void>
 static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* rowptr,
                       const I* colval, const V* nzval, int64_t nnz, const T* B, int64_t ldb,
                       int opB, T* C, int64_t ldc, double alpha, double beta, gsa_plan plan,
-                      int itype, const float* rscale = nullptr) {
+                      int itype, const float* rscale = nullptr, const PI* perm = nullptr) {
     if (m == 0 || k == 0) return GSA_OK;
     if (n == 0 || nnz == 0) {  // A*B contributes nothing: only β applies
         return apply_beta<T>(s, m, k, C, ldc, beta);
     }
+    GSA_CHECK(perm == nullptr || k == 1, GSA_ERR_INVALID, "row views serve SpMV only");
     PlanUse pu(s);
     GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr, k == 1));
     const int mode = beta_mode(beta);
@@ -830,9 +841,9 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         T* tail = head + pu.nwin;
         const int64_t grid1 = cdiv(pu.nwin, WAVES);
         GSA_CHECK(grid1 <= (int64_t)UINT32_MAX, GSA_ERR_INVALID, "SpMV grid too large");
-        hipLaunchKernelGGL((k_csr_spmv<T, I, V>), dim3((unsigned)grid1), dim3(64 * WAVES), 0, s, m,
-                           n, nnz, rowptr, colval, nzval, rscale, x, C, (T)alpha, (T)beta, mode,
-                           pu.win_row, pu.nwin, head, tail);
+        hipLaunchKernelGGL((k_csr_spmv<T, I, V, PI>), dim3((unsigned)grid1), dim3(64 * WAVES), 0, s,
+                           m, n, nnz, rowptr, colval, nzval, perm, rscale, x, C, (T)alpha,
+                           (T)beta, mode, pu.win_row, pu.nwin, head, tail);
         hipLaunchKernelGGL((k_spmv_fold<T, I>), dim3((unsigned)cdiv(pu.nwin, 256)), dim3(256), 0, s,
                            m, nnz, rowptr, pu.win_row, pu.nwin, (const T*)head, (const T*)tail,
                            rscale, C, (T)alpha, (T)beta, mode);
@@ -888,6 +899,37 @@ static int scatter_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* 
                        pu.win_row, pu.nwin);
     GSA_HIP(hipGetLastError());
     return GSA_OK;
+}
+
+// SpMV y = β y + α A x through a row view (m rows, x of length n): the CSR
+// window kernel over the view's row pointer, columns from the view (or the
+// caller's `colind` when the storage is row-ordered), values through the
+// view's storage positions.
+template <typename T, typename I>
+static int view_spmv(hipStream_t s, const RowView& v, int64_t n, const I* colind, const T* nzval,
+                     const T* B, int64_t ldb, int opB, T* C, int64_t ldc, double alpha,
+                     double beta, int itype) {
+    const I* col = v.col ? (const I*)v.col : colind;
+    if (!v.perm)
+        return gather_csr<T, I>(s, v.m, n, 1, (const I*)v.rowptr, col, nzval, v.nnz, B, ldb, opB,
+                                C, ldc, alpha, beta, v.win, itype);
+    if (v.perm64)
+        return gather_csr<T, I, T, uint64_t>(s, v.m, n, 1, (const I*)v.rowptr, col, nzval, v.nnz,
+                                             B, ldb, opB, C, ldc, alpha, beta, v.win, itype,
+                                             nullptr, (const uint64_t*)v.perm);
+    return gather_csr<T, I, T, uint32_t>(s, v.m, n, 1, (const I*)v.rowptr, col, nzval, v.nnz, B,
+                                         ldb, opB, C, ldc, alpha, beta, v.win, itype, nullptr,
+                                         (const uint32_t*)v.perm);
+}
+
+// The plan's row view when it was built on exactly these arrays.
+static const RowView* view_for(gsa_plan plan, int itype, int64_t m, int64_t nnz, const void* a,
+                               const void* b) {
+    if (!plan || !plan->view) return nullptr;
+    const RowView* v = plan->view;
+    if (v->itype != itype || v->m != m || v->nnz != nnz || v->src[0] != a || v->src[1] != b)
+        return nullptr;
+    return v;
 }
 
 static int check_common(int opA, int opB, int64_t m, int64_t n, int64_t k, int64_t nnz,
@@ -960,13 +1002,56 @@ int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz, const void*
     return GSA_OK;
 }
 
+int gsa_plan_create_coo(void* stream, int itype, int64_t m, int64_t n, int64_t nnz,
+                        const void* rowind, const void* colind, gsa_plan* plan) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(plan && m >= 0 && n >= 0 && nnz >= 0 && (nnz == 0 || (rowind && colind)),
+              GSA_ERR_INVALID, "gsa_plan_create_coo: bad arguments");
+    GSA_CHECK(itype == GSA_I32 || itype == GSA_I64, GSA_ERR_UNSUPPORTED, "bad itype %d", itype);
+    hipStream_t s = as_stream(stream);
+    auto* p = new gsa_plan_s();
+    p->kind = 2;
+    p->view = new RowView();
+    int r = build_row_view(s, itype, m, nnz, rowind, colind, false, *p->view);
+    if (r == GSA_OK) r = gsa_plan_create(stream, itype, m, nnz, p->view->rowptr, &p->view->win);
+    if (r != GSA_OK) { release_row_view(p->view); delete p; return r; }
+    p->view->src[0] = rowind;
+    p->view->src[1] = colind;
+    *plan = p;
+    return GSA_OK;
+}
+
+int gsa_plan_create_csc(void* stream, int itype, int64_t m, int64_t n, int64_t nnz,
+                        const void* colptr, const void* rowval, gsa_plan* plan) {
+    GSA_CHECK(plan && m >= 0 && n >= 0 && nnz >= 0 && colptr && (nnz == 0 || rowval),
+              GSA_ERR_INVALID, "gsa_plan_create_csc: bad arguments");
+    GSA_TRY(gsa_plan_create(stream, itype, n, nnz, colptr, plan));  // column blocks (Aᵀ x)
+    hipStream_t s = as_stream(stream);
+    int r = dispatch_i(itype, [&](auto ti) -> int {
+        using I = decltype(ti);
+        Scratch cols(s);  // the column of every stored entry
+        if (nnz > 0) {
+            GSA_TRY(cols.alloc(sizeof(I) * nnz));
+            GSA_TRY(expand_rows(s, itype, n, nnz, colptr, cols.p));
+        }
+        (*plan)->view = new RowView();
+        GSA_TRY(build_row_view(s, itype, m, nnz, rowval, cols.p, true, *(*plan)->view));
+        GSA_TRY(gsa_plan_create(stream, itype, m, nnz, (*plan)->view->rowptr, &(*plan)->view->win));
+        GSA_HIP(hipStreamSynchronize(s));  // the column scratch is released on return
+        return GSA_OK;
+    });
+    if (r != GSA_OK) { gsa_plan_destroy(*plan); *plan = nullptr; return r; }
+    (*plan)->view->src[0] = colptr;
+    (*plan)->view->src[1] = rowval;
+    return GSA_OK;
+}
+
 int gsa_plan_destroy(gsa_plan plan) {
     if (!plan) return GSA_OK;
-    if (plan->kind == 0) {
-        // the plan may still be in use by queued work on any stream
-        GSA_HIP(hipDeviceSynchronize());
-        GSA_TRY(plan_release(plan->csr));
-    }
+    // the plan may still be in use by queued work on any stream
+    if (plan->kind == 0 || plan->view) GSA_HIP(hipDeviceSynchronize());
+    if (plan->kind == 0) GSA_TRY(plan_release(plan->csr));
+    if (plan->view) release_row_view(plan->view);
     if (plan->kind == 1) batch_plan_release(plan);
     delete plan;
     return GSA_OK;
@@ -1003,10 +1088,15 @@ int gsa_spmm_csc(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
     return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
-        if (opA == GSA_OP_N)
+        if (opA == GSA_OP_N) {
+            const RowView* v = k == 1 ? view_for(plan, itype, m, nnz, colptr, rowval) : nullptr;
+            if (v)  // sorted-key form: rows gathered through the plan's row view
+                return view_spmv<T, I>(s, *v, n, (const I*)nullptr, (const T*)nzval, (const T*)B,
+                                       ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta, itype);
             return scatter_csr<T, I>(s, n, m, k, (const I*)colptr, (const I*)rowval,
                                      (const T*)nzval, nnz, (const T*)B, ldb, opB != GSA_OP_N,
                                      (T*)C, ldc, alpha, beta, plan, itype);
+        }
         return gather_csr<T, I>(s, n, m, k, (const I*)colptr, (const I*)rowval, (const T*)nzval,
                                 nnz, (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
                                 plan, itype);
@@ -1032,7 +1122,7 @@ int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
 int gsa_spmm_coo(void* stream, int vtype, int itype, int opA, int64_t m, int64_t n, int64_t k,
                  const void* rowind, const void* colind, const void* nzval, int64_t nnz,
                  const void* B, int64_t ldb, int opB, void* C, int64_t ldc, double alpha,
-                 double beta) {
+                 double beta, gsa_plan plan) {
     GSA_TRY(ensure_init());
     GSA_TRY(check_common(opA, opB, m, n, k, nnz, ldb, ldc));
     hipStream_t s = as_stream(stream);
@@ -1042,6 +1132,11 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
         using T = decltype(tv);
         using I = decltype(ti);
         if (rows_out == 0 || k == 0) return (int)GSA_OK;
+        const RowView* v = (!trans && k == 1) ? view_for(plan, itype, m, nnz, rowind, colind)
+                                              : nullptr;
+        if (v)  // sorted-key form: rows gathered through the plan's row view
+            return view_spmv<T, I>(s, *v, n, (const I*)colind, (const T*)nzval, (const T*)B, ldb,
+                                   opB != GSA_OP_N, (T*)C, ldc, alpha, beta, itype);
         GSA_TRY(apply_beta<T>(s, rows_out, k, (T*)C, ldc, beta));
         if (nnz == 0 || cols_in == 0) return (int)GSA_OK;
         hipLaunchKernelGGL((k_coo_scatter<T, I>), dim3(grid_for(nnz * k, 256)), dim3(256), 0, s,

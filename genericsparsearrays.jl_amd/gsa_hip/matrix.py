@@ -187,23 +187,39 @@ def _check_common(m, n, tensors, itype_tensor):
 
 
 class _PlanCache:
-    """Per-matrix cached analysis (gsa_plan) of one pointer array: the row
-    blocks and SpMV windows.  A plan also holds the partial sums of the SpMV in
-    flight, so products on different streams at the same time need different
-    plans."""
+    """Per-matrix cached analysis (gsa_plan): CSR row blocks and SpMV windows;
+    for CSC also the row view, for COO the row view alone (the sorted-key form
+    of mul!, without atomics).  It is rebuilt when an array it was built from
+    changes.  A plan also holds the partial sums of the SpMV in flight, so
+    products on different streams at the same time need different plans."""
 
     def __init__(self):
         self.handle = None
         self.key = None
 
     def get(self, ptr_t: torch.Tensor, m: int, nnz: int):
-        key = (ptr_t.data_ptr(), m, nnz, ptr_t.dtype, ptr_t._version)
+        """CSR: the analysis of row pointer `ptr_t` (m rows)."""
+        return self._get(("csr", ptr_t), lambda h: L.call(
+            "gsa_plan_create", stream_of(ptr_t.device), itype_of(ptr_t), m, nnz, ptr(ptr_t), h),
+            (ptr_t,), m, nnz)
+
+    def get_csc(self, colptr: torch.Tensor, rowval: torch.Tensor, m: int, n: int, nnz: int):
+        return self._get(("csc",), lambda h: L.call(
+            "gsa_plan_create_csc", stream_of(colptr.device), itype_of(colptr), m, n, nnz,
+            ptr(colptr), ptr(rowval), h), (colptr, rowval), m, n, nnz)
+
+    def get_coo(self, rowind: torch.Tensor, colind: torch.Tensor, m: int, n: int, nnz: int):
+        return self._get(("coo",), lambda h: L.call(
+            "gsa_plan_create_coo", stream_of(rowind.device), itype_of(rowind), m, n, nnz,
+            ptr(rowind), ptr(colind), h), (rowind, colind), m, n, nnz)
+
+    def _get(self, kind, create, arrays, *dims):
+        key = (kind[0],) + tuple((t.data_ptr(), t.dtype, t._version) for t in arrays) + dims
         if self.handle is not None and self.key == key:
             return self.handle
         self.release()
         h = ctypes.c_void_p()
-        L.call("gsa_plan_create", stream_of(ptr_t.device), itype_of(ptr_t), m, nnz,
-               ptr(ptr_t), ctypes.byref(h))
+        create(ctypes.byref(h))
         self.handle, self.key = h, key
         return h
 
@@ -314,7 +330,7 @@ class GenericSparseMatrixCSC(AbstractGenericSparseMatrix):
         return self.rowval
 
     def plan(self):
-        return self._plan.get(self.colptr, self.n, self.nnz())
+        return self._plan.get_csc(self.colptr, self.rowval, self.m, self.n, self.nnz())
 
     def copy(self):
         return GenericSparseMatrixCSC(self.m, self.n, self.colptr, self.rowval, self.nzval)
@@ -357,6 +373,14 @@ class GenericSparseMatrixCOO(AbstractGenericSparseMatrix):
 
     def _idx(self):
         return self.rowind
+
+    def plan(self):
+        """The row view of the entries (a stable sort by row, once; none when
+        the storage is already row-ordered): mul! then reduces rows without
+        atomics."""
+        if getattr(self, "_plan", None) is None:
+            self._plan = _PlanCache()
+        return self._plan.get_coo(self.rowind, self.colind, self.m, self.n, self.nnz())
 
     def getrowind(self):
         return self.rowind

@@ -103,8 +103,11 @@ def mul_(C: torch.Tensor, A, B: torch.Tensor, alpha=1.0, beta=0.0):
                Ap.plan() if Ap.nnz() > 0 else None)
     else:
         it = itype_of(Ap.rowind)
+        # the row view serves SpMV (k = 1, no transpose); other products scatter
+        plan = Ap.plan() if (Ap.nnz() > 0 and k == 1 and opA == L.GSA_OP_N) else None
         L.call("gsa_spmm_coo", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowind), ptr(Ap.colind),
-               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta))
+               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
+               plan)
     if Cw is not C:
         C.copy_(Cw)
     return C

@@ -96,10 +96,11 @@ int gsa_spmm_csr(void* stream, int vtype, int itype, int opA,
                  void* C, int64_t ldc,
                  double alpha, double beta, gsa_plan plan);
 
-/* Same for A in CSC (colptr length n+1): opA = N scatters (kernel_spmatmul_csc_N!),
- * opA = T/C gathers.  `plan` (over colptr) serves the gather direction.
- * Replaces mul! for GenericSparseMatrixCSC — src/matrix_csc/matrix_csc.jl:144-213,
- * kernels src/matrix_csc/matrix_csc_kernels.jl:1-45. */
+/* Same for A in CSC (colptr length n+1).  opA = T/C gathers (windows over
+ * colptr).  opA = N is the scatter of kernel_spmatmul_csc_N!; with a plan from
+ * gsa_plan_create_csc, an SpMV (k = 1) instead gathers each row through the
+ * plan's row view (no atomics).  Replaces mul! for GenericSparseMatrixCSC —
+ * src/matrix_csc/matrix_csc.jl:144-213, kernels src/matrix_csc/matrix_csc_kernels.jl:1-45. */
 int gsa_spmm_csc(void* stream, int vtype, int itype, int opA,
                  int64_t m, int64_t n, int64_t k,
                  const void* colptr, const void* rowval, const void* nzval, int64_t nnz,
@@ -107,7 +108,10 @@ int gsa_spmm_csc(void* stream, int vtype, int itype, int opA,
                  void* C, int64_t ldc,
                  double alpha, double beta, gsa_plan plan);
 
-/* Same for A in COO (any entry order, duplicates summed).
+/* Same for A in COO (any entry order, duplicates summed): atomic scatter, with
+ * equal consecutive outputs summed in the wavefront first; with a plan from
+ * gsa_plan_create_coo an SpMV (opA = N, k = 1) gathers rows through the row
+ * view instead.  `plan` may be NULL.
  * Replaces mul! for GenericSparseMatrixCOO — src/matrix_coo/matrix_coo.jl:155-221,
  * kernels src/matrix_coo/matrix_coo_kernels.jl:1-43. */
 int gsa_spmm_coo(void* stream, int vtype, int itype, int opA,
@@ -115,7 +119,21 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA,
                  const void* rowind, const void* colind, const void* nzval, int64_t nnz,
                  const void* B, int64_t ldb, int opB,
                  void* C, int64_t ldc,
-                 double alpha, double beta);
+                 double alpha, double beta, gsa_plan plan);
+
+/* Row-ordered views for the sorted-key mul! of COO and CSC (no atomics): the
+ * stored entries in row order as a CSR row pointer plus, unless the storage is
+ * already row-ordered, each entry's column and storage position (a stable
+ * sort by row, once).  Values are read through the positions at every call,
+ * so the plan stays valid while only values change.  Synchronises `stream`.
+ * COO: the rows of (rowind, colind); CSC (m x n): the column blocks of
+ * gsa_plan_create over colptr (for opA = T) and the row view of
+ * (colptr, rowval).  No counterpart in the reference (its COO/CSC mul! use
+ * @atomic, src/matrix_coo/matrix_coo_kernels.jl:1-43). */
+int gsa_plan_create_coo(void* stream, int itype, int64_t m, int64_t n, int64_t nnz,
+                        const void* rowind, const void* colind, gsa_plan* plan);
+int gsa_plan_create_csc(void* stream, int itype, int64_t m, int64_t n, int64_t nnz,
+                        const void* colptr, const void* rowval, gsa_plan* plan);
 
 /* BF16-stored CSR SpMV/SpMM with a Float32 scale per row, Float32 B/C and
  * Float32 accumulation (TwoSum-compensated on rows reduced across lanes,

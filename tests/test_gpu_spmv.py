@@ -352,3 +352,26 @@ def test_coo_runs_duplicates_and_order(dev, order, k):
     refT = O.spmm_coo(1, n, rows, cols, vals, xt)
     bT = O.spmm_coo(1, n, rows, cols, np.abs(vals), np.abs(xt))
     assert_normwise(host(G.transpose(A) @ dt), refT, bT, 1e-12)
+
+
+def test_row_view_follows_value_updates(dev):
+    """The COO/CSC row view holds positions, not values: products after an in-place
+    value update use the new values; a changed index array rebuilds the view."""
+    rng = np.random.default_rng(33)
+    H = synth.powerlaw_csr(5000, 4000, alpha=1.846, seed=34, hi=700)
+    A = dev_csr(H)
+    Ac = G.GenericSparseMatrixCSC(A)
+    Ao = G.GenericSparseMatrixCOO(Ac)  # column-major entry order: a sorted view with positions
+    x = rng.standard_normal(H.n)
+    xd = torch.as_tensor(x, device=dev)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, x)
+    bound = norm_bound(H.rowptr, H.colval, H.nzval, x)
+    for M in (Ac, Ao):
+        assert_normwise(host(M @ xd), ref, bound, 1e-12)
+        M.nzval.mul_(2.0)
+        assert_normwise(host(M @ xd), 2.0 * ref, 2.0 * bound, 1e-12)
+        M.nzval.mul_(0.5)
+    # permute the COO entries in place: the version bump rebuilds the view
+    o = torch.as_tensor(rng.permutation(Ao.nnz()), device=dev)
+    Ao.rowind.copy_(Ao.rowind[o]); Ao.colind.copy_(Ao.colind[o]); Ao.nzval.copy_(Ao.nzval[o])
+    assert_normwise(host(Ao @ xd), ref, bound, 1e-12)
