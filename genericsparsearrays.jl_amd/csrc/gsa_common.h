@@ -29,8 +29,8 @@ int hip_fail(hipError_t e, const char* what);
 
 int ensure_init();
 
-// Stream-ordered scratch memory from the device's default pool (kept cached by
-// gsa_init, so repeated calls do not return memory to the driver).
+// Stream-ordered scratch memory: device blocks cached per (device, stream); a
+// block released on a stream is reused only by later work on that stream.
 int ws_alloc(void** p, size_t bytes, hipStream_t s);
 void ws_free(void* p, hipStream_t s);
 

@@ -2,7 +2,10 @@
 #include "gsa_common.h"
 #include <cstdarg>
 #include <cstdio>
+#include <map>
 #include <mutex>
+#include <unordered_map>
+#include <vector>
 
 namespace gsa {
 
@@ -35,32 +38,132 @@ int ensure_init() {
             if (e == hipSuccess) set_error(GSA_ERR_HIP, "no HIP device visible");
             return;
         }
-        // Keep freed scratch memory cached in every device's default pool so
-        // stream-ordered scratch allocations cost no driver round trip.
-        int cur = 0;
-        (void)hipGetDevice(&cur);
-        for (int d = 0; d < n; ++d) {
-            hipMemPool_t pool;
-            if (hipDeviceGetDefaultMemPool(&pool, d) == hipSuccess) {
-                uint64_t thr = UINT64_MAX;
-                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &thr);
-            }
-        }
-        (void)hipSetDevice(cur);
     });
     return g_init_status;
 }
 
+// Scratch memory: a cache of device blocks per (device, stream).  A block
+// released on a stream is handed again only to work queued later on that same
+// stream, so stream order makes the reuse safe without events; a miss takes a
+// new block from hipMalloc.  (Stream-ordered pool allocation costs about
+// 0.1 ms per call on this runtime, several times per structural operation.)
+namespace {
+struct WsBlock {
+    void* p;
+    size_t bytes;
+};
+std::mutex g_ws_mu;
+std::map<std::pair<int, hipStream_t>, std::vector<WsBlock>> g_ws_free;
+std::unordered_map<void*, std::pair<int, size_t>> g_ws_live;  // block -> (device, bytes)
+// Blocks taken while the stream was being captured into a graph: stream-ordered
+// allocations (graph nodes), released the same way and never cached.
+std::unordered_map<void*, int> g_ws_captured;
+size_t g_ws_cached = 0;
+constexpr size_t WS_CACHE_MAX = 32ull << 30;  // cached bytes kept at most
+
+size_t ws_round(size_t b) {
+    if (b < 256) b = 256;
+    const size_t g = b > (1u << 20) ? (1u << 20) : 256;
+    return (b + g - 1) / g * g;
+}
+
+// Release every cached block (after the device has drained: a cached block may
+// still be read by queued work).
+void ws_trim() {
+    std::vector<std::pair<int, void*>> out;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        for (auto& kv : g_ws_free)
+            for (auto& b : kv.second) out.push_back({kv.first.first, b.p});
+        g_ws_free.clear();
+        g_ws_cached = 0;
+    }
+    if (out.empty()) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto& q : out) {
+        (void)hipSetDevice(q.first);
+        (void)hipDeviceSynchronize();
+        (void)hipFree(q.second);
+    }
+    (void)hipSetDevice(cur);
+}
+}  // namespace
+
 int ws_alloc(void** p, size_t bytes, hipStream_t s) {
     *p = nullptr;
-    if (bytes == 0) bytes = 256;
-    hipError_t e = hipMallocAsync(p, bytes, s);
-    if (e != hipSuccess) { *p = nullptr; return hip_fail(e, "hipMallocAsync(scratch)"); }
+    bytes = ws_round(bytes);
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusActive) {
+        hipError_t e = hipMallocAsync(p, bytes, s);
+        if (e != hipSuccess) {
+            *p = nullptr;
+            return hip_fail(e, "hipMallocAsync(scratch)");
+        }
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        g_ws_captured[*p] = dev;
+        return GSA_OK;
+    }
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        auto it = g_ws_free.find({dev, s});
+        if (it != g_ws_free.end()) {
+            // smallest cached block that fits and wastes at most its own size
+            auto& v = it->second;
+            size_t best = v.size();
+            for (size_t i = 0; i < v.size(); ++i)
+                if (v[i].bytes >= bytes && v[i].bytes <= 2 * bytes + (1u << 20) &&
+                    (best == v.size() || v[i].bytes < v[best].bytes))
+                    best = i;
+            if (best != v.size()) {
+                const WsBlock b = v[best];
+                v[best] = v.back();
+                v.pop_back();
+                g_ws_cached -= b.bytes;
+                g_ws_live[b.p] = {dev, b.bytes};
+                *p = b.p;
+                return GSA_OK;
+            }
+        }
+    }
+    hipError_t e = hipMalloc(p, bytes);
+    if (e == hipErrorOutOfMemory) {
+        (void)hipGetLastError();
+        ws_trim();
+        e = hipMalloc(p, bytes);
+    }
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return hip_fail(e, "hipMalloc(scratch)");
+    }
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    g_ws_live[*p] = {dev, bytes};
     return GSA_OK;
 }
 
 void ws_free(void* p, hipStream_t s) {
-    if (p) (void)hipFreeAsync(p, s);
+    if (!p) return;
+    bool trim = false;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        auto ct = g_ws_captured.find(p);
+        if (ct != g_ws_captured.end()) {
+            g_ws_captured.erase(ct);
+            (void)hipFreeAsync(p, s);
+            return;
+        }
+        auto it = g_ws_live.find(p);
+        if (it == g_ws_live.end()) return;
+        const int dev = it->second.first;
+        const size_t bytes = it->second.second;
+        g_ws_live.erase(it);
+        g_ws_free[{dev, s}].push_back({p, bytes});
+        g_ws_cached += bytes;
+        trim = g_ws_cached > WS_CACHE_MAX;
+    }
+    if (trim) ws_trim();
 }
 
 }  // namespace gsa
