@@ -114,6 +114,11 @@ struct CsrPlanData {
     // cross window ends, 2*nwin values of up to 8 bytes.  A plan serves one
     // SpMV at a time.
     void* part = nullptr;
+    // Optional launch order of the SpMV windows (gsa_plan_set_columns): slot
+    // 4*b + wave of workgroup b holds a window index or -1.
+    int64_t* order = nullptr;
+    int64_t nslot = 0;
+    const void* order_col = nullptr;  // the column array the order was built from
     const void* ptr = nullptr;    // the row pointer the plan was built on
 };
 

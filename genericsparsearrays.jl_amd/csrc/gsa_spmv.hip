@@ -265,15 +265,16 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
     int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowptr,
     const I* __restrict__ colval, const V* __restrict__ nzval, const PI* __restrict__ perm,
     const float* __restrict__ rscale, const T* __restrict__ x, T* __restrict__ y, T alpha,
-    T beta, int mode, const int64_t* __restrict__ win_row, int64_t nwin, T* __restrict__ head,
-    T* __restrict__ tail) {
+    T beta, int mode, const int64_t* __restrict__ win_row, int64_t nwin,
+    const int64_t* __restrict__ order, T* __restrict__ head, T* __restrict__ tail) {
     __shared__ T s_prod[WAVES][WS];
     __shared__ int s_off[WAVES][WRP];
 
     const int lane = threadIdx.x & 63;
     const int wv = threadIdx.x >> 6;
-    const int64_t w = (int64_t)blockIdx.x * WAVES + wv;
-    if (w >= nwin) return;
+    const int64_t slot = (int64_t)blockIdx.x * WAVES + wv;
+    const int64_t w = order ? order[slot] : slot;
+    if (w < 0 || w >= nwin) return;
     auto emit = [&](int64_t r, T acc) {
         store_out(y, r, row_scaled(acc, rscale, r), alpha, beta, mode);
     };
@@ -405,6 +406,87 @@ __global__ void k_spmv_fold(int64_t m, int64_t nnz, const I* __restrict__ rowptr
     for (int64_t v = w + 1; v <= we; ++v) cs.add(head[v]);
     const int64_t r = r1 - 1;
     store_out(y, r, row_scaled(cs.total(), rscale, r), alpha, beta, mode);
+}
+
+// ---- launch order of the SpMV windows ----------------------------------------
+// A window whose middle entry lies in a row longer than SPMV_LOCAL_ROW entries
+// gathers from a narrow band of x (the row's columns ascend), so it is given
+// to the XCD whose eighth of the columns holds that entry.  Workgroups go to
+// the 8 XCDs round-robin: workgroup b runs on XCD b % 8, so XCD x's windows go
+// to slots of workgroups 8j + x, and its L2 keeps that eighth of x.  The other
+// windows fill every XCD's list up to an even share.
+constexpr int SPMV_XCDS = 8;
+constexpr int64_t SPMV_LOCAL_ROW = SPMV_WIN;
+
+template <typename I>
+__global__ void k_win_bucket(int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowptr,
+                             const I* __restrict__ colval, const int64_t* __restrict__ win_row,
+                             int64_t nwin, int* __restrict__ bucket) {
+    const int64_t w = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (w >= nwin) return;
+    const int64_t E0 = w * SPMV_WIN < nnz ? w * SPMV_WIN : nnz;
+    const int64_t E1 = E0 + SPMV_WIN < nnz ? E0 + SPMV_WIN : nnz;
+    int b = SPMV_XCDS;  // any XCD
+    if (E1 > E0 && m > 0) {
+        const int64_t mid = E0 + (E1 - E0) / 2;
+        const int64_t r0 = clamp64(win_row[w], 0, m), r1 = clamp64(win_row[w + 1], r0, m);
+        // the row holding `mid`: the last row in [r0-1, r1-1] starting at or before it
+        int64_t lo = r0 > 0 ? r0 - 1 : 0, hi = r1 > lo ? r1 - 1 : lo;
+        while (lo < hi) {
+            const int64_t md = (lo + hi + 1) / 2;
+            if ((int64_t)rowptr[md] - 1 <= mid) lo = md;
+            else hi = md - 1;
+        }
+        const int64_t len = (int64_t)rowptr[lo + 1] - rowptr[lo];
+        if (len > SPMV_LOCAL_ROW && n > 0) {
+            const int64_t c = clamp64((int64_t)colval[mid] - 1, 0, n - 1);
+            b = (int)(c * SPMV_XCDS / n);
+        }
+    }
+    bucket[w] = b;
+}
+
+// One thread (plan time): the slot of every window; writes -1 to unused slots
+// and the number of slots used to *nslot.
+__global__ void k_win_order(int64_t nwin, const int* __restrict__ bucket,
+                            int64_t* __restrict__ order, int64_t cap, int64_t* __restrict__ nslot) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int64_t cnt[SPMV_XCDS + 1] = {0};
+    for (int64_t w = 0; w < nwin; ++w) cnt[bucket[w] >= 0 && bucket[w] <= SPMV_XCDS ? bucket[w] : SPMV_XCDS]++;
+    const int64_t share = (nwin + SPMV_XCDS - 1) / SPMV_XCDS;
+    int64_t len[SPMV_XCDS], pos[SPMV_XCDS], need[SPMV_XCDS];
+    int64_t most = 0;
+    for (int x = 0; x < SPMV_XCDS; ++x) {
+        need[x] = cnt[x] < share ? share - cnt[x] : 0;
+        len[x] = 0;
+        pos[x] = 0;
+    }
+    // lengths: own windows plus the shared ones handed out in XCD order
+    int64_t left = cnt[SPMV_XCDS];
+    for (int x = 0; x < SPMV_XCDS; ++x) {
+        const int64_t t = need[x] < left ? need[x] : left;
+        need[x] = t;
+        left -= t;
+        len[x] = cnt[x] + t;
+    }
+    for (int x = 0; x < SPMV_XCDS && left > 0; ++x) { ++need[x]; ++len[x]; --left; }  // none in practice
+    for (int x = 0; x < SPMV_XCDS; ++x) most = len[x] > most ? len[x] : most;
+    const int64_t rounds = (most + WAVES - 1) / WAVES;
+    const int64_t used = rounds * SPMV_XCDS * WAVES;
+    for (int64_t q = 0; q < used && q < cap; ++q) order[q] = -1;
+    int x_any = 0;
+    for (int64_t w = 0; w < nwin; ++w) {
+        int x = bucket[w];
+        if (x < 0 || x >= SPMV_XCDS) {  // shared: the next XCD with room
+            while (x_any < SPMV_XCDS && need[x_any] == 0) ++x_any;
+            x = x_any < SPMV_XCDS ? x_any : SPMV_XCDS - 1;
+            if (x_any < SPMV_XCDS) --need[x_any];
+        }
+        const int64_t j = pos[x]++;
+        const int64_t q = ((j / WAVES) * SPMV_XCDS + x) * WAVES + (j % WAVES);
+        if (q < cap) order[q] = w;
+    }
+    *nslot = used < cap ? used : cap;
 }
 
 // ------------------------------------------------------------ SpMM kernel --
@@ -765,6 +847,9 @@ struct PlanUse {
     int64_t nblocks = 0;
     const int64_t* win_row = nullptr;
     int64_t nwin = 0;
+    const int64_t* order = nullptr;  // window launch order, when built for these columns
+    int64_t nslot = 0;
+    const void* order_col = nullptr;
     CsrPlanData* owner = nullptr;
     Scratch tmp, tmp_part;
     explicit PlanUse(hipStream_t s) : tmp(s), tmp_part(s) {}
@@ -779,6 +864,9 @@ struct PlanUse {
             nblocks = owner->nblocks;
             win_row = owner->win_row;
             nwin = owner->nwin;
+            order = owner->order;
+            nslot = owner->nslot;
+            order_col = owner->order_col;
             return GSA_OK;
         }
         if (wave) {
@@ -839,11 +927,13 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         GSA_TRY(pu.part_buffer(&buf));
         T* head = static_cast<T*>(buf);
         T* tail = head + pu.nwin;
-        const int64_t grid1 = cdiv(pu.nwin, WAVES);
+        // the XCD launch order only for the column array it was built from
+        const int64_t* order = (pu.order && pu.order_col == (const void*)colval) ? pu.order : nullptr;
+        const int64_t grid1 = order ? pu.nslot / WAVES : cdiv(pu.nwin, WAVES);
         GSA_CHECK(grid1 <= (int64_t)UINT32_MAX, GSA_ERR_INVALID, "SpMV grid too large");
         hipLaunchKernelGGL((k_csr_spmv<T, I, V, PI>), dim3((unsigned)grid1), dim3(64 * WAVES), 0, s,
                            m, n, nnz, rowptr, colval, nzval, perm, rscale, x, C, (T)alpha,
-                           (T)beta, mode, pu.win_row, pu.nwin, head, tail);
+                           (T)beta, mode, pu.win_row, pu.nwin, order, head, tail);
         hipLaunchKernelGGL((k_spmv_fold<T, I>), dim3((unsigned)cdiv(pu.nwin, 256)), dim3(256), 0, s,
                            m, nnz, rowptr, pu.win_row, pu.nwin, (const T*)head, (const T*)tail,
                            rscale, C, (T)alpha, (T)beta, mode);
@@ -955,10 +1045,13 @@ static int check_common(int opA, int opB, int64_t m, int64_t n, int64_t k, int64
 
 static int plan_release(CsrPlanData& d) {
     hipError_t e = hipSuccess, f;
-    for (void* q : {(void*)d.blk_row, d.part})
+    for (void* q : {(void*)d.blk_row, d.part, (void*)d.order})
         if (q && (f = hipFree(q)) != hipSuccess && e == hipSuccess) e = f;
     d.blk_row = d.blk_pos = d.win_row = nullptr;
     d.part = nullptr;
+    d.order = nullptr;
+    d.nslot = 0;
+    d.order_col = nullptr;
     if (e != hipSuccess) return hip_fail(e, "hipFree(plan)");
     return GSA_OK;
 }
@@ -1002,6 +1095,50 @@ int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz, const void*
     return GSA_OK;
 }
 
+// The XCD launch order of a plan's SpMV windows for column array `colval`
+// (n columns).  Synchronises once.
+static int build_order(hipStream_t s, CsrPlanData& d, const void* colval, int64_t n) {
+    if (d.order) {
+        GSA_HIP(hipStreamSynchronize(s));
+        GSA_HIP(hipFree(d.order));
+        d.order = nullptr;
+        d.nslot = 0;
+        d.order_col = nullptr;
+    }
+    if (d.nwin == 0 || d.nnz == 0 || n <= 0) return GSA_OK;
+    const int64_t cap = SPMV_XCDS * (d.nwin + WAVES) + SPMV_XCDS * WAVES;
+    GSA_HIP(hipMalloc(&d.order, sizeof(int64_t) * cap));
+    Scratch bucket(s), ns(s);
+    GSA_TRY(bucket.alloc(sizeof(int) * d.nwin));
+    GSA_TRY(ns.alloc(sizeof(int64_t)));
+    int r = dispatch_i(d.itype, [&](auto ti) {
+        using I = decltype(ti);
+        hipLaunchKernelGGL(k_win_bucket<I>, dim3((unsigned)cdiv(d.nwin, 256)), dim3(256), 0, s, d.m, n,
+                           d.nnz, (const I*)d.ptr, (const I*)colval, (const int64_t*)d.win_row,
+                           d.nwin, bucket.as<int>());
+        return (int)GSA_OK;
+    });
+    GSA_TRY(r);
+    hipLaunchKernelGGL(k_win_order, dim3(1), dim3(1), 0, s, d.nwin, (const int*)bucket.as<int>(),
+                       d.order, cap, ns.as<int64_t>());
+    GSA_HIP(hipGetLastError());
+    int64_t nslot = 0;
+    GSA_HIP(hipMemcpyAsync(&nslot, ns.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    GSA_HIP(hipStreamSynchronize(s));
+    GSA_CHECK(nslot > 0 && nslot <= cap && nslot % (SPMV_XCDS * WAVES) == 0, GSA_ERR_INVALID,
+              "bad SpMV window order (%lld slots)", (long long)nslot);
+    d.nslot = nslot;
+    d.order_col = colval;
+    return GSA_OK;
+}
+
+int gsa_plan_set_columns(void* stream, gsa_plan plan, const void* colval, int64_t n) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(plan && plan->kind == 0 && n >= 0 && (colval || plan->csr.nnz == 0), GSA_ERR_INVALID,
+              "gsa_plan_set_columns: bad arguments");
+    return build_order(as_stream(stream), plan->csr, colval, n);
+}
+
 int gsa_plan_create_coo(void* stream, int itype, int64_t m, int64_t n, int64_t nnz,
                         const void* rowind, const void* colind, gsa_plan* plan) {
     GSA_TRY(ensure_init());
@@ -1014,6 +1151,8 @@ int gsa_plan_create_coo(void* stream, int itype, int64_t m, int64_t n, int64_t n
     p->view = new RowView();
     int r = build_row_view(s, itype, m, nnz, rowind, colind, false, *p->view);
     if (r == GSA_OK) r = gsa_plan_create(stream, itype, m, nnz, p->view->rowptr, &p->view->win);
+    if (r == GSA_OK)
+        r = build_order(s, p->view->win->csr, p->view->col ? p->view->col : colind, n);
     if (r != GSA_OK) { release_row_view(p->view); delete p; return r; }
     p->view->src[0] = rowind;
     p->view->src[1] = colind;
@@ -1037,6 +1176,7 @@ int gsa_plan_create_csc(void* stream, int itype, int64_t m, int64_t n, int64_t n
         (*plan)->view = new RowView();
         GSA_TRY(build_row_view(s, itype, m, nnz, rowval, cols.p, true, *(*plan)->view));
         GSA_TRY(gsa_plan_create(stream, itype, m, nnz, (*plan)->view->rowptr, &(*plan)->view->win));
+        GSA_TRY(build_order(s, (*plan)->view->win->csr, (*plan)->view->col, n));
         GSA_HIP(hipStreamSynchronize(s));  // the column scratch is released on return
         return GSA_OK;
     });

@@ -68,6 +68,7 @@ _SIGS = {
     "gsa_stream_sync": [_vp],
     "gsa_plan_create": [_vp, _i32, _i64, _i64, _vp, ctypes.POINTER(_vp)],
     "gsa_plan_destroy": [_vp],
+    "gsa_plan_set_columns": [_vp, _vp, _vp, _i64],
     "gsa_spmm_csr": [_vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,
                      _vp, _i64, _i32, _vp, _i64, _f64, _f64, _vp],
     "gsa_spmm_csc": [_vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,

@@ -197,21 +197,27 @@ class _PlanCache:
         self.handle = None
         self.key = None
 
-    def get(self, ptr_t: torch.Tensor, m: int, nnz: int):
-        """CSR: the analysis of row pointer `ptr_t` (m rows)."""
-        return self._get(("csr", ptr_t), lambda h: L.call(
-            "gsa_plan_create", stream_of(ptr_t.device), itype_of(ptr_t), m, nnz, ptr(ptr_t), h),
-            (ptr_t,), m, nnz)
+    def get(self, ptr_t: torch.Tensor, m: int, nnz: int, col_t: torch.Tensor | None = None,
+            n: int = 0):
+        """CSR: the analysis of row pointer `ptr_t` (m rows); with the column array
+        `col_t` (n columns) also the XCD launch order of the SpMV windows."""
+        def create(h):
+            L.call("gsa_plan_create", stream_of(ptr_t.device), itype_of(ptr_t), m, nnz,
+                   ptr(ptr_t), ctypes.byref(h))
+            if col_t is not None and nnz > 0:
+                L.call("gsa_plan_set_columns", stream_of(ptr_t.device), h, ptr(col_t), n)
+        arrays = (ptr_t,) if col_t is None else (ptr_t, col_t)
+        return self._get(("csr",), create, arrays, m, nnz, n)
 
     def get_csc(self, colptr: torch.Tensor, rowval: torch.Tensor, m: int, n: int, nnz: int):
         return self._get(("csc",), lambda h: L.call(
             "gsa_plan_create_csc", stream_of(colptr.device), itype_of(colptr), m, n, nnz,
-            ptr(colptr), ptr(rowval), h), (colptr, rowval), m, n, nnz)
+            ptr(colptr), ptr(rowval), ctypes.byref(h)), (colptr, rowval), m, n, nnz)
 
     def get_coo(self, rowind: torch.Tensor, colind: torch.Tensor, m: int, n: int, nnz: int):
         return self._get(("coo",), lambda h: L.call(
             "gsa_plan_create_coo", stream_of(rowind.device), itype_of(rowind), m, n, nnz,
-            ptr(rowind), ptr(colind), h), (rowind, colind), m, n, nnz)
+            ptr(rowind), ptr(colind), ctypes.byref(h)), (rowind, colind), m, n, nnz)
 
     def _get(self, kind, create, arrays, *dims):
         key = (kind[0],) + tuple((t.data_ptr(), t.dtype, t._version) for t in arrays) + dims
@@ -219,7 +225,12 @@ class _PlanCache:
             return self.handle
         self.release()
         h = ctypes.c_void_p()
-        create(ctypes.byref(h))
+        try:
+            create(h)
+        except Exception:
+            if h.value:
+                L.lib().gsa_plan_destroy(h)
+            raise
         self.handle, self.key = h, key
         return h
 
@@ -275,7 +286,7 @@ class GenericSparseMatrixCSR(AbstractGenericSparseMatrix):
         return self.colval
 
     def plan(self):
-        return self._plan.get(self.rowptr, self.m, self.nnz())
+        return self._plan.get(self.rowptr, self.m, self.nnz(), self.colval, self.n)
 
     def copy(self):
         return GenericSparseMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval)

@@ -38,7 +38,7 @@ class BF16CSR:
         self._plan = _PlanCache()
 
     def plan(self):
-        return self._plan.get(self.rowptr, self.m, self.nnz())
+        return self._plan.get(self.rowptr, self.m, self.nnz(), self.colval, self.n)
 
     @property
     def shape(self):

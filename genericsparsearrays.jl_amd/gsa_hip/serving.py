@@ -112,7 +112,7 @@ class MatrixRegistry:
         if pc is None:
             pc = _PlanCache()
             self._slot_plans[(mid, slot)] = pc
-        return pc.get(A.rowptr, A.shape[0], A.nnz())
+        return pc.get(A.rowptr, A.shape[0], A.nnz(), A.colval, A.n)
 
     def desc_table(self) -> np.ndarray:
         return np.array(self._desc, dtype=DESC_DTYPE)

@@ -80,6 +80,13 @@ int gsa_stream_sync(void* stream);
 int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz,
                     const void* ptr, gsa_plan* plan);
 int gsa_plan_destroy(gsa_plan plan);
+/* Give a plan the column array it will be used with (`colval`, n columns):
+ * the SpMV windows lying inside rows longer than 2304 entries (whose
+ * ascending columns span a narrow band of x) are then launched on the XCD
+ * that gathers that eighth of x, so their gathers hit its L2.  Used while the
+ * call's column array is this one; results never depend on it.  Synchronises
+ * `stream`.  No counterpart in the reference. */
+int gsa_plan_set_columns(void* stream, gsa_plan plan, const void* colval, int64_t n);
 
 /* C = beta*C + alpha*opA(A)*opB(B) with A in CSR (m x n, `nnz` stored entries).
  * C is size(opA(A),1) x k column-major (ldc); opB(B) is size(opA(A),2) x k:
