@@ -446,49 +446,6 @@ __global__ void k_win_bucket(int64_t m, int64_t n, int64_t nnz, const I* __restr
     bucket[w] = b;
 }
 
-// One thread (plan time): the slot of every window; writes -1 to unused slots
-// and the number of slots used to *nslot.
-__global__ void k_win_order(int64_t nwin, const int* __restrict__ bucket,
-                            int64_t* __restrict__ order, int64_t cap, int64_t* __restrict__ nslot) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    int64_t cnt[SPMV_XCDS + 1] = {0};
-    for (int64_t w = 0; w < nwin; ++w) cnt[bucket[w] >= 0 && bucket[w] <= SPMV_XCDS ? bucket[w] : SPMV_XCDS]++;
-    const int64_t share = (nwin + SPMV_XCDS - 1) / SPMV_XCDS;
-    int64_t len[SPMV_XCDS], pos[SPMV_XCDS], need[SPMV_XCDS];
-    int64_t most = 0;
-    for (int x = 0; x < SPMV_XCDS; ++x) {
-        need[x] = cnt[x] < share ? share - cnt[x] : 0;
-        len[x] = 0;
-        pos[x] = 0;
-    }
-    // lengths: own windows plus the shared ones handed out in XCD order
-    int64_t left = cnt[SPMV_XCDS];
-    for (int x = 0; x < SPMV_XCDS; ++x) {
-        const int64_t t = need[x] < left ? need[x] : left;
-        need[x] = t;
-        left -= t;
-        len[x] = cnt[x] + t;
-    }
-    for (int x = 0; x < SPMV_XCDS && left > 0; ++x) { ++need[x]; ++len[x]; --left; }  // none in practice
-    for (int x = 0; x < SPMV_XCDS; ++x) most = len[x] > most ? len[x] : most;
-    const int64_t rounds = (most + WAVES - 1) / WAVES;
-    const int64_t used = rounds * SPMV_XCDS * WAVES;
-    for (int64_t q = 0; q < used && q < cap; ++q) order[q] = -1;
-    int x_any = 0;
-    for (int64_t w = 0; w < nwin; ++w) {
-        int x = bucket[w];
-        if (x < 0 || x >= SPMV_XCDS) {  // shared: the next XCD with room
-            while (x_any < SPMV_XCDS && need[x_any] == 0) ++x_any;
-            x = x_any < SPMV_XCDS ? x_any : SPMV_XCDS - 1;
-            if (x_any < SPMV_XCDS) --need[x_any];
-        }
-        const int64_t j = pos[x]++;
-        const int64_t q = ((j / WAVES) * SPMV_XCDS + x) * WAVES + (j % WAVES);
-        if (q < cap) order[q] = w;
-    }
-    *nslot = used < cap ? used : cap;
-}
-
 // ------------------------------------------------------------ SpMM kernel --
 constexpr int SPMM_LONG = 64;  // staged rows longer than this are split over the workgroup
 constexpr int SPMM_MAX_LONG = SPMV_CAP / (SPMM_LONG + 1) + 1;
@@ -1095,6 +1052,42 @@ int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz, const void*
     return GSA_OK;
 }
 
+// Slots of the windows (host side, plan time): bucket b < SPMV_XCDS puts a
+// window on XCD b, the others top every XCD's list up to an even share; XCD
+// x's j-th window goes to slot 4*(8*(j/4) + x) + j%4 (workgroup 8*(j/4) + x,
+// which runs on XCD x).  Unused slots hold -1.
+static std::vector<int64_t> window_slots(const std::vector<int>& bucket) {
+    const int64_t nwin = (int64_t)bucket.size();
+    int64_t cnt[SPMV_XCDS + 1] = {0};
+    for (int b : bucket) cnt[(b >= 0 && b < SPMV_XCDS) ? b : SPMV_XCDS]++;
+    const int64_t share = (nwin + SPMV_XCDS - 1) / SPMV_XCDS;
+    int64_t extra[SPMV_XCDS], len[SPMV_XCDS], most = 0, left = cnt[SPMV_XCDS];
+    for (int x = 0; x < SPMV_XCDS; ++x) {
+        extra[x] = std::min<int64_t>(cnt[x] < share ? share - cnt[x] : 0, left);
+        left -= extra[x];
+    }
+    for (int x = 0; left > 0; x = (x + 1) % SPMV_XCDS) { ++extra[x]; --left; }
+    for (int x = 0; x < SPMV_XCDS; ++x) {
+        len[x] = cnt[x] + extra[x];
+        most = std::max(most, len[x]);
+    }
+    const int64_t rounds = (most + WAVES - 1) / WAVES;
+    std::vector<int64_t> order((size_t)(rounds * SPMV_XCDS * WAVES), -1);
+    int64_t pos[SPMV_XCDS] = {0};
+    int xs = 0;  // next XCD taking shared windows
+    for (int64_t w = 0; w < nwin; ++w) {
+        int x = bucket[w];
+        if (x < 0 || x >= SPMV_XCDS) {
+            while (extra[xs] == 0) ++xs;  // the extras sum to the shared count
+            --extra[xs];
+            x = xs;
+        }
+        const int64_t j = pos[x]++;
+        order[(size_t)(((j / WAVES) * SPMV_XCDS + x) * WAVES + (j % WAVES))] = w;
+    }
+    return order;
+}
+
 // The XCD launch order of a plan's SpMV windows for column array `colval`
 // (n columns).  Synchronises once.
 static int build_order(hipStream_t s, CsrPlanData& d, const void* colval, int64_t n) {
@@ -1106,28 +1099,25 @@ static int build_order(hipStream_t s, CsrPlanData& d, const void* colval, int64_
         d.order_col = nullptr;
     }
     if (d.nwin == 0 || d.nnz == 0 || n <= 0) return GSA_OK;
-    const int64_t cap = SPMV_XCDS * (d.nwin + WAVES) + SPMV_XCDS * WAVES;
-    GSA_HIP(hipMalloc(&d.order, sizeof(int64_t) * cap));
-    Scratch bucket(s), ns(s);
+    Scratch bucket(s);
     GSA_TRY(bucket.alloc(sizeof(int) * d.nwin));
-    GSA_TRY(ns.alloc(sizeof(int64_t)));
-    int r = dispatch_i(d.itype, [&](auto ti) {
+    GSA_TRY(dispatch_i(d.itype, [&](auto ti) {
         using I = decltype(ti);
         hipLaunchKernelGGL(k_win_bucket<I>, dim3((unsigned)cdiv(d.nwin, 256)), dim3(256), 0, s, d.m, n,
                            d.nnz, (const I*)d.ptr, (const I*)colval, (const int64_t*)d.win_row,
                            d.nwin, bucket.as<int>());
+        GSA_HIP(hipGetLastError());
         return (int)GSA_OK;
-    });
-    GSA_TRY(r);
-    hipLaunchKernelGGL(k_win_order, dim3(1), dim3(1), 0, s, d.nwin, (const int*)bucket.as<int>(),
-                       d.order, cap, ns.as<int64_t>());
-    GSA_HIP(hipGetLastError());
-    int64_t nslot = 0;
-    GSA_HIP(hipMemcpyAsync(&nslot, ns.p, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    }));
+    std::vector<int> hb((size_t)d.nwin);
+    GSA_HIP(hipMemcpyAsync(hb.data(), bucket.p, sizeof(int) * d.nwin, hipMemcpyDeviceToHost, s));
     GSA_HIP(hipStreamSynchronize(s));
-    GSA_CHECK(nslot > 0 && nslot <= cap && nslot % (SPMV_XCDS * WAVES) == 0, GSA_ERR_INVALID,
-              "bad SpMV window order (%lld slots)", (long long)nslot);
-    d.nslot = nslot;
+    const std::vector<int64_t> order = window_slots(hb);
+    GSA_HIP(hipMalloc(&d.order, sizeof(int64_t) * order.size()));
+    GSA_HIP(hipMemcpyAsync(d.order, order.data(), sizeof(int64_t) * order.size(),
+                           hipMemcpyHostToDevice, s));
+    GSA_HIP(hipStreamSynchronize(s));  // before the host vector goes
+    d.nslot = (int64_t)order.size();
     d.order_col = colval;
     return GSA_OK;
 }
