@@ -361,6 +361,9 @@ def main():
     ap.add_argument("--no-structural", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--only", choices=("precisions", "views", "alpha2", "spmm", "structural",
+                                       "batched", "serving"),
+                    help="after the headline SpMV, run this extra section alone (for profiling)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the headline SpMV steps (for rocprofv3)")
     ap.add_argument("--calibrate", action="store_true",
@@ -433,15 +436,22 @@ def main():
              "spmv_f64_i64_call_p99_us": round(float(np.percentile(lat, 99)) * 1e3, 2),
              "spmv_f64_i64_eager_us": round(ms_eager * 1e3, 2),
              "timed_steps": nsteps}
-    if not args.no_extra:
+
+    def want(section):
+        if args.no_extra or getattr(args, "no_" + section, False):
+            return False
+        return args.only is None or args.only == section
+
+    if want("precisions"):
         for dt, it, tag in ((torch.float64, torch.int32, "spmv_f64_i32"),
                             (torch.float32, torch.int32, "spmv_f32_i32")):
             ms2, gbs2, _, _, _ = spmv_run(dt, it, args.steps, args.warmup)
             extra[tag + "_GBs"] = round(gbs2, 1)
             extra[tag + "_us"] = round(ms2 * 1e3, 2)
             print(f"[bench] {tag} {ms2 * 1e3:.1f} us, {gbs2:.0f} GB/s", file=sys.stderr, flush=True)
-        # the same matrix as CSC (y = A x scatters) and COO (row-major from CSR, and
-        # column-major from CSC), f64/i64; bytes as the survey's formulas
+    if want("views"):
+        # the same matrix as CSC and COO (row-major from CSR, and column-major
+        # from CSC), f64/i64, through their row views; bytes as the survey's formulas
         A = G.GenericSparseMatrixCSR(H.m, H.n, torch.as_tensor(H.rowptr).to(dev),
                                      torch.as_tensor(H.colval).to(dev),
                                      torch.as_tensor(H.nzval).to(dev), copy=False)
@@ -461,6 +471,7 @@ def main():
             del M
         del A, Ac, xd, yd
         torch.cuda.empty_cache()
+    if want("alpha2"):
         # secondary point: Zipf alpha = 2.0 (mean 5.53 entries per row)
         H2 = synth.powerlaw_csr(args.rows, args.rows, alpha=2.0, seed=2)
         A2 = G.GenericSparseMatrixCSR(H2.m, H2.n, torch.as_tensor(H2.rowptr).to(dev),
@@ -476,6 +487,7 @@ def main():
         print(f"[bench] alpha=2.0 SpMV {ms5 * 1e3:.1f} us", file=sys.stderr, flush=True)
         del A2, x2, y2, H2
         torch.cuda.empty_cache()
+    if want("spmm"):
         # SpMM with a dense 1M x 32 right-hand side (column-major, as Julia's Matrix)
         k = 32
         A = G.GenericSparseMatrixCSR(H.m, H.n, torch.as_tensor(H.rowptr).to(dev),
@@ -493,13 +505,13 @@ def main():
         del A, B, C
         torch.cuda.empty_cache()
 
-    if not args.no_extra and not args.no_structural:
+    if want("structural"):
         extra["structural"] = bench_structural(H, dev)
         print(f"[bench] structural {extra['structural']}", file=sys.stderr, flush=True)
-    if not args.no_extra and not args.no_batched:
+    if want("batched"):
         extra["batched"] = bench_batched(dev)
         print(f"[bench] batched {extra['batched']}", file=sys.stderr, flush=True)
-    if not args.no_extra and not args.no_serving:
+    if want("serving"):
         extra["serving"] = bench_serving(dev)
 
     cpu = None

@@ -1,29 +1,31 @@
 // gsa_spmv.hip — mul! for CSR / CSC / COO on gfx950.
 //
-// CSR gather (CSR-N, CSC-T): "row-block streaming".  The nnz range is cut into
-// windows of SPMV_NB entries; block b owns the rows whose first entry lies in
-// window b (the plan: one lower bound per window, built in O(m) on the device).
-// A 256-lane workgroup
-//   1. streams its rows' colval/nzval with fully coalesced loads and stores the
-//      products nzval[j]*x[colval[j]] in LDS,
-//   2. sums each short row (<= LONG_T entries) on one lane, left to right — the
-//      summation order of kernel_spmatmul_csr_N! (src/matrix_csr/matrix_csr_kernels.jl:15-20),
-//      so these rows are bit-identical to the reference,
-//   3. reduces longer staged rows with one wavefront each (strided, then a
-//      butterfly), and
-//   4. reduces its last row, if it spills past the LDS capacity, with the whole
-//      workgroup straight from HBM.
-// Every output element is written by exactly one lane: no atomics, and the
-// result does not depend on scheduling.
+// CSR gather SpMV (CSR-N, CSC-T, and COO-N / CSC-N through a row view):
+// "flat windows".  Wavefront w reduces exactly the stored entries
+// [576w, 576(w+1)): it streams their indices and values with non-temporal
+// loads, gathers x, stages the products in LDS and sums the rows that start
+// in its window (the plan: one lower bound per window, built in O(m) on the
+// device).  Rows of at most 32 entries are summed left to right on one lane —
+// the summation order of kernel_spmatmul_csr_N!
+// (src/matrix_csr/matrix_csr_kernels.jl:15-20), so they are bit-identical to
+// the reference, including the ones that cross a window end (finished from 32
+// entries staged past it).  Longer rows are reduced by the whole wavefront;
+// those that run past the window leave partial sums that k_spmv_fold adds in
+// entry order.  Every output element is written once: no atomics, and the
+// result does not depend on scheduling.  A plan given its column array
+// launches the windows inside long sorted rows on the XCD whose L2 holds the
+// eighth of x they gather from.
 //
-// SpMM (k > 1) uses the same row blocks; the dense operand is read as a
-// row-major n x k array (transposed once into scratch when it is column-major),
-// so the k values a stored entry needs are one contiguous segment, and each
-// (row, k) output is summed on one lane in storage order.
+// SpMM (k > 1) uses 2048-entry row blocks per workgroup; the dense operand is
+// read as a row-major n x k array (transposed once into scratch when it is
+// column-major), so the k values a stored entry needs are one contiguous
+// segment, and each (row, k) output is summed on one lane in storage order.
 //
-// Scatter-shaped products (CSR-T, CSC-N, COO) add into C with hardware float
-// atomics, as the reference's @atomic kernels do (matrix_csr_kernels.jl:24-45,
-// src/matrix_csc/matrix_csc_kernels.jl:1-22, src/matrix_coo/matrix_coo_kernels.jl:1-43).
+// Scatter-shaped products without a row view (CSR-T, CSC-N, COO-T, COO SpMM)
+// add into C with hardware float atomics, as the reference's @atomic kernels
+// do (matrix_csr_kernels.jl:24-45, src/matrix_csc/matrix_csc_kernels.jl:1-22,
+// src/matrix_coo/matrix_coo_kernels.jl:1-43); in the COO kernel, runs of
+// equal output index are pre-reduced in the wavefront, one atomic per run.
 #include "gsa_common.h"
 #include "gsa_scan.h"
 

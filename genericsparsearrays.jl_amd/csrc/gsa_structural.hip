@@ -19,11 +19,14 @@
 //    (rowptrA[iA]-1)*nnzB + lenA(iA)*(rowptrB[iB]-1) and lists jA-major, jB-minor,
 //    which is exactly the reference's sorted COO -> CSR result.  Other operands
 //    take the reference's route: kron COO then the keyed sort.
-//  * SpGEMM: expand-sort-compress.  Every product a_ik*b_kj is generated in the
-//    reference's order (A-row order, then B-row order), keyed (row, col), stably
-//    radix-sorted, and each key's run summed left to right starting from +0.0 —
-//    the accumulation order and arithmetic of kernel_spgemm_csr!
-//    (src/matrix_csr/matrix_csr_kernels.jl:293-343), with cancellation zeros kept.
+//  * SpGEMM: every product a_ik*b_kj is numbered in the reference's order
+//    (A-row order, then B-row order) and each output entry is the left-to-right
+//    sum from +0.0 of its products — the accumulation order and arithmetic of
+//    kernel_spgemm_csr! (src/matrix_csr/matrix_csr_kernels.jl:293-343), with
+//    cancellation zeros kept.  Rows of at most 2048 products are sorted by
+//    (column, number) inside a lane group or a workgroup, in a symbolic (count)
+//    and a numeric (write) phase; a product with a longer row takes the global
+//    expand-sort-compress path (keyed (row, col), stable radix sort, run sums).
 #include "gsa_scan.h"
 #include "gsa_rows.h"
 
