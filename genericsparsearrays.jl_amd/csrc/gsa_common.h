@@ -49,6 +49,10 @@ struct Scratch {
 
 inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s); }
 
+// Positions per thread and pass in the gather kernels (reads by a stored
+// permutation): all of them are issued before the first write.
+constexpr int GATHER_U = 8;
+
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Grid size for grid-stride element loops: enough waves to fill 256 CUs.

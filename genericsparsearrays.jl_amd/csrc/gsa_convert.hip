@@ -59,16 +59,39 @@ __global__ void k_make_keys(int64_t nnz, const I* __restrict__ major, const I* _
     }
 }
 
-// out_idx[i] = idx[perm[i]], out_val[i] = val[perm[i]]
+// out_idx[i] = idx[perm[i]], out_val[i] = val[perm[i]].  Each thread takes
+// GATHER_U positions a grid stride apart and issues all their reads before
+// the first write, so that many random reads are in flight per wavefront.
 template <typename T, typename I, typename V>
 __global__ void k_gather(int64_t nnz, const V* __restrict__ perm, const I* __restrict__ minor,
                          const T* __restrict__ val, I* __restrict__ out_minor,
                          T* __restrict__ out_val) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t p = (int64_t)perm[i];
-        out_minor[i] = minor[p];
-        out_val[i] = val[p];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < nnz;
+         base += stride * GATHER_U) {
+        int64_t p[GATHER_U];
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            p[u] = i < nnz ? (int64_t)__builtin_nontemporal_load(perm + i) : 0;
+        }
+        I mi[GATHER_U];
+        T vv[GATHER_U];
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            if (base + u * stride < nnz) {
+                mi[u] = minor[p[u]];
+                vv[u] = val[p[u]];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < nnz) {
+                out_minor[i] = mi[u];
+                out_val[i] = vv[u];
+            }
+        }
     }
 }
 
@@ -160,9 +183,21 @@ __global__ void k_ptr_from_major(int64_t nnz, const I* __restrict__ major, int64
 template <typename I, typename V>
 __global__ void k_gather_idx(int64_t nnz, const V* __restrict__ perm, const I* __restrict__ in,
                              I* __restrict__ out) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
-         i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = in[(int64_t)perm[i]];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < nnz;
+         base += stride * GATHER_U) {
+        I v[GATHER_U];
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < nnz) v[u] = in[(int64_t)__builtin_nontemporal_load(perm + i)];
+        }
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < nnz) out[i] = v[u];
+        }
+    }
 }
 
 void release_row_view(RowView* v) {

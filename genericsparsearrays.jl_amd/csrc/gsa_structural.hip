@@ -159,11 +159,25 @@ __global__ void k_concat_gather(int64_t tot, const V* __restrict__ perm, int64_t
                                 const T* __restrict__ vA, const I* __restrict__ rB,
                                 const I* __restrict__ cB, const T* __restrict__ vB,
                                 I* __restrict__ ro, I* __restrict__ co, T* __restrict__ vo) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t p = (int64_t)perm[i];
-        if (p < nA) { ro[i] = rA[p]; co[i] = cA[p]; vo[i] = vA[p]; }
-        else { ro[i] = rB[p - nA]; co[i] = cB[p - nA]; vo[i] = vB[p - nA]; }
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < tot;
+         base += stride * GATHER_U) {
+        I r[GATHER_U], c[GATHER_U];
+        T v[GATHER_U];
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < tot) {
+                const int64_t p = (int64_t)perm[i];
+                if (p < nA) { r[u] = rA[p]; c[u] = cA[p]; v[u] = vA[p]; }
+                else { r[u] = rB[p - nA]; c[u] = cB[p - nA]; v[u] = vB[p - nA]; }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < tot) { ro[i] = r[u]; co[i] = c[u]; vo[i] = v[u]; }
+        }
     }
 }
 
@@ -234,33 +248,40 @@ __global__ void k_kron_rowptr(int64_t mA, int64_t mB, int64_t nnzB, const I* __r
     }
 }
 
-// One workgroup per A row: its C region [(pA[iA]-1)*nnzB, +lenA*nnzB) is
-// ordered (iB, a, b); each offset finds iB by binary search on B's pointer.
+// One wavefront per output row (iA, iB): its lenA*lenB entries are contiguous
+// at (pA[iA]-1)*nnzB + lenA*(pB[iB]-1), listed jA-major, jB-minor.
 template <typename T, typename I>
 __global__ __launch_bounds__(256) void k_kron_csr_fill(
     int64_t mA, int64_t mB, int64_t nB, int64_t nnzB, const I* __restrict__ pA,
     const I* __restrict__ cA, const T* __restrict__ vA, const I* __restrict__ pB,
     const I* __restrict__ cB, const T* __restrict__ vB, I* __restrict__ cC, T* __restrict__ vC) {
-    for (int64_t iA = blockIdx.x; iA < mA; iA += gridDim.x) {
-        const int64_t a0 = (int64_t)pA[iA] - 1;
-        const int64_t lenA = (int64_t)pA[iA + 1] - 1 - a0;
-        if (lenA <= 0) continue;
-        const int64_t base = a0 * nnzB;
-        const int64_t cnt = lenA * nnzB;
-        for (int64_t o = threadIdx.x; o < cnt; o += blockDim.x) {
-            // largest iB with lenA*(pB[iB]-1) <= o, skipping empty B rows
-            int64_t lo = 0, hi = mB - 1;
-            while (lo < hi) {
-                int64_t mid = (lo + hi + 1) >> 1;
-                if (lenA * ((int64_t)pB[mid] - 1) <= o) lo = mid; else hi = mid - 1;
-            }
-            const int64_t b0 = (int64_t)pB[lo] - 1;
-            const int64_t lenB = (int64_t)pB[lo + 1] - 1 - b0;
-            const int64_t rr = o - lenA * b0;
-            const int64_t aL = rr / lenB, bL = rr - aL * lenB;
+    const int lane = threadIdx.x & 63;
+    const int64_t nwaves = ((int64_t)gridDim.x * blockDim.x) >> 6;
+    const int64_t rows = mA * mB;
+    for (int64_t i = (blockIdx.x * (int64_t)blockDim.x + threadIdx.x) >> 6; i < rows;
+         i += nwaves) {
+        const int64_t iA = i / mB, iB = i - iA * mB;
+        const int64_t a0 = (int64_t)pA[iA] - 1, lenA = (int64_t)pA[iA + 1] - 1 - a0;
+        const int64_t b0 = (int64_t)pB[iB] - 1, lenB = (int64_t)pB[iB + 1] - 1 - b0;
+        if (lenA <= 0 || lenB <= 0) continue;
+        const int64_t cnt = lenA * lenB;
+        const int64_t base = a0 * nnzB + lenA * b0;
+        auto put = [&](int64_t o, int64_t aL, int64_t bL) {
             const int64_t a = a0 + aL, b = b0 + bL;
             cC[base + o] = (I)(((int64_t)cA[a] - 1) * nB + (int64_t)cB[b]);
             vC[base + o] = vA[a] * vB[b];
+        };
+        if (cnt <= (int64_t)UINT32_MAX) {
+            const uint32_t lb = (uint32_t)lenB;
+            for (uint32_t o = lane; o < (uint32_t)cnt; o += 64) {
+                const uint32_t aL = o / lb;
+                put(o, aL, o - aL * lb);
+            }
+        } else {
+            for (int64_t o = lane; o < cnt; o += 64) {
+                const int64_t aL = o / lenB;
+                put(o, aL, o - aL * lenB);
+            }
         }
     }
 }
@@ -307,9 +328,21 @@ __global__ void k_sg_expand(int64_t m, int64_t n, int64_t kdim, const I* __restr
 template <typename T, typename V>
 __global__ void k_gather_vals(int64_t n, const V* __restrict__ perm, const T* __restrict__ in,
                               T* __restrict__ out) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
-         i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = in[(int64_t)perm[i]];
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < n;
+         base += stride * GATHER_U) {
+        T v[GATHER_U];
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < n) v[u] = in[(int64_t)perm[i]];
+        }
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < n) out[i] = v[u];
+        }
+    }
 }
 
 template <typename T>
@@ -887,7 +920,8 @@ int gsa_kron_csr(void* stream, int vtype, int itype, int64_t mA, int64_t nA, int
             hipLaunchKernelGGL(k_kron_rowptr<I>, dim3(grid_for(mC + 1, 256)), dim3(256), 0, s, mA, mB,
                                nnzB, (const I*)rowptrA, (const I*)rowptrB, (I*)rowptrC);
             if (nnzC > 0 && mB > 0) {
-                unsigned g = (unsigned)(mA < 65536 ? mA : 65536);
+                const int64_t waves = mC < 256 * 32 * 4 * 8 ? mC : 256 * 32 * 4 * 8;
+                const unsigned g = (unsigned)cdiv(waves, 4);
                 hipLaunchKernelGGL((k_kron_csr_fill<T, I>), dim3(g), dim3(256), 0, s, mA, mB, nB, nnzB,
                                    (const I*)rowptrA, (const I*)colA, (const T*)valA,
                                    (const I*)rowptrB, (const I*)colB, (const T*)valB, (I*)colC,
