@@ -195,6 +195,46 @@ def bench_structural(H, dev):
     return out
 
 
+def cpu_structural(H):
+    """The oracle (the reference's CPU algorithms restated in C, one thread) on the
+    same structural workloads; kron on a 1e7-entry sample (the 1000^2 operand
+    with a 100 x 100 operand at density 0.1).  Algorithmic bytes as on the GPU."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+    from gsa_hip import synth
+
+    def tm(fn):
+        t0 = time.perf_counter()
+        r = fn()
+        return r, time.perf_counter() - t0
+    out = {}
+    nnz, m, n = H.nnz, H.m, H.n
+    c, t = tm(lambda: O.csr_to_csc(m, n, H.rowptr, H.colval, H.nzval))
+    out["csr_to_csc"] = (t, nnz * 16 + (m + 1) * 8 + nnz * 16 + (n + 1) * 8)
+    _, t = tm(lambda: O.csc_to_csr(m, n, c[0], c[1], c[2]))
+    out["csc_to_csr"] = (t, nnz * 16 + (n + 1) * 8 + nnz * 16 + (m + 1) * 8)
+    co, t = tm(lambda: O.csr_to_coo(m, H.rowptr, H.colval, H.nzval))
+    out["csr_to_coo"] = (t, nnz * 16 + (m + 1) * 8 + nnz * 24)
+    _, t = tm(lambda: O.coo_to_csr(m, n, co[0], co[1], co[2]))
+    out["coo_to_csr"] = (t, nnz * 24 + nnz * 16 + (m + 1) * 8)
+    del c, co
+    L = synth.laplacian2d(1000)
+    Lp = synth.drop_entries(L, 0.1, seed=3)
+    R = synth.fixed_row_csr(200_000, 200_000, 8, seed=3)
+    tup = lambda M: (M.rowptr, M.colval, M.nzval)
+    C, t = tm(lambda: O.csr_add(L.m, tup(L), tup(Lp)))
+    out["add_L_Lp"] = (t, (L.nnz + Lp.nnz) * 16 + 2 * (L.m + 1) * 8 + C[2].size * 16 + (L.m + 1) * 8)
+    for name, X in (("spgemm_LL", L), ("spgemm_RR", R)):
+        C, t = tm(lambda: O.spgemm_csr(X.m, X.n, tup(X), tup(X)))
+        out[name] = (t, (2 * X.nnz + C[2].size) * 16)
+    K1 = synth.sprand(1000, 1000, 0.01, seed=31)
+    Ks = synth.sprand(100, 100, 0.1, seed=32)
+    C, t = tm(lambda: O.kron_csr(K1.m, K1.n, Ks.m, Ks.n, tup(K1), tup(Ks)))
+    out["kron_csr_sample_1000x1000_d0.01_x_100x100_d0.1"] = (t, C[2].size * 16 + (C[0].size) * 8)
+    return {k: {"ms": round(t * 1e3, 1), "GBs": round(b / t / 1e9, 2), "threads": 1}
+            for k, (t, b) in out.items()}
+
+
 def bench_batched(dev, count=4096):
     """4096 uneven Float32 CSR instances (rows log-uniform in [64, 65536],
     Poisson(8)+1 per row, k in {1,2,4,8,16}) in one graph-captured persistent
@@ -469,6 +509,23 @@ def main():
             extra[tag + "_GBs"] = round(byts / (ms4 * 1e-3) / 1e9, 1)
             print(f"[bench] {tag} {ms4 * 1e3:.1f} us", file=sys.stderr, flush=True)
             del M
+        if not args.no_cpu:  # the oracle's scatters (restated reference loops), one thread
+            sys.path.insert(0, os.path.join(HERE, "oracle"))
+            import oracle as O
+            cp, rv, nzc = (t.cpu().numpy() for t in (Ac.colptr, Ac.rowval, Ac.nzval))
+            t0 = time.perf_counter()
+            O.spmm_csr_T(H.n, H.m, cp, rv, nzc, x_h)  # CSC A x (matrix_csc_kernels.jl:1-22)
+            tc = time.perf_counter() - t0
+            Ao = G.GenericSparseMatrixCOO(A)
+            ri, ci, nzo = (t.cpu().numpy() for t in (Ao.rowind, Ao.colind, Ao.nzval))
+            t0 = time.perf_counter()
+            O.spmm_coo(False, H.m, ri, ci, nzo, x_h)  # COO A x (matrix_coo_kernels.jl:1-20)
+            to = time.perf_counter() - t0
+            extra["spmv_csc_coo_cpu_oracle"] = {
+                "csc_ms": round(tc * 1e3, 1), "coo_rowmajor_ms": round(to * 1e3, 1), "threads": 1}
+            print(f"[bench] CSC / COO SpMV CPU oracle {tc * 1e3:.0f} / {to * 1e3:.0f} ms, 1 thread",
+                  file=sys.stderr, flush=True)
+            del cp, rv, nzc, ri, ci, nzo, Ao
         del A, Ac, xd, yd
         torch.cuda.empty_cache()
     if want("alpha2"):
@@ -504,10 +561,27 @@ def main():
         print(f"[bench] SpMM k=32 {ms3 * 1e3:.0f} us", file=sys.stderr, flush=True)
         del A, B, C
         torch.cuda.empty_cache()
+        if not args.no_cpu:  # the oracle's SpMM (restated reference loop) on the host cores
+            sys.path.insert(0, os.path.join(HERE, "oracle"))
+            import oracle as O
+            Bh = np.random.default_rng(3).standard_normal((k, H.n)).T  # column-major, as on the GPU
+            th = O.max_threads()
+            t0 = time.perf_counter()
+            O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, Bh, threads=th)
+            tc = time.perf_counter() - t0
+            extra["spmm_k32_f64_cpu_oracle"] = {"ms": round(tc * 1e3, 1),
+                                                "GBs": round(b3 / tc / 1e9, 2), "threads": th}
+            print(f"[bench] SpMM k=32 CPU oracle {tc * 1e3:.0f} ms on {th} threads",
+                  file=sys.stderr, flush=True)
+            del Bh
 
     if want("structural"):
         extra["structural"] = bench_structural(H, dev)
         print(f"[bench] structural {extra['structural']}", file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            extra["structural_cpu_oracle"] = cpu_structural(H)
+            print(f"[bench] structural CPU (oracle, 1 thread) {extra['structural_cpu_oracle']}",
+                  file=sys.stderr, flush=True)
     if want("batched"):
         extra["batched"] = bench_batched(dev)
         print(f"[bench] batched {extra['batched']}", file=sys.stderr, flush=True)

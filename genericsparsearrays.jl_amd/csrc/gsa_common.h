@@ -159,6 +159,22 @@ int build_row_view(hipStream_t s, int itype, int64_t m, int64_t nnz, const void*
                    const void* colidx, bool col_is_temp, RowView& v);
 void release_row_view(RowView* v);
 int expand_rows(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr, void* out);
+
+// Row bins of a CSC or COO matrix for its scatter-shaped SpMV (y = A x):
+// built from each entry's one-based row (rowidx) and, for CSC, the column
+// pointer; null in *out when the matrix has too many rows for the bins.
+// Synchronises.
+struct BinPlan;
+int build_bin_plan(hipStream_t s, int itype, int64_t m, int64_t n, int64_t nnz,
+                   const void* rowidx, const void* colptr, BinPlan** out);
+void release_bin_plan(BinPlan* p);
+bool bin_plan_matches(const BinPlan* p, int itype, int64_t m, int64_t n, int64_t nnz,
+                      const void* a, const void* b);
+void bin_plan_set_sources(BinPlan* p, const void* a, const void* b);
+int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* rowidx, const void* colidx,
+             const void* nzval, const void* x, int64_t ldx, void* y, double alpha, double beta);
+// 1 in *sorted when the one-based indices are non-decreasing.  Synchronises.
+int indices_sorted(hipStream_t s, int itype, int64_t nnz, const void* idx, int* sorted);
 }  // namespace gsa
 
 struct gsa_plan_s {
@@ -166,6 +182,7 @@ struct gsa_plan_s {
     gsa::CsrPlanData csr;
     void* batch = nullptr;           // gsa::BatchPlan*
     gsa::RowView* view = nullptr;    // row view (CSC: beside the column blocks)
+    gsa::BinPlan* bins = nullptr;    // row bins of a CSC or COO matrix (scatter-shaped SpMV)
 };
 
 struct gsa_ctx_s {

@@ -200,6 +200,25 @@ __global__ void k_gather_idx(int64_t nnz, const V* __restrict__ perm, const I* _
     }
 }
 
+int indices_sorted(hipStream_t s, int itype, int64_t nnz, const void* idx, int* sorted) {
+    *sorted = 1;
+    if (nnz <= 1) return GSA_OK;
+    return dispatch_i(itype, [&](auto ti) -> int {
+        using I = decltype(ti);
+        Scratch flag(s);
+        GSA_TRY(flag.alloc(sizeof(int)));
+        GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
+        hipLaunchKernelGGL(k_unsorted<I>, dim3(grid_for(nnz, 256)), dim3(256), 0, s, nnz,
+                           (const I*)idx, flag.as<int>());
+        GSA_HIP(hipGetLastError());
+        int unsorted = 0;
+        GSA_HIP(hipMemcpyAsync(&unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        *sorted = !unsorted;
+        return GSA_OK;
+    });
+}
+
 void release_row_view(RowView* v) {
     if (!v) return;
     if (v->win) gsa_plan_destroy(v->win);

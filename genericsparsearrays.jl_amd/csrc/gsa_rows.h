@@ -17,6 +17,20 @@ __device__ __forceinline__ void atomic_inc(int64_t* p) {
 }
 __device__ __forceinline__ void atomic_inc(int32_t* p) { atomicAdd(reinterpret_cast<int*>(p), 1); }
 
+// out = (β==0 ? 0 : β==1 ? y : y*β) + acc*α — the two roundings of
+// `_rmul_or_fill!(C, β)` followed by `C[row,k] += tmp*α` (matrix_csr.jl:192,
+// matrix_csr_kernels.jl:21).
+template <typename T>
+__device__ __forceinline__ void store_out(T* __restrict__ y, int64_t idx, T acc, T alpha,
+                                          T beta, int mode) {
+    T t = acc * alpha;
+    T base;
+    if (mode == 0) base = T(0);
+    else if (mode == 1) base = y[idx];
+    else base = y[idx] * beta;
+    y[idx] = base + t;
+}
+
 // Exclusive prefix sum over the 256 lanes of a workgroup; *total gets the sum.
 // Every lane of the workgroup must call it.
 template <typename T>

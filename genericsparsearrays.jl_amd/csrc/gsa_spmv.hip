@@ -27,6 +27,7 @@
 // src/matrix_coo/matrix_coo_kernels.jl:1-43); in the COO kernel, runs of
 // equal output index are pre-reduced in the wavefront, one atomic per run.
 #include "gsa_common.h"
+#include "gsa_rows.h"
 #include "gsa_scan.h"
 
 #include <algorithm>
@@ -124,20 +125,6 @@ struct CompSum {
         }
     }
 };
-
-// out = (β==0 ? 0 : β==1 ? y : y*β) + acc*α — the two roundings of
-// `_rmul_or_fill!(C, β)` followed by `C[row,k] += tmp*α` (matrix_csr.jl:192,
-// matrix_csr_kernels.jl:21).
-template <typename T>
-__device__ __forceinline__ void store_out(T* __restrict__ y, int64_t idx, T acc, T alpha,
-                                          T beta, int mode) {
-    T t = acc * alpha;
-    T base;
-    if (mode == 0) base = T(0);
-    else if (mode == 1) base = y[idx];
-    else base = y[idx] * beta;
-    y[idx] = base + t;
-}
 
 // ------------------------------------------------------------ plan kernels --
 template <typename I>
@@ -1140,6 +1127,16 @@ int gsa_plan_create_coo(void* stream, int itype, int64_t m, int64_t n, int64_t n
     hipStream_t s = as_stream(stream);
     auto* p = new gsa_plan_s();
     p->kind = 2;
+    // entries not in row order: row bins (two streaming passes per SpMV)
+    int sorted = 1;
+    int r0 = indices_sorted(s, itype, nnz, rowind, &sorted);
+    if (r0 == GSA_OK && !sorted) r0 = build_bin_plan(s, itype, m, n, nnz, rowind, nullptr, &p->bins);
+    if (r0 != GSA_OK) { delete p; return r0; }
+    if (p->bins) {
+        bin_plan_set_sources(p->bins, rowind, colind);
+        *plan = p;
+        return GSA_OK;
+    }
     p->view = new RowView();
     int r = build_row_view(s, itype, m, nnz, rowind, colind, false, *p->view);
     if (r == GSA_OK) r = gsa_plan_create(stream, itype, m, nnz, p->view->rowptr, &p->view->win);
@@ -1158,6 +1155,13 @@ int gsa_plan_create_csc(void* stream, int itype, int64_t m, int64_t n, int64_t n
               GSA_ERR_INVALID, "gsa_plan_create_csc: bad arguments");
     GSA_TRY(gsa_plan_create(stream, itype, n, nnz, colptr, plan));  // column blocks (Aᵀ x)
     hipStream_t s = as_stream(stream);
+    // A x: row bins when m allows them, else the row view
+    int rb = build_bin_plan(s, itype, m, n, nnz, rowval, colptr, &(*plan)->bins);
+    if (rb != GSA_OK) { gsa_plan_destroy(*plan); *plan = nullptr; return rb; }
+    if ((*plan)->bins) {
+        bin_plan_set_sources((*plan)->bins, colptr, rowval);
+        return GSA_OK;
+    }
     int r = dispatch_i(itype, [&](auto ti) -> int {
         using I = decltype(ti);
         Scratch cols(s);  // the column of every stored entry
@@ -1178,12 +1182,25 @@ int gsa_plan_create_csc(void* stream, int itype, int64_t m, int64_t n, int64_t n
     return GSA_OK;
 }
 
+int gsa_plan_describe(gsa_plan plan, int* flags) {
+    GSA_CHECK(plan && flags, GSA_ERR_INVALID, "gsa_plan_describe: null argument");
+    int f = 0;
+    if (plan->kind == 0) f |= GSA_PLAN_WINDOWS;
+    if (plan->kind == 0 && plan->csr.order) f |= GSA_PLAN_XCD_ORDER;
+    if (plan->view) f |= GSA_PLAN_ROW_VIEW;
+    if (plan->bins) f |= GSA_PLAN_ROW_BINS;
+    if (plan->kind == 1) f |= GSA_PLAN_BATCH;
+    *flags = f;
+    return GSA_OK;
+}
+
 int gsa_plan_destroy(gsa_plan plan) {
     if (!plan) return GSA_OK;
     // the plan may still be in use by queued work on any stream
-    if (plan->kind == 0 || plan->view) GSA_HIP(hipDeviceSynchronize());
+    if (plan->kind == 0 || plan->view || plan->bins) GSA_HIP(hipDeviceSynchronize());
     if (plan->kind == 0) GSA_TRY(plan_release(plan->csr));
     if (plan->view) release_row_view(plan->view);
+    if (plan->bins) release_bin_plan(plan->bins);
     if (plan->kind == 1) batch_plan_release(plan);
     delete plan;
     return GSA_OK;
@@ -1221,6 +1238,10 @@ int gsa_spmm_csc(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
         using T = decltype(tv);
         using I = decltype(ti);
         if (opA == GSA_OP_N) {
+            if (k == 1 && m > 0 && plan &&
+                bin_plan_matches(plan->bins, itype, m, n, nnz, colptr, rowval))
+                return bin_spmv(s, *plan->bins, vtype, rowval, nullptr, nzval, B,
+                                opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
             const RowView* v = k == 1 ? view_for(plan, itype, m, nnz, colptr, rowval) : nullptr;
             if (v)  // sorted-key form: rows gathered through the plan's row view
                 return view_spmv<T, I>(s, *v, n, (const I*)nullptr, (const T*)nzval, (const T*)B,
@@ -1264,6 +1285,10 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
         using T = decltype(tv);
         using I = decltype(ti);
         if (rows_out == 0 || k == 0) return (int)GSA_OK;
+        if (!trans && k == 1 && plan &&
+            bin_plan_matches(plan->bins, itype, m, n, nnz, rowind, colind))
+            return bin_spmv(s, *plan->bins, vtype, rowind, colind, nzval, B,
+                            opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
         const RowView* v = (!trans && k == 1) ? view_for(plan, itype, m, nnz, rowind, colind)
                                               : nullptr;
         if (v)  // sorted-key form: rows gathered through the plan's row view

@@ -80,6 +80,15 @@ int gsa_stream_sync(void* stream);
 int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz,
                     const void* ptr, gsa_plan* plan);
 int gsa_plan_destroy(gsa_plan plan);
+/* What a plan holds, as GSA_PLAN_* bits: the SpMV windows of a row or column
+ * pointer, their XCD launch order, a row view, row bins, a batch schedule.
+ * No counterpart in the reference. */
+#define GSA_PLAN_WINDOWS   1
+#define GSA_PLAN_XCD_ORDER 2
+#define GSA_PLAN_ROW_VIEW  4
+#define GSA_PLAN_ROW_BINS  8
+#define GSA_PLAN_BATCH     16
+int gsa_plan_describe(gsa_plan plan, int* flags);
 /* Give a plan the column array it will be used with (`colval`, n columns):
  * the SpMV windows lying inside rows longer than 2304 entries (whose
  * ascending columns span a narrow band of x) are then launched on the XCD
@@ -105,8 +114,9 @@ int gsa_spmm_csr(void* stream, int vtype, int itype, int opA,
 
 /* Same for A in CSC (colptr length n+1).  opA = T/C gathers (windows over
  * colptr).  opA = N is the scatter of kernel_spmatmul_csc_N!; with a plan from
- * gsa_plan_create_csc, an SpMV (k = 1) instead gathers each row through the
- * plan's row view (no atomics).  Replaces mul! for GenericSparseMatrixCSC —
+ * gsa_plan_create_csc, an SpMV (k = 1) instead runs the plan's row bins (or,
+ * for more rows than the bins hold, its row view): no atomics.  Replaces mul!
+ * for GenericSparseMatrixCSC —
  * src/matrix_csc/matrix_csc.jl:144-213, kernels src/matrix_csc/matrix_csc_kernels.jl:1-45. */
 int gsa_spmm_csc(void* stream, int vtype, int itype, int opA,
                  int64_t m, int64_t n, int64_t k,
@@ -128,15 +138,25 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA,
                  void* C, int64_t ldc,
                  double alpha, double beta, gsa_plan plan);
 
-/* Row-ordered views for the sorted-key mul! of COO and CSC (no atomics): the
- * stored entries in row order as a CSR row pointer plus, unless the storage is
- * already row-ordered, each entry's column and storage position (a stable
- * sort by row, once).  Values are read through the positions at every call,
- * so the plan stays valid while only values change.  Synchronises `stream`.
- * COO: the rows of (rowind, colind); CSC (m x n): the column blocks of
- * gsa_plan_create over colptr (for opA = T) and the row view of
- * (colptr, rowval).  No counterpart in the reference (its COO/CSC mul! use
- * @atomic, src/matrix_coo/matrix_coo_kernels.jl:1-43). */
+/* Plans for the SpMV (opA = N, k = 1) of COO and CSC without atomics.
+ * - Row bins (CSC, and COO whose entries are not in row order, for m up to
+ *   4,194,304 rows): y is cut into bins of 2048 rows; each call streams the
+ *   storage once, writing (row in bin, a_ij * x_j) pairs grouped by bin at
+ *   offsets the plan computed from the rows, then one workgroup per bin adds
+ *   its pairs into Float64 accumulators in LDS and writes its rows of y.  The
+ *   plan holds the offsets, the pair buffers (10 bytes per entry) and, for
+ *   CSC, each entry's column (4 bytes).  The summation order is fixed, so a
+ *   call's result does not change from run to run.
+ * - Row view (COO already in row order, or more rows): the stored entries in
+ *   row order as a CSR row pointer plus, unless the storage is row-ordered,
+ *   each entry's column and storage position (a stable sort by row, once);
+ *   the CSR window kernel then sums each row.
+ * Values and x are read at every call, so a plan stays valid while only the
+ * values change.  Synchronises `stream`.  COO: the rows of (rowind, colind);
+ * CSC (m x n): the column blocks of gsa_plan_create over colptr (for opA = T)
+ * plus the bins or the row view of (colptr, rowval).  No counterpart in the
+ * reference (its COO/CSC mul! use @atomic, src/matrix_coo/matrix_coo_kernels.jl:1-43,
+ * src/matrix_csc/matrix_csc_kernels.jl:1-22). */
 int gsa_plan_create_coo(void* stream, int itype, int64_t m, int64_t n, int64_t nnz,
                         const void* rowind, const void* colind, gsa_plan* plan);
 int gsa_plan_create_csc(void* stream, int itype, int64_t m, int64_t n, int64_t nnz,

@@ -375,3 +375,125 @@ def test_row_view_follows_value_updates(dev):
     o = torch.as_tensor(rng.permutation(Ao.nnz()), device=dev)
     Ao.rowind.copy_(Ao.rowind[o]); Ao.colind.copy_(Ao.colind[o]); Ao.nzval.copy_(Ao.nzval[o])
     assert_normwise(host(Ao @ xd), ref, bound, 1e-12)
+
+
+# ---- row bins: CSC-N and COO-N SpMV without atomics ------------------------------
+# With a plan, y = A x for CSC (and for COO not in row order) streams the storage
+# once into (row in bin, product) pairs grouped by 2048-row bins, then one
+# workgroup per bin sums them in LDS (Float64) and writes its rows once.
+
+def _flags(plan):
+    import ctypes
+    from gsa_hip import _lib as L
+    f = ctypes.c_int(-1)
+    L.call("gsa_plan_describe", plan, ctypes.byref(f))
+    return f.value
+
+
+def _csc_and_coo(H, rng, dev, dtype=torch.float64):
+    """The matrix as CSC, as COO in column-major order and as COO shuffled."""
+    A = dev_csr(H, dtype=dtype)
+    Ac = G.GenericSparseMatrixCSC(A)
+    Acol = G.GenericSparseMatrixCOO(Ac)
+    o = torch.as_tensor(rng.permutation(H.nnz), device=dev)
+    Ashuf = G.GenericSparseMatrixCOO(H.m, H.n, Acol.rowind[o], Acol.colind[o], Acol.nzval[o])
+    return {"csc": Ac, "coo_colmajor": Acol, "coo_shuffled": Ashuf}
+
+
+@pytest.mark.parametrize("shape", [
+    "powerlaw",      # 3 bins, the last one partial; rows up to 700 entries
+    "empty_bins",    # whole bins without entries, rows only in the middle bin
+    "one_row",       # a single row holding every entry
+    "tile_edges",    # nnz one past two 4096-entry tiles
+    "wide",          # more than 512 bins (the 2048-bin pass-1 layout)
+])
+def test_row_bins_parity(dev, shape):
+    from gsa_hip import _lib as L
+    rng = np.random.default_rng(51)
+    if shape == "powerlaw":
+        H = synth.powerlaw_csr(5000, 4000, alpha=1.846, seed=52, hi=700)
+    elif shape == "empty_bins":
+        lens = np.zeros(3 * 2048, dtype=np.int64)
+        lens[2048:4096] = rng.integers(0, 30, 2048)
+        H = synth.csr_from_lengths(lens, 5000, rng, "normal")
+    elif shape == "one_row":
+        H = synth.csr_from_lengths(np.array([20_000]), 30_000, rng, "normal")
+    elif shape == "tile_edges":
+        lens = np.full(2 * 4096 + 1, 1, dtype=np.int64)
+        H = synth.csr_from_lengths(lens, 3000, rng, "normal")
+    else:
+        H = synth.csr_from_lengths(rng.integers(0, 4, 1_200_000), 50_000, rng, "normal")
+    x = rng.standard_normal(H.n)
+    xd = torch.as_tensor(x, device=dev)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, x)
+    bound = norm_bound(H.rowptr, H.colval, H.nzval, x)
+    for name, M in _csc_and_coo(H, rng, dev).items():
+        flags = _flags(M.plan())
+        # a COO already in row order keeps the row view (pointer only)
+        in_row_order = name != "csc" and bool((M.rowind[1:] >= M.rowind[:-1]).all())
+        want = L.GSA_PLAN_ROW_VIEW if in_row_order else L.GSA_PLAN_ROW_BINS
+        assert flags & want, (name, flags)
+        y1 = host(M @ xd)
+        y2 = host(M @ xd)
+        assert np.array_equal(y1, y2), name  # fixed summation order
+        assert_normwise(y1, ref, bound, 1e-12)
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.5, 0.0), (1.0, 1.0), (0.5, 2.5)])
+def test_row_bins_alpha_beta(dev, alpha, beta):
+    rng = np.random.default_rng(53)
+    H = synth.powerlaw_csr(7000, 3000, alpha=1.846, seed=54, hi=900)
+    x = rng.standard_normal(H.n)
+    c0 = rng.standard_normal(H.m)
+    if beta == 0.0:
+        c0[::5] = np.nan  # cleared by β == 0, empty rows included
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, x, C=c0, alpha=alpha, beta=beta)
+    bound = abs(alpha) * norm_bound(H.rowptr, H.colval, H.nzval, x) + np.nan_to_num(np.abs(beta * c0))
+    for name, M in _csc_and_coo(H, rng, dev).items():
+        C = torch.as_tensor(c0.copy(), device=dev)
+        G.mul_(C, M, torch.as_tensor(x, device=dev), alpha, beta)
+        got = host(C)
+        assert not np.isnan(got).any(), name
+        assert_normwise(got, ref, bound, 1e-12)
+
+
+def test_row_bins_float32_and_strided_x(dev):
+    """Float32 values (accumulated in Float64 in the bins), and x read with a
+    stride through the C ABI (opB = T, ldb = 3)."""
+    import ctypes
+    from gsa_hip import _lib as L
+    from gsa_hip.matrix import stream_of
+    rng = np.random.default_rng(55)
+    H = synth.powerlaw_csr(6000, 5000, alpha=1.846, seed=56, hi=2000, dtype=np.float32)
+    x = rng.standard_normal(H.n).astype(np.float32)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval.astype(np.float64), x.astype(np.float64))
+    bound = norm_bound(H.rowptr, H.colval, H.nzval, x)
+    Ms = _csc_and_coo(H, rng, dev, dtype=torch.float32)
+    for name, M in Ms.items():
+        assert_normwise(host(M @ torch.as_tensor(x, device=dev)), ref, bound, 2e-6)
+    Ac = Ms["csc"]
+    X3 = torch.zeros((H.n, 3), dtype=torch.float32, device=dev)
+    X3[:, 1] = torch.as_tensor(x, device=dev)
+    y = torch.empty(H.m, dtype=torch.float32, device=dev)
+    p = lambda t: ctypes.c_void_p(t.data_ptr())
+    L.call("gsa_spmm_csc", stream_of(dev), L.GSA_F32, L.GSA_I64, L.GSA_OP_N, H.m, H.n, 1,
+           p(Ac.colptr), p(Ac.rowval), p(Ac.nzval), Ac.nnz(), ctypes.c_void_p(X3.data_ptr() + 4),
+           3, L.GSA_OP_T, p(y), H.m, 1.0, 0.0, Ac.plan())
+    assert_normwise(host(y), ref, bound, 2e-6)
+
+
+def test_row_bins_too_many_rows_use_the_row_view(dev):
+    """More rows than 2048 bins of 2048 hold: the plan keeps the row view."""
+    from gsa_hip import _lib as L
+    rng = np.random.default_rng(57)
+    m = 2048 * 2048 + 5
+    lens = np.zeros(m, dtype=np.int64)
+    lens[rng.choice(m, 20_000, replace=False)] = rng.integers(1, 6, 20_000)
+    H = synth.csr_from_lengths(lens, 1000, rng, "normal")
+    x = rng.standard_normal(H.n)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, x)
+    bound = norm_bound(H.rowptr, H.colval, H.nzval, x)
+    Ac = dev_csc_of(H)
+    flags = _flags(Ac.plan())
+    assert flags & L.GSA_PLAN_ROW_VIEW and not flags & L.GSA_PLAN_ROW_BINS
+    assert_normwise(host(Ac @ torch.as_tensor(x, device=dev)), ref, bound, 1e-12)
