@@ -31,7 +31,7 @@ constexpr int BIN_IPT = 16;                // entries per thread in pass 1
 constexpr int BIN_TILE = BIN_WG * BIN_IPT; // 4096 entries per tile
 constexpr int BIN_WTILE = 64 * BIN_IPT;    // 1024 entries per wavefront
 constexpr int BIN_SUM_WAVES = 8;
-constexpr int BIN_SUM_U = 8;               // pairs per lane in flight in pass 2
+constexpr int BIN_SUM_U = 32;              // pairs per lane in flight in pass 2
 constexpr int64_t BIN_MAX = 2048;          // bins (m <= 4,194,304 rows)
 
 struct BinPlan {
@@ -105,7 +105,8 @@ __global__ __launch_bounds__(BIN_WG) void k_bin_scatter(
     __shared__ uint16_t s_bin[BIN_TILE];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t tile = blockIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+    if (tile >= ntiles) return;
     const int64_t base = tile * BIN_TILE;
     const int tile_n = (int)(nnz - base < BIN_TILE ? nnz - base : BIN_TILE);
     for (int d = tid; d < BINS; d += BIN_WG) {
@@ -335,13 +336,14 @@ int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* rowidx, con
         auto scatter = [&](auto ci_t) {
             using CI = decltype(ci_t);
             const CI* ci = colidx ? (const CI*)colidx : (const CI*)p.col;
+            const unsigned g1 = (unsigned)xcd_tile_grid(p.ntiles);
             if (p.bins_cap == 512)
-                hipLaunchKernelGGL((k_bin_scatter<T, I, CI, 512>), dim3((unsigned)p.ntiles),
+                hipLaunchKernelGGL((k_bin_scatter<T, I, CI, 512>), dim3(g1),
                                    dim3(BIN_WG), 0, s, p.m, p.n, p.nnz, (const I*)rowidx, ci,
                                    (const T*)nzval, xv, (const int64_t*)p.off, p.nbins, p.ntiles,
                                    p.lrow, (T*)p.prod);
             else
-                hipLaunchKernelGGL((k_bin_scatter<T, I, CI, 2048>), dim3((unsigned)p.ntiles),
+                hipLaunchKernelGGL((k_bin_scatter<T, I, CI, 2048>), dim3(g1),
                                    dim3(BIN_WG), 0, s, p.m, p.n, p.nnz, (const I*)rowidx, ci,
                                    (const T*)nzval, xv, (const int64_t*)p.off, p.nbins, p.ntiles,
                                    p.lrow, (T*)p.prod);

@@ -53,6 +53,16 @@ inline hipStream_t as_stream(void* s) { return reinterpret_cast<hipStream_t>(s);
 // permutation): all of them are issued before the first write.
 constexpr int GATHER_U = 8;
 
+// Tiles in XCD-contiguous order: workgroup b runs on XCD b % 8, so it takes
+// tile (b % 8) * ceil(ntiles / 8) + b / 8, and each XCD works through one
+// contiguous eighth of the tiles (writes of neighbouring tiles to adjacent
+// addresses then meet in the same L2).  Launch xcd_tile_grid(ntiles)
+// workgroups; a workgroup whose tile is >= ntiles has nothing to do.
+inline int64_t xcd_tile_grid(int64_t ntiles) { return 8 * ((ntiles + 7) / 8); }
+__device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t ntiles) {
+    return (b % 8) * ((ntiles + 7) / 8) + b / 8;
+}
+
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
 // Grid size for grid-stride element loops: enough waves to fill 256 CUs.

@@ -95,11 +95,13 @@ __global__ __launch_bounds__(RS_WG) void k_rs_scatter(
     __shared__ V s_vals[RS_TILE];
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+    const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+    if (tile >= ntiles) return;
+    const int64_t base = tile * RS_TILE;
     const int tile_n = (int)(n - base < RS_TILE ? n - base : RS_TILE);
     {
-        const int64_t h = hist[(int64_t)tid * ntiles + blockIdx.x];
-        gbase[tid] = incl[(int64_t)tid * ntiles + blockIdx.x] - h;  // exclusive global offset
+        const int64_t h = hist[(int64_t)tid * ntiles + tile];
+        gbase[tid] = incl[(int64_t)tid * ntiles + tile] - h;  // exclusive global offset
     }
     for (int w = 0; w < RS_WAVES; ++w) wcnt[w][tid] = 0;
     __syncthreads();
@@ -189,7 +191,7 @@ static int radix_sort_impl(hipStream_t s, K* keys, V* vals, int64_t n, int bits)
                            shift, nb, hist.as<int64_t>(), ntiles);
         GSA_TRY(inclusive_scan<int64_t>(s, hist.as<int64_t>(), incl.as<int64_t>(),
                                         RS_BINS * ntiles, 0));
-        hipLaunchKernelGGL((k_rs_scatter<K, V>), dim3((unsigned)ntiles), dim3(RS_WG), 0, s,
+        hipLaunchKernelGGL((k_rs_scatter<K, V>), dim3((unsigned)xcd_tile_grid(ntiles)), dim3(RS_WG), 0, s,
                            (const K*)kin, (const V*)vin, n, shift, nb,
                            (const int64_t*)incl.as<int64_t>(), (const int64_t*)hist.as<int64_t>(),
                            ntiles, kout, vout);
