@@ -31,7 +31,7 @@ constexpr int BIN_IPT = 16;                // entries per thread in pass 1
 constexpr int BIN_TILE = BIN_WG * BIN_IPT; // 4096 entries per tile
 constexpr int BIN_WTILE = 64 * BIN_IPT;    // 1024 entries per wavefront
 constexpr int BIN_SUM_WAVES = 8;
-constexpr int BIN_SUM_U = 32;              // pairs per lane in flight in pass 2
+constexpr int BIN_SUM_U = 8;               // pairs per lane in flight in pass 2
 constexpr int64_t BIN_MAX = 2048;          // bins (m <= 4,194,304 rows)
 
 struct BinPlan {
