@@ -6,18 +6,21 @@
 // its summation order is undefined.  Here y is cut into bins of BIN_ROWS
 // rows, and one mul! is two streaming passes, with no random reads of the
 // matrix and no global atomics:
-//   1. k_bin_scatter: each 4096-entry tile of the storage forms its products
+//   1. k_bin_products: each 4096-entry tile of the storage forms its products
 //      a_ij * x_j (for CSC the entries of a column are consecutive, so x is
-//      read nearly in order), ranks them stably by bin (ballots, per-wavefront
-//      counters), reorders them by bin in LDS and writes each bin's run at the
-//      plan's offset for (bin, tile): (local row, product) pairs, bin-major;
+//      read nearly in order), stages them in LDS in bin order and writes each
+//      bin's run to its place in the bin-major product array.  Where each
+//      entry goes depends on the rows only, so k_bin_rank works it out once,
+//      at plan time (a stable ranking by bin per tile), with the row within
+//      the bin of every binned position;
 //   2. k_bin_sum: one workgroup per bin.  Each wavefront adds a contiguous
 //      share of the bin's pairs into its own Float64 accumulator in LDS, the
 //      eight accumulators are added in a fixed order, and each row of y is
 //      written once (beta, alpha applied there).
 // Float32 products are accumulated in Float64.  The plan holds the per-tile
-// offsets (from the row indices only), the pair buffers, and for CSC each
-// entry's column; values and x are read at every call.
+// offsets, slots and destinations (from the row indices only), the binned
+// rows and products, and for CSC each entry's column; values and x are read
+// at every call.
 #include "gsa_common.h"
 #include "gsa_rows.h"
 #include "gsa_scan.h"
@@ -43,6 +46,9 @@ struct BinPlan {
     void* col = nullptr;               // CSC: each entry's one-based column (u32 or I)
     bool col32 = false;
     uint16_t* lrow = nullptr;          // nnz: row within the bin, by bin
+    uint16_t* slot = nullptr;          // nnz: staging slot of each entry in its tile
+    void* dstq = nullptr;              // nnz: binned position of each staging slot (u32 or i64)
+    bool dst32 = false;
     void* prod = nullptr;              // nnz products (up to 8 bytes), by bin
     const void* src[2] = {nullptr, nullptr};
 };
@@ -89,18 +95,23 @@ __device__ __forceinline__ void bin_wave_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-template <typename T, typename I, typename CI, int BINS>
-__global__ __launch_bounds__(BIN_WG) void k_bin_scatter(
-    int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowidx, const CI* __restrict__ colidx,
-    const T* __restrict__ val, const T* __restrict__ x, const int64_t* __restrict__ off,
-    int64_t nbins, int64_t ntiles, uint16_t* __restrict__ blrow, T* __restrict__ bprod) {
+// Plan time: the place of every stored entry in the binned arrays.  Each
+// 4096-entry tile ranks its entries stably by bin (ballots, per-wavefront
+// counters), as the products will be staged in LDS in bin order: slot[j] is
+// entry j's position in that staging, dstq[tile*4096 + p] the binned position
+// of staged slot p (runs of consecutive positions per bin), and blrow the row
+// within the bin of every binned position.
+template <typename I, typename D, int BINS>
+__global__ __launch_bounds__(BIN_WG) void k_bin_rank(
+    int64_t m, int64_t nnz, const I* __restrict__ rowidx, const int64_t* __restrict__ off,
+    int64_t nbins, int64_t ntiles, uint16_t* __restrict__ slot, D* __restrict__ dstq,
+    uint16_t* __restrict__ blrow) {
     constexpr int BITS = BINS == 512 ? 9 : 11;
     constexpr int PER = BINS / BIN_WG;  // bins per thread in the tile scan
     __shared__ int wcnt[4][BINS];       // per-wavefront bin counts, then offsets
     __shared__ int tstart[BINS];
     __shared__ int64_t gbase[BINS];
     __shared__ int wsum[4];
-    __shared__ T s_prod[BIN_TILE];
     __shared__ uint16_t s_lrow[BIN_TILE];
     __shared__ uint16_t s_bin[BIN_TILE];
 
@@ -113,22 +124,13 @@ __global__ __launch_bounds__(BIN_WG) void k_bin_scatter(
         for (int w = 0; w < 4; ++w) wcnt[w][d] = 0;
         gbase[d] = d < nbins ? off[(int64_t)d * ntiles + tile] : 0;
     }
-
     // this wavefront's entries: tile positions wave*1024 + i*64 + lane
     I rr[BIN_IPT];
-    CI cc[BIN_IPT];
-    T vv[BIN_IPT];
 #pragma unroll
     for (int i = 0; i < BIN_IPT; ++i) {
         const int pos = wave * BIN_WTILE + i * 64 + lane;
-        const int64_t j = base + (pos < tile_n ? pos : tile_n - 1);
-        rr[i] = __builtin_nontemporal_load(rowidx + j);
-        cc[i] = __builtin_nontemporal_load(colidx + j);
-        vv[i] = __builtin_nontemporal_load(val + j);
+        rr[i] = rowidx[base + (pos < tile_n ? pos : tile_n - 1)];
     }
-    T pv[BIN_IPT];
-#pragma unroll
-    for (int i = 0; i < BIN_IPT; ++i) pv[i] = vv[i] * x[dclamp((int64_t)cc[i] - 1, 0, n - 1)];
     __syncthreads();
 
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -185,7 +187,7 @@ __global__ __launch_bounds__(BIN_WG) void k_bin_scatter(
             const int d = bin[i];
             const int p = tstart[d] + my[d] + rank[i];
             const int64_t r = dclamp((int64_t)rr[i] - 1, 0, m - 1);
-            s_prod[p] = pv[i];
+            slot[base + pos] = (uint16_t)p;
             s_lrow[p] = (uint16_t)(r & (BIN_ROWS - 1));
             s_bin[p] = (uint16_t)d;
         }
@@ -193,11 +195,44 @@ __global__ __launch_bounds__(BIN_WG) void k_bin_scatter(
     __syncthreads();
     for (int p = tid; p < tile_n; p += BIN_WG) {
         const int d = s_bin[p];
-        // in range for the row indices the plan was built from; clamped for others
         const int64_t dst = dclamp(gbase[d] + (p - tstart[d]), 0, nnz - 1);
+        dstq[base + p] = (D)dst;
         blrow[dst] = s_lrow[p];
-        bprod[dst] = s_prod[p];
     }
+}
+
+// Pass 1 of every call: the products of a tile, staged in LDS in bin order
+// through the plan's slots, then written to their binned positions (runs of
+// consecutive addresses).
+template <typename T, typename CI, typename D>
+__global__ __launch_bounds__(BIN_WG) void k_bin_products(
+    int64_t n, int64_t nnz, const CI* __restrict__ colidx, const T* __restrict__ val,
+    const T* __restrict__ x, const uint16_t* __restrict__ slot, const D* __restrict__ dstq,
+    int64_t ntiles, T* __restrict__ bprod) {
+    __shared__ T s_prod[BIN_TILE];
+    const int tid = threadIdx.x;
+    const int64_t tile = xcd_tile(blockIdx.x, ntiles);
+    if (tile >= ntiles) return;
+    const int64_t base = tile * BIN_TILE;
+    const int tile_n = (int)(nnz - base < BIN_TILE ? nnz - base : BIN_TILE);
+    CI cc[BIN_IPT];
+    T vv[BIN_IPT];
+    uint16_t q[BIN_IPT];
+#pragma unroll
+    for (int i = 0; i < BIN_IPT; ++i) {
+        const int p = i * BIN_WG + tid;
+        const int64_t j = base + (p < tile_n ? p : tile_n - 1);
+        cc[i] = __builtin_nontemporal_load(colidx + j);
+        vv[i] = __builtin_nontemporal_load(val + j);
+        q[i] = __builtin_nontemporal_load(slot + j);
+    }
+#pragma unroll
+    for (int i = 0; i < BIN_IPT; ++i)
+        if (i * BIN_WG + tid < tile_n)
+            s_prod[q[i] < tile_n ? q[i] : 0] = vv[i] * x[dclamp((int64_t)cc[i] - 1, 0, n - 1)];
+    __syncthreads();
+    for (int p = tid; p < tile_n; p += BIN_WG)
+        bprod[dclamp((int64_t)__builtin_nontemporal_load(dstq + base + p), 0, nnz - 1)] = s_prod[p];
 }
 
 template <typename T>
@@ -245,7 +280,7 @@ __global__ __launch_bounds__(64 * BIN_SUM_WAVES) void k_bin_sum(
 
 void release_bin_plan(BinPlan* p) {
     if (!p) return;
-    for (void* q : {(void*)p->off, p->col, (void*)p->lrow, p->prod})
+    for (void* q : {(void*)p->off, p->col, (void*)p->lrow, (void*)p->slot, p->dstq, p->prod})
         if (q) (void)hipFree(q);
     delete p;
 }
@@ -278,6 +313,24 @@ int build_bin_plan(hipStream_t s, int itype, int64_t m, int64_t n, int64_t nnz,
         hipLaunchKernelGGL(k_bin_excl, dim3(grid_for(cells + 1, 256)), dim3(256), 0, s, cells,
                            (const int64_t*)hist.as<int64_t>(), (const int64_t*)incl.as<int64_t>(),
                            p->off);
+        GSA_HIP(hipGetLastError());
+        GSA_HIP(hipMalloc(&p->slot, sizeof(uint16_t) * nnz));
+        p->dst32 = nnz <= (int64_t)UINT32_MAX;
+        GSA_HIP(hipMalloc(&p->dstq, (p->dst32 ? sizeof(uint32_t) : sizeof(int64_t)) * nnz));
+        auto rank = [&](auto d_t) {
+            using D = decltype(d_t);
+            const unsigned g = (unsigned)xcd_tile_grid(p->ntiles);
+            if (p->bins_cap == 512)
+                hipLaunchKernelGGL((k_bin_rank<I, D, 512>), dim3(g), dim3(BIN_WG), 0, s, m, nnz,
+                                   (const I*)rowidx, (const int64_t*)p->off, nbins, p->ntiles,
+                                   p->slot, (D*)p->dstq, p->lrow);
+            else
+                hipLaunchKernelGGL((k_bin_rank<I, D, 2048>), dim3(g), dim3(BIN_WG), 0, s, m, nnz,
+                                   (const I*)rowidx, (const int64_t*)p->off, nbins, p->ntiles,
+                                   p->slot, (D*)p->dstq, p->lrow);
+        };
+        if (p->dst32) rank(uint32_t{});
+        else rank(int64_t{});
         GSA_HIP(hipGetLastError());
         if (colptr) {  // CSC: the column of every stored entry
             Scratch cols(s);
@@ -316,10 +369,10 @@ void bin_plan_set_sources(BinPlan* p, const void* a, const void* b) {
     p->src[1] = b;
 }
 
-// y = beta y + alpha A x through a bin plan: rowidx holds each entry's
-// one-based row, colidx (COO) each entry's column, or null to use the plan's
-// columns (CSC).  x has stride ldx.
-int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* rowidx, const void* colidx,
+// y = beta y + alpha A x through a bin plan (built on the rows): colidx (COO)
+// holds each entry's one-based column, or is null to use the plan's columns
+// (CSC).  x has stride ldx.
+int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* colidx,
              const void* nzval, const void* x, int64_t ldx, void* y, double alpha, double beta) {
     const int mode = beta == 0.0 ? 0 : (beta == 1.0 ? 1 : 2);
     return dispatch_vi(vtype, p.itype, [&](auto tv, auto ti) -> int {
@@ -333,23 +386,21 @@ int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* rowidx, con
                                      hipMemcpyDeviceToDevice, s));
             xv = xs.as<T>();
         }
-        auto scatter = [&](auto ci_t) {
+        const unsigned g1 = (unsigned)xcd_tile_grid(p.ntiles);
+        auto products = [&](auto ci_t, auto d_t) {
             using CI = decltype(ci_t);
+            using D = decltype(d_t);
             const CI* ci = colidx ? (const CI*)colidx : (const CI*)p.col;
-            const unsigned g1 = (unsigned)xcd_tile_grid(p.ntiles);
-            if (p.bins_cap == 512)
-                hipLaunchKernelGGL((k_bin_scatter<T, I, CI, 512>), dim3(g1),
-                                   dim3(BIN_WG), 0, s, p.m, p.n, p.nnz, (const I*)rowidx, ci,
-                                   (const T*)nzval, xv, (const int64_t*)p.off, p.nbins, p.ntiles,
-                                   p.lrow, (T*)p.prod);
-            else
-                hipLaunchKernelGGL((k_bin_scatter<T, I, CI, 2048>), dim3(g1),
-                                   dim3(BIN_WG), 0, s, p.m, p.n, p.nnz, (const I*)rowidx, ci,
-                                   (const T*)nzval, xv, (const int64_t*)p.off, p.nbins, p.ntiles,
-                                   p.lrow, (T*)p.prod);
+            hipLaunchKernelGGL((k_bin_products<T, CI, D>), dim3(g1), dim3(BIN_WG), 0, s, p.n,
+                               p.nnz, ci, (const T*)nzval, xv, (const uint16_t*)p.slot,
+                               (const D*)p.dstq, p.ntiles, (T*)p.prod);
         };
-        if (colidx || !p.col32) scatter(I{});
-        else scatter(uint32_t{});
+        auto by_col = [&](auto d_t) {
+            if (colidx || !p.col32) products(I{}, d_t);
+            else products(uint32_t{}, d_t);
+        };
+        if (p.dst32) by_col(uint32_t{});
+        else by_col(int64_t{});
         hipLaunchKernelGGL(k_bin_sum<T>, dim3((unsigned)p.nbins), dim3(64 * BIN_SUM_WAVES), 0, s,
                            p.m, p.nnz, p.nbins, p.ntiles, (const int64_t*)p.off,
                            (const uint16_t*)p.lrow, (const T*)p.prod, (T*)y, (T)alpha, (T)beta,

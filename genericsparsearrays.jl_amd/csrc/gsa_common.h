@@ -181,7 +181,7 @@ void release_bin_plan(BinPlan* p);
 bool bin_plan_matches(const BinPlan* p, int itype, int64_t m, int64_t n, int64_t nnz,
                       const void* a, const void* b);
 void bin_plan_set_sources(BinPlan* p, const void* a, const void* b);
-int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* rowidx, const void* colidx,
+int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* colidx,
              const void* nzval, const void* x, int64_t ldx, void* y, double alpha, double beta);
 // 1 in *sorted when the one-based indices are non-decreasing.  Synchronises.
 int indices_sorted(hipStream_t s, int itype, int64_t nnz, const void* idx, int* sorted);

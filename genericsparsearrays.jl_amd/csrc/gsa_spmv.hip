@@ -1240,7 +1240,7 @@ int gsa_spmm_csc(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
         if (opA == GSA_OP_N) {
             if (k == 1 && m > 0 && plan &&
                 bin_plan_matches(plan->bins, itype, m, n, nnz, colptr, rowval))
-                return bin_spmv(s, *plan->bins, vtype, rowval, nullptr, nzval, B,
+                return bin_spmv(s, *plan->bins, vtype, nullptr, nzval, B,
                                 opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
             const RowView* v = k == 1 ? view_for(plan, itype, m, nnz, colptr, rowval) : nullptr;
             if (v)  // sorted-key form: rows gathered through the plan's row view
@@ -1287,7 +1287,7 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
         if (rows_out == 0 || k == 0) return (int)GSA_OK;
         if (!trans && k == 1 && plan &&
             bin_plan_matches(plan->bins, itype, m, n, nnz, rowind, colind))
-            return bin_spmv(s, *plan->bins, vtype, rowind, colind, nzval, B,
+            return bin_spmv(s, *plan->bins, vtype, colind, nzval, B,
                             opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
         const RowView* v = (!trans && k == 1) ? view_for(plan, itype, m, nnz, rowind, colind)
                                               : nullptr;

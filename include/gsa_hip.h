@@ -144,9 +144,10 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA,
  *   storage once, writing (row in bin, a_ij * x_j) pairs grouped by bin at
  *   offsets the plan computed from the rows, then one workgroup per bin adds
  *   its pairs into Float64 accumulators in LDS and writes its rows of y.  The
- *   plan holds the offsets, the pair buffers (10 bytes per entry) and, for
- *   CSC, each entry's column (4 bytes).  The summation order is fixed, so a
- *   call's result does not change from run to run.
+ *   plan holds where each entry goes, the binned rows and products (16 bytes
+ *   per entry in all) and, for CSC, each entry's column (4 bytes).  The
+ *   summation order is fixed, so a call's result does not change from run
+ *   to run.
  * - Row view (COO already in row order, or more rows): the stored entries in
  *   row order as a CSR row pointer plus, unless the storage is row-ordered,
  *   each entry's column and storage position (a stable sort by row, once);
