@@ -95,6 +95,51 @@ __global__ void k_gather(int64_t nnz, const V* __restrict__ perm, const I* __res
     }
 }
 
+// (minor index, value) of one stored entry, side by side, so that a gather by
+// a permutation touches one cache line per entry instead of two.
+template <typename T, typename I>
+struct alignas(sizeof(T) > sizeof(I) ? 2 * sizeof(T) : 2 * sizeof(I)) MinorVal {
+    I minor;
+    T val;
+};
+
+template <typename T, typename I>
+__global__ void k_pack(int64_t nnz, const I* __restrict__ minor, const T* __restrict__ val,
+                       MinorVal<T, I>* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        MinorVal<T, I> mv;
+        mv.minor = __builtin_nontemporal_load(minor + i);
+        mv.val = __builtin_nontemporal_load(val + i);
+        out[i] = mv;
+    }
+}
+
+// out_idx[i], out_val[i] = pairs[perm[i]], GATHER_U positions per thread in flight.
+template <typename T, typename I, typename V>
+__global__ void k_gather_pairs(int64_t nnz, const V* __restrict__ perm,
+                               const MinorVal<T, I>* __restrict__ pairs, I* __restrict__ out_minor,
+                               T* __restrict__ out_val) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < nnz;
+         base += stride * GATHER_U) {
+        MinorVal<T, I> mv[GATHER_U];
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < nnz) mv[u] = pairs[(int64_t)__builtin_nontemporal_load(perm + i)];
+        }
+#pragma unroll
+        for (int u = 0; u < GATHER_U; ++u) {
+            const int64_t i = base + u * stride;
+            if (i < nnz) {
+                out_minor[i] = mv[u].minor;
+                out_val[i] = mv[u].val;
+            }
+        }
+    }
+}
+
 // One-based major pointer from the sorted keys (major = key / kdim): ptr[r]
 // is one plus the first position whose major is >= r — searchsortedfirst over
 // the sorted majors (conversions.jl:194-199).  Position i writes the pointers
@@ -130,7 +175,19 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
     const int bits = bits_for(maxkey);
     const uint64_t kdim = minor_key ? (uint64_t)minor_dim : 1;
     const unsigned g = grid_for(nnz, 256);
-    Scratch keys(s), perm(s);
+    Scratch keys(s), perm(s), pairs(s);
+    // A transposition (key = major only) permutes every entry far from where it
+    // was: gather (minor, value) pairs, one line per entry.  A COO -> CSR of
+    // storage that is often in order already keeps the two-array gather,
+    // which then streams.
+    const bool packed = minor_key == nullptr;
+    const MinorVal<T, I>* pv = nullptr;
+    if (packed) {
+        GSA_TRY(pairs.alloc(sizeof(MinorVal<T, I>) * nnz));
+        hipLaunchKernelGGL((k_pack<T, I>), dim3(g), dim3(256), 0, s, nnz, minor_out, val,
+                           pairs.as<MinorVal<T, I>>());
+        pv = pairs.as<MinorVal<T, I>>();
+    }
     auto run = [&](auto key_t, auto pos_t, auto sorter) -> int {
         using K = decltype(key_t);
         using V = decltype(pos_t);
@@ -139,8 +196,12 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
         hipLaunchKernelGGL((k_make_keys<I, K, V>), dim3(g), dim3(256), 0, s, nnz, major, minor_key,
                            (int64_t)kdim, major_dim, keys.as<K>(), perm.as<V>());
         GSA_TRY(sorter(keys.as<K>(), perm.as<V>()));
-        hipLaunchKernelGGL((k_gather<T, I, V>), dim3(g), dim3(256), 0, s, nnz,
-                           (const V*)perm.as<V>(), minor_out, val, idx_out, val_out);
+        if (packed)
+            hipLaunchKernelGGL((k_gather_pairs<T, I, V>), dim3(g), dim3(256), 0, s, nnz,
+                               (const V*)perm.as<V>(), pv, idx_out, val_out);
+        else
+            hipLaunchKernelGGL((k_gather<T, I, V>), dim3(g), dim3(256), 0, s, nnz,
+                               (const V*)perm.as<V>(), minor_out, val, idx_out, val_out);
         hipLaunchKernelGGL((k_ptr_from_keys<I, K>), dim3(grid_for(nnz + 1, 256)), dim3(256), 0, s,
                            nnz, (const K*)keys.as<K>(), kdim, major_dim, ptr_out);
         GSA_HIP(hipGetLastError());
@@ -339,6 +400,8 @@ int gsa_csr_to_csc(void* stream, int vtype, int itype, int64_t m, int64_t n, int
                    void* rowval, void* nzval_out) {
     GSA_TRY(ensure_init());
     GSA_CHECK(m >= 0 && n >= 0 && nnz >= 0, GSA_ERR_INVALID, "gsa_csr_to_csc: negative size");
+    GSA_CHECK(nnz == 0 || (m > 0 && n > 0), GSA_ERR_INVALID,
+              "gsa_csr_to_csc: stored entries in a matrix with a zero dimension");
     hipStream_t s = as_stream(stream);
     return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
@@ -348,7 +411,7 @@ int gsa_csr_to_csc(void* stream, int vtype, int itype, int64_t m, int64_t n, int
             GSA_TRY(rows.alloc(sizeof(I) * nnz));
             GSA_TRY(expand_rows(s, itype, m, nnz, rowptr, rows.p));
         }
-        // stable sort by column only; payload gathers the (row, value) pairs
+        // stable sort by column only; the sorted positions gather the (row, value) pairs
         return sort_to_compressed<T, I>(s, n, m, nnz, (const I*)colval, nullptr,
                                         (const I*)rows.p, (const T*)nzval, (I*)colptr,
                                         (I*)rowval, (T*)nzval_out);
