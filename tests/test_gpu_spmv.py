@@ -123,8 +123,55 @@ def test_spmm_parity(dev, k, layout):
     ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, B)
     bound = O.spmm_csr_N(H.m, H.rowptr, H.colval, np.abs(H.nzval), np.abs(B))
     assert_normwise(C, ref, bound, 1e-12)
-    short = np.diff(H.rowptr) <= 64
+    short = np.diff(H.rowptr) <= 64  # one lane per (row, column), left to right
     assert np.array_equal(C[short], ref[short])
+
+
+@pytest.mark.parametrize("k", [2, 3, 4, 5, 8, 16])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_spmm_rows_across_blocks(dev, k, dtype):
+    """SpMM with rows around the one-lane and block limits, rows split over the
+    workgroup and spilled past the staging, empty rows; B row-major with a
+    leading dimension of k + 1, and column-major (transposed into scratch)."""
+    npd = np.float64 if dtype == torch.float64 else np.float32
+    rng = np.random.default_rng(61)
+    lens = np.r_[[575, 1, 576, 33, 1152, 0, 32, 31, 1153, 577, 5000], rng.integers(0, 40, 2000)]
+    H = synth.csr_from_lengths(lens, 9000, rng, "normal", dtype=npd)
+    B = rng.standard_normal((H.n, k)).astype(npd)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval.astype(np.float64), B.astype(np.float64))
+    bound = O.spmm_csr_N(H.m, H.rowptr, H.colval, np.abs(H.nzval).astype(np.float64),
+                         np.abs(B).astype(np.float64))
+    tol = 1e-12 if dtype == torch.float64 else 2e-6
+    A = dev_csr(H, dtype=dtype)
+    short = np.diff(H.rowptr) <= 32
+    Bpad = np.zeros((H.n, k + 1), dtype=npd)
+    Bpad[:, :k] = B
+    for Bd in (G.dense(B, device=dev), torch.as_tensor(Bpad, device=dev)[:, :k]):
+        C = host(A @ Bd)
+        assert_normwise(C, ref, bound, tol)
+        if dtype == torch.float64:
+            assert np.array_equal(C[short], ref[short])
+
+
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.5, 0.0), (1.0, 1.0), (0.5, 2.5)])
+def test_spmm_alpha_beta(dev, alpha, beta):
+    rng = np.random.default_rng(62)
+    H = synth.csr_from_lengths(np.r_[rng.integers(0, 40, 3000), [4000, 700, 513]], 5000, rng,
+                               "normal")
+    k = 4
+    B = rng.standard_normal((H.n, k))
+    C0 = rng.standard_normal((H.m, k))
+    if beta == 0.0:
+        C0[::5, :] = np.nan  # cleared by β == 0, long and spilled rows included
+    A = dev_csr(H)
+    Cd = G.dense(C0.copy(), device=dev)
+    G.mul_(Cd, A, G.dense(B, device=dev), alpha, beta)
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, B, C=C0, alpha=alpha, beta=beta)
+    bound = abs(alpha) * O.spmm_csr_N(H.m, H.rowptr, H.colval, np.abs(H.nzval), np.abs(B)) \
+        + np.nan_to_num(np.abs(beta * C0))
+    got = host(Cd)
+    assert not np.isnan(got).any()
+    assert_normwise(got, ref, bound, 1e-12)
 
 
 def test_spmm_beta_into_strided_output(dev):
@@ -390,9 +437,9 @@ def _flags(plan):
     return f.value
 
 
-def _csc_and_coo(H, rng, dev, dtype=torch.float64):
+def _csc_and_coo(H, rng, dev, dtype=torch.float64, itype=torch.int64):
     """The matrix as CSC, as COO in column-major order and as COO shuffled."""
-    A = dev_csr(H, dtype=dtype)
+    A = dev_csr(H, dtype=dtype, itype=itype)
     Ac = G.GenericSparseMatrixCSC(A)
     Acol = G.GenericSparseMatrixCOO(Ac)
     o = torch.as_tensor(rng.permutation(H.nnz), device=dev)
@@ -407,7 +454,8 @@ def _csc_and_coo(H, rng, dev, dtype=torch.float64):
     "tile_edges",    # nnz one past two 4096-entry tiles
     "wide",          # more than 512 bins (the 2048-bin pass-1 layout)
 ])
-def test_row_bins_parity(dev, shape):
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+def test_row_bins_parity(dev, shape, itype):
     from gsa_hip import _lib as L
     rng = np.random.default_rng(51)
     if shape == "powerlaw":
@@ -427,7 +475,7 @@ def test_row_bins_parity(dev, shape):
     xd = torch.as_tensor(x, device=dev)
     ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, x)
     bound = norm_bound(H.rowptr, H.colval, H.nzval, x)
-    for name, M in _csc_and_coo(H, rng, dev).items():
+    for name, M in _csc_and_coo(H, rng, dev, itype=itype).items():
         flags = _flags(M.plan())
         # a COO already in row order keeps the row view (pointer only)
         in_row_order = name != "csc" and bool((M.rowind[1:] >= M.rowind[:-1]).all())
