@@ -566,11 +566,16 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                         if (j + u < je) acc = acc + av[u] * bv[u];
                 }
             }
-            red[tid] = acc;
+            // the groups of a wavefront add their partials by shuffles, then
+            // the four wavefronts' sums are added in order
+            T v = acc;
+#pragma unroll
+            for (int o = LPR; o < 64; o <<= 1) v = v + __shfl_xor(v, o, 64);
+            if ((tid & 63) < LPR) red[(tid >> 6) * LPR + lane] = v;
             __syncthreads();
-            if (g == 0 && kk < k) {
+            if (tid < LPR && kk < k) {
                 T sum = red[lane];
-                for (int qq = 1; qq < GROUPS; ++qq) sum = sum + red[qq * LPR + lane];
+                for (int wq = 1; wq < WG / 64; ++wq) sum = sum + red[wq * LPR + lane];
                 store_out(C, r + kk * ldc, row_scaled(sum, rscale, r), alpha, beta, mode);
             }
             __syncthreads();
@@ -608,11 +613,14 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                 }
             }
             __syncthreads();
-            red[tid] = acc;
+            T v = acc;
+#pragma unroll
+            for (int o = LPR; o < 64; o <<= 1) v = v + __shfl_xor(v, o, 64);
+            if ((tid & 63) < LPR) red[(tid >> 6) * LPR + lane] = v;
             __syncthreads();
-            if (g == 0 && kk < k) {
+            if (tid < LPR && kk < k) {
                 T sum = red[lane];
-                for (int q = 1; q < GROUPS; ++q) sum = sum + red[q * LPR + lane];
+                for (int wq = 1; wq < WG / 64; ++wq) sum = sum + red[wq * LPR + lane];
                 store_out(C, r + kk * ldc, row_scaled(sum, rscale, r), alpha, beta, mode);
             }
         }
