@@ -392,7 +392,15 @@ __global__ void k_spmv_fold(int64_t m, int64_t nnz, const I* __restrict__ rowptr
     we = we < nwin - 1 ? we : nwin - 1;
     CompSum<T> cs;
     cs.add(tail[w]);
-    for (int64_t v = w + 1; v <= we; ++v) cs.add(head[v]);
+    // heads in order; eight loads in flight at a time
+    for (int64_t v = w + 1; v <= we; v += 8) {
+        T hv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) hv[u] = v + u <= we ? head[v + u] : T(0);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+            if (v + u <= we) cs.add(hv[u]);
+    }
     const int64_t r = r1 - 1;
     store_out(y, r, row_scaled(cs.total(), rscale, r), alpha, beta, mode);
 }
