@@ -235,6 +235,85 @@ def cpu_structural(H):
             for k, (t, b) in out.items()}
 
 
+def _reduce_workloads(H):
+    """Host inputs of the reduction benchmarks: the 1M power-law matrix
+    with every third diagonal entry added (so tr has terms), x, y, and the
+    symmetric 2-D Laplacian (1000 x 1000 grid)."""
+    import numpy as np
+    import scipy.sparse as sp
+    from gsa_hip import synth
+    S = H.to_scipy()
+    d = np.zeros(H.m)
+    d[::3] = 1.0
+    Hd = synth.HostCSR.from_scipy(sp.csr_matrix(S + sp.diags(d)))
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal(H.m)
+    y = rng.standard_normal(H.n)
+    return Hd, x, y, synth.laplacian2d(1000)
+
+
+def _reduce_bytes(M, x_len=0, y_len=0, write=False):
+    """Pointer, index and value arrays read once (plus dense vectors, plus the
+    written copy for A + λI)."""
+    b = (M.m + 1) * 8 + M.nnz * 16 + (x_len + y_len) * 8
+    return 2 * b + M.m * 16 if write else b
+
+
+def bench_reductions(H, dev):
+    """tr, sum, dot(x, A, y) (A and transpose(A)) and A + λI on the 1M power-law
+    matrix (with a third of its diagonal stored), issymmetric on the 1M-row
+    Laplacian; Float64 values, Int64 indices."""
+    import torch
+    import gsa_hip as G
+    Hd, x, y, L = _reduce_workloads(H)
+
+    def dcsr(M):
+        return G.GenericSparseMatrixCSR(M.m, M.n, torch.as_tensor(M.rowptr).to(dev),
+                                        torch.as_tensor(M.colval).to(dev),
+                                        torch.as_tensor(M.nzval).to(dev), copy=False)
+    A, dL = dcsr(Hd), dcsr(L)
+    xd, yd = torch.as_tensor(x).to(dev), torch.as_tensor(y).to(dev)
+    out = {}
+    for name, fn, byts in (
+            ("tr", lambda: G.tr(A), _reduce_bytes(Hd)),
+            ("sum", lambda: G.sum(A), Hd.nnz * 8),
+            ("dot_x_A_y", lambda: G.dot(xd, A, yd), _reduce_bytes(Hd, Hd.m, Hd.n)),
+            ("dot_x_At_y", lambda: G.dot(yd, G.transpose(A), xd), _reduce_bytes(Hd, Hd.m, Hd.n)),
+            ("issymmetric_laplacian", lambda: G.issymmetric(dL), _reduce_bytes(L)),
+            ("A_plus_2I", lambda: A + 2.0 * G.I, _reduce_bytes(Hd, write=True))):
+        t = _sync_time(fn, reps=5)
+        out[name] = {"us": round(t * 1e6, 1), "GBs": round(byts / t / 1e9, 1)}
+    assert G.issymmetric(dL)
+    del A, dL, xd, yd
+    torch.cuda.empty_cache()
+    return out
+
+
+def cpu_reductions(H):
+    """The oracle (one thread) on the same reduction workloads."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import oracle as O
+    Hd, x, y, L = _reduce_workloads(H)
+
+    def tm(fn):
+        t0 = time.perf_counter()
+        fn()
+        return time.perf_counter() - t0
+    out = {}
+    for name, fn, byts in (
+            ("tr", lambda: O.trace_ptr(Hd.m, Hd.rowptr, Hd.colval, Hd.nzval), _reduce_bytes(Hd)),
+            ("sum", lambda: float(Hd.nzval.sum()), Hd.nnz * 8),
+            ("dot_x_A_y", lambda: O.dot3_ptr(Hd.m, Hd.rowptr, Hd.colval, Hd.nzval, x, y, True),
+             _reduce_bytes(Hd, Hd.m, Hd.n)),
+            ("issymmetric_laplacian", lambda: O.symmetric_ptr(L.m, L.rowptr, L.colval, L.nzval),
+             _reduce_bytes(L)),
+            ("A_plus_2I", lambda: O.add_diag(Hd.m, Hd.rowptr, Hd.colval, Hd.nzval, 2.0),
+             _reduce_bytes(Hd, write=True))):
+        t = tm(fn)
+        out[name] = {"ms": round(t * 1e3, 2), "GBs": round(byts / t / 1e9, 2), "threads": 1}
+    return out
+
+
 def bench_batched(dev, count=4096):
     """4096 uneven Float32 CSR instances (rows log-uniform in [64, 65536],
     Poisson(8)+1 per row, k in {1,2,4,8,16}) in one graph-captured persistent
@@ -401,8 +480,9 @@ def main():
     ap.add_argument("--no-structural", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-reductions", action="store_true")
     ap.add_argument("--only", choices=("precisions", "views", "alpha2", "spmm", "structural",
-                                       "batched", "serving"),
+                                       "reductions", "batched", "serving"),
                     help="after the headline SpMV, run this extra section alone (for profiling)")
     ap.add_argument("--profile-only", action="store_true",
                     help="run only the headline SpMV steps (for rocprofv3)")
@@ -581,6 +661,13 @@ def main():
         if not args.no_cpu:
             extra["structural_cpu_oracle"] = cpu_structural(H)
             print(f"[bench] structural CPU (oracle, 1 thread) {extra['structural_cpu_oracle']}",
+                  file=sys.stderr, flush=True)
+    if want("reductions"):
+        extra["reductions"] = bench_reductions(H, dev)
+        print(f"[bench] reductions {extra['reductions']}", file=sys.stderr, flush=True)
+        if not args.no_cpu:
+            extra["reductions_cpu_oracle"] = cpu_reductions(H)
+            print(f"[bench] reductions CPU (oracle, 1 thread) {extra['reductions_cpu_oracle']}",
                   file=sys.stderr, flush=True)
     if want("batched"):
         extra["batched"] = bench_batched(dev)

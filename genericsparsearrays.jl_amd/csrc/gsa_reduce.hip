@@ -1,0 +1,529 @@
+// gsa_reduce.hip — traversals of the stored entries: tr, sum, dot(x, A, y),
+// issymmetric / ishermitian, and A + λI for CSR / CSC.
+//
+// "ptr form": a compressed matrix seen as `major` runs of entries (CSR: rows
+// with their column indices; CSC: columns with their row indices).  The
+// diagonal test (index == major) and the symmetry test read the same in both.
+//
+// Sums (tr, sum, dot) are accumulated in Float64 with a fixed assignment of
+// entries to workgroups and a fixed order inside each, then one workgroup adds
+// the partials in order: the result does not change from run to run.  The
+// reference adds with @atomic (tr) or per work-item partials (dot,
+// src/matrix_csr/matrix_csr_kernels.jl:47-131), so its order is undefined.
+#include "gsa_common.h"
+#include "gsa_rows.h"
+#include "gsa_scan.h"
+
+namespace gsa {
+
+constexpr int RD_PARTS = 1024;  // workgroups (partial sums) of the entry-parallel reductions
+
+// Sum over the workgroup's 256 lanes in a fixed order; lane 0 gets the total.
+__device__ __forceinline__ double wg_sum_fixed(double v, double* sh) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double t = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < ROW_WG / 64; ++w) t += sh[w];
+    __syncthreads();
+    return t;
+}
+
+__global__ void k_sum_parts(int64_t n, const double* __restrict__ part, double* __restrict__ out) {
+    __shared__ double sh[ROW_WG / 64];
+    double acc = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += ROW_WG) acc += part[i];
+    const double t = wg_sum_fixed(acc, sh);
+    if (threadIdx.x == 0) *out = t;
+}
+
+// ---- tr ----------------------------------------------------------------------
+// Entries whose index equals their major (kernel_tr, src/matrix_csr/matrix_csr.jl:118-142;
+// src/matrix_csc/matrix_csc.jl:117-140): every such entry counts, duplicates too.
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_trace_ptr(int64_t n, int64_t nnz,
+                                                      const I* __restrict__ ptr,
+                                                      const I* __restrict__ idx,
+                                                      const T* __restrict__ val,
+                                                      const int64_t* __restrict__ blk_row,
+                                                      double* __restrict__ part) {
+    __shared__ double sh[ROW_WG / 64];
+    double acc = 0.0;
+    visit_owned_rows<I>(
+        n, nnz, ptr, blk_row,
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s; j < e; ++j)
+                if ((int64_t)idx[j] - 1 == r) acc += (double)val[j];
+        },
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG)
+                if ((int64_t)idx[j] - 1 == r) acc += (double)val[j];
+        });
+    const double t = wg_sum_fixed(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// COO: entries with rowind == colind (src/matrix_coo/matrix_coo.jl:131-150);
+// with `colind` null, every stored value (sum(A) = sum(nonzeros(A)),
+// src/core.jl:24).
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_trace_coo(int64_t nnz, const I* __restrict__ rowind,
+                                                      const I* __restrict__ colind,
+                                                      const T* __restrict__ val,
+                                                      double* __restrict__ part) {
+    __shared__ double sh[ROW_WG / 64];
+    double acc = 0.0;
+    const int64_t per = (nnz + gridDim.x - 1) / gridDim.x;  // contiguous share per workgroup
+    const int64_t b0 = blockIdx.x * per, b1 = b0 + per < nnz ? b0 + per : nnz;
+    for (int64_t j = b0 + threadIdx.x; j < b1; j += ROW_WG)
+        if (!colind || rowind[j] == colind[j]) acc += (double)val[j];
+    const double t = wg_sum_fixed(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// ---- dot(x, A, y) ------------------------------------------------------------
+// Each stored entry a (major r, index i) adds x[p] * a * y[q], left to right
+// (dot(x, a, y) = conj(x) * a * y for real scalars,
+// kernel_workgroup_dot_csr_N!/T!, src/matrix_csr/matrix_csr_kernels.jl:47-131),
+// with (p, q) = (r, i) when major_left, else (i, r).
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_dot3_ptr(int64_t nmaj, int64_t nidx, int64_t nnz,
+                                                     const I* __restrict__ ptr,
+                                                     const I* __restrict__ idx,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ x,
+                                                     const T* __restrict__ y, int major_left,
+                                                     const int64_t* __restrict__ blk_row,
+                                                     double* __restrict__ part) {
+    __shared__ double sh[ROW_WG / 64];
+    double acc = 0.0;
+    auto term = [&](int64_t r, int64_t j) {
+        const int64_t i = dclamp((int64_t)idx[j] - 1, 0, nidx - 1);
+        const T xv = major_left ? x[r] : x[i];
+        const T yv = major_left ? y[i] : y[r];
+        const T t = (xv * val[j]) * yv;
+        acc += (double)t;
+    };
+    visit_owned_rows<I>(
+        nmaj, nnz, ptr, blk_row,
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s; j < e; ++j) term(r, j);
+        },
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) term(r, j);
+        });
+    const double t = wg_sum_fixed(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// COO: x[p] * a * y[q] with (p, q) = (rowind, colind), or swapped for opA = T.
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_dot3_coo(int64_t m, int64_t n, int64_t nnz,
+                                                     const I* __restrict__ rowind,
+                                                     const I* __restrict__ colind,
+                                                     const T* __restrict__ val,
+                                                     const T* __restrict__ x,
+                                                     const T* __restrict__ y, int trans,
+                                                     double* __restrict__ part) {
+    __shared__ double sh[ROW_WG / 64];
+    double acc = 0.0;
+    const int64_t per = (nnz + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = blockIdx.x * per, b1 = b0 + per < nnz ? b0 + per : nnz;
+    for (int64_t j = b0 + threadIdx.x; j < b1; j += ROW_WG) {
+        const int64_t r = dclamp((int64_t)rowind[j] - 1, 0, m - 1);
+        const int64_t c = dclamp((int64_t)colind[j] - 1, 0, n - 1);
+        const T t = trans ? (x[c] * val[j]) * y[r] : (x[r] * val[j]) * y[c];
+        acc += (double)t;
+    }
+    const double t = wg_sum_fixed(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// ---- issymmetric / ishermitian -------------------------------------------------
+// For every stored entry (major r, index c != r): a binary search of major c
+// for index r, exactly as kernel_check_symmetry_csr!/csc!
+// (src/matrix_csr/matrix_csr_kernels.jl:348-407, src/matrix_csc/matrix_csc_kernels.jl:350-405)
+// does it; the entry must be found and equal (for real values the Hermitian
+// test is the same).  flag = 0 on the first failure seen.
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_symmetric_ptr(int64_t n, int64_t nnz,
+                                                          const I* __restrict__ ptr,
+                                                          const I* __restrict__ idx,
+                                                          const T* __restrict__ val,
+                                                          const int64_t* __restrict__ blk_row,
+                                                          int* __restrict__ flag) {
+    auto check = [&](int64_t r, int64_t j) {
+        const int64_t c = (int64_t)idx[j] - 1;
+        if (c == r) return;
+        bool ok = false;
+        if (c >= 0 && c < n) {
+            int64_t lo = (int64_t)ptr[c], hi = (int64_t)ptr[c + 1] - 1;  // one-based, inclusive
+            while (lo <= hi) {
+                const int64_t mid = lo + (hi - lo) / 2;
+                const int64_t mc = (int64_t)idx[dclamp(mid - 1, 0, nnz - 1)] - 1;
+                if (mc == r) {
+                    ok = val[dclamp(mid - 1, 0, nnz - 1)] == val[j];
+                    break;
+                }
+                if (mc < r) lo = mid + 1;
+                else hi = mid - 1;
+            }
+        }
+        if (!ok) *flag = 0;
+    };
+    visit_owned_rows<I>(
+        n, nnz, ptr, blk_row,
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s; j < e; ++j) check(r, j);
+        },
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) check(r, j);
+        });
+}
+
+// ---- A + λI ------------------------------------------------------------------
+// Per major: one more entry when none of its entries sits on the diagonal
+// (kernel_count_diag_entries_csr!, src/matrix_csr/matrix_csr_kernels.jl:445-463).
+// Rows of up to ROW_SHORT entries are scanned by one lane, longer ones by the
+// whole workgroup.
+template <typename I>
+__global__ __launch_bounds__(ROW_WG) void k_diag_missing(int64_t n, int64_t nnz,
+                                                         const I* __restrict__ ptr,
+                                                         const I* __restrict__ idx,
+                                                         const int64_t* __restrict__ blk_row,
+                                                         I* __restrict__ cnt) {
+    visit_owned_rows<I>(
+        n, nnz, ptr, blk_row,
+        [&](int64_t r, int64_t s, int64_t e) {
+            bool found = false;
+            for (int64_t j = s; j < e && !found; ++j) found = (int64_t)idx[j] - 1 == r;
+            cnt[r + 1] = (I)((e - s) + (found ? 0 : 1));
+        },
+        [&](int64_t r, int64_t s, int64_t e) {
+            int f = 0;
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) f |= (int64_t)idx[j] - 1 == r;
+            f = __syncthreads_or(f);
+            if (threadIdx.x == 0) cnt[r + 1] = (I)((e - s) + (f ? 0 : 1));
+        });
+}
+
+// The smallest value over the workgroup (every lane calls it).
+__device__ __forceinline__ int64_t wg_min(int64_t v, int64_t* sh) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const int64_t u = __shfl_xor(v, o, 64);
+        v = u < v ? u : v;
+    }
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+    __syncthreads();
+    int64_t t = sh[0];
+    for (int w = 1; w < ROW_WG / 64; ++w) t = sh[w] < t ? sh[w] : t;
+    __syncthreads();
+    return t;
+}
+
+// The merge of kernel_add_uniformscaling_csr! (matrix_csr_kernels.jl:466-509)
+// for rows whose indices ascend: the entries in storage order with λ added to
+// every diagonal entry; when the major has no diagonal entry, λ inserted before
+// its first index past the diagonal (or appended).  Entry j of the major moves
+// to ptr_out + (j - start) + (1 if λ was inserted before it).  Writes stay
+// inside the major's new range.
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_add_diag(int64_t n, int64_t nnz,
+                                                     const I* __restrict__ ptr,
+                                                     const I* __restrict__ idx,
+                                                     const T* __restrict__ val, T lambda,
+                                                     const int64_t* __restrict__ blk_row,
+                                                     const I* __restrict__ ptr_out,
+                                                     I* __restrict__ idx_out,
+                                                     T* __restrict__ val_out) {
+    __shared__ int64_t sh[ROW_WG / 64];
+    auto put = [&](int64_t w, int64_t wend, I c, T v) {
+        if (w < wend) {
+            idx_out[w] = c;
+            val_out[w] = v;
+        }
+    };
+    visit_owned_rows<I>(
+        n, nnz, ptr, blk_row,
+        [&](int64_t r, int64_t s, int64_t e) {
+            const int64_t base = (int64_t)ptr_out[r] - 1, wend = (int64_t)ptr_out[r + 1] - 1;
+            bool found = false;
+            for (int64_t j = s; j < e && !found; ++j) found = (int64_t)idx[j] - 1 == r;
+            int64_t w = base;
+            bool inserted = found;
+            for (int64_t j = s; j < e; ++j) {
+                const int64_t c = (int64_t)idx[j] - 1;
+                if (!inserted && c > r) {
+                    put(w++, wend, (I)(r + 1), lambda);
+                    inserted = true;
+                }
+                put(w++, wend, idx[j], c == r ? val[j] + lambda : val[j]);
+            }
+            if (!inserted) put(w, wend, (I)(r + 1), lambda);
+        },
+        [&](int64_t r, int64_t s, int64_t e) {
+            const int64_t base = (int64_t)ptr_out[r] - 1, wend = (int64_t)ptr_out[r + 1] - 1;
+            int f = 0;
+            int64_t first_past = e;
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) {
+                const int64_t c = (int64_t)idx[j] - 1;
+                f |= c == r;
+                if (c > r && j < first_past) first_past = j;
+            }
+            f = __syncthreads_or(f);
+            const int64_t p = f ? e : wg_min(first_past, sh);
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) {
+                const int64_t c = (int64_t)idx[j] - 1;
+                put(base + (j - s) + (!f && j >= p ? 1 : 0), wend, idx[j],
+                    c == r ? val[j] + lambda : val[j]);
+            }
+            if (!f && threadIdx.x == 0) put(base + (p - s), wend, (I)(r + 1), lambda);
+        });
+}
+
+__global__ void k_set_flag(int* f) { *f = 1; }
+
+// The n diagonal entries (i, i, d_i) of a COO matrix, d_i = dvals[i] or λ
+// (kernel_fill_diagonal_coo!, src/matrix_coo/matrix_coo_kernels.jl:312-325).
+template <typename T, typename I>
+__global__ void k_diag_coo(int64_t n, const T* __restrict__ dvals, T lambda, I* __restrict__ row,
+                           I* __restrict__ col, T* __restrict__ val) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        row[i] = (I)(i + 1);
+        col[i] = (I)(i + 1);
+        val[i] = dvals ? dvals[i] : lambda;
+    }
+}
+
+// Row-block plan over `ptr` for the visit_owned_rows kernels: the caller's
+// plan when it was built on this pointer array, else one built into `tmp`.
+static int row_blocks(hipStream_t s, gsa_plan plan, int itype, int64_t n, int64_t nnz,
+                      const void* ptr, Scratch& tmp, const int64_t** blk_row, int64_t* nblocks) {
+    if (plan && plan->kind == 0 && plan->csr.m == n && plan->csr.nnz == nnz &&
+        plan->csr.itype == itype && plan->csr.ptr == ptr) {
+        *blk_row = plan->csr.blk_row;
+        *nblocks = plan->csr.nblocks;
+        return GSA_OK;
+    }
+    *nblocks = nnz / SPMV_NB + 1;
+    GSA_TRY(tmp.alloc(sizeof(int64_t) * (*nblocks + 1)));
+    *blk_row = tmp.as<int64_t>();
+    return build_csr_plan(s, itype, n, nnz, ptr, tmp.as<int64_t>(), *nblocks);
+}
+
+// Parts -> host double (synchronises).
+static int finish_sum(hipStream_t s, const double* part, int64_t nparts, double* out) {
+    Scratch res(s);
+    GSA_TRY(res.alloc(sizeof(double)));
+    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(ROW_WG), 0, s, nparts, part, res.as<double>());
+    GSA_HIP(hipGetLastError());
+    GSA_HIP(hipMemcpyAsync(out, res.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    GSA_HIP(hipStreamSynchronize(s));
+    return GSA_OK;
+}
+
+}  // namespace gsa
+
+using namespace gsa;
+
+extern "C" {
+
+int gsa_trace_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz, const void* ptr,
+                  const void* idx, const void* val, gsa_plan plan, double* out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(out && n >= 0 && nnz >= 0 && (n == 0 || ptr), GSA_ERR_INVALID,
+              "gsa_trace_ptr: bad arguments");
+    *out = 0.0;
+    if (n == 0 || nnz == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        Scratch tmp(s), part(s);
+        const int64_t* br = nullptr;
+        int64_t nb = 0;
+        GSA_TRY(row_blocks(s, plan, itype, n, nnz, ptr, tmp, &br, &nb));
+        GSA_TRY(part.alloc(sizeof(double) * nb));
+        hipLaunchKernelGGL((k_trace_ptr<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, n, nnz,
+                           (const I*)ptr, (const I*)idx, (const T*)val, br, part.as<double>());
+        GSA_HIP(hipGetLastError());
+        return finish_sum(s, part.as<double>(), nb, out);
+    });
+}
+
+int gsa_trace_coo(void* stream, int vtype, int itype, int64_t nnz, const void* rowind,
+                  const void* colind, const void* val, double* out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(out && nnz >= 0, GSA_ERR_INVALID, "gsa_trace_coo: bad arguments");
+    *out = 0.0;
+    if (nnz == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        Scratch part(s);
+        GSA_TRY(part.alloc(sizeof(double) * RD_PARTS));
+        hipLaunchKernelGGL((k_trace_coo<T, I>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, nnz,
+                           (const I*)rowind, (const I*)colind, (const T*)val, part.as<double>());
+        GSA_HIP(hipGetLastError());
+        return finish_sum(s, part.as<double>(), RD_PARTS, out);
+    });
+}
+
+int gsa_sum_values(void* stream, int vtype, int64_t nnz, const void* val, double* out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(out && nnz >= 0, GSA_ERR_INVALID, "gsa_sum_values: bad arguments");
+    *out = 0.0;
+    if (nnz == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_v(vtype, [&](auto tv) -> int {
+        using T = decltype(tv);
+        Scratch part(s);
+        GSA_TRY(part.alloc(sizeof(double) * RD_PARTS));
+        hipLaunchKernelGGL((k_trace_coo<T, int64_t>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, nnz,
+                           (const int64_t*)nullptr, (const int64_t*)nullptr, (const T*)val,
+                           part.as<double>());
+        GSA_HIP(hipGetLastError());
+        return finish_sum(s, part.as<double>(), RD_PARTS, out);
+    });
+}
+
+int gsa_dot3_ptr(void* stream, int vtype, int itype, int64_t nmaj, int64_t nidx, int64_t nnz,
+                 const void* ptr, const void* idx, const void* val, const void* x, const void* y,
+                 int major_left, gsa_plan plan, double* out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(out && nmaj >= 0 && nidx >= 0 && nnz >= 0 && (nmaj == 0 || ptr), GSA_ERR_INVALID,
+              "gsa_dot3_ptr: bad arguments");
+    *out = 0.0;
+    if (nmaj == 0 || nidx == 0 || nnz == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        Scratch tmp(s), part(s);
+        const int64_t* br = nullptr;
+        int64_t nb = 0;
+        GSA_TRY(row_blocks(s, plan, itype, nmaj, nnz, ptr, tmp, &br, &nb));
+        GSA_TRY(part.alloc(sizeof(double) * nb));
+        hipLaunchKernelGGL((k_dot3_ptr<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, nmaj, nidx,
+                           nnz, (const I*)ptr, (const I*)idx, (const T*)val, (const T*)x,
+                           (const T*)y, major_left, br, part.as<double>());
+        GSA_HIP(hipGetLastError());
+        return finish_sum(s, part.as<double>(), nb, out);
+    });
+}
+
+int gsa_dot3_coo(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnz,
+                 const void* rowind, const void* colind, const void* val, const void* x,
+                 const void* y, int trans, double* out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(out && m >= 0 && n >= 0 && nnz >= 0, GSA_ERR_INVALID, "gsa_dot3_coo: bad arguments");
+    *out = 0.0;
+    if (m == 0 || n == 0 || nnz == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        Scratch part(s);
+        GSA_TRY(part.alloc(sizeof(double) * RD_PARTS));
+        hipLaunchKernelGGL((k_dot3_coo<T, I>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, m, n, nnz,
+                           (const I*)rowind, (const I*)colind, (const T*)val, (const T*)x,
+                           (const T*)y, trans, part.as<double>());
+        GSA_HIP(hipGetLastError());
+        return finish_sum(s, part.as<double>(), RD_PARTS, out);
+    });
+}
+
+int gsa_symmetric_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
+                      const void* ptr, const void* idx, const void* val, gsa_plan plan,
+                      int* out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(out && n >= 0 && nnz >= 0 && (n == 0 || ptr), GSA_ERR_INVALID,
+              "gsa_symmetric_ptr: bad arguments");
+    *out = 1;
+    if (n == 0 || nnz == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        Scratch tmp(s), flag(s);
+        const int64_t* br = nullptr;
+        int64_t nb = 0;
+        GSA_TRY(row_blocks(s, plan, itype, n, nnz, ptr, tmp, &br, &nb));
+        GSA_TRY(flag.alloc(sizeof(int)));
+        hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, s, flag.as<int>());
+        hipLaunchKernelGGL((k_symmetric_ptr<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, n,
+                           nnz, (const I*)ptr, (const I*)idx, (const T*)val, br, flag.as<int>());
+        GSA_HIP(hipGetLastError());
+        GSA_HIP(hipMemcpyAsync(out, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        return GSA_OK;
+    });
+}
+
+int gsa_add_diag_count(void* stream, int itype, int64_t n, int64_t nnz, const void* ptr,
+                       const void* idx, gsa_plan plan, void* ptr_out, int64_t* nnz_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(nnz_out && n >= 0 && nnz >= 0 && ptr && ptr_out, GSA_ERR_INVALID,
+              "gsa_add_diag_count: bad arguments");
+    hipStream_t s = as_stream(stream);
+    return dispatch_i(itype, [&](auto ti) -> int {
+        using I = decltype(ti);
+        Scratch tmp(s);
+        if (n > 0) {
+            const int64_t* br = nullptr;
+            int64_t nb = 0;
+            GSA_TRY(row_blocks(s, plan, itype, n, nnz, ptr, tmp, &br, &nb));
+            hipLaunchKernelGGL(k_diag_missing<I>, dim3((unsigned)nb), dim3(ROW_WG), 0, s, n, nnz,
+                               (const I*)ptr, (const I*)idx, br, (I*)ptr_out);
+            GSA_HIP(hipGetLastError());
+        }
+        return counts_to_ptr<I>(s, (I*)ptr_out, n, nnz_out);
+    });
+}
+
+int gsa_add_diag_fill(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
+                      const void* ptr, const void* idx, const void* val, double lambda,
+                      gsa_plan plan, const void* ptr_out, void* idx_out, void* val_out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(n >= 0 && nnz >= 0 && ptr && ptr_out, GSA_ERR_INVALID,
+              "gsa_add_diag_fill: bad arguments");
+    if (n == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        Scratch tmp(s);
+        const int64_t* br = nullptr;
+        int64_t nb = 0;
+        GSA_TRY(row_blocks(s, plan, itype, n, nnz, ptr, tmp, &br, &nb));
+        hipLaunchKernelGGL((k_add_diag<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, n, nnz,
+                           (const I*)ptr, (const I*)idx, (const T*)val, (T)lambda, br,
+                           (const I*)ptr_out, (I*)idx_out, (T*)val_out);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    });
+}
+
+int gsa_diag_coo(void* stream, int vtype, int itype, int64_t n, const void* dvals,
+                 double lambda, void* row, void* col, void* val) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(n >= 0 && (n == 0 || (row && col && val)), GSA_ERR_INVALID,
+              "gsa_diag_coo: bad arguments");
+    if (n == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        hipLaunchKernelGGL((k_diag_coo<T, I>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,
+                           (const T*)dvals, (T)lambda, (I*)row, (I*)col, (T*)val);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    });
+}
+
+}  // extern "C"

@@ -3,13 +3,18 @@
 Types:      GenericSparseMatrixCSR, GenericSparseMatrixCSC, GenericSparseMatrixCOO
 Wrappers:   transpose / adjoint (lazy Transpose / Adjoint)
 Operators:  mul_ (mul!), A @ x / A * x, A * B (SpGEMM), A + B, A - B, kron,
-            dropzeros, dropzeros_, conversions through the constructors.
+            dropzeros, dropzeros_, conversions through the constructors,
+            tr, sum, dot(x, A, y), issymmetric, ishermitian, A ± λI (UniformScaling, I),
+            kron with a Diagonal.
 Every operator runs in libgsa_hip.so; there is no CPU fallback.
 """
 from ._lib import ArgumentError, DimensionMismatch, GSAError, load as load_library
 from .matrix import (GenericSparseMatrixCSR, GenericSparseMatrixCSC, GenericSparseMatrixCOO,
-                     Transpose, Adjoint, transpose, adjoint, adapt, nnz, nonzeros, dense)
-from .ops import mul_, times, add, kron, dropzeros, dropzeros_, spgemm, scale, negate
+                     Transpose, Adjoint, transpose, adjoint, adapt, nnz, nonzeros, dense,
+                     UniformScaling, I, Diagonal)
+from .ops import (mul_, times, add, kron, dropzeros, dropzeros_, spgemm, scale, negate, tr,
+                  sum_, dot, issymmetric, ishermitian, add_uniform_scaling)
+from .ops import sum_ as sum  # Base.sum(A) (shadows the builtin in this namespace only)
 from . import convert
 
 __all__ = [
@@ -17,5 +22,6 @@ __all__ = [
     "GenericSparseMatrixCSR", "GenericSparseMatrixCSC", "GenericSparseMatrixCOO",
     "Transpose", "Adjoint", "transpose", "adjoint", "adapt", "nnz", "nonzeros", "dense",
     "mul_", "times", "add", "kron", "dropzeros", "dropzeros_", "spgemm", "scale", "negate",
-    "convert",
+    "convert", "UniformScaling", "I", "Diagonal", "tr", "sum_", "dot", "issymmetric",
+    "ishermitian", "add_uniform_scaling",
 ]

@@ -101,6 +101,18 @@ _SIGS = {
                              _vp, _i64, ctypes.POINTER(_vp), _pi64],
     "gsa_spgemm_csr_finish": [_vp, _vp, _vp, _vp],
     "gsa_ctx_destroy": [_vp],
+    "gsa_trace_ptr": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp, ctypes.POINTER(_f64)],
+    "gsa_trace_coo": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, ctypes.POINTER(_f64)],
+    "gsa_sum_values": [_vp, _i32, _i64, _vp, ctypes.POINTER(_f64)],
+    "gsa_dot3_ptr": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp,
+                     ctypes.POINTER(_f64)],
+    "gsa_dot3_coo": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i32,
+                     ctypes.POINTER(_f64)],
+    "gsa_symmetric_ptr": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp,
+                          ctypes.POINTER(_i32)],
+    "gsa_add_diag_count": [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _pi64],
+    "gsa_add_diag_fill": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp, _vp],
+    "gsa_diag_coo": [_vp, _i32, _i32, _i64, _vp, _f64, _vp, _vp, _vp],
     "gsa_batch_plan_create": [_vp, _i32, _i32, _i64, ctypes.POINTER(BatchDesc), _i32,
                               ctypes.POINTER(_vp)],
     "gsa_batch_spmm": [_vp, _vp, _f32, _f32],

@@ -219,8 +219,18 @@ class _PlanCache:
             "gsa_plan_create_coo", stream_of(rowind.device), itype_of(rowind), m, n, nnz,
             ptr(rowind), ptr(colind), ctypes.byref(h)), (rowind, colind), m, n, nnz)
 
+    @staticmethod
+    def _key(kind, arrays, dims):
+        return (kind[0],) + tuple((t.data_ptr(), t.dtype, t._version) for t in arrays) + dims
+
+    def peek(self, kind, arrays, *dims):
+        """The cached plan if it was built on these arrays, else None (builds nothing)."""
+        if self.handle is not None and self.key == self._key(kind, arrays, dims):
+            return self.handle
+        return None
+
     def _get(self, kind, create, arrays, *dims):
-        key = (kind[0],) + tuple((t.data_ptr(), t.dtype, t._version) for t in arrays) + dims
+        key = self._key(kind, arrays, dims)
         if self.handle is not None and self.key == key:
             return self.handle
         self.release()
@@ -288,6 +298,10 @@ class GenericSparseMatrixCSR(AbstractGenericSparseMatrix):
     def plan(self):
         return self._plan.get(self.rowptr, self.m, self.nnz(), self.colval, self.n)
 
+    def cached_plan(self):
+        """plan() if it is already built for the current arrays, else None."""
+        return self._plan.peek(("csr",), (self.rowptr, self.colval), self.m, self.nnz(), self.n)
+
     def copy(self):
         return GenericSparseMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval)
 
@@ -342,6 +356,10 @@ class GenericSparseMatrixCSC(AbstractGenericSparseMatrix):
 
     def plan(self):
         return self._plan.get_csc(self.colptr, self.rowval, self.m, self.n, self.nnz())
+
+    def cached_plan(self):
+        """plan() if it is already built for the current arrays, else None."""
+        return self._plan.peek(("csc",), (self.colptr, self.rowval), self.m, self.n, self.nnz())
 
     def copy(self):
         return GenericSparseMatrixCSC(self.m, self.n, self.colptr, self.rowval, self.nzval)
@@ -464,6 +482,109 @@ class Adjoint(Transpose):
     conj = True
 
 
+def scalar_dtype(x):
+    """The element type a scalar brings to Julia's promote_type: None for Bool and
+    integers (they never widen Float32 or Float64), float64 for a Python float
+    (a Julia Float64 literal), the scalar's own type for numpy and torch scalars."""
+    if isinstance(x, (bool, int, np.bool_, np.integer)):
+        return None
+    if isinstance(x, np.floating):
+        return torch.float32 if x.dtype == np.float32 else torch.float64
+    if isinstance(x, torch.Tensor):
+        if x.numel() != 1:
+            raise ArgumentError("a scalar is needed")
+        return x.dtype if x.dtype.is_floating_point else None
+    if isinstance(x, float):
+        return torch.float64
+    raise ArgumentError(f"unsupported scalar {type(x).__name__}")
+
+
+def promote_dtype(*dts):
+    """promote_type over Float32 / Float64 (None = a type that widens nothing)."""
+    dts = [d for d in dts if d is not None]
+    if not dts:
+        return None
+    return torch.float64 if torch.float64 in dts else torch.float32
+
+
+class UniformScaling:
+    """LinearAlgebra.UniformScaling λI of any size.  `I` is λ = true.
+
+    A ± λI, λI ± A (src/core.jl:89-94) and A*λI, λI*A (src/core.jl:46-47)
+    dispatch here.  λ's element type joins A's by `scalar_dtype` / `promote_dtype`
+    (Julia's promote_type(Tv, typeof(λ)))."""
+
+    # numpy scalars defer to __rmul__ (so np.float32(α) * I keeps α's type)
+    __array_ufunc__ = None
+
+    def __init__(self, lam=True, dtype=...):
+        self.lam = lam
+        # dtype: the element type λ brings (None: one that widens nothing)
+        self.dtype = scalar_dtype(lam) if dtype is ... else dtype
+
+    @property
+    def value(self) -> float:
+        return float(self.lam.item() if isinstance(self.lam, torch.Tensor) else self.lam)
+
+    def __mul__(self, other):
+        if isinstance(other, (AbstractGenericSparseMatrix, Transpose)):
+            from . import ops
+            return ops.scale(other, self.value)
+        return UniformScaling(self.value * float(other),
+                              promote_dtype(self.dtype, scalar_dtype(other)))
+
+    def __rmul__(self, other):
+        return UniformScaling(float(other) * self.value,
+                              promote_dtype(self.dtype, scalar_dtype(other)))
+
+    def __neg__(self):
+        return UniformScaling(-self.value, self.dtype)
+
+    def __add__(self, other):
+        from . import ops
+        return ops.add(other, self)
+
+    __radd__ = __add__
+
+    def __sub__(self, other):
+        from . import ops
+        return ops.add(ops.negate(other), self)
+
+    def __rsub__(self, other):
+        from . import ops
+        return ops.add(other, -self)
+
+    def __repr__(self):
+        return f"UniformScaling({self.lam!r})"
+
+
+I = UniformScaling(True)
+
+
+class Diagonal:
+    """LinearAlgebra.Diagonal with its values in a device vector `diag` (n x n)."""
+
+    def __init__(self, diag: torch.Tensor):
+        if not isinstance(diag, torch.Tensor) or diag.dim() != 1:
+            raise ArgumentError("Diagonal needs a 1-D tensor of values")
+        self.diag = diag
+
+    @property
+    def shape(self):
+        n = int(self.diag.shape[0])
+        return (n, n)
+
+    def size(self, dim=None):
+        return self.shape if dim is None else self.shape[dim - 1]
+
+    @property
+    def dtype(self):
+        return self.diag.dtype
+
+    def collect(self) -> np.ndarray:
+        return np.diag(self.diag.detach().cpu().numpy())
+
+
 def transpose(A):
     return A.parent if isinstance(A, Transpose) and not isinstance(A, Adjoint) else Transpose(A)
 
@@ -498,6 +619,7 @@ def nonzeros(A):
 __all__ = [
     "GenericSparseMatrixCSR", "GenericSparseMatrixCSC", "GenericSparseMatrixCOO",
     "Transpose", "Adjoint", "transpose", "adjoint", "adapt", "nnz", "nonzeros", "dense",
+    "UniformScaling", "I", "Diagonal",
 ]
 
 # keep a module-level registry of live plans so interpreter shutdown frees them first

@@ -1,4 +1,5 @@
-"""Operators of the sparse kernel layer: mul!, *, +, -, kron, dropzeros.
+"""Operators of the sparse kernel layer: mul!, *, +, -, kron, dropzeros, tr, sum,
+dot(x, A, y), issymmetric / ishermitian, A ± λI, kron with a Diagonal.
 
 Each function keeps the reference's argument checks, messages and result
 types, then calls libgsa_hip.so.  References are to GenericSparseArrays.jl
@@ -8,13 +9,14 @@ from __future__ import annotations
 
 import ctypes
 
+import numpy as np
 import torch
 
 from . import _lib as L
 from ._lib import ArgumentError, DimensionMismatch
 from .matrix import (GenericSparseMatrixCSR, GenericSparseMatrixCSC, GenericSparseMatrixCOO,
-                     SPARSE_TYPES, Transpose, Adjoint, itype_of, vtype_of, ptr, stream_of,
-                     require_device, similar_dense)
+                     SPARSE_TYPES, Transpose, Adjoint, UniformScaling, Diagonal, itype_of,
+                     vtype_of, ptr, stream_of, require_device, similar_dense, promote_dtype)
 from . import convert
 
 _PROMOTE_OK = {
@@ -127,6 +129,8 @@ def times(A, B):
         return mul_(C, A, B, 1.0, 0.0)
     if isinstance(B, (int, float)):
         return scale(A, B)
+    if isinstance(B, UniformScaling):  # A * J = A * J.λ (src/core.jl:46)
+        return scale(A, B.value)
     return spgemm(A, B)
 
 
@@ -142,6 +146,8 @@ def scale(A, alpha):
 
 def negate(A):
     """Base.:-(A) (src/matrix_csr/matrix_csr.jl:88-90)."""
+    if isinstance(A, UniformScaling):
+        return -A
     if isinstance(A, Transpose):
         return type(A)(negate(A.parent))
     B = A.copy()
@@ -248,7 +254,12 @@ def _coo_add(A, B):
 
 def add(A, B):
     """Base.:+(A, B) for sparse operands (src/matrix_csr/matrix_csr.jl:290-355,409-440;
-    src/matrix_csc/matrix_csc.jl:292-357,411-442; src/matrix_coo/matrix_coo.jl:300-384,461-582)."""
+    src/matrix_csc/matrix_csc.jl:292-357,411-442; src/matrix_coo/matrix_coo.jl:300-384,461-582).
+    A UniformScaling on either side is A + λI (src/core.jl:89-94)."""
+    if isinstance(A, UniformScaling):
+        A, B = B, A
+    if isinstance(B, UniformScaling):
+        return add_uniform_scaling(A, B)
     if tuple(A.shape) != tuple(B.shape):
         raise DimensionMismatch(f"dimensions must match: A has dims {tuple(A.shape)}, "
                                 f"B has dims {tuple(B.shape)}")
@@ -277,7 +288,11 @@ def add(A, B):
 # ------------------------------------------------------------ kron ---------
 def kron(A, B):
     """LinearAlgebra.kron for sparse operands (src/matrix_coo/matrix_coo.jl:641-699;
-    CSR/CSC route through COO, src/matrix_csr/matrix_csr.jl:442-458)."""
+    CSR/CSC route through COO, src/matrix_csr/matrix_csr.jl:442-458), and with a
+    Diagonal on either side (src/matrix_coo/matrix_coo.jl:704-809,
+    src/matrix_csr/matrix_csr.jl:460-496, src/matrix_csc/matrix_csc.jl:462-498)."""
+    if isinstance(A, Diagonal) or isinstance(B, Diagonal):
+        return _kron_diagonal(A, B)
     Ap, opA = _unwrap(A)
     Bp, opB = _unwrap(B)
     if type(Ap) is not type(Bp):
@@ -395,3 +410,203 @@ def spgemm(A, B):
                                     Ap.nzval.to(T), Ap.nnz())
         return GenericSparseMatrixCSC(Ap.m, Bp.n, pC, iC, vC, copy=False)
     raise ArgumentError(f"*: unsupported operand {type(A).__name__}")
+
+
+# ------------------------------------------------------------ reductions ----
+def _as_scalar(v: float, dtype):
+    """The host result in the element type (a Float32 result is rounded once)."""
+    return float(np.float32(v)) if dtype == torch.float32 else float(v)
+
+
+def _ptr_form(A):
+    """(major count, index count, ptr, idx) of a CSR (rows) or CSC (columns)."""
+    if isinstance(A, GenericSparseMatrixCSR):
+        return A.m, A.n, A.rowptr, A.colval
+    return A.n, A.m, A.colptr, A.rowval
+
+
+def tr(A):
+    """LinearAlgebra.tr (src/matrix_csr/matrix_csr.jl:118-142,
+    src/matrix_csc/matrix_csc.jl:117-140, src/matrix_coo/matrix_coo.jl:131-150);
+    tr(transpose(A)) = tr(A) for real values."""
+    Ap, _ = _unwrap(A)
+    if not isinstance(Ap, SPARSE_TYPES):
+        raise ArgumentError(f"tr: unsupported operand {type(A).__name__}")
+    if Ap.m != Ap.n:
+        raise DimensionMismatch("Matrix must be square to compute the trace.")
+    dev = require_device(Ap._idx(), Ap.nzval)
+    out = ctypes.c_double()
+    if isinstance(Ap, GenericSparseMatrixCOO):
+        L.call("gsa_trace_coo", stream_of(dev), vtype_of(Ap.nzval), itype_of(Ap.rowind), Ap.nnz(),
+               ptr(Ap.rowind), ptr(Ap.colind), ptr(Ap.nzval), ctypes.byref(out))
+    else:
+        major, _, pt, ix = _ptr_form(Ap)
+        L.call("gsa_trace_ptr", stream_of(dev), vtype_of(Ap.nzval), itype_of(pt), major, Ap.nnz(),
+               ptr(pt), ptr(ix), ptr(Ap.nzval), Ap.cached_plan(), ctypes.byref(out))
+    return _as_scalar(out.value, Ap.nzval.dtype)
+
+
+def sum_(A):
+    """Base.sum(A) = sum(nonzeros(A)) (src/core.jl:24)."""
+    Ap, _ = _unwrap(A)
+    if not isinstance(Ap, SPARSE_TYPES):
+        raise ArgumentError(f"sum: unsupported operand {type(A).__name__}")
+    dev = require_device(Ap.nzval)
+    out = ctypes.c_double()
+    L.call("gsa_sum_values", stream_of(dev), vtype_of(Ap.nzval), Ap.nnz(), ptr(Ap.nzval),
+           ctypes.byref(out))
+    return _as_scalar(out.value, Ap.nzval.dtype)
+
+
+def dot(x: torch.Tensor, A, y: torch.Tensor):
+    """LinearAlgebra.dot(x, A, y) = x' A y over the stored entries
+    (src/matrix_csr/matrix_csr.jl:214-277, src/matrix_csc/matrix_csc.jl:216-279,
+    src/matrix_coo/matrix_coo.jl:224-285: same checks and messages)."""
+    Ap, opA = _unwrap(A)
+    if not isinstance(Ap, SPARSE_TYPES):
+        raise ArgumentError(f"dot: unsupported operand {type(A).__name__}")
+    if not isinstance(x, torch.Tensor) or not isinstance(y, torch.Tensor) or \
+            x.dim() != 1 or y.dim() != 1:
+        raise ArgumentError("dot(x, A, y): x and y must be device vectors")
+    mA, nA = (Ap.m, Ap.n) if opA == L.GSA_OP_N else (Ap.n, Ap.m)
+    if mA != x.shape[0]:
+        raise DimensionMismatch(f"first dimension of A, {mA}, does not match the length of x, "
+                                f"{x.shape[0]}")
+    if nA != y.shape[0]:
+        raise DimensionMismatch(f"second dimension of A, {nA}, does not match the length of y, "
+                                f"{y.shape[0]}")
+    dev = require_device(x, Ap.nzval, y)
+    T = promote_dtype(Ap.nzval.dtype, x.dtype, y.dtype)
+    if T is None or any(d not in (torch.float32, torch.float64)
+                        for d in (Ap.nzval.dtype, x.dtype, y.dtype)):
+        raise ArgumentError("dot(x, A, y): Float32 and Float64 operands are provided")
+    vals = Ap.nzval if Ap.nzval.dtype == T else Ap.nzval.to(T)
+    xt = x.to(T).contiguous()
+    yt = y.to(T).contiguous()
+    out = ctypes.c_double()
+    s = stream_of(dev)
+    if isinstance(Ap, GenericSparseMatrixCOO):
+        L.call("gsa_dot3_coo", s, vtype_of(vals), itype_of(Ap.rowind), Ap.m, Ap.n, Ap.nnz(),
+               ptr(Ap.rowind), ptr(Ap.colind), ptr(vals), ptr(xt), ptr(yt),
+               int(opA != L.GSA_OP_N), ctypes.byref(out))
+    else:
+        major, minor, pt, ix = _ptr_form(Ap)
+        # x pairs with the row: the major of CSR, the index of CSC (swapped by a wrapper)
+        major_left = isinstance(Ap, GenericSparseMatrixCSR) == (opA == L.GSA_OP_N)
+        L.call("gsa_dot3_ptr", s, vtype_of(vals), itype_of(pt), major, minor, Ap.nnz(), ptr(pt),
+               ptr(ix), ptr(vals), ptr(xt), ptr(yt), int(major_left), Ap.cached_plan(),
+               ctypes.byref(out))
+    return _as_scalar(out.value, T)
+
+
+def issymmetric(A) -> bool:
+    """LinearAlgebra.issymmetric (src/matrix_csr/matrix_csr.jl:611-628,
+    src/matrix_csc/matrix_csc.jl:616-633; COO through CSC, src/matrix_coo/matrix_coo.jl:811-815;
+    wrappers test the parent, src/core.jl:96-99)."""
+    Ap, _ = _unwrap(A)
+    if not isinstance(Ap, SPARSE_TYPES):
+        raise ArgumentError(f"issymmetric: unsupported operand {type(A).__name__}")
+    if Ap.m != Ap.n:
+        return False
+    if isinstance(Ap, GenericSparseMatrixCOO):
+        Ap = convert.coo_to_csc(Ap)
+    if Ap.nnz() == 0:
+        return True
+    major, _, pt, ix = _ptr_form(Ap)
+    dev = require_device(pt, ix, Ap.nzval)
+    out = ctypes.c_int()
+    L.call("gsa_symmetric_ptr", stream_of(dev), vtype_of(Ap.nzval), itype_of(pt), major, Ap.nnz(),
+           ptr(pt), ptr(ix), ptr(Ap.nzval), Ap.cached_plan(), ctypes.byref(out))
+    return bool(out.value)
+
+
+def ishermitian(A) -> bool:
+    """LinearAlgebra.ishermitian (src/matrix_csr/matrix_csr.jl:630-647): for the real
+    element types provided, conj(v) == v and the test is issymmetric's."""
+    return issymmetric(A)
+
+
+# ------------------------------------------------------------ A + λI --------
+def _materialise(A):
+    """A wrapped matrix as a plain one of its parent's format."""
+    if not isinstance(A, Transpose):
+        return A
+    p = A.parent
+    if isinstance(p, GenericSparseMatrixCSR):
+        return convert.to_csr(A)
+    if isinstance(p, GenericSparseMatrixCSC):
+        return convert.to_csc(A)
+    return convert.to_coo(A)
+
+
+def _diag_coo(n: int, dev, vdtype, idtype, lam: float = 0.0, dvals=None):
+    """The n diagonal entries (i, i, d_i) as a COO matrix (kernel_fill_diagonal_coo!,
+    src/matrix_coo/matrix_coo_kernels.jl:312-325)."""
+    r = torch.empty(n, dtype=idtype, device=dev)
+    c = torch.empty(n, dtype=idtype, device=dev)
+    v = torch.empty(n, dtype=vdtype, device=dev)
+    src = None
+    if dvals is not None:
+        src = dvals if dvals.dtype == vdtype else dvals.to(vdtype)
+        src = src.contiguous()
+    L.call("gsa_diag_coo", stream_of(dev), vtype_of(v), itype_of(r), n, ptr(src), float(lam),
+           ptr(r), ptr(c), ptr(v))
+    return GenericSparseMatrixCOO(n, n, r, c, v, copy=False)
+
+
+def add_uniform_scaling(A, J: UniformScaling):
+    """Base.:+(A, λI) (src/matrix_csr/matrix_csr.jl:357-406, src/matrix_csc/matrix_csc.jl:359-408,
+    src/matrix_coo/matrix_coo.jl:386-457): λ == 0 returns a copy; otherwise the
+    element type is promote_type(Tv, typeof(λ))."""
+    A = _materialise(A)
+    if not isinstance(A, SPARSE_TYPES):
+        raise ArgumentError(f"+: unsupported operand {type(A).__name__}")
+    if A.m != A.n:
+        raise DimensionMismatch("Matrix must be square to add UniformScaling.")
+    lam = J.value
+    if lam == 0:
+        return A.copy()
+    T = promote_dtype(A.nzval.dtype, J.dtype)
+    dev = require_device(A._idx(), A.nzval)
+    if isinstance(A, GenericSparseMatrixCOO):
+        # concatenate the n entries λ, sum duplicates in column-major order, drop zeros
+        D = _diag_coo(A.n, dev, T, A.rowind.dtype, lam=lam)
+        return _coo_add(A, D)
+    major, _, pt, ix = _ptr_form(A)
+    vals = A.nzval if A.nzval.dtype == T else A.nzval.to(T)
+    s = stream_of(dev)
+    newptr = torch.empty(major + 1, dtype=pt.dtype, device=dev)
+    cnt = ctypes.c_int64()
+    plan = A.cached_plan()
+    L.call("gsa_add_diag_count", s, itype_of(pt), major, A.nnz(), ptr(pt), ptr(ix), plan,
+           ptr(newptr), ctypes.byref(cnt))
+    newix = torch.empty(cnt.value, dtype=pt.dtype, device=dev)
+    newv = torch.empty(cnt.value, dtype=T, device=dev)
+    L.call("gsa_add_diag_fill", s, vtype_of(vals), itype_of(pt), major, A.nnz(), ptr(pt), ptr(ix),
+           ptr(vals), lam, plan, ptr(newptr), ptr(newix), ptr(newv))
+    return type(A)(A.m, A.n, newptr, newix, newv, copy=False)
+
+
+# ------------------------------------------------------------ kron(Diagonal) -
+def _kron_diagonal(A, B):
+    """kron(D, B) and kron(A, D): kron of COO operands with D's n entries (i, i, d_i),
+    the entry order of kernel_kron_diag_coo! / kernel_kron_coo_diag!
+    (src/matrix_coo/matrix_coo_kernels.jl:185-258); a CSR or CSC operand gives
+    a result of its format by conversion from that COO (src/matrix_csr/matrix_csr.jl:460-470)."""
+    if isinstance(A, Diagonal) and isinstance(B, Diagonal):
+        raise ArgumentError("kron of two Diagonals is not a sparse operation here")
+    D, S = (A, B) if isinstance(A, Diagonal) else (B, A)
+    Sp, _ = _unwrap(S)
+    if not isinstance(Sp, SPARSE_TYPES):
+        raise ArgumentError(f"kron: unsupported operand {type(S).__name__}")
+    Scoo = convert.to_coo(S)
+    dev = require_device(Scoo.rowind, Scoo.nzval, D.diag)
+    if D.diag.dtype not in (torch.float32, torch.float64):
+        raise ArgumentError("kron: Diagonal values must be Float32 or Float64")
+    Dcoo = _diag_coo(D.shape[0], dev, D.diag.dtype, Scoo.rowind.dtype, dvals=D.diag)
+    C = _kron_coo(Dcoo, Scoo) if D is A else _kron_coo(Scoo, Dcoo)
+    if isinstance(Sp, GenericSparseMatrixCSR):
+        return convert.coo_to_csr(C)
+    if isinstance(Sp, GenericSparseMatrixCSC):
+        return convert.coo_to_csc(C)
+    return C

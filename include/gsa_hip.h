@@ -287,6 +287,79 @@ int gsa_spgemm_csr_finish(gsa_ctx ctx, void* rowptrC, void* colC, void* valC);
 /* Release a ctx without finishing (error paths). */
 int gsa_ctx_destroy(gsa_ctx ctx);
 
+/* ========================== reductions over stored entries ================= */
+/* "ptr form": CSR as (n = m rows, ptr = rowptr, idx = colval); CSC as
+ * (n = columns, ptr = colptr, idx = rowval).  The sums are accumulated in
+ * Float64 in a fixed order (the same result on every call) and returned on the
+ * host; the call synchronises the stream.  `plan` (may be NULL) is a plan built
+ * on this `ptr` (gsa_plan_create, or gsa_plan_create_csc for CSC): its row
+ * blocks are reused, else they are rebuilt for the call.  Majors of up to 64
+ * entries are read by one lane, longer ones by a whole workgroup.
+ *
+ * tr: the sum of the entries whose index equals their row (column), duplicates
+ * included.  The caller checks squareness.  Replaces LinearAlgebra.tr(::CSR)
+ * src/matrix_csr/matrix_csr.jl:118-142 and tr(::CSC) src/matrix_csc/matrix_csc.jl:117-140. */
+int gsa_trace_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
+                  const void* ptr, const void* idx, const void* val, gsa_plan plan, double* out);
+/* tr(::COO): entries with rowind == colind.  src/matrix_coo/matrix_coo.jl:131-150. */
+int gsa_trace_coo(void* stream, int vtype, int itype, int64_t nnz, const void* rowind,
+                  const void* colind, const void* val, double* out);
+/* sum(A) = sum(nonzeros(A)) for every format.  src/core.jl:24. */
+int gsa_sum_values(void* stream, int vtype, int64_t nnz, const void* val, double* out);
+/* dot(x, A, y) = sum over stored entries of (x[p]*a)*y[q], each term rounded
+ * in the value type: (p, q) = (major, index) when major_left != 0, else
+ * (index, major).  CSR: major_left = 1 for A, 0 for transpose(A)/adjoint(A);
+ * CSC the opposite.  x and y have the lengths the caller checked (size(opA(A))).
+ * Replaces LinearAlgebra.dot(x, ::CSR, y) src/matrix_csr/matrix_csr.jl:214-277
+ * (kernels matrix_csr_kernels.jl:47-131) and dot(x, ::CSC, y)
+ * src/matrix_csc/matrix_csc.jl:216-279 (kernels matrix_csc_kernels.jl:47-131). */
+int gsa_dot3_ptr(void* stream, int vtype, int itype, int64_t nmaj, int64_t nidx, int64_t nnz,
+                 const void* ptr, const void* idx, const void* val, const void* x, const void* y,
+                 int major_left, gsa_plan plan, double* out);
+/* dot(x, op(A), y) for COO (m x n): (x[row]*a)*y[col], or (x[col]*a)*y[row] when
+ * trans != 0.  src/matrix_coo/matrix_coo.jl:224-285 (kernels matrix_coo_kernels.jl:45-131). */
+int gsa_dot3_coo(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnz,
+                 const void* rowind, const void* colind, const void* val, const void* x,
+                 const void* y, int trans, double* out);
+/* issymmetric / ishermitian for a square matrix in ptr form: *out = 1 when
+ * every off-diagonal entry (r, c) finds, by binary search of major c, an entry
+ * at index r with an equal value (the indices of each major sorted, as the
+ * reference's search assumes); for real values the two tests coincide.
+ * Replaces issymmetric/ishermitian(::CSR) src/matrix_csr/matrix_csr.jl:611-647
+ * (kernel_check_symmetry_csr! matrix_csr_kernels.jl:348-407) and (::CSC)
+ * src/matrix_csc/matrix_csc.jl:616-652.  COO converts to CSC first
+ * (src/matrix_coo/matrix_coo.jl:811-821). */
+int gsa_symmetric_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
+                      const void* ptr, const void* idx, const void* val, gsa_plan plan,
+                      int* out);
+
+/* ========================== A + λI ========================================= */
+/* CSR / CSC (ptr form, n x n) + UniformScaling λ != 0, two phases.  count:
+ * ptr_out (n+1) gets the new pointer, one more entry for each major with no
+ * diagonal entry; *nnz_out its total (synchronises).  fill: the entries in
+ * storage order with λ inserted before the first index past the diagonal (or
+ * appended), and λ added to each stored diagonal entry; `plan` as for tr.
+ * The indices of each major ascend (the reference's merge assumes it).  Replaces
+ * Base.:+(::CSR, ::UniformScaling) src/matrix_csr/matrix_csr.jl:357-406
+ * (kernels matrix_csr_kernels.jl:445-509) and (::CSC)
+ * src/matrix_csc/matrix_csc.jl:359-408.  COO + λI is gsa_coo_add_begin with
+ * the n diagonal entries as the second operand (src/matrix_coo/matrix_coo.jl:386-457). */
+int gsa_add_diag_count(void* stream, int itype, int64_t n, int64_t nnz, const void* ptr,
+                       const void* idx, gsa_plan plan, void* ptr_out, int64_t* nnz_out);
+int gsa_add_diag_fill(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
+                      const void* ptr, const void* idx, const void* val, double lambda,
+                      gsa_plan plan, const void* ptr_out, void* idx_out, void* val_out);
+
+/* The n diagonal entries (i, i, d_i), i = 1..n, of an n x n COO matrix: d_i =
+ * dvals[i] (a Diagonal's values, in the value type) or, when dvals is null,
+ * lambda.  The second operand of COO + λI (kernel_fill_diagonal_coo!
+ * src/matrix_coo/matrix_coo_kernels.jl:312-325) and of kron with a Diagonal,
+ * which is kron(COO, COO) with this operand (kernel_kron_diag_coo! /
+ * kernel_kron_coo_diag! matrix_coo_kernels.jl:185-258 index the same entries;
+ * src/matrix_coo/matrix_coo.jl:704-809). */
+int gsa_diag_coo(void* stream, int vtype, int itype, int64_t n, const void* dvals,
+                 double lambda, void* row, void* col, void* val);
+
 /* ========================== batched uneven CSR SpMM ======================== */
 /* Many independent CSR matrices in ONE persistent launch.  Each instance i has
  * its own (m_i, n_i, k_i) and pointers; the descriptor table lives in device

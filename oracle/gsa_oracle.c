@@ -320,6 +320,119 @@ void ora_cumsum(I n, const I* x, I* out) {
 DEFINE_OPS(double, f64)
 DEFINE_OPS(float, f32)
 
+/* ---------------------------------------------------------------------------
+ * Reductions over the stored entries, A + λI.  "ptr form": CSR as (rows,
+ * rowptr, colval), CSC as (columns, colptr, rowval).  The reference adds the
+ * terms of tr with @atomic and those of dot in per-work-item partials, so its
+ * summation order is undefined; here terms are added in storage order in
+ * double, and *abs_out (if not NULL) gets the sum of |term| for the error bound.
+ * ------------------------------------------------------------------------- */
+#define DEFINE_REDUCE(T, S)                                                         \
+/* kernel_tr (src/matrix_csr/matrix_csr.jl:118-142, matrix_csc.jl:117-140):        \
+ * entries whose index equals their major, duplicates included. */                 \
+double ora_trace_ptr_##S(I n, const I* ptr, const I* idx, const T* val, double* abs_out) { \
+    double s = 0.0, a = 0.0;                                                        \
+    for (I r = 0; r < n; ++r)                                                       \
+        for (I j = ptr[r] - 1; j < ptr[r + 1] - 1; ++j)                             \
+            if (idx[j] == r + 1) { s += (double)val[j]; a += val[j] < 0 ? -(double)val[j] : (double)val[j]; } \
+    if (abs_out) *abs_out = a;                                                      \
+    return s;                                                                       \
+}                                                                                   \
+/* tr(::COO) (src/matrix_coo/matrix_coo.jl:131-150): rowind == colind. */           \
+double ora_trace_coo_##S(I nnz, const I* r, const I* c, const T* val, double* abs_out) { \
+    double s = 0.0, a = 0.0;                                                        \
+    for (I j = 0; j < nnz; ++j)                                                     \
+        if (r[j] == c[j]) { s += (double)val[j]; a += val[j] < 0 ? -(double)val[j] : (double)val[j]; } \
+    if (abs_out) *abs_out = a;                                                      \
+    return s;                                                                       \
+}                                                                                   \
+/* dot(x, A, y) (kernel_workgroup_dot_csr_N!/T!, src/matrix_csr/matrix_csr_kernels.jl:47-131;\
+ * csc :47-131): term = (x[p]*a)*y[q] rounded in T (dot(x,a,y) = conj(x)*a*y),      \
+ * (p, q) = (major, index) when major_left, else (index, major). */                \
+double ora_dot3_ptr_##S(I nmaj, const I* ptr, const I* idx, const T* val, const T* x, \
+                        const T* y, int major_left, double* abs_out) {              \
+    double s = 0.0, a = 0.0;                                                        \
+    for (I r = 0; r < nmaj; ++r)                                                    \
+        for (I j = ptr[r] - 1; j < ptr[r + 1] - 1; ++j) {                           \
+            I i = idx[j] - 1;                                                       \
+            T t = major_left ? (x[r] * val[j]) * y[i] : (x[i] * val[j]) * y[r];     \
+            s += (double)t; a += t < 0 ? -(double)t : (double)t;                   \
+        }                                                                           \
+    if (abs_out) *abs_out = a;                                                      \
+    return s;                                                                       \
+}                                                                                   \
+/* dot(x, op(A::COO), y) (kernel_workgroup_dot_coo_N!/T!,                          \
+ * src/matrix_coo/matrix_coo_kernels.jl:45-131). */                                 \
+double ora_dot3_coo_##S(I nnz, const I* rr, const I* cc, const T* val, const T* x,  \
+                        const T* y, int trans, double* abs_out) {                   \
+    double s = 0.0, a = 0.0;                                                        \
+    for (I j = 0; j < nnz; ++j) {                                                   \
+        I r = rr[j] - 1, c = cc[j] - 1;                                             \
+        T t = trans ? (x[c] * val[j]) * y[r] : (x[r] * val[j]) * y[c];              \
+        s += (double)t; a += t < 0 ? -(double)t : (double)t;                       \
+    }                                                                               \
+    if (abs_out) *abs_out = a;                                                      \
+    return s;                                                                       \
+}                                                                                   \
+/* kernel_check_symmetry_csr! (src/matrix_csr/matrix_csr_kernels.jl:348-407): for  \
+ * each off-diagonal entry (r, c), binary search of major c (lo/hi/mid one-based,  \
+ * mid = (lo+hi)÷2) for index r; found and equal, else not symmetric.  Real       \
+ * values: the Hermitian test is the same. */                                       \
+int ora_symmetric_ptr_##S(I n, const I* ptr, const I* idx, const T* val) {          \
+    for (I r = 0; r < n; ++r)                                                       \
+        for (I j = ptr[r] - 1; j < ptr[r + 1] - 1; ++j) {                           \
+            I c = idx[j];                                                           \
+            if (c == r + 1) continue;                                               \
+            I lo = ptr[c - 1], hi = ptr[c] - 1;                                     \
+            int found = 0;                                                          \
+            while (lo <= hi) {                                                      \
+                I mid = (lo + hi) / 2;                                              \
+                I mc = idx[mid - 1];                                                \
+                if (mc == r + 1) {                                                  \
+                    if (val[mid - 1] != val[j]) return 0;                           \
+                    found = 1;                                                      \
+                    break;                                                          \
+                } else if (mc < r + 1) lo = mid + 1;                                \
+                else hi = mid - 1;                                                  \
+            }                                                                       \
+            if (!found) return 0;                                                   \
+        }                                                                           \
+    return 1;                                                                       \
+}                                                                                   \
+/* Base.:+(::CSR, ::UniformScaling) (src/matrix_csr/matrix_csr.jl:357-406):        \
+ * kernel_count_diag_entries_csr! then kernel_add_uniformscaling_csr!             \
+ * (matrix_csr_kernels.jl:445-509).  Returns nnz; outputs may be NULL to count. */  \
+I ora_add_diag_##S(I n, const I* ptr, const I* idx, const T* val, T lambda,         \
+                   I* ptr_out, I* idx_out, T* val_out) {                            \
+    I w = 0;                                                                        \
+    if (ptr_out) ptr_out[0] = 1;                                                    \
+    for (I r = 0; r < n; ++r) {                                                     \
+        int written = 0;                                                            \
+        for (I j = ptr[r] - 1; j < ptr[r + 1] - 1; ++j) {                           \
+            I c = idx[j];                                                           \
+            if (!written && c > r + 1) {                                            \
+                if (idx_out) { idx_out[w] = r + 1; val_out[w] = lambda; }           \
+                ++w; written = 1;                                                   \
+            }                                                                       \
+            if (idx_out) {                                                          \
+                idx_out[w] = c;                                                     \
+                val_out[w] = c == r + 1 ? val[j] + lambda : val[j];                 \
+            }                                                                       \
+            if (c == r + 1) written = 1;                                            \
+            ++w;                                                                    \
+        }                                                                           \
+        if (!written) {                                                             \
+            if (idx_out) { idx_out[w] = r + 1; val_out[w] = lambda; }               \
+            ++w;                                                                    \
+        }                                                                           \
+        if (ptr_out) ptr_out[r + 1] = w + 1;                                        \
+    }                                                                               \
+    return w;                                                                       \
+}
+
+DEFINE_REDUCE(double, f64)
+DEFINE_REDUCE(float, f32)
+
 int ora_max_threads(void) {
 #ifdef _OPENMP
     return omp_get_max_threads();

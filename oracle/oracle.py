@@ -251,3 +251,90 @@ def searchsortedfirst(v, x):
     fn.restype = ctypes.c_int64
     a = _i(v)
     return int(fn(_p(a), a.size, int(x)))
+
+
+# ---------------------------------------------------------------- reductions
+def _red(name, T, argtypes, *args):
+    fn = getattr(lib(), name + "_" + _sfx(T))
+    fn.argtypes = argtypes + [ctypes.POINTER(ctypes.c_double)]
+    fn.restype = ctypes.c_double
+    a = ctypes.c_double()
+    s = fn(*args, ctypes.byref(a))
+    return float(s), float(a.value)
+
+
+def trace_ptr(n, ptr, idx, val):
+    """tr for CSR (rows) or CSC (columns) (matrix_csr.jl:118-142).  Returns
+    (sum, sum of |term|)."""
+    p, i, v = _i(ptr), _i(idx), np.ascontiguousarray(val)
+    return _red("ora_trace_ptr", v.dtype, [ctypes.c_int64, _P, _P, _P], n, _p(p), _p(i), _p(v))
+
+
+def trace_coo(rowind, colind, val):
+    """tr(::COO) (matrix_coo.jl:131-150)."""
+    r, c, v = _i(rowind), _i(colind), np.ascontiguousarray(val)
+    return _red("ora_trace_coo", v.dtype, [ctypes.c_int64, _P, _P, _P], v.size, _p(r), _p(c),
+                _p(v))
+
+
+def dot3_ptr(nmaj, ptr, idx, val, x, y, major_left):
+    """dot(x, A, y) in ptr form (matrix_csr_kernels.jl:47-131).  Returns (sum, Σ|term|)."""
+    p, i, v = _i(ptr), _i(idx), np.ascontiguousarray(val)
+    xs = np.ascontiguousarray(x, v.dtype)
+    ys = np.ascontiguousarray(y, v.dtype)
+    return _red("ora_dot3_ptr", v.dtype, [ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.c_int],
+                nmaj, _p(p), _p(i), _p(v), _p(xs), _p(ys), int(major_left))
+
+
+def dot3_coo(rowind, colind, val, x, y, trans):
+    """dot(x, op(A::COO), y) (matrix_coo_kernels.jl:45-131)."""
+    r, c, v = _i(rowind), _i(colind), np.ascontiguousarray(val)
+    xs = np.ascontiguousarray(x, v.dtype)
+    ys = np.ascontiguousarray(y, v.dtype)
+    return _red("ora_dot3_coo", v.dtype, [ctypes.c_int64, _P, _P, _P, _P, _P, ctypes.c_int],
+                v.size, _p(r), _p(c), _p(v), _p(xs), _p(ys), int(trans))
+
+
+def symmetric_ptr(n, ptr, idx, val) -> bool:
+    """issymmetric for a square matrix in ptr form (matrix_csr_kernels.jl:348-407)."""
+    p, i, v = _i(ptr), _i(idx), np.ascontiguousarray(val)
+    fn = getattr(lib(), "ora_symmetric_ptr_" + _sfx(v.dtype))
+    fn.argtypes = [ctypes.c_int64, _P, _P, _P]
+    fn.restype = ctypes.c_int
+    return bool(fn(n, _p(p), _p(i), _p(v)))
+
+
+def add_diag(n, ptr, idx, val, lam):
+    """CSR/CSC + λI, λ != 0 (matrix_csr.jl:357-406).  Returns (ptr, idx, val)."""
+    p, i, v = _i(ptr), _i(idx), np.ascontiguousarray(val)
+    T = v.dtype
+    fn = getattr(lib(), "ora_add_diag_" + _sfx(T))
+    fn.argtypes = [ctypes.c_int64, _P, _P, _P, _ctv(T), _P, _P, _P]
+    fn.restype = ctypes.c_int64
+    cnt = fn(n, _p(p), _p(i), _p(v), lam, _p(None), _p(None), _p(None))
+    po = np.empty(n + 1, _I); io = np.empty(cnt, _I); vo = np.empty(cnt, T)
+    fn(n, _p(p), _p(i), _p(v), lam, _p(po), _p(io), _p(vo))
+    return po, io, vo
+
+
+def diag_coo(n, dvals):
+    """The n entries (i, i, d_i) (kernel_fill_diagonal_coo!, matrix_coo_kernels.jl:312-325)."""
+    k = np.arange(1, n + 1, dtype=_I)
+    return k, k.copy(), np.ascontiguousarray(dvals)
+
+
+def coo_add_diag(n, A, lam):
+    """COO + λI, λ != 0 (matrix_coo.jl:386-457): COO + the diagonal entries."""
+    T = A[2].dtype
+    return coo_add(n, A, diag_coo(n, np.full(n, lam, dtype=T)))
+
+
+def kron_diag_coo(mB, nB, d, B):
+    """kron(Diagonal(d), B::COO) (kernel_kron_diag_coo!, matrix_coo_kernels.jl:185-221)."""
+    return kron_coo(mB, nB, diag_coo(len(d), d), B)
+
+
+def kron_coo_diag(A, d):
+    """kron(A::COO, Diagonal(d)) (kernel_kron_coo_diag!, matrix_coo_kernels.jl:223-258)."""
+    p = len(d)
+    return kron_coo(p, p, A, diag_coo(p, d))

@@ -150,6 +150,37 @@ def main():
          **{f"C_{k}": v for k, v in zip(("rowptr", "colval", "nzval"), csr_arrays(K))},
          Kcoo_0=K.row.astype(np.int64) + 1, Kcoo_1=K.col.astype(np.int64) + 1, Kcoo_2=K.data)
 
+    # A + I on a matrix with no diagonal entries, test/shared/matrix_coo.jl:277-281:
+    # sparse([1, 2], [2, 1], [1, 2], 3, 3) + I == [1 1 0; 2 1 0; 0 0 1]
+    N = sp.coo_matrix(([1.0, 2.0], ([0, 1], [1, 0])), shape=(3, 3))
+    E = np.array([[1.0, 1.0, 0.0], [2.0, 1.0, 0.0], [0.0, 0.0, 1.0]])
+    # A + 3I on test/shared/matrix_coo.jl:237-256's shape (20 x 20 @ 0.2), values in (0, 1)
+    rng = np.random.default_rng(15)
+    A4 = sp.random(20, 20, density=0.2, random_state=rng, format="csr")
+    A4p3 = (A4 + 3.0 * sp.identity(20, format="csr")).tocsr()
+    A4p3.sort_indices()
+    # kron with a Diagonal, test/shared/matrix_coo.jl:437-467 (30x25, 20x15 @ 0.1, D = rand(4)):
+    # scipy's kron of COO operands lists entries A-major / B-minor, the reference's order
+    rng = np.random.default_rng(16)
+    A5 = sp.random(30, 25, density=0.1, random_state=rng, format="csr")
+    B5 = sp.random(20, 15, density=0.1, random_state=rng, format="csr")
+    d5 = rng.random(4)
+    Dc = sp.coo_matrix((d5, (np.arange(4), np.arange(4))), shape=(4, 4))
+    rb, cb, vb = coo_colmajor(B5)
+    ra, ca, va = coo_colmajor(A5)
+    Bc = sp.coo_matrix((vb, (rb - 1, cb - 1)), shape=B5.shape)
+    Ac = sp.coo_matrix((va, (ra - 1, ca - 1)), shape=A5.shape)
+    KD = sp.kron(Dc, Bc, format="coo")
+    KA = sp.kron(Ac, Dc, format="coo")
+    save("diag_ops.npz", E_nodiag=E,
+         **{f"N_{i}": a for i, a in enumerate(csr_arrays(N))},
+         **{f"A4_{k}": v for k, v in zip(("rowptr", "colval", "nzval"), csr_arrays(A4))},
+         **{f"A4p3_{k}": v for k, v in zip(("rowptr", "colval", "nzval"), csr_arrays(A4p3))},
+         d5=d5, **{f"B5coo_{i}": a for i, a in enumerate((rb, cb, vb))},
+         **{f"A5coo_{i}": a for i, a in enumerate((ra, ca, va))},
+         KD_0=KD.row.astype(np.int64) + 1, KD_1=KD.col.astype(np.int64) + 1, KD_2=KD.data,
+         KA_0=KA.row.astype(np.int64) + 1, KA_1=KA.col.astype(np.int64) + 1, KA_2=KA.data)
+
 
 if __name__ == "__main__":
     main()

@@ -244,3 +244,140 @@ def test_spgemm_integer_exact_vs_dense():
          @ sp.csr_matrix((np.ones_like(B.data), B.indices, B.indptr), shape=B.shape)).tocsr()
     S.sort_indices()
     assert np.array_equal(rp - 1, S.indptr) and np.array_equal(cv - 1, S.indices)
+
+
+# ------------------------------------------------------------- reductions --
+def _ptr_arrays(A, csc=False):
+    B = sp.csc_matrix(A) if csc else sp.csr_matrix(A)
+    B.sort_indices()
+    return B.indptr.astype(np.int64) + 1, B.indices.astype(np.int64) + 1, B.data.copy()
+
+
+@pytest.mark.parametrize("csc", [False, True])
+def test_trace_and_sum_integer_exact(csc):
+    A = int_csr(90, 90, 0.08, 21)
+    A = A + sp.diags(np.arange(1, 91, dtype=np.float64) % 5 - 2)  # some diagonal entries, some zeros
+    A = sp.csr_matrix(A)
+    p, i, v = _ptr_arrays(A, csc)
+    s, a = O.trace_ptr(90, p, i, v)
+    assert s == A.diagonal().sum()
+    assert a == np.abs(A.diagonal()).sum()
+    # COO with duplicates on and off the diagonal: every stored diagonal entry counts
+    r = np.array([1, 2, 2, 3, 1, 2], np.int64)
+    c = np.array([1, 2, 2, 1, 1, 3], np.int64)
+    vals = np.array([1.0, 2.0, -5.0, 7.0, 4.0, 8.0])
+    s2, a2 = O.trace_coo(r, c, vals)
+    D = sp.coo_matrix((vals, (r - 1, c - 1)), shape=(3, 3)).tocsr()  # sums duplicates
+    assert s2 == D.diagonal().sum() == 2.0 and a2 == 12.0
+    # and in ptr form: rows [1: (1, 1.0), (1, 4.0)], [2: (2, 2.0), (2, -5.0), (3, 8.0)], [3: (1, 7.0)]
+    s3, _ = O.trace_ptr(3, np.array([1, 3, 6, 7]), np.array([1, 1, 2, 2, 3, 1]),
+                        np.array([1.0, 4.0, 2.0, -5.0, 8.0, 7.0]))
+    assert s3 == 2.0
+
+
+@pytest.mark.parametrize("fmt", ["csr", "csc", "coo"])
+@pytest.mark.parametrize("trans", [False, True])
+def test_dot3_integer_exact_vs_dense(fmt, trans):
+    m, n = (100, 80)
+    A = int_csr(m, n, 0.1, 22)
+    rng = np.random.default_rng(23)
+    Ad = A.toarray()
+    opA = Ad.T if trans else Ad
+    x = rng.integers(-3, 4, size=opA.shape[0]).astype(np.float64)
+    y = rng.integers(-3, 4, size=opA.shape[1]).astype(np.float64)
+    exp = float(x @ opA @ y)
+    if fmt == "coo":
+        C = A.tocoo()
+        got, _ = O.dot3_coo(C.row + 1, C.col + 1, C.data, x, y, trans)
+    elif fmt == "csr":
+        p, i, v = _ptr_arrays(A)
+        got, _ = O.dot3_ptr(m, p, i, v, x, y, major_left=not trans)
+    else:
+        p, i, v = _ptr_arrays(A, csc=True)
+        got, _ = O.dot3_ptr(n, p, i, v, x, y, major_left=trans)
+    assert got == exp
+
+
+def test_dot3_float_close_to_dense():
+    rng = np.random.default_rng(24)
+    A = sp.random(200, 150, density=0.05, random_state=rng, format="csr")
+    x, y = rng.standard_normal(200), rng.standard_normal(150)
+    p, i, v = _ptr_arrays(A)
+    got, bound = O.dot3_ptr(200, p, i, v, x, y, True)
+    assert abs(got - x @ (A @ y)) <= 1e-14 * bound
+    # Float32 terms are rounded in Float32: x*a*y of each entry
+    got32, _ = O.dot3_ptr(200, p, i, v.astype(np.float32), x.astype(np.float32),
+                          y.astype(np.float32), True)
+    C = A.tocoo()
+    terms = (x.astype(np.float32)[C.row] * C.data.astype(np.float32)) * y.astype(np.float32)[C.col]
+    assert abs(got32 - terms.astype(np.float64).sum()) <= 1e-12 * np.abs(terms).sum()
+
+
+def test_symmetric_vs_scipy_and_known_cases():
+    rng = np.random.default_rng(25)
+    for seed in range(6):
+        A = sp.random(50, 50, density=0.1, random_state=np.random.default_rng(seed), format="csr")
+        S = sp.csr_matrix(A + A.T)
+        for M, expect in ((A, (abs(A - A.T)).nnz == 0), (S, True)):
+            p, i, v = _ptr_arrays(M)
+            assert O.symmetric_ptr(50, p, i, v) == expect
+            p, i, v = _ptr_arrays(M, csc=True)
+            assert O.symmetric_ptr(50, p, i, v) == expect
+    # the reference test's non-symmetric pair (test/shared/matrix_coo.jl:96-120): A[1,2] = 1, A[2,1] = 2
+    N = sp.csr_matrix(([1.0, 2.0], ([0, 1], [1, 0])), shape=(2, 2))
+    assert not O.symmetric_ptr(2, *_ptr_arrays(N))
+    # a value differs, a transpose entry is missing, a lone diagonal, an empty matrix
+    S = sp.csr_matrix(([1.0, 2.0, 2.0, 5.0], ([0, 0, 2, 1], [0, 2, 0, 1])), shape=(3, 3))
+    assert O.symmetric_ptr(3, *_ptr_arrays(S))
+    S2 = S.copy(); S2.data[S2.data == 2.0] = [2.0, 2.5]
+    assert not O.symmetric_ptr(3, *_ptr_arrays(S2))
+    S3 = sp.csr_matrix(([1.0, 2.0], ([0, 0], [0, 2])), shape=(3, 3))
+    assert not O.symmetric_ptr(3, *_ptr_arrays(S3))
+    assert O.symmetric_ptr(3, np.ones(4, np.int64), np.zeros(0, np.int64), np.zeros(0))
+    # a long symmetric row exercises the search: arrow matrix
+    n = 300
+    rows = np.r_[np.zeros(n - 1, int), np.arange(1, n), np.arange(n)]
+    cols = np.r_[np.arange(1, n), np.zeros(n - 1, int), np.arange(n)]
+    vals = np.r_[np.arange(1, n), np.arange(1, n), np.ones(n)].astype(np.float64)
+    Arw = sp.csr_matrix((vals, (rows, cols)), shape=(n, n))
+    assert O.symmetric_ptr(n, *_ptr_arrays(Arw))
+    Arw2 = Arw.tolil(); Arw2[0, 150] = 7.0
+    assert not O.symmetric_ptr(n, *_ptr_arrays(sp.csr_matrix(Arw2)))
+
+
+def test_add_diag_golden_and_scipy():
+    d = g("diag_ops.npz")
+    p, i, v = O.add_diag(3, d["N_0"], d["N_1"], d["N_2"], 1.0)
+    assert np.array_equal(to_sp(3, 3, p, i, v).toarray(), d["E_nodiag"])
+    assert p.tolist() == [1, 3, 5, 6] and i.tolist() == [1, 2, 1, 2, 3]  # λ before/after
+    p, i, v = O.add_diag(20, d["A4_rowptr"], d["A4_colval"], d["A4_nzval"], 3.0)
+    assert np.array_equal(p, d["A4p3_rowptr"]) and np.array_equal(i, d["A4p3_colval"])
+    assert np.array_equal(v, d["A4p3_nzval"])
+    # CSR + λI keeps a diagonal entry that cancels to zero (no dropzeros, matrix_csr.jl:357-406)
+    Z = sp.csr_matrix(([-2.0, 1.0], ([0, 0], [0, 1])), shape=(2, 2))
+    p, i, v = O.add_diag(2, *_ptr_arrays(Z), 2.0)
+    assert p.tolist() == [1, 3, 4] and i.tolist() == [1, 2, 2] and v.tolist() == [0.0, 1.0, 2.0]
+    # COO + λI sums then drops exact zeros (matrix_coo.jl:386-457), column-major order
+    C = Z.tocoo()
+    r, c, v = O.coo_add_diag(2, (C.row + 1, C.col + 1, C.data), 2.0)
+    assert r.tolist() == [1, 2] and c.tolist() == [2, 2] and v.tolist() == [1.0, 2.0]
+    A4 =to_sp(20, 20, d["A4_rowptr"], d["A4_colval"], d["A4_nzval"]).tocoo()
+    r, c, v = O.coo_add_diag(20, (A4.row + 1, A4.col + 1, A4.data), 3.0)
+    exp = sp.csc_matrix(to_sp(20, 20, d["A4p3_rowptr"], d["A4p3_colval"], d["A4p3_nzval"]))
+    exp.sort_indices()
+    ec = exp.tocoo()
+    o = np.lexsort((ec.row, ec.col))
+    assert np.array_equal(r, ec.row[o] + 1) and np.array_equal(c, ec.col[o] + 1)
+    assert np.array_equal(v, ec.data[o])
+
+
+def test_kron_diagonal_golden():
+    d = g("diag_ops.npz")
+    B = (d["B5coo_0"], d["B5coo_1"], d["B5coo_2"])
+    A = (d["A5coo_0"], d["A5coo_1"], d["A5coo_2"])
+    r, c, v = O.kron_diag_coo(20, 15, d["d5"], B)
+    assert np.array_equal(r, d["KD_0"]) and np.array_equal(c, d["KD_1"])
+    assert np.array_equal(v, d["KD_2"])
+    r, c, v = O.kron_coo_diag(A, d["d5"])
+    assert np.array_equal(r, d["KA_0"]) and np.array_equal(c, d["KA_1"])
+    assert np.array_equal(v, d["KA_2"])
