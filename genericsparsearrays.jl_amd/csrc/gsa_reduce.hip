@@ -51,11 +51,10 @@ __global__ __launch_bounds__(ROW_WG) void k_trace_ptr(int64_t n, int64_t nnz,
                                                       double* __restrict__ part) {
     __shared__ double sh[ROW_WG / 64];
     double acc = 0.0;
-    visit_owned_rows<I>(
-        n, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s; j < e; ++j)
-                if ((int64_t)idx[j] - 1 == r) acc += (double)val[j];
+    visit_owned_entries<I>(
+        n, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {},
+        [&](int64_t r, int64_t j) {
+            if ((int64_t)idx[j] - 1 == r) acc += (double)val[j];
         },
         [&](int64_t r, int64_t s, int64_t e) {
             for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG)
@@ -106,11 +105,9 @@ __global__ __launch_bounds__(ROW_WG) void k_dot3_ptr(int64_t nmaj, int64_t nidx,
         const T t = (xv * val[j]) * yv;
         acc += (double)t;
     };
-    visit_owned_rows<I>(
-        nmaj, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s; j < e; ++j) term(r, j);
-        },
+    visit_owned_entries<I>(
+        nmaj, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {},
+        [&](int64_t r, int64_t j) { term(r, j); },
         [&](int64_t r, int64_t s, int64_t e) {
             for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) term(r, j);
         });
@@ -173,11 +170,9 @@ __global__ __launch_bounds__(ROW_WG) void k_symmetric_ptr(int64_t n, int64_t nnz
         }
         if (!ok) *flag = 0;
     };
-    visit_owned_rows<I>(
-        n, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s; j < e; ++j) check(r, j);
-        },
+    visit_owned_entries<I>(
+        n, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {},
+        [&](int64_t r, int64_t j) { check(r, j); },
         [&](int64_t r, int64_t s, int64_t e) {
             for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) check(r, j);
         });
@@ -186,20 +181,19 @@ __global__ __launch_bounds__(ROW_WG) void k_symmetric_ptr(int64_t n, int64_t nnz
 // ---- A + λI ------------------------------------------------------------------
 // Per major: one more entry when none of its entries sits on the diagonal
 // (kernel_count_diag_entries_csr!, src/matrix_csr/matrix_csr_kernels.jl:445-463).
-// Rows of up to ROW_SHORT entries are scanned by one lane, longer ones by the
-// whole workgroup.
+// Majors of up to ROW_SHORT entries: length + 1 by the major's lane, then
+// length by any diagonal entry (entry-parallel); longer ones by the workgroup.
 template <typename I>
 __global__ __launch_bounds__(ROW_WG) void k_diag_missing(int64_t n, int64_t nnz,
                                                          const I* __restrict__ ptr,
                                                          const I* __restrict__ idx,
                                                          const int64_t* __restrict__ blk_row,
                                                          I* __restrict__ cnt) {
-    visit_owned_rows<I>(
+    visit_owned_entries<I>(
         n, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) {
-            bool found = false;
-            for (int64_t j = s; j < e && !found; ++j) found = (int64_t)idx[j] - 1 == r;
-            cnt[r + 1] = (I)((e - s) + (found ? 0 : 1));
+        [&](int64_t r, int64_t s, int64_t e) { cnt[r + 1] = (I)((e - s) + 1); },
+        [&](int64_t r, int64_t j) {  // a diagonal entry: nothing to insert
+            if ((int64_t)idx[j] - 1 == r) cnt[r + 1] = (I)((int64_t)ptr[r + 1] - (int64_t)ptr[r]);
         },
         [&](int64_t r, int64_t s, int64_t e) {
             int f = 0;
@@ -228,8 +222,10 @@ __device__ __forceinline__ int64_t wg_min(int64_t v, int64_t* sh) {
 // for rows whose indices ascend: the entries in storage order with λ added to
 // every diagonal entry; when the major has no diagonal entry, λ inserted before
 // its first index past the diagonal (or appended).  Entry j of the major moves
-// to ptr_out + (j - start) + (1 if λ was inserted before it).  Writes stay
-// inside the major's new range.
+// to ptr_out + (j - start) + (1 if λ was inserted before it); whether λ is
+// inserted is ptr_out's row length against ptr's.  λ itself is written by the
+// first entry past the diagonal, else by the last entry, else (an empty
+// major) by the major's lane.  Writes stay inside the major's new range.
 template <typename T, typename I>
 __global__ __launch_bounds__(ROW_WG) void k_add_diag(int64_t n, int64_t nnz,
                                                      const I* __restrict__ ptr,
@@ -246,23 +242,22 @@ __global__ __launch_bounds__(ROW_WG) void k_add_diag(int64_t n, int64_t nnz,
             val_out[w] = v;
         }
     };
-    visit_owned_rows<I>(
+    visit_owned_entries<I>(
         n, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) {
+        [&](int64_t r, int64_t s, int64_t e) {  // an empty major gets λ alone
+            if (s == e && ptr_out[r + 1] > ptr_out[r])
+                put((int64_t)ptr_out[r] - 1, (int64_t)ptr_out[r + 1] - 1, (I)(r + 1), lambda);
+        },
+        [&](int64_t r, int64_t j) {
+            const int64_t s = (int64_t)ptr[r] - 1, e = (int64_t)ptr[r + 1] - 1;
             const int64_t base = (int64_t)ptr_out[r] - 1, wend = (int64_t)ptr_out[r + 1] - 1;
-            bool found = false;
-            for (int64_t j = s; j < e && !found; ++j) found = (int64_t)idx[j] - 1 == r;
-            int64_t w = base;
-            bool inserted = found;
-            for (int64_t j = s; j < e; ++j) {
-                const int64_t c = (int64_t)idx[j] - 1;
-                if (!inserted && c > r) {
-                    put(w++, wend, (I)(r + 1), lambda);
-                    inserted = true;
-                }
-                put(w++, wend, idx[j], c == r ? val[j] + lambda : val[j]);
-            }
-            if (!inserted) put(w, wend, (I)(r + 1), lambda);
+            const bool missing = wend - base > e - s;
+            const int64_t c = (int64_t)idx[j] - 1;
+            const bool past = missing && c > r;
+            const int64_t w = base + (j - s) + (past ? 1 : 0);
+            put(w, wend, idx[j], c == r ? val[j] + lambda : val[j]);
+            if (past && (j == s || (int64_t)idx[j - 1] - 1 < r)) put(w - 1, wend, (I)(r + 1), lambda);
+            if (missing && !past && j == e - 1) put(w + 1, wend, (I)(r + 1), lambda);
         },
         [&](int64_t r, int64_t s, int64_t e) {
             const int64_t base = (int64_t)ptr_out[r] - 1, wend = (int64_t)ptr_out[r + 1] - 1;

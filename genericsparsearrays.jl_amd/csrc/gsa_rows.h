@@ -95,4 +95,70 @@ __device__ __forceinline__ void visit_owned_rows(int64_t m, int64_t nnz, const I
     }
 }
 
+// Entries staged per chunk by visit_owned_entries (their rows, in LDS).
+constexpr int ENT_CAP = 2048;
+
+// Visit the rows and entries owned by this workgroup under a row-block plan,
+// entry-parallel: row_fn(r, s, e) once per owned row of length <= ROW_SHORT
+// (one lane each); then entry_fn(r, j) for every entry j of those rows, the
+// lanes taking consecutive entries (coalesced reads of the index and value
+// arrays; each chunk of ENT_CAP entries gets its rows written to LDS first);
+// then long_fn(r, s, e) for each longer row, called by ALL lanes together.
+// Writes made by row_fn are visible to entry_fn, and both to long_fn.
+template <typename I, typename RowFn, typename EntryFn, typename LongFn>
+__device__ __forceinline__ void visit_owned_entries(int64_t m, int64_t nnz,
+                                                    const I* __restrict__ ptr,
+                                                    const int64_t* __restrict__ blk_row,
+                                                    RowFn row_fn, EntryFn entry_fn,
+                                                    LongFn long_fn) {
+    __shared__ int64_t rid[ENT_CAP];
+    __shared__ int64_t longs[ROW_MAX_LONG];
+    __shared__ int nlong;
+    const int64_t b = blockIdx.x;
+    const int64_t r0 = dclamp(blk_row[b], 0, m);
+    const int64_t r1 = dclamp(blk_row[b + 1], r0, m);
+    if (r0 >= r1) return;
+    const int64_t P0 = dclamp((int64_t)ptr[r0] - 1, 0, nnz);
+    const int64_t P1 = dclamp((int64_t)ptr[r1] - 1, P0, nnz);
+    if (threadIdx.x == 0) nlong = 0;
+    __syncthreads();
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += ROW_WG) {
+        const int64_t s = dclamp((int64_t)ptr[r] - 1, 0, nnz);
+        const int64_t e = dclamp((int64_t)ptr[r + 1] - 1, s, nnz);
+        if (e - s <= ROW_SHORT) {
+            row_fn(r, s, e);
+        } else {
+            const int slot = atomicAdd(&nlong, 1);
+            if (slot < ROW_MAX_LONG) longs[slot] = r;
+        }
+    }
+    __syncthreads();
+    for (int64_t c0 = P0; c0 < P1; c0 += ENT_CAP) {
+        const int64_t c1 = c0 + ENT_CAP < P1 ? c0 + ENT_CAP : P1;
+        for (int64_t k = threadIdx.x; k < c1 - c0; k += ROW_WG) rid[k] = -1;
+        __syncthreads();
+        for (int64_t r = r0 + threadIdx.x; r < r1; r += ROW_WG) {
+            const int64_t s = dclamp((int64_t)ptr[r] - 1, 0, nnz);
+            const int64_t e = dclamp((int64_t)ptr[r + 1] - 1, s, nnz);
+            if (e - s > ROW_SHORT) continue;
+            const int64_t lo = s > c0 ? s : c0, hi = e < c1 ? e : c1;
+            for (int64_t j = lo; j < hi; ++j) rid[j - c0] = r;
+        }
+        __syncthreads();
+        for (int64_t k = threadIdx.x; k < c1 - c0; k += ROW_WG) {
+            const int64_t r = rid[k];
+            if (r >= 0) entry_fn(r, c0 + k);
+        }
+        __syncthreads();
+    }
+    const int nl = nlong < ROW_MAX_LONG ? nlong : ROW_MAX_LONG;
+    for (int q = 0; q < nl; ++q) {
+        const int64_t r = longs[q];
+        const int64_t s = dclamp((int64_t)ptr[r] - 1, 0, nnz);
+        const int64_t e = dclamp((int64_t)ptr[r + 1] - 1, s, nnz);
+        long_fn(r, s, e);
+        __syncthreads();
+    }
+}
+
 }  // namespace gsa
