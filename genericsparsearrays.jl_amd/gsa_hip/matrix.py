@@ -562,26 +562,48 @@ I = UniformScaling(True)
 
 
 class Diagonal:
-    """LinearAlgebra.Diagonal with its values in a device vector `diag` (n x n)."""
+    """LinearAlgebra.Diagonal with its values in a device vector `diag` (n x n), or,
+    from `Diagonal.fill`, one value for the whole diagonal (FillArrays'
+    Diagonal(Fill(v, n)), ext/GenericSparseArraysFillArraysExt/kron_coo.jl:5-48)."""
 
     def __init__(self, diag: torch.Tensor):
         if not isinstance(diag, torch.Tensor) or diag.dim() != 1:
             raise ArgumentError("Diagonal needs a 1-D tensor of values")
         self.diag = diag
+        self.fill_value = None
+        self._n = int(diag.shape[0])
+        self._dtype = diag.dtype
+        self._device = diag.device
+
+    @classmethod
+    def fill(cls, value, n: int, dtype=torch.float64, device="cuda"):
+        """Diagonal(Fill(value, n)): n copies of one value, stored as that value."""
+        D = cls.__new__(cls)
+        D.diag = None
+        D.fill_value = float(value)
+        D._n = int(n)
+        D._dtype = dtype
+        D._device = torch.device(device)
+        return D
 
     @property
     def shape(self):
-        n = int(self.diag.shape[0])
-        return (n, n)
+        return (self._n, self._n)
 
     def size(self, dim=None):
         return self.shape if dim is None else self.shape[dim - 1]
 
     @property
     def dtype(self):
-        return self.diag.dtype
+        return self._dtype
+
+    @property
+    def device(self):
+        return self._device
 
     def collect(self) -> np.ndarray:
+        if self.diag is None:
+            return np.diag(np.full(self._n, self.fill_value))
         return np.diag(self.diag.detach().cpu().numpy())
 
 

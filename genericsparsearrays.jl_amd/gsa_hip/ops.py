@@ -591,8 +591,11 @@ def add_uniform_scaling(A, J: UniformScaling):
 def _kron_diagonal(A, B):
     """kron(D, B) and kron(A, D): kron of COO operands with D's n entries (i, i, d_i),
     the entry order of kernel_kron_diag_coo! / kernel_kron_coo_diag!
-    (src/matrix_coo/matrix_coo_kernels.jl:185-258); a CSR or CSC operand gives
-    a result of its format by conversion from that COO (src/matrix_csr/matrix_csr.jl:460-470)."""
+    (src/matrix_coo/matrix_coo_kernels.jl:185-258; for Diagonal(Fill(v, n)),
+    kernel_kron_scaled_identity_coo! / kernel_kron_coo_scaled_identity!,
+    ext/GenericSparseArraysFillArraysExt/kernels.jl:4-61, which list the same
+    entries); a CSR or CSC operand gives a result of its format by conversion
+    from that COO (src/matrix_csr/matrix_csr.jl:460-470, ext/.../kron_csr.jl:5-46)."""
     if isinstance(A, Diagonal) and isinstance(B, Diagonal):
         raise ArgumentError("kron of two Diagonals is not a sparse operation here")
     D, S = (A, B) if isinstance(A, Diagonal) else (B, A)
@@ -601,9 +604,14 @@ def _kron_diagonal(A, B):
         raise ArgumentError(f"kron: unsupported operand {type(S).__name__}")
     Scoo = convert.to_coo(S)
     dev = require_device(Scoo.rowind, Scoo.nzval, D.diag)
-    if D.diag.dtype not in (torch.float32, torch.float64):
+    if D.dtype not in (torch.float32, torch.float64):
         raise ArgumentError("kron: Diagonal values must be Float32 or Float64")
-    Dcoo = _diag_coo(D.shape[0], dev, D.diag.dtype, Scoo.rowind.dtype, dvals=D.diag)
+    if D.diag is None:  # Diagonal(Fill(v, n)): kernel_kron_scaled_identity_coo!, same entries
+        if D.device != dev:
+            raise ArgumentError("All arrays must have the same backend")
+        Dcoo = _diag_coo(D.shape[0], dev, D.dtype, Scoo.rowind.dtype, lam=D.fill_value)
+    else:
+        Dcoo = _diag_coo(D.shape[0], dev, D.dtype, Scoo.rowind.dtype, dvals=D.diag)
     C = _kron_coo(Dcoo, Scoo) if D is A else _kron_coo(Scoo, Dcoo)
     if isinstance(Sp, GenericSparseMatrixCSR):
         return convert.coo_to_csr(C)

@@ -326,8 +326,13 @@ def test_kron_diagonal_formats_and_wrappers(dev, dtype):
                     assert isinstance(got, G.GenericSparseMatrixCSC)
                     _same(_ptr_tuple(got), O.coo_to_csc(*shape, *exp))
     # FillArrays' Diagonal(Fill(2, 4)) of test/shared/matrix_coo.jl:443: a constant diagonal
-    D2 = G.Diagonal(torch.full((4,), 2.0, dtype=dtype, device=dev))
+    D2 = G.Diagonal.fill(2.0, 4, dtype=dtype, device=dev)
     got = G.kron(D2, Bo)
     exp = O.kron_diag_coo(300, 200, np.full(4, 2.0, NPT[dtype]),
                           (host(Bo.rowind), host(Bo.colind), host(Bo.nzval)))
     _same((host(got.rowind), host(got.colind), host(got.nzval)), exp)
+    got = G.kron(G.transpose(B), D2)  # CSR operand: CSR result
+    Mo = G.convert.to_coo(G.transpose(B))
+    exp = O.kron_coo_diag((host(Mo.rowind), host(Mo.colind), host(Mo.nzval)),
+                          np.full(4, 2.0, NPT[dtype]))
+    _same(_ptr_tuple(got), O.coo_to_csr(200 * 4, 300 * 4, *exp))
