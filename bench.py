@@ -180,6 +180,12 @@ def bench_structural(H, dev):
     out["add_L_Lp"] = {"ms": round(t * 1e3, 3), "nnzC": C.nnz(),
                        "GBs": round(((L.nnz + Lp.nnz) * 16 + 2 * (L.m + 1) * 8 + C.nnz() * 16
                                      + (L.m + 1) * 8) / t / 1e9, 1)}
+    cL, cLp = G.GenericSparseMatrixCOO(dL), G.GenericSparseMatrixCOO(dLp)  # row-major COO
+    Cc = cL + cLp
+    t = _sync_time(lambda: cL + cLp)
+    out["add_coo_L_Lp"] = {"ms": round(t * 1e3, 3), "nnzC": Cc.nnz(),
+                           "GBs": round(((L.nnz + Lp.nnz) * 24 + Cc.nnz() * 24) / t / 1e9, 1)}
+    del cL, cLp, Cc
     for name, X in (("spgemm_LL", dL), ("spgemm_RR", dR)):
         Cx = X * X
         t = _sync_time(lambda: X * X)
@@ -224,6 +230,10 @@ def cpu_structural(H):
     tup = lambda M: (M.rowptr, M.colval, M.nzval)
     C, t = tm(lambda: O.csr_add(L.m, tup(L), tup(Lp)))
     out["add_L_Lp"] = (t, (L.nnz + Lp.nnz) * 16 + 2 * (L.m + 1) * 8 + C[2].size * 16 + (L.m + 1) * 8)
+    cL = O.csr_to_coo(L.m, *tup(L))
+    cLp = O.csr_to_coo(Lp.m, *tup(Lp))
+    C, t = tm(lambda: O.coo_add(L.m, cL, cLp))
+    out["add_coo_L_Lp"] = (t, (L.nnz + Lp.nnz) * 24 + C[2].size * 24)
     for name, X in (("spgemm_LL", L), ("spgemm_RR", R)):
         C, t = tm(lambda: O.spgemm_csr(X.m, X.n, tup(X), tup(X)))
         out[name] = (t, (2 * X.nnz + C[2].size) * 16)
@@ -581,9 +591,15 @@ def main():
         for tag, M, byts in (
                 ("spmv_csc_f64_i64", Ac, spmv_bytes(H.m, H.n, H.nnz, 8, 8) - (H.m + 1) * 8 + (H.n + 1) * 8),
                 ("spmv_coo_rowmajor_f64_i64", G.GenericSparseMatrixCOO(A), H.nnz * 24 + (H.n + H.m) * 8),
-                ("spmv_coo_colmajor_f64_i64", G.GenericSparseMatrixCOO(Ac), H.nnz * 24 + (H.n + H.m) * 8)):
+                ("spmv_coo_colmajor_f64_i64", G.GenericSparseMatrixCOO(Ac), H.nnz * 24 + (H.n + H.m) * 8),
+                # transposed products: the reference's atomic scatters (CSR-T, COO-T)
+                ("spmv_csr_T_f64_i64", G.transpose(A), spmv_bytes(H.m, H.n, H.nnz, 8, 8)),
+                ("spmv_coo_T_f64_i64", G.transpose(G.GenericSparseMatrixCOO(A)),
+                 H.nnz * 24 + (H.n + H.m) * 8)):
             if hasattr(M, "plan"):
                 M.plan()
+            elif hasattr(M, "parent") and hasattr(M.parent, "plan"):
+                M.parent.plan()
             ms4 = time_steps(lambda i: G.mul_(yd, M, xd, 1.0, 0.0), 20, 3, stream)
             extra[tag + "_us"] = round(ms4 * 1e3, 1)
             extra[tag + "_GBs"] = round(byts / (ms4 * 1e-3) / 1e9, 1)
@@ -601,8 +617,15 @@ def main():
             t0 = time.perf_counter()
             O.spmm_coo(False, H.m, ri, ci, nzo, x_h)  # COO A x (matrix_coo_kernels.jl:1-20)
             to = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            O.spmm_csr_T(H.m, H.n, H.rowptr, H.colval, H.nzval, x_h)  # CSR-T (matrix_csr_kernels.jl:24-45)
+            tt = time.perf_counter() - t0
+            t0 = time.perf_counter()
+            O.spmm_coo(True, H.n, ri, ci, nzo, x_h)  # COO-T (matrix_coo_kernels.jl:22-43)
+            tot = time.perf_counter() - t0
             extra["spmv_csc_coo_cpu_oracle"] = {
-                "csc_ms": round(tc * 1e3, 1), "coo_rowmajor_ms": round(to * 1e3, 1), "threads": 1}
+                "csc_ms": round(tc * 1e3, 1), "coo_rowmajor_ms": round(to * 1e3, 1),
+                "csr_T_ms": round(tt * 1e3, 1), "coo_T_ms": round(tot * 1e3, 1), "threads": 1}
             print(f"[bench] CSC / COO SpMV CPU oracle {tc * 1e3:.0f} / {to * 1e3:.0f} ms, 1 thread",
                   file=sys.stderr, flush=True)
             del cp, rv, nzc, ri, ci, nzo, Ao

@@ -193,6 +193,7 @@ struct gsa_plan_s {
     void* batch = nullptr;           // gsa::BatchPlan*
     gsa::RowView* view = nullptr;    // row view (CSC: beside the column blocks)
     gsa::BinPlan* bins = nullptr;    // row bins of a CSC or COO matrix (scatter-shaped SpMV)
+    gsa::BinPlan* bins_t = nullptr;  // bins of the transposed product of a CSR or COO matrix
 };
 
 struct gsa_ctx_s {

@@ -1198,6 +1198,27 @@ int gsa_plan_create_csc(void* stream, int itype, int64_t m, int64_t n, int64_t n
     return GSA_OK;
 }
 
+int gsa_plan_add_transpose(void* stream, gsa_plan plan, int format, int itype, int64_t m,
+                           int64_t n, int64_t nnz, const void* a, const void* b) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(plan && (format == GSA_FMT_CSR || format == GSA_FMT_COO) && m >= 0 && n >= 0 &&
+                  nnz >= 0 && a && (nnz == 0 || b),
+              GSA_ERR_INVALID, "gsa_plan_add_transpose: bad arguments");
+    if (bin_plan_matches(plan->bins_t, itype, n, m, nnz, a, b)) return GSA_OK;
+    if (plan->bins_t) {
+        GSA_HIP(hipDeviceSynchronize());  // queued products may still read the old bins
+        release_bin_plan(plan->bins_t);
+        plan->bins_t = nullptr;
+    }
+    hipStream_t s = as_stream(stream);
+    // Aᵀ x scatters into the n columns: bins over the column indices; the sources
+    // are the rows (CSR: from the row pointer; COO: rowind, given at each call)
+    GSA_TRY(build_bin_plan(s, itype, n, m, nnz, b, format == GSA_FMT_CSR ? a : nullptr,
+                           &plan->bins_t));
+    if (plan->bins_t) bin_plan_set_sources(plan->bins_t, a, b);
+    return GSA_OK;
+}
+
 int gsa_plan_describe(gsa_plan plan, int* flags) {
     GSA_CHECK(plan && flags, GSA_ERR_INVALID, "gsa_plan_describe: null argument");
     int f = 0;
@@ -1205,6 +1226,7 @@ int gsa_plan_describe(gsa_plan plan, int* flags) {
     if (plan->kind == 0 && plan->csr.order) f |= GSA_PLAN_XCD_ORDER;
     if (plan->view) f |= GSA_PLAN_ROW_VIEW;
     if (plan->bins) f |= GSA_PLAN_ROW_BINS;
+    if (plan->bins_t) f |= GSA_PLAN_TRANSPOSE_BINS;
     if (plan->kind == 1) f |= GSA_PLAN_BATCH;
     *flags = f;
     return GSA_OK;
@@ -1213,10 +1235,12 @@ int gsa_plan_describe(gsa_plan plan, int* flags) {
 int gsa_plan_destroy(gsa_plan plan) {
     if (!plan) return GSA_OK;
     // the plan may still be in use by queued work on any stream
-    if (plan->kind == 0 || plan->view || plan->bins) GSA_HIP(hipDeviceSynchronize());
+    if (plan->kind == 0 || plan->view || plan->bins || plan->bins_t)
+        GSA_HIP(hipDeviceSynchronize());
     if (plan->kind == 0) GSA_TRY(plan_release(plan->csr));
     if (plan->view) release_row_view(plan->view);
     if (plan->bins) release_bin_plan(plan->bins);
+    if (plan->bins_t) release_bin_plan(plan->bins_t);
     if (plan->kind == 1) batch_plan_release(plan);
     delete plan;
     return GSA_OK;
@@ -1236,6 +1260,10 @@ int gsa_spmm_csr(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
             return gather_csr<T, I>(s, m, n, k, (const I*)rowptr, (const I*)colval,
                                     (const T*)nzval, nnz, (const T*)B, ldb, opB != GSA_OP_N,
                                     (T*)C, ldc, alpha, beta, plan, itype);
+        if (k == 1 && n > 0 && plan &&
+            bin_plan_matches(plan->bins_t, itype, n, m, nnz, rowptr, colval))
+            return bin_spmv(s, *plan->bins_t, vtype, nullptr, nzval, B,
+                            opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
         return scatter_csr<T, I>(s, m, n, k, (const I*)rowptr, (const I*)colval, (const T*)nzval,
                                  nnz, (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
                                  plan, itype);
@@ -1310,6 +1338,10 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
         if (v)  // sorted-key form: rows gathered through the plan's row view
             return view_spmv<T, I>(s, *v, n, (const I*)colind, (const T*)nzval, (const T*)B, ldb,
                                    opB != GSA_OP_N, (T*)C, ldc, alpha, beta, itype);
+        if (trans && k == 1 && plan &&
+            bin_plan_matches(plan->bins_t, itype, n, m, nnz, rowind, colind))
+            return bin_spmv(s, *plan->bins_t, vtype, rowind, nzval, B,
+                            opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
         GSA_TRY(apply_beta<T>(s, rows_out, k, (T*)C, ldc, beta));
         if (nnz == 0 || cols_in == 0) return (int)GSA_OK;
         hipLaunchKernelGGL((k_coo_scatter<T, I>), dim3(grid_for(nnz * k, 256)), dim3(256), 0, s,

@@ -196,6 +196,7 @@ class _PlanCache:
     def __init__(self):
         self.handle = None
         self.key = None
+        self.transposed = False  # the handle holds the transposed product's bins
 
     def get(self, ptr_t: torch.Tensor, m: int, nnz: int, col_t: torch.Tensor | None = None,
             n: int = 0):
@@ -242,6 +243,15 @@ class _PlanCache:
                 L.lib().gsa_plan_destroy(h)
             raise
         self.handle, self.key = h, key
+        self.transposed = False
+        return h
+
+    def add_transpose(self, h, fmt, itype, m, n, nnz, a, b):
+        """Give plan `h` the bins of transpose(A) * x (built once per plan)."""
+        if not self.transposed:
+            L.call("gsa_plan_add_transpose", stream_of(a.device), h, fmt, itype, m, n, nnz,
+                   ptr(a), ptr(b))
+            self.transposed = True
         return h
 
     def release(self):
@@ -301,6 +311,11 @@ class GenericSparseMatrixCSR(AbstractGenericSparseMatrix):
     def cached_plan(self):
         """plan() if it is already built for the current arrays, else None."""
         return self._plan.peek(("csr",), (self.rowptr, self.colval), self.m, self.nnz(), self.n)
+
+    def plan_transposed(self):
+        """plan() with the row bins of transpose(A) * x added (no atomics)."""
+        return self._plan.add_transpose(self.plan(), L.GSA_FMT_CSR, itype_of(self.rowptr), self.m,
+                                        self.n, self.nnz(), self.rowptr, self.colval)
 
     def copy(self):
         return GenericSparseMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval)
@@ -410,6 +425,12 @@ class GenericSparseMatrixCOO(AbstractGenericSparseMatrix):
         if getattr(self, "_plan", None) is None:
             self._plan = _PlanCache()
         return self._plan.get_coo(self.rowind, self.colind, self.m, self.n, self.nnz())
+
+    def plan_transposed(self):
+        """plan() with the row bins of transpose(A) * x added (no atomics)."""
+        h = self.plan()  # creates the cache on first use
+        return self._plan.add_transpose(h, L.GSA_FMT_COO, itype_of(self.rowind), self.m, self.n,
+                                        self.nnz(), self.rowind, self.colind)
 
     def getrowind(self):
         return self.rowind

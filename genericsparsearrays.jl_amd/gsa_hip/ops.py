@@ -95,9 +95,12 @@ def mul_(C: torch.Tensor, A, B: torch.Tensor, alpha=1.0, beta=0.0):
     vt = vtype_of(Cw)
     if isinstance(Ap, GenericSparseMatrixCSR):
         it = itype_of(Ap.rowptr)
+        plan = None
+        if Ap.nnz() > 0:  # transpose(A) * x: the plan's bins replace the atomic scatter
+            plan = Ap.plan_transposed() if (opA != L.GSA_OP_N and k == 1) else Ap.plan()
         L.call("gsa_spmm_csr", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowptr), ptr(Ap.colval),
                ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
-               Ap.plan() if Ap.nnz() > 0 else None)
+               plan)
     elif isinstance(Ap, GenericSparseMatrixCSC):
         it = itype_of(Ap.colptr)
         L.call("gsa_spmm_csc", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.colptr), ptr(Ap.rowval),
@@ -105,8 +108,11 @@ def mul_(C: torch.Tensor, A, B: torch.Tensor, alpha=1.0, beta=0.0):
                Ap.plan() if Ap.nnz() > 0 else None)
     else:
         it = itype_of(Ap.rowind)
-        # the row view serves SpMV (k = 1, no transpose); other products scatter
-        plan = Ap.plan() if (Ap.nnz() > 0 and k == 1 and opA == L.GSA_OP_N) else None
+        # SpMV (k = 1): the row view or bins for A * x, the transposed bins for
+        # transpose(A) * x; other products scatter
+        plan = None
+        if Ap.nnz() > 0 and k == 1:
+            plan = Ap.plan() if opA == L.GSA_OP_N else Ap.plan_transposed()
         L.call("gsa_spmm_coo", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowind), ptr(Ap.colind),
                ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
                plan)

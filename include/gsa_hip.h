@@ -88,7 +88,21 @@ int gsa_plan_destroy(gsa_plan plan);
 #define GSA_PLAN_ROW_VIEW  4
 #define GSA_PLAN_ROW_BINS  8
 #define GSA_PLAN_BATCH     16
+#define GSA_PLAN_TRANSPOSE_BINS 32
 int gsa_plan_describe(gsa_plan plan, int* flags);
+/* Add to a plan the row bins of the transposed product: transpose(A) * x
+ * (k = 1) then runs the two streaming passes of the CSC SpMV (no atomics, a
+ * fixed summation order) instead of the reference's atomic scatter
+ * (kernel_spmatmul_csr_T! src/matrix_csr/matrix_csr_kernels.jl:24-45,
+ * kernel_spmatmul_coo_T! src/matrix_coo/matrix_coo_kernels.jl:22-43).
+ * format GSA_FMT_CSR: a plan from gsa_plan_create, a = rowptr, b = colval;
+ * GSA_FMT_COO: a plan from gsa_plan_create_coo, a = rowind, b = colind.  A
+ * (m x n) with n above 4,194,304 columns gets no bins (the scatter stays).
+ * About 20 bytes per stored entry.  No counterpart in the reference. */
+#define GSA_FMT_CSR 0
+#define GSA_FMT_COO 2
+int gsa_plan_add_transpose(void* stream, gsa_plan plan, int format, int itype, int64_t m,
+                           int64_t n, int64_t nnz, const void* a, const void* b);
 /* Give a plan the column array it will be used with (`colval`, n columns):
  * the SpMV windows lying inside rows longer than 2304 entries (whose
  * ascending columns span a narrow band of x) are then launched on the XCD
