@@ -536,10 +536,10 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             for (int j = s; j < e; j += 8) {
                 T av[8], bv[8];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 8; ++u) {  // lanes past the row's end issue no load
                     const int jj = j + u < e ? j + u : e - 1;
                     av[u] = s_val[jj];
-                    bv[u] = BT[(int64_t)s_col[jj] * ldbt + kk];
+                    bv[u] = j + u < e ? BT[(int64_t)s_col[jj] * ldbt + kk] : T(0);
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
