@@ -169,6 +169,22 @@ def bench_structural(H, dev):
         t = _sync_time(fn)
         out[name] = {"ms": round(t * 1e3, 3), "GBs": round(byts / t / 1e9, 1)}
     del Ac, Ao, A
+    # host ingest: GenericSparseMatrixCSR(::SparseMatrixCSC) of the same matrix held
+    # as a scipy CSC (the arrays copied as they are, transposed on the device),
+    # against the host transposition the reference performs (scipy's tocsr)
+    S = H.to_scipy().tocsc()
+    G.convert.csr_from_scipy(S, device=dev)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    G.convert.csr_from_scipy(S, device=dev)
+    torch.cuda.synchronize()
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    S.tocsr()
+    t_host = time.perf_counter() - t0
+    out["ingest_csr_from_scipy_csc"] = {"ms": round(t_gpu * 1e3, 1),
+                                        "host_transpose_ms": round(t_host * 1e3, 1)}
+    del S
     L = synth.laplacian2d(1000)
     Lp = synth.drop_entries(L, 0.1, seed=3)
     R = synth.fixed_row_csr(200_000, 200_000, 8, seed=3)

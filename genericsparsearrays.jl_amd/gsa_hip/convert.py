@@ -1,8 +1,10 @@
 """Format conversions (src/conversions/conversions.jl) over libgsa_hip.so.
 
-Device-to-device conversions are HIP kernels; host ingest from a
-scipy.sparse matrix (the SparseMatrixCSC interop of conversions.jl:7-79) builds
-the one-based arrays on the host once and copies them to the device.
+Device-to-device conversions are HIP kernels.  Host ingest from a
+scipy.sparse matrix (the SparseMatrixCSC interop of conversions.jl:7-79)
+copies a CSC or CSR matrix's arrays to the device as they are (one-based) and
+transposes there when the other format is asked for; other scipy formats are
+made canonical on the host first.
 """
 from __future__ import annotations
 
@@ -35,9 +37,27 @@ def _conj(v: torch.Tensor, conj: bool) -> torch.Tensor:
     return v.conj().resolve_conj() if conj and v.is_complex() else v
 
 
+def _upload(A, dev, index_dtype, dtype):
+    """The compressed arrays of a scipy CSR or CSC matrix as one-based device vectors."""
+    vals = A.data if dtype is None else A.data.astype(dtype)
+    return (_t(A.indptr.astype(np.int64) + 1, index_dtype, dev),
+            _t(A.indices.astype(np.int64) + 1, index_dtype, dev),
+            torch.as_tensor(np.ascontiguousarray(vals)).to(dev))
+
+
 # ------------------------------------------------------------------ to CSR --
 def csr_from_scipy(A, device=None, index_dtype=torch.int64, dtype=None):
-    """GenericSparseMatrixCSR(::SparseMatrixCSC) (conversions.jl:25-30): canonical CSR."""
+    """GenericSparseMatrixCSR(::SparseMatrixCSC) (conversions.jl:25-30).
+
+    The reference transposes the CSC arrays twice on the host.  A scipy CSC
+    input is copied to the device as it is and transposed there (the stable
+    CSC -> CSR sort: rows in column order, as the reference's transposition
+    leaves them); other inputs are made canonical CSR on the host."""
+    dev = _dev(device)
+    if A.format == "csc":
+        m, n = A.shape
+        Ac = GenericSparseMatrixCSC(m, n, *_upload(A, dev, index_dtype, dtype), copy=False)
+        return csc_to_csr(Ac)
     B = A.tocsr(copy=True)
     B.sum_duplicates()
     B.sort_indices()
@@ -108,7 +128,16 @@ def to_csr(src, device=None, index_dtype=torch.int64) -> GenericSparseMatrixCSR:
 
 # ------------------------------------------------------------------ to CSC --
 def csc_from_scipy(A, device=None, index_dtype=torch.int64, dtype=None):
-    """GenericSparseMatrixCSC(::SparseMatrixCSC) (conversions.jl:7-8)."""
+    """GenericSparseMatrixCSC(::SparseMatrixCSC) (conversions.jl:7-8): the CSC arrays
+    as they are.  A scipy CSR input is copied to the device and transposed there."""
+    if A.format == "csc":
+        return GenericSparseMatrixCSC(A.shape[0], A.shape[1],
+                                      *_upload(A, _dev(device), index_dtype, dtype), copy=False)
+    if A.format == "csr":
+        m, n = A.shape
+        Ar = GenericSparseMatrixCSR(m, n, *_upload(A, _dev(device), index_dtype, dtype),
+                                    copy=False)
+        return csr_to_csc(Ar)
     B = A.tocsc(copy=True)
     B.sum_duplicates()
     B.sort_indices()

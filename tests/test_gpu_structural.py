@@ -339,3 +339,36 @@ def test_spgemm_row_paths(dev, case):
     C = dev_csr(H1) * dev_csr(H2)
     same(csr_tuple(C), O.spgemm_csr(H1.m, H2.n, (H1.rowptr, H1.colval, H1.nzval),
                                     (H2.rowptr, H2.colval, H2.nzval)))
+
+
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+def test_scipy_ingest_transposes_on_the_device(dev, itype):
+    """GenericSparseMatrixCSR(::SparseMatrixCSC) (conversions.jl:25-30): the CSC
+    arrays as given (rows unsorted within a column, a duplicate) transposed on the
+    device; the result is the oracle's CSC -> CSR of the same arrays."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(71)
+    S = sp.random(300, 200, density=0.05, random_state=rng, format="csc")
+    S = sp.csc_matrix((S.data, S.indices, S.indptr), shape=S.shape)
+    for c in range(0, 200, 3):  # reverse the rows of every third column
+        a, b = S.indptr[c], S.indptr[c + 1]
+        S.indices[a:b] = S.indices[a:b][::-1].copy()
+        S.data[a:b] = S.data[a:b][::-1].copy()
+    # a duplicate (row, column) pair: column 1's first entry stored twice
+    a = S.indptr[1]
+    ind = np.insert(S.indices, a, S.indices[a])
+    dat = np.insert(S.data, a, 0.25)
+    ptr_ = S.indptr.copy()
+    ptr_[2:] += 1
+    S = sp.csc_matrix((dat, ind, ptr_), shape=(300, 200))
+    S.has_sorted_indices = False
+    cp, rv, nz = S.indptr.astype(np.int64) + 1, S.indices.astype(np.int64) + 1, S.data.copy()
+    assert nz.size == ind.size  # scipy kept the duplicate
+    A = G.convert.csr_from_scipy(S, device=dev, index_dtype=itype)
+    same(csr_tuple(A), O.csc_to_csr(300, 200, cp, rv, nz))
+    Ac = G.convert.csc_from_scipy(S, device=dev, index_dtype=itype)
+    same(csc_tuple(Ac), (cp, rv, nz))  # the arrays as given (conversions.jl:7-8)
+    R = sp.random(250, 180, density=0.05, random_state=rng, format="csr")
+    Acr = G.convert.csc_from_scipy(R, device=dev, index_dtype=itype)
+    rp, cv, nzr = R.indptr.astype(np.int64) + 1, R.indices.astype(np.int64) + 1, R.data.copy()
+    same(csc_tuple(Acr), O.csr_to_csc(250, 180, rp, cv, nzr))
