@@ -454,8 +454,9 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
     int64_t m, int64_t n, int64_t k, int64_t nnz, const I* __restrict__ rowptr,
     const I* __restrict__ colval, const V* __restrict__ nzval, const float* __restrict__ rscale,
     const T* __restrict__ BT,
-    int64_t ldbt, T* __restrict__ C, int64_t ldc, T alpha, T beta, int mode,
+    int64_t ldbt, T* __restrict__ C, int64_t ldc, int64_t ldk, T alpha, T beta, int mode,
     const int64_t* __restrict__ blk_row, const int64_t* __restrict__ blk_pos) {
+    // output (r, kk) at C[r * ldc + kk * ldk]: column-major (1, ld) or row-major (ld, 1)
     __shared__ CI s_col[SPMV_CAP];
     __shared__ T s_val[SPMV_CAP];
     __shared__ T red[WG];
@@ -545,7 +546,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                 for (int u = 0; u < 8; ++u)
                     if (j + u < e) acc = acc + av[u] * bv[u];
             }
-            store_out(C, r + kk * ldc, row_scaled(acc, rscale, r), alpha, beta, mode);
+            store_out(C, r * ldc + kk * ldk, row_scaled(acc, rscale, r), alpha, beta, mode);
         }
     }
     __syncthreads();
@@ -584,7 +585,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             if (tid < LPR && kk < k) {
                 T sum = red[lane];
                 for (int wq = 1; wq < WG / 64; ++wq) sum = sum + red[wq * LPR + lane];
-                store_out(C, r + kk * ldc, row_scaled(sum, rscale, r), alpha, beta, mode);
+                store_out(C, r * ldc + kk * ldk, row_scaled(sum, rscale, r), alpha, beta, mode);
             }
             __syncthreads();
         }
@@ -629,7 +630,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             if (tid < LPR && kk < k) {
                 T sum = red[lane];
                 for (int wq = 1; wq < WG / 64; ++wq) sum = sum + red[wq * LPR + lane];
-                store_out(C, r + kk * ldc, row_scaled(sum, rscale, r), alpha, beta, mode);
+                store_out(C, r * ldc + kk * ldk, row_scaled(sum, rscale, r), alpha, beta, mode);
             }
         }
     }
@@ -866,12 +867,18 @@ template <typename T, typename I, typename V = T, typename PI = void>
 static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* rowptr,
                       const I* colval, const V* nzval, int64_t nnz, const T* B, int64_t ldb,
                       int opB, T* C, int64_t ldc, double alpha, double beta, gsa_plan plan,
-                      int itype, const float* rscale = nullptr, const PI* perm = nullptr) {
+                      int itype, const float* rscale = nullptr, const PI* perm = nullptr,
+                      bool c_rowmajor = false) {
     if (m == 0 || k == 0) return GSA_OK;
     if (n == 0 || nnz == 0) {  // A*B contributes nothing: only β applies
-        return apply_beta<T>(s, m, k, C, ldc, beta);
+        return c_rowmajor ? apply_beta<T>(s, k, m, C, ldc, beta)
+                          : apply_beta<T>(s, m, k, C, ldc, beta);
     }
+    GSA_CHECK(!c_rowmajor || k > 1 || ldc == 1, GSA_ERR_INVALID,
+              "a row-major C with one column needs ldc = 1");
     GSA_CHECK(perm == nullptr || k == 1, GSA_ERR_INVALID, "row views serve SpMV only");
+    if constexpr (!std::is_void_v<PI>)  // values are read through perm: it must exist
+        GSA_CHECK(perm != nullptr || nnz == 0, GSA_ERR_INVALID, "value permutation is null");
     PlanUse pu(s);
     GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr, k == 1));
     const int mode = beta_mode(beta);
@@ -913,15 +920,16 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         BT = bt.as<T>();
         ldbt = k;
     }
+    const int64_t c_ld = c_rowmajor ? ldc : 1, c_lk = c_rowmajor ? 1 : ldc;
 #define GSA_SPMM_LAUNCH(L)                                                                  \
     do {                                                                                    \
         if (n <= INT32_MAX)                                                                 \
             hipLaunchKernelGGL((k_csr_spmm<T, I, int32_t, L, V>), dim3(grid), dim3(WG), 0, s, m, n, k, \
-                               nnz, rowptr, colval, nzval, rscale, BT, ldbt, C, ldc,        \
+                               nnz, rowptr, colval, nzval, rscale, BT, ldbt, C, c_ld, c_lk, \
                                (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos);            \
         else                                                                                \
             hipLaunchKernelGGL((k_csr_spmm<T, I, I, L, V>), dim3(grid), dim3(WG), 0, s, m, n, k, nnz, \
-                               rowptr, colval, nzval, rscale, BT, ldbt, C, ldc,             \
+                               rowptr, colval, nzval, rscale, BT, ldbt, C, c_ld, c_lk,      \
                                (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos);            \
     } while (0)
     if (k <= 2) GSA_SPMM_LAUNCH(2);
@@ -1300,19 +1308,49 @@ int gsa_spmm_csc(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
     });
 }
 
+// opC of the row-major-capable entries: N = column-major C (ldc >= m), T = row-major
+// C (ldc >= k).
+static int check_c_layout(int opC, int64_t k, int64_t ldc) {
+    GSA_CHECK(opC == GSA_OP_N || opC == GSA_OP_T, GSA_ERR_INVALID, "invalid opC %d", opC);
+    if (opC == GSA_OP_T)
+        GSA_CHECK(ldc >= (k > 1 ? k : 1), GSA_ERR_INVALID, "ldc %lld < k %lld for a row-major C",
+                  (long long)ldc, (long long)k);
+    return GSA_OK;
+}
+
 int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
                       const void* rowptr, const void* colval, const uint16_t* nzval_bf16,
                       const float* row_scale, int64_t nnz, const float* B, int64_t ldb, int opB,
-                      float* C, int64_t ldc, float alpha, float beta, gsa_plan plan) {
+                      float* C, int64_t ldc, int opC, float alpha, float beta, gsa_plan plan) {
     GSA_TRY(ensure_init());
-    GSA_TRY(check_common(GSA_OP_N, opB, m, n, k, nnz, ldb, ldc));
+    GSA_TRY(check_c_layout(opC, k, ldc));
+    GSA_TRY(check_common(GSA_OP_N, opB, m, n, k, nnz, ldb, opC == GSA_OP_T ? m : ldc));
     GSA_CHECK(m == 0 || row_scale, GSA_ERR_INVALID, "gsa_spmm_csr_bf16: row_scale is null");
     hipStream_t s = as_stream(stream);
     return dispatch_i(itype, [&](auto ti) {
         using I = decltype(ti);
-        return gather_csr<float, I, uint16_t>(s, m, n, k, (const I*)rowptr, (const I*)colval,
-                                              nzval_bf16, nnz, B, ldb, opB != GSA_OP_N, C, ldc,
-                                              alpha, beta, plan, itype, row_scale);
+        return gather_csr<float, I, uint16_t, void>(s, m, n, k, (const I*)rowptr,
+                                                    (const I*)colval, nzval_bf16, nnz, B, ldb,
+                                                    opB != GSA_OP_N, C, ldc, alpha, beta, plan,
+                                                    itype, row_scale, nullptr, opC == GSA_OP_T);
+    });
+}
+
+int gsa_spmm_csr_ex(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t k,
+                    const void* rowptr, const void* colval, const void* nzval, int64_t nnz,
+                    const void* B, int64_t ldb, int opB, void* C, int64_t ldc, int opC,
+                    double alpha, double beta, gsa_plan plan) {
+    GSA_TRY(ensure_init());
+    GSA_TRY(check_c_layout(opC, k, ldc));
+    GSA_TRY(check_common(GSA_OP_N, opB, m, n, k, nnz, ldb, opC == GSA_OP_T ? m : ldc));
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        return gather_csr<T, I, T, void>(s, m, n, k, (const I*)rowptr, (const I*)colval,
+                                         (const T*)nzval, nnz, (const T*)B, ldb, opB != GSA_OP_N,
+                                         (T*)C, ldc, alpha, beta, plan, itype, nullptr, nullptr,
+                                         opC == GSA_OP_T);
     });
 }
 

@@ -77,5 +77,5 @@ def mul_bf16(C: torch.Tensor, A: BF16CSR, B: torch.Tensor, alpha=1.0, beta=0.0):
     ldc = max(A.m, 1) if C.dim() == 1 else max(C.stride(1), A.m, 1)
     L.call("gsa_spmm_csr_bf16", stream_of(dev), itype_of(A.rowptr), A.m, A.n, k, ptr(A.rowptr),
            ptr(A.colval), ptr(A.nzval), ptr(A.row_scale), A.nnz(), ptr(Bt), ldb, opB, ptr(C), ldc,
-           float(alpha), float(beta), A.plan() if A.nnz() > 0 else None)
+           L.GSA_OP_N, float(alpha), float(beta), A.plan() if A.nnz() > 0 else None)
     return C

@@ -197,9 +197,9 @@ class Server:
         return self._m[mids] * ks * 4
 
     def _big_call(self, slot: _Slot, mid: int, k: int, cptr: int):
-        """One big-matrix request (k right-hand sides, column-major m x k result at
-        device address cptr) through the dedicated SpMV/SpMM kernels on the
-        slot's stream."""
+        """One big-matrix request (k right-hand sides, row-major m x k result at
+        device address cptr: each row's k outputs in one segment) through the
+        dedicated SpMV/SpMM kernels on the slot's stream."""
         reg = self.reg
         A = reg.mats[mid]
         m = A.shape[0]
@@ -216,12 +216,13 @@ class Server:
             L.call("gsa_spmm_csr_bf16", st, reg.itype, m, A.n, k,
                    ctypes.c_void_p(A.rowptr.data_ptr()), ctypes.c_void_p(A.colval.data_ptr()),
                    ctypes.c_void_p(A.nzval.data_ptr()), ctypes.c_void_p(A.row_scale.data_ptr()),
-                   A.nnz(), ctypes.c_void_p(bptr), ldb, opB, c, max(m, 1), 1.0, 0.0, plan)
+                   A.nnz(), ctypes.c_void_p(bptr), ldb, opB, c, max(k, 1), L.GSA_OP_T, 1.0, 0.0,
+                   plan)
         else:
-            L.call("gsa_spmm_csr", st, L.GSA_F32, reg.itype, L.GSA_OP_N, m, A.n, k,
+            L.call("gsa_spmm_csr_ex", st, L.GSA_F32, reg.itype, m, A.n, k,
                    ctypes.c_void_p(A.rowptr.data_ptr()), ctypes.c_void_p(A.colval.data_ptr()),
                    ctypes.c_void_p(A.nzval.data_ptr()), A.nnz(), ctypes.c_void_p(bptr), ldb, opB,
-                   c, max(m, 1), 1.0, 0.0, plan)
+                   c, max(k, 1), L.GSA_OP_T, 1.0, 0.0, plan)
 
     def _launch(self, slot: _Slot, idx: np.ndarray, mids, ks, out_list):
         """One group: the small-matrix requests in one batched launch, then each
@@ -281,8 +282,7 @@ class Server:
         slot.used_words = int(offs[-1])
         if out_list is not None:
             for j, i in enumerate(idx.tolist()):
-                out_list[i] = (out, int(offs[j]), int(ms[j]), int(ks[j]),
-                               "col" if isbig[j] else "row")
+                out_list[i] = (out, int(offs[j]), int(ms[j]), int(ks[j]), "row")
 
     def run(self, req: Requests, keep_outputs: bool = False, spin_sleep: float = 0.0):
         """Serve `req` in real time; returns per-request latency (s) and summary stats.
@@ -413,8 +413,8 @@ class PlainServer:
 
 
 def output_view(entry):
-    """The (m x k) result of a served request (row-major from the batched
-    launch, column-major from the dedicated kernels)."""
+    """The (m x k) result of a served request (row-major, from the batched
+    launch and from the dedicated kernels alike)."""
     out, off, m, k, layout = entry
     seg = out[off:off + m * k]
     return seg.view(m, k) if layout == "row" else seg.view(k, m).t()

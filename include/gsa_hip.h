@@ -177,16 +177,27 @@ int gsa_plan_create_coo(void* stream, int itype, int64_t m, int64_t n, int64_t n
 int gsa_plan_create_csc(void* stream, int itype, int64_t m, int64_t n, int64_t nnz,
                         const void* colptr, const void* rowval, gsa_plan* plan);
 
+/* gsa_spmm_csr with opA = N and a choice of C layout: opC = GSA_OP_N keeps C
+ * column-major (ldc >= m), opC = GSA_OP_T stores it row-major, C[r*ldc + j]
+ * (ldc >= k; ldc = 1 when k = 1), so the k outputs of a row are written as one
+ * contiguous segment instead of k lines ldc apart.  For callers that keep
+ * results row-major (the serving loop); no counterpart in the reference, whose
+ * C is a column-major DenseVecOrMat. */
+int gsa_spmm_csr_ex(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t k,
+                    const void* rowptr, const void* colval, const void* nzval, int64_t nnz,
+                    const void* B, int64_t ldb, int opB, void* C, int64_t ldc, int opC,
+                    double alpha, double beta, gsa_plan plan);
+
 /* BF16-stored CSR SpMV/SpMM with a Float32 scale per row, Float32 B/C and
  * Float32 accumulation (TwoSum-compensated on rows reduced across lanes,
  * which include every row over 256 entries):
  * C = beta*C + alpha * diag(row_scale) * A_bf16 * opB(B), same kernels and
- * layouts as gsa_spmm_csr with opA = N.  Serving precision variant; no
+ * layouts as gsa_spmm_csr_ex (opC as there).  Serving precision variant; no
  * counterpart in the reference (whose Tv is a Julia number type). */
 int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
                       const void* rowptr, const void* colval, const uint16_t* nzval_bf16,
                       const float* row_scale, int64_t nnz,
-                      const float* B, int64_t ldb, int opB, float* C, int64_t ldc,
+                      const float* B, int64_t ldb, int opB, float* C, int64_t ldc, int opC,
                       float alpha, float beta, gsa_plan plan);
 
 /* ========================== format conversion =============================== */

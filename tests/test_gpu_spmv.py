@@ -605,3 +605,47 @@ def test_transposed_bins_float32(dev):
                          np.abs(x.astype(np.float64)))
     got = host(G.transpose(A) @ torch.as_tensor(x, device=dev))
     assert_normwise(got, ref, bound, 2e-6)
+
+
+@pytest.mark.parametrize("k", [1, 3, 16, 40])
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_spmm_rowmajor_c_matches_columnmajor(dev, k, dtype):
+    """gsa_spmm_csr_ex with a row-major C (opC = T, ldc >= k) gives the same bits as
+    the column-major product, β modes included; the BF16 variant likewise."""
+    from gsa_hip import _lib as L
+    from gsa_hip.matrix import stream_of, ptr, itype_of, vtype_of
+    from gsa_hip.precision import BF16CSR
+    rng = np.random.default_rng(81)
+    H = synth.powerlaw_csr(20_000, 15_000, alpha=1.846, seed=82, hi=3000)
+    A = dev_csr(H, dtype=dtype)
+    A.plan()
+    B = G.dense(rng.standard_normal((H.n, k)), device=dev, dtype=dtype) if k > 1 else \
+        torch.as_tensor(rng.standard_normal(H.n), device=dev, dtype=dtype)
+    c0 = rng.standard_normal((H.m, k))
+    for alpha, beta in ((1.0, 0.0), (0.5, -2.0), (1.0, 1.0)):
+        Cc = G.dense(c0, device=dev, dtype=dtype) if k > 1 else \
+            torch.as_tensor(c0[:, 0].copy(), device=dev, dtype=dtype)
+        G.mul_(Cc, A, B, alpha, beta)
+        ldc = k + 3 if k > 1 else 1
+        Cr = torch.zeros((H.m, ldc), dtype=dtype, device=dev)
+        Cr[:, :k] = torch.as_tensor(c0, dtype=dtype, device=dev)
+        Bt, kk, ldb, opB = G.ops._dense_operand(B)
+        L.call("gsa_spmm_csr_ex", stream_of(dev), vtype_of(A.nzval), itype_of(A.rowptr), H.m,
+               H.n, k, ptr(A.rowptr), ptr(A.colval), ptr(A.nzval), H.nnz, ptr(Bt), ldb, opB,
+               ptr(Cr), ldc, L.GSA_OP_T, alpha, beta, A.plan())
+        got = host(Cr)[:, :k]
+        exp = host(Cc).reshape(H.m, k)
+        assert np.array_equal(got, exp)
+        if k > 1:
+            assert np.array_equal(host(Cr)[:, k:], np.zeros((H.m, 3)))  # padding untouched
+    if dtype == torch.float32:
+        Ab = BF16CSR(A)
+        Cc = torch.zeros((k, H.m), dtype=torch.float32, device=dev).t() if k > 1 else \
+            torch.zeros(H.m, dtype=torch.float32, device=dev)
+        G.precision.mul_bf16(Cc, Ab, B, 1.0, 0.0)
+        Cr = torch.zeros((H.m, max(k, 1)), dtype=torch.float32, device=dev)
+        Bt, kk, ldb, opB = G.ops._dense_operand(B)
+        L.call("gsa_spmm_csr_bf16", stream_of(dev), itype_of(A.rowptr), H.m, H.n, k,
+               ptr(Ab.rowptr), ptr(Ab.colval), ptr(Ab.nzval), ptr(Ab.row_scale), H.nnz, ptr(Bt),
+               ldb, opB, ptr(Cr), max(k, 1), L.GSA_OP_T, 1.0, 0.0, Ab.plan())
+        assert np.array_equal(host(Cr), host(Cc).reshape(H.m, k))
