@@ -201,6 +201,13 @@ def bench_structural(H, dev):
     t = _sync_time(lambda: cL + cLp)
     out["add_coo_L_Lp"] = {"ms": round(t * 1e3, 3), "nnzC": Cc.nnz(),
                            "GBs": round(((L.nnz + Lp.nnz) * 24 + Cc.nnz() * 24) / t / 1e9, 1)}
+    # column-major COO (findnz order, as COO(::SparseMatrixCSC) stores it): the merge
+    cL = G.GenericSparseMatrixCOO(G.GenericSparseMatrixCSC(dL))
+    cLp = G.GenericSparseMatrixCOO(G.GenericSparseMatrixCSC(dLp))
+    Cc = cL + cLp
+    t = _sync_time(lambda: cL + cLp)
+    out["add_coo_colmajor_L_Lp"] = {"ms": round(t * 1e3, 3), "nnzC": Cc.nnz(),
+                                    "GBs": round(((L.nnz + Lp.nnz) * 24 + Cc.nnz() * 24) / t / 1e9, 1)}
     del cL, cLp, Cc
     for name, X in (("spgemm_LL", dL), ("spgemm_RR", dR)):
         Cx = X * X

@@ -153,6 +153,92 @@ __global__ void k_concat_keys(int64_t nA, const I* __restrict__ rA, const I* __r
     }
 }
 
+// Set *flag when keys[0, nA) or keys[nA, tot) is not non-decreasing.
+__global__ void k_parts_unsorted(int64_t nA, int64_t tot, const uint64_t* __restrict__ keys,
+                                 int* __restrict__ flag) {
+    for (int64_t i = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (i != nA && keys[i] < keys[i - 1]) *flag = 1;
+}
+
+// Stable merge of the two sorted parts keys[0, nA) and keys[nA, tot) into
+// keys_out / pos_out (pos = index into the concatenation).  Merge path: output
+// position d of the merge holds, before it, the first i entries of part A and
+// the first d - i of part B, with i found by a binary search on that diagonal;
+// on equal keys part A's entry comes first, so the order is that of a stable
+// sort of the concatenation.  Each workgroup takes MERGE_TILE outputs: it finds
+// its two diagonals, loads the A and B ranges between them into LDS, merges
+// MERGE_CH outputs per lane from LDS and writes the tile coalesced.
+constexpr int MERGE_CH = 8;
+constexpr int MERGE_WG = 256;
+constexpr int MERGE_TILE = MERGE_CH * MERGE_WG;
+
+__device__ __forceinline__ int64_t merge_split(const uint64_t* __restrict__ A, int64_t nA,
+                                               const uint64_t* __restrict__ B, int64_t nB,
+                                               int64_t d) {
+    int64_t lo = d - nB > 0 ? d - nB : 0, hi = d < nA ? d : nA;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) >> 1;
+        if (A[mid] <= B[d - 1 - mid]) lo = mid + 1;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename V>
+__global__ __launch_bounds__(MERGE_WG) void k_merge_parts(int64_t nA, int64_t tot,
+                                                          const uint64_t* __restrict__ keys,
+                                                          uint64_t* __restrict__ keys_out,
+                                                          V* __restrict__ pos_out) {
+    __shared__ uint64_t s_in[MERGE_TILE];   // A's range, then B's
+    __shared__ uint64_t s_key[MERGE_TILE];
+    __shared__ V s_pos[MERGE_TILE];
+    __shared__ int64_t s_split[2];
+    const int64_t nB = tot - nA;
+    const uint64_t* __restrict__ B = keys + nA;
+    const int64_t d0 = (int64_t)blockIdx.x * MERGE_TILE;
+    if (d0 >= tot) return;
+    const int64_t d1 = d0 + MERGE_TILE < tot ? d0 + MERGE_TILE : tot;
+    if (threadIdx.x < 2) s_split[threadIdx.x] = merge_split(keys, nA, B, nB, threadIdx.x ? d1 : d0);
+    __syncthreads();
+    const int64_t a0 = s_split[0], a1 = s_split[1];
+    const int64_t b0 = d0 - a0, b1 = d1 - a1;
+    const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
+    for (int t = threadIdx.x; t < na + nb; t += MERGE_WG)
+        s_in[t] = t < na ? keys[a0 + t] : B[b0 + (t - na)];
+    __syncthreads();
+    const uint64_t* sA = s_in;
+    const uint64_t* sB = s_in + na;
+    const int dl = threadIdx.x * MERGE_CH;
+    if (dl < na + nb) {
+        int lo = dl - nb > 0 ? dl - nb : 0, hi = dl < na ? dl : na;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (sA[mid] <= sB[dl - 1 - mid]) lo = mid + 1;
+            else hi = mid;
+        }
+        int i = lo, j = dl - lo;
+        const int e = dl + MERGE_CH < na + nb ? dl + MERGE_CH : na + nb;
+        for (int q = dl; q < e; ++q) {
+            const bool take_a = i < na && (j >= nb || sA[i] <= sB[j]);
+            if (take_a) {
+                s_key[q] = sA[i];
+                s_pos[q] = (V)(a0 + i);
+                ++i;
+            } else {
+                s_key[q] = sB[j];
+                s_pos[q] = (V)(nA + b0 + j);
+                ++j;
+            }
+        }
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < na + nb; t += MERGE_WG) {
+        keys_out[d0 + t] = s_key[t];
+        pos_out[d0 + t] = s_pos[t];
+    }
+}
+
 template <typename T, typename I, typename V>
 __global__ void k_concat_gather(int64_t tot, const V* __restrict__ perm, int64_t nA,
                                 const I* __restrict__ rA, const I* __restrict__ cA,
@@ -820,16 +906,46 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
         c->aux[0] = tot;
         auto fail = [&](int r) { gsa_ctx_destroy(c); return r; };
         if (tot > 0) {
-            Scratch keys(s), perm(s);
+            Scratch keys(s), perm(s), flag(s);
             int r;
-            if ((r = keys.alloc(sizeof(uint64_t) * tot)) != GSA_OK) return fail(r);
-            if ((r = perm.alloc(sizeof(uint64_t) * tot)) != GSA_OK) return fail(r);
-            hipLaunchKernelGGL((k_concat_keys<I, uint64_t>), dim3(grid_for(tot, 256)), dim3(256), 0, s,
-                               nnzA, (const I*)rowA, (const I*)colA, nnzB, (const I*)rowB,
-                               (const I*)colB, m, n, keys.as<uint64_t>(), perm.as<uint64_t>());
-            const int bits = bits_for((uint64_t)(n - 1) * (uint64_t)m + (uint64_t)(m - 1));
-            if ((r = radix_sort_pairs_u64(s, keys.as<uint64_t>(), perm.as<uint64_t>(), tot, bits)) != GSA_OK)
-                return fail(r);
+            const uint64_t* skeys = nullptr;  // keys in column-major order
+            // the concatenation sorted stably by (col, row): a merge when both operands
+            // are already in that order (COO from CSC or from a SparseMatrixCSC), else
+            // the radix sort; positions are 32-bit when they fit
+            auto order = [&](auto pos_t) -> int {
+                using V = decltype(pos_t);
+                GSA_TRY(keys.alloc(sizeof(uint64_t) * 2 * tot));
+                GSA_TRY(perm.alloc(sizeof(V) * tot));
+                GSA_TRY(flag.alloc(sizeof(int)));
+                uint64_t* k0 = keys.as<uint64_t>();
+                uint64_t* k1 = k0 + tot;
+                hipLaunchKernelGGL((k_concat_keys<I, V>), dim3(grid_for(tot, 256)), dim3(256), 0,
+                                   s, nnzA, (const I*)rowA, (const I*)colA, nnzB, (const I*)rowB,
+                                   (const I*)colB, m, n, k0, perm.as<V>());
+                GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
+                hipLaunchKernelGGL(k_parts_unsorted, dim3(grid_for(tot, 256)), dim3(256), 0, s,
+                                   nnzA, tot, (const uint64_t*)k0, flag.as<int>());
+                int unsorted = 1;
+                GSA_HIP(hipMemcpyAsync(&unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                GSA_HIP(hipStreamSynchronize(s));
+                if (!unsorted) {
+                    hipLaunchKernelGGL((k_merge_parts<V>), dim3((unsigned)cdiv(tot, MERGE_TILE)),
+                                       dim3(MERGE_WG), 0, s, nnzA, tot, (const uint64_t*)k0, k1,
+                                       perm.as<V>());
+                    GSA_HIP(hipGetLastError());
+                    skeys = k1;
+                    return GSA_OK;
+                }
+                const int bits = bits_for((uint64_t)(n - 1) * (uint64_t)m + (uint64_t)(m - 1));
+                if constexpr (std::is_same_v<V, uint32_t>)
+                    GSA_TRY(radix_sort_pairs_u64_u32(s, k0, perm.as<V>(), tot, bits));
+                else
+                    GSA_TRY(radix_sort_pairs_u64(s, k0, perm.as<V>(), tot, bits));
+                skeys = k0;
+                return GSA_OK;
+            };
+            const bool pos32 = tot < (int64_t)UINT32_MAX;
+            if ((r = pos32 ? order(uint32_t{}) : order(uint64_t{})) != GSA_OK) return fail(r);
             for (int b = 0; b < 4; ++b) {
                 size_t sz = b < 2 ? sizeof(I) * tot : (b == 2 ? sizeof(T) * tot : sizeof(T) * tot);
                 if ((r = ws_alloc(&c->buf[b], sz, s)) != GSA_OK) return fail(r);
@@ -837,12 +953,18 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
             if ((r = ws_alloc(&c->buf[4], sizeof(int64_t) * tot, s)) != GSA_OK) return fail(r);
             I* ro = (I*)c->buf[0]; I* co = (I*)c->buf[1]; T* vo = (T*)c->buf[2];
             T* sums = (T*)c->buf[3]; int64_t* keep = (int64_t*)c->buf[4];
-            hipLaunchKernelGGL((k_concat_gather<T, I, uint64_t>), dim3(grid_for(tot, 256)), dim3(256), 0,
-                               s, tot, (const uint64_t*)perm.as<uint64_t>(), nnzA, (const I*)rowA,
-                               (const I*)colA, (const T*)valA, (const I*)rowB, (const I*)colB,
-                               (const T*)valB, ro, co, vo);
+            if (pos32)
+                hipLaunchKernelGGL((k_concat_gather<T, I, uint32_t>), dim3(grid_for(tot, 256)),
+                                   dim3(256), 0, s, tot, (const uint32_t*)perm.p, nnzA,
+                                   (const I*)rowA, (const I*)colA, (const T*)valA, (const I*)rowB,
+                                   (const I*)colB, (const T*)valB, ro, co, vo);
+            else
+                hipLaunchKernelGGL((k_concat_gather<T, I, uint64_t>), dim3(grid_for(tot, 256)),
+                                   dim3(256), 0, s, tot, (const uint64_t*)perm.p, nnzA,
+                                   (const I*)rowA, (const I*)colA, (const T*)valA, (const I*)rowB,
+                                   (const I*)colB, (const T*)valB, ro, co, vo);
             hipLaunchKernelGGL(k_run_sums<T>, dim3(grid_for(tot, 256)), dim3(256), 0, s, tot,
-                               (const uint64_t*)keys.as<uint64_t>(), (const T*)vo, sums, keep);
+                               skeys, (const T*)vo, sums, keep);
             if ((r = inclusive_scan<int64_t>(s, keep, keep, tot, 0)) != GSA_OK) return fail(r);
             hipError_t e = hipMemcpyAsync(&c->nnz_out, keep + tot - 1, sizeof(int64_t),
                                           hipMemcpyDeviceToHost, s);

@@ -372,3 +372,38 @@ def test_scipy_ingest_transposes_on_the_device(dev, itype):
     Acr = G.convert.csc_from_scipy(R, device=dev, index_dtype=itype)
     rp, cv, nzr = R.indptr.astype(np.int64) + 1, R.indices.astype(np.int64) + 1, R.data.copy()
     same(csc_tuple(Acr), O.csr_to_csc(250, 180, rp, cv, nzr))
+
+
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+def test_coo_add_merge_and_sort_paths(dev, itype):
+    """COO + COO (matrix_coo.jl:300-384): operands already in column-major order take
+    the merge; others the radix sort.  Both equal the oracle's stable sort of the
+    concatenation, with duplicates inside an operand and across the two operands
+    (the first operand's entry first in each run), cancellations dropped."""
+    rng = np.random.default_rng(91)
+    m, n = 700, 500
+
+    def colmajor(nnz, dup):
+        r = rng.integers(1, m + 1, nnz)
+        c = rng.integers(1, n + 1, nnz)
+        r = np.r_[r, r[:dup]]
+        c = np.r_[c, c[:dup]]
+        o = np.lexsort((r, c))  # column-major, stable
+        return r[o].astype(np.int64), c[o].astype(np.int64), rng.integers(-3, 4, o.size).astype(np.float64)
+    A = colmajor(20_000, 300)
+    B = colmajor(15_000, 200)
+    B = (np.r_[B[0], A[0][:500]], np.r_[B[1], A[1][:500]], np.r_[B[2], -A[2][:500]])
+    o = np.lexsort((B[0], B[1]))
+    B = tuple(x[o] for x in B)  # shares 500 entries of A with negated values: exact zeros
+    exp = O.coo_add(m, A, B)
+    dA = from_arrays(G.GenericSparseMatrixCOO, m, n, *A, dev, itype)
+    dB = from_arrays(G.GenericSparseMatrixCOO, m, n, *B, dev, itype)
+    same(coo_tuple(dA + dB), exp)  # merge path
+    pA = rng.permutation(A[0].size)
+    Ash = tuple(x[pA] for x in A)
+    dAs = from_arrays(G.GenericSparseMatrixCOO, m, n, *Ash, dev, itype)
+    same(coo_tuple(dAs + dB), O.coo_add(m, Ash, B))  # sort path
+    E = from_arrays(G.GenericSparseMatrixCOO, m, n, np.zeros(0, np.int64), np.zeros(0, np.int64),
+                    np.zeros(0), dev, itype)
+    same(coo_tuple(E + dB), O.coo_add(m, (np.zeros(0), np.zeros(0), np.zeros(0)), B))
+    same(coo_tuple(dA + E), O.coo_add(m, A, (np.zeros(0), np.zeros(0), np.zeros(0))))
