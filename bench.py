@@ -157,6 +157,7 @@ def bench_structural(H, dev):
     nnz, m, n = H.nnz, H.m, H.n
     Ac = G.GenericSparseMatrixCSC(A)
     Ao = G.GenericSparseMatrixCOO(A)
+    Aoc = G.GenericSparseMatrixCOO(Ac)
     for name, fn, byts in (
             ("csr_to_csc", lambda: G.GenericSparseMatrixCSC(A),
              nnz * 16 + (m + 1) * 8 + nnz * 16 + (n + 1) * 8),
@@ -165,10 +166,13 @@ def bench_structural(H, dev):
             ("csr_to_coo", lambda: G.GenericSparseMatrixCOO(A),
              nnz * 16 + (m + 1) * 8 + nnz * 24),
             ("coo_to_csr", lambda: G.GenericSparseMatrixCSR(Ao),
+             nnz * 24 + nnz * 16 + (m + 1) * 8),
+            # the same from column-major storage: keys out of order, the sort runs
+            ("coo_colmajor_to_csr", lambda: G.GenericSparseMatrixCSR(Aoc),
              nnz * 24 + nnz * 16 + (m + 1) * 8)):
         t = _sync_time(fn)
         out[name] = {"ms": round(t * 1e3, 3), "GBs": round(byts / t / 1e9, 1)}
-    del Ac, Ao, A
+    del Ac, Ao, Aoc, A
     # host ingest: GenericSparseMatrixCSR(::SparseMatrixCSC) of the same matrix held
     # as a scipy CSC (the arrays copied as they are, transposed on the device),
     # against the host transposition the reference performs (scipy's tocsr)

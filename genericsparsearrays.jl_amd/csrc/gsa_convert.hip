@@ -59,6 +59,14 @@ __global__ void k_make_keys(int64_t nnz, const I* __restrict__ major, const I* _
     }
 }
 
+// Set *flag when the keys are not non-decreasing.
+template <typename K>
+__global__ void k_keys_unsorted(int64_t nnz, const K* __restrict__ keys, int* __restrict__ flag) {
+    for (int64_t i = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x)
+        if (keys[i] < keys[i - 1]) *flag = 1;
+}
+
 // out_idx[i] = idx[perm[i]], out_val[i] = val[perm[i]].  Each thread takes
 // GATHER_U positions a grid stride apart and issues all their reads before
 // the first write, so that many random reads are in flight per wavefront.
@@ -195,6 +203,27 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
         GSA_TRY(perm.alloc(sizeof(V) * nnz));
         hipLaunchKernelGGL((k_make_keys<I, K, V>), dim3(g), dim3(256), 0, s, nnz, major, minor_key,
                            (int64_t)kdim, major_dim, keys.as<K>(), perm.as<V>());
+        if (!packed) {
+            // COO storage already in the target order (a COO made from CSR, or from CSC
+            // for CSC): the stable sort is the identity, so the arrays are copied
+            Scratch flag(s);
+            GSA_TRY(flag.alloc(sizeof(int)));
+            GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
+            hipLaunchKernelGGL((k_keys_unsorted<K>), dim3(g), dim3(256), 0, s, nnz,
+                               (const K*)keys.as<K>(), flag.as<int>());
+            int unsorted = 1;
+            GSA_HIP(hipMemcpyAsync(&unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+            GSA_HIP(hipStreamSynchronize(s));
+            if (!unsorted) {
+                GSA_HIP(hipMemcpyAsync(idx_out, minor_out, sizeof(I) * nnz,
+                                       hipMemcpyDeviceToDevice, s));
+                GSA_HIP(hipMemcpyAsync(val_out, val, sizeof(T) * nnz, hipMemcpyDeviceToDevice, s));
+                hipLaunchKernelGGL((k_ptr_from_keys<I, K>), dim3(grid_for(nnz + 1, 256)), dim3(256),
+                                   0, s, nnz, (const K*)keys.as<K>(), kdim, major_dim, ptr_out);
+                GSA_HIP(hipGetLastError());
+                return GSA_OK;
+            }
+        }
         GSA_TRY(sorter(keys.as<K>(), perm.as<V>()));
         if (packed)
             hipLaunchKernelGGL((k_gather_pairs<T, I, V>), dim3(g), dim3(256), 0, s, nnz,

@@ -211,7 +211,9 @@ int gsa_csr_to_coo(void* stream, int vtype, int itype, int64_t m, int64_t nnz,
                    void* rowind, void* colind, void* nzval_out);
 
 /* COO -> CSR: stable sort by key (row, col), gather, row pointer by
- * searchsortedfirst; duplicates are kept in input order.  Replaces
+ * searchsortedfirst; duplicates are kept in input order.  Keys already in
+ * order (a COO made from a CSR) skip the sort: the arrays are copied.  The
+ * call synchronises the stream once, to learn which.  Replaces
  * GenericSparseMatrixCSR(::COO), src/conversions/conversions.jl:282-323
  * (keys kernel_make_csr_keys! conversion_kernels.jl:49-57).  For COO -> CSC
  * (conversions.jl:171-212) pass (n, m, colind, rowind) and read the outputs as

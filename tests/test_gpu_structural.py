@@ -92,6 +92,17 @@ def test_coo_to_csr_unsorted_duplicates_and_empty(dev):
                     np.zeros(0), dev)
     Ecsr = G.GenericSparseMatrixCSR(E)
     assert host(Ecsr.rowptr).tolist() == [1] * 5 and Ecsr.nnz() == 0
+    # storage already in (row, col) order, duplicates and empty rows included: the
+    # sort is skipped and the arrays copied; the same result as the sort
+    o = np.lexsort((c, r))
+    rs, cs, vs = r[o], c[o], v[o]
+    As = from_arrays(G.GenericSparseMatrixCOO, 300, 200, rs, cs, vs, dev)
+    same(csr_tuple(G.GenericSparseMatrixCSR(As)), O.coo_to_csr(300, 200, rs, cs, vs))
+    o = np.lexsort((r, c))  # column-major: sorted for COO -> CSC
+    rc, cc, vc = r[o], c[o], v[o]
+    Ac = from_arrays(G.GenericSparseMatrixCOO, 300, 200, rc, cc, vc, dev)
+    same(csc_tuple(G.GenericSparseMatrixCSC(Ac)), O.coo_to_csc(300, 200, rc, cc, vc))
+    same(csr_tuple(G.GenericSparseMatrixCSR(Ac)), O.coo_to_csr(300, 200, rc, cc, vc))
 
 
 def test_transposed_conversions(dev):
