@@ -59,48 +59,34 @@ __global__ void k_make_keys(int64_t nnz, const I* __restrict__ major, const I* _
     }
 }
 
-// Set *flag when the keys are not non-decreasing.
-template <typename K>
-__global__ void k_keys_unsorted(int64_t nnz, const K* __restrict__ keys, int* __restrict__ flag) {
-    for (int64_t i = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
-         i += (int64_t)gridDim.x * blockDim.x)
-        if (keys[i] < keys[i - 1]) *flag = 1;
+// One-based row pointer from non-decreasing one-based rows (as k_ptr_from_keys).
+template <typename I>
+__global__ void k_ptr_from_major(int64_t nnz, const I* __restrict__ major, int64_t major_dim,
+                                 I* __restrict__ ptr) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t prev = -1, cur = major_dim;
+        if (i > 0) prev = dclamp((int64_t)major[i - 1] - 1, 0, major_dim - 1);
+        if (i < nnz) cur = dclamp((int64_t)major[i] - 1, 0, major_dim - 1);
+        for (int64_t r = prev + 1; r <= cur; ++r) ptr[r] = (I)(i + 1);
+    }
 }
 
-// out_idx[i] = idx[perm[i]], out_val[i] = val[perm[i]].  Each thread takes
-// GATHER_U positions a grid stride apart and issues all their reads before
-// the first write, so that many random reads are in flight per wavefront.
-template <typename T, typename I, typename V>
-__global__ void k_gather(int64_t nnz, const V* __restrict__ perm, const I* __restrict__ minor,
-                         const T* __restrict__ val, I* __restrict__ out_minor,
-                         T* __restrict__ out_val) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < nnz;
-         base += stride * GATHER_U) {
-        int64_t p[GATHER_U];
-#pragma unroll
-        for (int u = 0; u < GATHER_U; ++u) {
-            const int64_t i = base + u * stride;
-            p[u] = i < nnz ? (int64_t)__builtin_nontemporal_load(perm + i) : 0;
-        }
-        I mi[GATHER_U];
-        T vv[GATHER_U];
-#pragma unroll
-        for (int u = 0; u < GATHER_U; ++u) {
-            if (base + u * stride < nnz) {
-                mi[u] = minor[p[u]];
-                vv[u] = val[p[u]];
-            }
-        }
-#pragma unroll
-        for (int u = 0; u < GATHER_U; ++u) {
-            const int64_t i = base + u * stride;
-            if (i < nnz) {
-                out_minor[i] = mi[u];
-                out_val[i] = vv[u];
-            }
-        }
+// Order of COO storage: bit 0 of *flag when (major, minor) pairs decrease
+// somewhere, bit 1 when the minor indices alone do.
+template <typename I>
+__global__ void k_order_flags(int64_t nnz, const I* __restrict__ major, const I* __restrict__ minor,
+                              int* __restrict__ flag) {
+    int f = 0;
+    for (int64_t i = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const I a0 = major[i - 1], a1 = major[i], b0 = minor[i - 1], b1 = minor[i];
+        if (a1 < a0 || (a1 == a0 && b1 < b0)) f |= 1;
+        if (b1 < b0) f |= 2;
     }
+    // one update per workgroup, and none once another has set the same bits
+    f = (__syncthreads_or(f & 1) ? 1 : 0) | (__syncthreads_or(f & 2) ? 2 : 0);
+    if (threadIdx.x == 0 && (f & ~*(volatile int*)flag)) atomicOr(flag, f);
 }
 
 // (minor index, value) of one stored entry, side by side, so that a gather by
@@ -177,60 +163,57 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
         GSA_TRY(zero_fill(s, ptr_out, sizeof(I) * (major_dim + 1)));
         return counts_to_ptr<I>(s, ptr_out, major_dim, nullptr);
     }
-    const uint64_t maxkey = minor_key
+    const unsigned g = grid_for(nnz, 256);
+    const bool transposition = minor_key == nullptr;  // CSR <-> CSC: key = major only
+    // COO storage: one check of the (major, minor) order first.  In order (a COO
+    // made from a CSR, or from a CSC for CSC): the stable sort is the identity and
+    // the arrays are copied.  Ordered by the minor index only (a COO from the
+    // other compressed format): the stable sort by the major index alone gives
+    // the (major, minor) order, with far fewer key bits.  The call synchronises
+    // once to learn which.
+    const I* kminor = minor_key;
+    if (!transposition) {
+        Scratch flag(s);
+        GSA_TRY(flag.alloc(sizeof(int)));
+        GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
+        hipLaunchKernelGGL((k_order_flags<I>), dim3(g), dim3(256), 0, s, nnz, major, minor_key,
+                           flag.as<int>());
+        int flags = 3;
+        GSA_HIP(hipMemcpyAsync(&flags, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        if (!(flags & 1)) {
+            GSA_HIP(hipMemcpyAsync(idx_out, minor_out, sizeof(I) * nnz, hipMemcpyDeviceToDevice, s));
+            GSA_HIP(hipMemcpyAsync(val_out, val, sizeof(T) * nnz, hipMemcpyDeviceToDevice, s));
+            hipLaunchKernelGGL(k_ptr_from_major<I>, dim3(grid_for(nnz + 1, 256)), dim3(256), 0, s,
+                               nnz, major, major_dim, ptr_out);
+            GSA_HIP(hipGetLastError());
+            return GSA_OK;
+        }
+        if (!(flags & 2)) kminor = nullptr;
+    }
+    const uint64_t maxkey = kminor
         ? (uint64_t)(major_dim - 1) * (uint64_t)minor_dim + (uint64_t)(minor_dim - 1)
         : (uint64_t)(major_dim - 1);
     const int bits = bits_for(maxkey);
-    const uint64_t kdim = minor_key ? (uint64_t)minor_dim : 1;
-    const unsigned g = grid_for(nnz, 256);
+    const uint64_t kdim = kminor ? (uint64_t)minor_dim : 1;
+    // Entries that are not in order move far from where they were (a transposition,
+    // or storage in another order): gather (minor, value) pairs, one line per entry
+    // (two separate arrays cost two lines per entry).
     Scratch keys(s), perm(s), pairs(s);
-    // A transposition (key = major only) permutes every entry far from where it
-    // was: gather (minor, value) pairs, one line per entry.  A COO -> CSR of
-    // storage that is often in order already keeps the two-array gather,
-    // which then streams.
-    const bool packed = minor_key == nullptr;
-    const MinorVal<T, I>* pv = nullptr;
-    if (packed) {
-        GSA_TRY(pairs.alloc(sizeof(MinorVal<T, I>) * nnz));
-        hipLaunchKernelGGL((k_pack<T, I>), dim3(g), dim3(256), 0, s, nnz, minor_out, val,
-                           pairs.as<MinorVal<T, I>>());
-        pv = pairs.as<MinorVal<T, I>>();
-    }
+    GSA_TRY(pairs.alloc(sizeof(MinorVal<T, I>) * nnz));
+    hipLaunchKernelGGL((k_pack<T, I>), dim3(g), dim3(256), 0, s, nnz, minor_out, val,
+                       pairs.as<MinorVal<T, I>>());
+    const MinorVal<T, I>* pv = pairs.as<MinorVal<T, I>>();
     auto run = [&](auto key_t, auto pos_t, auto sorter) -> int {
         using K = decltype(key_t);
         using V = decltype(pos_t);
         GSA_TRY(keys.alloc(sizeof(K) * nnz));
         GSA_TRY(perm.alloc(sizeof(V) * nnz));
-        hipLaunchKernelGGL((k_make_keys<I, K, V>), dim3(g), dim3(256), 0, s, nnz, major, minor_key,
+        hipLaunchKernelGGL((k_make_keys<I, K, V>), dim3(g), dim3(256), 0, s, nnz, major, kminor,
                            (int64_t)kdim, major_dim, keys.as<K>(), perm.as<V>());
-        if (!packed) {
-            // COO storage already in the target order (a COO made from CSR, or from CSC
-            // for CSC): the stable sort is the identity, so the arrays are copied
-            Scratch flag(s);
-            GSA_TRY(flag.alloc(sizeof(int)));
-            GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
-            hipLaunchKernelGGL((k_keys_unsorted<K>), dim3(g), dim3(256), 0, s, nnz,
-                               (const K*)keys.as<K>(), flag.as<int>());
-            int unsorted = 1;
-            GSA_HIP(hipMemcpyAsync(&unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-            GSA_HIP(hipStreamSynchronize(s));
-            if (!unsorted) {
-                GSA_HIP(hipMemcpyAsync(idx_out, minor_out, sizeof(I) * nnz,
-                                       hipMemcpyDeviceToDevice, s));
-                GSA_HIP(hipMemcpyAsync(val_out, val, sizeof(T) * nnz, hipMemcpyDeviceToDevice, s));
-                hipLaunchKernelGGL((k_ptr_from_keys<I, K>), dim3(grid_for(nnz + 1, 256)), dim3(256),
-                                   0, s, nnz, (const K*)keys.as<K>(), kdim, major_dim, ptr_out);
-                GSA_HIP(hipGetLastError());
-                return GSA_OK;
-            }
-        }
         GSA_TRY(sorter(keys.as<K>(), perm.as<V>()));
-        if (packed)
-            hipLaunchKernelGGL((k_gather_pairs<T, I, V>), dim3(g), dim3(256), 0, s, nnz,
-                               (const V*)perm.as<V>(), pv, idx_out, val_out);
-        else
-            hipLaunchKernelGGL((k_gather<T, I, V>), dim3(g), dim3(256), 0, s, nnz,
-                               (const V*)perm.as<V>(), minor_out, val, idx_out, val_out);
+        hipLaunchKernelGGL((k_gather_pairs<T, I, V>), dim3(g), dim3(256), 0, s, nnz,
+                           (const V*)perm.as<V>(), pv, idx_out, val_out);
         hipLaunchKernelGGL((k_ptr_from_keys<I, K>), dim3(grid_for(nnz + 1, 256)), dim3(256), 0, s,
                            nnz, (const K*)keys.as<K>(), kdim, major_dim, ptr_out);
         GSA_HIP(hipGetLastError());
@@ -257,18 +240,6 @@ __global__ void k_unsorted(int64_t nnz, const I* __restrict__ major, int* __rest
         if (major[i] < major[i - 1]) *flag = 1;
 }
 
-// One-based row pointer from non-decreasing one-based rows (as k_ptr_from_keys).
-template <typename I>
-__global__ void k_ptr_from_major(int64_t nnz, const I* __restrict__ major, int64_t major_dim,
-                                 I* __restrict__ ptr) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= nnz;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        int64_t prev = -1, cur = major_dim;
-        if (i > 0) prev = dclamp((int64_t)major[i - 1] - 1, 0, major_dim - 1);
-        if (i < nnz) cur = dclamp((int64_t)major[i] - 1, 0, major_dim - 1);
-        for (int64_t r = prev + 1; r <= cur; ++r) ptr[r] = (I)(i + 1);
-    }
-}
 
 template <typename I, typename V>
 __global__ void k_gather_idx(int64_t nnz, const V* __restrict__ perm, const I* __restrict__ in,
