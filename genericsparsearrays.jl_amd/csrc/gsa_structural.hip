@@ -153,12 +153,80 @@ __global__ void k_concat_keys(int64_t nA, const I* __restrict__ rA, const I* __r
     }
 }
 
-// Set *flag when keys[0, nA) or keys[nA, tot) is not non-decreasing.
-__global__ void k_parts_unsorted(int64_t nA, int64_t tot, const uint64_t* __restrict__ keys,
-                                 int* __restrict__ flag) {
+// key / m and key % m of a column-major key, for m > 0: a double-precision
+// estimate of the quotient corrected to the exact one (keys below 2^53, which
+// covers m * n < 2^53), integer division otherwise.
+__device__ __forceinline__ void key_split(uint64_t key, uint64_t m, double inv_m, uint64_t& q,
+                                          uint64_t& r) {
+    if (key < (1ull << 53)) {
+        int64_t qq = (int64_t)((double)key * inv_m);
+        int64_t rr = (int64_t)key - qq * (int64_t)m;
+        while (rr < 0) { --qq; rr += (int64_t)m; }
+        while (rr >= (int64_t)m) { ++qq; rr -= (int64_t)m; }
+        q = (uint64_t)qq;
+        r = (uint64_t)rr;
+    } else {
+        q = key / m;
+        r = key - q * m;
+    }
+}
+
+// Order of the two parts keys[0, nA) and keys[nA, tot) (column-major keys
+// c*m + r): bit 0 / 2 of *flag when part A / B is not in column-major order,
+// bit 1 / 3 when it is not in row-major order either ((r, c) decreases).
+__global__ void k_parts_order(int64_t nA, int64_t tot, uint64_t m,
+                              const uint64_t* __restrict__ keys, int* __restrict__ flag) {
+    int f = 0;
+    const double inv_m = 1.0 / (double)m;
     for (int64_t i = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        if (i == nA) continue;
+        const uint64_t k0 = keys[i - 1], k1 = keys[i];
+        const int sh = i < nA ? 0 : 2;
+        if (k1 < k0) f |= 1 << sh;
+        uint64_t c0, r0, c1, r1;
+        key_split(k0, m, inv_m, c0, r0);
+        key_split(k1, m, inv_m, c1, r1);
+        if (r1 < r0 || (r1 == r0 && c1 < c0)) f |= 2 << sh;
+    }
+    int g = 0;
+    for (int b = 0; b < 4; ++b) g |= __syncthreads_or(f & (1 << b)) ? 1 << b : 0;
+    if (threadIdx.x == 0 && (g & ~*(volatile int*)flag)) atomicOr(flag, g);
+}
+
+// The column of each entry of one part (its key / m) and its position in the part.
+__global__ void k_part_cols(int64_t cnt, const uint64_t* __restrict__ keys, uint64_t m,
+                            uint32_t* __restrict__ col, uint32_t* __restrict__ pos) {
+    const double inv_m = 1.0 / (double)m;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        uint64_t c, r;
+        key_split(keys[i], m, inv_m, c, r);
+        col[i] = (uint32_t)c;
+        pos[i] = (uint32_t)i;
+    }
+}
+
+// A part put in the order of `pos` (positions within the part): its keys, and
+// each entry's position in the concatenation (offset + pos).
+template <typename V>
+__global__ void k_part_apply(int64_t cnt, int64_t offset, const uint32_t* __restrict__ pos,
+                             const uint64_t* __restrict__ keys, uint64_t* __restrict__ keys_out,
+                             V* __restrict__ cat_pos) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < cnt;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const uint32_t p = pos[i];
+        keys_out[i] = keys[p];
+        cat_pos[i] = (V)(offset + p);
+    }
+}
+
+// perm[i] = part_pos[merged[i]]: merge positions back to concatenation positions.
+template <typename V>
+__global__ void k_compose(int64_t tot, const V* __restrict__ part_pos, V* __restrict__ perm) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
          i += (int64_t)gridDim.x * blockDim.x)
-        if (i != nA && keys[i] < keys[i - 1]) *flag = 1;
+        perm[i] = part_pos[(int64_t)perm[i]];
 }
 
 // Stable merge of the two sorted parts keys[0, nA) and keys[nA, tot) into
@@ -185,23 +253,32 @@ __device__ __forceinline__ int64_t merge_split(const uint64_t* __restrict__ A, i
     return lo;
 }
 
+// The merge-path split of every tile boundary d = t * MERGE_TILE (t = 0..ntiles),
+// all searched at once (the tiles then start without a chain of global loads).
+__global__ void k_merge_splits(int64_t nA, int64_t tot, const uint64_t* __restrict__ keys,
+                               int64_t ntiles, int64_t* __restrict__ split) {
+    const int64_t nB = tot - nA;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t <= ntiles;
+         t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t d = t * MERGE_TILE < tot ? t * MERGE_TILE : tot;
+        split[t] = merge_split(keys, nA, keys + nA, nB, d);
+    }
+}
+
 template <typename V>
 __global__ __launch_bounds__(MERGE_WG) void k_merge_parts(int64_t nA, int64_t tot,
                                                           const uint64_t* __restrict__ keys,
+                                                          const int64_t* __restrict__ split,
                                                           uint64_t* __restrict__ keys_out,
                                                           V* __restrict__ pos_out) {
     __shared__ uint64_t s_in[MERGE_TILE];   // A's range, then B's
     __shared__ uint64_t s_key[MERGE_TILE];
     __shared__ V s_pos[MERGE_TILE];
-    __shared__ int64_t s_split[2];
-    const int64_t nB = tot - nA;
     const uint64_t* __restrict__ B = keys + nA;
     const int64_t d0 = (int64_t)blockIdx.x * MERGE_TILE;
     if (d0 >= tot) return;
     const int64_t d1 = d0 + MERGE_TILE < tot ? d0 + MERGE_TILE : tot;
-    if (threadIdx.x < 2) s_split[threadIdx.x] = merge_split(keys, nA, B, nB, threadIdx.x ? d1 : d0);
-    __syncthreads();
-    const int64_t a0 = s_split[0], a1 = s_split[1];
+    const int64_t a0 = split[blockIdx.x], a1 = split[blockIdx.x + 1];
     const int64_t b0 = d0 - a0, b1 = d1 - a1;
     const int na = (int)(a1 - a0), nb = (int)(b1 - b0);
     for (int t = threadIdx.x; t < na + nb; t += MERGE_WG)
@@ -239,30 +316,53 @@ __global__ __launch_bounds__(MERGE_WG) void k_merge_parts(int64_t nA, int64_t to
     }
 }
 
+// keys[0, nA) and keys[nA, tot), each sorted: their stable merge into
+// keys_out, with each output's position in the concatenation in pos_out.
+template <typename V>
+static int merge_sorted_parts(hipStream_t s, int64_t nA, int64_t tot, const uint64_t* keys,
+                              uint64_t* keys_out, V* pos_out) {
+    const int64_t ntiles = cdiv(tot, MERGE_TILE);
+    Scratch split(s);
+    GSA_TRY(split.alloc(sizeof(int64_t) * (ntiles + 1)));
+    hipLaunchKernelGGL(k_merge_splits, dim3(grid_for(ntiles + 1, 256)), dim3(256), 0, s, nA, tot,
+                       keys, ntiles, split.as<int64_t>());
+    hipLaunchKernelGGL((k_merge_parts<V>), dim3((unsigned)ntiles), dim3(MERGE_WG), 0, s, nA, tot,
+                       keys, (const int64_t*)split.as<int64_t>(), keys_out, pos_out);
+    GSA_HIP(hipGetLastError());
+    return GSA_OK;
+}
+
+// The concatenation in sorted order: row and column from the sorted key
+// (c*m + r), the value gathered through the permutation (GATHER_U positions
+// per thread in flight).
 template <typename T, typename I, typename V>
-__global__ void k_concat_gather(int64_t tot, const V* __restrict__ perm, int64_t nA,
-                                const I* __restrict__ rA, const I* __restrict__ cA,
-                                const T* __restrict__ vA, const I* __restrict__ rB,
-                                const I* __restrict__ cB, const T* __restrict__ vB,
-                                I* __restrict__ ro, I* __restrict__ co, T* __restrict__ vo) {
+__global__ void k_sorted_entries(int64_t tot, const V* __restrict__ perm,
+                                 const uint64_t* __restrict__ skeys, uint64_t m, int64_t nA,
+                                 const T* __restrict__ vA, const T* __restrict__ vB,
+                                 I* __restrict__ ro, I* __restrict__ co, T* __restrict__ vo) {
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const double inv_m = 1.0 / (double)m;
     for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < tot;
          base += stride * GATHER_U) {
-        I r[GATHER_U], c[GATHER_U];
         T v[GATHER_U];
 #pragma unroll
         for (int u = 0; u < GATHER_U; ++u) {
             const int64_t i = base + u * stride;
             if (i < tot) {
                 const int64_t p = (int64_t)perm[i];
-                if (p < nA) { r[u] = rA[p]; c[u] = cA[p]; v[u] = vA[p]; }
-                else { r[u] = rB[p - nA]; c[u] = cB[p - nA]; v[u] = vB[p - nA]; }
+                v[u] = p < nA ? vA[p] : vB[p - nA];
             }
         }
 #pragma unroll
         for (int u = 0; u < GATHER_U; ++u) {
             const int64_t i = base + u * stride;
-            if (i < tot) { ro[i] = r[u]; co[i] = c[u]; vo[i] = v[u]; }
+            if (i < tot) {
+                uint64_t c, r;
+                key_split(skeys[i], m, inv_m, c, r);
+                ro[i] = (I)(r + 1);
+                co[i] = (I)(c + 1);
+                vo[i] = v[u];
+            }
         }
     }
 }
@@ -923,15 +1023,50 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
                                    s, nnzA, (const I*)rowA, (const I*)colA, nnzB, (const I*)rowB,
                                    (const I*)colB, m, n, k0, perm.as<V>());
                 GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
-                hipLaunchKernelGGL(k_parts_unsorted, dim3(grid_for(tot, 256)), dim3(256), 0, s,
-                                   nnzA, tot, (const uint64_t*)k0, flag.as<int>());
-                int unsorted = 1;
-                GSA_HIP(hipMemcpyAsync(&unsorted, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+                hipLaunchKernelGGL(k_parts_order, dim3(grid_for(tot, 256)), dim3(256), 0, s, nnzA,
+                                   tot, (uint64_t)m, (const uint64_t*)k0, flag.as<int>());
+                int fl = 15;
+                GSA_HIP(hipMemcpyAsync(&fl, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
                 GSA_HIP(hipStreamSynchronize(s));
-                if (!unsorted) {
-                    hipLaunchKernelGGL((k_merge_parts<V>), dim3((unsigned)cdiv(tot, MERGE_TILE)),
-                                       dim3(MERGE_WG), 0, s, nnzA, tot, (const uint64_t*)k0, k1,
-                                       perm.as<V>());
+                const bool colA = !(fl & 1), colB = !(fl & 4);
+                const bool rowA = !(fl & 2), rowB = !(fl & 8);
+                if (colA && colB) {
+                    GSA_TRY(merge_sorted_parts<V>(s, nnzA, tot, k0, k1, perm.as<V>()));
+                    GSA_HIP(hipGetLastError());
+                    skeys = k1;
+                    return GSA_OK;
+                }
+                if ((colA || rowA) && (colB || rowB) && n <= (int64_t)UINT32_MAX &&
+                    tot < (int64_t)UINT32_MAX) {
+                    // an operand in row-major order (a COO made from a CSR) becomes
+                    // column-major by a stable sort on the column alone; then the merge
+                    Scratch k2(s), ppos(s), cols(s), lpos(s);
+                    GSA_TRY(k2.alloc(sizeof(uint64_t) * tot));
+                    GSA_TRY(ppos.alloc(sizeof(V) * tot));
+                    const int64_t mx = nnzA > nnzB ? nnzA : nnzB;
+                    GSA_TRY(cols.alloc(sizeof(uint32_t) * (mx > 0 ? mx : 1)));
+                    GSA_TRY(lpos.alloc(sizeof(uint32_t) * (mx > 0 ? mx : 1)));
+                    for (int part = 0; part < 2; ++part) {
+                        const int64_t off = part ? nnzA : 0, cnt = part ? nnzB : nnzA;
+                        if (cnt == 0) continue;
+                        const bool colsorted = part ? colB : colA;
+                        const unsigned gp = grid_for(cnt, 256);
+                        hipLaunchKernelGGL(k_part_cols, dim3(gp), dim3(256), 0, s, cnt,
+                                           (const uint64_t*)(k0 + off), (uint64_t)m,
+                                           cols.as<uint32_t>(), lpos.as<uint32_t>());
+                        if (!colsorted)
+                            GSA_TRY(radix_sort_pairs_u32_u32(s, cols.as<uint32_t>(),
+                                                             lpos.as<uint32_t>(), cnt,
+                                                             bits_for((uint64_t)(n - 1))));
+                        hipLaunchKernelGGL((k_part_apply<V>), dim3(gp), dim3(256), 0, s, cnt, off,
+                                           (const uint32_t*)lpos.as<uint32_t>(),
+                                           (const uint64_t*)(k0 + off), k2.as<uint64_t>() + off,
+                                           ppos.as<V>() + off);
+                    }
+                    GSA_TRY(merge_sorted_parts<V>(s, nnzA, tot, k2.as<uint64_t>(), k1,
+                                                  perm.as<V>()));
+                    hipLaunchKernelGGL((k_compose<V>), dim3(grid_for(tot, 256)), dim3(256), 0, s,
+                                       tot, (const V*)ppos.as<V>(), perm.as<V>());
                     GSA_HIP(hipGetLastError());
                     skeys = k1;
                     return GSA_OK;
@@ -954,15 +1089,13 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
             I* ro = (I*)c->buf[0]; I* co = (I*)c->buf[1]; T* vo = (T*)c->buf[2];
             T* sums = (T*)c->buf[3]; int64_t* keep = (int64_t*)c->buf[4];
             if (pos32)
-                hipLaunchKernelGGL((k_concat_gather<T, I, uint32_t>), dim3(grid_for(tot, 256)),
-                                   dim3(256), 0, s, tot, (const uint32_t*)perm.p, nnzA,
-                                   (const I*)rowA, (const I*)colA, (const T*)valA, (const I*)rowB,
-                                   (const I*)colB, (const T*)valB, ro, co, vo);
+                hipLaunchKernelGGL((k_sorted_entries<T, I, uint32_t>), dim3(grid_for(tot, 256)),
+                                   dim3(256), 0, s, tot, (const uint32_t*)perm.p, skeys,
+                                   (uint64_t)m, nnzA, (const T*)valA, (const T*)valB, ro, co, vo);
             else
-                hipLaunchKernelGGL((k_concat_gather<T, I, uint64_t>), dim3(grid_for(tot, 256)),
-                                   dim3(256), 0, s, tot, (const uint64_t*)perm.p, nnzA,
-                                   (const I*)rowA, (const I*)colA, (const T*)valA, (const I*)rowB,
-                                   (const I*)colB, (const T*)valB, ro, co, vo);
+                hipLaunchKernelGGL((k_sorted_entries<T, I, uint64_t>), dim3(grid_for(tot, 256)),
+                                   dim3(256), 0, s, tot, (const uint64_t*)perm.p, skeys,
+                                   (uint64_t)m, nnzA, (const T*)valA, (const T*)valB, ro, co, vo);
             hipLaunchKernelGGL(k_run_sums<T>, dim3(grid_for(tot, 256)), dim3(256), 0, s, tot,
                                skeys, (const T*)vo, sums, keep);
             if ((r = inclusive_scan<int64_t>(s, keep, keep, tot, 0)) != GSA_OK) return fail(r);

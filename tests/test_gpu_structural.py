@@ -414,6 +414,17 @@ def test_coo_add_merge_and_sort_paths(dev, itype):
     Ash = tuple(x[pA] for x in A)
     dAs = from_arrays(G.GenericSparseMatrixCOO, m, n, *Ash, dev, itype)
     same(coo_tuple(dAs + dB), O.coo_add(m, Ash, B))  # sort path
+    # row-major operands (a COO made from a CSR, duplicates kept in storage order):
+    # sorted by column alone, then merged
+    oA = np.lexsort((A[1], A[0]))
+    Ar = tuple(x[oA] for x in A)
+    oB = np.lexsort((B[1], B[0]))
+    Br = tuple(x[oB] for x in B)
+    dAr = from_arrays(G.GenericSparseMatrixCOO, m, n, *Ar, dev, itype)
+    dBr = from_arrays(G.GenericSparseMatrixCOO, m, n, *Br, dev, itype)
+    same(coo_tuple(dAr + dBr), O.coo_add(m, Ar, Br))
+    same(coo_tuple(dAr + dB), O.coo_add(m, Ar, B))
+    same(coo_tuple(dA + dBr), O.coo_add(m, A, Br))
     E = from_arrays(G.GenericSparseMatrixCOO, m, n, np.zeros(0, np.int64), np.zeros(0, np.int64),
                     np.zeros(0), dev, itype)
     same(coo_tuple(E + dB), O.coo_add(m, (np.zeros(0), np.zeros(0), np.zeros(0)), B))
