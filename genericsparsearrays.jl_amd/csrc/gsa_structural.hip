@@ -598,7 +598,11 @@ __device__ __forceinline__ int64_t sg_products(const int64_t* __restrict__ pp, i
 // Rows with at most G products (and more than G/2, or any for the smallest
 // group): groups of G lanes, 64/G rows per wavefront.  Scans, the bitonic sort
 // and the run sums stay inside a group.
-template <bool NUM, int G, typename T, typename I>
+// STAGE (with NUM): one pass that also counts.  Row r's distinct columns and
+// sums go to a staging area at its product offset pp[r] - 1 (the row's output
+// is never longer than its products), and rowcnt[r + 1] gets their number;
+// k_sg_compact later moves them to their final place.
+template <bool NUM, int G, typename T, typename I, bool STAGE = false>
 __global__ __launch_bounds__(64 * SG_WAVES) void k_sg_row_wave(
     int64_t m, int64_t n, int64_t kdim, int64_t nnzA, int64_t nnzB, const I* __restrict__ pA,
     const I* __restrict__ cA, const T* __restrict__ vA, const I* __restrict__ pB,
@@ -677,9 +681,9 @@ __global__ __launch_bounds__(64 * SG_WAVES) void k_sg_row_wave(
     const bool head = valid && (gl == 0 || pcol != col);
     const uint64_t gmask = (G == 64) ? ~0ull : (((1ull << G) - 1) << gbase);
     const uint64_t heads = __ballot(head) & gmask;
-    if (!NUM) {
+    if (!NUM || STAGE) {
         if (mine && gl == 0) rowcnt[r + 1] = (I)__popcll(heads);
-        return;
+        if (!NUM) return;
     }
     s_val[wv][lane] = val;
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -691,25 +695,45 @@ __global__ __launch_bounds__(64 * SG_WAVES) void k_sg_row_wave(
         T acc = T(0);
         for (int j = lane; j < stop; ++j) acc = acc + s_val[wv][j];
         const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
-        const int64_t o = (int64_t)rowptrC[r] - 1 + __popcll(heads & lt);
+        const int64_t o = (STAGE ? pp[r] : (int64_t)rowptrC[r]) - 1 + __popcll(heads & lt);
         cC[o] = (I)(col + 1);
         vC[o] = acc;
     }
 }
 
+// Staged rows (k_sg_row_wave with STAGE) to their final place: row r's entries
+// from offset pp[r] - 1 to rowptrC[r] - 1, 16 lanes per row.
+template <typename T, typename I>
+__global__ void k_sg_compact(int64_t m, const int64_t* __restrict__ pp,
+                             const I* __restrict__ rowptrC, const I* __restrict__ sc,
+                             const T* __restrict__ sv, I* __restrict__ cC, T* __restrict__ vC) {
+    const int64_t nthreads = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < m * 16; t += nthreads) {
+        const int64_t r = t >> 4;
+        const int gl = (int)(t & 15);
+        const int64_t src = pp[r] - 1;
+        const int64_t dst = (int64_t)rowptrC[r] - 1;
+        const int64_t cnt = (int64_t)rowptrC[r + 1] - 1 - dst;
+        for (int64_t j = gl; j < cnt; j += 16) {
+            cC[dst + j] = sc[src + j];
+            vC[dst + j] = sv[src + j];
+        }
+    }
+}
+
 // Launch the grouped row kernels for the size classes (0,16], (16,32], (32,64].
-template <bool NUM, typename T, typename I>
+template <bool NUM, typename T, typename I, bool STAGE = false>
 static void sg_rows_small(hipStream_t s, int64_t m, int64_t n, int64_t kdim, int64_t nnzA,
                           int64_t nnzB, const I* pA, const I* cA, const T* vA, const I* pB,
                           const I* cB, const T* vB, const int64_t* pp, I* rowcnt,
                           const I* rowptrC, I* cC, T* vC) {
-    hipLaunchKernelGGL((k_sg_row_wave<NUM, 16, T, I>), dim3((unsigned)cdiv(m, SG_WAVES * 4)),
+    hipLaunchKernelGGL((k_sg_row_wave<NUM, 16, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES * 4)),
                        dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
                        pp, 0, rowcnt, rowptrC, cC, vC);
-    hipLaunchKernelGGL((k_sg_row_wave<NUM, 32, T, I>), dim3((unsigned)cdiv(m, SG_WAVES * 2)),
+    hipLaunchKernelGGL((k_sg_row_wave<NUM, 32, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES * 2)),
                        dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
                        pp, 16, rowcnt, rowptrC, cC, vC);
-    hipLaunchKernelGGL((k_sg_row_wave<NUM, 64, T, I>), dim3((unsigned)cdiv(m, SG_WAVES)),
+    hipLaunchKernelGGL((k_sg_row_wave<NUM, 64, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES)),
                        dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
                        pp, 32, rowcnt, rowptrC, cC, vC);
 }
@@ -1274,6 +1298,27 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
                 c->in[0] = rowptrA; c->in[1] = colA; c->in[2] = valA;
                 c->in[3] = rowptrB; c->in[4] = colB; c->in[5] = valB;
                 I* cnt = (I*)c->buf[0];
+                // every row within one wavefront's sort and a staging area of modest
+                // size: one pass sums and counts, the finish only moves the rows
+                const size_t stage = (size_t)P * (sizeof(I) + sizeof(T));
+                if (nbig == 0 && stage <= ((size_t)4 << 30)) {
+                    if ((r = ws_alloc(&c->buf[5], sizeof(I) * P, s)) != GSA_OK) return fail(r);
+                    if ((r = ws_alloc(&c->buf[6], sizeof(T) * P, s)) != GSA_OK) return fail(r);
+                    sg_rows_small<true, T, I, true>(s, m, n, kdim, nnzA, nnzB, (const I*)rowptrA,
+                                                    (const I*)colA, (const T*)valA,
+                                                    (const I*)rowptrB, (const I*)colB,
+                                                    (const T*)valB, (const int64_t*)c->buf[3],
+                                                    cnt, (const I*)nullptr, (I*)c->buf[5],
+                                                    (T*)c->buf[6]);
+                    if ((e = hipGetLastError()) != hipSuccess) return fail(hip_fail(e, "k_sg_row"));
+                    c->aux[1] = 2;  // staged
+                    int64_t total = 0;
+                    if ((r = counts_to_ptr<I>(s, cnt, m, &total)) != GSA_OK) return fail(r);
+                    c->nnz_out = total;
+                    *nnzC = total;
+                    *ctx = c;
+                    return (int)GSA_OK;
+                }
                 sg_rows_small<false, T, I>(s, m, n, kdim, nnzA, nnzB, (const I*)rowptrA,
                                            (const I*)colA, (const T*)valA, (const I*)rowptrB,
                                            (const I*)colB, (const T*)valB,
@@ -1344,6 +1389,15 @@ int gsa_spgemm_csr_finish(gsa_ctx ctx, void* rowptrC, void* colC, void* valC) {
         using I = decltype(ti);
         const int64_t m = ctx->m;
         GSA_HIP(hipMemcpyAsync(rowptrC, ctx->buf[0], sizeof(I) * (m + 1), hipMemcpyDeviceToDevice, s));
+        if (ctx->aux[1] == 2) {  // staged: move the rows to their place
+            if (ctx->nnz_out > 0)
+                hipLaunchKernelGGL((k_sg_compact<T, I>), dim3(grid_for(m * 16, 256)), dim3(256), 0,
+                                   s, m, (const int64_t*)ctx->buf[3], (const I*)rowptrC,
+                                   (const I*)ctx->buf[5], (const T*)ctx->buf[6], (I*)colC,
+                                   (T*)valC);
+            GSA_HIP(hipGetLastError());
+            return (int)GSA_OK;
+        }
         if (ctx->aux[1] == 1) {  // row-local: the pointer is final; write the rows
             if (ctx->nnz_out > 0) {
                 const int64_t n = ctx->n, kdim = ctx->aux[3], nnzA = ctx->aux[4], nnzB = ctx->aux[5];
