@@ -701,8 +701,8 @@ __global__ __launch_bounds__(64 * SG_WAVES) void k_sg_row_wave(
     }
 }
 
-// Staged rows (k_sg_row_wave with STAGE) to their final place: row r's entries
-// from offset pp[r] - 1 to rowptrC[r] - 1, 16 lanes per row.
+// Staged rows (k_sg_row_wave / k_sg_row_block with STAGE) to their final place:
+// row r's entries from offset pp[r] - 1 to rowptrC[r] - 1, 16 lanes per row.
 template <typename T, typename I>
 __global__ void k_sg_compact(int64_t m, const int64_t* __restrict__ pp,
                              const I* __restrict__ rowptrC, const I* __restrict__ sc,
@@ -738,7 +738,7 @@ static void sg_rows_small(hipStream_t s, int64_t m, int64_t n, int64_t kdim, int
                        pp, 32, rowcnt, rowptrC, cC, vC);
 }
 
-template <bool NUM, typename T, typename I>
+template <bool NUM, typename T, typename I, bool STAGE = false>
 __global__ __launch_bounds__(ROW_WG) void k_sg_row_block(
     int64_t n, int64_t kdim, int64_t nnzA, int64_t nnzB, const I* __restrict__ pA,
     const I* __restrict__ cA, const T* __restrict__ vA, const I* __restrict__ pB,
@@ -806,11 +806,11 @@ __global__ __launch_bounds__(ROW_WG) void k_sg_row_block(
     }
     int64_t total = 0;
     int64_t rank = wg_exclusive_sum<int64_t>(mine, lds4, &total);
-    if (!NUM) {
+    if (!NUM || STAGE) {
         if (tid == 0) rowcnt[r + 1] = (I)total;
-        return;
+        if (!NUM) return;
     }
-    const int64_t off = (int64_t)rowptrC[r] - 1;
+    const int64_t off = (STAGE ? pp[r] : (int64_t)rowptrC[r]) - 1;
     for (int q = 0; q < per; ++q) {
         const int i = tid * per + q;
         if (i >= P) break;
@@ -1298,10 +1298,10 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
                 c->in[0] = rowptrA; c->in[1] = colA; c->in[2] = valA;
                 c->in[3] = rowptrB; c->in[4] = colB; c->in[5] = valB;
                 I* cnt = (I*)c->buf[0];
-                // every row within one wavefront's sort and a staging area of modest
-                // size: one pass sums and counts, the finish only moves the rows
+                // a staging area of modest size: one pass sorts, sums and counts every
+                // row; the finish only moves the rows
                 const size_t stage = (size_t)P * (sizeof(I) + sizeof(T));
-                if (nbig == 0 && stage <= ((size_t)4 << 30)) {
+                if (stage <= ((size_t)4 << 30)) {
                     if ((r = ws_alloc(&c->buf[5], sizeof(I) * P, s)) != GSA_OK) return fail(r);
                     if ((r = ws_alloc(&c->buf[6], sizeof(T) * P, s)) != GSA_OK) return fail(r);
                     sg_rows_small<true, T, I, true>(s, m, n, kdim, nnzA, nnzB, (const I*)rowptrA,
@@ -1310,6 +1310,13 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
                                                     (const T*)valB, (const int64_t*)c->buf[3],
                                                     cnt, (const I*)nullptr, (I*)c->buf[5],
                                                     (T*)c->buf[6]);
+                    if (nbig > 0)
+                        hipLaunchKernelGGL((k_sg_row_block<true, T, I, true>), dim3((unsigned)nbig),
+                                           dim3(ROW_WG), 0, s, n, kdim, nnzA, nnzB,
+                                           (const I*)rowptrA, (const I*)colA, (const T*)valA,
+                                           (const I*)rowptrB, (const I*)colB, (const T*)valB,
+                                           (const int64_t*)c->buf[3], (const int64_t*)c->buf[4],
+                                           cnt, (const I*)nullptr, (I*)c->buf[5], (T*)c->buf[6]);
                     if ((e = hipGetLastError()) != hipSuccess) return fail(hip_fail(e, "k_sg_row"));
                     c->aux[1] = 2;  // staged
                     int64_t total = 0;
