@@ -121,3 +121,35 @@ def test_synthetic_generators_shapes():
     assert L.nnz == 5 * 100 - 4 * 10
     P = synth.powerlaw_csr(20000, 20000, seed=3)
     assert 1 <= np.diff(P.rowptr).min() and np.diff(P.rowptr).max() <= 5000
+
+
+def test_uniform_scaling_and_diagonal_host_rules():
+    """UniformScaling's element type joins A's as promote_type does; the checks of
+    tr, dot, issymmetric and A + λI run on the host before any device work."""
+    import numpy as np
+    import gsa_hip as G
+    assert G.I.dtype is None and G.UniformScaling(2.5).dtype == torch.float64
+    assert (np.float32(0.5) * G.I).dtype == torch.float32
+    assert (2 * G.I).dtype is None and (2 * G.I).value == 2.0
+    assert (-(3.0 * G.I)).value == -3.0
+    R = G.GenericSparseMatrixCSR(2, 3, torch.tensor([1, 2, 2]), torch.tensor([1]),
+                                 torch.tensor([1.0], dtype=torch.float64))
+    with pytest.raises(G.DimensionMismatch, match="square to compute the trace"):
+        G.tr(R)
+    with pytest.raises(G.DimensionMismatch, match="square to add UniformScaling"):
+        R + G.I
+    assert G.issymmetric(R) is False and G.ishermitian(G.transpose(R)) is False
+    with pytest.raises(G.DimensionMismatch, match="first dimension of A, 2, does not match the "
+                                                 "length of x, 3"):
+        G.dot(torch.zeros(3, dtype=torch.float64), R, torch.zeros(3, dtype=torch.float64))
+    with pytest.raises(G.DimensionMismatch, match="second dimension of A, 3"):
+        G.dot(torch.zeros(2, dtype=torch.float64), R, torch.zeros(2, dtype=torch.float64))
+    D = G.Diagonal(torch.ones(4, dtype=torch.float64))
+    F = G.Diagonal.fill(2.0, 5, dtype=torch.float32, device="cpu")
+    assert D.shape == (4, 4) and F.shape == (5, 5) and F.dtype == torch.float32
+    assert np.array_equal(F.collect(), 2.0 * np.eye(5))
+    with pytest.raises(G.ArgumentError):
+        G.kron(D, F)
+    with pytest.raises(G.ArgumentError, match="no CPU path|HIP kernel layer needs device"):
+        G.tr(G.GenericSparseMatrixCSR(2, 2, torch.tensor([1, 2, 3]), torch.tensor([1, 2]),
+                                      torch.tensor([1.0, 2.0], dtype=torch.float64)))
