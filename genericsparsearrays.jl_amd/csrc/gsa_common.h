@@ -194,6 +194,11 @@ struct gsa_plan_s {
     gsa::RowView* view = nullptr;    // row view (CSC: beside the column blocks)
     gsa::BinPlan* bins = nullptr;    // row bins of a CSC or COO matrix (scatter-shaped SpMV)
     gsa::BinPlan* bins_t = nullptr;  // bins of the transposed product of a CSR or COO matrix
+    // Gather views of scatter-shaped products with several right-hand sides: the
+    // output rows of A*B (CSC, COO) or transpose(A)*B (CSR, COO) in row order, with
+    // each entry's B row and storage position (gsa_plan_add_gather_view).
+    gsa::RowView* gview = nullptr;
+    gsa::RowView* gview_t = nullptr;
 };
 
 struct gsa_ctx_s {

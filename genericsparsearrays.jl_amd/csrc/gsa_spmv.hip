@@ -449,10 +449,11 @@ constexpr int SPMM_MAX_LONG = SPMV_CAP / (SPMM_LONG + 1) + 1;
 constexpr int SPMM_RP = 1024;  // row offsets staged in LDS per workgroup
 
 // Dense operand read as BT[col*ldbt + kk] (row-major).  LPR lanes per row.
-template <typename T, typename I, typename CI, int LPR, typename V = T>
+template <typename T, typename I, typename CI, int LPR, typename V = T, typename PI = void>
 __global__ __launch_bounds__(WG) void k_csr_spmm(
     int64_t m, int64_t n, int64_t k, int64_t nnz, const I* __restrict__ rowptr,
-    const I* __restrict__ colval, const V* __restrict__ nzval, const float* __restrict__ rscale,
+    const I* __restrict__ colval, const V* __restrict__ nzval, const PI* __restrict__ perm,
+    const float* __restrict__ rscale,
     const T* __restrict__ BT,
     int64_t ldbt, T* __restrict__ C, int64_t ldc, int64_t ldk, T alpha, T beta, int mode,
     const int64_t* __restrict__ blk_row, const int64_t* __restrict__ blk_pos) {
@@ -490,7 +491,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             int i = base + u * WG + tid;
             i = i < cnt ? i : cnt - 1;
             ci[u] = ldnt(colval + P0 + i);
-            vv[u] = widen<T>(ldnt(nzval + P0 + i));
+            vv[u] = widen<T>(load_val(nzval, perm, P0 + i));
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -601,7 +602,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                 __syncthreads();  // previous users of s_col/s_val are done
                 for (int i = tid; i < cn; i += WG) {
                     s_col[i] = (CI)clamp64((int64_t)ldnt(colval + c0 + i) - 1, 0, n - 1);
-                    s_val[i] = widen<T>(ldnt(nzval + c0 + i));
+                    s_val[i] = widen<T>(load_val(nzval, perm, c0 + i));
                 }
                 __syncthreads();
                 if (kk < k) {
@@ -876,7 +877,6 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
     }
     GSA_CHECK(!c_rowmajor || k > 1 || ldc == 1, GSA_ERR_INVALID,
               "a row-major C with one column needs ldc = 1");
-    GSA_CHECK(perm == nullptr || k == 1, GSA_ERR_INVALID, "row views serve SpMV only");
     if constexpr (!std::is_void_v<PI>)  // values are read through perm: it must exist
         GSA_CHECK(perm != nullptr || nnz == 0, GSA_ERR_INVALID, "value permutation is null");
     PlanUse pu(s);
@@ -924,13 +924,13 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
 #define GSA_SPMM_LAUNCH(L)                                                                  \
     do {                                                                                    \
         if (n <= INT32_MAX)                                                                 \
-            hipLaunchKernelGGL((k_csr_spmm<T, I, int32_t, L, V>), dim3(grid), dim3(WG), 0, s, m, n, k, \
-                               nnz, rowptr, colval, nzval, rscale, BT, ldbt, C, c_ld, c_lk, \
-                               (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos);            \
+            hipLaunchKernelGGL((k_csr_spmm<T, I, int32_t, L, V, PI>), dim3(grid), dim3(WG), 0, s, m, \
+                               n, k, nnz, rowptr, colval, nzval, perm, rscale, BT, ldbt, C, \
+                               c_ld, c_lk, (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos); \
         else                                                                                \
-            hipLaunchKernelGGL((k_csr_spmm<T, I, I, L, V>), dim3(grid), dim3(WG), 0, s, m, n, k, nnz, \
-                               rowptr, colval, nzval, rscale, BT, ldbt, C, c_ld, c_lk,      \
-                               (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos);            \
+            hipLaunchKernelGGL((k_csr_spmm<T, I, I, L, V, PI>), dim3(grid), dim3(WG), 0, s, m, n, k, \
+                               nnz, rowptr, colval, nzval, perm, rscale, BT, ldbt, C, c_ld,  \
+                               c_lk, (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos);       \
     } while (0)
     if (k <= 2) GSA_SPMM_LAUNCH(2);
     else if (k <= 4) GSA_SPMM_LAUNCH(4);
@@ -968,18 +968,28 @@ static int scatter_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* 
 template <typename T, typename I>
 static int view_spmv(hipStream_t s, const RowView& v, int64_t n, const I* colind, const T* nzval,
                      const T* B, int64_t ldb, int opB, T* C, int64_t ldc, double alpha,
-                     double beta, int itype) {
+                     double beta, int itype, int64_t k = 1) {
     const I* col = v.col ? (const I*)v.col : colind;
     if (!v.perm)
-        return gather_csr<T, I>(s, v.m, n, 1, (const I*)v.rowptr, col, nzval, v.nnz, B, ldb, opB,
+        return gather_csr<T, I>(s, v.m, n, k, (const I*)v.rowptr, col, nzval, v.nnz, B, ldb, opB,
                                 C, ldc, alpha, beta, v.win, itype);
     if (v.perm64)
-        return gather_csr<T, I, T, uint64_t>(s, v.m, n, 1, (const I*)v.rowptr, col, nzval, v.nnz,
+        return gather_csr<T, I, T, uint64_t>(s, v.m, n, k, (const I*)v.rowptr, col, nzval, v.nnz,
                                              B, ldb, opB, C, ldc, alpha, beta, v.win, itype,
                                              nullptr, (const uint64_t*)v.perm);
-    return gather_csr<T, I, T, uint32_t>(s, v.m, n, 1, (const I*)v.rowptr, col, nzval, v.nnz, B,
+    return gather_csr<T, I, T, uint32_t>(s, v.m, n, k, (const I*)v.rowptr, col, nzval, v.nnz, B,
                                          ldb, opB, C, ldc, alpha, beta, v.win, itype, nullptr,
                                          (const uint32_t*)v.perm);
+}
+
+// A plan's gather view (the output rows of a scatter-shaped product, with each
+// entry's B row and storage position) when it was built on these arrays.
+static const RowView* gview_for(const RowView* v, int itype, int64_t rows, int64_t nnz,
+                                const void* a, const void* b) {
+    if (!v || v->itype != itype || v->m != rows || v->nnz != nnz || v->src[0] != a ||
+        v->src[1] != b)
+        return nullptr;
+    return v;
 }
 
 // The plan's row view when it was built on exactly these arrays.
@@ -1227,6 +1237,56 @@ int gsa_plan_add_transpose(void* stream, gsa_plan plan, int format, int itype, i
     return GSA_OK;
 }
 
+int gsa_plan_add_gather_view(void* stream, gsa_plan plan, int format, int trans, int itype,
+                             int64_t m, int64_t n, int64_t nnz, const void* a, const void* b) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(plan && (format == GSA_FMT_CSR || format == GSA_FMT_CSC || format == GSA_FMT_COO) &&
+                  m >= 0 && n >= 0 && nnz >= 0 && a && (nnz == 0 || b),
+              GSA_ERR_INVALID, "gsa_plan_add_gather_view: bad arguments");
+    // CSR serves transpose(A)*B, CSC serves A*B; COO either
+    GSA_CHECK(format == GSA_FMT_COO || (format == GSA_FMT_CSR) == (trans != 0), GSA_ERR_INVALID,
+              "gsa_plan_add_gather_view: a CSR view is for transpose(A)*B, a CSC one for A*B");
+    const int64_t rows = trans ? n : m;  // output rows
+    RowView*& slot = trans ? plan->gview_t : plan->gview;
+    if (gview_for(slot, itype, rows, nnz, a, b)) return GSA_OK;
+    if (slot) {
+        GSA_HIP(hipDeviceSynchronize());  // queued products may still read the old view
+        release_row_view(slot);
+        slot = nullptr;
+    }
+    hipStream_t s = as_stream(stream);
+    return dispatch_i(itype, [&](auto ti) -> int {
+        using I = decltype(ti);
+        // each entry's output row and B row
+        const void* out_row = nullptr;
+        const void* b_row = nullptr;
+        Scratch expanded(s);
+        if (format == GSA_FMT_COO) {
+            out_row = trans ? b : a;
+            b_row = trans ? a : b;
+        } else {  // the compressed index is the output row; the other comes from the pointer
+            if (nnz > 0) {
+                GSA_TRY(expanded.alloc(sizeof(I) * nnz));
+                GSA_TRY(expand_rows(s, itype, format == GSA_FMT_CSR ? m : n, nnz, a, expanded.p));
+            }
+            out_row = b;
+            b_row = expanded.p;
+        }
+        auto* v = new RowView();
+        int r = build_row_view(s, itype, rows, nnz, out_row, b_row, true, *v);
+        if (r == GSA_OK) r = gsa_plan_create(stream, itype, rows, nnz, v->rowptr, &v->win);
+        if (r != GSA_OK) {
+            release_row_view(v);
+            return r;
+        }
+        v->src[0] = a;
+        v->src[1] = b;
+        slot = v;
+        GSA_HIP(hipStreamSynchronize(s));  // the expanded rows are released on return
+        return GSA_OK;
+    });
+}
+
 int gsa_plan_describe(gsa_plan plan, int* flags) {
     GSA_CHECK(plan && flags, GSA_ERR_INVALID, "gsa_plan_describe: null argument");
     int f = 0;
@@ -1235,6 +1295,7 @@ int gsa_plan_describe(gsa_plan plan, int* flags) {
     if (plan->view) f |= GSA_PLAN_ROW_VIEW;
     if (plan->bins) f |= GSA_PLAN_ROW_BINS;
     if (plan->bins_t) f |= GSA_PLAN_TRANSPOSE_BINS;
+    if (plan->gview || plan->gview_t) f |= GSA_PLAN_GATHER_VIEW;
     if (plan->kind == 1) f |= GSA_PLAN_BATCH;
     *flags = f;
     return GSA_OK;
@@ -1243,10 +1304,13 @@ int gsa_plan_describe(gsa_plan plan, int* flags) {
 int gsa_plan_destroy(gsa_plan plan) {
     if (!plan) return GSA_OK;
     // the plan may still be in use by queued work on any stream
-    if (plan->kind == 0 || plan->view || plan->bins || plan->bins_t)
+    if (plan->kind == 0 || plan->view || plan->bins || plan->bins_t || plan->gview ||
+        plan->gview_t)
         GSA_HIP(hipDeviceSynchronize());
     if (plan->kind == 0) GSA_TRY(plan_release(plan->csr));
     if (plan->view) release_row_view(plan->view);
+    if (plan->gview) release_row_view(plan->gview);
+    if (plan->gview_t) release_row_view(plan->gview_t);
     if (plan->bins) release_bin_plan(plan->bins);
     if (plan->bins_t) release_bin_plan(plan->bins_t);
     if (plan->kind == 1) batch_plan_release(plan);
@@ -1272,6 +1336,12 @@ int gsa_spmm_csr(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
             bin_plan_matches(plan->bins_t, itype, n, m, nnz, rowptr, colval))
             return bin_spmv(s, *plan->bins_t, vtype, nullptr, nzval, B,
                             opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
+        if (k > 1 && plan) {
+            if (const RowView* v = gview_for(plan->gview_t, itype, n, nnz, rowptr, colval))
+                return view_spmv<T, I>(s, *v, m, (const I*)nullptr, (const T*)nzval,
+                                       (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha,
+                                       beta, itype, k);
+        }
         return scatter_csr<T, I>(s, m, n, k, (const I*)rowptr, (const I*)colval, (const T*)nzval,
                                  nnz, (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
                                  plan, itype);
@@ -1298,6 +1368,12 @@ int gsa_spmm_csc(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
             if (v)  // sorted-key form: rows gathered through the plan's row view
                 return view_spmv<T, I>(s, *v, n, (const I*)nullptr, (const T*)nzval, (const T*)B,
                                        ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta, itype);
+            if (k > 1 && plan) {
+                if (const RowView* g = gview_for(plan->gview, itype, m, nnz, colptr, rowval))
+                    return view_spmv<T, I>(s, *g, n, (const I*)nullptr, (const T*)nzval,
+                                           (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha,
+                                           beta, itype, k);
+            }
             return scatter_csr<T, I>(s, n, m, k, (const I*)colptr, (const I*)rowval,
                                      (const T*)nzval, nnz, (const T*)B, ldb, opB != GSA_OP_N,
                                      (T*)C, ldc, alpha, beta, plan, itype);
@@ -1380,6 +1456,13 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
             bin_plan_matches(plan->bins_t, itype, n, m, nnz, rowind, colind))
             return bin_spmv(s, *plan->bins_t, vtype, rowind, nzval, B,
                             opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
+        if (k > 1 && plan) {
+            if (const RowView* g = gview_for(trans ? plan->gview_t : plan->gview, itype, rows_out,
+                                             nnz, rowind, colind))
+                return view_spmv<T, I>(s, *g, cols_in, (const I*)nullptr, (const T*)nzval,
+                                       (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
+                                       itype, k);
+        }
         GSA_TRY(apply_beta<T>(s, rows_out, k, (T*)C, ldc, beta));
         if (nnz == 0 || cols_in == 0) return (int)GSA_OK;
         hipLaunchKernelGGL((k_coo_scatter<T, I>), dim3(grid_for(nnz * k, 256)), dim3(256), 0, s,

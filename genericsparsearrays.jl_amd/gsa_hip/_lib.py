@@ -23,8 +23,8 @@ GSA_F64, GSA_F32, GSA_BF16 = 0, 1, 2
 GSA_I32, GSA_I64 = 0, 1
 GSA_OP_N, GSA_OP_T, GSA_OP_C = 0, 1, 2
 GSA_PLAN_WINDOWS, GSA_PLAN_XCD_ORDER, GSA_PLAN_ROW_VIEW, GSA_PLAN_ROW_BINS, GSA_PLAN_BATCH = 1, 2, 4, 8, 16
-GSA_PLAN_TRANSPOSE_BINS = 32
-GSA_FMT_CSR, GSA_FMT_COO = 0, 2
+GSA_PLAN_TRANSPOSE_BINS, GSA_PLAN_GATHER_VIEW = 32, 64
+GSA_FMT_CSR, GSA_FMT_CSC, GSA_FMT_COO = 0, 1, 2
 GSA_BATCH_B_ROWMAJOR, GSA_BATCH_C_ROWMAJOR = 1, 2
 
 
@@ -74,6 +74,7 @@ _SIGS = {
     "gsa_plan_set_columns": [_vp, _vp, _vp, _i64],
     "gsa_plan_describe": [_vp, ctypes.POINTER(ctypes.c_int)],
     "gsa_plan_add_transpose": [_vp, _vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp],
+    "gsa_plan_add_gather_view": [_vp, _vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp],
     "gsa_spmm_csr": [_vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,
                      _vp, _i64, _i32, _vp, _i64, _f64, _f64, _vp],
     "gsa_spmm_csc": [_vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,

@@ -197,6 +197,7 @@ class _PlanCache:
         self.handle = None
         self.key = None
         self.transposed = False  # the handle holds the transposed product's bins
+        self.gathers = set()     # gather views it holds: 0 = A*B, 1 = transpose(A)*B
 
     def get(self, ptr_t: torch.Tensor, m: int, nnz: int, col_t: torch.Tensor | None = None,
             n: int = 0):
@@ -244,6 +245,16 @@ class _PlanCache:
             raise
         self.handle, self.key = h, key
         self.transposed = False
+        self.gathers = set()
+        return h
+
+    def add_gather(self, h, fmt, trans, itype, m, n, nnz, a, b):
+        """Give plan `h` the gather view of A*B (trans 0) or transpose(A)*B (trans 1)
+        for several right-hand sides (built once per plan)."""
+        if trans not in self.gathers:
+            L.call("gsa_plan_add_gather_view", stream_of(a.device), h, fmt, int(trans), itype, m,
+                   n, nnz, ptr(a), ptr(b))
+            self.gathers.add(trans)
         return h
 
     def add_transpose(self, h, fmt, itype, m, n, nnz, a, b):
@@ -317,6 +328,11 @@ class GenericSparseMatrixCSR(AbstractGenericSparseMatrix):
         return self._plan.add_transpose(self.plan(), L.GSA_FMT_CSR, itype_of(self.rowptr), self.m,
                                         self.n, self.nnz(), self.rowptr, self.colval)
 
+    def plan_gather(self, trans: bool = True):
+        """plan() with the gather view of transpose(A) * B (several right-hand sides)."""
+        return self._plan.add_gather(self.plan(), L.GSA_FMT_CSR, 1, itype_of(self.rowptr), self.m,
+                                     self.n, self.nnz(), self.rowptr, self.colval)
+
     def copy(self):
         return GenericSparseMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval)
 
@@ -376,6 +392,11 @@ class GenericSparseMatrixCSC(AbstractGenericSparseMatrix):
         """plan() if it is already built for the current arrays, else None."""
         return self._plan.peek(("csc",), (self.colptr, self.rowval), self.m, self.n, self.nnz())
 
+    def plan_gather(self, trans: bool = False):
+        """plan() with the gather view of A * B (several right-hand sides)."""
+        return self._plan.add_gather(self.plan(), L.GSA_FMT_CSC, 0, itype_of(self.colptr), self.m,
+                                     self.n, self.nnz(), self.colptr, self.rowval)
+
     def copy(self):
         return GenericSparseMatrixCSC(self.m, self.n, self.colptr, self.rowval, self.nzval)
 
@@ -431,6 +452,12 @@ class GenericSparseMatrixCOO(AbstractGenericSparseMatrix):
         h = self.plan()  # creates the cache on first use
         return self._plan.add_transpose(h, L.GSA_FMT_COO, itype_of(self.rowind), self.m, self.n,
                                         self.nnz(), self.rowind, self.colind)
+
+    def plan_gather(self, trans: bool = False):
+        """plan() with the gather view of A * B or transpose(A) * B (several right-hand sides)."""
+        h = self.plan()
+        return self._plan.add_gather(h, L.GSA_FMT_COO, 1 if trans else 0, itype_of(self.rowind),
+                                     self.m, self.n, self.nnz(), self.rowind, self.colind)
 
     def getrowind(self):
         return self.rowind

@@ -96,23 +96,32 @@ def mul_(C: torch.Tensor, A, B: torch.Tensor, alpha=1.0, beta=0.0):
     if isinstance(Ap, GenericSparseMatrixCSR):
         it = itype_of(Ap.rowptr)
         plan = None
-        if Ap.nnz() > 0:  # transpose(A) * x: the plan's bins replace the atomic scatter
-            plan = Ap.plan_transposed() if (opA != L.GSA_OP_N and k == 1) else Ap.plan()
+        if Ap.nnz() > 0:  # transpose(A) * B: the plan's bins (x) or gather view (B) replace
+            if opA == L.GSA_OP_N:  # the atomic scatter
+                plan = Ap.plan()
+            else:
+                plan = Ap.plan_transposed() if k == 1 else Ap.plan_gather()
         L.call("gsa_spmm_csr", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowptr), ptr(Ap.colval),
                ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
                plan)
     elif isinstance(Ap, GenericSparseMatrixCSC):
         it = itype_of(Ap.colptr)
+        plan = None
+        if Ap.nnz() > 0:  # A * B with several columns: the gather view replaces the scatter
+            plan = Ap.plan_gather() if (opA == L.GSA_OP_N and k > 1) else Ap.plan()
         L.call("gsa_spmm_csc", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.colptr), ptr(Ap.rowval),
                ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
-               Ap.plan() if Ap.nnz() > 0 else None)
+               plan)
     else:
         it = itype_of(Ap.rowind)
         # SpMV (k = 1): the row view or bins for A * x, the transposed bins for
         # transpose(A) * x; other products scatter
         plan = None
-        if Ap.nnz() > 0 and k == 1:
-            plan = Ap.plan() if opA == L.GSA_OP_N else Ap.plan_transposed()
+        if Ap.nnz() > 0:
+            if k == 1:
+                plan = Ap.plan() if opA == L.GSA_OP_N else Ap.plan_transposed()
+            else:
+                plan = Ap.plan_gather(trans=opA != L.GSA_OP_N)
         L.call("gsa_spmm_coo", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowind), ptr(Ap.colind),
                ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
                plan)

@@ -100,9 +100,25 @@ int gsa_plan_describe(gsa_plan plan, int* flags);
  * (m x n) with n above 4,194,304 columns gets no bins (the scatter stays).
  * About 20 bytes per stored entry.  No counterpart in the reference. */
 #define GSA_FMT_CSR 0
+#define GSA_FMT_CSC 1
 #define GSA_FMT_COO 2
 int gsa_plan_add_transpose(void* stream, gsa_plan plan, int format, int itype, int64_t m,
                            int64_t n, int64_t nnz, const void* a, const void* b);
+/* Add to a plan the gather view of a scatter-shaped product with several
+ * right-hand sides (k > 1): the output rows in order, each entry's row of B and
+ * its storage position.  The product then runs the gather SpMM kernel, reading
+ * values through the positions, instead of the reference's atomic scatter
+ * (kernel_spmatmul_csc_N! src/matrix_csc/matrix_csc_kernels.jl:1-22,
+ * kernel_spmatmul_csr_T! src/matrix_csr/matrix_csr_kernels.jl:24-45,
+ * kernel_spmatmul_coo_N!/T! src/matrix_coo/matrix_coo_kernels.jl:1-43); each
+ * output is then summed in a fixed order.  format GSA_FMT_CSC with trans = 0
+ * (a = colptr, b = rowval), GSA_FMT_CSR with trans != 0 (a = rowptr,
+ * b = colval), GSA_FMT_COO with either (a = rowind, b = colind).  About 12-16
+ * bytes per stored entry; synchronises the stream.  No counterpart in the
+ * reference. */
+#define GSA_PLAN_GATHER_VIEW 64
+int gsa_plan_add_gather_view(void* stream, gsa_plan plan, int format, int trans, int itype,
+                             int64_t m, int64_t n, int64_t nnz, const void* a, const void* b);
 /* Give a plan the column array it will be used with (`colval`, n columns):
  * the SpMV windows lying inside rows longer than 2304 entries (whose
  * ascending columns span a narrow band of x) are then launched on the XCD
