@@ -689,7 +689,19 @@ def main():
         extra["spmm_k32_f64_GBs"] = round(b3 / (ms3 * 1e-3) / 1e9, 1)
         extra["spmm_k32_f64_GFLOPs"] = round(2 * H.nnz * k / (ms3 * 1e-3) / 1e9, 1)
         print(f"[bench] SpMM k=32 {ms3 * 1e3:.0f} us", file=sys.stderr, flush=True)
-        del A, B, C
+        # the scatter-shaped products (CSC * B, COO * B, transpose(CSR) * B) run
+        # the same gather kernel through a plan's gather view
+        Ct = torch.empty((k, H.n), dtype=torch.float64, device=dev).t()
+        for tag, M, Cm in (("csc", G.GenericSparseMatrixCSC(A), C),
+                           ("coo", G.GenericSparseMatrixCOO(A), C),
+                           ("csr_T", G.transpose(A), Ct)):
+            Bm = B if tag != "csr_T" else torch.randn((k, H.m), dtype=torch.float64, device=dev).t()
+            G.mul_(Cm, M, Bm, 1.0, 0.0)  # builds the gather view
+            ms5 = time_steps(lambda i: G.mul_(Cm, M, Bm, 1.0, 0.0), 10, 3, stream)
+            extra[f"spmm_k32_{tag}_f64_us"] = round(ms5 * 1e3, 1)
+            print(f"[bench] SpMM k=32 {tag} {ms5 * 1e3:.0f} us", file=sys.stderr, flush=True)
+            del M, Bm
+        del A, B, C, Ct
         torch.cuda.empty_cache()
         if not args.no_cpu:  # the oracle's SpMM (restated reference loop) on the host cores
             sys.path.insert(0, os.path.join(HERE, "oracle"))
