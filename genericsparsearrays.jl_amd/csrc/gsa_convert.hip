@@ -5,13 +5,14 @@
 // rows are written by a whole workgroup.  COO -> CSR is the reference's
 // keyed stable sort (conversions.jl:282-323): 64-bit keys (row-1)*n + (col-1)
 // — the same order as the reference's Ti keys rowind*n + colind, without their
-// Int32 overflow — radix-sorted with the entry position as payload, then the
-// index/value gathers and a row pointer from per-row counts (equal to
+// Int32 overflow — radix-sorted stably with each entry's (column, value) pair
+// carried through the passes, then a row pointer from the sorted keys (equal to
 // searchsortedfirst over the sorted rows).  CSR -> CSC is a stable sort on the
 // column alone: CSR storage is already row-ordered, so the result equals the
 // reference's CSR -> COO -> CSC route (conversions.jl:85-86) bit for bit.
 #include "gsa_scan.h"
 #include "gsa_rows.h"
+#include "gsa_radix_pairs.h"
 
 namespace gsa {
 
@@ -89,51 +90,6 @@ __global__ void k_order_flags(int64_t nnz, const I* __restrict__ major, const I*
     if (threadIdx.x == 0 && (f & ~*(volatile int*)flag)) atomicOr(flag, f);
 }
 
-// (minor index, value) of one stored entry, side by side, so that a gather by
-// a permutation touches one cache line per entry instead of two.
-template <typename T, typename I>
-struct alignas(sizeof(T) > sizeof(I) ? 2 * sizeof(T) : 2 * sizeof(I)) MinorVal {
-    I minor;
-    T val;
-};
-
-template <typename T, typename I>
-__global__ void k_pack(int64_t nnz, const I* __restrict__ minor, const T* __restrict__ val,
-                       MinorVal<T, I>* __restrict__ out) {
-    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
-         i += (int64_t)gridDim.x * blockDim.x) {
-        MinorVal<T, I> mv;
-        mv.minor = __builtin_nontemporal_load(minor + i);
-        mv.val = __builtin_nontemporal_load(val + i);
-        out[i] = mv;
-    }
-}
-
-// out_idx[i], out_val[i] = pairs[perm[i]], GATHER_U positions per thread in flight.
-template <typename T, typename I, typename V>
-__global__ void k_gather_pairs(int64_t nnz, const V* __restrict__ perm,
-                               const MinorVal<T, I>* __restrict__ pairs, I* __restrict__ out_minor,
-                               T* __restrict__ out_val) {
-    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
-    for (int64_t base = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; base < nnz;
-         base += stride * GATHER_U) {
-        MinorVal<T, I> mv[GATHER_U];
-#pragma unroll
-        for (int u = 0; u < GATHER_U; ++u) {
-            const int64_t i = base + u * stride;
-            if (i < nnz) mv[u] = pairs[(int64_t)__builtin_nontemporal_load(perm + i)];
-        }
-#pragma unroll
-        for (int u = 0; u < GATHER_U; ++u) {
-            const int64_t i = base + u * stride;
-            if (i < nnz) {
-                out_minor[i] = mv[u].minor;
-                out_val[i] = mv[u].val;
-            }
-        }
-    }
-}
-
 // One-based major pointer from the sorted keys (major = key / kdim): ptr[r]
 // is one plus the first position whose major is >= r — searchsortedfirst over
 // the sorted majors (conversions.jl:194-199).  Position i writes the pointers
@@ -196,40 +152,24 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
         : (uint64_t)(major_dim - 1);
     const int bits = bits_for(maxkey);
     const uint64_t kdim = kminor ? (uint64_t)minor_dim : 1;
-    // Entries that are not in order move far from where they were (a transposition,
-    // or storage in another order): gather (minor, value) pairs, one line per entry
-    // (two separate arrays cost two lines per entry).
-    Scratch keys(s), perm(s), pairs(s);
-    GSA_TRY(pairs.alloc(sizeof(MinorVal<T, I>) * nnz));
-    hipLaunchKernelGGL((k_pack<T, I>), dim3(g), dim3(256), 0, s, nnz, minor_out, val,
-                       pairs.as<MinorVal<T, I>>());
-    const MinorVal<T, I>* pv = pairs.as<MinorVal<T, I>>();
-    auto run = [&](auto key_t, auto pos_t, auto sorter) -> int {
+    // Entries that are not in order move far from where they were (a
+    // transposition, or storage in another order): each entry's (minor, value)
+    // pair rides through the radix passes with its key, so no pass reads by a
+    // permutation.
+    auto run = [&](auto key_t) -> int {
         using K = decltype(key_t);
-        using V = decltype(pos_t);
+        Scratch keys(s);
         GSA_TRY(keys.alloc(sizeof(K) * nnz));
-        GSA_TRY(perm.alloc(sizeof(V) * nnz));
-        hipLaunchKernelGGL((k_make_keys<I, K, V>), dim3(g), dim3(256), 0, s, nnz, major, kminor,
-                           (int64_t)kdim, major_dim, keys.as<K>(), perm.as<V>());
-        GSA_TRY(sorter(keys.as<K>(), perm.as<V>()));
-        hipLaunchKernelGGL((k_gather_pairs<T, I, V>), dim3(g), dim3(256), 0, s, nnz,
-                           (const V*)perm.as<V>(), pv, idx_out, val_out);
+        const KeyFromIdx<K, I> key0{major, kminor, major_dim, (int64_t)kdim};
+        GSA_TRY((radix_sort_payload<K, T, I>(s, nnz, bits, key0, minor_out, val, keys.as<K>(),
+                                             idx_out, val_out)));
         hipLaunchKernelGGL((k_ptr_from_keys<I, K>), dim3(grid_for(nnz + 1, 256)), dim3(256), 0, s,
                            nnz, (const K*)keys.as<K>(), kdim, major_dim, ptr_out);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
     };
-    if (bits <= 32 && nnz < (int64_t)UINT32_MAX)  // 32-bit keys and positions
-        return run(uint32_t{}, uint32_t{}, [&](uint32_t* k, uint32_t* p) {
-            return radix_sort_pairs_u32_u32(s, k, p, nnz, bits);
-        });
-    if (nnz < (int64_t)UINT32_MAX)
-        return run(uint64_t{}, uint32_t{}, [&](uint64_t* k, uint32_t* p) {
-            return radix_sort_pairs_u64_u32(s, k, p, nnz, bits);
-        });
-    return run(uint64_t{}, uint64_t{}, [&](uint64_t* k, uint64_t* p) {
-        return radix_sort_pairs_u64(s, k, p, nnz, bits);
-    });
+    if (bits <= 32) return run(uint32_t{});
+    return run(uint64_t{});
 }
 
 // ---- row views of COO / CSC storage (sorted-key SpMV) -------------------------
