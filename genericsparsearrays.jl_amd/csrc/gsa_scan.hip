@@ -1,8 +1,9 @@
 // gsa_scan.hip — device-wide inclusive prefix sums (the _cumsum_AK primitive,
 // src/helpers.jl:3) used to turn per-row counts into one-based pointers and
 // masks into write positions.  Reduce-then-scan over 4096-element tiles:
-// tile sums, one-block scan of the tile sums, then a per-tile scan with its
-// offset.  Deterministic (integer).
+// tile sums, then a per-tile scan with its offset (up to 1024 tiles each tile
+// adds up the sums before it; beyond, one block scans the tile sums first).
+// Deterministic (integer).
 #include "gsa_scan.h"
 
 namespace gsa {
@@ -65,43 +66,94 @@ __global__ __launch_bounds__(SCAN_WG) void k_scan_sums(T* __restrict__ sums, int
     }
 }
 
-// inclusive scan of each tile plus its offset; out may alias in
+// Sum of sums[0 .. count) over the whole workgroup (every thread gets it).
+template <typename T>
+__device__ __forceinline__ T block_prefix_of(const T* __restrict__ sums, int64_t count, T* lds) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T part = 0;
+    for (int64_t t = threadIdx.x; t < count; t += SCAN_WG) part += sums[t];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) part += __shfl_xor(part, off, 64);
+    if (lane == 0) lds[wave] = part;
+    __syncthreads();
+    T all = 0;
+    for (int w = 0; w < SCAN_WG / 64; ++w) all += lds[w];
+    return all;
+}
+
+// Inclusive scan of each tile plus its offset; out may alias in.  The offset is
+// offs[tile] (the scanned tile sums), or the sum of sums[0 .. tile) when offs is
+// null and sums is not (few tiles: no separate scan of the sums), or zero.
+// Each wavefront scans its 1024 consecutive elements as 16 coalesced rows of
+// 64, each row's lane scan carried into the next.
 template <typename T>
 __global__ __launch_bounds__(SCAN_WG) void k_scan_tiles(const T* in, int64_t n,
-                                                        const T* __restrict__ offs, T* out,
+                                                        const T* __restrict__ offs,
+                                                        const T* __restrict__ sums, T* out,
                                                         T add) {
     __shared__ T lds[SCAN_WG / 64];
-    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)threadIdx.x * SCAN_IPT;
+    __shared__ T lds2[SCAN_WG / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int64_t base = (int64_t)blockIdx.x * SCAN_TILE + (int64_t)wave * (64 * SCAN_IPT);
     T v[SCAN_IPT];
-    T s = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_IPT; ++i) {
-        int64_t idx = base + i;
+        const int64_t idx = base + i * 64 + lane;
         v[i] = idx < n ? in[idx] : T(0);
-        s += v[i];
     }
-    T total;
-    T ex = block_exclusive_sum(s, lds, &total);
-    T run = offs[blockIdx.x] + ex + add;
+    T carry = 0;
 #pragma unroll
     for (int i = 0; i < SCAN_IPT; ++i) {
-        int64_t idx = base + i;
-        run += v[i];
-        if (idx < n) out[idx] = run;
+        T x = v[i];
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const T y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        const T row = __shfl(x, 63, 64);
+        v[i] = x + carry;
+        carry += row;
+    }
+    if (lane == 0) lds[wave] = carry;
+    T tile_off = add;
+    if (offs) tile_off += offs[blockIdx.x];
+    else if (sums) tile_off += block_prefix_of(sums, (int64_t)blockIdx.x, lds2);
+    __syncthreads();
+    T wpre = 0;
+    for (int w = 0; w < wave; ++w) wpre += lds[w];
+    const T o = tile_off + wpre;
+#pragma unroll
+    for (int i = 0; i < SCAN_IPT; ++i) {
+        const int64_t idx = base + i * 64 + lane;
+        if (idx < n) out[idx] = v[i] + o;
     }
 }
+
+// Up to this many tiles, each tile sums the tile sums before it itself.
+constexpr int64_t SCAN_FUSE_TILES = 1024;
 
 template <typename T>
 int inclusive_scan(hipStream_t s, const T* in, T* out, int64_t n, T add) {
     if (n <= 0) return GSA_OK;
     const int64_t tiles = cdiv(n, SCAN_TILE);
+    if (tiles == 1) {
+        hipLaunchKernelGGL(k_scan_tiles<T>, dim3(1), dim3(SCAN_WG), 0, s, in, n, (const T*)nullptr,
+                           (const T*)nullptr, out, add);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    }
     Scratch sums(s);
     GSA_TRY(sums.alloc(sizeof(T) * tiles));
     hipLaunchKernelGGL(k_tile_sums<T>, dim3((unsigned)tiles), dim3(SCAN_WG), 0, s, in, n,
                        sums.as<T>());
-    hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_WG), 0, s, sums.as<T>(), tiles);
-    hipLaunchKernelGGL(k_scan_tiles<T>, dim3((unsigned)tiles), dim3(SCAN_WG), 0, s, in, n,
-                       (const T*)sums.as<T>(), out, add);
+    if (tiles <= SCAN_FUSE_TILES) {
+        hipLaunchKernelGGL(k_scan_tiles<T>, dim3((unsigned)tiles), dim3(SCAN_WG), 0, s, in, n,
+                           (const T*)nullptr, (const T*)sums.as<T>(), out, add);
+    } else {
+        hipLaunchKernelGGL(k_scan_sums<T>, dim3(1), dim3(SCAN_WG), 0, s, sums.as<T>(), tiles);
+        hipLaunchKernelGGL(k_scan_tiles<T>, dim3((unsigned)tiles), dim3(SCAN_WG), 0, s, in, n,
+                           (const T*)sums.as<T>(), (const T*)nullptr, out, add);
+    }
     GSA_HIP(hipGetLastError());
     return GSA_OK;
 }
