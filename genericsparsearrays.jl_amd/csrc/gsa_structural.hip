@@ -722,20 +722,25 @@ __global__ void k_sg_compact(int64_t m, const int64_t* __restrict__ pp,
 }
 
 // Launch the grouped row kernels for the size classes (0,16], (16,32], (32,64].
+// Launch the classes whose bit is set in `classes` (k_sg_flags): each launch
+// covers every row, so an empty class would still cost a wave per row group.
 template <bool NUM, typename T, typename I, bool STAGE = false>
 static void sg_rows_small(hipStream_t s, int64_t m, int64_t n, int64_t kdim, int64_t nnzA,
                           int64_t nnzB, const I* pA, const I* cA, const T* vA, const I* pB,
                           const I* cB, const T* vB, const int64_t* pp, I* rowcnt,
-                          const I* rowptrC, I* cC, T* vC) {
-    hipLaunchKernelGGL((k_sg_row_wave<NUM, 16, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES * 4)),
-                       dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
-                       pp, 0, rowcnt, rowptrC, cC, vC);
-    hipLaunchKernelGGL((k_sg_row_wave<NUM, 32, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES * 2)),
-                       dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
-                       pp, 16, rowcnt, rowptrC, cC, vC);
-    hipLaunchKernelGGL((k_sg_row_wave<NUM, 64, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES)),
-                       dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
-                       pp, 32, rowcnt, rowptrC, cC, vC);
+                          const I* rowptrC, I* cC, T* vC, unsigned classes) {
+    if (classes & 1u)
+        hipLaunchKernelGGL((k_sg_row_wave<NUM, 16, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES * 4)),
+                           dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB,
+                           vB, pp, 0, rowcnt, rowptrC, cC, vC);
+    if (classes & 2u)
+        hipLaunchKernelGGL((k_sg_row_wave<NUM, 32, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES * 2)),
+                           dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB,
+                           vB, pp, 16, rowcnt, rowptrC, cC, vC);
+    if (classes & 4u)
+        hipLaunchKernelGGL((k_sg_row_wave<NUM, 64, T, I, STAGE>), dim3((unsigned)cdiv(m, SG_WAVES)),
+                           dim3(64 * SG_WAVES), 0, s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB,
+                           vB, pp, 32, rowcnt, rowptrC, cC, vC);
 }
 
 template <bool NUM, typename T, typename I, bool STAGE = false>
@@ -825,19 +830,31 @@ __global__ __launch_bounds__(ROW_WG) void k_sg_row_block(
 }
 
 // Products over SG_WAVE_P per row -> the row list of the workgroup kernel.
+// out[0] gets the largest product count of a row, out[1] a bit per size class
+// of the grouped row kernels that holds a row: 1 for (0,16], 2 for (16,32],
+// 4 for (32,64].
 __global__ void k_sg_flags(int64_t m, const int64_t* __restrict__ pp, int64_t* __restrict__ f,
-                           unsigned long long* __restrict__ maxp) {
+                           unsigned long long* __restrict__ out) {
     __shared__ unsigned long long bmax;
-    if (threadIdx.x == 0) bmax = 0;
+    __shared__ unsigned int bcls;
+    if (threadIdx.x == 0) {
+        bmax = 0;
+        bcls = 0;
+    }
     __syncthreads();
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r < m) {
         const int64_t P = sg_products(pp, r);
         f[r] = P > SG_WAVE_P;
         atomicMax(&bmax, (unsigned long long)P);
+        const unsigned cls = P <= 0 ? 0u : (P <= 16 ? 1u : (P <= 32 ? 2u : (P <= 64 ? 4u : 0u)));
+        if (cls) atomicOr(&bcls, cls);
     }
     __syncthreads();
-    if (threadIdx.x == 0) atomicMax(maxp, bmax);
+    if (threadIdx.x == 0) {
+        atomicMax(out, bmax);
+        if (bcls) atomicOr(out + 1, (unsigned long long)bcls);
+    }
 }
 
 __global__ void k_sg_list(int64_t m, const int64_t* __restrict__ pp,
@@ -1264,21 +1281,25 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
         {
             Scratch fl(s), mx(s);
             if ((r = fl.alloc(sizeof(int64_t) * m)) != GSA_OK) return fail(r);
-            if ((r = mx.alloc(sizeof(unsigned long long))) != GSA_OK) return fail(r);
-            if ((r = zero_fill(s, mx.p, sizeof(unsigned long long))) != GSA_OK) return fail(r);
+            if ((r = mx.alloc(2 * sizeof(unsigned long long))) != GSA_OK) return fail(r);
+            if ((r = zero_fill(s, mx.p, 2 * sizeof(unsigned long long))) != GSA_OK) return fail(r);
             hipLaunchKernelGGL(k_sg_flags, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m,
                                (const int64_t*)pptr.as<int64_t>(), fl.as<int64_t>(),
                                (unsigned long long*)mx.p);
             if ((r = inclusive_scan<int64_t>(s, fl.as<int64_t>(), fl.as<int64_t>(), m, 0)) != GSA_OK)
                 return fail(r);
-            int64_t h[2] = {0, 0};
+            int64_t h[3] = {0, 0, 0};
             hipError_t e = hipMemcpyAsync(&h[0], mx.p, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&h[2], (const int64_t*)mx.p + 1, sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(&h[1], fl.as<int64_t>() + (m - 1), sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) return fail(hip_fail(e, "gsa_spgemm_csr_begin"));
             const int64_t maxP = h[0], nbig = h[1];
+            const unsigned classes = (unsigned)h[2];
             if (maxP <= SG_BLOCK_P && n < (int64_t)1 << 40) {
                 // keep the product pointer and the long-row list for the numeric phase
                 if ((r = ws_alloc(&c->buf[3], sizeof(int64_t) * (m + 1), s)) != GSA_OK) return fail(r);
@@ -1295,6 +1316,7 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
                 c->aux[3] = kdim;
                 c->aux[4] = nnzA;
                 c->aux[5] = nnzB;
+                c->aux[6] = classes;
                 c->in[0] = rowptrA; c->in[1] = colA; c->in[2] = valA;
                 c->in[3] = rowptrB; c->in[4] = colB; c->in[5] = valB;
                 I* cnt = (I*)c->buf[0];
@@ -1309,7 +1331,7 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
                                                     (const I*)rowptrB, (const I*)colB,
                                                     (const T*)valB, (const int64_t*)c->buf[3],
                                                     cnt, (const I*)nullptr, (I*)c->buf[5],
-                                                    (T*)c->buf[6]);
+                                                    (T*)c->buf[6], classes);
                     if (nbig > 0)
                         hipLaunchKernelGGL((k_sg_row_block<true, T, I, true>), dim3((unsigned)nbig),
                                            dim3(ROW_WG), 0, s, n, kdim, nnzA, nnzB,
@@ -1330,7 +1352,7 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
                                            (const I*)colA, (const T*)valA, (const I*)rowptrB,
                                            (const I*)colB, (const T*)valB,
                                            (const int64_t*)c->buf[3], cnt, (const I*)nullptr,
-                                           (I*)nullptr, (T*)nullptr);
+                                           (I*)nullptr, (T*)nullptr, classes);
                 if (nbig > 0)
                     hipLaunchKernelGGL((k_sg_row_block<false, T, I>), dim3((unsigned)nbig), dim3(ROW_WG),
                                        0, s, n, kdim, nnzA, nnzB, (const I*)rowptrA, (const I*)colA,
@@ -1416,7 +1438,8 @@ int gsa_spgemm_csr_finish(gsa_ctx ctx, void* rowptrC, void* colC, void* valC) {
                 const T* vB = (const T*)ctx->in[5];
                 sg_rows_small<true, T, I>(s, m, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB, vB,
                                           (const int64_t*)ctx->buf[3], (I*)nullptr,
-                                          (const I*)rowptrC, (I*)colC, (T*)valC);
+                                          (const I*)rowptrC, (I*)colC, (T*)valC,
+                                          (unsigned)ctx->aux[6]);
                 if (ctx->aux[2] > 0)
                     hipLaunchKernelGGL((k_sg_row_block<true, T, I>), dim3((unsigned)ctx->aux[2]),
                                        dim3(ROW_WG), 0, s, n, kdim, nnzA, nnzB, pA, cA, vA, pB, cB,
