@@ -309,6 +309,10 @@ def bench_reductions(H, dev):
                                         torch.as_tensor(M.colval).to(dev),
                                         torch.as_tensor(M.nzval).to(dev), copy=False)
     A, dL = dcsr(Hd), dcsr(L)
+    # the matrices' plans (row blocks and SpMV windows), built once per matrix as
+    # for mul!: tr, sum and A + lambda*I reuse its row blocks, Float64 dot its windows
+    A.plan()
+    dL.plan()
     xd, yd = torch.as_tensor(x).to(dev), torch.as_tensor(y).to(dev)
     out = {}
     for name, fn, byts in (

@@ -115,6 +115,20 @@ __global__ __launch_bounds__(ROW_WG) void k_dot3_ptr(int64_t nmaj, int64_t nidx,
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// sum of w[i] * t[i] per workgroup (double), grid-stride, fixed order.
+template <typename T>
+__global__ __launch_bounds__(ROW_WG) void k_dot2(int64_t n, const T* __restrict__ w,
+                                                 const T* __restrict__ t,
+                                                 double* __restrict__ part) {
+    __shared__ double sh[ROW_WG / 64];
+    double acc = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)ROW_WG + threadIdx.x; i < n;
+         i += (int64_t)gridDim.x * ROW_WG)
+        acc += (double)(w[i] * t[i]);
+    const double r = wg_sum_fixed(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = r;
+}
+
 // COO: x[p] * a * y[q] with (p, q) = (rowind, colind), or swapped for opA = T.
 template <typename T, typename I>
 __global__ __launch_bounds__(ROW_WG) void k_dot3_coo(int64_t m, int64_t n, int64_t nnz,
@@ -396,6 +410,27 @@ int gsa_dot3_ptr(void* stream, int vtype, int itype, int64_t nmaj, int64_t nidx,
     *out = 0.0;
     if (nmaj == 0 || nidx == 0 || nnz == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
+    // Float64 with the matrix's SpMV plan: t = the SpMV of the stored entries
+    // with the vector their index selects (the flat windows gather it), then
+    // sum of w[r] * t[r] with the vector the major index selects.  Each row's
+    // sum of a*v is rounded before its weight multiplies it, where the
+    // reference multiplies term by term; both stay within the same normalised
+    // bound, and the reference's own summation order depends on its launch.
+    if (vtype == GSA_F64 && plan && plan->kind == 0 && plan->csr.m == nmaj &&
+        plan->csr.nnz == nnz && plan->csr.itype == itype && plan->csr.ptr == ptr &&
+        plan->csr.win_row) {
+        Scratch t(s), part(s);
+        GSA_TRY(t.alloc(sizeof(double) * nmaj));
+        GSA_TRY(part.alloc(sizeof(double) * RD_PARTS));
+        const void* v = major_left ? y : x;
+        const double* w = (const double*)(major_left ? x : y);
+        GSA_TRY(gsa_spmm_csr(stream, GSA_F64, itype, GSA_OP_N, nmaj, nidx, 1, ptr, idx, val, nnz,
+                             v, nidx, GSA_OP_N, t.p, nmaj, 1.0, 0.0, plan));
+        hipLaunchKernelGGL((k_dot2<double>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, nmaj, w,
+                           (const double*)t.p, part.as<double>());
+        GSA_HIP(hipGetLastError());
+        return finish_sum(s, part.as<double>(), RD_PARTS, out);
+    }
     return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
         using T = decltype(tv);
         using I = decltype(ti);

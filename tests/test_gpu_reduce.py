@@ -141,8 +141,14 @@ def test_dot3(dev, dtype, itype):
             close(G.dot(xd, wrap(M), yd), ref, bound, dtype)
     A.plan()
     Ac.plan()
-    for M in (A, Ac):  # through the cached plans
+    for M in (A, Ac):  # through the cached plans (Float64: the SpMV windows)
         close(G.dot(xd, G.transpose(M), yd), ref, bound, dtype)
+    x2 = rng.standard_normal(m).astype(NPT[dtype])
+    y2 = rng.standard_normal(n).astype(NPT[dtype])
+    ref2, bound2 = O.dot3_ptr(m, p, i, v, x2, y2, major_left=True)
+    for M in (A, Ac):
+        close(G.dot(torch.as_tensor(x2).to(dev), M, torch.as_tensor(y2).to(dev)), ref2, bound2,
+              dtype)
 
 
 def test_dot3_promotion_and_checks(dev):
