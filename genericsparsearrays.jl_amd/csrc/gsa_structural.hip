@@ -590,16 +590,35 @@ __device__ __forceinline__ uint64_t sg_key(int64_t col, int p) {
 }
 __device__ __forceinline__ int64_t sg_col(uint64_t k) { return (int64_t)(k >> 24); }
 
-// The value held by lane (lane ^ j) of the same G-lane group.  Partners at
-// distance 1 and 2 come from the quad permutations [1,0,3,2] and [2,3,0,1],
-// distance 8 from the rotation by 8 inside each row of 16 lanes (the same lane
-// either way round): data-parallel moves in the vector ALU instead of a trip
-// through the LDS crossbar.  Other distances use the shuffle.
+// The value held by lane (lane ^ j) of the same G-lane group, by
+// data-parallel moves in the vector ALU instead of a trip through the LDS
+// crossbar (every lane of the wavefront must be active):
+//   j = 1, 2: the quad permutations [1,0,3,2] and [2,3,0,1];
+//   j = 4, 8: rotations inside each row of 16 lanes (row_ror:n gives lane i the
+//             value of lane (i - n) mod 16): by 12 where bit 2 of the lane is
+//             clear, by 4 where it is set; by 8 for every lane;
+//   j = 16, 32: the swaps of row pairs / wavefront halves (permlane16_swap,
+//             permlane32_swap of the value with itself: the first result holds
+//             the lower partner's copy, the second the upper's).
 template <int G>
 __device__ __forceinline__ uint32_t sg_xor32(uint32_t v, int j) {
+    const unsigned lane = __lane_id();
     if (j == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
     if (j == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    if (j == 4) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x12C, 0xF, 0xF, false);
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);
+        return (lane & 4u) ? dn : up;
+    }
     if (j == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+    if (j == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16u) ? (uint32_t)p[0] : (uint32_t)p[1];
+    }
+    if (j == 32) {
+        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
+        return (lane & 32u) ? (uint32_t)p[0] : (uint32_t)p[1];
+    }
     return (uint32_t)__shfl_xor((int)v, j, G);
 }
 template <int G>
