@@ -65,6 +65,47 @@ __device__ __forceinline__ int64_t xcd_tile(int64_t b, int64_t ntiles) {
 
 inline int64_t cdiv(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// The value held by lane (lane ^ j) of the wavefront, j in {1, 2, 4, 8, 16, 32},
+// by data-parallel moves in the vector ALU instead of a trip through the LDS
+// crossbar (the lanes involved must be active):
+//   j = 1, 2: the quad permutations [1,0,3,2] and [2,3,0,1];
+//   j = 4, 8: rotations inside each row of 16 lanes (row_ror:n gives lane i the
+//             value of lane (i - n) mod 16): by 12 where bit 2 of the lane is
+//             clear, by 4 where it is set; by 8 for every lane;
+//   j = 16, 32: the swaps of row pairs / wavefront halves (permlane16_swap,
+//             permlane32_swap of the value with itself: the first result holds
+//             the lower partner's copy, the second the upper's).
+// The same partners as __shfl_xor(v, j, 64), so sums built on it round alike.
+__device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int j) {
+    const unsigned lane = __lane_id();
+    if (j == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
+    if (j == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
+    if (j == 4) {
+        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x12C, 0xF, 0xF, false);
+        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);
+        return (lane & 4u) ? dn : up;
+    }
+    if (j == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
+    if (j == 16) {
+        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
+        return (lane & 16u) ? (uint32_t)p[0] : (uint32_t)p[1];
+    }
+    const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);  // j == 32
+    return (lane & 32u) ? (uint32_t)p[0] : (uint32_t)p[1];
+}
+template <typename T>
+__device__ __forceinline__ T lane_xor(T v, int j) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "lane_xor moves 32- or 64-bit values");
+    if constexpr (sizeof(T) == 8) {
+        const uint64_t u = __builtin_bit_cast(uint64_t, v);
+        const uint64_t lo = lane_xor32((uint32_t)u, j);
+        const uint64_t hi = lane_xor32((uint32_t)(u >> 32), j);
+        return __builtin_bit_cast(T, (hi << 32) | lo);
+    } else {
+        return __builtin_bit_cast(T, lane_xor32(__builtin_bit_cast(uint32_t, v), j));
+    }
+}
+
 // Grid size for grid-stride element loops: enough waves to fill 256 CUs.
 inline unsigned grid_for(int64_t n, int block) {
     int64_t g = cdiv(n > 0 ? n : 1, block);

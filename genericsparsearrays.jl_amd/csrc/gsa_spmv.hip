@@ -74,7 +74,7 @@ __device__ __forceinline__ T row_scaled(T acc, const float* __restrict__ rscale,
 template <typename T>
 __device__ __forceinline__ T wave_sum(T v) {
 #pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 64);
+    for (int off = 32; off >= 1; off >>= 1) v = v + lane_xor(v, off);
     return v;
 }
 
@@ -113,7 +113,7 @@ struct CompSum {
         if constexpr (sizeof(T) == 4) {
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) {
-                T os = __shfl_xor(s, off, 64), oc = __shfl_xor(c, off, 64);
+                T os = lane_xor(s, off), oc = lane_xor(c, off);
                 T ns, er;
                 two_sum(s, os, ns, er);
                 s = ns;
@@ -580,7 +580,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             // the four wavefronts' sums are added in order
             T v = acc;
 #pragma unroll
-            for (int o = LPR; o < 64; o <<= 1) v = v + __shfl_xor(v, o, 64);
+            for (int o = LPR; o < 64; o <<= 1) v = v + lane_xor(v, o);
             if ((tid & 63) < LPR) red[(tid >> 6) * LPR + lane] = v;
             __syncthreads();
             if (tid < LPR && kk < k) {
@@ -625,7 +625,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             __syncthreads();
             T v = acc;
 #pragma unroll
-            for (int o = LPR; o < 64; o <<= 1) v = v + __shfl_xor(v, o, 64);
+            for (int o = LPR; o < 64; o <<= 1) v = v + lane_xor(v, o);
             if ((tid & 63) < LPR) red[(tid >> 6) * LPR + lane] = v;
             __syncthreads();
             if (tid < LPR && kk < k) {

@@ -590,51 +590,6 @@ __device__ __forceinline__ uint64_t sg_key(int64_t col, int p) {
 }
 __device__ __forceinline__ int64_t sg_col(uint64_t k) { return (int64_t)(k >> 24); }
 
-// The value held by lane (lane ^ j) of the same G-lane group, by
-// data-parallel moves in the vector ALU instead of a trip through the LDS
-// crossbar (every lane of the wavefront must be active):
-//   j = 1, 2: the quad permutations [1,0,3,2] and [2,3,0,1];
-//   j = 4, 8: rotations inside each row of 16 lanes (row_ror:n gives lane i the
-//             value of lane (i - n) mod 16): by 12 where bit 2 of the lane is
-//             clear, by 4 where it is set; by 8 for every lane;
-//   j = 16, 32: the swaps of row pairs / wavefront halves (permlane16_swap,
-//             permlane32_swap of the value with itself: the first result holds
-//             the lower partner's copy, the second the upper's).
-template <int G>
-__device__ __forceinline__ uint32_t sg_xor32(uint32_t v, int j) {
-    const unsigned lane = __lane_id();
-    if (j == 1) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false);
-    if (j == 2) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false);
-    if (j == 4) {
-        const uint32_t up = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x12C, 0xF, 0xF, false);
-        const uint32_t dn = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false);
-        return (lane & 4u) ? dn : up;
-    }
-    if (j == 8) return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false);
-    if (j == 16) {
-        const auto p = __builtin_amdgcn_permlane16_swap(v, v, false, false);
-        return (lane & 16u) ? (uint32_t)p[0] : (uint32_t)p[1];
-    }
-    if (j == 32) {
-        const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);
-        return (lane & 32u) ? (uint32_t)p[0] : (uint32_t)p[1];
-    }
-    return (uint32_t)__shfl_xor((int)v, j, G);
-}
-template <int G>
-__device__ __forceinline__ uint64_t sg_xor64(uint64_t v, int j) {
-    const uint32_t lo = sg_xor32<G>((uint32_t)v, j);
-    const uint32_t hi = sg_xor32<G>((uint32_t)(v >> 32), j);
-    return ((uint64_t)hi << 32) | lo;
-}
-template <int G, typename T>
-__device__ __forceinline__ T sg_xor_val(T v, int j) {
-    if constexpr (sizeof(T) == 8) {
-        return __builtin_bit_cast(T, sg_xor64<G>(__builtin_bit_cast(uint64_t, v), j));
-    } else {
-        return __builtin_bit_cast(T, sg_xor32<G>(__builtin_bit_cast(uint32_t, v), j));
-    }
-}
 
 // Products of output row r: one-based exclusive prefix pp (m+1 entries).
 __device__ __forceinline__ int64_t sg_products(const int64_t* __restrict__ pp, int64_t r) {
@@ -713,8 +668,8 @@ __global__ __launch_bounds__(64 * SG_WAVES) void k_sg_row_wave(
     for (int k = 2; k <= G; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            const uint64_t ok = sg_xor64<G>(key, j);
-            const T ov = sg_xor_val<G>(val, j);
+            const uint64_t ok = lane_xor(key, j);
+            const T ov = lane_xor(val, j);
             const bool up = (gl & k) == 0;
             const bool lower = (gl & j) == 0;
             const bool take = (lower == up) ? (ok < key) : (ok > key);
