@@ -29,6 +29,7 @@
 //    expand-sort-compress path (keyed (row, col), stable radix sort, run sums).
 #include "gsa_scan.h"
 #include "gsa_rows.h"
+#include "gsa_radix_pairs.h"
 
 namespace gsa {
 
@@ -88,31 +89,130 @@ __global__ __launch_bounds__(ROW_WG) void k_dz_fill(int64_t m, int64_t nnz,
         });
 }
 
-// ------------------------------------------------------------ CSR + --------
-template <typename T, typename I, bool FILL>
-__global__ void k_add_rows(int64_t m, const I* __restrict__ pA, const I* __restrict__ cA,
-                           const T* __restrict__ vA, const I* __restrict__ pB,
-                           const I* __restrict__ cB, const T* __restrict__ vB,
-                           I* __restrict__ ptrC /* counts at +1 when !FILL */,
-                           I* __restrict__ cC, T* __restrict__ vC) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
-         r += (int64_t)gridDim.x * blockDim.x) {
+// Row merge of A + B, one row per thread, MR_WG consecutive rows per
+// workgroup.  When the block's entries of A and B fit MR_CAP they are read into
+// LDS together (coalesced) and each row is merged from there; in the fill pass
+// the block's output, contiguous in C, is staged in LDS and written out
+// together.  Larger blocks merge straight from global memory.  Row segments and
+// output slots are clamped into the block's ranges, so pointer arrays that
+// decrease cannot reach outside the staged entries.
+//   RUNS = false: CSR + (kernel_add_csr!, matrix_csr_kernels.jl:188-248): a
+//     two-pointer merge, equal columns summed as vA + vB;
+//   RUNS = true: COO + of operands stored in (row, col) order: equal (row, col)
+//     entries are summed left to right, A's in storage order then B's -- the
+//     order of the reference's stable sort of the concatenation and its run
+//     sums (matrix_coo.jl:300-384, matrix_coo_kernels.jl:260-310); columns are
+//     clamped into range as the sort keys are.
+// Exact zeros are dropped.  Without FILL each row's count goes to ptrC[r + 1].
+constexpr int MR_WG = 128;
+constexpr int MR_CAP = 2048;
+
+template <typename T, typename I, bool RUNS, bool FILL>
+__global__ __launch_bounds__(MR_WG) void k_merge_rows(int64_t m, int64_t n,
+                                                      const I* __restrict__ pA,
+                                                      const I* __restrict__ cA,
+                                                      const T* __restrict__ vA,
+                                                      const I* __restrict__ pB,
+                                                      const I* __restrict__ cB,
+                                                      const T* __restrict__ vB,
+                                                      I* __restrict__ ptrC, I* __restrict__ cC,
+                                                      T* __restrict__ vC) {
+    __shared__ I s_c[MR_CAP];
+    __shared__ T s_v[MR_CAP];
+    __shared__ I o_c[FILL ? MR_CAP : 1];
+    __shared__ T o_v[FILL ? MR_CAP : 1];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * MR_WG;
+    if (r0 >= m) return;
+    const int64_t r1 = r0 + MR_WG < m ? r0 + MR_WG : m;
+    const int64_t a0 = (int64_t)pA[r0] - 1, a1 = (int64_t)pA[r1] - 1;
+    const int64_t b0 = (int64_t)pB[r0] - 1, b1 = (int64_t)pB[r1] - 1;
+    const int64_t na = a1 - a0, nb = b1 - b0;
+    const bool staged = a0 >= 0 && b0 >= 0 && na >= 0 && nb >= 0 && na + nb <= MR_CAP;
+    if (staged) {
+        for (int64_t k = tid; k < na; k += MR_WG) { s_c[k] = cA[a0 + k]; s_v[k] = vA[a0 + k]; }
+        for (int64_t k = tid; k < nb; k += MR_WG) {
+            s_c[na + k] = cB[b0 + k];
+            s_v[na + k] = vB[b0 + k];
+        }
+    }
+    int64_t obase = 0, ocount = 0;
+    if (FILL) {
+        obase = (int64_t)ptrC[r0] - 1;
+        ocount = (int64_t)ptrC[r1] - 1 - obase;
+    }
+    const bool staged_out = FILL && staged && obase >= 0 && ocount >= 0 && ocount <= MR_CAP;
+    __syncthreads();
+    const int64_t r = r0 + tid;
+    if (r < r1) {
         int64_t a = (int64_t)pA[r] - 1, ea = (int64_t)pA[r + 1] - 1;
         int64_t b = (int64_t)pB[r] - 1, eb = (int64_t)pB[r + 1] - 1;
+        if (staged) {
+            a = dclamp(a, a0, a1);
+            ea = dclamp(ea, a, a1);
+            b = dclamp(b, b0, b1);
+            eb = dclamp(eb, b, b1);
+        }
+        auto colA = [&](int64_t k) -> I { return staged ? s_c[k - a0] : cA[k]; };
+        auto valA = [&](int64_t k) -> T { return staged ? s_v[k - a0] : vA[k]; };
+        auto colB = [&](int64_t k) -> I { return staged ? s_c[na + (k - b0)] : cB[k]; };
+        auto valB = [&](int64_t k) -> T { return staged ? s_v[na + (k - b0)] : vB[k]; };
         int64_t w = FILL ? (int64_t)ptrC[r] - 1 : 0;
         int64_t cnt = 0;
+        auto emit = [&](I c, T v) {
+            if (v == T(0)) return;
+            if (FILL) {
+                if (staged_out) {
+                    const int64_t q = w - obase;
+                    if (q >= 0 && q < ocount) { o_c[q] = c; o_v[q] = v; }
+                } else {
+                    cC[w] = c;
+                    vC[w] = v;
+                }
+                ++w;
+            }
+            ++cnt;
+        };
         while (a < ea || b < eb) {
-            I c;
-            T v;
-            if (a < ea && (b >= eb || cA[a] < cB[b])) { c = cA[a]; v = vA[a]; ++a; }
-            else if (b < eb && (a >= ea || cA[a] > cB[b])) { c = cB[b]; v = vB[b]; ++b; }
-            else { c = cA[a]; v = vA[a] + vB[b]; ++a; ++b; }
-            if (v != T(0)) {
-                if (FILL) { cC[w] = c; vC[w] = v; ++w; }
-                ++cnt;
+            if constexpr (RUNS) {
+                const int64_t ca = a < ea ? dclamp((int64_t)colA(a) - 1, 0, n - 1) : n;
+                const int64_t cb = b < eb ? dclamp((int64_t)colB(b) - 1, 0, n - 1) : n;
+                const int64_t c = ca < cb ? ca : cb;
+                T sum = T(0);
+                bool first = true;
+                while (a < ea && dclamp((int64_t)colA(a) - 1, 0, n - 1) == c) {
+                    sum = first ? valA(a) : sum + valA(a);
+                    first = false;
+                    ++a;
+                }
+                while (b < eb && dclamp((int64_t)colB(b) - 1, 0, n - 1) == c) {
+                    sum = first ? valB(b) : sum + valB(b);
+                    first = false;
+                    ++b;
+                }
+                emit((I)(c + 1), sum);
+            } else {
+                if (a < ea && (b >= eb || colA(a) < colB(b))) {
+                    emit(colA(a), valA(a));
+                    ++a;
+                } else if (b < eb && (a >= ea || colA(a) > colB(b))) {
+                    emit(colB(b), valB(b));
+                    ++b;
+                } else {
+                    emit(colA(a), valA(a) + valB(b));
+                    ++a;
+                    ++b;
+                }
             }
         }
         if (!FILL) ptrC[r + 1] = (I)cnt;
+    }
+    if (staged_out) {
+        __syncthreads();
+        for (int64_t k = tid; k < ocount; k += MR_WG) {
+            cC[obase + k] = o_c[k];
+            vC[obase + k] = o_v[k];
+        }
     }
 }
 
@@ -171,27 +271,55 @@ __device__ __forceinline__ void key_split(uint64_t key, uint64_t m, double inv_m
     }
 }
 
-// Order of the two parts keys[0, nA) and keys[nA, tot) (column-major keys
-// c*m + r): bit 0 / 2 of *flag when part A / B is not in column-major order,
-// bit 1 / 3 when it is not in row-major order either ((r, c) decreases).
-__global__ void k_parts_order(int64_t nA, int64_t tot, uint64_t m,
-                              const uint64_t* __restrict__ keys, int* __restrict__ flag) {
+// Order of the two operands' storage, from their one-based indices (clamped
+// into range): bit 0 (A) / bit 2 (B) when (col, row) decreases somewhere in
+// the part, bit 1 / bit 3 when (row, col) does.  One update per workgroup.
+template <typename I>
+__global__ void k_parts_order(int64_t nA, int64_t tot, const I* __restrict__ rA,
+                              const I* __restrict__ cA, const I* __restrict__ rB,
+                              const I* __restrict__ cB, int64_t m, int64_t n,
+                              int* __restrict__ flag) {
     int f = 0;
-    const double inv_m = 1.0 / (double)m;
     for (int64_t i = 1 + blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < tot;
          i += (int64_t)gridDim.x * blockDim.x) {
         if (i == nA) continue;
-        const uint64_t k0 = keys[i - 1], k1 = keys[i];
-        const int sh = i < nA ? 0 : 2;
-        if (k1 < k0) f |= 1 << sh;
-        uint64_t c0, r0, c1, r1;
-        key_split(k0, m, inv_m, c0, r0);
-        key_split(k1, m, inv_m, c1, r1);
+        const bool inA = i < nA;
+        const I* rr = inA ? rA : rB;
+        const I* cc = inA ? cA : cB;
+        const int64_t k = inA ? i : i - nA;
+        const int64_t r0 = dclamp((int64_t)rr[k - 1] - 1, 0, m - 1);
+        const int64_t r1 = dclamp((int64_t)rr[k] - 1, 0, m - 1);
+        const int64_t c0 = dclamp((int64_t)cc[k - 1] - 1, 0, n - 1);
+        const int64_t c1 = dclamp((int64_t)cc[k] - 1, 0, n - 1);
+        const int sh = inA ? 0 : 2;
+        if (c1 < c0 || (c1 == c0 && r1 < r0)) f |= 1 << sh;
         if (r1 < r0 || (r1 == r0 && c1 < c0)) f |= 2 << sh;
     }
     int g = 0;
     for (int b = 0; b < 4; ++b) g |= __syncthreads_or(f & (1 << b)) ? 1 << b : 0;
     if (threadIdx.x == 0 && (g & ~*(volatile int*)flag)) atomicOr(flag, g);
+}
+
+// One-based row pointer of entries whose one-based rows never decrease:
+// position i writes the pointers of the rows in (row[i-1], row[i]].
+template <typename I>
+__global__ void k_rows_ptr(int64_t nnz, const I* __restrict__ rows, int64_t m,
+                           I* __restrict__ ptr) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i <= nnz;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        int64_t prev = -1, cur = m;
+        if (i > 0) prev = dclamp((int64_t)rows[i - 1] - 1, 0, m - 1);
+        if (i < nnz) cur = dclamp((int64_t)rows[i] - 1, 0, m - 1);
+        for (int64_t r = prev + 1; r <= cur; ++r) ptr[r] = (I)(i + 1);
+    }
+}
+
+// One-based indices from zero-based sorted keys.
+template <typename I, typename K>
+__global__ void k_keys_to_idx(int64_t nnz, const K* __restrict__ keys, I* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (I)((int64_t)keys[i] + 1);
 }
 
 // The column of each entry of one part (its key / m) and its position in the part.
@@ -998,10 +1126,10 @@ int gsa_csr_add_count(void* stream, int vtype, int itype, int64_t m, const void*
         using T = decltype(tv);
         using I = decltype(ti);
         if (m > 0)
-            hipLaunchKernelGGL((k_add_rows<T, I, false>), dim3(grid_for(m, 256)), dim3(256), 0, s, m,
-                               (const I*)rowptrA, (const I*)colA, (const T*)valA, (const I*)rowptrB,
-                               (const I*)colB, (const T*)valB, (I*)rowptrC, (I*)nullptr,
-                               (T*)nullptr);
+            hipLaunchKernelGGL((k_merge_rows<T, I, false, false>), dim3((unsigned)cdiv(m, MR_WG)),
+                               dim3(MR_WG), 0, s, m, (int64_t)0, (const I*)rowptrA, (const I*)colA,
+                               (const T*)valA, (const I*)rowptrB, (const I*)colB, (const T*)valB,
+                               (I*)rowptrC, (I*)nullptr, (T*)nullptr);
         GSA_HIP(hipGetLastError());
         return counts_to_ptr<I>(s, (I*)rowptrC, m, nnzC);
     });
@@ -1017,9 +1145,10 @@ int gsa_csr_add_fill(void* stream, int vtype, int itype, int64_t m, const void* 
         using T = decltype(tv);
         using I = decltype(ti);
         if (m == 0) return (int)GSA_OK;
-        hipLaunchKernelGGL((k_add_rows<T, I, true>), dim3(grid_for(m, 256)), dim3(256), 0, s, m,
-                           (const I*)rowptrA, (const I*)colA, (const T*)valA, (const I*)rowptrB,
-                           (const I*)colB, (const T*)valB, (I*)rowptrC, (I*)colC, (T*)valC);
+        hipLaunchKernelGGL((k_merge_rows<T, I, false, true>), dim3((unsigned)cdiv(m, MR_WG)),
+                           dim3(MR_WG), 0, s, m, (int64_t)0, (const I*)rowptrA, (const I*)colA,
+                           (const T*)valA, (const I*)rowptrB, (const I*)colB, (const T*)valB,
+                           (I*)rowptrC, (I*)colC, (T*)valC);
         GSA_HIP(hipGetLastError());
         return (int)GSA_OK;
     });
@@ -1050,6 +1179,54 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
         if (tot > 0) {
             Scratch keys(s), perm(s), flag(s);
             int r;
+            // one check of both operands' order decides the path (synchronises)
+            if ((r = flag.alloc(sizeof(int))) != GSA_OK) return fail(r);
+            if ((r = zero_fill(s, flag.p, sizeof(int))) != GSA_OK) return fail(r);
+            hipLaunchKernelGGL((k_parts_order<I>), dim3(grid_for(tot, 256)), dim3(256), 0, s, nnzA,
+                               tot, (const I*)rowA, (const I*)colA, (const I*)rowB, (const I*)colB,
+                               m, n, flag.as<int>());
+            int fl = 15;
+            hipError_t e0 = hipMemcpyAsync(&fl, flag.p, sizeof(int), hipMemcpyDeviceToHost, s);
+            if (e0 == hipSuccess) e0 = hipStreamSynchronize(s);
+            if (e0 != hipSuccess) return fail(hip_fail(e0, "gsa_coo_add_begin"));
+            // each operand's storage order: by (col, row), by (row, col)
+            const bool cmA = !(fl & 1), cmB = !(fl & 4);
+            const bool rmA = !(fl & 2), rmB = !(fl & 8);
+            if (rmA && rmB && !(cmA && cmB)) {
+                // both operands in (row, col) order (COO made from a CSR): a merge of
+                // each row's entries, then (in the finish) one stable sort by column
+                // that carries each entry's (row, value) into the caller's arrays
+                Scratch pA(s), pB(s);
+                if ((r = pA.alloc(sizeof(I) * (m + 1))) != GSA_OK) return fail(r);
+                if ((r = pB.alloc(sizeof(I) * (m + 1))) != GSA_OK) return fail(r);
+                hipLaunchKernelGGL((k_rows_ptr<I>), dim3(grid_for(nnzA + 1, 256)), dim3(256), 0, s,
+                                   nnzA, (const I*)rowA, m, pA.as<I>());
+                hipLaunchKernelGGL((k_rows_ptr<I>), dim3(grid_for(nnzB + 1, 256)), dim3(256), 0, s,
+                                   nnzB, (const I*)rowB, m, pB.as<I>());
+                if ((r = ws_alloc(&c->buf[0], sizeof(I) * (m + 1), s)) != GSA_OK) return fail(r);
+                I* ptrC = (I*)c->buf[0];
+                hipLaunchKernelGGL((k_merge_rows<T, I, true, false>), dim3((unsigned)cdiv(m, MR_WG)),
+                                   dim3(MR_WG), 0, s, m, n, (const I*)pA.as<I>(), (const I*)colA,
+                                   (const T*)valA, (const I*)pB.as<I>(), (const I*)colB,
+                                   (const T*)valB, ptrC, (I*)nullptr, (T*)nullptr);
+                int64_t total = 0;
+                if ((r = counts_to_ptr<I>(s, ptrC, m, &total)) != GSA_OK) return fail(r);
+                if (total > 0) {
+                    if ((r = ws_alloc(&c->buf[1], sizeof(I) * total, s)) != GSA_OK) return fail(r);
+                    if ((r = ws_alloc(&c->buf[2], sizeof(T) * total, s)) != GSA_OK) return fail(r);
+                    hipLaunchKernelGGL((k_merge_rows<T, I, true, true>), dim3((unsigned)cdiv(m, MR_WG)),
+                                       dim3(MR_WG), 0, s, m, n, (const I*)pA.as<I>(), (const I*)colA,
+                                       (const T*)valA, (const I*)pB.as<I>(), (const I*)colB,
+                                       (const T*)valB, ptrC, (I*)c->buf[1], (T*)c->buf[2]);
+                }
+                hipError_t e1 = hipGetLastError();
+                if (e1 != hipSuccess) return fail(hip_fail(e1, "gsa_coo_add_begin"));
+                c->aux[1] = 1;  // row merge: the finish sorts by column
+                c->nnz_out = total;
+                *nnzC = total;
+                *ctx = c;
+                return (int)GSA_OK;
+            }
             const uint64_t* skeys = nullptr;  // keys in column-major order
             // the concatenation sorted stably by (col, row): a merge when both operands
             // are already in that order (COO from CSC or from a SparseMatrixCSC), else
@@ -1058,27 +1235,18 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
                 using V = decltype(pos_t);
                 GSA_TRY(keys.alloc(sizeof(uint64_t) * 2 * tot));
                 GSA_TRY(perm.alloc(sizeof(V) * tot));
-                GSA_TRY(flag.alloc(sizeof(int)));
                 uint64_t* k0 = keys.as<uint64_t>();
                 uint64_t* k1 = k0 + tot;
                 hipLaunchKernelGGL((k_concat_keys<I, V>), dim3(grid_for(tot, 256)), dim3(256), 0,
                                    s, nnzA, (const I*)rowA, (const I*)colA, nnzB, (const I*)rowB,
                                    (const I*)colB, m, n, k0, perm.as<V>());
-                GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
-                hipLaunchKernelGGL(k_parts_order, dim3(grid_for(tot, 256)), dim3(256), 0, s, nnzA,
-                                   tot, (uint64_t)m, (const uint64_t*)k0, flag.as<int>());
-                int fl = 15;
-                GSA_HIP(hipMemcpyAsync(&fl, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
-                GSA_HIP(hipStreamSynchronize(s));
-                const bool colA = !(fl & 1), colB = !(fl & 4);
-                const bool rowA = !(fl & 2), rowB = !(fl & 8);
-                if (colA && colB) {
+                if (cmA && cmB) {
                     GSA_TRY(merge_sorted_parts<V>(s, nnzA, tot, k0, k1, perm.as<V>()));
                     GSA_HIP(hipGetLastError());
                     skeys = k1;
                     return GSA_OK;
                 }
-                if ((colA || rowA) && (colB || rowB) && n <= (int64_t)UINT32_MAX &&
+                if ((cmA || rmA) && (cmB || rmB) && n <= (int64_t)UINT32_MAX &&
                     tot < (int64_t)UINT32_MAX) {
                     // an operand in row-major order (a COO made from a CSR) becomes
                     // column-major by a stable sort on the column alone; then the merge
@@ -1091,7 +1259,7 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
                     for (int part = 0; part < 2; ++part) {
                         const int64_t off = part ? nnzA : 0, cnt = part ? nnzB : nnzA;
                         if (cnt == 0) continue;
-                        const bool colsorted = part ? colB : colA;
+                        const bool colsorted = part ? cmB : cmA;
                         const unsigned gp = grid_for(cnt, 256);
                         hipLaunchKernelGGL(k_part_cols, dim3(gp), dim3(256), 0, s, cnt,
                                            (const uint64_t*)(k0 + off), (uint64_t)m,
@@ -1160,6 +1328,27 @@ int gsa_coo_add_finish(gsa_ctx ctx, void* rowC, void* colC, void* valC) {
         using T = decltype(tv);
         using I = decltype(ti);
         if (tot == 0) return (int)GSA_OK;
+        if (ctx->aux[1] == 1) {  // row merge: stable sort by column, (row, value) carried
+            const int64_t nz = ctx->nnz_out, m = ctx->m, n = ctx->n;
+            if (nz == 0) return (int)GSA_OK;
+            Scratch rows(s), keys(s);
+            GSA_TRY(rows.alloc(sizeof(I) * nz));
+            GSA_TRY(expand_rows(s, ctx->itype, m, nz, ctx->buf[0], rows.p));
+            const int bits = bits_for((uint64_t)(n - 1));
+            auto run = [&](auto key_t) -> int {
+                using K = decltype(key_t);
+                GSA_TRY(keys.alloc(sizeof(K) * nz));
+                const KeyFromIdx<K, I> key0{(const I*)ctx->buf[1], nullptr, n, 1};
+                GSA_TRY((radix_sort_payload<K, T, I>(s, nz, bits, key0, (const I*)rows.p,
+                                                     (const T*)ctx->buf[2], keys.as<K>(),
+                                                     (I*)rowC, (T*)valC)));
+                hipLaunchKernelGGL((k_keys_to_idx<I, K>), dim3(grid_for(nz, 256)), dim3(256), 0, s,
+                                   nz, (const K*)keys.as<K>(), (I*)colC);
+                GSA_HIP(hipGetLastError());
+                return (int)GSA_OK;
+            };
+            return bits <= 32 ? run(uint32_t{}) : run(uint64_t{});
+        }
         hipLaunchKernelGGL((k_compact<T, I>), dim3(grid_for(tot, 256)), dim3(256), 0, s, tot,
                            (const int64_t*)ctx->buf[4], (const I*)ctx->buf[0],
                            (const I*)ctx->buf[1], (const T*)ctx->buf[3], (I*)rowC, (I*)colC,

@@ -425,7 +425,20 @@ def test_coo_add_merge_and_sort_paths(dev, itype):
     same(coo_tuple(dAr + dBr), O.coo_add(m, Ar, Br))
     same(coo_tuple(dAr + dB), O.coo_add(m, Ar, B))
     same(coo_tuple(dA + dBr), O.coo_add(m, A, Br))
+    # both row-major with Float64 values: every run summed in the oracle's order, bit for bit
+    Af = (Ar[0], Ar[1], rng.standard_normal(Ar[0].size))
+    Bf = (Br[0], Br[1], rng.standard_normal(Br[0].size))
+    dAf = from_arrays(G.GenericSparseMatrixCOO, m, n, *Af, dev, itype)
+    dBf = from_arrays(G.GenericSparseMatrixCOO, m, n, *Bf, dev, itype)
+    same(coo_tuple(dAf + dBf), O.coo_add(m, Af, Bf))
+    Z = (np.zeros(0), np.zeros(0), np.zeros(0))
     E = from_arrays(G.GenericSparseMatrixCOO, m, n, np.zeros(0, np.int64), np.zeros(0, np.int64),
                     np.zeros(0), dev, itype)
-    same(coo_tuple(E + dB), O.coo_add(m, (np.zeros(0), np.zeros(0), np.zeros(0)), B))
-    same(coo_tuple(dA + E), O.coo_add(m, A, (np.zeros(0), np.zeros(0), np.zeros(0))))
+    same(coo_tuple(E + dB), O.coo_add(m, Z, B))
+    same(coo_tuple(dA + E), O.coo_add(m, A, Z))
+    same(coo_tuple(E + dBr), O.coo_add(m, Z, Br))
+    same(coo_tuple(dAr + E), O.coo_add(m, Ar, Z))
+    # a row-major operand whose entries all cancel
+    Neg = (Ar[0], Ar[1], -Ar[2])
+    dNeg = from_arrays(G.GenericSparseMatrixCOO, m, n, *Neg, dev, itype)
+    same(coo_tuple(dAr + dNeg), O.coo_add(m, Ar, Neg))
