@@ -89,13 +89,7 @@ __global__ __launch_bounds__(ROW_WG) void k_dz_fill(int64_t m, int64_t nnz,
         });
 }
 
-// Row merge of A + B, one row per thread, MR_WG consecutive rows per
-// workgroup.  When the block's entries of A and B fit MR_CAP they are read into
-// LDS together (coalesced) and each row is merged from there; in the fill pass
-// the block's output, contiguous in C, is staged in LDS and written out
-// together.  Larger blocks merge straight from global memory.  Row segments and
-// output slots are clamped into the block's ranges, so pointer arrays that
-// decrease cannot reach outside the staged entries.
+// Row merge of A + B, MR_WG consecutive rows per workgroup.
 //   RUNS = false: CSR + (kernel_add_csr!, matrix_csr_kernels.jl:188-248): a
 //     two-pointer merge, equal columns summed as vA + vB;
 //   RUNS = true: COO + of operands stored in (row, col) order: equal (row, col)
@@ -104,8 +98,187 @@ __global__ __launch_bounds__(ROW_WG) void k_dz_fill(int64_t m, int64_t nnz,
 //     sums (matrix_coo.jl:300-384, matrix_coo_kernels.jl:260-310); columns are
 //     clamped into range as the sort keys are.
 // Exact zeros are dropped.  Without FILL each row's count goes to ptrC[r + 1].
-constexpr int MR_WG = 128;
-constexpr int MR_CAP = 2048;
+// Rows go one per thread.  When the block's entries of A and B fit MR_CAP they
+// are first read into LDS together (coalesced) and each row is merged from
+// there (row segments clamped into the staged ranges, so pointer arrays that
+// decrease cannot reach outside them).  In a block that is not staged, a row
+// of more than MR_LONG entries is merged by the whole workgroup afterwards
+// (merge_long_row).  Staging the output block in LDS as well measured slower:
+// the LDS it takes costs more occupancy than the coalesced write saves.
+constexpr int MR_WG = 256;
+constexpr int MR_CAP = 3072;  // 48 KB of Float64/Int64 entries: three workgroups per CU
+constexpr int64_t MR_LONG = 128;  // measured: 1024 -> 3.0 ms, 256 -> 2.2, 128 -> 2.06 on A + A (power law)
+
+// The serial merge of one row; colA(k) etc. read entry k of A / B.
+template <typename T, typename I, bool RUNS, bool FILL, typename CA, typename VA, typename CB,
+          typename VB>
+__device__ __forceinline__ void merge_row_serial(int64_t r, int64_t n, int64_t a, int64_t ea,
+                                                 int64_t b, int64_t eb, CA colA, VA valA,
+                                                 CB colB, VB valB, I* __restrict__ ptrC,
+                                                 I* __restrict__ cC, T* __restrict__ vC) {
+    int64_t w = FILL ? (int64_t)ptrC[r] - 1 : 0;
+    int64_t cnt = 0;
+    auto emit = [&](I c, T v) {
+        if (v == T(0)) return;
+        if (FILL) {
+            cC[w] = c;
+            vC[w] = v;
+            ++w;
+        }
+        ++cnt;
+    };
+    while (a < ea || b < eb) {
+        if constexpr (RUNS) {
+            const int64_t ca = a < ea ? dclamp((int64_t)colA(a) - 1, 0, n - 1) : n;
+            const int64_t cb = b < eb ? dclamp((int64_t)colB(b) - 1, 0, n - 1) : n;
+            const int64_t c = ca < cb ? ca : cb;
+            T sum = T(0);
+            bool first = true;
+            while (a < ea && dclamp((int64_t)colA(a) - 1, 0, n - 1) == c) {
+                sum = first ? valA(a) : sum + valA(a);
+                first = false;
+                ++a;
+            }
+            while (b < eb && dclamp((int64_t)colB(b) - 1, 0, n - 1) == c) {
+                sum = first ? valB(b) : sum + valB(b);
+                first = false;
+                ++b;
+            }
+            emit((I)(c + 1), sum);
+        } else {
+            if (a < ea && (b >= eb || colA(a) < colB(b))) {
+                emit(colA(a), valA(a));
+                ++a;
+            } else if (b < eb && (a >= ea || colA(a) > colB(b))) {
+                emit(colB(b), valB(b));
+                ++b;
+            } else {
+                emit(colA(a), valA(a) + valB(b));
+                ++a;
+                ++b;
+            }
+        }
+    }
+    if (!FILL) ptrC[r + 1] = (I)cnt;
+}
+
+// Exclusive scan in place of f[0 .. len) (uint16) by the whole workgroup;
+// f[len] gets the total.  Every thread must call it.
+__device__ __forceinline__ int block_scan_u16(uint16_t* f, int64_t len, int* lds4) {
+    const int tid = threadIdx.x;
+    const int64_t per = (len + MR_WG - 1) / MR_WG;
+    const int64_t s0 = tid * per < len ? tid * per : len;
+    const int64_t s1 = s0 + per < len ? s0 + per : len;
+    int mine = 0;
+    for (int64_t k = s0; k < s1; ++k) mine += f[k];
+    int total = 0;
+    int run = wg_exclusive_sum<int>(mine, lds4, &total);
+    for (int64_t k = s0; k < s1; ++k) {
+        const int v = f[k];
+        f[k] = (uint16_t)run;
+        run += v;
+    }
+    __syncthreads();
+    if (tid == 0) f[len] = (uint16_t)total;
+    __syncthreads();
+    return total;
+}
+
+// One long row merged by the whole workgroup.  When both segments have
+// strictly increasing columns (the canonical case), each entry of A finds its
+// column in B's segment by binary search (found: vA + vB), each entry of B its
+// column in A's (found: left to A); the kept (nonzero) flags are scanned, and
+// an entry's output slot is the number of kept entries of its own segment
+// before it plus those of the other segment with smaller columns.  This is
+// the two-pointer merge's output, and for RUNS every run has one entry, so
+// its sums too.  Other rows (repeated or unordered columns, or more entries
+// than the flags' LDS holds) are merged serially by one thread.
+template <typename T, typename I, bool RUNS, bool FILL>
+__device__ void merge_long_row(int64_t r, int64_t n, const I* __restrict__ pA,
+                               const I* __restrict__ cA, const T* __restrict__ vA,
+                               const I* __restrict__ pB, const I* __restrict__ cB,
+                               const T* __restrict__ vB, I* __restrict__ ptrC,
+                               I* __restrict__ cC, T* __restrict__ vC, unsigned char* lds,
+                               int64_t lds_bytes, int* lds4) {
+    const int tid = threadIdx.x;
+    const int64_t a = (int64_t)pA[r] - 1, ea = (int64_t)pA[r + 1] - 1;
+    const int64_t b = (int64_t)pB[r] - 1, eb = (int64_t)pB[r + 1] - 1;
+    const int64_t na = ea > a ? ea - a : 0, nb = eb > b ? eb - b : 0;
+    auto key = [&](const I* c, int64_t k) -> int64_t {
+        return RUNS ? dclamp((int64_t)c[k] - 1, 0, n - 1) : (int64_t)c[k];
+    };
+    auto out_col = [&](const I* c, int64_t k) -> I { return RUNS ? (I)(key(c, k) + 1) : c[k]; };
+    int bad = 0;
+    for (int64_t k = 1 + tid; k < na; k += MR_WG) bad |= key(cA, a + k) <= key(cA, a + k - 1);
+    for (int64_t k = 1 + tid; k < nb; k += MR_WG) bad |= key(cB, b + k) <= key(cB, b + k - 1);
+    bad = __syncthreads_or(bad);
+    const int64_t cap = lds_bytes / (int64_t)sizeof(uint16_t);
+    if (bad || na + nb + 2 > cap || na + nb >= 65535) {
+        if (tid == 0)
+            merge_row_serial<T, I, RUNS, FILL>(
+                r, n, a, ea, b, eb, [&](int64_t k) { return cA[k]; },
+                [&](int64_t k) { return vA[k]; }, [&](int64_t k) { return cB[k]; },
+                [&](int64_t k) { return vB[k]; }, ptrC, cC, vC);
+        __syncthreads();
+        return;
+    }
+    // position of the first entry of [s, s + len) of c whose key is >= x
+    auto lower = [&](const I* c, int64_t s, int64_t len, int64_t x) -> int64_t {
+        int64_t lo = 0, hi = len;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (key(c, s + mid) < x) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    uint16_t* fa = reinterpret_cast<uint16_t*>(lds);
+    uint16_t* fb = fa + (na + 1);
+    for (int64_t i = tid; i < na; i += MR_WG) {
+        const int64_t x = key(cA, a + i);
+        const int64_t j = lower(cB, b, nb, x);
+        const bool match = j < nb && key(cB, b + j) == x;
+        const T v = match ? vA[a + i] + vB[b + j] : vA[a + i];
+        fa[i] = v != T(0);
+    }
+    for (int64_t j = tid; j < nb; j += MR_WG) {
+        const int64_t x = key(cB, b + j);
+        const int64_t i = lower(cA, a, na, x);
+        const bool match = i < na && key(cA, a + i) == x;
+        fb[j] = !match && vB[b + j] != T(0);
+    }
+    __syncthreads();
+    const int totA = block_scan_u16(fa, na, lds4);
+    const int totB = block_scan_u16(fb, nb, lds4);
+    if (!FILL) {
+        if (tid == 0) ptrC[r + 1] = (I)(totA + totB);
+        __syncthreads();
+        return;
+    }
+    const int64_t w0 = (int64_t)ptrC[r] - 1;
+    for (int64_t i = tid; i < na; i += MR_WG) {
+        const int64_t x = key(cA, a + i);
+        const int64_t j = lower(cB, b, nb, x);
+        const bool match = j < nb && key(cB, b + j) == x;
+        const T v = match ? vA[a + i] + vB[b + j] : vA[a + i];
+        if (v != T(0)) {
+            const int64_t q = w0 + fa[i] + fb[j];
+            cC[q] = out_col(cA, a + i);
+            vC[q] = v;
+        }
+    }
+    for (int64_t j = tid; j < nb; j += MR_WG) {
+        const int64_t x = key(cB, b + j);
+        const int64_t i = lower(cA, a, na, x);
+        const bool match = i < na && key(cA, a + i) == x;
+        if (!match && vB[b + j] != T(0)) {
+            const int64_t q = w0 + fb[j] + fa[i];
+            cC[q] = out_col(cB, b + j);
+            vC[q] = vB[b + j];
+        }
+    }
+    __syncthreads();
+}
 
 template <typename T, typename I, bool RUNS, bool FILL>
 __global__ __launch_bounds__(MR_WG) void k_merge_rows(int64_t m, int64_t n,
@@ -117,10 +290,13 @@ __global__ __launch_bounds__(MR_WG) void k_merge_rows(int64_t m, int64_t n,
                                                       const T* __restrict__ vB,
                                                       I* __restrict__ ptrC, I* __restrict__ cC,
                                                       T* __restrict__ vC) {
-    __shared__ I s_c[MR_CAP];
-    __shared__ T s_v[MR_CAP];
-    __shared__ I o_c[FILL ? MR_CAP : 1];
-    __shared__ T o_v[FILL ? MR_CAP : 1];
+    constexpr int64_t LDS_BYTES = (int64_t)MR_CAP * (sizeof(I) + sizeof(T));
+    __shared__ __align__(16) unsigned char lds[LDS_BYTES];
+    __shared__ int64_t longs[MR_WG];
+    __shared__ int nlong;
+    __shared__ int lds4[4];
+    I* s_c = reinterpret_cast<I*>(lds);
+    T* s_v = reinterpret_cast<T*>(lds + (int64_t)MR_CAP * sizeof(I));
     const int tid = threadIdx.x;
     const int64_t r0 = (int64_t)blockIdx.x * MR_WG;
     if (r0 >= m) return;
@@ -129,6 +305,7 @@ __global__ __launch_bounds__(MR_WG) void k_merge_rows(int64_t m, int64_t n,
     const int64_t b0 = (int64_t)pB[r0] - 1, b1 = (int64_t)pB[r1] - 1;
     const int64_t na = a1 - a0, nb = b1 - b0;
     const bool staged = a0 >= 0 && b0 >= 0 && na >= 0 && nb >= 0 && na + nb <= MR_CAP;
+    if (tid == 0) nlong = 0;
     if (staged) {
         for (int64_t k = tid; k < na; k += MR_WG) { s_c[k] = cA[a0 + k]; s_v[k] = vA[a0 + k]; }
         for (int64_t k = tid; k < nb; k += MR_WG) {
@@ -136,12 +313,6 @@ __global__ __launch_bounds__(MR_WG) void k_merge_rows(int64_t m, int64_t n,
             s_v[na + k] = vB[b0 + k];
         }
     }
-    int64_t obase = 0, ocount = 0;
-    if (FILL) {
-        obase = (int64_t)ptrC[r0] - 1;
-        ocount = (int64_t)ptrC[r1] - 1 - obase;
-    }
-    const bool staged_out = FILL && staged && obase >= 0 && ocount >= 0 && ocount <= MR_CAP;
     __syncthreads();
     const int64_t r = r0 + tid;
     if (r < r1) {
@@ -152,68 +323,24 @@ __global__ __launch_bounds__(MR_WG) void k_merge_rows(int64_t m, int64_t n,
             ea = dclamp(ea, a, a1);
             b = dclamp(b, b0, b1);
             eb = dclamp(eb, b, b1);
-        }
-        auto colA = [&](int64_t k) -> I { return staged ? s_c[k - a0] : cA[k]; };
-        auto valA = [&](int64_t k) -> T { return staged ? s_v[k - a0] : vA[k]; };
-        auto colB = [&](int64_t k) -> I { return staged ? s_c[na + (k - b0)] : cB[k]; };
-        auto valB = [&](int64_t k) -> T { return staged ? s_v[na + (k - b0)] : vB[k]; };
-        int64_t w = FILL ? (int64_t)ptrC[r] - 1 : 0;
-        int64_t cnt = 0;
-        auto emit = [&](I c, T v) {
-            if (v == T(0)) return;
-            if (FILL) {
-                if (staged_out) {
-                    const int64_t q = w - obase;
-                    if (q >= 0 && q < ocount) { o_c[q] = c; o_v[q] = v; }
-                } else {
-                    cC[w] = c;
-                    vC[w] = v;
-                }
-                ++w;
-            }
-            ++cnt;
-        };
-        while (a < ea || b < eb) {
-            if constexpr (RUNS) {
-                const int64_t ca = a < ea ? dclamp((int64_t)colA(a) - 1, 0, n - 1) : n;
-                const int64_t cb = b < eb ? dclamp((int64_t)colB(b) - 1, 0, n - 1) : n;
-                const int64_t c = ca < cb ? ca : cb;
-                T sum = T(0);
-                bool first = true;
-                while (a < ea && dclamp((int64_t)colA(a) - 1, 0, n - 1) == c) {
-                    sum = first ? valA(a) : sum + valA(a);
-                    first = false;
-                    ++a;
-                }
-                while (b < eb && dclamp((int64_t)colB(b) - 1, 0, n - 1) == c) {
-                    sum = first ? valB(b) : sum + valB(b);
-                    first = false;
-                    ++b;
-                }
-                emit((I)(c + 1), sum);
-            } else {
-                if (a < ea && (b >= eb || colA(a) < colB(b))) {
-                    emit(colA(a), valA(a));
-                    ++a;
-                } else if (b < eb && (a >= ea || colA(a) > colB(b))) {
-                    emit(colB(b), valB(b));
-                    ++b;
-                } else {
-                    emit(colA(a), valA(a) + valB(b));
-                    ++a;
-                    ++b;
-                }
-            }
-        }
-        if (!FILL) ptrC[r + 1] = (I)cnt;
-    }
-    if (staged_out) {
-        __syncthreads();
-        for (int64_t k = tid; k < ocount; k += MR_WG) {
-            cC[obase + k] = o_c[k];
-            vC[obase + k] = o_v[k];
+            merge_row_serial<T, I, RUNS, FILL>(
+                r, n, a, ea, b, eb, [&](int64_t k) { return s_c[k - a0]; },
+                [&](int64_t k) { return s_v[k - a0]; }, [&](int64_t k) { return s_c[na + (k - b0)]; },
+                [&](int64_t k) { return s_v[na + (k - b0)]; }, ptrC, cC, vC);
+        } else if ((ea - a) + (eb - b) > MR_LONG) {
+            longs[atomicAdd(&nlong, 1)] = r;  // at most MR_WG rows
+        } else {
+            merge_row_serial<T, I, RUNS, FILL>(
+                r, n, a, ea, b, eb, [&](int64_t k) { return cA[k]; },
+                [&](int64_t k) { return vA[k]; }, [&](int64_t k) { return cB[k]; },
+                [&](int64_t k) { return vB[k]; }, ptrC, cC, vC);
         }
     }
+    __syncthreads();
+    const int nl = nlong;
+    for (int q = 0; q < nl; ++q)
+        merge_long_row<T, I, RUNS, FILL>(longs[q], n, pA, cA, vA, pB, cB, vB, ptrC, cC, vC, lds,
+                                         LDS_BYTES, lds4);
 }
 
 // ------------------------------------------------------------ COO helpers --

@@ -195,6 +195,44 @@ def test_add_laplacian_perturbed_full_size(dev):
     assert D.nnz() == L.nnz - Lp.nnz
 
 
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+def test_add_long_rows_workgroup_merge(dev, itype):
+    """Rows of more than 128 entries (A's and B's together) in blocks too large for
+    LDS are merged by the whole workgroup (binary searches and scans) when their
+    columns strictly increase, and by one thread otherwise; both equal the oracle's
+    two-pointer merge (CSR) and stable run sums (COO from CSR) bit for bit."""
+    rng = np.random.default_rng(61)
+    H = synth.powerlaw_csr(4000, 6000, seed=6, hi=3000)
+    P = synth.drop_entries(H, 0.3, seed=7)
+    vals = P.nzval.copy()
+    flip = rng.random(vals.size) < 0.5
+    vals[flip] = -vals[flip]          # these cancel A's entries exactly
+    vals[~flip] = rng.standard_normal(int((~flip).sum()))
+    m, n = H.m, H.n
+    A_t = (H.rowptr, H.colval, H.nzval)
+    B_t = (P.rowptr, P.colval, vals)
+    assert np.diff(H.rowptr).max() + np.diff(P.rowptr).max() > 128
+    A = from_arrays(G.GenericSparseMatrixCSR, m, n, *A_t, dev, itype)
+    B = from_arrays(G.GenericSparseMatrixCSR, m, n, *B_t, dev, itype)
+    same(csr_tuple(A + B), O.csr_add(m, A_t, B_t))
+    same(csr_tuple(A + A), O.csr_add(m, A_t, A_t))
+    # the same sums as row-major COO operands (the row-merge path of COO +)
+    coo = lambda T: (np.repeat(np.arange(1, T[0].size), np.diff(T[0])), T[1], T[2])
+    Ao = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo(A_t), dev, itype)
+    Bo = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo(B_t), dev, itype)
+    same(coo_tuple(Ao + Bo), O.coo_add(m, coo(A_t), coo(B_t)))
+    # a long row with repeated columns: the serial merge keeps the two-pointer semantics
+    lens = np.r_[rng.integers(0, 8, 300), [1500], rng.integers(0, 8, 300)]
+    rp = np.r_[0, np.cumsum(lens)] + 1
+    cv = np.concatenate([np.sort(rng.integers(1, 700, k)) for k in lens])  # repeats allowed
+    nz = rng.standard_normal(cv.size)
+    R_t = (rp, cv, nz)
+    R = from_arrays(G.GenericSparseMatrixCSR, lens.size, 700, *R_t, dev, itype)
+    same(csr_tuple(R + R), O.csr_add(lens.size, R_t, R_t))
+    Ro = from_arrays(G.GenericSparseMatrixCOO, lens.size, 700, *coo(R_t), dev, itype)
+    same(coo_tuple(Ro + Ro), O.coo_add(lens.size, coo(R_t), coo(R_t)))
+
+
 def test_dropzeros_csr_csc_coo(dev):
     rng = np.random.default_rng(24)
     H = synth.csr_from_lengths(np.r_[rng.integers(0, 20, 3000), [3000, 0, 5]], 5000, rng, "normal")
