@@ -112,7 +112,8 @@ __global__ void k_ptr_from_keys(int64_t nnz, const K* __restrict__ keys, uint64_
 template <typename T, typename I>
 static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_dim, int64_t nnz,
                               const I* major, const I* minor_key, const I* minor_out,
-                              const T* val, I* ptr_out, I* idx_out, T* val_out) {
+                              const T* val, I* ptr_out, I* idx_out, T* val_out,
+                              const I* minor_ptr = nullptr) {
     GSA_CHECK(nnz == 0 || (major_dim > 0 && (minor_key == nullptr || minor_dim > 0)),
               GSA_ERR_INVALID, "stored entries in a matrix with a zero dimension");
     if (nnz == 0) {  // every pointer is 1 (conversions.jl:178-184, 289-295)
@@ -162,7 +163,7 @@ static int sort_to_compressed(hipStream_t s, int64_t major_dim, int64_t minor_di
         GSA_TRY(keys.alloc(sizeof(K) * nnz));
         const KeyFromIdx<K, I> key0{major, kminor, major_dim, (int64_t)kdim};
         GSA_TRY((radix_sort_payload<K, T, I>(s, nnz, bits, key0, minor_out, val, keys.as<K>(),
-                                             idx_out, val_out)));
+                                             idx_out, val_out, minor_ptr, minor_dim)));
         hipLaunchKernelGGL((k_ptr_from_keys<I, K>), dim3(grid_for(nnz + 1, 256)), dim3(256), 0, s,
                            nnz, (const K*)keys.as<K>(), kdim, major_dim, ptr_out);
         GSA_HIP(hipGetLastError());
@@ -346,15 +347,19 @@ int gsa_csr_to_csc(void* stream, int vtype, int itype, int64_t m, int64_t n, int
     return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
+        // stable sort by column only, each entry's (row, value) carried; the first
+        // pass finds the rows from the row pointer tile by tile (below 2^31 rows),
+        // otherwise they are expanded first
         Scratch rows(s);
-        if (nnz > 0) {
+        const bool from_ptr = m < ((int64_t)1 << 31);
+        if (nnz > 0 && !from_ptr) {
             GSA_TRY(rows.alloc(sizeof(I) * nnz));
             GSA_TRY(expand_rows(s, itype, m, nnz, rowptr, rows.p));
         }
-        // stable sort by column only; the sorted positions gather the (row, value) pairs
         return sort_to_compressed<T, I>(s, n, m, nnz, (const I*)colval, nullptr,
                                         (const I*)rows.p, (const T*)nzval, (I*)colptr,
-                                        (I*)rowval, (T*)nzval_out);
+                                        (I*)rowval, (T*)nzval_out,
+                                        from_ptr ? (const I*)rowptr : nullptr);
     });
 }
 

@@ -67,7 +67,7 @@ struct PaySplit {
     const T* val;
     __device__ __forceinline__ MinorVal<T, I> operator()(int64_t i) const {
         MinorVal<T, I> m;
-        m.minor = minor[i];
+        m.minor = minor ? minor[i] : I(0);
         m.val = val[i];
         return m;
     }
@@ -150,10 +150,66 @@ __device__ __forceinline__ int rp_block_excl_sum(int v, int* wsum) {
     return pre + x - v;
 }
 
-template <typename K, typename P, typename KeySrc, typename PaySrc, typename Sink>
+// Where the first pass takes each entry's minor index from: the payload source
+// (NoRows), or, for a CSR-to-CSC transposition, the row each entry lies in,
+// found per tile from the one-based row pointer (TileRows; tile_rows holds the
+// rows of each tile's first and last entries, from k_tile_rows).
+struct NoRows {
+    static constexpr bool active = false;
+};
+template <typename I>
+struct TileRows {
+    static constexpr bool active = true;
+    const I* ptr;
+    const int64_t* tile_rows;
+};
+
+// Rows (zero-based) of the first and last entry of every RP_TILE-entry tile:
+// the last row r with ptr[r] - 1 <= position, by binary search.
+template <typename I>
+__global__ void k_tile_rows(int64_t m, int64_t nnz, const I* __restrict__ ptr, int64_t ntiles,
+                            int64_t* __restrict__ out) {
+    const int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const int64_t first = t * RP_TILE;
+    const int64_t last = (first + RP_TILE < nnz ? first + RP_TILE : nnz) - 1;
+    auto row_of = [&](int64_t pos) {
+        int64_t lo = 0, hi = m - 1;  // the answer lies in [lo, hi]
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if ((int64_t)ptr[mid] - 1 <= pos) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    out[2 * t] = row_of(first);
+    out[2 * t + 1] = row_of(last);
+}
+
+// Exclusive max over the 256 threads of one workgroup (values >= 0).
+__device__ __forceinline__ uint32_t rp_block_excl_max(uint32_t v, uint32_t* wmax) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t x = v;
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+        const uint32_t y = __shfl_up(x, off, 64);
+        if (lane >= off) x = x > y ? x : y;
+    }
+    if (lane == 63) wmax[wave] = x;
+    __syncthreads();
+    uint32_t pre = 0;
+    for (int w = 0; w < wave; ++w) pre = pre > wmax[w] ? pre : wmax[w];
+    const uint32_t excl = __shfl_up(x, 1, 64);
+    const uint32_t inwave = lane == 0 ? 0u : excl;
+    return pre > inwave ? pre : inwave;
+}
+
+template <typename K, typename P, typename KeySrc, typename PaySrc, typename Sink,
+          typename RowSrc = NoRows>
 __global__ __launch_bounds__(RP_WG) void k_rp_scatter(
     KeySrc key, PaySrc pay, int64_t n, int shift, int nbits, const int64_t* __restrict__ incl,
-    const int64_t* __restrict__ hist, int64_t ntiles, K* __restrict__ keys_out, Sink sink) {
+    const int64_t* __restrict__ hist, int64_t ntiles, K* __restrict__ keys_out, Sink sink,
+    RowSrc rows = RowSrc{}) {
     __shared__ int wcnt[RP_WAVES][RP_BINS];   // per-wavefront digit counts, then offsets
     __shared__ int tile_start[RP_BINS];
     __shared__ int wsum[RP_WAVES];
@@ -174,6 +230,39 @@ __global__ __launch_bounds__(RP_WG) void k_rp_scatter(
     }
     __syncthreads();
 
+    // TileRows: each entry's row, relative to the tile's first row, in LDS (the
+    // payload staging area, unused until the ranking is done): every row that
+    // starts inside the tile marks its first slot (the largest row wins where
+    // empty rows share a slot), then an inclusive max-scan fills the rest
+    int64_t row0 = 0;
+    uint32_t* rid = reinterpret_cast<uint32_t*>(s_pay);
+    if constexpr (RowSrc::active) {
+        row0 = rows.tile_rows[2 * tile];
+        const int64_t row1 = rows.tile_rows[2 * tile + 1];
+        for (int p = tid; p < tile_n; p += RP_WG) rid[p] = 0u;
+        __syncthreads();
+        for (int64_t r = row0 + 1 + tid; r <= row1; r += RP_WG) {
+            const int64_t st = (int64_t)rows.ptr[r] - 1 - base;
+            if (st > 0 && st < tile_n) atomicMax(&rid[st], (uint32_t)(r - row0));
+        }
+        __syncthreads();
+        constexpr int PER = RP_TILE / RP_WG;
+        uint32_t mx = 0;
+        for (int q = 0; q < PER; ++q) {
+            const int p = tid * PER + q;
+            if (p < tile_n) mx = mx > rid[p] ? mx : rid[p];
+        }
+        uint32_t run = rp_block_excl_max(mx, reinterpret_cast<uint32_t*>(wsum));
+        for (int q = 0; q < PER; ++q) {
+            const int p = tid * PER + q;
+            if (p < tile_n) {
+                run = run > rid[p] ? run : rid[p];
+                rid[p] = run;
+            }
+        }
+        __syncthreads();
+    }
+
     // this wavefront's entries: tile positions wave*RP_WTILE + round*64 + lane
     K k[RP_IPT];
     P v[RP_IPT];
@@ -183,6 +272,7 @@ __global__ __launch_bounds__(RP_WG) void k_rp_scatter(
         if (pos < tile_n) {
             k[i] = key(base + pos);
             v[i] = pay(base + pos);
+            if constexpr (RowSrc::active) v[i].minor = (decltype(v[i].minor))(row0 + rid[pos] + 1);
         }
     }
     const uint64_t lt_mask = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -256,12 +346,14 @@ __global__ __launch_bounds__(RP_WG) void k_rp_scatter(
 
 // Stable sort of n entries on the low `bits` bits of their keys (key0 over the
 // index arrays).  keys_final receives the sorted keys, minor_out and val_out the
-// entries' minor indices and values in sorted order.  Inputs and outputs must
-// not overlap.
+// entries' minor indices and values in sorted order.  The minor indices come
+// from minor_in, or, when rows_ptr is given (m_rows < 2^31 rows, one-based
+// pointer over the n entries in order), are the rows the entries lie in.
+// Inputs and outputs must not overlap.
 template <typename K, typename T, typename I>
 int radix_sort_payload(hipStream_t s, int64_t n, int bits, KeyFromIdx<K, I> key0,
                        const I* minor_in, const T* val_in, K* keys_final, I* minor_out,
-                       T* val_out) {
+                       T* val_out, const I* rows_ptr = nullptr, int64_t m_rows = 0) {
     using P = MinorVal<T, I>;
     if (n <= 0) return GSA_OK;
     GSA_CHECK(bits >= 0 && bits <= (int)(8 * sizeof(K)), GSA_ERR_INVALID,
@@ -279,7 +371,16 @@ int radix_sort_payload(hipStream_t s, int64_t n, int bits, KeyFromIdx<K, I> key0
         GSA_TRY(kB.alloc(sizeof(K) * n));
         GSA_TRY(pB.alloc(sizeof(P) * n));
     }
-    const PaySplit<T, I> pay0{minor_in, val_in};
+    GSA_CHECK(rows_ptr == nullptr || (m_rows > 0 && m_rows < ((int64_t)1 << 31)), GSA_ERR_INVALID,
+              "radix sort: rows from a pointer need 0 < m < 2^31");
+    Scratch trows(s);
+    if (rows_ptr) {
+        GSA_TRY(trows.alloc(sizeof(int64_t) * 2 * ntiles));
+        hipLaunchKernelGGL((k_tile_rows<I>), dim3((unsigned)cdiv(ntiles, 256)), dim3(256), 0, s,
+                           m_rows, n, rows_ptr, ntiles, trows.as<int64_t>());
+    }
+    const TileRows<I> rsrc{rows_ptr, trows.as<int64_t>()};
+    const PaySplit<T, I> pay0{rows_ptr ? nullptr : minor_in, val_in};
     const SinkSplit<T, I> sink_out{minor_out, val_out};
     const unsigned grid = (unsigned)xcd_tile_grid(ntiles);
     const K* kin = nullptr;
@@ -303,22 +404,32 @@ int radix_sort_payload(hipStream_t s, int64_t n, int bits, KeyFromIdx<K, I> key0
                                         ((int64_t)1 << nb) * ntiles, 0));
         const int64_t* ic = incl.as<int64_t>();
         const int64_t* hc = hist.as<int64_t>();
-        if (p == 0 && last)
+        if (p == 0 && last && rows_ptr)
+            hipLaunchKernelGGL((k_rp_scatter<K, P, KeyFromIdx<K, I>, PaySplit<T, I>, SinkSplit<T, I>,
+                                             TileRows<I>>),
+                               dim3(grid), dim3(RP_WG), 0, s, key0, pay0, n, shift, nb, ic, hc,
+                               ntiles, kout, sink_out, rsrc);
+        else if (p == 0 && rows_ptr)
+            hipLaunchKernelGGL((k_rp_scatter<K, P, KeyFromIdx<K, I>, PaySplit<T, I>, SinkArr<P>,
+                                             TileRows<I>>),
+                               dim3(grid), dim3(RP_WG), 0, s, key0, pay0, n, shift, nb, ic, hc,
+                               ntiles, kout, sink_mid, rsrc);
+        else if (p == 0 && last)
             hipLaunchKernelGGL((k_rp_scatter<K, P, KeyFromIdx<K, I>, PaySplit<T, I>, SinkSplit<T, I>>),
                                dim3(grid), dim3(RP_WG), 0, s, key0, pay0, n, shift, nb, ic, hc,
-                               ntiles, kout, sink_out);
+                               ntiles, kout, sink_out, NoRows{});
         else if (p == 0)
             hipLaunchKernelGGL((k_rp_scatter<K, P, KeyFromIdx<K, I>, PaySplit<T, I>, SinkArr<P>>),
                                dim3(grid), dim3(RP_WG), 0, s, key0, pay0, n, shift, nb, ic, hc,
-                               ntiles, kout, sink_mid);
+                               ntiles, kout, sink_mid, NoRows{});
         else if (last)
             hipLaunchKernelGGL((k_rp_scatter<K, P, KeyArr<K>, PayArr<P>, SinkSplit<T, I>>),
                                dim3(grid), dim3(RP_WG), 0, s, kprev, pprev, n, shift, nb, ic, hc,
-                               ntiles, kout, sink_out);
+                               ntiles, kout, sink_out, NoRows{});
         else
             hipLaunchKernelGGL((k_rp_scatter<K, P, KeyArr<K>, PayArr<P>, SinkArr<P>>), dim3(grid),
                                dim3(RP_WG), 0, s, kprev, pprev, n, shift, nb, ic, hc, ntiles, kout,
-                               sink_mid);
+                               sink_mid, NoRows{});
         GSA_HIP(hipGetLastError());
         kin = kout;
         pin = pout;
