@@ -728,19 +728,6 @@ __global__ __launch_bounds__(256) void k_kron_csr_fill(
 }
 
 // ------------------------------------------------------------ SpGEMM ------
-template <typename I>
-__global__ void k_sg_count(int64_t m, const I* __restrict__ pA, const I* __restrict__ cA,
-                           const I* __restrict__ pB, int64_t kdim, int64_t* __restrict__ cnt) {
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < m;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        int64_t c = 0;
-        for (int64_t a = (int64_t)pA[r] - 1; a < (int64_t)pA[r + 1] - 1; ++a) {
-            int64_t k = dclamp((int64_t)cA[a] - 1, 0, kdim - 1);
-            c += (int64_t)pB[k + 1] - pB[k];
-        }
-        cnt[r + 1] = c;
-    }
-}
 
 template <typename T, typename I, typename V>
 __global__ void k_sg_expand(int64_t m, int64_t n, int64_t kdim, const I* __restrict__ pA,
@@ -978,7 +965,7 @@ __global__ void k_sg_compact(int64_t m, const int64_t* __restrict__ pp,
 }
 
 // Launch the grouped row kernels for the size classes (0,16], (16,32], (32,64].
-// Launch the classes whose bit is set in `classes` (k_sg_flags): each launch
+// Launch the classes whose bit is set in `classes` (k_sg_count_flags): each launch
 // covers every row, so an empty class would still cost a wave per row group.
 template <bool NUM, typename T, typename I, bool STAGE = false>
 static void sg_rows_small(hipStream_t s, int64_t m, int64_t n, int64_t kdim, int64_t nnzA,
@@ -1085,12 +1072,15 @@ __global__ __launch_bounds__(ROW_WG) void k_sg_row_block(
     }
 }
 
-// Products over SG_WAVE_P per row -> the row list of the workgroup kernel.
-// out[0] gets the largest product count of a row, out[1] a bit per size class
-// of the grouped row kernels that holds a row: 1 for (0,16], 2 for (16,32],
-// 4 for (32,64].
-__global__ void k_sg_flags(int64_t m, const int64_t* __restrict__ pp, int64_t* __restrict__ f,
-                           unsigned long long* __restrict__ out) {
+// Products of each row (cnt[r + 1]), the flags of rows over SG_WAVE_P products
+// (f[r], for the row list of the workgroup kernel), out[0] the largest count
+// and out[1] a bit per size class of the grouped row kernels that holds a row:
+// 1 for (0,16], 2 for (16,32], 4 for (32,64].
+template <typename I>
+__global__ void k_sg_count_flags(int64_t m, const I* __restrict__ pA, const I* __restrict__ cA,
+                                 const I* __restrict__ pB, int64_t kdim,
+                                 int64_t* __restrict__ cnt, int64_t* __restrict__ f,
+                                 unsigned long long* __restrict__ out) {
     __shared__ unsigned long long bmax;
     __shared__ unsigned int bcls;
     if (threadIdx.x == 0) {
@@ -1100,7 +1090,12 @@ __global__ void k_sg_flags(int64_t m, const int64_t* __restrict__ pp, int64_t* _
     __syncthreads();
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
     if (r < m) {
-        const int64_t P = sg_products(pp, r);
+        int64_t P = 0;
+        for (int64_t a = (int64_t)pA[r] - 1; a < (int64_t)pA[r + 1] - 1; ++a) {
+            const int64_t k = dclamp((int64_t)cA[a] - 1, 0, kdim - 1);
+            P += (int64_t)pB[k + 1] - pB[k];
+        }
+        cnt[r + 1] = P;
         f[r] = P > SG_WAVE_P;
         atomicMax(&bmax, (unsigned long long)P);
         const unsigned cls = P <= 0 ? 0u : (P <= 16 ? 1u : (P <= 32 ? 2u : (P <= 64 ? 4u : 0u)));
@@ -1588,33 +1583,40 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
         }
         Scratch pptr(s);
         if ((r = pptr.alloc(sizeof(int64_t) * (m + 1))) != GSA_OK) return fail(r);
-        hipLaunchKernelGGL(k_sg_count<I>, dim3(grid_for(m, 256)), dim3(256), 0, s, m, (const I*)rowptrA,
-                           (const I*)colA, (const I*)rowptrB, kdim, pptr.as<int64_t>());
-        int64_t P = 0;
-        if ((r = counts_to_ptr<int64_t>(s, pptr.as<int64_t>(), m, &P)) != GSA_OK) return fail(r);
-        c->aux[0] = P;
-        if (P == 0) { *ctx = c; return (int)GSA_OK; }
-        // row-local path when every row's products fit one workgroup's sort
+        // per-row products with the long-row flags, largest count and size classes,
+        // both scans, then one read-back of the four figures
+        Scratch fl(s), mx(s);
+        if ((r = fl.alloc(sizeof(int64_t) * (m > 0 ? m : 1))) != GSA_OK) return fail(r);
+        if ((r = mx.alloc(2 * sizeof(unsigned long long))) != GSA_OK) return fail(r);
+        if ((r = zero_fill(s, mx.p, 2 * sizeof(unsigned long long))) != GSA_OK) return fail(r);
+        if (m > 0)
+            hipLaunchKernelGGL((k_sg_count_flags<I>), dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s,
+                               m, (const I*)rowptrA, (const I*)colA, (const I*)rowptrB, kdim,
+                               pptr.as<int64_t>(), fl.as<int64_t>(), (unsigned long long*)mx.p);
+        if ((r = counts_to_ptr<int64_t>(s, pptr.as<int64_t>(), m, nullptr)) != GSA_OK) return fail(r);
+        if ((r = inclusive_scan<int64_t>(s, fl.as<int64_t>(), fl.as<int64_t>(), m, 0)) != GSA_OK)
+            return fail(r);
+        int64_t h[4] = {0, 0, 0, 1};  // largest count, long rows, classes, pointer end
         {
-            Scratch fl(s), mx(s);
-            if ((r = fl.alloc(sizeof(int64_t) * m)) != GSA_OK) return fail(r);
-            if ((r = mx.alloc(2 * sizeof(unsigned long long))) != GSA_OK) return fail(r);
-            if ((r = zero_fill(s, mx.p, 2 * sizeof(unsigned long long))) != GSA_OK) return fail(r);
-            hipLaunchKernelGGL(k_sg_flags, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m,
-                               (const int64_t*)pptr.as<int64_t>(), fl.as<int64_t>(),
-                               (unsigned long long*)mx.p);
-            if ((r = inclusive_scan<int64_t>(s, fl.as<int64_t>(), fl.as<int64_t>(), m, 0)) != GSA_OK)
-                return fail(r);
-            int64_t h[3] = {0, 0, 0};
             hipError_t e = hipMemcpyAsync(&h[0], mx.p, sizeof(int64_t), hipMemcpyDeviceToHost, s);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(&h[2], (const int64_t*)mx.p + 1, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess)
+            if (e == hipSuccess && m > 0)
                 e = hipMemcpyAsync(&h[1], fl.as<int64_t>() + (m - 1), sizeof(int64_t),
+                                   hipMemcpyDeviceToHost, s);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(&h[3], pptr.as<int64_t>() + m, sizeof(int64_t),
                                    hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) return fail(hip_fail(e, "gsa_spgemm_csr_begin"));
+        }
+        const int64_t P = h[3] - 1;
+        c->aux[0] = P;
+        if (P == 0) { *ctx = c; return (int)GSA_OK; }
+        // row-local path when every row's products fit one workgroup's sort
+        {
+            hipError_t e = hipSuccess;
             const int64_t maxP = h[0], nbig = h[1];
             const unsigned classes = (unsigned)h[2];
             if (maxP <= SG_BLOCK_P && n < (int64_t)1 << 40) {
