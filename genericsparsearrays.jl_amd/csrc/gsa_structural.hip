@@ -1453,17 +1453,22 @@ int gsa_coo_add_finish(gsa_ctx ctx, void* rowC, void* colC, void* valC) {
         if (ctx->aux[1] == 1) {  // row merge: stable sort by column, (row, value) carried
             const int64_t nz = ctx->nnz_out, m = ctx->m, n = ctx->n;
             if (nz == 0) return (int)GSA_OK;
+            // the rows come from the result's row pointer (the first pass finds them
+            // tile by tile), expanded first only for 2^31 rows or more
             Scratch rows(s), keys(s);
-            GSA_TRY(rows.alloc(sizeof(I) * nz));
-            GSA_TRY(expand_rows(s, ctx->itype, m, nz, ctx->buf[0], rows.p));
+            const bool from_ptr = m < ((int64_t)1 << 31);
+            if (!from_ptr) {
+                GSA_TRY(rows.alloc(sizeof(I) * nz));
+                GSA_TRY(expand_rows(s, ctx->itype, m, nz, ctx->buf[0], rows.p));
+            }
             const int bits = bits_for((uint64_t)(n - 1));
             auto run = [&](auto key_t) -> int {
                 using K = decltype(key_t);
                 GSA_TRY(keys.alloc(sizeof(K) * nz));
                 const KeyFromIdx<K, I> key0{(const I*)ctx->buf[1], nullptr, n, 1};
-                GSA_TRY((radix_sort_payload<K, T, I>(s, nz, bits, key0, (const I*)rows.p,
-                                                     (const T*)ctx->buf[2], keys.as<K>(),
-                                                     (I*)rowC, (T*)valC)));
+                GSA_TRY((radix_sort_payload<K, T, I>(
+                    s, nz, bits, key0, (const I*)rows.p, (const T*)ctx->buf[2], keys.as<K>(),
+                    (I*)rowC, (T*)valC, from_ptr ? (const I*)ctx->buf[0] : nullptr, m)));
                 hipLaunchKernelGGL((k_keys_to_idx<I, K>), dim3(grid_for(nz, 256)), dim3(256), 0, s,
                                    nz, (const K*)keys.as<K>(), (I*)colC);
                 GSA_HIP(hipGetLastError());
