@@ -105,6 +105,31 @@ def test_coo_to_csr_unsorted_duplicates_and_empty(dev):
     same(csr_tuple(G.GenericSparseMatrixCSR(Ac)), O.coo_to_csr(300, 200, rc, cc, vc))
 
 
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+def test_transposition_rows_across_tiles(dev, itype):
+    """CSR <-> CSC finds each entry's row from the row pointer, 2048 entries per
+    tile: empty rows (also in runs, and at tile starts), rows spanning several
+    tiles, rows of one entry.  Index arrays bit-exact with the oracle's route
+    through COO; the same rows through COO + of row-major operands."""
+    rng = np.random.default_rng(71)
+    lengths = rng.integers(0, 6, 20000)
+    lengths[rng.random(lengths.size) < 0.4] = 0
+    lengths[[100, 7000, 7001, 15000]] = [3000, 5000, 2048, 4096]
+    lengths[7002:7400] = 0
+    H = synth.csr_from_lengths(lengths, 6000, rng, "normal")
+    m, n = H.m, H.n
+    A = from_arrays(G.GenericSparseMatrixCSR, m, n, H.rowptr, H.colval, H.nzval, dev, itype)
+    exp = O.csr_to_csc(m, n, H.rowptr, H.colval, H.nzval)
+    Ac = G.GenericSparseMatrixCSC(A)
+    same(csc_tuple(Ac), exp)
+    same(csr_tuple(G.GenericSparseMatrixCSR(Ac)), (H.rowptr, H.colval, H.nzval))
+    Ao = G.GenericSparseMatrixCOO(A)
+    coo = coo_tuple(Ao)
+    half = (coo[0], coo[1], coo[2] * 0.5)
+    Bo = from_arrays(G.GenericSparseMatrixCOO, m, n, *half, dev, itype)
+    same(coo_tuple(Ao + Bo), O.coo_add(m, coo, half))
+
+
 def test_transposed_conversions(dev):
     H = synth.sprand(300, 200, 0.03, seed=21, values="normal")
     A = dev_csr(H)
