@@ -1314,7 +1314,35 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
             // each operand's storage order: by (col, row), by (row, col)
             const bool cmA = !(fl & 1), cmB = !(fl & 4);
             const bool rmA = !(fl & 2), rmB = !(fl & 8);
-            if (rmA && rmB && !(cmA && cmB)) {
+            if (cmA && cmB) {
+                // both operands in (col, row) order (COO from a CSC or a SparseMatrixCSC):
+                // the row merge with the roles of rows and columns exchanged gives the
+                // sum in column-major order directly.  Here the column pointers and the
+                // counts; the finish fills the caller's arrays (the operands are read
+                // again there) and expands the column pointer into their column indices.
+                for (int b = 0; b < 3; ++b)
+                    if ((r = ws_alloc(&c->buf[b], sizeof(I) * (n + 1), s)) != GSA_OK) return fail(r);
+                I* qA = (I*)c->buf[1];
+                I* qB = (I*)c->buf[2];
+                I* ptrC = (I*)c->buf[0];
+                hipLaunchKernelGGL((k_rows_ptr<I>), dim3(grid_for(nnzA + 1, 256)), dim3(256), 0, s,
+                                   nnzA, (const I*)colA, n, qA);
+                hipLaunchKernelGGL((k_rows_ptr<I>), dim3(grid_for(nnzB + 1, 256)), dim3(256), 0, s,
+                                   nnzB, (const I*)colB, n, qB);
+                hipLaunchKernelGGL((k_merge_rows<T, I, true, false>), dim3((unsigned)cdiv(n, MR_WG)),
+                                   dim3(MR_WG), 0, s, n, m, (const I*)qA, (const I*)rowA,
+                                   (const T*)valA, (const I*)qB, (const I*)rowB, (const T*)valB,
+                                   ptrC, (I*)nullptr, (T*)nullptr);
+                int64_t total = 0;
+                if ((r = counts_to_ptr<I>(s, ptrC, n, &total)) != GSA_OK) return fail(r);
+                c->in[0] = rowA; c->in[1] = valA; c->in[2] = rowB; c->in[3] = valB;
+                c->aux[1] = 2;  // column merge: the finish fills
+                c->nnz_out = total;
+                *nnzC = total;
+                *ctx = c;
+                return (int)GSA_OK;
+            }
+            if (rmA && rmB) {
                 // both operands in (row, col) order (COO made from a CSR): a merge of
                 // each row's entries, then (in the finish) one stable sort by column
                 // that carries each entry's (row, value) into the caller's arrays
@@ -1350,9 +1378,9 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
                 return (int)GSA_OK;
             }
             const uint64_t* skeys = nullptr;  // keys in column-major order
-            // the concatenation sorted stably by (col, row): a merge when both operands
-            // are already in that order (COO from CSC or from a SparseMatrixCSC), else
-            // the radix sort; positions are 32-bit when they fit
+            // mixed or unordered storage: the concatenation sorted stably by (col, row),
+            // a row-major operand by its column alone and then merged with the other,
+            // else the radix sort; positions are 32-bit when they fit
             auto order = [&](auto pos_t) -> int {
                 using V = decltype(pos_t);
                 GSA_TRY(keys.alloc(sizeof(uint64_t) * 2 * tot));
@@ -1362,12 +1390,6 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
                 hipLaunchKernelGGL((k_concat_keys<I, V>), dim3(grid_for(tot, 256)), dim3(256), 0,
                                    s, nnzA, (const I*)rowA, (const I*)colA, nnzB, (const I*)rowB,
                                    (const I*)colB, m, n, k0, perm.as<V>());
-                if (cmA && cmB) {
-                    GSA_TRY(merge_sorted_parts<V>(s, nnzA, tot, k0, k1, perm.as<V>()));
-                    GSA_HIP(hipGetLastError());
-                    skeys = k1;
-                    return GSA_OK;
-                }
                 if ((cmA || rmA) && (cmB || rmB) && n <= (int64_t)UINT32_MAX &&
                     tot < (int64_t)UINT32_MAX) {
                     // an operand in row-major order (a COO made from a CSR) becomes
@@ -1450,6 +1472,16 @@ int gsa_coo_add_finish(gsa_ctx ctx, void* rowC, void* colC, void* valC) {
         using T = decltype(tv);
         using I = decltype(ti);
         if (tot == 0) return (int)GSA_OK;
+        if (ctx->aux[1] == 2) {  // column merge: fill the rows and values, expand the columns
+            const int64_t nz = ctx->nnz_out, m = ctx->m, n = ctx->n;
+            if (nz == 0) return (int)GSA_OK;
+            hipLaunchKernelGGL((k_merge_rows<T, I, true, true>), dim3((unsigned)cdiv(n, MR_WG)),
+                               dim3(MR_WG), 0, s, n, m, (const I*)ctx->buf[1], (const I*)ctx->in[0],
+                               (const T*)ctx->in[1], (const I*)ctx->buf[2], (const I*)ctx->in[2],
+                               (const T*)ctx->in[3], (I*)ctx->buf[0], (I*)rowC, (T*)valC);
+            GSA_HIP(hipGetLastError());
+            return expand_rows(s, ctx->itype, n, nz, ctx->buf[0], colC);
+        }
         if (ctx->aux[1] == 1) {  // row merge: stable sort by column, (row, value) carried
             const int64_t nz = ctx->nnz_out, m = ctx->m, n = ctx->n;
             if (nz == 0) return (int)GSA_OK;
