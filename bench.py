@@ -612,7 +612,7 @@ def main():
             print(f"[bench] {tag} {ms2 * 1e3:.1f} us, {gbs2:.0f} GB/s", file=sys.stderr, flush=True)
     if want("views"):
         # the same matrix as CSC and COO (row-major from CSR, and column-major
-        # from CSC), f64/i64, through their row views; bytes as the survey's formulas
+        # from CSC), f64/i64, through their row views; algorithmic bytes of each format
         A = G.GenericSparseMatrixCSR(H.m, H.n, torch.as_tensor(H.rowptr).to(dev),
                                      torch.as_tensor(H.colval).to(dev),
                                      torch.as_tensor(H.nzval).to(dev), copy=False)
