@@ -769,7 +769,10 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(gbs, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4),
                      "traffic": measured_traffic(), "algorithmic_bytes": algo,
-                     "kernel": "k_csr_spmv<double,long>"},
+                     "kernel": "k_csr_spmv<double,long>",
+                     "timed": "HIP events around each mul!, i.e. k_csr_spmv and the short "
+                              "k_spmv_fold that adds the partial sums of rows crossing windows; "
+                              "the kernel's own average is in the rocprofv3 summary"},
         "cpu_baseline": cpu,
         "extra": extra,
     }
