@@ -93,6 +93,24 @@ __device__ __forceinline__ uint32_t lane_xor32(uint32_t v, int j) {
     const auto p = __builtin_amdgcn_permlane32_swap(v, v, false, false);  // j == 32
     return (lane & 32u) ? (uint32_t)p[0] : (uint32_t)p[1];
 }
+// Inclusive prefix sum of an int inside each group of G lanes (G = 16, 32, 64),
+// by data-parallel moves: shifts by 1, 2, 4, 8 inside each row of 16 lanes
+// (lanes before the row start add nothing), then lane 15 of each even row added
+// to the next row (row_bcast:15, 32- and 64-lane groups) and lane 31 to rows 2
+// and 3 (row_bcast:31, 64-lane groups).  Every lane of the wavefront must be
+// active.  The same sums as the __shfl_up ladder.
+template <int G>
+__device__ __forceinline__ int group_incl_sum(int v) {
+    static_assert(G == 16 || G == 32 || G == 64, "groups of 16, 32 or 64 lanes");
+    v += __builtin_amdgcn_update_dpp(0, v, 0x111, 0xF, 0xF, false);  // row_shr:1
+    v += __builtin_amdgcn_update_dpp(0, v, 0x112, 0xF, 0xF, false);  // row_shr:2
+    v += __builtin_amdgcn_update_dpp(0, v, 0x114, 0xF, 0xF, false);  // row_shr:4
+    v += __builtin_amdgcn_update_dpp(0, v, 0x118, 0xF, 0xF, false);  // row_shr:8
+    if constexpr (G >= 32) v += __builtin_amdgcn_update_dpp(0, v, 0x142, 0xA, 0xF, false);
+    if constexpr (G == 64) v += __builtin_amdgcn_update_dpp(0, v, 0x143, 0xC, 0xF, false);
+    return v;
+}
+
 template <typename T>
 __device__ __forceinline__ T lane_xor(T v, int j) {
     static_assert(sizeof(T) == 4 || sizeof(T) == 8, "lane_xor moves 32- or 64-bit values");

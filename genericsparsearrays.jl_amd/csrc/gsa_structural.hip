@@ -878,12 +878,9 @@ __global__ __launch_bounds__(64 * SG_WAVES) void k_sg_row_wave(
                 bs = dclamp((int64_t)pB[k] - 1, 0, nnzB);
                 bl = dclamp((int64_t)pB[k + 1] - 1, bs, nnzB) - bs;
             }
-            int incl = (int)bl;
-#pragma unroll
-            for (int o = 1; o < G; o <<= 1) {
-                const int y = __shfl_up(incl, o, G);
-                if (gl >= o) incl += y;
-            }
+            // inclusive sum over the group (every source lane lies in the group, and
+            // a group is active or inactive as a whole)
+            const int incl = group_incl_sum<G>((int)bl);
             const int chunkP = __shfl(incl, G - 1, G);
             const int q = gl - base;  // this lane's product inside the chunk
             int lo = 0;
