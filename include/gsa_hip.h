@@ -263,7 +263,10 @@ int gsa_csr_add_fill(void* stream, int vtype, int itype, int64_t m,
                      const void* rowptrC, void* colC, void* valC);
 
 /* COO + COO: concatenate, stable sort by column-major key, sum duplicate runs in
- * order, drop exact zeros.  Output is column-major sorted COO.
+ * order, drop exact zeros.  Output is column-major sorted COO.  begin returns
+ * nnzC and a context; the caller allocates the outputs and calls finish, which
+ * releases the context.  The operands must stay unchanged until finish (with
+ * both stored in column-major order, finish reads them again).
  * Replaces Base.:+(::COO, ::COO) src/matrix_coo/matrix_coo.jl:300-384
  * (kernel_mark_unique_coo!/kernel_compact_coo! matrix_coo_kernels.jl:260-310). */
 int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n,
