@@ -4,7 +4,8 @@
 // Work unit: a tile = (instance, [r0, r1)) from a device table built once per
 // batch.  A persistent grid of workgroups takes tiles from an atomic counter
 // (reset by a one-thread kernel launched just before, so the pair is
-// graph-capturable: no allocation, no memset).  Inside a tile each row gets
+// graph-capturable: no memset).  The counter is scratch of the call, so a plan
+// is read-only and may serve concurrent launches on several streams.  Inside a tile each row gets
 // LPR = next_pow2(k) lanes (one (row, kk) output per lane, summed left to
 // right); rows over BATCH_LONG entries are reduced by the whole workgroup.
 //
@@ -29,7 +30,6 @@ struct BatchPlan {
     gsa_batch_desc* d_desc = nullptr;
     int32_t* d_tiles = nullptr;   // 3 per tile: instance, r0, r1
     int64_t ntiles = 0;
-    int* d_counter = nullptr;
     int grid = 0;
 };
 
@@ -310,7 +310,6 @@ void batch_plan_release(gsa_plan_s* plan) {
     (void)hipDeviceSynchronize();
     (void)hipFree(bp->d_desc);
     (void)hipFree(bp->d_tiles);
-    (void)hipFree(bp->d_counter);
     delete bp;
     plan->batch = nullptr;
 }
@@ -356,7 +355,6 @@ int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
     bp->grid = cus * 8;
     hipError_t e = hipMalloc(&bp->d_desc, sizeof(gsa_batch_desc) * (count > 0 ? count : 1));
     if (e == hipSuccess) e = hipMalloc(&bp->d_tiles, sizeof(int32_t) * (tiles.size() + 3));
-    if (e == hipSuccess) e = hipMalloc(&bp->d_counter, sizeof(int));
     if (e == hipSuccess && count > 0)
         e = hipMemcpyAsync(bp->d_desc, host_desc, sizeof(gsa_batch_desc) * count,
                            hipMemcpyHostToDevice, s);
@@ -365,7 +363,7 @@ int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
                            hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);  // host vectors go out of scope
     if (e != hipSuccess) {
-        (void)hipFree(bp->d_desc); (void)hipFree(bp->d_tiles); (void)hipFree(bp->d_counter);
+        (void)hipFree(bp->d_desc); (void)hipFree(bp->d_tiles);
         delete bp;
         return hip_fail(e, "gsa_batch_plan_create");
     }
@@ -379,8 +377,11 @@ int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
 int gsa_batch_spmm(void* stream, gsa_plan plan, float alpha, float beta) {
     GSA_CHECK(plan && plan->kind == 1 && plan->batch, GSA_ERR_INVALID, "gsa_batch_spmm: not a batch plan");
     auto* bp = (BatchPlan*)plan->batch;
-    return launch_batch(as_stream(stream), bp->vtype, bp->itype, bp->d_desc, bp->d_tiles, bp->ntiles,
-                        bp->d_counter, alpha, beta, bp->grid);
+    hipStream_t s = as_stream(stream);
+    Scratch counter(s);  // this launch's tile counter
+    GSA_TRY(counter.alloc(sizeof(int)));
+    return launch_batch(s, bp->vtype, bp->itype, bp->d_desc, bp->d_tiles, bp->ntiles,
+                        counter.as<int>(), alpha, beta, bp->grid);
 }
 
 int gsa_batch_spmm_tables(void* stream, int vtype, int itype, const gsa_batch_desc* d_desc,

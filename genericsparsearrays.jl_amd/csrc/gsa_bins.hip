@@ -19,8 +19,9 @@
 //      written once (beta, alpha applied there).
 // Float32 products are accumulated in Float64.  The plan holds the per-tile
 // offsets, slots and destinations (from the row indices only), the binned
-// rows and products, and for CSC each entry's column; values and x are read
-// at every call.
+// rows, and for CSC each entry's column; values and x are read at every call,
+// and the binned products live in scratch of the call (the plan is read-only,
+// so concurrent calls may share it).
 #include "gsa_common.h"
 #include "gsa_rows.h"
 #include "gsa_scan.h"
@@ -49,7 +50,6 @@ struct BinPlan {
     uint16_t* slot = nullptr;          // nnz: staging slot of each entry in its tile
     void* dstq = nullptr;              // nnz: binned position of each staging slot (u32 or i64)
     bool dst32 = false;
-    void* prod = nullptr;              // nnz products (up to 8 bytes), by bin
     const void* src[2] = {nullptr, nullptr};
 };
 
@@ -280,7 +280,7 @@ __global__ __launch_bounds__(64 * BIN_SUM_WAVES) void k_bin_sum(
 
 void release_bin_plan(BinPlan* p) {
     if (!p) return;
-    for (void* q : {(void*)p->off, p->col, (void*)p->lrow, (void*)p->slot, p->dstq, p->prod})
+    for (void* q : {(void*)p->off, p->col, (void*)p->lrow, (void*)p->slot, p->dstq})
         if (q) (void)hipFree(q);
     delete p;
 }
@@ -303,7 +303,6 @@ int build_bin_plan(hipStream_t s, int itype, int64_t m, int64_t n, int64_t nnz,
         const int64_t cells = nbins * p->ntiles;
         GSA_HIP(hipMalloc(&p->off, sizeof(int64_t) * (cells + 1)));
         GSA_HIP(hipMalloc(&p->lrow, sizeof(uint16_t) * nnz));
-        GSA_HIP(hipMalloc(&p->prod, sizeof(double) * nnz));
         Scratch hist(s), incl(s);
         GSA_TRY(hist.alloc(sizeof(int64_t) * cells));
         GSA_TRY(incl.alloc(sizeof(int64_t) * cells));
@@ -386,6 +385,8 @@ int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* colidx,
                                      hipMemcpyDeviceToDevice, s));
             xv = xs.as<T>();
         }
+        Scratch prod(s);  // the binned products of this call
+        GSA_TRY(prod.alloc(sizeof(T) * p.nnz));
         const unsigned g1 = (unsigned)xcd_tile_grid(p.ntiles);
         auto products = [&](auto ci_t, auto d_t) {
             using CI = decltype(ci_t);
@@ -393,7 +394,7 @@ int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* colidx,
             const CI* ci = colidx ? (const CI*)colidx : (const CI*)p.col;
             hipLaunchKernelGGL((k_bin_products<T, CI, D>), dim3(g1), dim3(BIN_WG), 0, s, p.n,
                                p.nnz, ci, (const T*)nzval, xv, (const uint16_t*)p.slot,
-                               (const D*)p.dstq, p.ntiles, (T*)p.prod);
+                               (const D*)p.dstq, p.ntiles, prod.as<T>());
         };
         auto by_col = [&](auto d_t) {
             if (colidx || !p.col32) products(I{}, d_t);
@@ -403,7 +404,7 @@ int bin_spmv(hipStream_t s, const BinPlan& p, int vtype, const void* colidx,
         else by_col(int64_t{});
         hipLaunchKernelGGL(k_bin_sum<T>, dim3((unsigned)p.nbins), dim3(64 * BIN_SUM_WAVES), 0, s,
                            p.m, p.nnz, p.nbins, p.ntiles, (const int64_t*)p.off,
-                           (const uint16_t*)p.lrow, (const T*)p.prod, (T*)y, (T)alpha, (T)beta,
+                           (const uint16_t*)p.lrow, (const T*)prod.as<T>(), (T*)y, (T)alpha, (T)beta,
                            mode);
         GSA_HIP(hipGetLastError());
         return GSA_OK;

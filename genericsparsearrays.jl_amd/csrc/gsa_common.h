@@ -33,6 +33,12 @@ int ensure_init();
 // block released on a stream is reused only by later work on that stream.
 int ws_alloc(void** p, size_t bytes, hipStream_t s);
 void ws_free(void* p, hipStream_t s);
+// Free the blocks cached for stream s (its work must have drained).
+void ws_release_stream(hipStream_t s);
+// Bytes cached for stream s on any device.
+size_t ws_cached_bytes(hipStream_t s);
+// Scratch blocks held by live graphs (taken while a stream was being captured).
+int64_t ws_graph_blocks();
 
 // RAII scratch buffer; frees on the stream it was taken on.
 struct Scratch {
@@ -175,6 +181,9 @@ constexpr int SPMV_WIN = 576;
 // entries) that crosses it is finished, left to right, by its owner.
 constexpr int SPMV_WX = 32;
 
+// A plan is read-only once built: products keep their in-flight state (the
+// SpMV's head/tail partial sums, the bins' staged products) in scratch of the
+// call, so one plan may serve concurrent calls on several streams.
 struct CsrPlanData {
     int itype = 0;
     int64_t m = 0, nnz = 0;
@@ -183,10 +192,6 @@ struct CsrPlanData {
     int64_t* blk_pos = nullptr;   // device, PLAN_POS*nblocks
     int64_t nwin = 0;             // SpMV windows (SPMV_WIN entries each)
     int64_t* win_row = nullptr;   // device, nwin+1: first row starting in each window
-    // Partial sums of the SpMV in flight: the heads and tails of rows that
-    // cross window ends, 2*nwin values of up to 8 bytes.  A plan serves one
-    // SpMV at a time.
-    void* part = nullptr;
     // Optional launch order of the SpMV windows (gsa_plan_set_columns): slot
     // 4*b + wave of workgroup b holds a window index or -1.
     int64_t* order = nullptr;

@@ -55,11 +55,74 @@ struct WsBlock {
 std::mutex g_ws_mu;
 std::map<std::pair<int, hipStream_t>, std::vector<WsBlock>> g_ws_free;
 std::unordered_map<void*, std::pair<int, size_t>> g_ws_live;  // block -> (device, bytes)
-// Blocks taken while the stream was being captured into a graph: stream-ordered
-// allocations (graph nodes), released the same way and never cached.
-std::unordered_map<void*, int> g_ws_captured;
+// Blocks taken while the stream was being captured into a graph.  Each is plain
+// device memory owned by the graph being captured (a user object of that
+// graph), so the graph holds no allocation node and a replay costs nothing
+// extra; when the graph and its executables are gone the block moves to the
+// graveyard, freed at the next trim (after a device-wide synchronisation).
+// Fallback when the graph cannot take the block: stream-ordered allocation
+// (graph nodes), released the same way.
+struct GraphBlock {
+    int dev;
+    void* p;
+    size_t bytes;
+};
+std::unordered_map<void*, int> g_ws_captured;  // block -> 0: graph-owned, 1: allocation node
+std::vector<GraphBlock> g_ws_graveyard;
+size_t g_ws_graveyard_bytes = 0;
+int64_t g_ws_graph_live = 0;  // blocks held by live graphs
 size_t g_ws_cached = 0;
-constexpr size_t WS_CACHE_MAX = 32ull << 30;  // cached bytes kept at most
+constexpr size_t WS_CACHE_MAX = 32ull << 30;     // cached bytes kept at most
+constexpr size_t WS_GRAVEYARD_MAX = 1ull << 30;  // released graph blocks kept at most
+
+// User-object destructor (an internal runtime thread; no HIP calls here).
+void graph_block_release(void* arg) {
+    auto* b = static_cast<GraphBlock*>(arg);
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        g_ws_graveyard.push_back(*b);
+        g_ws_graveyard_bytes += b->bytes;
+        --g_ws_graph_live;
+    }
+    delete b;
+}
+
+// A block for the graph being captured on s, owned by that graph.
+hipError_t graph_block(void** p, size_t bytes, hipStream_t s, int dev) {
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamGetCaptureInfo_v2(s, &st, nullptr, &g, nullptr, nullptr);
+    if (e != hipSuccess || st != hipStreamCaptureStatusActive || !g)
+        return e != hipSuccess ? e : hipErrorUnknown;
+    // plain allocation while this thread captures: relaxed mode for the call
+    hipStreamCaptureMode mode = hipStreamCaptureModeRelaxed;
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
+    e = hipMalloc(p, bytes);
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return e;
+    }
+    auto* b = new GraphBlock{dev, *p, bytes};
+    hipUserObject_t obj = nullptr;
+    e = hipUserObjectCreate(&obj, b, graph_block_release, 1, hipUserObjectNoDestructorSync);
+    if (e == hipSuccess) {
+        {
+            std::lock_guard<std::mutex> lk(g_ws_mu);
+            ++g_ws_graph_live;
+        }
+        e = hipGraphRetainUserObject(g, obj, 1, hipGraphUserObjectMove);
+        if (e != hipSuccess) (void)hipUserObjectRelease(obj, 1);  // runs the destructor
+        return e;  // on failure the block sits in the graveyard
+    }
+    delete b;
+    mode = hipStreamCaptureModeRelaxed;
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
+    (void)hipFree(*p);
+    (void)hipThreadExchangeStreamCaptureMode(&mode);
+    *p = nullptr;
+    return e;
+}
 
 size_t ws_round(size_t b) {
     if (b < 256) b = 256;
@@ -77,6 +140,9 @@ void ws_trim() {
             for (auto& b : kv.second) out.push_back({kv.first.first, b.p});
         g_ws_free.clear();
         g_ws_cached = 0;
+        for (auto& b : g_ws_graveyard) out.push_back({b.dev, b.p});
+        g_ws_graveyard.clear();
+        g_ws_graveyard_bytes = 0;
     }
     if (out.empty()) return;
     int cur = 0;
@@ -97,15 +163,26 @@ int ws_alloc(void** p, size_t bytes, hipStream_t s) {
     (void)hipGetDevice(&dev);
     hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
     if (hipStreamIsCapturing(s, &cap) == hipSuccess && cap == hipStreamCaptureStatusActive) {
-        hipError_t e = hipMallocAsync(p, bytes, s);
-        if (e != hipSuccess) {
-            *p = nullptr;
-            return hip_fail(e, "hipMallocAsync(scratch)");
+        int kind = 0;
+        if (graph_block(p, bytes, s, dev) != hipSuccess) {
+            (void)hipGetLastError();
+            kind = 1;
+            hipError_t e = hipMallocAsync(p, bytes, s);
+            if (e != hipSuccess) {
+                *p = nullptr;
+                return hip_fail(e, "hipMallocAsync(scratch)");
+            }
         }
         std::lock_guard<std::mutex> lk(g_ws_mu);
-        g_ws_captured[*p] = dev;
+        g_ws_captured[*p] = kind;
         return GSA_OK;
     }
+    bool trim_graveyard = false;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        trim_graveyard = g_ws_graveyard_bytes > WS_GRAVEYARD_MAX;
+    }
+    if (trim_graveyard) ws_trim();
     {
         std::lock_guard<std::mutex> lk(g_ws_mu);
         auto it = g_ws_free.find({dev, s});
@@ -150,9 +227,10 @@ void ws_free(void* p, hipStream_t s) {
         std::lock_guard<std::mutex> lk(g_ws_mu);
         auto ct = g_ws_captured.find(p);
         if (ct != g_ws_captured.end()) {
+            const int kind = ct->second;
             g_ws_captured.erase(ct);
-            (void)hipFreeAsync(p, s);
-            return;
+            if (kind == 1) (void)hipFreeAsync(p, s);  // an allocation node: its free node
+            return;                                   // graph-owned: stays with the graph
         }
         auto it = g_ws_live.find(p);
         if (it == g_ws_live.end()) return;
@@ -164,6 +242,46 @@ void ws_free(void* p, hipStream_t s) {
         trim = g_ws_cached > WS_CACHE_MAX;
     }
     if (trim) ws_trim();
+}
+
+void ws_release_stream(hipStream_t s) {
+    std::vector<std::pair<int, void*>> out;
+    {
+        std::lock_guard<std::mutex> lk(g_ws_mu);
+        for (auto it = g_ws_free.begin(); it != g_ws_free.end();) {
+            if (it->first.second == s) {
+                for (auto& b : it->second) {
+                    out.push_back({it->first.first, b.p});
+                    g_ws_cached -= b.bytes;
+                }
+                it = g_ws_free.erase(it);
+            } else {
+                ++it;
+            }
+        }
+    }
+    if (out.empty()) return;
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto& q : out) {
+        (void)hipSetDevice(q.first);
+        (void)hipFree(q.second);
+    }
+    (void)hipSetDevice(cur);
+}
+
+int64_t ws_graph_blocks() {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    return g_ws_graph_live;
+}
+
+size_t ws_cached_bytes(hipStream_t s) {
+    std::lock_guard<std::mutex> lk(g_ws_mu);
+    size_t t = 0;
+    for (auto& kv : g_ws_free)
+        if (kv.first.second == s)
+            for (auto& b : kv.second) t += b.bytes;
+    return t;
 }
 
 }  // namespace gsa
@@ -219,9 +337,20 @@ int gsa_stream_create(void** stream) {
 }
 
 int gsa_stream_destroy(void* stream) {
-    if (stream) GSA_HIP(hipStreamDestroy(as_stream(stream)));
+    if (!stream) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    // The stream's cached scratch goes with it: wait for its queued work (which
+    // may still read those blocks), then free them, so neither the memory leaks
+    // nor a later stream that receives the same handle inherits blocks in use.
+    GSA_HIP(hipStreamSynchronize(s));
+    ws_release_stream(s);
+    GSA_HIP(hipStreamDestroy(s));
     return GSA_OK;
 }
+
+int64_t gsa_scratch_cached(void* stream) { return (int64_t)ws_cached_bytes(as_stream(stream)); }
+
+int64_t gsa_scratch_graph_blocks(void) { return ws_graph_blocks(); }
 
 int gsa_stream_sync(void* stream) {
     GSA_HIP(hipStreamSynchronize(as_stream(stream)));

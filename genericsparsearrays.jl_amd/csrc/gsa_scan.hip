@@ -189,15 +189,51 @@ int zero_fill(hipStream_t s, void* p, size_t bytes) {
 template <typename I>
 __global__ void k_set_one(I* p) { *p = 1; }
 
+// The 64-bit sum of m non-negative 32-bit counts (one atomic per workgroup).
+__global__ __launch_bounds__(256) void k_sum_counts32(const int32_t* __restrict__ c, int64_t m,
+                                                      unsigned long long* __restrict__ out) {
+    __shared__ unsigned long long part[4];
+    unsigned long long t = 0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < m;
+         i += (int64_t)gridDim.x * blockDim.x)
+        t += (unsigned long long)(uint32_t)c[i];
+    for (int off = 32; off >= 1; off >>= 1) t += __shfl_xor(t, off, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = t;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(out, part[0] + part[1] + part[2] + part[3]);
+}
+
 template <typename I>
 int counts_to_ptr(hipStream_t s, I* ptr, int64_t m, int64_t* total) {
     // ptr[1..m] hold counts; make ptr one-based pointers (ptr[0] = 1).
+    // With 32-bit indices and a total asked for, the counts are first summed in
+    // 64 bits: a result of 2^31 - 1 entries or more has no 32-bit pointer, and
+    // the call fails instead of handing back a wrapped one.
+    Scratch wide(s);
+    if constexpr (sizeof(I) == 4) {
+        if (total && m > 0) {
+            GSA_TRY(wide.alloc(sizeof(unsigned long long)));
+            GSA_TRY(zero_fill(s, wide.p, sizeof(unsigned long long)));
+            hipLaunchKernelGGL(k_sum_counts32, dim3(grid_for(m, 256)), dim3(256), 0, s,
+                               (const int32_t*)(ptr + 1), m, wide.as<unsigned long long>());
+            GSA_HIP(hipGetLastError());
+        }
+    }
     hipLaunchKernelGGL(k_set_one<I>, dim3(1), dim3(1), 0, s, ptr);
     if (m > 0) GSA_TRY(inclusive_scan<I>(s, ptr + 1, ptr + 1, m, (I)1));
     if (total) {
         I last = 1;
+        unsigned long long sum64 = 0;
         GSA_HIP(hipMemcpyAsync(&last, ptr + m, sizeof(I), hipMemcpyDeviceToHost, s));
+        if (wide.p)
+            GSA_HIP(hipMemcpyAsync(&sum64, wide.p, sizeof sum64, hipMemcpyDeviceToHost, s));
         GSA_HIP(hipStreamSynchronize(s));
+        if (wide.p && sum64 >= (unsigned long long)INT32_MAX) {
+            *total = (int64_t)sum64;
+            GSA_CHECK(false, GSA_ERR_INVALID,
+                      "the result has %llu stored entries: more than 32-bit indices can point to",
+                      sum64);
+        }
         *total = (int64_t)last - 1;
     }
     return GSA_OK;

@@ -34,6 +34,7 @@
 #include <cstdlib>
 #include <vector>
 
+
 namespace gsa {
 
 constexpr int WG = 256;
@@ -850,12 +851,10 @@ struct PlanUse {
         blk_row = br; blk_pos = bp; nblocks = cnt;
         return GSA_OK;
     }
-    // Head/tail partial sums of the SpMV windows: the plan's, or call scratch.
+    // Head/tail partial sums of the SpMV windows: scratch of this call, taken
+    // on its stream, so a plan is never written by a product and one plan
+    // serves any number of concurrent calls on any streams.
     int part_buffer(void** out) {
-        if (owner) {
-            *out = owner->part;
-            return GSA_OK;
-        }
         GSA_TRY(tmp_part.alloc(sizeof(double) * 2 * nwin));
         *out = tmp_part.p;
         return GSA_OK;
@@ -1025,10 +1024,9 @@ static int check_common(int opA, int opB, int64_t m, int64_t n, int64_t k, int64
 
 static int plan_release(CsrPlanData& d) {
     hipError_t e = hipSuccess, f;
-    for (void* q : {(void*)d.blk_row, d.part, (void*)d.order})
+    for (void* q : {(void*)d.blk_row, (void*)d.order})
         if (q && (f = hipFree(q)) != hipSuccess && e == hipSuccess) e = f;
     d.blk_row = d.blk_pos = d.win_row = nullptr;
-    d.part = nullptr;
     d.order = nullptr;
     d.nslot = 0;
     d.order_col = nullptr;
@@ -1059,7 +1057,6 @@ int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz, const void*
     p->csr.nwin = nnz / SPMV_WIN + 1;
     const int64_t words = (p->csr.nblocks + 1) + PLAN_POS * p->csr.nblocks + (p->csr.nwin + 1);
     hipError_t e = hipMalloc(&p->csr.blk_row, sizeof(int64_t) * words);
-    if (e == hipSuccess) e = hipMalloc(&p->csr.part, sizeof(double) * 2 * p->csr.nwin);
     if (e != hipSuccess) {
         (void)plan_release(p->csr);
         delete p;

@@ -1309,8 +1309,12 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
             if (e0 == hipSuccess) e0 = hipStreamSynchronize(s);
             if (e0 != hipSuccess) return fail(hip_fail(e0, "gsa_coo_add_begin"));
             // each operand's storage order: by (col, row), by (row, col)
-            const bool cmA = !(fl & 1), cmB = !(fl & 4);
-            const bool rmA = !(fl & 2), rmB = !(fl & 8);
+            // The merge paths build a pointer of the result in the index type; with
+            // 32-bit indices and 2^31 - 1 entries or more it could not hold the
+            // total, so such sums take the sort path (64-bit keys, no pointer).
+            const bool ptr_fits = sizeof(I) == 8 || tot < (int64_t)INT32_MAX;
+            const bool cmA = !(fl & 1) && ptr_fits, cmB = !(fl & 4) && ptr_fits;
+            const bool rmA = !(fl & 2) && ptr_fits, rmB = !(fl & 8) && ptr_fits;
             if (cmA && cmB) {
                 // both operands in (col, row) order (COO from a CSC or a SparseMatrixCSC):
                 // the row merge with the roles of rows and columns exchanged gives the
@@ -1656,8 +1660,9 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
             if (maxP <= SG_BLOCK_P && n < (int64_t)1 << 40) {
                 // keep the product pointer and the long-row list for the numeric phase
                 if ((r = ws_alloc(&c->buf[3], sizeof(int64_t) * (m + 1), s)) != GSA_OK) return fail(r);
-                GSA_HIP(hipMemcpyAsync(c->buf[3], pptr.p, sizeof(int64_t) * (m + 1),
-                                       hipMemcpyDeviceToDevice, s));
+                e = hipMemcpyAsync(c->buf[3], pptr.p, sizeof(int64_t) * (m + 1),
+                                   hipMemcpyDeviceToDevice, s);
+                if (e != hipSuccess) return fail(hip_fail(e, "gsa_spgemm_csr_begin"));
                 if (nbig > 0) {
                     if ((r = ws_alloc(&c->buf[4], sizeof(int64_t) * nbig, s)) != GSA_OK) return fail(r);
                     hipLaunchKernelGGL(k_sg_list, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m,

@@ -129,6 +129,10 @@ _SIGS = {
     "gsa_csr_to_bf16": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp],
 }
 _RESTYPES = {"gsa_last_error": ctypes.c_char_p}
+# functions returning something other than an int status: name -> (argtypes, restype)
+_OTHER = {"gsa_scratch_cached": ([_vp], ctypes.c_int64),
+          "gsa_scratch_graph_blocks": ([], ctypes.c_int64)}
+
 
 _lib = None
 _lock = threading.Lock()
@@ -136,7 +140,7 @@ _lock = threading.Lock()
 
 def exported_symbols():
     """Names this binding expects the library to export."""
-    return sorted(list(_SIGS) + list(_RESTYPES))
+    return sorted(list(_SIGS) + list(_RESTYPES) + list(_OTHER))
 
 
 def load(path: str | None = None):
@@ -158,6 +162,10 @@ def load(path: str | None = None):
         for name, res in _RESTYPES.items():
             fn = getattr(lib, name)
             fn.argtypes = []
+            fn.restype = res
+        for name, (args, res) in _OTHER.items():
+            fn = getattr(lib, name)
+            fn.argtypes = args
             fn.restype = res
         if path is None:
             _lib = lib

@@ -65,18 +65,30 @@ int gsa_free(void* ptr);
 int gsa_memcpy(void* dst, const void* src, int64_t bytes, int kind, void* stream);
 int gsa_memset(void* dst, int value, int64_t bytes, void* stream);
 int gsa_stream_create(void** stream);
+/* Waits for the stream's queued work, frees the scratch blocks cached for it,
+ * then destroys it. */
 int gsa_stream_destroy(void* stream);
 int gsa_stream_sync(void* stream);
+/* Bytes of scratch cached for `stream` (a host-side lookup; no device call). */
+int64_t gsa_scratch_cached(void* stream);
+/* Scratch blocks held by live HIP graphs.  A call captured into a graph takes
+ * its scratch as plain device memory owned by that graph (released when the
+ * graph and its executables are destroyed), so a replay runs no allocation. */
+int64_t gsa_scratch_graph_blocks(void);
 
 /* ========================== SpMV / SpMM (mul!) ============================== */
 /* Row-block analysis of a CSR (or, on its column pointer, CSC) matrix, computed
  * once and reused by every mul! on the same structure: the first row starting
  * in each block of entries, for the SpMV's 576-entry windows and the SpMM's
  * 2048-entry blocks.  `ptr` has length `m+1`.  Stream-ordered device work; the
- * plan is usable on `stream` immediately.  A plan also holds the partial sums
- * of the SpMV in flight (rows that cross window ends): calls that may run at
- * the same time, on different streams, need different plans.  No counterpart
- * in the reference, whose mul! analyses nothing (src/matrix_csr/matrix_csr.jl:145-211). */
+ * plan is usable on `stream` immediately.  Once built, a plan (of any kind) is
+ * read-only: every product keeps its in-flight state (partial sums of rows
+ * that cross window ends, binned products, tile counters) in scratch of its
+ * own call, so one plan may serve concurrent calls on any streams and host
+ * threads.  A plan describes the index arrays it was built on (matched by
+ * address): after an in-place write to those arrays, build a new plan.  No
+ * counterpart in the reference, whose mul! analyses nothing
+ * (src/matrix_csr/matrix_csr.jl:145-211). */
 int gsa_plan_create(void* stream, int itype, int64_t m, int64_t nnz,
                     const void* ptr, gsa_plan* plan);
 int gsa_plan_destroy(gsa_plan plan);

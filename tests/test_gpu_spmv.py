@@ -708,3 +708,116 @@ def test_gather_views_for_scatter_products(dev, k, itype):
            G.matrix.ptr(Ac.nzval), H.nnz, G.matrix.ptr(Bd), max(Bd.stride(1), H.n), L.GSA_OP_N,
            G.matrix.ptr(C2), H.m, 1.0, 0.0, None)
     assert_normwise(host(C1), host(C2), norm_bound(H.rowptr, H.colval, H.nzval, np.abs(Bn)), 1e-12)
+
+
+def test_one_plan_serves_two_streams_at_once(dev):
+    """A plan is read-only: the SpMVs of one matrix on two streams at the same
+    time (one cached plan; CSR windows with long rows crossing window ends, and
+    the CSC row bins) give the single-stream bits, as do two batched launches of
+    one batch plan.  The reference's mul! may be called from any task."""
+    from gsa_hip import batched
+    rng = np.random.default_rng(301)
+    H = synth.powerlaw_csr(300_000, 300_000, alpha=1.846, seed=302)
+    A = dev_csr(H)
+    Ac = G.GenericSparseMatrixCSC(A)
+    xs = [torch.as_tensor(rng.standard_normal(H.n), device=dev) for _ in range(2)]
+    ref = {(id(M), i): host(M @ xs[i]) for M in (A, Ac) for i in range(2)}
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(), torch.cuda.Stream()]
+    for M in (A, Ac):
+        outs = [[torch.empty(H.m, dtype=torch.float64, device=dev) for _ in range(6)]
+                for _ in range(2)]
+        for st in streams:
+            st.wait_stream(torch.cuda.current_stream())
+        for rep in range(6):  # interleaved launches keep both streams busy together
+            for i, st in enumerate(streams):
+                with torch.cuda.stream(st):
+                    G.mul_(outs[i][rep], M, xs[i])
+        torch.cuda.synchronize()
+        for i in range(2):
+            for rep in range(6):
+                assert np.array_equal(host(outs[i][rep]), ref[(id(M), i)]), (type(M), i, rep)
+    # one batch plan, two streams
+    items = []
+    for j in range(40):
+        Hj = synth.powerlaw_csr(2000 + 100 * j, 2000 + 100 * j, alpha=1.846, seed=400 + j,
+                                dtype=np.float32)
+        Aj = dev_csr(Hj, dtype=torch.float32, itype=torch.int32)
+        Bj = torch.as_tensor(rng.standard_normal((Hj.n, 4)).astype(np.float32), device=dev)
+        items.append((Aj, Bj, torch.empty((Hj.m, 4), dtype=torch.float32, device=dev).t()
+                      .contiguous().t()))
+    bt = batched.BatchedSpMM(items)
+    bt.run()
+    torch.cuda.synchronize()
+    want = [host(c).copy() for _, _, c in items]
+    for rep in range(4):
+        for st in streams:
+            with torch.cuda.stream(st):
+                bt.run()  # same values into the same outputs from both streams
+    torch.cuda.synchronize()
+    for (_, _, c), w in zip(items, want):
+        assert np.array_equal(host(c), w)
+
+
+def test_stream_destroy_releases_cached_scratch(dev):
+    """gsa_stream_destroy waits for the stream, then frees the scratch cached for
+    it: nothing stays cached under the dead handle."""
+    import ctypes
+    from gsa_hip import _lib as L
+    lib = L.load()
+    s = ctypes.c_void_p()
+    L.call("gsa_stream_create", ctypes.byref(s))
+    H = synth.powerlaw_csr(100_000, 100_000, alpha=1.846, seed=311)
+    A = dev_csr(H)
+    x = torch.ones(H.n, dtype=torch.float64, device=dev)
+    y = torch.zeros(H.m, dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+    p = G.matrix.ptr
+    L.call("gsa_spmm_csr", s, L.GSA_F64, L.GSA_I64, L.GSA_OP_N, H.m, H.n, 1, p(A.rowptr),
+           p(A.colval), p(A.nzval), H.nnz, p(x), H.n, L.GSA_OP_N, p(y), H.m, 1.0, 0.0, None)
+    L.call("gsa_stream_sync", s)
+    assert lib.gsa_scratch_cached(s) > 0  # the window table and partial sums
+    np.testing.assert_allclose(host(y), H.to_scipy() @ np.ones(H.n), rtol=1e-12, atol=1e-12)
+    L.call("gsa_stream_destroy", s)
+    assert lib.gsa_scratch_cached(s) == 0
+
+
+def test_graph_capture_owns_its_scratch(dev):
+    """A mul! captured into a HIP graph takes its scratch (window table, partial
+    sums) as device memory owned by the graph: the executable keeps it after the
+    capture's graph object is gone, replays equal the eager bits, and destroying
+    the executable releases it."""
+    from gsa_hip import _lib as L
+    lib = L.load()
+    H = synth.powerlaw_csr(200_000, 200_000, alpha=1.846, seed=321)
+    A = dev_csr(H)
+    x = torch.as_tensor(np.random.default_rng(322).standard_normal(H.n), device=dev)
+    want = host(A @ x)
+    y = torch.empty(H.m, dtype=torch.float64, device=dev)
+    A.plan()
+    torch.cuda.synchronize()
+    base = lib.gsa_scratch_graph_blocks()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g):
+        for _ in range(3):
+            G.mul_(y, A, x)
+    torch.cuda.synchronize()
+    held = lib.gsa_scratch_graph_blocks() - base
+    # the executable still owns the blocks (checked before any replay)
+    assert held >= 3, held
+    for rep in range(4):
+        y.fill_(float("nan"))
+        g.replay()
+        torch.cuda.synchronize()
+        assert np.array_equal(host(y), want), rep
+        assert np.array_equal(host(A @ x), want)  # eager work between replays
+    del g
+    import gc
+    gc.collect()
+    torch.cuda.synchronize()
+    import time
+    for _ in range(50):  # the release may run on a runtime thread
+        if lib.gsa_scratch_graph_blocks() == base:
+            break
+        time.sleep(0.02)
+    assert lib.gsa_scratch_graph_blocks() == base
