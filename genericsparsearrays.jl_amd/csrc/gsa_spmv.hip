@@ -32,8 +32,14 @@
 
 #include <algorithm>
 #include <cstdlib>
+#include <mutex>
 #include <vector>
 
+// The scratch of gsa_spmv_csr_noalloc: the window table and the head/tail
+// partial sums of one SpMV, in a fixed device array (one copy per GPU, at its
+// own address on each).  Host code never passes this name to a kernel: it is
+// a host-side stand-in; the device address comes from hipGetSymbolAddress.
+static __device__ __align__(256) uint8_t scratch[GSA_NOALLOC_SCRATCH_BYTES];
 
 namespace gsa {
 
@@ -1022,6 +1028,73 @@ static int check_common(int opA, int opB, int64_t m, int64_t n, int64_t k, int64
     return GSA_OK;
 }
 
+// ---- allocation-free SpMV ----------------------------------------------------
+// Device address of `scratch` on each GPU, looked up once per device (with that
+// device current) on its first allocation-free call.
+constexpr int NOALLOC_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[NOALLOC_DEVICES] = {};
+
+static int noalloc_scratch(uint8_t** out) {
+    int dev = 0;
+    GSA_HIP(hipGetDevice(&dev));
+    GSA_CHECK(dev >= 0 && dev < NOALLOC_DEVICES, GSA_ERR_INVALID,
+              "device ordinal %d beyond the %d the allocation-free path serves", dev,
+              NOALLOC_DEVICES);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void* p = nullptr;
+        GSA_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        GSA_CHECK(p != nullptr, GSA_ERR_HIP, "no device address for the allocation-free scratch");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = static_cast<uint8_t*>(g_noalloc_ptr[dev]);
+    return GSA_OK;
+}
+
+// Bytes of `scratch` one SpMV of nnz entries uses: the window table (nwin + 1
+// row indices), then head and tail partial sums (nwin values each).
+static int64_t noalloc_bytes(int64_t nnz, size_t value_bytes, int64_t* head_off) {
+    const int64_t nwin = nnz / SPMV_WIN + 1;
+    const int64_t table = (sizeof(int64_t) * (nwin + 1) + 255) / 256 * 256;
+    *head_off = table;
+    return table + 2 * (int64_t)value_bytes * nwin;
+}
+
+// The k = 1, no-plan path of gather_csr (window table built for the call, the
+// window kernel, the fold), with every buffer taken from `ws`.
+template <typename T, typename I>
+static int spmv_noalloc(hipStream_t s, int itype, int64_t m, int64_t n, const I* rowptr,
+                        const I* colval, const T* nzval, int64_t nnz, const T* x, T* y,
+                        double alpha, double beta, uint8_t* ws) {
+    if (m == 0) return GSA_OK;
+    if (n == 0 || nnz == 0) return apply_beta<T>(s, m, 1, y, m, beta);
+    int64_t head_off = 0;
+    const int64_t need = noalloc_bytes(nnz, sizeof(T), &head_off);
+    GSA_CHECK(need <= (int64_t)GSA_NOALLOC_SCRATCH_BYTES, GSA_ERR_INVALID,
+              "gsa_spmv_csr_noalloc: %lld bytes of scratch needed, %d available",
+              (long long)need, (int)GSA_NOALLOC_SCRATCH_BYTES);
+    const int64_t nwin = nnz / SPMV_WIN + 1;
+    int64_t* win_row = reinterpret_cast<int64_t*>(ws);
+    T* head = reinterpret_cast<T*>(ws + head_off);
+    T* tail = head + nwin;
+    // the table is cleared and filled here; every head and tail the fold reads is
+    // written by the window kernel of this call first (see window_tail)
+    GSA_TRY(build_csr_plan(s, itype, m, nnz, rowptr, win_row, nwin, nullptr, SPMV_WIN));
+    const int mode = beta_mode(beta);
+    const int64_t grid = cdiv(nwin, WAVES);
+    GSA_CHECK(grid <= (int64_t)UINT32_MAX, GSA_ERR_INVALID, "SpMV grid too large");
+    hipLaunchKernelGGL((k_csr_spmv<T, I>), dim3((unsigned)grid), dim3(64 * WAVES), 0, s, m, n, nnz,
+                       rowptr, colval, nzval, (const void*)nullptr, (const float*)nullptr, x, y,
+                       (T)alpha, (T)beta, mode, (const int64_t*)win_row, nwin,
+                       (const int64_t*)nullptr, head, tail);
+    hipLaunchKernelGGL((k_spmv_fold<T, I>), dim3((unsigned)cdiv(nwin, 256)), dim3(256), 0, s, m,
+                       nnz, rowptr, (const int64_t*)win_row, nwin, (const T*)head,
+                       (const T*)tail, (const float*)nullptr, y, (T)alpha, (T)beta, mode);
+    GSA_HIP(hipGetLastError());
+    return GSA_OK;
+}
+
 static int plan_release(CsrPlanData& d) {
     hipError_t e = hipSuccess, f;
     for (void* q : {(void*)d.blk_row, (void*)d.order})
@@ -1342,6 +1415,30 @@ int gsa_spmm_csr(void* stream, int vtype, int itype, int opA, int64_t m, int64_t
         return scatter_csr<T, I>(s, m, n, k, (const I*)rowptr, (const I*)colval, (const T*)nzval,
                                  nnz, (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
                                  plan, itype);
+    });
+}
+
+int gsa_spmv_csr_noalloc(void* stream, int vtype, int itype, int64_t m, int64_t n,
+                         const void* rowptr, const void* colval, const void* nzval, int64_t nnz,
+                         const void* x, void* y, double alpha, double beta) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && n >= 0 && nnz >= 0, GSA_ERR_INVALID,
+              "gsa_spmv_csr_noalloc: negative dimension (m=%lld n=%lld nnz=%lld)", (long long)m,
+              (long long)n, (long long)nnz);
+    GSA_CHECK(nnz <= GSA_NOALLOC_MAX_NNZ, GSA_ERR_INVALID,
+              "gsa_spmv_csr_noalloc: %lld stored entries, at most %lld", (long long)nnz,
+              (long long)GSA_NOALLOC_MAX_NNZ);
+    GSA_CHECK(m == 0 || (rowptr && y), GSA_ERR_INVALID, "gsa_spmv_csr_noalloc: null pointer");
+    GSA_CHECK(m == 0 || n == 0 || nnz == 0 || (colval && nzval && x), GSA_ERR_INVALID,
+              "gsa_spmv_csr_noalloc: null pointer");
+    uint8_t* ws = nullptr;
+    GSA_TRY(noalloc_scratch(&ws));
+    hipStream_t s = as_stream(stream);
+    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        return spmv_noalloc<T, I>(s, itype, m, n, (const I*)rowptr, (const I*)colval,
+                                  (const T*)nzval, nnz, (const T*)x, (T*)y, alpha, beta, ws);
     });
 }
 

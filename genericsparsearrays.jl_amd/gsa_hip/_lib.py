@@ -127,12 +127,15 @@ _SIGS = {
     "gsa_graph_launch": [_vp, _vp],
     "gsa_graph_destroy": [_vp],
     "gsa_csr_to_bf16": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp],
+    "gsa_spmv_csr_noalloc": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp,
+                             _f64, _f64],
 }
 _RESTYPES = {"gsa_last_error": ctypes.c_char_p}
 # functions returning something other than an int status: name -> (argtypes, restype)
 _OTHER = {"gsa_scratch_cached": ([_vp], ctypes.c_int64),
           "gsa_scratch_graph_blocks": ([], ctypes.c_int64)}
-
+GSA_NOALLOC_SCRATCH_BYTES = 4 << 20
+GSA_NOALLOC_MAX_NNZ = 100_000_000
 
 _lib = None
 _lock = threading.Lock()

@@ -216,6 +216,23 @@ int gsa_spmm_csr_ex(void* stream, int vtype, int itype, int64_t m, int64_t n, in
                     const void* B, int64_t ldb, int opB, void* C, int64_t ldc, int opC,
                     double alpha, double beta, gsa_plan plan);
 
+/* CSR SpMV y = beta*y + alpha*A*x (A m x n, `nnz` stored entries; x and y
+ * contiguous) that allocates nothing on the device: the same kernels and
+ * results as gsa_spmm_csr with opA = N, k = 1 and no plan, bit for bit, with
+ * the per-call window table and partial sums kept in a fixed device array of
+ * GSA_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends) instead of
+ * allocated scratch.  nnz above GSA_NOALLOC_MAX_NNZ is rejected with
+ * GSA_ERR_INVALID before any device work.  The array is shared by every call
+ * on one GPU: calls that overlap on one GPU (several streams or host threads
+ * at once) race and must be serialised by the caller.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc). */
+#define GSA_NOALLOC_SCRATCH_BYTES (4 << 20)
+#define GSA_NOALLOC_MAX_NNZ 100000000LL
+int gsa_spmv_csr_noalloc(void* stream, int vtype, int itype, int64_t m, int64_t n,
+                         const void* rowptr, const void* colval, const void* nzval, int64_t nnz,
+                         const void* x, void* y, double alpha, double beta);
+
 /* BF16-stored CSR SpMV/SpMM with a Float32 scale per row, Float32 B/C and
  * Float32 accumulation (TwoSum-compensated on rows reduced across lanes,
  * which include every row over 256 entries):
