@@ -5,9 +5,102 @@
 #include <stddef.h>
 #include <string>
 #include <type_traits>
+#include <cstring>
 #include "../../include/gsa_hip.h"
 
 namespace gsa {
+
+// ------------------------------------------------------------ value types --
+// Complex values in Julia's layout (re, im).  Arithmetic is Julia's
+// (base/complex.jl): sums and differences by parts; the product
+// (a.re*b.re - a.im*b.im, a.re*b.im + a.im*b.re), every operation rounded on
+// its own (the library is built without FMA contraction).
+template <typename R>
+struct cplx {
+    R re, im;
+    cplx() = default;
+    __host__ __device__ constexpr cplx(R r, R i) : re(r), im(i) {}
+    __host__ __device__ constexpr explicit cplx(int z) : re((R)z), im((R)0) {}
+};
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) {
+    return cplx<R>(a.re + b.re, a.im + b.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) {
+    return cplx<R>(a.re - b.re, a.im - b.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> operator-(cplx<R> a) {
+    return cplx<R>(-a.re, -a.im);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b) {
+    const R rr = a.re * b.re, ii = a.im * b.im, ri = a.re * b.im, ir = a.im * b.re;
+    return cplx<R>(rr - ii, ri + ir);
+}
+template <typename R>
+__host__ __device__ __forceinline__ bool operator==(cplx<R> a, cplx<R> b) {
+    return a.re == b.re && a.im == b.im;
+}
+template <typename R>
+__host__ __device__ __forceinline__ bool operator!=(cplx<R> a, cplx<R> b) {
+    return !(a == b);
+}
+template <typename R>
+__host__ __device__ __forceinline__ cplx<R>& operator+=(cplx<R>& a, cplx<R> b) {
+    a = a + b;
+    return a;
+}
+
+// Traits of the arithmetic types: double, float, cplx<double>, cplx<float>, and
+// the integer value types, held as uint64_t / uint32_t so that sums and
+// products wrap modulo 2^64 / 2^32 exactly as Julia's Int64 / Int32 do (the
+// bit patterns of two's-complement and unsigned arithmetic agree).
+template <typename T>
+struct VT {
+    static constexpr bool cx = false, f32 = false, integer = false;
+};
+template <>
+struct VT<float> {
+    static constexpr bool cx = false, f32 = true, integer = false;
+};
+template <typename R>
+struct VT<cplx<R>> {
+    static constexpr bool cx = true, f32 = false, integer = false;
+};
+template <>
+struct VT<uint32_t> {
+    static constexpr bool cx = false, f32 = false, integer = true;
+};
+template <>
+struct VT<uint64_t> {
+    static constexpr bool cx = false, f32 = false, integer = true;
+};
+
+template <typename T>
+__host__ __device__ __forceinline__ T conj_v(T v) {
+    if constexpr (VT<T>::cx) return T(v.re, -v.im);
+    else return v;
+}
+template <typename T>
+__host__ __device__ __forceinline__ T conj_if(T v, bool c) {
+    if constexpr (VT<T>::cx) return c ? conj_v(v) : v;
+    else return v;
+}
+
+// v * s for a scalar s (alpha, beta): a complex scalar whose imaginary part is
+// zero multiplies the parts separately, as Julia's Complex * Real does (so 1
+// leaves every value, Inf and NaN included, unchanged).
+template <typename T>
+__host__ __device__ __forceinline__ T mul_scalar(T v, T s) {
+    if constexpr (VT<T>::cx) {
+        if (s.im == 0) return T(v.re * s.re, v.im * s.re);
+        return v * s;
+    } else {
+        return v * s;
+    }
+}
 
 // ---------------------------------------------------------------- errors ----
 void set_error(int code, const char* fmt, ...);
@@ -119,14 +212,50 @@ __device__ __forceinline__ int group_incl_sum(int v) {
 
 template <typename T>
 __device__ __forceinline__ T lane_xor(T v, int j) {
-    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "lane_xor moves 32- or 64-bit values");
-    if constexpr (sizeof(T) == 8) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16,
+                  "lane_xor moves 32-, 64- or 128-bit values");
+    if constexpr (VT<T>::cx) {
+        return T(lane_xor(v.re, j), lane_xor(v.im, j));
+    } else if constexpr (sizeof(T) == 8) {
         const uint64_t u = __builtin_bit_cast(uint64_t, v);
         const uint64_t lo = lane_xor32((uint32_t)u, j);
         const uint64_t hi = lane_xor32((uint32_t)(u >> 32), j);
         return __builtin_bit_cast(T, (hi << 32) | lo);
     } else {
         return __builtin_bit_cast(T, lane_xor32(__builtin_bit_cast(uint32_t, v), j));
+    }
+}
+
+// Shuffles of any value type (complex values move part by part).
+template <typename T>
+__device__ __forceinline__ T shfl_v(T v, int src) {
+    if constexpr (VT<T>::cx) return T(shfl_v(v.re, src), shfl_v(v.im, src));
+    else return __shfl(v, src, 64);
+}
+template <typename T>
+__device__ __forceinline__ T shfl_up_v(T v, int o) {
+    if constexpr (VT<T>::cx) return T(shfl_up_v(v.re, o), shfl_up_v(v.im, o));
+    else return __shfl_up(v, o, 64);
+}
+template <typename T>
+__device__ __forceinline__ T shfl_down_v(T v, int o) {
+    if constexpr (VT<T>::cx) return T(shfl_down_v(v.re, o), shfl_down_v(v.im, o));
+    else return __shfl_down(v, o, 64);
+}
+
+// Atomic add of any value type (complex: one atomic per part; integers wrap).
+template <typename T>
+__device__ __forceinline__ void atomic_add_v(T* p, T v) {
+    if constexpr (VT<T>::cx) {
+        auto* q = reinterpret_cast<decltype(v.re)*>(p);
+        unsafeAtomicAdd(q, v.re);
+        unsafeAtomicAdd(q + 1, v.im);
+    } else if constexpr (std::is_same_v<T, uint64_t>) {
+        atomicAdd(reinterpret_cast<unsigned long long*>(p), (unsigned long long)v);
+    } else if constexpr (std::is_same_v<T, uint32_t>) {
+        atomicAdd(reinterpret_cast<unsigned int*>(p), (unsigned int)v);
+    } else {
+        unsafeAtomicAdd(p, v);
     }
 }
 
@@ -161,6 +290,72 @@ int dispatch_vi(int vtype, int itype, F&& f) {
     return dispatch_v(vtype, [&](auto tv) {
         return dispatch_i(itype, [&](auto ti) { return f(tv, ti); });
     });
+}
+// Every arithmetic value type: Float64, Float32, ComplexF64, ComplexF32, Int64,
+// Int32 (the integers as wrapping unsigned arithmetic of the same width).
+template <typename F>
+int dispatch_va(int vtype, F&& f) {
+    switch (vtype) {
+        case GSA_F64: return f(double{});
+        case GSA_F32: return f(float{});
+        case GSA_C128: return f(cplx<double>{});
+        case GSA_C64: return f(cplx<float>{});
+        case GSA_VI64: return f(uint64_t{});
+        case GSA_VI32: return f(uint32_t{});
+        default: set_error(GSA_ERR_UNSUPPORTED, "unsupported value type %d", vtype);
+                 return GSA_ERR_UNSUPPORTED;
+    }
+}
+template <typename F>
+int dispatch_vai(int vtype, int itype, F&& f) {
+    return dispatch_va(vtype, [&](auto tv) {
+        return dispatch_i(itype, [&](auto ti) { return f(tv, ti); });
+    });
+}
+// Values that are only moved (conversions): a bit pattern of the value's width.
+struct alignas(16) bits128 {
+    uint64_t lo, hi;
+};
+template <typename F>
+int dispatch_vbits(int vtype, F&& f) {
+    switch (vtype) {
+        case GSA_F64: case GSA_C64: case GSA_VI64: return f(uint64_t{});
+        case GSA_F32: case GSA_VI32: return f(uint32_t{});
+        case GSA_C128: return f(bits128{});
+        default: set_error(GSA_ERR_UNSUPPORTED, "unsupported value type %d", vtype);
+                 return GSA_ERR_UNSUPPORTED;
+    }
+}
+template <typename F>
+int dispatch_vbi(int vtype, int itype, F&& f) {
+    return dispatch_vbits(vtype, [&](auto tv) {
+        return dispatch_i(itype, [&](auto ti) { return f(tv, ti); });
+    });
+}
+inline bool vtype_real(int vtype) { return vtype == GSA_F64 || vtype == GSA_F32; }
+inline size_t vtype_bytes(int vtype) {
+    switch (vtype) {
+        case GSA_F64: case GSA_C64: case GSA_VI64: return 8;
+        case GSA_F32: case GSA_VI32: return 4;
+        case GSA_C128: return 16;
+        case GSA_BF16: return 2;
+        default: return 0;
+    }
+}
+// A host scalar of the value type (NULL: `def`).
+template <typename T>
+inline T host_scalar(const void* p, T def) {
+    if (!p) return def;
+    T v;
+    memcpy(&v, p, sizeof(T));
+    return v;
+}
+// beta mode: 0 = fill with zero, 1 = keep C, 2 = scale C.
+template <typename T>
+inline int beta_mode_of(T beta) {
+    if (beta == T(0)) return 0;
+    if (beta == T(1)) return 1;
+    return 2;
 }
 
 // ------------------------------------------------------- plan object ----

@@ -308,7 +308,7 @@ int gsa_csr_to_coo(void* stream, int vtype, int itype, int64_t m, int64_t nnz, c
     GSA_TRY(ensure_init());
     GSA_CHECK(m >= 0 && nnz >= 0, GSA_ERR_INVALID, "gsa_csr_to_coo: negative size");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vbi(vtype, itype, [&](auto tv, auto ti) {  // values moved as bits
         using T = decltype(tv);
         using I = decltype(ti);
         if (nnz == 0) return (int)GSA_OK;
@@ -327,7 +327,7 @@ int gsa_coo_to_csr(void* stream, int vtype, int itype, int64_t m, int64_t n, int
     GSA_CHECK(m == 0 || n == 0 || (uint64_t)m <= UINT64_MAX / (uint64_t)n, GSA_ERR_INVALID,
               "gsa_coo_to_csr: m*n overflows 64-bit keys");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vbi(vtype, itype, [&](auto tv, auto ti) {  // values moved as bits
         using T = decltype(tv);
         using I = decltype(ti);
         return sort_to_compressed<T, I>(s, m, n, nnz, (const I*)rowind, (const I*)colind,
@@ -344,7 +344,7 @@ int gsa_csr_to_csc(void* stream, int vtype, int itype, int64_t m, int64_t n, int
     GSA_CHECK(nnz == 0 || (m > 0 && n > 0), GSA_ERR_INVALID,
               "gsa_csr_to_csc: stored entries in a matrix with a zero dimension");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vbi(vtype, itype, [&](auto tv, auto ti) {  // values moved as bits
         using T = decltype(tv);
         using I = decltype(ti);
         // stable sort by column only, each entry's (row, value) carried; the first

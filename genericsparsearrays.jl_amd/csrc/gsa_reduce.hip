@@ -5,37 +5,58 @@
 // with their column indices; CSC: columns with their row indices).  The
 // diagonal test (index == major) and the symmetry test read the same in both.
 //
-// Sums (tr, sum, dot) are accumulated in Float64 with a fixed assignment of
-// entries to workgroups and a fixed order inside each, then one workgroup adds
-// the partials in order: the result does not change from run to run.  The
+// Sums (tr, sum, dot) are accumulated in Float64 (real values), ComplexF64
+// (complex values) or 64-bit wrapping integers (integer values, sign-extended:
+// the low 32 bits are Int32's wrapping sum), with a fixed assignment of entries
+// to workgroups and a fixed order inside each, then one workgroup adds the
+// partials in order: the result does not change from run to run.  The
 // reference adds with @atomic (tr) or per work-item partials (dot,
 // src/matrix_csr/matrix_csr_kernels.jl:47-131), so its order is undefined.
 #include "gsa_common.h"
 #include "gsa_rows.h"
 #include "gsa_scan.h"
 
+#include <cmath>
+
 namespace gsa {
 
 constexpr int RD_PARTS = 1024;  // workgroups (partial sums) of the entry-parallel reductions
 
+// Accumulator of each value type: Float64 for real values, ComplexF64 for
+// complex values, wrapping 64-bit integers for integer values.
+template <typename T> struct RAcc { using A = double; };
+template <typename R> struct RAcc<cplx<R>> { using A = cplx<double>; };
+template <> struct RAcc<uint32_t> { using A = uint64_t; };
+template <> struct RAcc<uint64_t> { using A = uint64_t; };
+
+template <typename T>
+__device__ __forceinline__ typename RAcc<T>::A to_acc(T v) {
+    if constexpr (VT<T>::cx) return cplx<double>((double)v.re, (double)v.im);
+    else if constexpr (std::is_same_v<T, uint32_t>) return (uint64_t)(int64_t)(int32_t)v;
+    else if constexpr (std::is_same_v<T, uint64_t>) return v;
+    else return (double)v;
+}
+
 // Sum over the workgroup's 256 lanes in a fixed order; lane 0 gets the total.
-__device__ __forceinline__ double wg_sum_fixed(double v, double* sh) {
+template <typename A>
+__device__ __forceinline__ A wg_sum_fixed(A v, A* sh) {
 #pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    for (int o = 32; o >= 1; o >>= 1) v = v + lane_xor(v, o);
     if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
     __syncthreads();
-    double t = 0.0;
+    A t = A(0);
     if (threadIdx.x == 0)
-        for (int w = 0; w < ROW_WG / 64; ++w) t += sh[w];
+        for (int w = 0; w < ROW_WG / 64; ++w) t = t + sh[w];
     __syncthreads();
     return t;
 }
 
-__global__ void k_sum_parts(int64_t n, const double* __restrict__ part, double* __restrict__ out) {
-    __shared__ double sh[ROW_WG / 64];
-    double acc = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += ROW_WG) acc += part[i];
-    const double t = wg_sum_fixed(acc, sh);
+template <typename A>
+__global__ void k_sum_parts(int64_t n, const A* __restrict__ part, A* __restrict__ out) {
+    __shared__ A sh[ROW_WG / 64];
+    A acc = A(0);
+    for (int64_t i = threadIdx.x; i < n; i += ROW_WG) acc = acc + part[i];
+    const A t = wg_sum_fixed(acc, sh);
     if (threadIdx.x == 0) *out = t;
 }
 
@@ -48,19 +69,20 @@ __global__ __launch_bounds__(ROW_WG) void k_trace_ptr(int64_t n, int64_t nnz,
                                                       const I* __restrict__ idx,
                                                       const T* __restrict__ val,
                                                       const int64_t* __restrict__ blk_row,
-                                                      double* __restrict__ part) {
-    __shared__ double sh[ROW_WG / 64];
-    double acc = 0.0;
+                                                      typename RAcc<T>::A* __restrict__ part) {
+    using A = typename RAcc<T>::A;
+    __shared__ A sh[ROW_WG / 64];
+    A acc = A(0);
     visit_owned_entries<I>(
         n, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {},
         [&](int64_t r, int64_t j) {
-            if ((int64_t)idx[j] - 1 == r) acc += (double)val[j];
+            if ((int64_t)idx[j] - 1 == r) acc = acc + to_acc(val[j]);
         },
         [&](int64_t r, int64_t s, int64_t e) {
             for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG)
-                if ((int64_t)idx[j] - 1 == r) acc += (double)val[j];
+                if ((int64_t)idx[j] - 1 == r) acc = acc + to_acc(val[j]);
         });
-    const double t = wg_sum_fixed(acc, sh);
+    const A t = wg_sum_fixed(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
@@ -71,20 +93,21 @@ template <typename T, typename I>
 __global__ __launch_bounds__(ROW_WG) void k_trace_coo(int64_t nnz, const I* __restrict__ rowind,
                                                       const I* __restrict__ colind,
                                                       const T* __restrict__ val,
-                                                      double* __restrict__ part) {
-    __shared__ double sh[ROW_WG / 64];
-    double acc = 0.0;
+                                                      typename RAcc<T>::A* __restrict__ part) {
+    using A = typename RAcc<T>::A;
+    __shared__ A sh[ROW_WG / 64];
+    A acc = A(0);
     const int64_t per = (nnz + gridDim.x - 1) / gridDim.x;  // contiguous share per workgroup
     const int64_t b0 = blockIdx.x * per, b1 = b0 + per < nnz ? b0 + per : nnz;
     for (int64_t j = b0 + threadIdx.x; j < b1; j += ROW_WG)
-        if (!colind || rowind[j] == colind[j]) acc += (double)val[j];
-    const double t = wg_sum_fixed(acc, sh);
+        if (!colind || rowind[j] == colind[j]) acc = acc + to_acc(val[j]);
+    const A t = wg_sum_fixed(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
 // ---- dot(x, A, y) ------------------------------------------------------------
-// Each stored entry a (major r, index i) adds x[p] * a * y[q], left to right
-// (dot(x, a, y) = conj(x) * a * y for real scalars,
+// Each stored entry a (major r, index i) adds conj(x[p]) * a * y[q], left to
+// right (dot(x, a, y) of scalars; a conjugated under Adjoint, CONJA;
 // kernel_workgroup_dot_csr_N!/T!, src/matrix_csr/matrix_csr_kernels.jl:47-131),
 // with (p, q) = (r, i) when major_left, else (i, r).
 template <typename T, typename I>
@@ -94,16 +117,18 @@ __global__ __launch_bounds__(ROW_WG) void k_dot3_ptr(int64_t nmaj, int64_t nidx,
                                                      const T* __restrict__ val,
                                                      const T* __restrict__ x,
                                                      const T* __restrict__ y, int major_left,
+                                                     int cja,
                                                      const int64_t* __restrict__ blk_row,
-                                                     double* __restrict__ part) {
-    __shared__ double sh[ROW_WG / 64];
-    double acc = 0.0;
+                                                     typename RAcc<T>::A* __restrict__ part) {
+    using A = typename RAcc<T>::A;
+    __shared__ A sh[ROW_WG / 64];
+    A acc = A(0);
     auto term = [&](int64_t r, int64_t j) {
         const int64_t i = dclamp((int64_t)idx[j] - 1, 0, nidx - 1);
-        const T xv = major_left ? x[r] : x[i];
+        const T xv = conj_v(major_left ? x[r] : x[i]);
         const T yv = major_left ? y[i] : y[r];
-        const T t = (xv * val[j]) * yv;
-        acc += (double)t;
+        const T t = (xv * conj_if(val[j], cja != 0)) * yv;
+        acc = acc + to_acc(t);
     };
     visit_owned_entries<I>(
         nmaj, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {},
@@ -111,7 +136,7 @@ __global__ __launch_bounds__(ROW_WG) void k_dot3_ptr(int64_t nmaj, int64_t nidx,
         [&](int64_t r, int64_t s, int64_t e) {
             for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) term(r, j);
         });
-    const double t = wg_sum_fixed(acc, sh);
+    const A t = wg_sum_fixed(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
@@ -125,7 +150,7 @@ __global__ __launch_bounds__(ROW_WG) void k_dot2(int64_t n, const T* __restrict_
     for (int64_t i = blockIdx.x * (int64_t)ROW_WG + threadIdx.x; i < n;
          i += (int64_t)gridDim.x * ROW_WG)
         acc += (double)(w[i] * t[i]);
-    const double r = wg_sum_fixed(acc, sh);
+    const double r = wg_sum_fixed<double>(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = r;
 }
 
@@ -137,18 +162,21 @@ __global__ __launch_bounds__(ROW_WG) void k_dot3_coo(int64_t m, int64_t n, int64
                                                      const T* __restrict__ val,
                                                      const T* __restrict__ x,
                                                      const T* __restrict__ y, int trans,
-                                                     double* __restrict__ part) {
-    __shared__ double sh[ROW_WG / 64];
-    double acc = 0.0;
+                                                     int cja,
+                                                     typename RAcc<T>::A* __restrict__ part) {
+    using A = typename RAcc<T>::A;
+    __shared__ A sh[ROW_WG / 64];
+    A acc = A(0);
     const int64_t per = (nnz + gridDim.x - 1) / gridDim.x;
     const int64_t b0 = blockIdx.x * per, b1 = b0 + per < nnz ? b0 + per : nnz;
     for (int64_t j = b0 + threadIdx.x; j < b1; j += ROW_WG) {
         const int64_t r = dclamp((int64_t)rowind[j] - 1, 0, m - 1);
         const int64_t c = dclamp((int64_t)colind[j] - 1, 0, n - 1);
-        const T t = trans ? (x[c] * val[j]) * y[r] : (x[r] * val[j]) * y[c];
-        acc += (double)t;
+        const T a = conj_if(val[j], cja != 0);
+        const T t = trans ? (conj_v(x[c]) * a) * y[r] : (conj_v(x[r]) * a) * y[c];
+        acc = acc + to_acc(t);
     }
-    const double t = wg_sum_fixed(acc, sh);
+    const A t = wg_sum_fixed(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
@@ -156,18 +184,23 @@ __global__ __launch_bounds__(ROW_WG) void k_dot3_coo(int64_t m, int64_t n, int64
 // For every stored entry (major r, index c != r): a binary search of major c
 // for index r, exactly as kernel_check_symmetry_csr!/csc!
 // (src/matrix_csr/matrix_csr_kernels.jl:348-407, src/matrix_csc/matrix_csc_kernels.jl:350-405)
-// does it; the entry must be found and equal (for real values the Hermitian
-// test is the same).  flag = 0 on the first failure seen.
+// does it; the entry must be found and equal (Hermitian: equal to the
+// conjugate, and every diagonal value equal to its own conjugate).  flag = 0 on
+// the first failure seen.
 template <typename T, typename I>
 __global__ __launch_bounds__(ROW_WG) void k_symmetric_ptr(int64_t n, int64_t nnz,
                                                           const I* __restrict__ ptr,
                                                           const I* __restrict__ idx,
                                                           const T* __restrict__ val,
+                                                          int hermitian,
                                                           const int64_t* __restrict__ blk_row,
                                                           int* __restrict__ flag) {
     auto check = [&](int64_t r, int64_t j) {
         const int64_t c = (int64_t)idx[j] - 1;
-        if (c == r) return;
+        if (c == r) {
+            if (hermitian && val[j] != conj_v(val[j])) *flag = 0;
+            return;
+        }
         bool ok = false;
         if (c >= 0 && c < n) {
             int64_t lo = (int64_t)ptr[c], hi = (int64_t)ptr[c + 1] - 1;  // one-based, inclusive
@@ -175,7 +208,8 @@ __global__ __launch_bounds__(ROW_WG) void k_symmetric_ptr(int64_t n, int64_t nnz
                 const int64_t mid = lo + (hi - lo) / 2;
                 const int64_t mc = (int64_t)idx[dclamp(mid - 1, 0, nnz - 1)] - 1;
                 if (mc == r) {
-                    ok = val[dclamp(mid - 1, 0, nnz - 1)] == val[j];
+                    const T tv = val[dclamp(mid - 1, 0, nnz - 1)];
+                    ok = val[j] == (hermitian ? conj_v(tv) : tv);
                     break;
                 }
                 if (mc < r) lo = mid + 1;
@@ -308,6 +342,105 @@ __global__ void k_diag_coo(int64_t n, const T* __restrict__ dvals, T lambda, I* 
     }
 }
 
+// ---- GenericSparseVector: dot(x, y) and norms of the stored values ---------------
+// dot(x::GenericSparseVector, y::DenseVector) (src/vector/vector.jl:102-145):
+// sum over stored entries of dot(nzval[i], y[nzind[i]]) = conj(nzval[i]) * y[..],
+// the reference's @atomic sum replaced by fixed-order per-workgroup partials.
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_spvec_dot(int64_t n, int64_t nnz,
+                                                      const I* __restrict__ nzind,
+                                                      const T* __restrict__ nzval,
+                                                      const T* __restrict__ y,
+                                                      typename RAcc<T>::A* __restrict__ part) {
+    using A = typename RAcc<T>::A;
+    __shared__ A sh[ROW_WG / 64];
+    A acc = A(0);
+    const int64_t per = (nnz + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = blockIdx.x * per, b1 = b0 + per < nnz ? b0 + per : nnz;
+    for (int64_t j = b0 + threadIdx.x; j < b1; j += ROW_WG) {
+        const int64_t i = dclamp((int64_t)nzind[j] - 1, 0, n - 1);
+        acc = acc + to_acc(conj_v(nzval[j]) * y[i]);
+    }
+    const A t = wg_sum_fixed(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
+// |v| in Float64 (complex: the modulus, by hypot).
+template <typename T>
+__device__ __forceinline__ double abs_d(T v) {
+    if constexpr (VT<T>::cx) return hypot((double)v.re, (double)v.im);
+    else if constexpr (std::is_same_v<T, uint32_t>) return fabs((double)(int32_t)v);
+    else if constexpr (std::is_same_v<T, uint64_t>) return fabs((double)(int64_t)v);
+    else return fabs((double)v);
+}
+
+// Per workgroup: max |v| and min |v| over a contiguous share of the values.
+template <typename T>
+__global__ __launch_bounds__(ROW_WG) void k_abs_extrema(int64_t nnz, const T* __restrict__ val,
+                                                        double* __restrict__ part) {
+    __shared__ double smax[ROW_WG / 64], smin[ROW_WG / 64];
+    double mx = 0.0, mn = INFINITY;
+    const int64_t per = (nnz + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = blockIdx.x * per, b1 = b0 + per < nnz ? b0 + per : nnz;
+    for (int64_t j = b0 + threadIdx.x; j < b1; j += ROW_WG) {
+        const double a = abs_d(val[j]);
+        mx = (a > mx || a != a) ? a : mx;  // NaN propagates to the maximum
+        mn = a < mn ? a : mn;
+    }
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        const double u = __shfl_xor(mx, o, 64), w = __shfl_xor(mn, o, 64);
+        mx = (u > mx || u != u) ? u : mx;
+        mn = w < mn ? w : mn;
+    }
+    if ((threadIdx.x & 63) == 0) {
+        smax[threadIdx.x >> 6] = mx;
+        smin[threadIdx.x >> 6] = mn;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < ROW_WG / 64; ++w) {
+            mx = (smax[w] > mx || smax[w] != smax[w]) ? smax[w] : mx;
+            mn = smin[w] < mn ? smin[w] : mn;
+        }
+        part[2 * blockIdx.x] = mx;
+        part[2 * blockIdx.x + 1] = mn;
+    }
+}
+
+__global__ void k_extrema_parts(int64_t n, const double* __restrict__ part,
+                                double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    double mx = 0.0, mn = INFINITY;
+    for (int64_t i = 0; i < n; ++i) {
+        const double a = part[2 * i], b = part[2 * i + 1];
+        mx = (a > mx || a != a) ? a : mx;
+        mn = b < mn ? b : mn;
+    }
+    out[0] = mx;
+    out[1] = mn;
+}
+
+// Per workgroup: sum of (|v| / scale)^p, or of (v != 0) for p = 0.
+template <typename T>
+__global__ __launch_bounds__(ROW_WG) void k_pow_sum(int64_t nnz, const T* __restrict__ val,
+                                                    double p, double scale,
+                                                    double* __restrict__ part) {
+    __shared__ double sh[ROW_WG / 64];
+    double acc = 0.0;
+    const int64_t per = (nnz + gridDim.x - 1) / gridDim.x;
+    const int64_t b0 = blockIdx.x * per, b1 = b0 + per < nnz ? b0 + per : nnz;
+    for (int64_t j = b0 + threadIdx.x; j < b1; j += ROW_WG) {
+        const double a = abs_d(val[j]);
+        if (p == 0.0) acc += a != 0.0 ? 1.0 : 0.0;
+        else if (p == 1.0) acc += a / scale;
+        else if (p == 2.0) acc += (a / scale) * (a / scale);
+        else acc += pow(a / scale, p);
+    }
+    const double t = wg_sum_fixed<double>(acc, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+
 // Row-block plan over `ptr` for the visit_owned_rows kernels: the caller's
 // plan when it was built on this pointer array, else one built into `tmp`.
 static int row_blocks(hipStream_t s, gsa_plan plan, int itype, int64_t n, int64_t nnz,
@@ -324,15 +457,21 @@ static int row_blocks(hipStream_t s, gsa_plan plan, int itype, int64_t n, int64_
     return build_csr_plan(s, itype, n, nnz, ptr, tmp.as<int64_t>(), *nblocks);
 }
 
-// Parts -> host double (synchronises).
-static int finish_sum(hipStream_t s, const double* part, int64_t nparts, double* out) {
+// Parts -> one host accumulator value (synchronises).
+template <typename A>
+static int finish_sum(hipStream_t s, const A* part, int64_t nparts, void* out) {
     Scratch res(s);
-    GSA_TRY(res.alloc(sizeof(double)));
-    hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(ROW_WG), 0, s, nparts, part, res.as<double>());
+    GSA_TRY(res.alloc(sizeof(A)));
+    hipLaunchKernelGGL(k_sum_parts<A>, dim3(1), dim3(ROW_WG), 0, s, nparts, part, res.as<A>());
     GSA_HIP(hipGetLastError());
-    GSA_HIP(hipMemcpyAsync(out, res.p, sizeof(double), hipMemcpyDeviceToHost, s));
+    GSA_HIP(hipMemcpyAsync(out, res.p, sizeof(A), hipMemcpyDeviceToHost, s));
     GSA_HIP(hipStreamSynchronize(s));
     return GSA_OK;
+}
+
+// Bytes of the accumulator a reduction of vtype hands back.
+static size_t acc_bytes(int vtype) {
+    return (vtype == GSA_C64 || vtype == GSA_C128) ? 16 : 8;
 }
 
 }  // namespace gsa
@@ -342,72 +481,75 @@ using namespace gsa;
 extern "C" {
 
 int gsa_trace_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz, const void* ptr,
-                  const void* idx, const void* val, gsa_plan plan, double* out) {
+                  const void* idx, const void* val, gsa_plan plan, void* out) {
     GSA_TRY(ensure_init());
     GSA_CHECK(out && n >= 0 && nnz >= 0 && (n == 0 || ptr), GSA_ERR_INVALID,
               "gsa_trace_ptr: bad arguments");
-    *out = 0.0;
+    memset(out, 0, acc_bytes(vtype));
     if (n == 0 || nnz == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
         using T = decltype(tv);
         using I = decltype(ti);
+        using A = typename RAcc<T>::A;
         Scratch tmp(s), part(s);
         const int64_t* br = nullptr;
         int64_t nb = 0;
         GSA_TRY(row_blocks(s, plan, itype, n, nnz, ptr, tmp, &br, &nb));
-        GSA_TRY(part.alloc(sizeof(double) * nb));
+        GSA_TRY(part.alloc(sizeof(A) * nb));
         hipLaunchKernelGGL((k_trace_ptr<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, n, nnz,
-                           (const I*)ptr, (const I*)idx, (const T*)val, br, part.as<double>());
+                           (const I*)ptr, (const I*)idx, (const T*)val, br, part.as<A>());
         GSA_HIP(hipGetLastError());
-        return finish_sum(s, part.as<double>(), nb, out);
+        return finish_sum<A>(s, part.as<A>(), nb, out);
     });
 }
 
 int gsa_trace_coo(void* stream, int vtype, int itype, int64_t nnz, const void* rowind,
-                  const void* colind, const void* val, double* out) {
+                  const void* colind, const void* val, void* out) {
     GSA_TRY(ensure_init());
     GSA_CHECK(out && nnz >= 0, GSA_ERR_INVALID, "gsa_trace_coo: bad arguments");
-    *out = 0.0;
+    memset(out, 0, acc_bytes(vtype));
     if (nnz == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
         using T = decltype(tv);
         using I = decltype(ti);
+        using A = typename RAcc<T>::A;
         Scratch part(s);
-        GSA_TRY(part.alloc(sizeof(double) * RD_PARTS));
+        GSA_TRY(part.alloc(sizeof(A) * RD_PARTS));
         hipLaunchKernelGGL((k_trace_coo<T, I>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, nnz,
-                           (const I*)rowind, (const I*)colind, (const T*)val, part.as<double>());
+                           (const I*)rowind, (const I*)colind, (const T*)val, part.as<A>());
         GSA_HIP(hipGetLastError());
-        return finish_sum(s, part.as<double>(), RD_PARTS, out);
+        return finish_sum<A>(s, part.as<A>(), RD_PARTS, out);
     });
 }
 
-int gsa_sum_values(void* stream, int vtype, int64_t nnz, const void* val, double* out) {
+int gsa_sum_values(void* stream, int vtype, int64_t nnz, const void* val, void* out) {
     GSA_TRY(ensure_init());
     GSA_CHECK(out && nnz >= 0, GSA_ERR_INVALID, "gsa_sum_values: bad arguments");
-    *out = 0.0;
+    memset(out, 0, acc_bytes(vtype));
     if (nnz == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
-    return dispatch_v(vtype, [&](auto tv) -> int {
+    return dispatch_va(vtype, [&](auto tv) -> int {
         using T = decltype(tv);
+        using A = typename RAcc<T>::A;
         Scratch part(s);
-        GSA_TRY(part.alloc(sizeof(double) * RD_PARTS));
+        GSA_TRY(part.alloc(sizeof(A) * RD_PARTS));
         hipLaunchKernelGGL((k_trace_coo<T, int64_t>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, nnz,
                            (const int64_t*)nullptr, (const int64_t*)nullptr, (const T*)val,
-                           part.as<double>());
+                           part.as<A>());
         GSA_HIP(hipGetLastError());
-        return finish_sum(s, part.as<double>(), RD_PARTS, out);
+        return finish_sum<A>(s, part.as<A>(), RD_PARTS, out);
     });
 }
 
 int gsa_dot3_ptr(void* stream, int vtype, int itype, int64_t nmaj, int64_t nidx, int64_t nnz,
                  const void* ptr, const void* idx, const void* val, const void* x, const void* y,
-                 int major_left, gsa_plan plan, double* out) {
+                 int major_left, int conj_a, gsa_plan plan, void* out) {
     GSA_TRY(ensure_init());
     GSA_CHECK(out && nmaj >= 0 && nidx >= 0 && nnz >= 0 && (nmaj == 0 || ptr), GSA_ERR_INVALID,
               "gsa_dot3_ptr: bad arguments");
-    *out = 0.0;
+    memset(out, 0, acc_bytes(vtype));
     if (nmaj == 0 || nidx == 0 || nnz == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
     // Float64 with the matrix's SpMV plan: t = the SpMV of the stored entries
@@ -425,59 +567,61 @@ int gsa_dot3_ptr(void* stream, int vtype, int itype, int64_t nmaj, int64_t nidx,
         const void* v = major_left ? y : x;
         const double* w = (const double*)(major_left ? x : y);
         GSA_TRY(gsa_spmm_csr(stream, GSA_F64, itype, GSA_OP_N, nmaj, nidx, 1, ptr, idx, val, nnz,
-                             v, nidx, GSA_OP_N, t.p, nmaj, 1.0, 0.0, plan));
+                             v, nidx, GSA_OP_N, t.p, nmaj, nullptr, nullptr, plan));
         hipLaunchKernelGGL((k_dot2<double>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, nmaj, w,
                            (const double*)t.p, part.as<double>());
         GSA_HIP(hipGetLastError());
-        return finish_sum(s, part.as<double>(), RD_PARTS, out);
+        return finish_sum<double>(s, part.as<double>(), RD_PARTS, out);
     }
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
         using T = decltype(tv);
         using I = decltype(ti);
+        using A = typename RAcc<T>::A;
         Scratch tmp(s), part(s);
         const int64_t* br = nullptr;
         int64_t nb = 0;
         GSA_TRY(row_blocks(s, plan, itype, nmaj, nnz, ptr, tmp, &br, &nb));
-        GSA_TRY(part.alloc(sizeof(double) * nb));
+        GSA_TRY(part.alloc(sizeof(A) * nb));
         hipLaunchKernelGGL((k_dot3_ptr<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, nmaj, nidx,
                            nnz, (const I*)ptr, (const I*)idx, (const T*)val, (const T*)x,
-                           (const T*)y, major_left, br, part.as<double>());
+                           (const T*)y, major_left, conj_a, br, part.as<A>());
         GSA_HIP(hipGetLastError());
-        return finish_sum(s, part.as<double>(), nb, out);
+        return finish_sum<A>(s, part.as<A>(), nb, out);
     });
 }
 
 int gsa_dot3_coo(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnz,
                  const void* rowind, const void* colind, const void* val, const void* x,
-                 const void* y, int trans, double* out) {
+                 const void* y, int trans, int conj_a, void* out) {
     GSA_TRY(ensure_init());
     GSA_CHECK(out && m >= 0 && n >= 0 && nnz >= 0, GSA_ERR_INVALID, "gsa_dot3_coo: bad arguments");
-    *out = 0.0;
+    memset(out, 0, acc_bytes(vtype));
     if (m == 0 || n == 0 || nnz == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
         using T = decltype(tv);
         using I = decltype(ti);
+        using A = typename RAcc<T>::A;
         Scratch part(s);
-        GSA_TRY(part.alloc(sizeof(double) * RD_PARTS));
+        GSA_TRY(part.alloc(sizeof(A) * RD_PARTS));
         hipLaunchKernelGGL((k_dot3_coo<T, I>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, m, n, nnz,
                            (const I*)rowind, (const I*)colind, (const T*)val, (const T*)x,
-                           (const T*)y, trans, part.as<double>());
+                           (const T*)y, trans, conj_a, part.as<A>());
         GSA_HIP(hipGetLastError());
-        return finish_sum(s, part.as<double>(), RD_PARTS, out);
+        return finish_sum<A>(s, part.as<A>(), RD_PARTS, out);
     });
 }
 
 int gsa_symmetric_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
-                      const void* ptr, const void* idx, const void* val, gsa_plan plan,
-                      int* out) {
+                      const void* ptr, const void* idx, const void* val, int hermitian,
+                      gsa_plan plan, int* out) {
     GSA_TRY(ensure_init());
     GSA_CHECK(out && n >= 0 && nnz >= 0 && (n == 0 || ptr), GSA_ERR_INVALID,
               "gsa_symmetric_ptr: bad arguments");
     *out = 1;
     if (n == 0 || nnz == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
         using T = decltype(tv);
         using I = decltype(ti);
         Scratch tmp(s), flag(s);
@@ -487,7 +631,8 @@ int gsa_symmetric_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz
         GSA_TRY(flag.alloc(sizeof(int)));
         hipLaunchKernelGGL(k_set_flag, dim3(1), dim3(1), 0, s, flag.as<int>());
         hipLaunchKernelGGL((k_symmetric_ptr<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, n,
-                           nnz, (const I*)ptr, (const I*)idx, (const T*)val, br, flag.as<int>());
+                           nnz, (const I*)ptr, (const I*)idx, (const T*)val, hermitian, br,
+                           flag.as<int>());
         GSA_HIP(hipGetLastError());
         GSA_HIP(hipMemcpyAsync(out, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
         GSA_HIP(hipStreamSynchronize(s));
@@ -517,22 +662,23 @@ int gsa_add_diag_count(void* stream, int itype, int64_t n, int64_t nnz, const vo
 }
 
 int gsa_add_diag_fill(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
-                      const void* ptr, const void* idx, const void* val, double lambda,
+                      const void* ptr, const void* idx, const void* val, const void* lambda,
                       gsa_plan plan, const void* ptr_out, void* idx_out, void* val_out) {
     GSA_TRY(ensure_init());
-    GSA_CHECK(n >= 0 && nnz >= 0 && ptr && ptr_out, GSA_ERR_INVALID,
+    GSA_CHECK(n >= 0 && nnz >= 0 && ptr && ptr_out && lambda, GSA_ERR_INVALID,
               "gsa_add_diag_fill: bad arguments");
     if (n == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
         using T = decltype(tv);
         using I = decltype(ti);
+        const T lam = host_scalar<T>(lambda, T(0));
         Scratch tmp(s);
         const int64_t* br = nullptr;
         int64_t nb = 0;
         GSA_TRY(row_blocks(s, plan, itype, n, nnz, ptr, tmp, &br, &nb));
         hipLaunchKernelGGL((k_add_diag<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, n, nnz,
-                           (const I*)ptr, (const I*)idx, (const T*)val, (T)lambda, br,
+                           (const I*)ptr, (const I*)idx, (const T*)val, lam, br,
                            (const I*)ptr_out, (I*)idx_out, (T*)val_out);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
@@ -540,19 +686,81 @@ int gsa_add_diag_fill(void* stream, int vtype, int itype, int64_t n, int64_t nnz
 }
 
 int gsa_diag_coo(void* stream, int vtype, int itype, int64_t n, const void* dvals,
-                 double lambda, void* row, void* col, void* val) {
+                 const void* lambda, void* row, void* col, void* val) {
     GSA_TRY(ensure_init());
-    GSA_CHECK(n >= 0 && (n == 0 || (row && col && val)), GSA_ERR_INVALID,
+    GSA_CHECK(n >= 0 && (n == 0 || (row && col && val)) && (dvals || lambda), GSA_ERR_INVALID,
               "gsa_diag_coo: bad arguments");
     if (n == 0) return GSA_OK;
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) -> int {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
         using T = decltype(tv);
         using I = decltype(ti);
+        const T lam = host_scalar<T>(lambda, T(0));
         hipLaunchKernelGGL((k_diag_coo<T, I>), dim3(grid_for(n, 256)), dim3(256), 0, s, n,
-                           (const T*)dvals, (T)lambda, (I*)row, (I*)col, (T*)val);
+                           (const T*)dvals, lam, (I*)row, (I*)col, (T*)val);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
+    });
+}
+
+int gsa_spvec_dot(void* stream, int vtype, int itype, int64_t n, int64_t nnz, const void* nzind,
+                  const void* nzval, const void* y, void* out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(out && n >= 0 && nnz >= 0 && (nnz == 0 || (nzind && nzval && y)), GSA_ERR_INVALID,
+              "gsa_spvec_dot: bad arguments");
+    memset(out, 0, acc_bytes(vtype));
+    if (nnz == 0 || n == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        using A = typename RAcc<T>::A;
+        Scratch part(s);
+        GSA_TRY(part.alloc(sizeof(A) * RD_PARTS));
+        hipLaunchKernelGGL((k_spvec_dot<T, I>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, n, nnz,
+                           (const I*)nzind, (const T*)nzval, (const T*)y, part.as<A>());
+        GSA_HIP(hipGetLastError());
+        return finish_sum<A>(s, part.as<A>(), RD_PARTS, out);
+    });
+}
+
+int gsa_norm_values(void* stream, int vtype, int64_t nnz, const void* val, double p,
+                    double* out) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(out && nnz >= 0, GSA_ERR_INVALID, "gsa_norm_values: bad arguments");
+    *out = 0.0;
+    if (nnz == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_va(vtype, [&](auto tv) -> int {
+        using T = decltype(tv);
+        Scratch part(s), res(s);
+        GSA_TRY(part.alloc(sizeof(double) * 2 * RD_PARTS));
+        GSA_TRY(res.alloc(sizeof(double) * 2));
+        // pass 1: the largest and smallest |v| (the scale of a p-norm, and the
+        // norms of order +Inf / -Inf)
+        hipLaunchKernelGGL((k_abs_extrema<T>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, nnz,
+                           (const T*)val, part.as<double>());
+        hipLaunchKernelGGL(k_extrema_parts, dim3(1), dim3(ROW_WG), 0, s, (int64_t)RD_PARTS,
+                           (const double*)part.as<double>(), res.as<double>());
+        GSA_HIP(hipGetLastError());
+        double ext[2] = {0.0, 0.0};
+        GSA_HIP(hipMemcpyAsync(ext, res.p, sizeof ext, hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        if (p == INFINITY) { *out = ext[0]; return (int)GSA_OK; }
+        if (p == -INFINITY) { *out = ext[1]; return (int)GSA_OK; }
+        const double scale = ext[0];
+        if (p != 0.0 && (scale == 0.0 || !(scale < INFINITY))) { *out = scale; return (int)GSA_OK; }
+        // pass 2: sum of (|v| / scale)^p (p = 0: the count of nonzero values)
+        hipLaunchKernelGGL((k_pow_sum<T>), dim3(RD_PARTS), dim3(ROW_WG), 0, s, nnz, (const T*)val,
+                           p, scale, part.as<double>());
+        GSA_HIP(hipGetLastError());
+        double sum = 0.0;
+        GSA_TRY(finish_sum<double>(s, part.as<double>(), RD_PARTS, &sum));
+        if (p == 0.0) *out = sum;
+        else if (p == 1.0) *out = scale * sum;
+        else if (p == 2.0) *out = scale * sqrt(sum);
+        else *out = scale * pow(sum, 1.0 / p);
+        return (int)GSA_OK;
     });
 }
 

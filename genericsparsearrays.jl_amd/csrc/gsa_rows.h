@@ -23,11 +23,11 @@ __device__ __forceinline__ void atomic_inc(int32_t* p) { atomicAdd(reinterpret_c
 template <typename T>
 __device__ __forceinline__ void store_out(T* __restrict__ y, int64_t idx, T acc, T alpha,
                                           T beta, int mode) {
-    T t = acc * alpha;
+    T t = mul_scalar(acc, alpha);
     T base;
     if (mode == 0) base = T(0);
     else if (mode == 1) base = y[idx];
-    else base = y[idx] * beta;
+    else base = mul_scalar(y[idx], beta);
     y[idx] = base + t;
 }
 

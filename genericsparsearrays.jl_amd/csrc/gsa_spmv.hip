@@ -53,7 +53,13 @@ __device__ __forceinline__ int64_t clamp64(int64_t v, int64_t lo, int64_t hi) {
 // non-temporal loads so they do not evict the gathered x from L2.
 template <typename U>
 __device__ __forceinline__ U ldnt(const U* p) {
-    return __builtin_nontemporal_load(p);
+    if constexpr (VT<U>::cx) {  // complex: the two parts
+        using R = decltype(p->re);
+        const R* q = reinterpret_cast<const R*>(p);
+        return U(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1));
+    } else {
+        return __builtin_nontemporal_load(p);
+    }
 }
 
 // Stored value -> arithmetic type: identity, or BF16 bits -> Float32.
@@ -68,14 +74,15 @@ __device__ __forceinline__ T widen(V v) {
 template <typename PI, typename V>
 __device__ __forceinline__ V load_val(const V* __restrict__ nz, const PI* __restrict__ perm,
                                       int64_t j) {
-    if constexpr (std::is_void_v<PI>) return __builtin_nontemporal_load(nz + j);
+    if constexpr (std::is_void_v<PI>) return ldnt(nz + j);
     else return nz[(int64_t)__builtin_nontemporal_load(perm + j)];
 }
 
 // Row sum times the row's scale (BF16 storage), or unchanged.
 template <typename T>
 __device__ __forceinline__ T row_scaled(T acc, const float* __restrict__ rscale, int64_t r) {
-    return rscale ? acc * (T)rscale[r] : acc;
+    if constexpr (std::is_floating_point_v<T>) return rscale ? acc * (T)rscale[r] : acc;
+    else return acc;
 }
 
 template <typename T>
@@ -97,12 +104,12 @@ __device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
 
 // Float32 rows reduced across lanes are summed with TwoSum: each lane keeps a
 // (sum, compensation) pair, pairs are combined error-free in the butterfly.
-// Float64 keeps the plain sum.
+// Float64, complex and integer values keep the plain sum.
 template <typename T>
 struct CompSum {
     T s = T(0), c = T(0);
     __device__ __forceinline__ void add(T p) {
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (VT<T>::f32) {
             T ns, er;
             two_sum(s, p, ns, er);
             s = ns;
@@ -113,11 +120,11 @@ struct CompSum {
     }
     // this lane's compensated total
     __device__ __forceinline__ T total() const {
-        if constexpr (sizeof(T) == 4) return s + c;
+        if constexpr (VT<T>::f32) return s + c;
         else return s;
     }
     __device__ __forceinline__ T wave_total() {
-        if constexpr (sizeof(T) == 4) {
+        if constexpr (VT<T>::f32) {
 #pragma unroll
             for (int off = 32; off >= 1; off >>= 1) {
                 T os = lane_xor(s, off), oc = lane_xor(c, off);
@@ -261,7 +268,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
     int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowptr,
     const I* __restrict__ colval, const V* __restrict__ nzval, const PI* __restrict__ perm,
     const float* __restrict__ rscale, const T* __restrict__ x, T* __restrict__ y, T alpha,
-    T beta, int mode, const int64_t* __restrict__ win_row, int64_t nwin,
+    T beta, int mode, int cj, const int64_t* __restrict__ win_row, int64_t nwin,
     const int64_t* __restrict__ order, T* __restrict__ head, T* __restrict__ tail) {
     __shared__ T s_prod[WAVES][WS];
     __shared__ int s_off[WAVES][WRP];
@@ -307,10 +314,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
             int i = u * 64 + lane;
             i = i < staged ? i : staged - 1;
             ci[u] = ldnt(colval + E0 + i);
-            vv[u] = widen<T>(load_val(nzval, perm, E0 + i));
+            vv[u] = conj_if(widen<T>(load_val(nzval, perm, E0 + i)), cj & 1);
         }
 #pragma unroll
-        for (int u = 0; u <= WU; ++u) xv[u] = x[clamp64((int64_t)ci[u] - 1, 0, n - 1)];
+        for (int u = 0; u <= WU; ++u)
+            xv[u] = conj_if(x[clamp64((int64_t)ci[u] - 1, 0, n - 1)], (cj & 2) != 0);
 #pragma unroll
         for (int u = 0; u <= WU; ++u) {
             int i = u * 64 + lane;
@@ -366,7 +374,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
         while (lm) {
             const int src = __ffsll((long long)lm) - 1;
             lm &= lm - 1;
-            const int ls = __shfl(s, src, 64), le = __shfl(e, src, 64);
+            const int ls = __shfl(s, src, 64), le = __shfl(e, src, 64);  // int lanes
             CompSum<T> cs;
             for (int j = ls + lane; j < le; j += 64) cs.add(prod[j]);
             const T acc = cs.wave_total();
@@ -463,7 +471,8 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
     const float* __restrict__ rscale,
     const T* __restrict__ BT,
     int64_t ldbt, T* __restrict__ C, int64_t ldc, int64_t ldk, T alpha, T beta, int mode,
-    const int64_t* __restrict__ blk_row, const int64_t* __restrict__ blk_pos) {
+    int cj, const int64_t* __restrict__ blk_row, const int64_t* __restrict__ blk_pos) {
+    const bool cjb = (cj & 2) != 0;
     // output (r, kk) at C[r * ldc + kk * ldk]: column-major (1, ld) or row-major (ld, 1)
     __shared__ CI s_col[SPMV_CAP];
     __shared__ T s_val[SPMV_CAP];
@@ -498,7 +507,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             int i = base + u * WG + tid;
             i = i < cnt ? i : cnt - 1;
             ci[u] = ldnt(colval + P0 + i);
-            vv[u] = widen<T>(load_val(nzval, perm, P0 + i));
+            vv[u] = conj_if(widen<T>(load_val(nzval, perm, P0 + i)), cj & 1);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -548,7 +557,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                 for (int u = 0; u < 8; ++u) {  // lanes past the row's end issue no load
                     const int jj = j + u < e ? j + u : e - 1;
                     av[u] = s_val[jj];
-                    bv[u] = j + u < e ? BT[(int64_t)s_col[jj] * ldbt + kk] : T(0);
+                    bv[u] = j + u < e ? conj_if(BT[(int64_t)s_col[jj] * ldbt + kk], cjb) : T(0);
                 }
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
@@ -576,7 +585,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                     for (int u = 0; u < 8; ++u) {
                         const int jj = j + u < je ? j + u : je - 1;
                         av[u] = s_val[jj];
-                        bv[u] = BT[(int64_t)s_col[jj] * ldbt + kk];
+                        bv[u] = conj_if(BT[(int64_t)s_col[jj] * ldbt + kk], cjb);
                     }
 #pragma unroll
                     for (int u = 0; u < 8; ++u)
@@ -609,7 +618,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                 __syncthreads();  // previous users of s_col/s_val are done
                 for (int i = tid; i < cn; i += WG) {
                     s_col[i] = (CI)clamp64((int64_t)ldnt(colval + c0 + i) - 1, 0, n - 1);
-                    s_val[i] = widen<T>(load_val(nzval, perm, c0 + i));
+                    s_val[i] = conj_if(widen<T>(load_val(nzval, perm, c0 + i)), cj & 1);
                 }
                 __syncthreads();
                 if (kk < k) {
@@ -621,7 +630,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
                         for (int u = 0; u < 8; ++u) {
                             const int jj = j + u < je ? j + u : je - 1;
                             av[u] = s_val[jj];
-                            bv[u] = BT[(int64_t)s_col[jj] * ldbt + kk];
+                            bv[u] = conj_if(BT[(int64_t)s_col[jj] * ldbt + kk], cjb);
                         }
 #pragma unroll
                         for (int u = 0; u < 8; ++u)
@@ -653,7 +662,7 @@ __global__ void k_beta(int64_t rows, int64_t k, T* __restrict__ C, int64_t ldc, 
          t += (int64_t)gridDim.x * blockDim.x) {
         int64_t i = t % rows, j = t / rows;
         T* p = C + i + j * ldc;
-        *p = (mode == 0) ? T(0) : (*p) * beta;
+        *p = (mode == 0) ? T(0) : mul_scalar(*p, beta);
     }
 }
 
@@ -681,7 +690,7 @@ __global__ __launch_bounds__(256) void k_to_rowmajor(int64_t rows, int64_t cols,
 
 template <typename T>
 __device__ __forceinline__ void atomic_add(T* p, T v) {
-    unsafeAtomicAdd(p, v);
+    atomic_add_v(p, v);
 }
 
 // Scatter: for every stored (row, col, v) of the CSR arrays,
@@ -697,7 +706,7 @@ template <typename T, typename I>
 __global__ __launch_bounds__(64 * WAVES) void k_csr_scatter(
     int64_t m, int64_t n, int64_t k, int64_t nnz, const I* __restrict__ rowptr,
     const I* __restrict__ colval, const T* __restrict__ nzval, const T* __restrict__ B,
-    int64_t ldb, int opB, T* __restrict__ C, int64_t ldc, T alpha,
+    int64_t ldb, int opB, T* __restrict__ C, int64_t ldc, T alpha, int cj,
     const int64_t* __restrict__ win_row, int64_t nwin) {
     __shared__ int s_off[WAVES][WRP];
     const int lane = threadIdx.x & 63;
@@ -739,11 +748,11 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_scatter(
         if (i >= cnt) continue;
         const int64_t j = E0 + i;
         const int64_t c = clamp64((int64_t)ldnt(colval + j) - 1, 0, n - 1);
-        const T v = ldnt(nzval + j);
+        const T v = conj_if(ldnt(nzval + j), cj & 1);
         const int64_t r = clamp64(row_of(i), 0, m - 1);
         for (int64_t kk = 0; kk < k; ++kk) {
-            const T xb = opB ? B[kk + r * ldb] : B[r + kk * ldb];
-            atomic_add(&C[c + kk * ldc], v * (xb * alpha));
+            const T xb = conj_if(opB ? B[kk + r * ldb] : B[r + kk * ldb], (cj & 2) != 0);
+            atomic_add(&C[c + kk * ldc], v * mul_scalar(xb, alpha));
         }
     }
 }
@@ -757,7 +766,7 @@ __global__ void k_coo_scatter(int64_t rows_out, int64_t cols_in, int64_t k, int6
                               const I* __restrict__ rowind, const I* __restrict__ colind,
                               const T* __restrict__ nzval, int trans,
                               const T* __restrict__ B, int64_t ldb, int opB,
-                              T* __restrict__ C, int64_t ldc, T alpha) {
+                              T* __restrict__ C, int64_t ldc, T alpha, int cj) {
     const int lane = threadIdx.x & 63;
     const int64_t total = nnz * k;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
@@ -774,8 +783,8 @@ __global__ void k_coo_scatter(int64_t rows_out, int64_t cols_in, int64_t k, int6
             int64_t c = (int64_t)(trans ? rowind[i] : colind[i]) - 1;
             r = clamp64(r, 0, rows_out - 1);
             c = clamp64(c, 0, cols_in - 1);
-            const T xb = opB ? B[kk + c * ldb] : B[c + kk * ldb];
-            val = nzval[i] * (xb * alpha);
+            const T xb = conj_if(opB ? B[kk + c * ldb] : B[c + kk * ldb], (cj & 2) != 0);
+            val = conj_if(nzval[i], cj & 1) * mul_scalar(xb, alpha);
             key = r + kk * ldc;
         }
         // segmented inclusive scan over runs of equal keys (head = key differs
@@ -784,7 +793,7 @@ __global__ void k_coo_scatter(int64_t rows_out, int64_t cols_in, int64_t k, int6
         int head = (lane == 0 || kprev != key) ? 1 : 0;
 #pragma unroll
         for (int o = 1; o < 64; o <<= 1) {
-            const T vu = __shfl_up(val, o, 64);
+            const T vu = shfl_up_v(val, o);
             const int hu = __shfl_up(head, o, 64);
             if (lane >= o && !head) {
                 val = val + vu;
@@ -798,14 +807,12 @@ __global__ void k_coo_scatter(int64_t rows_out, int64_t cols_in, int64_t k, int6
 }
 
 // ---------------------------------------------------------------- drivers --
-static int beta_mode(double beta) { return beta == 0.0 ? 0 : (beta == 1.0 ? 1 : 2); }
-
 template <typename T>
-static int apply_beta(hipStream_t s, int64_t rows, int64_t k, T* C, int64_t ldc, double beta) {
-    int mode = beta_mode(beta);
+static int apply_beta(hipStream_t s, int64_t rows, int64_t k, T* C, int64_t ldc, T beta) {
+    int mode = beta_mode_of(beta);
     if (mode == 1 || rows == 0 || k == 0) return GSA_OK;
     hipLaunchKernelGGL(k_beta<T>, dim3(grid_for(rows * k, 256)), dim3(256), 0, s, rows, k, C,
-                       ldc, (T)beta, mode);
+                       ldc, beta, mode);
     GSA_HIP(hipGetLastError());
     return GSA_OK;
 }
@@ -872,9 +879,9 @@ struct PlanUse {
 template <typename T, typename I, typename V = T, typename PI = void>
 static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* rowptr,
                       const I* colval, const V* nzval, int64_t nnz, const T* B, int64_t ldb,
-                      int opB, T* C, int64_t ldc, double alpha, double beta, gsa_plan plan,
+                      int opB, T* C, int64_t ldc, T alpha, T beta, gsa_plan plan,
                       int itype, const float* rscale = nullptr, const PI* perm = nullptr,
-                      bool c_rowmajor = false) {
+                      bool c_rowmajor = false, int cj = 0) {
     if (m == 0 || k == 0) return GSA_OK;
     if (n == 0 || nnz == 0) {  // A*B contributes nothing: only β applies
         return c_rowmajor ? apply_beta<T>(s, k, m, C, ldc, beta)
@@ -886,7 +893,7 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         GSA_CHECK(perm != nullptr || nnz == 0, GSA_ERR_INVALID, "value permutation is null");
     PlanUse pu(s);
     GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr, k == 1));
-    const int mode = beta_mode(beta);
+    const int mode = beta_mode_of(beta);
     const unsigned grid = (unsigned)pu.nblocks;
     if (k == 1) {
         const T* x = B;
@@ -906,11 +913,11 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         const int64_t grid1 = order ? pu.nslot / WAVES : cdiv(pu.nwin, WAVES);
         GSA_CHECK(grid1 <= (int64_t)UINT32_MAX, GSA_ERR_INVALID, "SpMV grid too large");
         hipLaunchKernelGGL((k_csr_spmv<T, I, V, PI>), dim3((unsigned)grid1), dim3(64 * WAVES), 0, s,
-                           m, n, nnz, rowptr, colval, nzval, perm, rscale, x, C, (T)alpha,
-                           (T)beta, mode, pu.win_row, pu.nwin, order, head, tail);
+                           m, n, nnz, rowptr, colval, nzval, perm, rscale, x, C, alpha,
+                           beta, mode, cj, pu.win_row, pu.nwin, order, head, tail);
         hipLaunchKernelGGL((k_spmv_fold<T, I>), dim3((unsigned)cdiv(pu.nwin, 256)), dim3(256), 0, s,
                            m, nnz, rowptr, pu.win_row, pu.nwin, (const T*)head, (const T*)tail,
-                           rscale, C, (T)alpha, (T)beta, mode);
+                           rscale, C, alpha, beta, mode);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
     }
@@ -931,11 +938,11 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         if (n <= INT32_MAX)                                                                 \
             hipLaunchKernelGGL((k_csr_spmm<T, I, int32_t, L, V, PI>), dim3(grid), dim3(WG), 0, s, m, \
                                n, k, nnz, rowptr, colval, nzval, perm, rscale, BT, ldbt, C, \
-                               c_ld, c_lk, (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos); \
+                               c_ld, c_lk, alpha, beta, mode, cj, pu.blk_row, pu.blk_pos);   \
         else                                                                                \
             hipLaunchKernelGGL((k_csr_spmm<T, I, I, L, V, PI>), dim3(grid), dim3(WG), 0, s, m, n, k, \
                                nnz, rowptr, colval, nzval, perm, rscale, BT, ldbt, C, c_ld,  \
-                               c_lk, (T)alpha, (T)beta, mode, pu.blk_row, pu.blk_pos);       \
+                               c_lk, alpha, beta, mode, cj, pu.blk_row, pu.blk_pos);         \
     } while (0)
     if (k <= 2) GSA_SPMM_LAUNCH(2);
     else if (k <= 4) GSA_SPMM_LAUNCH(4);
@@ -952,16 +959,16 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
 template <typename T, typename I>
 static int scatter_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* rowptr,
                        const I* colval, const T* nzval, int64_t nnz, const T* B, int64_t ldb,
-                       int opB, T* C, int64_t ldc, double alpha, double beta, gsa_plan plan,
-                       int itype) {
+                       int opB, T* C, int64_t ldc, T alpha, T beta, gsa_plan plan,
+                       int itype, int cj = 0) {
     if (n == 0 || k == 0) return GSA_OK;
     GSA_TRY(apply_beta<T>(s, n, k, C, ldc, beta));
     if (m == 0 || nnz == 0) return GSA_OK;
     PlanUse pu(s);
     GSA_TRY(pu.get(s, plan, itype, m, nnz, rowptr, true));
     hipLaunchKernelGGL((k_csr_scatter<T, I>), dim3((unsigned)cdiv(pu.nwin, WAVES)), dim3(64 * WAVES),
-                       0, s, m, n, k, nnz, rowptr, colval, nzval, B, ldb, opB, C, ldc, (T)alpha,
-                       pu.win_row, pu.nwin);
+                       0, s, m, n, k, nnz, rowptr, colval, nzval, B, ldb, opB, C, ldc, alpha,
+                       cj, pu.win_row, pu.nwin);
     GSA_HIP(hipGetLastError());
     return GSA_OK;
 }
@@ -972,19 +979,20 @@ static int scatter_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* 
 // view's storage positions.
 template <typename T, typename I>
 static int view_spmv(hipStream_t s, const RowView& v, int64_t n, const I* colind, const T* nzval,
-                     const T* B, int64_t ldb, int opB, T* C, int64_t ldc, double alpha,
-                     double beta, int itype, int64_t k = 1) {
+                     const T* B, int64_t ldb, int opB, T* C, int64_t ldc, T alpha,
+                     T beta, int itype, int64_t k = 1, int cj = 0) {
     const I* col = v.col ? (const I*)v.col : colind;
     if (!v.perm)
         return gather_csr<T, I>(s, v.m, n, k, (const I*)v.rowptr, col, nzval, v.nnz, B, ldb, opB,
-                                C, ldc, alpha, beta, v.win, itype);
+                                C, ldc, alpha, beta, v.win, itype, nullptr, (const void*)nullptr,
+                                false, cj);
     if (v.perm64)
         return gather_csr<T, I, T, uint64_t>(s, v.m, n, k, (const I*)v.rowptr, col, nzval, v.nnz,
                                              B, ldb, opB, C, ldc, alpha, beta, v.win, itype,
-                                             nullptr, (const uint64_t*)v.perm);
+                                             nullptr, (const uint64_t*)v.perm, false, cj);
     return gather_csr<T, I, T, uint32_t>(s, v.m, n, k, (const I*)v.rowptr, col, nzval, v.nnz, B,
                                          ldb, opB, C, ldc, alpha, beta, v.win, itype, nullptr,
-                                         (const uint32_t*)v.perm);
+                                         (const uint32_t*)v.perm, false, cj);
 }
 
 // A plan's gather view (the output rows of a scatter-shaped product, with each
@@ -1066,7 +1074,7 @@ static int64_t noalloc_bytes(int64_t nnz, size_t value_bytes, int64_t* head_off)
 template <typename T, typename I>
 static int spmv_noalloc(hipStream_t s, int itype, int64_t m, int64_t n, const I* rowptr,
                         const I* colval, const T* nzval, int64_t nnz, const T* x, T* y,
-                        double alpha, double beta, uint8_t* ws) {
+                        T alpha, T beta, uint8_t* ws) {
     if (m == 0) return GSA_OK;
     if (n == 0 || nnz == 0) return apply_beta<T>(s, m, 1, y, m, beta);
     int64_t head_off = 0;
@@ -1081,16 +1089,16 @@ static int spmv_noalloc(hipStream_t s, int itype, int64_t m, int64_t n, const I*
     // the table is cleared and filled here; every head and tail the fold reads is
     // written by the window kernel of this call first (see window_tail)
     GSA_TRY(build_csr_plan(s, itype, m, nnz, rowptr, win_row, nwin, nullptr, SPMV_WIN));
-    const int mode = beta_mode(beta);
+    const int mode = beta_mode_of(beta);
     const int64_t grid = cdiv(nwin, WAVES);
     GSA_CHECK(grid <= (int64_t)UINT32_MAX, GSA_ERR_INVALID, "SpMV grid too large");
     hipLaunchKernelGGL((k_csr_spmv<T, I>), dim3((unsigned)grid), dim3(64 * WAVES), 0, s, m, n, nnz,
                        rowptr, colval, nzval, (const void*)nullptr, (const float*)nullptr, x, y,
-                       (T)alpha, (T)beta, mode, (const int64_t*)win_row, nwin,
+                       alpha, beta, mode, 0, (const int64_t*)win_row, nwin,
                        (const int64_t*)nullptr, head, tail);
     hipLaunchKernelGGL((k_spmv_fold<T, I>), dim3((unsigned)cdiv(nwin, 256)), dim3(256), 0, s, m,
                        nnz, rowptr, (const int64_t*)win_row, nwin, (const T*)head,
-                       (const T*)tail, (const float*)nullptr, y, (T)alpha, (T)beta, mode);
+                       (const T*)tail, (const float*)nullptr, y, alpha, beta, mode);
     GSA_HIP(hipGetLastError());
     return GSA_OK;
 }
@@ -1390,31 +1398,35 @@ int gsa_plan_destroy(gsa_plan plan) {
 
 int gsa_spmm_csr(void* stream, int vtype, int itype, int opA, int64_t m, int64_t n, int64_t k,
                  const void* rowptr, const void* colval, const void* nzval, int64_t nnz,
-                 const void* B, int64_t ldb, int opB, void* C, int64_t ldc, double alpha,
-                 double beta, gsa_plan plan) {
+                 const void* B, int64_t ldb, int opB, void* C, int64_t ldc, const void* alpha,
+                 const void* beta, gsa_plan plan) {
     GSA_TRY(ensure_init());
     GSA_TRY(check_common(opA, opB, m, n, k, nnz, ldb, ldc));
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    const int bt = opB != GSA_OP_N;                        // B read transposed (TRANSB)
+    const int cjb = opB == GSA_OP_C ? 2 : 0;               // CONJB
+    const int cj = (opA == GSA_OP_C ? 1 : 0) | cjb;        // CONJA | CONJB
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
+        const T a = host_scalar<T>(alpha, T(1)), b = host_scalar<T>(beta, T(0));
         if (opA == GSA_OP_N)
             return gather_csr<T, I>(s, m, n, k, (const I*)rowptr, (const I*)colval,
-                                    (const T*)nzval, nnz, (const T*)B, ldb, opB != GSA_OP_N,
-                                    (T*)C, ldc, alpha, beta, plan, itype);
-        if (k == 1 && n > 0 && plan &&
-            bin_plan_matches(plan->bins_t, itype, n, m, nnz, rowptr, colval))
-            return bin_spmv(s, *plan->bins_t, vtype, nullptr, nzval, B,
-                            opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
-        if (k > 1 && plan) {
+                                    (const T*)nzval, nnz, (const T*)B, ldb, bt, (T*)C, ldc, a, b,
+                                    plan, itype, nullptr, (const void*)nullptr, false, cjb);
+        if constexpr (std::is_floating_point_v<T>) {  // the row bins hold real products
+            if (k == 1 && n > 0 && plan &&
+                bin_plan_matches(plan->bins_t, itype, n, m, nnz, rowptr, colval))
+                return bin_spmv(s, *plan->bins_t, vtype, nullptr, nzval, B, bt ? ldb : 1, C,
+                                (double)a, (double)b);
+        }
+        if ((k > 1 || !std::is_floating_point_v<T>) && plan) {  // gather view
             if (const RowView* v = gview_for(plan->gview_t, itype, n, nnz, rowptr, colval))
                 return view_spmv<T, I>(s, *v, m, (const I*)nullptr, (const T*)nzval,
-                                       (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha,
-                                       beta, itype, k);
+                                       (const T*)B, ldb, bt, (T*)C, ldc, a, b, itype, k, cj);
         }
         return scatter_csr<T, I>(s, m, n, k, (const I*)rowptr, (const I*)colval, (const T*)nzval,
-                                 nnz, (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
-                                 plan, itype);
+                                 nnz, (const T*)B, ldb, bt, (T*)C, ldc, a, b, plan, itype, cj);
     });
 }
 
@@ -1438,43 +1450,49 @@ int gsa_spmv_csr_noalloc(void* stream, int vtype, int itype, int64_t m, int64_t 
         using T = decltype(tv);
         using I = decltype(ti);
         return spmv_noalloc<T, I>(s, itype, m, n, (const I*)rowptr, (const I*)colval,
-                                  (const T*)nzval, nnz, (const T*)x, (T*)y, alpha, beta, ws);
+                                  (const T*)nzval, nnz, (const T*)x, (T*)y, (T)alpha, (T)beta,
+                                  ws);
     });
 }
 
 int gsa_spmm_csc(void* stream, int vtype, int itype, int opA, int64_t m, int64_t n, int64_t k,
                  const void* colptr, const void* rowval, const void* nzval, int64_t nnz,
-                 const void* B, int64_t ldb, int opB, void* C, int64_t ldc, double alpha,
-                 double beta, gsa_plan plan) {
+                 const void* B, int64_t ldb, int opB, void* C, int64_t ldc, const void* alpha,
+                 const void* beta, gsa_plan plan) {
     GSA_TRY(ensure_init());
     GSA_TRY(check_common(opA, opB, m, n, k, nnz, ldb, ldc));
     hipStream_t s = as_stream(stream);
+    const int bt = opB != GSA_OP_N;
+    const int cjb = opB == GSA_OP_C ? 2 : 0;
+    const int cj = (opA == GSA_OP_C ? 1 : 0) | cjb;
     // A (m x n) in CSC is Aᵀ (n x m) in CSR: A x is the scatter, Aᵀ x the gather.
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
+        const T a = host_scalar<T>(alpha, T(1)), b = host_scalar<T>(beta, T(0));
         if (opA == GSA_OP_N) {
-            if (k == 1 && m > 0 && plan &&
-                bin_plan_matches(plan->bins, itype, m, n, nnz, colptr, rowval))
-                return bin_spmv(s, *plan->bins, vtype, nullptr, nzval, B,
-                                opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
+            if constexpr (std::is_floating_point_v<T>) {
+                if (k == 1 && m > 0 && plan &&
+                    bin_plan_matches(plan->bins, itype, m, n, nnz, colptr, rowval))
+                    return bin_spmv(s, *plan->bins, vtype, nullptr, nzval, B, bt ? ldb : 1, C,
+                                    (double)a, (double)b);
+            }
             const RowView* v = k == 1 ? view_for(plan, itype, m, nnz, colptr, rowval) : nullptr;
             if (v)  // sorted-key form: rows gathered through the plan's row view
                 return view_spmv<T, I>(s, *v, n, (const I*)nullptr, (const T*)nzval, (const T*)B,
-                                       ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta, itype);
-            if (k > 1 && plan) {
+                                       ldb, bt, (T*)C, ldc, a, b, itype, 1, cjb);
+            if ((k > 1 || !std::is_floating_point_v<T>) && plan) {  // gather view
                 if (const RowView* g = gview_for(plan->gview, itype, m, nnz, colptr, rowval))
                     return view_spmv<T, I>(s, *g, n, (const I*)nullptr, (const T*)nzval,
-                                           (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha,
-                                           beta, itype, k);
+                                           (const T*)B, ldb, bt, (T*)C, ldc, a, b, itype, k, cjb);
             }
             return scatter_csr<T, I>(s, n, m, k, (const I*)colptr, (const I*)rowval,
-                                     (const T*)nzval, nnz, (const T*)B, ldb, opB != GSA_OP_N,
-                                     (T*)C, ldc, alpha, beta, plan, itype);
+                                     (const T*)nzval, nnz, (const T*)B, ldb, bt, (T*)C, ldc, a, b,
+                                     plan, itype, cjb);
         }
         return gather_csr<T, I>(s, n, m, k, (const I*)colptr, (const I*)rowval, (const T*)nzval,
-                                nnz, (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
-                                plan, itype);
+                                nnz, (const T*)B, ldb, bt, (T*)C, ldc, a, b, plan, itype, nullptr,
+                                (const void*)nullptr, false, cj);
     });
 }
 
@@ -1519,50 +1537,54 @@ int gsa_spmm_csr_ex(void* stream, int vtype, int itype, int64_t m, int64_t n, in
         using I = decltype(ti);
         return gather_csr<T, I, T, void>(s, m, n, k, (const I*)rowptr, (const I*)colval,
                                          (const T*)nzval, nnz, (const T*)B, ldb, opB != GSA_OP_N,
-                                         (T*)C, ldc, alpha, beta, plan, itype, nullptr, nullptr,
-                                         opC == GSA_OP_T);
+                                         (T*)C, ldc, (T)alpha, (T)beta, plan, itype, nullptr,
+                                         nullptr, opC == GSA_OP_T);
     });
 }
 
 int gsa_spmm_coo(void* stream, int vtype, int itype, int opA, int64_t m, int64_t n, int64_t k,
                  const void* rowind, const void* colind, const void* nzval, int64_t nnz,
-                 const void* B, int64_t ldb, int opB, void* C, int64_t ldc, double alpha,
-                 double beta, gsa_plan plan) {
+                 const void* B, int64_t ldb, int opB, void* C, int64_t ldc, const void* alpha,
+                 const void* beta, gsa_plan plan) {
     GSA_TRY(ensure_init());
     GSA_TRY(check_common(opA, opB, m, n, k, nnz, ldb, ldc));
     hipStream_t s = as_stream(stream);
     const int trans = opA != GSA_OP_N;
     const int64_t rows_out = trans ? n : m, cols_in = trans ? m : n;
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    const int bt = opB != GSA_OP_N;
+    const int cjb = opB == GSA_OP_C ? 2 : 0;
+    const int cj = (opA == GSA_OP_C ? 1 : 0) | cjb;
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
+        const T a = host_scalar<T>(alpha, T(1)), b = host_scalar<T>(beta, T(0));
         if (rows_out == 0 || k == 0) return (int)GSA_OK;
-        if (!trans && k == 1 && plan &&
-            bin_plan_matches(plan->bins, itype, m, n, nnz, rowind, colind))
-            return bin_spmv(s, *plan->bins, vtype, colind, nzval, B,
-                            opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
+        if constexpr (std::is_floating_point_v<T>) {  // the row bins hold real products
+            if (!trans && k == 1 && plan &&
+                bin_plan_matches(plan->bins, itype, m, n, nnz, rowind, colind))
+                return bin_spmv(s, *plan->bins, vtype, colind, nzval, B, bt ? ldb : 1, C,
+                                (double)a, (double)b);
+            if (trans && k == 1 && plan &&
+                bin_plan_matches(plan->bins_t, itype, n, m, nnz, rowind, colind))
+                return bin_spmv(s, *plan->bins_t, vtype, rowind, nzval, B, bt ? ldb : 1, C,
+                                (double)a, (double)b);
+        }
         const RowView* v = (!trans && k == 1) ? view_for(plan, itype, m, nnz, rowind, colind)
                                               : nullptr;
         if (v)  // sorted-key form: rows gathered through the plan's row view
             return view_spmv<T, I>(s, *v, n, (const I*)colind, (const T*)nzval, (const T*)B, ldb,
-                                   opB != GSA_OP_N, (T*)C, ldc, alpha, beta, itype);
-        if (trans && k == 1 && plan &&
-            bin_plan_matches(plan->bins_t, itype, n, m, nnz, rowind, colind))
-            return bin_spmv(s, *plan->bins_t, vtype, rowind, nzval, B,
-                            opB != GSA_OP_N ? ldb : 1, C, alpha, beta);
-        if (k > 1 && plan) {
+                                   bt, (T*)C, ldc, a, b, itype, 1, cj);
+        if ((k > 1 || !std::is_floating_point_v<T>) && plan) {  // gather view
             if (const RowView* g = gview_for(trans ? plan->gview_t : plan->gview, itype, rows_out,
                                              nnz, rowind, colind))
                 return view_spmv<T, I>(s, *g, cols_in, (const I*)nullptr, (const T*)nzval,
-                                       (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc, alpha, beta,
-                                       itype, k);
+                                       (const T*)B, ldb, bt, (T*)C, ldc, a, b, itype, k, cj);
         }
-        GSA_TRY(apply_beta<T>(s, rows_out, k, (T*)C, ldc, beta));
+        GSA_TRY(apply_beta<T>(s, rows_out, k, (T*)C, ldc, b));
         if (nnz == 0 || cols_in == 0) return (int)GSA_OK;
         hipLaunchKernelGGL((k_coo_scatter<T, I>), dim3(grid_for(nnz * k, 256)), dim3(256), 0, s,
                            rows_out, cols_in, k, nnz, (const I*)rowind, (const I*)colind,
-                           (const T*)nzval, trans, (const T*)B, ldb, opB != GSA_OP_N, (T*)C, ldc,
-                           (T)alpha);
+                           (const T*)nzval, trans, (const T*)B, ldb, bt, (T*)C, ldc, a, cj);
         GSA_HIP(hipGetLastError());
         return (int)GSA_OK;
     });

@@ -1137,7 +1137,7 @@ int gsa_csr_dropzeros_count(void* stream, int vtype, int itype, int64_t m, const
     GSA_TRY(ensure_init());
     GSA_CHECK(m >= 0 && nnz_out, GSA_ERR_INVALID, "gsa_csr_dropzeros_count: bad arguments");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         I last = 1;
@@ -1167,7 +1167,7 @@ int gsa_csr_dropzeros_fill(void* stream, int vtype, int itype, int64_t m, const 
     GSA_TRY(ensure_init());
     GSA_CHECK(m >= 0, GSA_ERR_INVALID, "gsa_csr_dropzeros_fill: bad arguments");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         if (m == 0) return (int)GSA_OK;
@@ -1194,7 +1194,7 @@ int gsa_coo_dropzeros_count(void* stream, int vtype, int64_t nnz, const void* nz
     GSA_TRY(ensure_init());
     GSA_CHECK(nnz >= 0 && nnz_out, GSA_ERR_INVALID, "gsa_coo_dropzeros_count: bad arguments");
     hipStream_t s = as_stream(stream);
-    return dispatch_v(vtype, [&](auto tv) {
+    return dispatch_va(vtype, [&](auto tv) {
         using T = decltype(tv);
         *nnz_out = 0;
         if (nnz == 0) return (int)GSA_OK;
@@ -1218,7 +1218,7 @@ int gsa_coo_dropzeros_fill(void* stream, int vtype, int itype, int64_t nnz, cons
     GSA_TRY(ensure_init());
     GSA_CHECK(nnz >= 0, GSA_ERR_INVALID, "gsa_coo_dropzeros_fill: bad arguments");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         if (nnz == 0) return (int)GSA_OK;
@@ -1241,7 +1241,7 @@ int gsa_csr_add_count(void* stream, int vtype, int itype, int64_t m, const void*
     GSA_TRY(ensure_init());
     GSA_CHECK(m >= 0 && nnzC, GSA_ERR_INVALID, "gsa_csr_add_count: bad arguments");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         if (m > 0)
@@ -1260,7 +1260,7 @@ int gsa_csr_add_fill(void* stream, int vtype, int itype, int64_t m, const void* 
     GSA_TRY(ensure_init());
     GSA_CHECK(m >= 0, GSA_ERR_INVALID, "gsa_csr_add_fill: bad arguments");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         if (m == 0) return (int)GSA_OK;
@@ -1288,7 +1288,7 @@ int gsa_coo_add_begin(void* stream, int vtype, int itype, int64_t m, int64_t n, 
     hipStream_t s = as_stream(stream);
     *ctx = nullptr;
     *nnzC = 0;
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         auto* c = new gsa_ctx_s();
@@ -1469,7 +1469,7 @@ int gsa_coo_add_finish(gsa_ctx ctx, void* rowC, void* colC, void* valC) {
     GSA_CHECK(ctx && ctx->kind == 1, GSA_ERR_INVALID, "gsa_coo_add_finish: bad context");
     hipStream_t s = ctx->stream;
     const int64_t tot = ctx->aux[0];
-    int r = dispatch_vi(ctx->vtype, ctx->itype, [&](auto tv, auto ti) {
+    int r = dispatch_vai(ctx->vtype, ctx->itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         if (tot == 0) return (int)GSA_OK;
@@ -1528,7 +1528,7 @@ int gsa_kron_coo(void* stream, int vtype, int itype, int64_t mB, int64_t nB, int
     GSA_CHECK(mB >= 0 && nB >= 0 && nnzA >= 0 && nnzB >= 0, GSA_ERR_INVALID,
               "gsa_kron_coo: bad arguments");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         return run_kron_coo<T, I>(s, mB, nB, nnzA, (const I*)rowA, (const I*)colA, (const T*)valA,
@@ -1545,7 +1545,7 @@ int gsa_kron_csr(void* stream, int vtype, int itype, int64_t mA, int64_t nA, int
     GSA_CHECK(mA >= 0 && nA >= 0 && mB >= 0 && nB >= 0 && nnzA >= 0 && nnzB >= 0, GSA_ERR_INVALID,
               "gsa_kron_csr: bad arguments");
     hipStream_t s = as_stream(stream);
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         const int64_t mC = mA * mB, nnzC = nnzA * nnzB;
@@ -1605,7 +1605,7 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
     hipStream_t s = as_stream(stream);
     *ctx = nullptr;
     *nnzC = 0;
-    return dispatch_vi(vtype, itype, [&](auto tv, auto ti) {
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         auto* c = new gsa_ctx_s();
@@ -1771,7 +1771,7 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
 int gsa_spgemm_csr_finish(gsa_ctx ctx, void* rowptrC, void* colC, void* valC) {
     GSA_CHECK(ctx && ctx->kind == 2, GSA_ERR_INVALID, "gsa_spgemm_csr_finish: bad context");
     hipStream_t s = ctx->stream;
-    int r = dispatch_vi(ctx->vtype, ctx->itype, [&](auto tv, auto ti) {
+    int r = dispatch_vai(ctx->vtype, ctx->itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
         using I = decltype(ti);
         const int64_t m = ctx->m;

@@ -20,6 +20,7 @@ GSA_ERR_OOM = -4
 GSA_ERR_UNSUPPORTED = -5
 
 GSA_F64, GSA_F32, GSA_BF16 = 0, 1, 2
+GSA_C64, GSA_C128, GSA_VI32, GSA_VI64 = 3, 4, 5, 6
 GSA_I32, GSA_I64 = 0, 1
 GSA_OP_N, GSA_OP_T, GSA_OP_C = 0, 1, 2
 GSA_PLAN_WINDOWS, GSA_PLAN_XCD_ORDER, GSA_PLAN_ROW_VIEW, GSA_PLAN_ROW_BINS, GSA_PLAN_BATCH = 1, 2, 4, 8, 16
@@ -76,11 +77,11 @@ _SIGS = {
     "gsa_plan_add_transpose": [_vp, _vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp],
     "gsa_plan_add_gather_view": [_vp, _vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp],
     "gsa_spmm_csr": [_vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,
-                     _vp, _i64, _i32, _vp, _i64, _f64, _f64, _vp],
+                     _vp, _i64, _i32, _vp, _i64, _vp, _vp, _vp],
     "gsa_spmm_csc": [_vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,
-                     _vp, _i64, _i32, _vp, _i64, _f64, _f64, _vp],
+                     _vp, _i64, _i32, _vp, _i64, _vp, _vp, _vp],
     "gsa_spmm_coo": [_vp, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,
-                     _vp, _i64, _i32, _vp, _i64, _f64, _f64, _vp],
+                     _vp, _i64, _i32, _vp, _i64, _vp, _vp, _vp],
     "gsa_plan_create_coo": [_vp, _i32, _i64, _i64, _i64, _vp, _vp, ctypes.POINTER(_vp)],
     "gsa_plan_create_csc": [_vp, _i32, _i64, _i64, _i64, _vp, _vp, ctypes.POINTER(_vp)],
     "gsa_spmm_csr_bf16": [_vp, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64,
@@ -107,18 +108,20 @@ _SIGS = {
                              _vp, _i64, ctypes.POINTER(_vp), _pi64],
     "gsa_spgemm_csr_finish": [_vp, _vp, _vp, _vp],
     "gsa_ctx_destroy": [_vp],
-    "gsa_trace_ptr": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp, ctypes.POINTER(_f64)],
-    "gsa_trace_coo": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, ctypes.POINTER(_f64)],
-    "gsa_sum_values": [_vp, _i32, _i64, _vp, ctypes.POINTER(_f64)],
-    "gsa_dot3_ptr": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _vp,
-                     ctypes.POINTER(_f64)],
-    "gsa_dot3_coo": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i32,
-                     ctypes.POINTER(_f64)],
-    "gsa_symmetric_ptr": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp,
+    "gsa_trace_ptr": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp],
+    "gsa_trace_coo": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp],
+    "gsa_sum_values": [_vp, _i32, _i64, _vp, _vp],
+    "gsa_dot3_ptr": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _i32, _vp,
+                     _vp],
+    "gsa_dot3_coo": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _i32, _i32,
+                     _vp],
+    "gsa_symmetric_ptr": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _i32, _vp,
                           ctypes.POINTER(_i32)],
     "gsa_add_diag_count": [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _pi64],
-    "gsa_add_diag_fill": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _f64, _vp, _vp, _vp, _vp],
-    "gsa_diag_coo": [_vp, _i32, _i32, _i64, _vp, _f64, _vp, _vp, _vp],
+    "gsa_add_diag_fill": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "gsa_diag_coo": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp],
+    "gsa_spvec_dot": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp],
+    "gsa_norm_values": [_vp, _i32, _i64, _vp, _f64, ctypes.POINTER(_f64)],
     "gsa_batch_plan_create": [_vp, _i32, _i32, _i64, ctypes.POINTER(BatchDesc), _i32,
                               ctypes.POINTER(_vp)],
     "gsa_batch_spmm": [_vp, _vp, _f32, _f32],

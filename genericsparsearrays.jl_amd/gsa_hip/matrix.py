@@ -21,8 +21,15 @@ import torch
 from . import _lib as L
 from ._lib import ArgumentError, DimensionMismatch
 
-VTYPES = {torch.float64: L.GSA_F64, torch.float32: L.GSA_F32}
+# value types: the reference's test matrix (test/runtests.jl:34-68) -- Int32,
+# Int64, Float32, Float64, ComplexF32, ComplexF64
+VTYPES = {torch.float64: L.GSA_F64, torch.float32: L.GSA_F32,
+          torch.complex64: L.GSA_C64, torch.complex128: L.GSA_C128,
+          torch.int32: L.GSA_VI32, torch.int64: L.GSA_VI64}
 ITYPES = {torch.int32: L.GSA_I32, torch.int64: L.GSA_I64}
+_SCALAR_CT = {torch.float64: ctypes.c_double, torch.float32: ctypes.c_float,
+              torch.complex128: ctypes.c_double * 2, torch.complex64: ctypes.c_float * 2,
+              torch.int64: ctypes.c_int64, torch.int32: ctypes.c_int32}
 
 
 def vtype_of(t: torch.Tensor) -> int:
@@ -30,7 +37,36 @@ def vtype_of(t: torch.Tensor) -> int:
         return VTYPES[t.dtype]
     except KeyError:
         raise ArgumentError(f"element type {t.dtype} is not provided by the HIP path "
-                            "(Float64 and Float32 are)") from None
+                            "(Int32, Int64, Float32, Float64, ComplexF32 and ComplexF64 "
+                            "are)") from None
+
+
+def is_complex_dtype(dt) -> bool:
+    return dt in (torch.complex64, torch.complex128)
+
+
+def scalar_ref(value, dtype):
+    """A host value of the element type `dtype`, passed to the C ABI by pointer
+    (alpha, beta, lambda).  Integers wrap to their width, as Julia's do."""
+    ct = _SCALAR_CT.get(dtype)
+    if ct is None:
+        raise ArgumentError(f"no scalar of element type {dtype}")
+    if dtype in (torch.complex64, torch.complex128):
+        z = complex(value)
+        return ct(z.real, z.imag)
+    if dtype in (torch.int64, torch.int32):
+        if isinstance(value, complex) or (isinstance(value, float) and not float(value).is_integer()):
+            raise ArgumentError(f"InexactError: {value} is not an integer of {dtype}")
+        bits = 64 if dtype == torch.int64 else 32
+        v = int(value) & ((1 << bits) - 1)
+        if v >= 1 << (bits - 1):
+            v -= 1 << bits
+        return ct(v)
+    if isinstance(value, complex):
+        if value.imag != 0:
+            raise ArgumentError(f"InexactError: {value} is not real")
+        value = value.real
+    return ct(float(value))
 
 
 def itype_of(t: torch.Tensor) -> int:
@@ -525,9 +561,13 @@ class Transpose:
 
 
 class Adjoint(Transpose):
-    """LinearAlgebra.Adjoint; equal to Transpose for the real element types here."""
+    """LinearAlgebra.Adjoint: the transpose with conjugated values (equal to
+    Transpose for real element types)."""
 
     conj = True
+
+    def collect(self):
+        return np.conj(self.parent.collect().T)
 
 
 def scalar_dtype(x):
@@ -538,21 +578,49 @@ def scalar_dtype(x):
         return None
     if isinstance(x, np.floating):
         return torch.float32 if x.dtype == np.float32 else torch.float64
+    if isinstance(x, np.complexfloating):
+        return torch.complex64 if x.dtype == np.complex64 else torch.complex128
     if isinstance(x, torch.Tensor):
         if x.numel() != 1:
             raise ArgumentError("a scalar is needed")
-        return x.dtype if x.dtype.is_floating_point else None
+        return x.dtype if (x.dtype.is_floating_point or x.dtype.is_complex) else None
     if isinstance(x, float):
         return torch.float64
+    if isinstance(x, complex):
+        return torch.complex128
     raise ArgumentError(f"unsupported scalar {type(x).__name__}")
 
 
+_KIND = {torch.int32: (0, 32), torch.int64: (0, 64), torch.float32: (1, 32),
+         torch.float64: (1, 64), torch.complex64: (2, 32), torch.complex128: (2, 64)}
+_OF_KIND = {(0, 32): torch.int32, (0, 64): torch.int64, (1, 32): torch.float32,
+            (1, 64): torch.float64, (2, 32): torch.complex64, (2, 64): torch.complex128}
+
+
+def promote_types(a, b):
+    """Julia's promote_type over Int32, Int64, Float32, Float64, ComplexF32 and
+    ComplexF64: integers widen to the wider integer and give way to any float or
+    complex type; floats widen; a complex type takes the wider real part."""
+    ka, wa = _KIND[a]
+    kb, wb = _KIND[b]
+    if ka == 0 and kb == 0:
+        return _OF_KIND[(0, max(wa, wb))]
+    if ka == 0:
+        return b
+    if kb == 0:
+        return a
+    return _OF_KIND[(max(ka, kb), max(wa, wb))]
+
+
 def promote_dtype(*dts):
-    """promote_type over Float32 / Float64 (None = a type that widens nothing)."""
+    """promote_type over the element types (None = a type that widens nothing)."""
     dts = [d for d in dts if d is not None]
     if not dts:
         return None
-    return torch.float64 if torch.float64 in dts else torch.float32
+    out = dts[0]
+    for d in dts[1:]:
+        out = promote_types(out, d)
+    return out
 
 
 class UniformScaling:
@@ -571,19 +639,26 @@ class UniformScaling:
         self.dtype = scalar_dtype(lam) if dtype is ... else dtype
 
     @property
-    def value(self) -> float:
-        return float(self.lam.item() if isinstance(self.lam, torch.Tensor) else self.lam)
+    def value(self):
+        """λ as a Python number: int for Bool and integers, complex for complex
+        scalars, float otherwise."""
+        v = self.lam.item() if isinstance(self.lam, torch.Tensor) else self.lam
+        if isinstance(v, (bool, np.bool_, int, np.integer)):
+            return int(v)
+        if isinstance(v, (complex, np.complexfloating)):
+            return complex(v)
+        return float(v)
 
     def __mul__(self, other):
         if isinstance(other, (AbstractGenericSparseMatrix, Transpose)):
             from . import ops
             return ops.scale(other, self.value)
-        return UniformScaling(self.value * float(other),
-                              promote_dtype(self.dtype, scalar_dtype(other)))
+        o = other.item() if isinstance(other, torch.Tensor) else other
+        return UniformScaling(self.value * o, promote_dtype(self.dtype, scalar_dtype(other)))
 
     def __rmul__(self, other):
-        return UniformScaling(float(other) * self.value,
-                              promote_dtype(self.dtype, scalar_dtype(other)))
+        o = other.item() if isinstance(other, torch.Tensor) else other
+        return UniformScaling(o * self.value, promote_dtype(self.dtype, scalar_dtype(other)))
 
     def __neg__(self):
         return UniformScaling(-self.value, self.dtype)

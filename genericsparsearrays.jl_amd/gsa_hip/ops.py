@@ -16,13 +16,10 @@ from . import _lib as L
 from ._lib import ArgumentError, DimensionMismatch
 from .matrix import (GenericSparseMatrixCSR, GenericSparseMatrixCSC, GenericSparseMatrixCOO,
                      SPARSE_TYPES, Transpose, Adjoint, UniformScaling, Diagonal, itype_of,
-                     vtype_of, ptr, stream_of, require_device, similar_dense, promote_dtype)
+                     vtype_of, ptr, stream_of, require_device, similar_dense, promote_dtype,
+                     promote_types, scalar_ref, is_complex_dtype, VTYPES)
 from . import convert
 
-_PROMOTE_OK = {
-    (torch.float32, torch.float32), (torch.float32, torch.float64),
-    (torch.float64, torch.float64),
-}
 
 
 def _unwrap(A):
@@ -35,7 +32,15 @@ def _unwrap(A):
 
 
 def _dense_operand(B: torch.Tensor):
-    """(tensor, k, ld, opB) for a dense vector or matrix in any stride order."""
+    """(tensor, k, ld, opB) for a dense vector or matrix in any stride order.  A
+    torch conjugate view read in row-major order is Julia's Adjoint of the
+    column-major matrix (opB = C, CONJB); other conjugate views are resolved."""
+    if B.is_conj():
+        if B.dim() == 2 and B.stride(1) == 1 and B.shape[0] > 1 and B.stride(0) >= B.shape[1]:
+            base = B.conj()  # the stored values; the kernels conjugate them
+            _, k, ld, _ = _dense_operand(base)
+            return base, k, ld, L.GSA_OP_C
+        B = B.resolve_conj()
     if B.dim() == 1:
         if B.stride(0) != 1:
             B = B.contiguous()
@@ -76,23 +81,34 @@ def mul_(C: torch.Tensor, A, B: torch.Tensor, alpha=1.0, beta=0.0):
         raise DimensionMismatch(f"second dimension of B, {b_cols}, does not match the second "
                                 f"dimension of C, {c_cols}")
     T3 = C.dtype
-    for src in (Ap.nzval.dtype, B.dtype):
-        if (src, T3) not in _PROMOTE_OK:
-            raise ArgumentError("element types of A, B, α, and β must be promotable to the "
-                                "element type of C")
+    # promote_type(T1, T2, eltype(α), eltype(β)) <: T3 (matrix_csr.jl:176-180)
+    bad = T3 not in VTYPES or any(d not in VTYPES or promote_types(d, T3) != T3
+                                  for d in (Ap.nzval.dtype, B.dtype))
+    try:
+        a_ref = scalar_ref(alpha, T3) if not bad else None
+        b_ref = scalar_ref(beta, T3) if not bad else None
+    except ArgumentError:
+        bad = True
+    if bad:
+        raise ArgumentError("element types of A, B, α, and β must be promotable to the "
+                            "element type of C")
     dev = require_device(C, Ap.nzval, B)
     vals = Ap.nzval if Ap.nzval.dtype == T3 else Ap.nzval.to(T3)
     Bt = B if B.dtype == T3 else B.to(T3)
     Bt, k, ldb, opB = _dense_operand(Bt)
     # C must be column-major for the kernels; compute into a temp otherwise
     Cw = C
-    if C.dim() == 2 and not (C.stride(0) == 1 and (c_cols == 1 or C.stride(1) >= max(c_rows, 1))):
+    if C.is_conj() or (C.dim() == 2 and not (C.stride(0) == 1 and
+                                             (c_cols == 1 or C.stride(1) >= max(c_rows, 1)))):
         Cw = similar_dense(c_rows, c_cols, T3, dev)
         if beta != 0:
             Cw.copy_(C)
     ldc = max(c_rows, 1) if Cw.dim() == 1 else max(Cw.stride(1), c_rows, 1)
     s = stream_of(dev)
     vt = vtype_of(Cw)
+    # the row bins accumulate real products; other value types take the gather
+    # views for every scatter-shaped product (no atomics there either)
+    real = T3 in (torch.float32, torch.float64)
     if isinstance(Ap, GenericSparseMatrixCSR):
         it = itype_of(Ap.rowptr)
         plan = None
@@ -100,38 +116,38 @@ def mul_(C: torch.Tensor, A, B: torch.Tensor, alpha=1.0, beta=0.0):
             if opA == L.GSA_OP_N:  # the atomic scatter
                 plan = Ap.plan()
             else:
-                plan = Ap.plan_transposed() if k == 1 else Ap.plan_gather()
+                plan = Ap.plan_transposed() if (k == 1 and real) else Ap.plan_gather()
         L.call("gsa_spmm_csr", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowptr), ptr(Ap.colval),
-               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
-               plan)
+               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, ctypes.byref(a_ref),
+               ctypes.byref(b_ref), plan)
     elif isinstance(Ap, GenericSparseMatrixCSC):
         it = itype_of(Ap.colptr)
         plan = None
         if Ap.nnz() > 0:  # A * B with several columns: the gather view replaces the scatter
-            plan = Ap.plan_gather() if (opA == L.GSA_OP_N and k > 1) else Ap.plan()
+            plan = Ap.plan_gather() if (opA == L.GSA_OP_N and (k > 1 or not real)) else Ap.plan()
         L.call("gsa_spmm_csc", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.colptr), ptr(Ap.rowval),
-               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
-               plan)
+               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, ctypes.byref(a_ref),
+               ctypes.byref(b_ref), plan)
     else:
         it = itype_of(Ap.rowind)
         # SpMV (k = 1): the row view or bins for A * x, the transposed bins for
         # transpose(A) * x; other products scatter
         plan = None
         if Ap.nnz() > 0:
-            if k == 1:
+            if k == 1 and real:
                 plan = Ap.plan() if opA == L.GSA_OP_N else Ap.plan_transposed()
             else:
                 plan = Ap.plan_gather(trans=opA != L.GSA_OP_N)
         L.call("gsa_spmm_coo", s, vt, it, opA, Ap.m, Ap.n, k, ptr(Ap.rowind), ptr(Ap.colind),
-               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, float(alpha), float(beta),
-               plan)
+               ptr(vals), Ap.nnz(), ptr(Bt), ldb, opB, ptr(Cw), ldc, ctypes.byref(a_ref),
+               ctypes.byref(b_ref), plan)
     if Cw is not C:
         C.copy_(Cw)
     return C
 
 
 def _result_dtype(*dts):
-    return torch.float64 if torch.float64 in dts else torch.float32
+    return promote_dtype(*dts)
 
 
 def times(A, B):
@@ -428,8 +444,33 @@ def spgemm(A, B):
 
 
 # ------------------------------------------------------------ reductions ----
+def _acc_buffer():
+    """Room for the accumulator a reduction returns: a Float64, a ComplexF64 or
+    an Int64 (16 bytes covers each)."""
+    return (ctypes.c_byte * 16)()
+
+
+def _acc_value(buf, dtype, widen_int=False):
+    """The accumulator in `buf` as a host scalar of the element type `dtype`
+    (Float32 rounded once; integers wrap to their width, or stay Int64 when
+    `widen_int`, as Julia's sum of small integers does)."""
+    raw = bytes(buf)
+    if dtype in (torch.complex64, torch.complex128):
+        re, im = np.frombuffer(raw, dtype=np.float64, count=2)
+        if dtype == torch.complex64:
+            return complex(np.complex64(complex(float(re), float(im))))
+        return complex(float(re), float(im))
+    if dtype in (torch.int32, torch.int64):
+        v = int(np.frombuffer(raw, dtype=np.int64, count=1)[0])
+        if dtype == torch.int32 and not widen_int:
+            v = int(np.int64(v).astype(np.int32))
+        return v
+    v = float(np.frombuffer(raw, dtype=np.float64, count=1)[0])
+    return float(np.float32(v)) if dtype == torch.float32 else v
+
+
 def _as_scalar(v: float, dtype):
-    """The host result in the element type (a Float32 result is rounded once)."""
+    """A host Float64 in the element type (a Float32 result is rounded once)."""
     return float(np.float32(v)) if dtype == torch.float32 else float(v)
 
 
@@ -443,34 +484,36 @@ def _ptr_form(A):
 def tr(A):
     """LinearAlgebra.tr (src/matrix_csr/matrix_csr.jl:118-142,
     src/matrix_csc/matrix_csc.jl:117-140, src/matrix_coo/matrix_coo.jl:131-150);
-    tr(transpose(A)) = tr(A) for real values."""
-    Ap, _ = _unwrap(A)
+    tr(transpose(A)) = tr(A), tr(adjoint(A)) = conj(tr(A))."""
+    Ap, opA = _unwrap(A)
     if not isinstance(Ap, SPARSE_TYPES):
         raise ArgumentError(f"tr: unsupported operand {type(A).__name__}")
     if Ap.m != Ap.n:
         raise DimensionMismatch("Matrix must be square to compute the trace.")
     dev = require_device(Ap._idx(), Ap.nzval)
-    out = ctypes.c_double()
+    out = _acc_buffer()
     if isinstance(Ap, GenericSparseMatrixCOO):
         L.call("gsa_trace_coo", stream_of(dev), vtype_of(Ap.nzval), itype_of(Ap.rowind), Ap.nnz(),
-               ptr(Ap.rowind), ptr(Ap.colind), ptr(Ap.nzval), ctypes.byref(out))
+               ptr(Ap.rowind), ptr(Ap.colind), ptr(Ap.nzval), out)
     else:
         major, _, pt, ix = _ptr_form(Ap)
         L.call("gsa_trace_ptr", stream_of(dev), vtype_of(Ap.nzval), itype_of(pt), major, Ap.nnz(),
-               ptr(pt), ptr(ix), ptr(Ap.nzval), Ap.cached_plan(), ctypes.byref(out))
-    return _as_scalar(out.value, Ap.nzval.dtype)
+               ptr(pt), ptr(ix), ptr(Ap.nzval), Ap.cached_plan(), out)
+    v = _acc_value(out, Ap.nzval.dtype)
+    return v.conjugate() if opA == L.GSA_OP_C and isinstance(v, complex) else v
 
 
 def sum_(A):
-    """Base.sum(A) = sum(nonzeros(A)) (src/core.jl:24)."""
-    Ap, _ = _unwrap(A)
+    """Base.sum(A) = sum(nonzeros(A)) (src/core.jl:24); Int32 values sum to an
+    Int64, as Julia's sum of small integers does; sum(adjoint(A)) is conjugated."""
+    Ap, opA = _unwrap(A)
     if not isinstance(Ap, SPARSE_TYPES):
         raise ArgumentError(f"sum: unsupported operand {type(A).__name__}")
     dev = require_device(Ap.nzval)
-    out = ctypes.c_double()
-    L.call("gsa_sum_values", stream_of(dev), vtype_of(Ap.nzval), Ap.nnz(), ptr(Ap.nzval),
-           ctypes.byref(out))
-    return _as_scalar(out.value, Ap.nzval.dtype)
+    out = _acc_buffer()
+    L.call("gsa_sum_values", stream_of(dev), vtype_of(Ap.nzval), Ap.nnz(), ptr(Ap.nzval), out)
+    v = _acc_value(out, Ap.nzval.dtype, widen_int=True)
+    return v.conjugate() if opA == L.GSA_OP_C and isinstance(v, complex) else v
 
 
 def dot(x: torch.Tensor, A, y: torch.Tensor):
@@ -491,33 +534,30 @@ def dot(x: torch.Tensor, A, y: torch.Tensor):
         raise DimensionMismatch(f"second dimension of A, {nA}, does not match the length of y, "
                                 f"{y.shape[0]}")
     dev = require_device(x, Ap.nzval, y)
-    T = promote_dtype(Ap.nzval.dtype, x.dtype, y.dtype)
-    if T is None or any(d not in (torch.float32, torch.float64)
-                        for d in (Ap.nzval.dtype, x.dtype, y.dtype)):
-        raise ArgumentError("dot(x, A, y): Float32 and Float64 operands are provided")
+    if any(d not in VTYPES for d in (Ap.nzval.dtype, x.dtype, y.dtype)):
+        raise ArgumentError("dot(x, A, y): unsupported element type")
+    T = promote_dtype(Ap.nzval.dtype, x.dtype, y.dtype)  # promote_type(T1, T2, T3)
     vals = Ap.nzval if Ap.nzval.dtype == T else Ap.nzval.to(T)
-    xt = x.to(T).contiguous()
-    yt = y.to(T).contiguous()
-    out = ctypes.c_double()
+    xt = x.resolve_conj().to(T).contiguous()
+    yt = y.resolve_conj().to(T).contiguous()
+    out = _acc_buffer()
     s = stream_of(dev)
+    conj_a = int(opA == L.GSA_OP_C)
     if isinstance(Ap, GenericSparseMatrixCOO):
         L.call("gsa_dot3_coo", s, vtype_of(vals), itype_of(Ap.rowind), Ap.m, Ap.n, Ap.nnz(),
                ptr(Ap.rowind), ptr(Ap.colind), ptr(vals), ptr(xt), ptr(yt),
-               int(opA != L.GSA_OP_N), ctypes.byref(out))
+               int(opA != L.GSA_OP_N), conj_a, out)
     else:
         major, minor, pt, ix = _ptr_form(Ap)
         # x pairs with the row: the major of CSR, the index of CSC (swapped by a wrapper)
         major_left = isinstance(Ap, GenericSparseMatrixCSR) == (opA == L.GSA_OP_N)
         L.call("gsa_dot3_ptr", s, vtype_of(vals), itype_of(pt), major, minor, Ap.nnz(), ptr(pt),
-               ptr(ix), ptr(vals), ptr(xt), ptr(yt), int(major_left), Ap.cached_plan(),
-               ctypes.byref(out))
-    return _as_scalar(out.value, T)
+               ptr(ix), ptr(vals), ptr(xt), ptr(yt), int(major_left), conj_a, Ap.cached_plan(),
+               out)
+    return _acc_value(out, T)
 
 
-def issymmetric(A) -> bool:
-    """LinearAlgebra.issymmetric (src/matrix_csr/matrix_csr.jl:611-628,
-    src/matrix_csc/matrix_csc.jl:616-633; COO through CSC, src/matrix_coo/matrix_coo.jl:811-815;
-    wrappers test the parent, src/core.jl:96-99)."""
+def _symmetric(A, hermitian: bool) -> bool:
     Ap, _ = _unwrap(A)
     if not isinstance(Ap, SPARSE_TYPES):
         raise ArgumentError(f"issymmetric: unsupported operand {type(A).__name__}")
@@ -531,14 +571,22 @@ def issymmetric(A) -> bool:
     dev = require_device(pt, ix, Ap.nzval)
     out = ctypes.c_int()
     L.call("gsa_symmetric_ptr", stream_of(dev), vtype_of(Ap.nzval), itype_of(pt), major, Ap.nnz(),
-           ptr(pt), ptr(ix), ptr(Ap.nzval), Ap.cached_plan(), ctypes.byref(out))
+           ptr(pt), ptr(ix), ptr(Ap.nzval), int(hermitian), Ap.cached_plan(), ctypes.byref(out))
     return bool(out.value)
 
 
+def issymmetric(A) -> bool:
+    """LinearAlgebra.issymmetric (src/matrix_csr/matrix_csr.jl:611-628,
+    src/matrix_csc/matrix_csc.jl:616-633; COO through CSC, src/matrix_coo/matrix_coo.jl:811-815;
+    wrappers test the parent, src/core.jl:96-99)."""
+    return _symmetric(A, False)
+
+
 def ishermitian(A) -> bool:
-    """LinearAlgebra.ishermitian (src/matrix_csr/matrix_csr.jl:630-647): for the real
-    element types provided, conj(v) == v and the test is issymmetric's."""
-    return issymmetric(A)
+    """LinearAlgebra.ishermitian (src/matrix_csr/matrix_csr.jl:630-647): every
+    off-diagonal value the conjugate of its mirror, every diagonal value its own
+    conjugate (kernel_check_symmetry_csr! with HERMITIAN, matrix_csr_kernels.jl:348-407)."""
+    return _symmetric(A, True)
 
 
 # ------------------------------------------------------------ A + λI --------
@@ -564,8 +612,9 @@ def _diag_coo(n: int, dev, vdtype, idtype, lam: float = 0.0, dvals=None):
     if dvals is not None:
         src = dvals if dvals.dtype == vdtype else dvals.to(vdtype)
         src = src.contiguous()
-    L.call("gsa_diag_coo", stream_of(dev), vtype_of(v), itype_of(r), n, ptr(src), float(lam),
-           ptr(r), ptr(c), ptr(v))
+    lam_ref = scalar_ref(lam, vdtype)
+    L.call("gsa_diag_coo", stream_of(dev), vtype_of(v), itype_of(r), n, ptr(src),
+           ctypes.byref(lam_ref), ptr(r), ptr(c), ptr(v))
     return GenericSparseMatrixCOO(n, n, r, c, v, copy=False)
 
 
@@ -597,8 +646,9 @@ def add_uniform_scaling(A, J: UniformScaling):
            ptr(newptr), ctypes.byref(cnt))
     newix = torch.empty(cnt.value, dtype=pt.dtype, device=dev)
     newv = torch.empty(cnt.value, dtype=T, device=dev)
+    lam_ref = scalar_ref(lam, T)
     L.call("gsa_add_diag_fill", s, vtype_of(vals), itype_of(pt), major, A.nnz(), ptr(pt), ptr(ix),
-           ptr(vals), lam, plan, ptr(newptr), ptr(newix), ptr(newv))
+           ptr(vals), ctypes.byref(lam_ref), plan, ptr(newptr), ptr(newix), ptr(newv))
     return type(A)(A.m, A.n, newptr, newix, newv, copy=False)
 
 
@@ -619,8 +669,8 @@ def _kron_diagonal(A, B):
         raise ArgumentError(f"kron: unsupported operand {type(S).__name__}")
     Scoo = convert.to_coo(S)
     dev = require_device(Scoo.rowind, Scoo.nzval, D.diag)
-    if D.dtype not in (torch.float32, torch.float64):
-        raise ArgumentError("kron: Diagonal values must be Float32 or Float64")
+    if D.dtype not in VTYPES:
+        raise ArgumentError(f"kron: unsupported Diagonal element type {D.dtype}")
     if D.diag is None:  # Diagonal(Fill(v, n)): kernel_kron_scaled_identity_coo!, same entries
         if D.device != dev:
             raise ArgumentError("All arrays must have the same backend")

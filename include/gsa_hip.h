@@ -37,12 +37,25 @@ extern "C" {
 #define GSA_F64  0   /* Float64                                                 */
 #define GSA_F32  1   /* Float32                                                 */
 #define GSA_BF16 2   /* BFloat16 storage with one Float32 scale per row (serving)*/
+#define GSA_C64  3   /* ComplexF32: (re, im) Float32 pairs, Julia's layout       */
+#define GSA_C128 4   /* ComplexF64: (re, im) Float64 pairs                       */
+#define GSA_VI32 5   /* Int32 values (wrapping two's-complement arithmetic)      */
+#define GSA_VI64 6   /* Int64 values (wrapping two's-complement arithmetic)      */
+/* Index types (Ti). */
 #define GSA_I32  0   /* Int32                                                   */
 #define GSA_I64  1   /* Int64                                                   */
+/* Value types the reference's test suite runs every matrix and vector family on
+ * (test/runtests.jl:34-68): Int32, Int64, Float32, Float64, ComplexF32,
+ * ComplexF64.  Scalars (alpha, beta, lambda) and reduction results are passed
+ * as host pointers to one value of the operand's value type.  Complex
+ * products are Julia's (re*re - im*im, re*im + im*re), each operation rounded
+ * on its own; a scalar whose imaginary part is zero multiplies both parts
+ * separately, as a real scalar does in Julia. */
 
 /* ---- operand wrappers: identity / Transpose / Adjoint ----------------------
- * Mirrors trans_adj_wrappers (src/core.jl:102-113).  For the real element types
- * provided here Adjoint == Transpose. */
+ * Mirrors trans_adj_wrappers (src/core.jl:102-113).  Adjoint conjugates the
+ * stored values (CONJA) and, on a dense operand, the dense values (CONJB);
+ * for real value types it equals Transpose. */
 #define GSA_OP_N 0
 #define GSA_OP_T 1
 #define GSA_OP_C 2
@@ -141,10 +154,13 @@ int gsa_plan_set_columns(void* stream, gsa_plan plan, const void* colval, int64_
 
 /* C = beta*C + alpha*opA(A)*opB(B) with A in CSR (m x n, `nnz` stored entries).
  * C is size(opA(A),1) x k column-major (ldc); opB(B) is size(opA(A),2) x k:
- * opB = N reads B[i + j*ldb], opB = T reads B[j + i*ldb].
+ * opB = N reads B[i + j*ldb], opB = T reads B[j + i*ldb], opB = C reads the
+ * conjugate of B[j + i*ldb].  Every value type (GSA_F64 ... GSA_VI32); alpha
+ * and beta point to host values of that type (NULL: alpha = 1, beta = 0).
  * beta == 0 zero-fills C first, beta == 1 leaves it, else scales it.
  * opA = N is the row gather of kernel_spmatmul_csr_N! and opA = T/C the scatter
- * of kernel_spmatmul_csr_T!.  `plan` may be NULL (analysis done inline).
+ * of kernel_spmatmul_csr_T! (C conjugates A's values, CONJA).  `plan` may be
+ * NULL (analysis done inline).
  * Replaces LinearAlgebra.mul!(C, ::W{GenericSparseMatrixCSR}, ::W{DenseVecOrMat}, α, β)
  * — src/matrix_csr/matrix_csr.jl:145-211, kernels src/matrix_csr/matrix_csr_kernels.jl:1-45. */
 int gsa_spmm_csr(void* stream, int vtype, int itype, int opA,
@@ -152,7 +168,7 @@ int gsa_spmm_csr(void* stream, int vtype, int itype, int opA,
                  const void* rowptr, const void* colval, const void* nzval, int64_t nnz,
                  const void* B, int64_t ldb, int opB,
                  void* C, int64_t ldc,
-                 double alpha, double beta, gsa_plan plan);
+                 const void* alpha, const void* beta, gsa_plan plan);
 
 /* Same for A in CSC (colptr length n+1).  opA = T/C gathers (windows over
  * colptr).  opA = N is the scatter of kernel_spmatmul_csc_N!; with a plan from
@@ -165,7 +181,7 @@ int gsa_spmm_csc(void* stream, int vtype, int itype, int opA,
                  const void* colptr, const void* rowval, const void* nzval, int64_t nnz,
                  const void* B, int64_t ldb, int opB,
                  void* C, int64_t ldc,
-                 double alpha, double beta, gsa_plan plan);
+                 const void* alpha, const void* beta, gsa_plan plan);
 
 /* Same for A in COO (any entry order, duplicates summed): atomic scatter, with
  * equal consecutive outputs summed in the wavefront first; with a plan from
@@ -178,7 +194,7 @@ int gsa_spmm_coo(void* stream, int vtype, int itype, int opA,
                  const void* rowind, const void* colind, const void* nzval, int64_t nnz,
                  const void* B, int64_t ldb, int opB,
                  void* C, int64_t ldc,
-                 double alpha, double beta, gsa_plan plan);
+                 const void* alpha, const void* beta, gsa_plan plan);
 
 /* Plans for the SpMV (opA = N, k = 1) of COO and CSC without atomics.
  * - Row bins (CSC, and COO whose entries are not in row order, for m up to
@@ -364,9 +380,11 @@ int gsa_ctx_destroy(gsa_ctx ctx);
 
 /* ========================== reductions over stored entries ================= */
 /* "ptr form": CSR as (n = m rows, ptr = rowptr, idx = colval); CSC as
- * (n = columns, ptr = colptr, idx = rowval).  The sums are accumulated in
- * Float64 in a fixed order (the same result on every call) and returned on the
- * host; the call synchronises the stream.  `plan` (may be NULL) is a plan built
+ * (n = columns, ptr = colptr, idx = rowval).  The sums are accumulated in a
+ * fixed order (the same result on every call) and returned on the host, in
+ * `out`: a Float64 for real value types, a ComplexF64 (re, im) for complex ones,
+ * an Int64 for integer ones (wrapping; its low 32 bits are the Int32 sum); the
+ * call synchronises the stream.  `plan` (may be NULL) is a plan built
  * on this `ptr` (gsa_plan_create, or gsa_plan_create_csc for CSC): its row
  * blocks are reused, else they are rebuilt for the call.  Majors of up to 64
  * entries are read by one lane, longer ones by a whole workgroup.
@@ -375,38 +393,42 @@ int gsa_ctx_destroy(gsa_ctx ctx);
  * included.  The caller checks squareness.  Replaces LinearAlgebra.tr(::CSR)
  * src/matrix_csr/matrix_csr.jl:118-142 and tr(::CSC) src/matrix_csc/matrix_csc.jl:117-140. */
 int gsa_trace_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
-                  const void* ptr, const void* idx, const void* val, gsa_plan plan, double* out);
+                  const void* ptr, const void* idx, const void* val, gsa_plan plan, void* out);
 /* tr(::COO): entries with rowind == colind.  src/matrix_coo/matrix_coo.jl:131-150. */
 int gsa_trace_coo(void* stream, int vtype, int itype, int64_t nnz, const void* rowind,
-                  const void* colind, const void* val, double* out);
-/* sum(A) = sum(nonzeros(A)) for every format.  src/core.jl:24. */
-int gsa_sum_values(void* stream, int vtype, int64_t nnz, const void* val, double* out);
-/* dot(x, A, y) = sum over stored entries of (x[p]*a)*y[q], each term rounded
- * in the value type: (p, q) = (major, index) when major_left != 0, else
- * (index, major).  CSR: major_left = 1 for A, 0 for transpose(A)/adjoint(A);
- * CSC the opposite.  x and y have the lengths the caller checked (size(opA(A))).
+                  const void* colind, const void* val, void* out);
+/* sum(A) = sum(nonzeros(A)) for every format; also sum(::GenericSparseVector).
+ * src/core.jl:24, src/vector/vector.jl. */
+int gsa_sum_values(void* stream, int vtype, int64_t nnz, const void* val, void* out);
+/* dot(x, A, y) = sum over stored entries of (conj(x[p])*a)*y[q], each term
+ * rounded in the value type, a conjugated when conj_a != 0 (Adjoint, CONJA):
+ * (p, q) = (major, index) when major_left != 0, else (index, major).  CSR:
+ * major_left = 1 for A, 0 for transpose(A)/adjoint(A); CSC the opposite.  x and
+ * y have the lengths the caller checked (size(opA(A))).
  * Replaces LinearAlgebra.dot(x, ::CSR, y) src/matrix_csr/matrix_csr.jl:214-277
  * (kernels matrix_csr_kernels.jl:47-131) and dot(x, ::CSC, y)
  * src/matrix_csc/matrix_csc.jl:216-279 (kernels matrix_csc_kernels.jl:47-131). */
 int gsa_dot3_ptr(void* stream, int vtype, int itype, int64_t nmaj, int64_t nidx, int64_t nnz,
                  const void* ptr, const void* idx, const void* val, const void* x, const void* y,
-                 int major_left, gsa_plan plan, double* out);
-/* dot(x, op(A), y) for COO (m x n): (x[row]*a)*y[col], or (x[col]*a)*y[row] when
- * trans != 0.  src/matrix_coo/matrix_coo.jl:224-285 (kernels matrix_coo_kernels.jl:45-131). */
+                 int major_left, int conj_a, gsa_plan plan, void* out);
+/* dot(x, op(A), y) for COO (m x n): (conj(x[row])*a)*y[col], or
+ * (conj(x[col])*a)*y[row] when trans != 0; a conjugated when conj_a != 0.
+ * src/matrix_coo/matrix_coo.jl:224-285 (kernels matrix_coo_kernels.jl:45-131). */
 int gsa_dot3_coo(void* stream, int vtype, int itype, int64_t m, int64_t n, int64_t nnz,
                  const void* rowind, const void* colind, const void* val, const void* x,
-                 const void* y, int trans, double* out);
+                 const void* y, int trans, int conj_a, void* out);
 /* issymmetric / ishermitian for a square matrix in ptr form: *out = 1 when
  * every off-diagonal entry (r, c) finds, by binary search of major c, an entry
- * at index r with an equal value (the indices of each major sorted, as the
- * reference's search assumes); for real values the two tests coincide.
+ * at index r with an equal value (hermitian != 0: equal to its conjugate, and
+ * every diagonal value equal to its own conjugate); the indices of each major
+ * sorted, as the reference's search assumes.
  * Replaces issymmetric/ishermitian(::CSR) src/matrix_csr/matrix_csr.jl:611-647
  * (kernel_check_symmetry_csr! matrix_csr_kernels.jl:348-407) and (::CSC)
  * src/matrix_csc/matrix_csc.jl:616-652.  COO converts to CSC first
  * (src/matrix_coo/matrix_coo.jl:811-821). */
 int gsa_symmetric_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
-                      const void* ptr, const void* idx, const void* val, gsa_plan plan,
-                      int* out);
+                      const void* ptr, const void* idx, const void* val, int hermitian,
+                      gsa_plan plan, int* out);
 
 /* ========================== A + λI ========================================= */
 /* CSR / CSC (ptr form, n x n) + UniformScaling λ != 0, two phases.  count:
@@ -422,18 +444,36 @@ int gsa_symmetric_ptr(void* stream, int vtype, int itype, int64_t n, int64_t nnz
 int gsa_add_diag_count(void* stream, int itype, int64_t n, int64_t nnz, const void* ptr,
                        const void* idx, gsa_plan plan, void* ptr_out, int64_t* nnz_out);
 int gsa_add_diag_fill(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
-                      const void* ptr, const void* idx, const void* val, double lambda,
+                      const void* ptr, const void* idx, const void* val, const void* lambda,
                       gsa_plan plan, const void* ptr_out, void* idx_out, void* val_out);
 
 /* The n diagonal entries (i, i, d_i), i = 1..n, of an n x n COO matrix: d_i =
  * dvals[i] (a Diagonal's values, in the value type) or, when dvals is null,
- * lambda.  The second operand of COO + λI (kernel_fill_diagonal_coo!
+ * *lambda (a host value of the value type).  The second operand of COO + λI (kernel_fill_diagonal_coo!
  * src/matrix_coo/matrix_coo_kernels.jl:312-325) and of kron with a Diagonal,
  * which is kron(COO, COO) with this operand (kernel_kron_diag_coo! /
  * kernel_kron_coo_diag! matrix_coo_kernels.jl:185-258 index the same entries;
  * src/matrix_coo/matrix_coo.jl:704-809). */
 int gsa_diag_coo(void* stream, int vtype, int itype, int64_t n, const void* dvals,
-                 double lambda, void* row, void* col, void* val);
+                 const void* lambda, void* row, void* col, void* val);
+
+/* ========================== GenericSparseVector ============================ */
+/* dot(x, y) for x a sparse vector (length n, nnz stored entries, one-based
+ * nzind) and y a dense device vector: sum of conj(nzval[i]) * y[nzind[i]] in a
+ * fixed order, returned in `out` as the reductions above return theirs; no
+ * atomics.  Synchronises.  Replaces LinearAlgebra.dot(::GenericSparseVector,
+ * ::DenseVector) src/vector/vector.jl:102-139 (dot(y, x) is its conjugate,
+ * :140-145). */
+int gsa_spvec_dot(void* stream, int vtype, int itype, int64_t n, int64_t nnz,
+                  const void* nzind, const void* nzval, const void* y, void* out);
+/* norm(v, p) of the stored values (nnz of them): p = 2, 1, Inf, -Inf, 0 (the
+ * count of nonzeros) or any other real p, as the largest |v| times the p-norm
+ * of v / max|v| (no overflow or underflow of the powers); complex values by
+ * their modulus.  *out is a Float64.  Synchronises.  Replaces
+ * LinearAlgebra.norm(::GenericSparseVector, p) src/vector/vector.jl:91-93
+ * (norm(nonzeros(V), p)). */
+int gsa_norm_values(void* stream, int vtype, int64_t nnz, const void* val, double p,
+                    double* out);
 
 /* ========================== batched uneven CSR SpMM ======================== */
 /* Many independent CSR matrices in ONE persistent launch.  Each instance i has

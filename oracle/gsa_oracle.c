@@ -15,6 +15,8 @@
  *
  * Each function cites the reference file:line it restates.
  */
+#include <complex.h>
+#undef I /* complex.h's imaginary unit: I is the index type here */
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -39,13 +41,43 @@ static void beta_apply_##S(I rows, I k, T* C, I ldc, T beta) {                  
     for (I j = 0; j < k; ++j)                                                       \
         for (I i = 0; i < rows; ++i) {                                              \
             if (beta == (T)0) C[i + j * ldc] = (T)0;                                \
-            else C[i + j * ldc] = C[i + j * ldc] * beta;                            \
+            else C[i + j * ldc] = SCAL_##S(C[i + j * ldc], beta);                   \
         }                                                                           \
 }
 
-/* B accessor for opB: N reads B[i + j*ldb]; T reads B[j + i*ldb]
- * (TRANSB handling, src/matrix_csr/matrix_csr_kernels.jl:16). */
+/* B accessor for opB: N reads B[i + j*ldb]; T (1) and C (2) read B[j + i*ldb],
+ * C conjugating it (TRANSB / CONJB, src/matrix_csr/matrix_csr_kernels.jl:16-18). */
 #define BAT(B, i, j, ldb, opB) ((opB) ? (B)[(j) + (i) * (ldb)] : (B)[(i) + (j) * (ldb)])
+
+/* Value-type helpers.  SCAL: x * s for a scalar s (alpha, beta); a complex
+ * scalar with a zero imaginary part multiplies the parts separately, as
+ * Julia's Complex * Real does (C's complex * real is also by parts).  CONJA
+ * conjugates a stored value when the current product is an Adjoint
+ * (ora_conj_a, set by ora_set_conj_a); CONJB conjugates B under opB == 2.
+ * Integer values use unsigned arithmetic of their width: two's-complement
+ * wrapping, as Julia's Int32/Int64 arithmetic wraps. */
+typedef double _Complex zc128;
+typedef float _Complex zc64;
+static int ora_conj_a = 0;
+void ora_set_conj_a(int c) { ora_conj_a = c; }
+#define SCAL_f64(x, s) ((x) * (s))
+#define SCAL_f32(x, s) ((x) * (s))
+#define SCAL_i64(x, s) ((x) * (s))
+#define SCAL_i32(x, s) ((x) * (s))
+#define SCAL_c128(x, s) (cimag(s) == 0 ? (x) * creal(s) : (x) * (s))
+#define SCAL_c64(x, s) (cimagf(s) == 0 ? (x) * crealf(s) : (x) * (s))
+#define CONJA_f64(v) (v)
+#define CONJA_f32(v) (v)
+#define CONJA_i64(v) (v)
+#define CONJA_i32(v) (v)
+#define CONJA_c128(v) (ora_conj_a ? conj(v) : (v))
+#define CONJA_c64(v) (ora_conj_a ? conjf(v) : (v))
+#define CONJB_f64(v, op) (v)
+#define CONJB_f32(v, op) (v)
+#define CONJB_i64(v, op) (v)
+#define CONJB_i32(v, op) (v)
+#define CONJB_c128(v, op) ((op) == 2 ? conj(v) : (v))
+#define CONJB_c64(v, op) ((op) == 2 ? conjf(v) : (v))
 
 #define DEFINE_OPS(T, S)                                                            \
 DEFINE_BETA(T, S)                                                                   \
@@ -54,8 +86,9 @@ DEFINE_BETA(T, S)                                                               
  * C[row,k] += tmp*α.  Rows are independent, so OpenMP over rows keeps every       \
  * result bit-identical to the sequential loop. */                                 \
 void ora_spmm_csr_N_##S(I m, I k, const I* rowptr, const I* colval, const T* nzval, \
-                        const T* B, I ldb, int opB, T* C, I ldc, T alpha, T beta,   \
-                        int threads) {                                              \
+                        const T* B, I ldb, int opB, T* C, I ldc, const T* alpha_p,  \
+                        const T* beta_p, int threads) {                             \
+    const T alpha = *alpha_p, beta = *beta_p;                                       \
     beta_apply_##S(m, k, C, ldc, beta);                                             \
     (void)threads;                                                                  \
     _Pragma("omp parallel for schedule(dynamic, 1024) num_threads(threads)")        \
@@ -63,10 +96,10 @@ void ora_spmm_csr_N_##S(I m, I k, const I* rowptr, const I* colval, const T* nzv
         for (I kk = 0; kk < k; ++kk) {                                              \
             T tmp = (T)0;                                                           \
             for (I j = rowptr[row] - 1; j < rowptr[row + 1] - 1; ++j) {             \
-                T p = nzval[j] * BAT(B, colval[j] - 1, kk, ldb, opB);               \
+                T p = CONJA_##S(nzval[j]) * CONJB_##S(BAT(B, colval[j] - 1, kk, ldb, opB), opB); \
                 tmp = tmp + p;                                                      \
             }                                                                       \
-            T t = tmp * alpha;                                                      \
+            T t = SCAL_##S(tmp, alpha);                                             \
             C[row + kk * ldc] = C[row + kk * ldc] + t;                              \
         }                                                                           \
     }                                                                               \
@@ -77,13 +110,14 @@ void ora_spmm_csr_N_##S(I m, I k, const I* rowptr, const I* colval, const T* nzv
  * (colptr, rowval). */                                                             \
 void ora_spmm_csr_T_##S(I m, I n, I k, const I* rowptr, const I* colval,            \
                         const T* nzval, const T* B, I ldb, int opB, T* C, I ldc,    \
-                        T alpha, T beta) {                                          \
+                        const T* alpha_p, const T* beta_p) {                        \
+    const T alpha = *alpha_p, beta = *beta_p;                                       \
     beta_apply_##S(n, k, C, ldc, beta);                                             \
     for (I kk = 0; kk < k; ++kk)                                                    \
         for (I row = 0; row < m; ++row) {                                           \
-            T axj = BAT(B, row, kk, ldb, opB) * alpha;                              \
+            T axj = SCAL_##S(CONJB_##S(BAT(B, row, kk, ldb, opB), opB), alpha);     \
             for (I j = rowptr[row] - 1; j < rowptr[row + 1] - 1; ++j) {             \
-                T p = nzval[j] * axj;                                               \
+                T p = CONJA_##S(nzval[j]) * axj;                                    \
                 C[(colval[j] - 1) + kk * ldc] = C[(colval[j] - 1) + kk * ldc] + p;  \
             }                                                                       \
         }                                                                           \
@@ -92,14 +126,15 @@ void ora_spmm_csr_T_##S(I m, I n, I k, const I* rowptr, const I* colval,        
  * update per stored entry, C[row,k] += nzval*(B[col,k]*α) (T swaps row/col). */    \
 void ora_spmm_coo_##S(int trans, I rows_out, I k, I nnz, const I* rowind,           \
                       const I* colind, const T* nzval, const T* B, I ldb, int opB,  \
-                      T* C, I ldc, T alpha, T beta) {                               \
+                      T* C, I ldc, const T* alpha_p, const T* beta_p) {             \
+    const T alpha = *alpha_p, beta = *beta_p;                                       \
     beta_apply_##S(rows_out, k, C, ldc, beta);                                      \
     for (I kk = 0; kk < k; ++kk)                                                    \
         for (I i = 0; i < nnz; ++i) {                                               \
             I r = trans ? colind[i] - 1 : rowind[i] - 1;                            \
             I c = trans ? rowind[i] - 1 : colind[i] - 1;                            \
-            T axj = BAT(B, c, kk, ldb, opB) * alpha;                                \
-            T p = nzval[i] * axj;                                                   \
+            T axj = SCAL_##S(CONJB_##S(BAT(B, c, kk, ldb, opB), opB), alpha);       \
+            T p = CONJA_##S(nzval[i]) * axj;                                        \
             C[r + kk * ldc] = C[r + kk * ldc] + p;                                  \
         }                                                                           \
 }                                                                                   \
@@ -319,6 +354,10 @@ void ora_cumsum(I n, const I* x, I* out) {
 
 DEFINE_OPS(double, f64)
 DEFINE_OPS(float, f32)
+DEFINE_OPS(zc128, c128)
+DEFINE_OPS(zc64, c64)
+DEFINE_OPS(uint64_t, i64)
+DEFINE_OPS(uint32_t, i32)
 
 /* ---------------------------------------------------------------------------
  * Reductions over the stored entries, A + λI.  "ptr form": CSR as (rows,
@@ -398,12 +437,18 @@ int ora_symmetric_ptr_##S(I n, const I* ptr, const I* idx, const T* val) {      
             if (!found) return 0;                                                   \
         }                                                                           \
     return 1;                                                                       \
-}                                                                                   \
+}
+
+DEFINE_REDUCE(double, f64)
+DEFINE_REDUCE(float, f32)
+
+#define DEFINE_ADD_DIAG(T, S)                                                       \
 /* Base.:+(::CSR, ::UniformScaling) (src/matrix_csr/matrix_csr.jl:357-406):        \
  * kernel_count_diag_entries_csr! then kernel_add_uniformscaling_csr!             \
  * (matrix_csr_kernels.jl:445-509).  Returns nnz; outputs may be NULL to count. */  \
-I ora_add_diag_##S(I n, const I* ptr, const I* idx, const T* val, T lambda,         \
+I ora_add_diag_##S(I n, const I* ptr, const I* idx, const T* val, const T* lambda_p, \
                    I* ptr_out, I* idx_out, T* val_out) {                            \
+    const T lambda = *lambda_p;                                                     \
     I w = 0;                                                                        \
     if (ptr_out) ptr_out[0] = 1;                                                    \
     for (I r = 0; r < n; ++r) {                                                     \
@@ -430,8 +475,12 @@ I ora_add_diag_##S(I n, const I* ptr, const I* idx, const T* val, T lambda,     
     return w;                                                                       \
 }
 
-DEFINE_REDUCE(double, f64)
-DEFINE_REDUCE(float, f32)
+DEFINE_ADD_DIAG(double, f64)
+DEFINE_ADD_DIAG(float, f32)
+DEFINE_ADD_DIAG(zc128, c128)
+DEFINE_ADD_DIAG(zc64, c64)
+DEFINE_ADD_DIAG(uint64_t, i64)
+DEFINE_ADD_DIAG(uint32_t, i32)
 
 int ora_max_threads(void) {
 #ifdef _OPENMP

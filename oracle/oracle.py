@@ -40,12 +40,25 @@ def _p(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else ctypes.c_void_p(0)
 
 
+_SFX = {np.dtype(np.float64): "f64", np.dtype(np.float32): "f32",
+        np.dtype(np.complex128): "c128", np.dtype(np.complex64): "c64",
+        np.dtype(np.int64): "i64", np.dtype(np.int32): "i32"}
+
+
 def _sfx(dtype):
-    return "f64" if np.dtype(dtype) == np.float64 else "f32"
+    """Suffix of the oracle's instantiation for a value type (integers run as
+    unsigned wrapping arithmetic of their width: the same bits)."""
+    return _SFX[np.dtype(dtype)]
 
 
 def _ctv(dtype):
     return ctypes.c_double if np.dtype(dtype) == np.float64 else ctypes.c_float
+
+
+def _sref(value, dtype):
+    """A one-element numpy array holding a scalar of the value type, passed by
+    pointer (complex scalars cannot be passed by value through ctypes)."""
+    return np.array([value], dtype=dtype)
 
 
 def _i(a):
@@ -69,16 +82,22 @@ def spmm_csr_N(m, rowptr, colval, nzval, B, C=None, alpha=1.0, beta=0.0, threads
         np.asarray(C, dtype=T).reshape(m, k), order="F", copy=True)
     fn = getattr(lib(), "ora_spmm_csr_N_" + _sfx(T))
     fn.argtypes = [ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P, ctypes.c_int64, ctypes.c_int,
-                   _P, ctypes.c_int64, _ctv(T), _ctv(T), ctypes.c_int]
+                   _P, ctypes.c_int64, _P, _P, ctypes.c_int]
     rp, cv, nz = _i(rowptr), _i(colval), np.ascontiguousarray(nzval)
+    a_ref, b_ref = _sref(alpha, T), _sref(beta, T)  # kept alive across the call
     fn(m, k, _p(rp), _p(cv), _p(nz), _p(Bf), max(Bf.shape[0], 1), 0, _p(Cf), max(m, 1),
-       alpha, beta, int(threads))
+       _p(a_ref), _p(b_ref), int(threads))
     return Cf[:, 0].copy() if vec else Cf
 
 
-def spmm_csr_T(m, n, rowptr, colval, nzval, B, C=None, alpha=1.0, beta=0.0):
+def _conj_a(flag):
+    lib().ora_set_conj_a(int(bool(flag)))
+
+
+def spmm_csr_T(m, n, rowptr, colval, nzval, B, C=None, alpha=1.0, beta=0.0, conj_a=False):
     """mul!(C, transpose(A::CSR), B, α, β): scatter in storage order
-    (matrix_csr_kernels.jl:24-45); also CSC-N on (colptr, rowval)."""
+    (matrix_csr_kernels.jl:24-45); also CSC-N on (colptr, rowval).  conj_a:
+    adjoint(A), the stored values conjugated (CONJA)."""
     T = nzval.dtype
     B2 = np.asarray(B, dtype=T)
     vec = B2.ndim == 1
@@ -88,15 +107,22 @@ def spmm_csr_T(m, n, rowptr, colval, nzval, B, C=None, alpha=1.0, beta=0.0):
         np.asarray(C, dtype=T).reshape(n, k), order="F", copy=True)
     fn = getattr(lib(), "ora_spmm_csr_T_" + _sfx(T))
     fn.argtypes = [ctypes.c_int64] * 3 + [_P] * 4 + [ctypes.c_int64, ctypes.c_int, _P,
-                                                     ctypes.c_int64, _ctv(T), _ctv(T)]
+                                                     ctypes.c_int64, _P, _P]
     rp, cv, nz = _i(rowptr), _i(colval), np.ascontiguousarray(nzval)
-    fn(m, n, k, _p(rp), _p(cv), _p(nz), _p(Bf), max(Bf.shape[0], 1), 0, _p(Cf), max(n, 1),
-       alpha, beta)
+    a_ref, b_ref = _sref(alpha, T), _sref(beta, T)
+    _conj_a(conj_a)
+    try:
+        fn(m, n, k, _p(rp), _p(cv), _p(nz), _p(Bf), max(Bf.shape[0], 1), 0, _p(Cf), max(n, 1),
+           _p(a_ref), _p(b_ref))
+    finally:
+        _conj_a(False)
     return Cf[:, 0].copy() if vec else Cf
 
 
-def spmm_coo(trans, rows_out, rowind, colind, nzval, B, C=None, alpha=1.0, beta=0.0):
-    """mul! for COO (matrix_coo_kernels.jl:1-43), entries in storage order."""
+def spmm_coo(trans, rows_out, rowind, colind, nzval, B, C=None, alpha=1.0, beta=0.0,
+             conj_a=False):
+    """mul! for COO (matrix_coo_kernels.jl:1-43), entries in storage order;
+    conj_a: adjoint(A) (CONJA)."""
     T = nzval.dtype
     B2 = np.asarray(B, dtype=T)
     vec = B2.ndim == 1
@@ -106,10 +132,15 @@ def spmm_coo(trans, rows_out, rowind, colind, nzval, B, C=None, alpha=1.0, beta=
         np.asarray(C, dtype=T).reshape(rows_out, k), order="F", copy=True)
     fn = getattr(lib(), "ora_spmm_coo_" + _sfx(T))
     fn.argtypes = [ctypes.c_int, ctypes.c_int64, ctypes.c_int64, ctypes.c_int64, _P, _P, _P, _P,
-                   ctypes.c_int64, ctypes.c_int, _P, ctypes.c_int64, _ctv(T), _ctv(T)]
+                   ctypes.c_int64, ctypes.c_int, _P, ctypes.c_int64, _P, _P]
     r, c, v = _i(rowind), _i(colind), np.ascontiguousarray(nzval)
-    fn(int(trans), rows_out, k, v.size, _p(r), _p(c), _p(v), _p(Bf), max(Bf.shape[0], 1), 0,
-       _p(Cf), max(rows_out, 1), alpha, beta)
+    a_ref, b_ref = _sref(alpha, T), _sref(beta, T)
+    _conj_a(conj_a)
+    try:
+        fn(int(trans), rows_out, k, v.size, _p(r), _p(c), _p(v), _p(Bf), max(Bf.shape[0], 1), 0,
+           _p(Cf), max(rows_out, 1), _p(a_ref), _p(b_ref))
+    finally:
+        _conj_a(False)
     return Cf[:, 0].copy() if vec else Cf
 
 
@@ -309,11 +340,12 @@ def add_diag(n, ptr, idx, val, lam):
     p, i, v = _i(ptr), _i(idx), np.ascontiguousarray(val)
     T = v.dtype
     fn = getattr(lib(), "ora_add_diag_" + _sfx(T))
-    fn.argtypes = [ctypes.c_int64, _P, _P, _P, _ctv(T), _P, _P, _P]
+    fn.argtypes = [ctypes.c_int64, _P, _P, _P, _P, _P, _P, _P]
     fn.restype = ctypes.c_int64
-    cnt = fn(n, _p(p), _p(i), _p(v), lam, _p(None), _p(None), _p(None))
+    lam_a = _sref(lam, T)
+    cnt = fn(n, _p(p), _p(i), _p(v), _p(lam_a), _p(None), _p(None), _p(None))
     po = np.empty(n + 1, _I); io = np.empty(cnt, _I); vo = np.empty(cnt, T)
-    fn(n, _p(p), _p(i), _p(v), lam, _p(po), _p(io), _p(vo))
+    fn(n, _p(p), _p(i), _p(v), _p(lam_a), _p(po), _p(io), _p(vo))
     return po, io, vo
 
 

@@ -6,6 +6,8 @@ reference's order must match bit for bit.  Float32: 2e-6 normwise (row sums
 here stay under 1000 entries, so any summation order stays inside
 n*eps32*Σ|a||x| ~ 1e-4 and in practice ~1e-6).
 """
+import ctypes
+
 import numpy as np
 import pytest
 import torch
@@ -250,7 +252,7 @@ def _spmv(dev, A, x, planned):
         y = torch.empty(A.m, dtype=torch.float64, device=dev)
         L.call("gsa_spmm_csr", st, L.GSA_F64, it, L.GSA_OP_N, A.m, A.n, 1, p(A.rowptr),
                p(A.colval), p(A.nzval), A.nnz(), p(x), max(A.n, 1), L.GSA_OP_N, p(y), max(A.m, 1),
-               1.0, 0.0, h if planned else None)
+               None, None, h if planned else None)
         return host(y)
     finally:
         if planned:
@@ -526,7 +528,7 @@ def test_row_bins_float32_and_strided_x(dev):
     p = lambda t: ctypes.c_void_p(t.data_ptr())
     L.call("gsa_spmm_csc", stream_of(dev), L.GSA_F32, L.GSA_I64, L.GSA_OP_N, H.m, H.n, 1,
            p(Ac.colptr), p(Ac.rowval), p(Ac.nzval), Ac.nnz(), ctypes.c_void_p(X3.data_ptr() + 4),
-           3, L.GSA_OP_T, p(y), H.m, 1.0, 0.0, Ac.plan())
+           3, L.GSA_OP_T, p(y), H.m, None, None, Ac.plan())
     assert_normwise(host(y), ref, bound, 2e-6)
 
 
@@ -583,14 +585,16 @@ def test_transposed_bins(dev, itype, alpha, beta):
     C = torch.as_tensor(c0.copy(), device=dev)
     L.call("gsa_spmm_csr", stream_of(dev), L.GSA_F64, G.matrix.itype_of(A.rowptr), L.GSA_OP_T,
            H.m, H.n, 1, G.matrix.ptr(A.rowptr), G.matrix.ptr(A.colval), G.matrix.ptr(A.nzval),
-           H.nnz, G.matrix.ptr(xd), 1, L.GSA_OP_N, G.matrix.ptr(C), H.n, alpha, beta, None)
+           H.nnz, G.matrix.ptr(xd), 1, L.GSA_OP_N, G.matrix.ptr(C), H.n,
+           ctypes.byref(ctypes.c_double(alpha)), ctypes.byref(ctypes.c_double(beta)), None)
     assert_normwise(host(C), ref, bound, 1e-12)
     xs = torch.zeros((H.m, 2), dtype=torch.float64, device=dev)
     xs[:, 0] = xd
     C = torch.as_tensor(c0.copy(), device=dev)
     L.call("gsa_spmm_csr", stream_of(dev), L.GSA_F64, G.matrix.itype_of(A.rowptr), L.GSA_OP_T,
            H.m, H.n, 1, G.matrix.ptr(A.rowptr), G.matrix.ptr(A.colval), G.matrix.ptr(A.nzval),
-           H.nnz, G.matrix.ptr(xs), 2, L.GSA_OP_T, G.matrix.ptr(C), H.n, alpha, beta,
+           H.nnz, G.matrix.ptr(xs), 2, L.GSA_OP_T, G.matrix.ptr(C), H.n,
+           ctypes.byref(ctypes.c_double(alpha)), ctypes.byref(ctypes.c_double(beta)),
            A.plan_transposed())
     assert_normwise(host(C), ref, bound, 1e-12)
 
@@ -706,7 +710,7 @@ def test_gather_views_for_scatter_products(dev, k, itype):
     L.call("gsa_spmm_csc", G.matrix.stream_of(dev), L.GSA_F64, G.matrix.itype_of(Ac.colptr),
            L.GSA_OP_N, H.m, H.n, k, G.matrix.ptr(Ac.colptr), G.matrix.ptr(Ac.rowval),
            G.matrix.ptr(Ac.nzval), H.nnz, G.matrix.ptr(Bd), max(Bd.stride(1), H.n), L.GSA_OP_N,
-           G.matrix.ptr(C2), H.m, 1.0, 0.0, None)
+           G.matrix.ptr(C2), H.m, None, None, None)
     assert_normwise(host(C1), host(C2), norm_bound(H.rowptr, H.colval, H.nzval, np.abs(Bn)), 1e-12)
 
 
@@ -774,7 +778,7 @@ def test_stream_destroy_releases_cached_scratch(dev):
     torch.cuda.synchronize()
     p = G.matrix.ptr
     L.call("gsa_spmm_csr", s, L.GSA_F64, L.GSA_I64, L.GSA_OP_N, H.m, H.n, 1, p(A.rowptr),
-           p(A.colval), p(A.nzval), H.nnz, p(x), H.n, L.GSA_OP_N, p(y), H.m, 1.0, 0.0, None)
+           p(A.colval), p(A.nzval), H.nnz, p(x), H.n, L.GSA_OP_N, p(y), H.m, None, None, None)
     L.call("gsa_stream_sync", s)
     assert lib.gsa_scratch_cached(s) > 0  # the window table and partial sums
     np.testing.assert_allclose(host(y), H.to_scipy() @ np.ones(H.n), rtol=1e-12, atol=1e-12)

@@ -381,3 +381,132 @@ def test_kron_diagonal_golden():
     r, c, v = O.kron_coo_diag(A, d["d5"])
     assert np.array_equal(r, d["KA_0"]) and np.array_equal(c, d["KA_1"])
     assert np.array_equal(v, d["KA_2"])
+
+
+# ------------------------------------------- complex and integer value types --
+# The reference runs every family on Int32, Int64, ComplexF32 and ComplexF64
+# values too (test/runtests.jl:34-68).  The oracle's instantiations for them are
+# pinned here against exact dense arithmetic: integer-valued complex entries
+# make every product and sum exact in any order, and integer values wrap modulo
+# 2^64 / 2^32 as numpy's int64 / int32 arithmetic does.
+VALUE_KINDS = [np.complex128, np.complex64, np.int64, np.int32]
+
+
+def typed_csr(m, n, density, seed, dtype, big=False):
+    """A random CSR matrix with integer-valued entries of `dtype` (complex: both
+    parts; `big`: integers large enough that products wrap)."""
+    rng = np.random.default_rng(seed)
+    A = sp.random(m, n, density=density, random_state=rng, format="csr")
+    k = A.data.size
+    if np.issubdtype(dtype, np.complexfloating):
+        vals = rng.integers(-4, 5, k) + 1j * rng.integers(-4, 5, k)
+        vals[vals == 0] = 2 - 1j
+    else:
+        info = np.iinfo(dtype)
+        vals = (rng.integers(info.min // 4, info.max // 4, k) if big
+                else rng.integers(-4, 5, k))
+        vals[vals == 0] = 3
+    A.data = vals.astype(dtype)
+    A.sort_indices()
+    return A
+
+
+def typed_dense(shape, seed, dtype, big=False):
+    rng = np.random.default_rng(seed)
+    if np.issubdtype(dtype, np.complexfloating):
+        return (rng.integers(-4, 5, shape) + 1j * rng.integers(-4, 5, shape)).astype(dtype)
+    info = np.iinfo(dtype)
+    if big:
+        return rng.integers(info.min // 4, info.max // 4, shape).astype(dtype)
+    return rng.integers(-4, 5, shape).astype(dtype)
+
+
+def dense_matmul(A, B):
+    """A @ B on dense arrays in A's type: exact for integer-valued complex data,
+    wrapping for integers (numpy's integer arithmetic wraps)."""
+    with np.errstate(over="ignore"):
+        return np.asarray(A) @ np.asarray(B)
+
+
+@pytest.mark.parametrize("dtype", VALUE_KINDS)
+def test_products_all_value_types_exact(dtype):
+    big = np.issubdtype(dtype, np.integer)
+    A = typed_csr(60, 45, 0.15, 601, dtype, big=big)
+    Ad = A.toarray()
+    rp, cv, nz = arrs(A)
+    x = typed_dense(45, 602, dtype, big=big)
+    X = typed_dense((45, 3), 603, dtype, big=big)
+    assert np.array_equal(O.spmm_csr_N(60, rp, cv, nz, x), dense_matmul(Ad, x))
+    assert np.array_equal(O.spmm_csr_N(60, rp, cv, nz, X), dense_matmul(Ad, X))
+    xt = typed_dense(60, 604, dtype, big=big)
+    assert np.array_equal(O.spmm_csr_T(60, 45, rp, cv, nz, xt), dense_matmul(Ad.T, xt))
+    if np.issubdtype(dtype, np.complexfloating):
+        # adjoint(A) * x conjugates the stored values (CONJA)
+        assert np.array_equal(O.spmm_csr_T(60, 45, rp, cv, nz, xt, conj_a=True),
+                              dense_matmul(Ad.conj().T, xt))
+        r, c = np.nonzero(Ad)
+        got = O.spmm_coo(True, 45, r + 1, c + 1, Ad[r, c], xt, conj_a=True)
+        assert np.array_equal(got, dense_matmul(Ad.conj().T, xt))
+    # alpha, beta: C = beta*C0 + alpha*A*x
+    c0 = typed_dense(60, 605, dtype)
+    if np.issubdtype(dtype, np.complexfloating):
+        alpha, beta = dtype(2 - 1j), dtype(3 + 0j)
+    else:
+        alpha, beta = dtype(3), dtype(-2)
+    got = O.spmm_csr_N(60, rp, cv, nz, x, C=c0, alpha=alpha, beta=beta)
+    with np.errstate(over="ignore"):
+        want = (beta * c0 + dense_matmul(Ad, x) * alpha).astype(dtype)
+    assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("dtype", VALUE_KINDS)
+def test_structural_all_value_types_vs_scipy(dtype):
+    A = typed_csr(40, 30, 0.2, 611, dtype)
+    B = typed_csr(40, 30, 0.2, 612, dtype)
+    # + with dropped exact zeros: integer values make A + (-A) cancel exactly
+    ra, ca, va = arrs(A)
+    rb, cb, vb = arrs(B)
+    nnz = O.csr_add(40, (ra, ca, va), (rb, cb, vb))
+    S = (A + B).tocsr()
+    S.eliminate_zeros()
+    S.sort_indices()
+    want = arrs(S)
+    for got, w in zip(nnz, want):
+        assert np.array_equal(got, w)
+    neg = O.csr_add(40, (ra, ca, va), (ra, ca, (-va).astype(dtype)))
+    assert neg[1].size == 0
+    # kron: values bit-exact (one multiply each), pattern of scipy's
+    K = O.kron_csr(40, 30, 40, 30, (ra, ca, va), (rb, cb, vb))
+    Ks = sp.kron(A, B, format="csr")
+    Ks.sort_indices()
+    assert np.array_equal(K[0], Ks.indptr + 1) and np.array_equal(K[1], Ks.indices + 1)
+    assert np.array_equal(K[2], Ks.data.astype(dtype))
+    # SpGEMM: values equal the dense product, pattern the structural product
+    # (cancellation zeros are kept, as the reference keeps them)
+    Bt = typed_csr(30, 25, 0.2, 613, dtype)
+    rt, ct, vt = arrs(Bt)
+    rp, cv, nz = O.spgemm_csr(40, 25, (ra, ca, va), (rt, ct, vt))
+    C = sp.csr_matrix((nz, cv - 1, rp - 1), shape=(40, 25))
+    assert np.array_equal(C.toarray(), dense_matmul(A.toarray(), Bt.toarray()))
+    pattern = ((np.abs(A.toarray()) > 0).astype(np.int64) @
+               (np.abs(Bt.toarray()) > 0).astype(np.int64)) > 0
+    got_pat = np.zeros((40, 25), dtype=bool)
+    got_pat[np.repeat(np.arange(40), np.diff(rp)), cv - 1] = True
+    assert np.array_equal(got_pat, pattern) and C.nnz == pattern.sum()
+    # conversions move values unchanged
+    r, c, v = O.csr_to_coo(40, ra, ca, va)
+    assert v.dtype == np.dtype(dtype) and np.array_equal(v, va)
+    cp, rv, vv = O.csr_to_csc(40, 30, ra, ca, va)
+    Ac = A.tocsc()
+    Ac.sort_indices()
+    assert np.array_equal(vv, Ac.data) and np.array_equal(rv, Ac.indices + 1)
+
+
+def test_complex_dropzeros_and_zero_kinds():
+    """iszero: both parts zero, -0.0 included; (0, 1) is kept."""
+    v = np.array([0 + 0j, -0.0 + 0j, 0 + 1j, 1 + 0j, complex(0.0, -0.0)], dtype=np.complex128)
+    rp = np.array([1, 4, 6])
+    cv = np.array([1, 2, 3, 1, 2])
+    p, c, vv = O.csr_dropzeros(2, rp, cv, v)
+    assert np.array_equal(p, [1, 2, 3]) and np.array_equal(c, [3, 1])
+    assert np.array_equal(vv, [1j, 1 + 0j])

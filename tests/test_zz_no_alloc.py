@@ -39,9 +39,11 @@ def _both(H, x, y0, dtype, itype, dev, alpha, beta):
     y_new = y_ref.clone()
     s = ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
     lib = L.load()
+    ct = ctypes.c_double if dtype == torch.float64 else ctypes.c_float
+    a_ref, b_ref = ct(alpha), ct(beta)  # host scalars of the value type
     L.check(lib.gsa_spmm_csr(s, _VT[dtype], _IT[itype], L.GSA_OP_N, H.m, H.n, 1, _p(rp), _p(cv),
-                             _p(nz), H.nnz, _p(xd), H.n, L.GSA_OP_N, _p(y_ref), H.m, alpha,
-                             beta, None), "gsa_spmm_csr")
+                             _p(nz), H.nnz, _p(xd), H.n, L.GSA_OP_N, _p(y_ref), H.m,
+                             ctypes.byref(a_ref), ctypes.byref(b_ref), None), "gsa_spmm_csr")
     L.check(lib.gsa_spmv_csr_noalloc(s, _VT[dtype], _IT[itype], H.m, H.n, _p(rp), _p(cv),
                                      _p(nz), H.nnz, _p(xd), _p(y_new), alpha, beta),
             "gsa_spmv_csr_noalloc")
