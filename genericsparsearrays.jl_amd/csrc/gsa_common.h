@@ -392,6 +392,9 @@ struct CsrPlanData {
     int64_t* order = nullptr;
     int64_t nslot = 0;
     const void* order_col = nullptr;  // the column array the order was built from
+    // With Int64 indices and n < 2^32: a 32-bit copy of those columns (built
+    // with the order; the SpMV streams 4 bytes less per stored entry)
+    uint32_t* col32 = nullptr;
     const void* ptr = nullptr;    // the row pointer the plan was built on
 };
 

@@ -263,10 +263,10 @@ __device__ __forceinline__ bool window_tail(const I* __restrict__ ptr, int64_t r
     return end > e1 && end - a > W_LONG;
 }
 
-template <typename T, typename I, typename V = T, typename PI = void>
+template <typename T, typename I, typename V = T, typename PI = void, typename CI = I>
 __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
     int64_t m, int64_t n, int64_t nnz, const I* __restrict__ rowptr,
-    const I* __restrict__ colval, const V* __restrict__ nzval, const PI* __restrict__ perm,
+    const CI* __restrict__ colval, const V* __restrict__ nzval, const PI* __restrict__ perm,
     const float* __restrict__ rscale, const T* __restrict__ x, T* __restrict__ y, T alpha,
     T beta, int mode, int cj, const int64_t* __restrict__ win_row, int64_t nwin,
     const int64_t* __restrict__ order, T* __restrict__ head, T* __restrict__ tail) {
@@ -307,7 +307,7 @@ __global__ __launch_bounds__(64 * WAVES) void k_csr_spmv(
         rpv[k] = ldnt(rowptr + r0 + i);
     }
     if (staged > 0) {
-        I ci[WU + 1];
+        CI ci[WU + 1];
         T vv[WU + 1], xv[WU + 1];
 #pragma unroll
         for (int u = 0; u <= WU; ++u) {
@@ -828,6 +828,7 @@ struct PlanUse {
     const int64_t* order = nullptr;  // window launch order, when built for these columns
     int64_t nslot = 0;
     const void* order_col = nullptr;
+    const uint32_t* col32 = nullptr;  // 32-bit copy of those columns (Int64 indices)
     CsrPlanData* owner = nullptr;
     Scratch tmp, tmp_part;
     explicit PlanUse(hipStream_t s) : tmp(s), tmp_part(s) {}
@@ -845,6 +846,7 @@ struct PlanUse {
             order = owner->order;
             nslot = owner->nslot;
             order_col = owner->order_col;
+            col32 = owner->col32;
             return GSA_OK;
         }
         if (wave) {
@@ -912,9 +914,17 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         const int64_t* order = (pu.order && pu.order_col == (const void*)colval) ? pu.order : nullptr;
         const int64_t grid1 = order ? pu.nslot / WAVES : cdiv(pu.nwin, WAVES);
         GSA_CHECK(grid1 <= (int64_t)UINT32_MAX, GSA_ERR_INVALID, "SpMV grid too large");
-        hipLaunchKernelGGL((k_csr_spmv<T, I, V, PI>), dim3((unsigned)grid1), dim3(64 * WAVES), 0, s,
-                           m, n, nnz, rowptr, colval, nzval, perm, rscale, x, C, alpha,
-                           beta, mode, cj, pu.win_row, pu.nwin, order, head, tail);
+        // the plan's 32-bit copy of these Int64 columns: 4 bytes less per entry
+        const uint32_t* c32 = (order && sizeof(I) == 8) ? pu.col32 : nullptr;
+        if (c32)
+            hipLaunchKernelGGL((k_csr_spmv<T, I, V, PI, uint32_t>), dim3((unsigned)grid1),
+                               dim3(64 * WAVES), 0, s, m, n, nnz, rowptr, c32, nzval, perm,
+                               rscale, x, C, alpha, beta, mode, cj, pu.win_row, pu.nwin, order,
+                               head, tail);
+        else
+            hipLaunchKernelGGL((k_csr_spmv<T, I, V, PI>), dim3((unsigned)grid1), dim3(64 * WAVES),
+                               0, s, m, n, nnz, rowptr, colval, nzval, perm, rscale, x, C, alpha,
+                               beta, mode, cj, pu.win_row, pu.nwin, order, head, tail);
         hipLaunchKernelGGL((k_spmv_fold<T, I>), dim3((unsigned)cdiv(pu.nwin, 256)), dim3(256), 0, s,
                            m, nnz, rowptr, pu.win_row, pu.nwin, (const T*)head, (const T*)tail,
                            rscale, C, alpha, beta, mode);
@@ -1105,9 +1115,10 @@ static int spmv_noalloc(hipStream_t s, int itype, int64_t m, int64_t n, const I*
 
 static int plan_release(CsrPlanData& d) {
     hipError_t e = hipSuccess, f;
-    for (void* q : {(void*)d.blk_row, (void*)d.order})
+    for (void* q : {(void*)d.blk_row, (void*)d.order, (void*)d.col32})
         if (q && (f = hipFree(q)) != hipSuccess && e == hipSuccess) e = f;
     d.blk_row = d.blk_pos = d.win_row = nullptr;
+    d.col32 = nullptr;
     d.order = nullptr;
     d.nslot = 0;
     d.order_col = nullptr;
@@ -1191,11 +1202,20 @@ static std::vector<int64_t> window_slots(const std::vector<int>& bucket) {
 
 // The XCD launch order of a plan's SpMV windows for column array `colval`
 // (n columns).  Synchronises once.
+__global__ void k_cols_to_u32(int64_t nnz, const int64_t* __restrict__ col,
+                              uint32_t* __restrict__ out) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < nnz;
+         i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = (uint32_t)col[i];
+}
+
 static int build_order(hipStream_t s, CsrPlanData& d, const void* colval, int64_t n) {
-    if (d.order) {
+    if (d.order || d.col32) {
         GSA_HIP(hipStreamSynchronize(s));
-        GSA_HIP(hipFree(d.order));
+        if (d.order) GSA_HIP(hipFree(d.order));
+        if (d.col32) GSA_HIP(hipFree(d.col32));
         d.order = nullptr;
+        d.col32 = nullptr;
         d.nslot = 0;
         d.order_col = nullptr;
     }
@@ -1214,6 +1234,13 @@ static int build_order(hipStream_t s, CsrPlanData& d, const void* colval, int64_
     GSA_HIP(hipMemcpyAsync(hb.data(), bucket.p, sizeof(int) * d.nwin, hipMemcpyDeviceToHost, s));
     GSA_HIP(hipStreamSynchronize(s));
     const std::vector<int64_t> order = window_slots(hb);
+    if (d.itype == GSA_I64 && n <= (int64_t)UINT32_MAX) {
+        // structure only, like the order: a 32-bit copy of the one-based columns
+        GSA_HIP(hipMalloc(&d.col32, sizeof(uint32_t) * d.nnz));
+        hipLaunchKernelGGL(k_cols_to_u32, dim3(grid_for(d.nnz, 256)), dim3(256), 0, s, d.nnz,
+                           (const int64_t*)colval, d.col32);
+        GSA_HIP(hipGetLastError());
+    }
     GSA_HIP(hipMalloc(&d.order, sizeof(int64_t) * order.size()));
     GSA_HIP(hipMemcpyAsync(d.order, order.data(), sizeof(int64_t) * order.size(),
                            hipMemcpyHostToDevice, s));

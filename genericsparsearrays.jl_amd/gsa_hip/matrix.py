@@ -226,8 +226,9 @@ class _PlanCache:
     """Per-matrix cached analysis (gsa_plan): CSR row blocks and SpMV windows;
     for CSC also the row view, for COO the row view alone (the sorted-key form
     of mul!, without atomics).  It is rebuilt when an array it was built from
-    changes.  A plan also holds the partial sums of the SpMV in flight, so
-    products on different streams at the same time need different plans."""
+    changes (keyed on each array's address, type and torch version counter).
+    A plan is read-only once built, so products on several streams at the same
+    time may share it."""
 
     def __init__(self):
         self.handle = None

@@ -145,11 +145,14 @@ int gsa_plan_add_transpose(void* stream, gsa_plan plan, int format, int itype, i
 int gsa_plan_add_gather_view(void* stream, gsa_plan plan, int format, int trans, int itype,
                              int64_t m, int64_t n, int64_t nnz, const void* a, const void* b);
 /* Give a plan the column array it will be used with (`colval`, n columns):
- * the SpMV windows lying inside rows longer than 2304 entries (whose
+ * the SpMV windows lying inside rows longer than 576 entries (whose
  * ascending columns span a narrow band of x) are then launched on the XCD
- * that gathers that eighth of x, so their gathers hit its L2.  Used while the
- * call's column array is this one; results never depend on it.  Synchronises
- * `stream`.  No counterpart in the reference. */
+ * that gathers that eighth of x, so their gathers hit its L2; with Int64
+ * indices and n < 2^32 the plan also keeps a 32-bit copy of the columns
+ * (4 bytes per stored entry), which the SpMV then streams instead.  Used
+ * while the call's column array is this one (an in-place write to it needs a
+ * new plan); results never depend on it.  Synchronises `stream`.  No
+ * counterpart in the reference. */
 int gsa_plan_set_columns(void* stream, gsa_plan plan, const void* colval, int64_t n);
 
 /* C = beta*C + alpha*opA(A)*opB(B) with A in CSR (m x n, `nnz` stored entries).
