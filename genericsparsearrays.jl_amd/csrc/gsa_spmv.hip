@@ -464,10 +464,11 @@ constexpr int SPMM_MAX_LONG = SPMV_CAP / (SPMM_LONG + 1) + 1;
 constexpr int SPMM_RP = 1024;  // row offsets staged in LDS per workgroup
 
 // Dense operand read as BT[col*ldbt + kk] (row-major).  LPR lanes per row.
-template <typename T, typename I, typename CI, int LPR, typename V = T, typename PI = void>
+template <typename T, typename I, typename CI, int LPR, typename V = T, typename PI = void,
+          typename SC = I>
 __global__ __launch_bounds__(WG) void k_csr_spmm(
     int64_t m, int64_t n, int64_t k, int64_t nnz, const I* __restrict__ rowptr,
-    const I* __restrict__ colval, const V* __restrict__ nzval, const PI* __restrict__ perm,
+    const SC* __restrict__ colval, const V* __restrict__ nzval, const PI* __restrict__ perm,
     const float* __restrict__ rscale,
     const T* __restrict__ BT,
     int64_t ldbt, T* __restrict__ C, int64_t ldc, int64_t ldk, T alpha, T beta, int mode,
@@ -500,7 +501,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
     }
     constexpr int U = 8;
     for (int base = 0; base < cnt; base += WG * U) {
-        I ci[U];
+        SC ci[U];
         T vv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -539,32 +540,38 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
     };
     // rows up to SPMM_LONG entries: one group of LPR lanes per row, each lane
     // one (row, kk) output summed left to right with 8 gathers in flight
+    auto short_row = [&](int s, int e, int64_t kk) -> T {
+        T acc = T(0);
+        for (int j = s; j < e; j += 8) {
+            T av[8], bv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {  // lanes past the row's end issue no load
+                const int jj = j + u < e ? j + u : e - 1;
+                av[u] = s_val[jj];
+                bv[u] = j + u < e ? conj_if(BT[(int64_t)s_col[jj] * ldbt + kk], cjb) : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                if (j + u < e) acc = acc + av[u] * bv[u];
+        }
+        return acc;
+    };
+    auto defer_long = [&](int64_t r) {
+        if (lane == 0) {
+            int q = atomicAdd(&nlong, 1);
+            if (q < SPMM_MAX_LONG) long_rows[q] = r;
+        }
+    };
     for (int64_t r = r0 + g; r < rs_end; r += GROUPS) {
         int s, e;
         row_range(r, s, e);
         if (e - s > SPMM_LONG) {
-            if (lane == 0) {
-                int q = atomicAdd(&nlong, 1);
-                if (q < SPMM_MAX_LONG) long_rows[q] = r;
-            }
+            defer_long(r);
             continue;
         }
-        for (int64_t kk = lane; kk < k; kk += LPR) {
-            T acc = T(0);
-            for (int j = s; j < e; j += 8) {
-                T av[8], bv[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {  // lanes past the row's end issue no load
-                    const int jj = j + u < e ? j + u : e - 1;
-                    av[u] = s_val[jj];
-                    bv[u] = j + u < e ? conj_if(BT[(int64_t)s_col[jj] * ldbt + kk], cjb) : T(0);
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j + u < e) acc = acc + av[u] * bv[u];
-            }
-            store_out(C, r * ldc + kk * ldk, row_scaled(acc, rscale, r), alpha, beta, mode);
-        }
+        for (int64_t kk = lane; kk < k; kk += LPR)
+            store_out(C, r * ldc + kk * ldk, row_scaled(short_row(s, e, kk), rscale, r), alpha,
+                      beta, mode);
     }
     __syncthreads();
     // longer staged rows: every group takes a contiguous share, LDS combine
@@ -666,23 +673,43 @@ __global__ void k_beta(int64_t rows, int64_t k, T* __restrict__ C, int64_t ldc, 
     }
 }
 
-// B (rows x cols, column-major, ld) -> BT row-major (ld = cols).  64-row tiles.
+// B (rows x cols, column-major, ld) -> BT row-major (ld = cols), in tiles of
+// 64 rows x 32 columns: each wavefront issues its eight column reads (64
+// consecutive rows each) before the first LDS write, then the tile leaves as
+// contiguous row-major runs.
 template <typename T>
 __global__ __launch_bounds__(256) void k_to_rowmajor(int64_t rows, int64_t cols,
                                                      const T* __restrict__ B, int64_t ldb,
                                                      T* __restrict__ BT) {
-    __shared__ T tile[64][65];
-    const int64_t r0 = blockIdx.x * 64;
-    for (int64_t c0 = 0; c0 < cols; c0 += 64) {
-        const int cw = (int)((cols - c0) < 64 ? (cols - c0) : 64);
-        for (int t = threadIdx.x; t < 64 * cw; t += 256) {
-            int rr = t & 63, cc = t >> 6;
-            if (r0 + rr < rows) tile[rr][cc] = B[(r0 + rr) + (c0 + cc) * ldb];
+    constexpr int RB = 64, CB = 32, Q = CB / 4;
+    __shared__ T tile[CB][RB + 1];  // [column][row]
+    const int64_t r0 = blockIdx.x * (int64_t)RB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rn = (int)(rows - r0 < RB ? rows - r0 : RB);
+    for (int64_t c0 = 0; c0 < cols; c0 += CB) {
+        const int cw = (int)(cols - c0 < CB ? cols - c0 : CB);
+        T v[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int cc = wv + 4 * q;
+            v[q] = (cc < cw && lane < rn) ? B[(r0 + lane) + (c0 + cc) * ldb] : T(0);
         }
+#pragma unroll
+        for (int q = 0; q < Q; ++q) tile[wv + 4 * q][lane] = v[q];
         __syncthreads();
-        for (int t = threadIdx.x; t < 64 * cw; t += 256) {
-            int rr = t / cw, cc = t % cw;
-            if (r0 + rr < rows) BT[(r0 + rr) * cols + c0 + cc] = tile[rr][cc];
+        if (cw == CB && cols == CB) {  // the tile is one contiguous run of rn rows
+            T* out = BT + r0 * cols;
+#pragma unroll
+            for (int q = 0; q < RB * CB / 256; ++q) {
+                const int e = threadIdx.x + 256 * q;
+                const int rr = e / CB, cc = e % CB;
+                if (rr < rn) out[e] = tile[cc][rr];
+            }
+        } else {
+            for (int e = threadIdx.x; e < rn * cw; e += 256) {
+                const int rr = e / cw, cc = e % cw;
+                BT[(r0 + rr) * cols + c0 + cc] = tile[cc][rr];
+            }
         }
         __syncthreads();
     }
