@@ -459,9 +459,9 @@ __global__ void k_win_bucket(int64_t m, int64_t n, int64_t nnz, const I* __restr
 }
 
 // ------------------------------------------------------------ SpMM kernel --
-constexpr int SPMM_LONG = 64;  // staged rows longer than this are split over the workgroup
-constexpr int SPMM_MAX_LONG = SPMV_CAP / (SPMM_LONG + 1) + 1;
+constexpr int SPMM_LONG = 64;  // staged rows up to this length are summed by one group
 constexpr int SPMM_RP = 1024;  // row offsets staged in LDS per workgroup
+constexpr int64_t SPMM_VIA_ROWS_MIN = 4096;  // rows from which column-major C goes via scratch
 
 // Dense operand read as BT[col*ldbt + kk] (row-major).  LPR lanes per row.
 template <typename T, typename I, typename CI, int LPR, typename V = T, typename PI = void,
@@ -474,13 +474,18 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
     int64_t ldbt, T* __restrict__ C, int64_t ldc, int64_t ldk, T alpha, T beta, int mode,
     int cj, const int64_t* __restrict__ blk_row, const int64_t* __restrict__ blk_pos) {
     const bool cjb = (cj & 2) != 0;
+    // mode < 0: the value itself (row-major scratch that k_from_rowmajor finishes)
+    auto out = [&](int64_t idx, T v) {
+        if (mode < 0) C[idx] = v;
+        else store_out(C, idx, v, alpha, beta, mode);
+    };
     // output (r, kk) at C[r * ldc + kk * ldk]: column-major (1, ld) or row-major (ld, 1)
     __shared__ CI s_col[SPMV_CAP];
     __shared__ T s_val[SPMV_CAP];
     __shared__ T red[WG];
     __shared__ int s_off[SPMM_RP];
-    __shared__ int64_t long_rows[SPMM_MAX_LONG];
-    __shared__ int nlong;
+    // per group: the row whose head part it left in red, -1 for none, -2 for an empty share
+    __shared__ int64_t s_hrow[WG / LPR];
 
     const int tid = threadIdx.x;
     const Window w = load_window(blockIdx.x, m, nnz, blk_row, blk_pos);
@@ -524,95 +529,129 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
         int i = tid + q * WG;
         if (i <= nst) s_off[i] = (int)(clamp64((int64_t)rpv[q] - 1, P0, Ps) - P0);
     }
-    if (tid == 0) nlong = 0;
     __syncthreads();
 
     constexpr int GROUPS = WG / LPR;
     const int g = tid / LPR, lane = tid % LPR;
-    auto row_range = [&](int64_t r, int& s, int& e) {
-        if (rp_in_lds) {
-            s = s_off[r - r0];
-            e = s_off[r - r0 + 1];
-        } else {
-            s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
-            e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
-        }
+    // staged rows 0..nrows-1 (relative to r0) cover entries [off(0), off(nrows)) of the
+    // LDS stage; each group of LPR lanes takes a contiguous share of those entries
+    const int64_t nrows = rs_end - r0;
+    auto off = [&](int64_t i) -> int {
+        return rp_in_lds ? s_off[i]
+                         : (int)(clamp64((int64_t)rowptr[r0 + i] - 1, P0, Ps) - P0);
     };
-    // rows up to SPMM_LONG entries: one group of LPR lanes per row, each lane
-    // one (row, kk) output summed left to right with 8 gathers in flight
-    auto short_row = [&](int s, int e, int64_t kk) -> T {
+    // the row holding entry p (off(i) <= p < off(i + 1)), for begin <= p < end
+    auto row_of = [&](int p) -> int64_t {
+        int64_t lo = 0, hi = nrows - 1;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi + 1) >> 1;
+            if (off(mid) <= p) lo = mid;
+            else hi = mid - 1;
+        }
+        return lo;
+    };
+    const int begin = nrows > 0 ? off(0) : 0, end = nrows > 0 ? off(nrows) : 0;
+    // share boundaries: the even split of [begin, end), moved back to the start of
+    // the row it falls in when that row has at most SPMM_LONG entries, so such a
+    // row is summed by one group, left to right, as the reference loop sums it
+    auto boundary = [&](int q) -> int {
+        const int p = begin + (int)((int64_t)(end - begin) * q / GROUPS);
+        if (q == 0 || p >= end) return q == 0 ? begin : end;
+        const int64_t i = row_of(p);
+        const int s = off(i);
+        return off(i + 1) - s <= SPMM_LONG ? s : p;
+    };
+    const int js = boundary(g), je = boundary(g + 1);
+    // rows are walked in order; the first row of a share may have started in an
+    // earlier share (its "head" part is left in LDS for the row's owner), the last
+    // may run on into later shares (its owner adds their head parts after a barrier)
+    int64_t first = 0;  // the first row starting at or after js
+    {
+        int64_t lo = 0, hi = nrows;
+        while (lo < hi) {
+            const int64_t mid = (lo + hi) >> 1;
+            if (off(mid) < js) lo = mid + 1;
+            else hi = mid;
+        }
+        first = lo;
+    }
+    const bool has_head = js < je && first > 0 && off(first) > js;
+    const int64_t head_row = has_head ? first - 1 : -1;
+    for (int64_t k0 = 0; k0 < k; k0 += LPR) {
+        const int64_t kk = k0 + lane;
+        const bool kin = kk < k;
+        int64_t cur = has_head ? head_row : first;
+        int cend = (js < je) ? off(cur + 1) : 0;
         T acc = T(0);
-        for (int j = s; j < e; j += 8) {
+        auto put = [&](int64_t r, T v) {  // the finished row r (absolute)
+            if (kin) out(r * ldc + kk * ldk, v);
+        };
+        // row `cur` is finished: a head part to LDS, an owned row to C
+        auto flush = [&]() {
+            if (cur == head_row) {
+                red[g * LPR + lane] = acc;
+            } else {
+                const int64_t r = r0 + cur;
+                put(r, row_scaled(acc, rscale, r));
+            }
+            acc = T(0);
+            ++cur;
+        };
+        for (int j = js; j < je; j += 8) {
             T av[8], bv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {  // lanes past the row's end issue no load
-                const int jj = j + u < e ? j + u : e - 1;
+            for (int u = 0; u < 8; ++u) {  // entries past the share's end issue no load
+                const int jj = j + u < je ? j + u : je - 1;
                 av[u] = s_val[jj];
-                bv[u] = j + u < e ? conj_if(BT[(int64_t)s_col[jj] * ldbt + kk], cjb) : T(0);
+                bv[u] = (j + u < je && kin) ? conj_if(BT[(int64_t)s_col[jj] * ldbt + kk], cjb)
+                                            : T(0);
             }
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
-                if (j + u < e) acc = acc + av[u] * bv[u];
-        }
-        return acc;
-    };
-    auto defer_long = [&](int64_t r) {
-        if (lane == 0) {
-            int q = atomicAdd(&nlong, 1);
-            if (q < SPMM_MAX_LONG) long_rows[q] = r;
-        }
-    };
-    for (int64_t r = r0 + g; r < rs_end; r += GROUPS) {
-        int s, e;
-        row_range(r, s, e);
-        if (e - s > SPMM_LONG) {
-            defer_long(r);
-            continue;
-        }
-        for (int64_t kk = lane; kk < k; kk += LPR)
-            store_out(C, r * ldc + kk * ldk, row_scaled(short_row(s, e, kk), rscale, r), alpha,
-                      beta, mode);
-    }
-    __syncthreads();
-    // longer staged rows: every group takes a contiguous share, LDS combine
-    const int nl = nlong < SPMM_MAX_LONG ? nlong : SPMM_MAX_LONG;
-    for (int q = 0; q < nl; ++q) {
-        const int64_t r = long_rows[q];
-        const int s = (int)(clamp64((int64_t)rowptr[r] - 1, P0, Ps) - P0);
-        const int e = (int)(clamp64((int64_t)rowptr[r + 1] - 1, P0, Ps) - P0);
-        const int per = (e - s + GROUPS - 1) / GROUPS;
-        const int js = s + g * per, je = (js + per < e) ? js + per : e;
-        for (int64_t k0 = 0; k0 < k; k0 += LPR) {
-            const int64_t kk = k0 + lane;
-            T acc = T(0);
-            if (kk < k) {
-                for (int j = js; j < je; j += 8) {
-                    T av[8], bv[8];
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int jj = j + u < je ? j + u : je - 1;
-                        av[u] = s_val[jj];
-                        bv[u] = conj_if(BT[(int64_t)s_col[jj] * ldbt + kk], cjb);
+            for (int u = 0; u < 8; ++u) {  // in entry order, closing the rows that end first
+                if (j + u < je) {
+                    while (cend <= j + u) {  // empty rows too
+                        flush();
+                        cend = off(cur + 1);
                     }
-#pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        if (j + u < je) acc = acc + av[u] * bv[u];
+                    acc = acc + av[u] * bv[u];
                 }
             }
-            // the groups of a wavefront add their partials by shuffles, then
-            // the four wavefronts' sums are added in order
-            T v = acc;
-#pragma unroll
-            for (int o = LPR; o < 64; o <<= 1) v = v + lane_xor(v, o);
-            if ((tid & 63) < LPR) red[(tid >> 6) * LPR + lane] = v;
-            __syncthreads();
-            if (tid < LPR && kk < k) {
-                T sum = red[lane];
-                for (int wq = 1; wq < WG / 64; ++wq) sum = sum + red[wq * LPR + lane];
-                store_out(C, r * ldc + kk * ldk, row_scaled(sum, rscale, r), alpha, beta, mode);
-            }
-            __syncthreads();
         }
+        int64_t tail_row = -1;
+        if (js < je) {
+            if (cend == je) {  // the share ends with its row
+                flush();
+            } else if (cur == head_row) {  // the share lies inside one row owned earlier
+                red[g * LPR + lane] = acc;
+                ++cur;
+            } else {
+                tail_row = cur++;
+            }
+            // empty rows at the end of the stage belong to the share that reaches it
+            if (je == end)
+                for (; cur < nrows; ++cur) put(r0 + cur, row_scaled(T(0), rscale, r0 + cur));
+        } else if (g == 0 && begin == end) {  // a stage of empty rows only
+            for (int64_t i = 0; i < nrows; ++i)
+                if (kin) {
+                    const int64_t r = r0 + i;
+                    out(r * ldc + kk * ldk, row_scaled(T(0), rscale, r));
+                }
+        }
+        if (lane == 0) s_hrow[g] = js == je ? -2 : head_row;
+        __syncthreads();
+        if (tail_row >= 0) {  // later shares' parts of this row, in entry order
+            for (int h = g + 1; h < GROUPS; ++h) {
+                const int64_t hr = s_hrow[h];
+                if (hr == -2) continue;
+                if (hr != tail_row) break;
+                acc = acc + red[h * LPR + lane];
+            }
+            if (kin) {
+                const int64_t r = r0 + tail_row;
+                out(r * ldc + kk * ldk, row_scaled(acc, rscale, r));
+            }
+        }
+        __syncthreads();
     }
 
     if (rs_end < r1) {  // spill row: staged in LDS chunk by chunk, groups split each chunk
@@ -654,7 +693,7 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
             if (tid < LPR && kk < k) {
                 T sum = red[lane];
                 for (int wq = 1; wq < WG / 64; ++wq) sum = sum + red[wq * LPR + lane];
-                store_out(C, r * ldc + kk * ldk, row_scaled(sum, rscale, r), alpha, beta, mode);
+                out(r * ldc + kk * ldk, row_scaled(sum, rscale, r));
             }
         }
     }
@@ -710,6 +749,53 @@ __global__ __launch_bounds__(256) void k_to_rowmajor(int64_t rows, int64_t cols,
                 const int rr = e / cw, cc = e % cw;
                 BT[(r0 + rr) * cols + c0 + cc] = tile[cc][rr];
             }
+        }
+        __syncthreads();
+    }
+}
+
+// CT (rows x cols, row-major, ld = cols) -> C (column-major, ldc) through store_out,
+// in tiles of 64 rows x 32 columns: the tile arrives as contiguous row-major runs
+// and leaves as 64-row column segments (512 contiguous bytes per wavefront for
+// Float64), so the SpMM kernel can write its rows whole and C still receives
+// whole lines.  The value in CT is the row's sum before α and β.
+template <typename T>
+__global__ __launch_bounds__(256) void k_from_rowmajor(int64_t rows, int64_t cols,
+                                                       const T* __restrict__ CT,
+                                                       T* __restrict__ C, int64_t ldc, T alpha,
+                                                       T beta, int mode) {
+    constexpr int RB = 64, CB = 32, Q = CB / 4;
+    __shared__ T tile[CB][RB + 1];  // [column][row]
+    const int64_t r0 = blockIdx.x * (int64_t)RB;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int rn = (int)(rows - r0 < RB ? rows - r0 : RB);
+    for (int64_t c0 = 0; c0 < cols; c0 += CB) {
+        const int cw = (int)(cols - c0 < CB ? cols - c0 : CB);
+        if (cw == CB && cols == CB) {  // the tile is one contiguous run of rn rows
+            const T* in = CT + r0 * cols;
+            T v[RB * CB / 256];
+#pragma unroll
+            for (int q = 0; q < RB * CB / 256; ++q) {
+                const int e = threadIdx.x + 256 * q;
+                v[q] = e / CB < rn ? in[e] : T(0);
+            }
+#pragma unroll
+            for (int q = 0; q < RB * CB / 256; ++q) {
+                const int e = threadIdx.x + 256 * q;
+                tile[e % CB][e / CB] = v[q];
+            }
+        } else {
+            for (int e = threadIdx.x; e < rn * cw; e += 256) {
+                const int rr = e / cw, cc = e % cw;
+                tile[cc][rr] = CT[(r0 + rr) * cols + c0 + cc];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int cc = wv + 4 * q;
+            if (cc < cw && lane < rn)
+                store_out(C, (r0 + lane) + (c0 + cc) * ldc, tile[cc][lane], alpha, beta, mode);
         }
         __syncthreads();
     }
@@ -969,17 +1055,32 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         BT = bt.as<T>();
         ldbt = k;
     }
-    const int64_t c_ld = c_rowmajor ? ldc : 1, c_lk = c_rowmajor ? 1 : ldc;
+    // a column-major C of more than a few rows is written through row-major scratch:
+    // the kernel then stores each row's k sums as one contiguous segment, and
+    // k_from_rowmajor turns the tiles into whole column segments of C (one 8-byte
+    // store per (row, column) into k different lines costs more than the extra pass)
+    const bool via_rows = !c_rowmajor && k > 1 && m >= SPMM_VIA_ROWS_MIN;
+    Scratch ct(s);
+    T* Cw = C;
+    int kmode = mode;
+    int64_t c_ld = c_rowmajor ? ldc : 1, c_lk = c_rowmajor ? 1 : ldc;
+    if (via_rows) {
+        GSA_TRY(ct.alloc(sizeof(T) * m * k));
+        Cw = ct.as<T>();
+        c_ld = k;
+        c_lk = 1;
+        kmode = -1;
+    }
 #define GSA_SPMM_LAUNCH(L)                                                                  \
     do {                                                                                    \
         if (n <= INT32_MAX)                                                                 \
             hipLaunchKernelGGL((k_csr_spmm<T, I, int32_t, L, V, PI>), dim3(grid), dim3(WG), 0, s, m, \
-                               n, k, nnz, rowptr, colval, nzval, perm, rscale, BT, ldbt, C, \
-                               c_ld, c_lk, alpha, beta, mode, cj, pu.blk_row, pu.blk_pos);   \
+                               n, k, nnz, rowptr, colval, nzval, perm, rscale, BT, ldbt, Cw, \
+                               c_ld, c_lk, alpha, beta, kmode, cj, pu.blk_row, pu.blk_pos);  \
         else                                                                                \
             hipLaunchKernelGGL((k_csr_spmm<T, I, I, L, V, PI>), dim3(grid), dim3(WG), 0, s, m, n, k, \
-                               nnz, rowptr, colval, nzval, perm, rscale, BT, ldbt, C, c_ld,  \
-                               c_lk, alpha, beta, mode, cj, pu.blk_row, pu.blk_pos);         \
+                               nnz, rowptr, colval, nzval, perm, rscale, BT, ldbt, Cw, c_ld, \
+                               c_lk, alpha, beta, kmode, cj, pu.blk_row, pu.blk_pos);        \
     } while (0)
     if (k <= 2) GSA_SPMM_LAUNCH(2);
     else if (k <= 4) GSA_SPMM_LAUNCH(4);
@@ -989,6 +1090,11 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
     else GSA_SPMM_LAUNCH(64);
 #undef GSA_SPMM_LAUNCH
     GSA_HIP(hipGetLastError());
+    if (via_rows) {
+        hipLaunchKernelGGL(k_from_rowmajor<T>, dim3((unsigned)cdiv(m, 64)), dim3(256), 0, s, m, k,
+                           (const T*)Cw, C, ldc, alpha, beta, mode);
+        GSA_HIP(hipGetLastError());
+    }
     return GSA_OK;
 }
 

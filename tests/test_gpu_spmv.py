@@ -176,6 +176,47 @@ def test_spmm_alpha_beta(dev, alpha, beta):
     assert_normwise(got, ref, bound, 1e-12)
 
 
+@pytest.mark.parametrize("k", [3, 32, 40, 70])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 0.0), (-1.5, 0.5), (2.0, 1.0)])
+def test_spmm_column_major_through_row_scratch(dev, k, alpha, beta):
+    """A column-major C of 4096 rows or more is written through row-major scratch
+    and a tile transpose that applies α and β: the same bits as the row-major
+    output of the same kernel, the oracle's bits on rows of at most 64 entries,
+    NaN in C cleared by β == 0, and a leading dimension larger than m kept."""
+    rng = np.random.default_rng(63)
+    lens = np.r_[rng.integers(0, 40, 6000), [0, 65, 4000, 700, 2300, 1], rng.integers(0, 9, 300)]
+    H = synth.csr_from_lengths(lens, 7000, rng, "normal")
+    B = rng.standard_normal((H.n, k))
+    ld = H.m + 3
+    C0 = rng.standard_normal((H.m, k))
+    if beta == 0.0:
+        C0[::7, :] = np.nan
+    A = dev_csr(H)
+    big = torch.full((k, ld), 7.0, dtype=torch.float64, device=dev)
+    big[:, :H.m] = torch.as_tensor(C0.T.copy(), device=dev)
+    Cd = big.t()[:H.m]  # column-major, ldc = m + 3
+    G.mul_(Cd, A, G.dense(B, device=dev), alpha, beta)
+    got = host(Cd)
+    assert torch.all(big[:, H.m:] == 7.0)  # the padding rows are not written
+    ref = O.spmm_csr_N(H.m, H.rowptr, H.colval, H.nzval, B, C=C0, alpha=alpha, beta=beta)
+    bound = abs(alpha) * O.spmm_csr_N(H.m, H.rowptr, H.colval, np.abs(H.nzval), np.abs(B)) \
+        + np.nan_to_num(np.abs(beta * C0))
+    assert not np.isnan(got).any()
+    assert_normwise(got, ref, bound, 1e-12)
+    short = np.diff(H.rowptr) <= 64
+    assert np.array_equal(got[short], ref[short])
+    # the row-major output of the same product (no scratch, same kernel): same bits
+    from gsa_hip import _lib as L
+    from gsa_hip.matrix import ptr, stream_of
+    Cr = torch.as_tensor(C0.copy(), device=dev)
+    s = stream_of(torch.device(dev))
+    Bt = torch.as_tensor(B, device=dev)
+    L.call("gsa_spmm_csr_ex", s, L.GSA_F64, L.GSA_I64, H.m, H.n, k, ptr(A.rowptr), ptr(A.colval),
+           ptr(A.nzval), A.nnz(), ptr(Bt), k, L.GSA_OP_T, ptr(Cr), k, L.GSA_OP_T, alpha, beta,
+           A.plan())
+    assert np.array_equal(host(Cr), got)
+
+
 def test_spmm_beta_into_strided_output(dev):
     H = synth.sprand(500, 400, 0.02, seed=14, values="normal")
     rng = np.random.default_rng(15)
