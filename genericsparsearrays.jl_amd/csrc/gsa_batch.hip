@@ -22,7 +22,8 @@ namespace gsa {
 constexpr int BWG = 256;
 constexpr int BATCH_LONG = 256;   // rows above this: workgroup-cooperative, compensated
 constexpr int BATCH_MAX_TILE_ROWS = 1024;
-constexpr int BATCH_TCAP = 3072;   // entries of a tile staged in LDS
+constexpr int BATCH_TCAP = 3072;   // entries of a tile staged in LDS at a time
+constexpr int BATCH_CHUNK_LONG = BATCH_TCAP / (BATCH_LONG + 1) + 1;
 
 struct BatchPlan {
     int vtype = 0, itype = 0;
@@ -72,17 +73,21 @@ __global__ void k_reset_counter(int* c) { *c = 0; }
 template <typename T>
 struct TileLds {
     int nlong;
-    int long_rows[BATCH_MAX_TILE_ROWS];
+    int long_rows[BATCH_CHUNK_LONG];    // a chunk's rows over BATCH_LONG entries
     int off[BATCH_MAX_TILE_ROWS + 1];   // staged row offsets (relative to P0)
-    int col[BATCH_TCAP];                // staged column indices (0-based, clamped)
-    T val[BATCH_TCAP];                  // staged values (BF16 widened)
+    int col[BATCH_TCAP];                // one chunk's column indices (0-based, clamped)
+    T val[BATCH_TCAP];                  // and values (BF16 widened)
     T red[BWG];
     T cmp[BWG];
 };
 
 // One tile: rows [r0, r1) of instance d.  All BWG lanes call it together.
-// The tile's row offsets and its first BATCH_TCAP entries are staged in LDS
-// with coalesced loads; rows beyond the staged extent read HBM directly.
+// The tile's row offsets are staged in LDS; its entries go through LDS in
+// chunks of whole rows of at most BATCH_TCAP entries (coalesced loads).  In a
+// chunk each row of at most BATCH_LONG entries gets LPR = next_pow2(k) lanes
+// (one (row, kk) output per lane, summed left to right); longer rows are
+// reduced by the whole workgroup with TwoSum, from LDS, or from HBM for a row
+// longer than a chunk.
 template <typename V, typename I>
 __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0, int64_t r1,
                                              typename Acc<V>::T alpha, typename Acc<V>::T beta,
@@ -105,70 +110,24 @@ __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0
     const int64_t csr = crow ? d.ldc : 1, csc = crow ? 1 : d.ldc;
     const int nrows = (int)(r1 - r0);
     const int64_t P0 = (int64_t)rowptr[r0] - 1;
-    const int64_t P1 = (int64_t)rowptr[r1] - 1;
-    const int cnt = (int)(P1 - P0 < BATCH_TCAP ? P1 - P0 : BATCH_TCAP);
-    if (tid == 0) sh.nlong = 0;
     for (int i = tid; i <= nrows; i += BWG) sh.off[i] = (int)((int64_t)rowptr[r0 + i] - 1 - P0);
-    for (int i = tid; i < cnt; i += BWG) {
-        int64_t c = (int64_t)colval[P0 + i] - 1;
-        sh.col[i] = (int)(c < 0 ? 0 : (c >= n ? n - 1 : c));
-        sh.val[i] = vload(nzval + P0 + i);
-    }
     __syncthreads();
     int lpr = 1;
     while (lpr < k && lpr < 64) lpr <<= 1;
     const int groups = BWG / lpr;
     const int g = tid / lpr, lane = tid % lpr;
-    for (int rr = g; rr < nrows; rr += groups) {
-        const int64_t r = r0 + rr;
-        const int s = sh.off[rr], e = sh.off[rr + 1];
-        if (e - s > BATCH_LONG) {
-            if (lane == 0) {
-                int q = atomicAdd(&sh.nlong, 1);
-                if (q < BATCH_MAX_TILE_ROWS) sh.long_rows[q] = rr;
-            }
-            continue;
-        }
-        const bool staged = e <= cnt;
-        for (int kk = lane; kk < k; kk += lpr) {
-            // 8 gathers in flight per lane; the sum stays left to right
-            T acc = T(0);
-            for (int j = s; j < e; j += 8) {
-                T pv[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int jj = j + u < e ? j + u : e - 1;
-                    int64_t c;
-                    T v;
-                    if (staged) {
-                        c = sh.col[jj];
-                        v = sh.val[jj];
-                    } else {
-                        c = (int64_t)colval[P0 + jj] - 1;
-                        c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-                        v = vload(nzval + P0 + jj);
-                    }
-                    pv[u] = v * B[c * bsr + kk * bsc];
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u)
-                    if (j + u < e) acc = acc + pv[u];
-            }
-            if (scale) acc = acc * (T)scale[r];
-            T* out = C + r * csr + kk * csc;
-            T tt = acc * alpha;
-            T base = mode == 0 ? T(0) : (mode == 1 ? *out : *out * beta);
-            *out = base + tt;
-        }
-    }
-    __syncthreads();
-    // long rows: the groups stride over the entries, each lane owns one kk
-    // (all k columns in one pass), TwoSum per lane, then an error-free
-    // combine of the groups' partial sums through LDS
-    const int nl = sh.nlong < BATCH_MAX_TILE_ROWS ? sh.nlong : BATCH_MAX_TILE_ROWS;
-    for (int q = 0; q < nl; ++q) {
-        const int64_t r = r0 + sh.long_rows[q];
-        const int64_t s = P0 + sh.off[sh.long_rows[q]], e = P0 + sh.off[sh.long_rows[q] + 1];
+    auto emit = [&](int64_t r, int kk, T acc) {
+        if (scale) acc = acc * (T)scale[r];
+        T* out = C + r * csr + kk * csc;
+        T tt = acc * alpha;
+        T base = mode == 0 ? T(0) : (mode == 1 ? *out : *out * beta);
+        *out = base + tt;
+    };
+    // one row by the whole workgroup: the groups stride over its entries, each
+    // lane owns one kk (all k columns in one pass), TwoSum per lane, then an
+    // error-free combine of the groups' partial sums through LDS.  `at(j)` gives
+    // entry j's (column, value), j in [s, e).
+    auto wg_row = [&](int64_t r, int64_t s, int64_t e, auto at) {
         for (int k0 = 0; k0 < k; k0 += lpr) {
             const int kk = k0 + lane;
             const bool act = kk < k;
@@ -180,9 +139,10 @@ __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0
                 for (int u = 0; u < 4; ++u) {
                     const int64_t j2 = j + u * (int64_t)groups;
                     const int64_t jj = j2 < e ? j2 : e - 1;
-                    int64_t c = (int64_t)colval[jj] - 1;
-                    c = c < 0 ? 0 : (c >= n ? n - 1 : c);
-                    pv[u] = (j2 < e) ? vload(nzval + jj) * B[c * bsr + kc * bsc] : T(0);
+                    int64_t c;
+                    T v;
+                    at(jj, c, v);
+                    pv[u] = (j2 < e) ? v * B[c * bsr + kc * bsc] : T(0);
                 }
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
@@ -205,18 +165,82 @@ __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0
                 }
                 __syncthreads();
             }
-            if (g == 0 && act) {
-                T acc = sh.red[tid] + sh.cmp[tid];
-                if (scale) acc = acc * (T)scale[r];
-                T* out = C + r * csr + kk * csc;
-                T tt = acc * alpha;
-                T base = mode == 0 ? T(0) : (mode == 1 ? *out : *out * beta);
-                *out = base + tt;
-            }
+            if (g == 0 && act) emit(r, kk, sh.red[tid] + sh.cmp[tid]);
             __syncthreads();
         }
+    };
+    for (int ca = 0; ca < nrows;) {
+        // the chunk: rows [ca, cb) with at most BATCH_TCAP entries (at least one row)
+        const int ea = sh.off[ca];
+        int cb;
+        {
+            int lo = ca, hi = nrows;  // the last cb with off[cb] - ea <= BATCH_TCAP
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (sh.off[mid] - ea <= BATCH_TCAP) lo = mid;
+                else hi = mid - 1;
+            }
+            cb = lo;
+        }
+        if (cb == ca) {  // a row longer than a chunk, from HBM
+            wg_row(r0 + ca, P0 + sh.off[ca], P0 + sh.off[ca + 1],
+                   [&](int64_t j, int64_t& c, T& v) {
+                       c = (int64_t)colval[j] - 1;
+                       c = c < 0 ? 0 : (c >= n ? n - 1 : c);
+                       v = vload(nzval + j);
+                   });
+            ++ca;
+            continue;
+        }
+        const int eb = sh.off[cb];
+        if (tid == 0) sh.nlong = 0;
+        for (int i = tid; i < eb - ea; i += BWG) {
+            int64_t c = (int64_t)colval[P0 + ea + i] - 1;
+            sh.col[i] = (int)(c < 0 ? 0 : (c >= n ? n - 1 : c));
+            sh.val[i] = vload(nzval + P0 + ea + i);
+        }
+        __syncthreads();
+        for (int rr = ca + g; rr < cb; rr += groups) {
+            const int64_t r = r0 + rr;
+            const int s = sh.off[rr] - ea, e = sh.off[rr + 1] - ea;
+            if (e - s > BATCH_LONG) {
+                if (lane == 0) {
+                    int q = atomicAdd(&sh.nlong, 1);
+                    if (q < BATCH_CHUNK_LONG) sh.long_rows[q] = rr;
+                }
+                continue;
+            }
+            for (int kk = lane; kk < k; kk += lpr) {
+                // 8 gathers in flight per lane; the sum stays left to right
+                T acc = T(0);
+                for (int j = s; j < e; j += 8) {
+                    T pv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) {
+                        const int jj = j + u < e ? j + u : e - 1;
+                        pv[u] = sh.val[jj] * B[(int64_t)sh.col[jj] * bsr + kk * bsc];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 8; ++u)
+                        if (j + u < e) acc = acc + pv[u];
+                }
+                emit(r, kk, acc);
+            }
+        }
+        __syncthreads();
+        // the chunk's long rows, from LDS
+        const int nl = sh.nlong < BATCH_CHUNK_LONG ? sh.nlong : BATCH_CHUNK_LONG;
+        for (int q = 0; q < nl; ++q) {
+            const int rr = sh.long_rows[q];
+            wg_row(r0 + rr, sh.off[rr] - ea, sh.off[rr + 1] - ea,
+                   [&](int64_t j, int64_t& c, T& v) {
+                       c = sh.col[j];
+                       v = sh.val[j];
+                   });
+        }
+        __syncthreads();  // the staging is reused by the next chunk
+        ca = cb;
     }
-    __syncthreads();  // the LDS staging is reused by the next tile
 }
 
 // Persistent: tiles from the table, taken through the atomic counter.
@@ -236,7 +260,19 @@ __global__ __launch_bounds__(BWG) void k_batch_spmm(const gsa_batch_desc* __rest
         __syncthreads();  // every lane has read s_tile before lane 0 takes the next
         if (t >= ntiles) break;
         const gsa_batch_desc d = desc[tiles[3 * t]];
-        process_tile<V, I>(d, tiles[3 * t + 1], tiles[3 * t + 2], alpha, beta, mode, sh);
+        const int64_t ta = tiles[3 * t + 1] > 0 ? tiles[3 * t + 1] : 0;
+        const int64_t tb = tiles[3 * t + 2] < d.m ? tiles[3 * t + 2] : d.m;
+        const I* __restrict__ rp = (const I*)d.rowptr;
+        // pieces of at most BATCH_MAX_TILE_ROWS rows whose entries an int counts;
+        // failing that, one row at a time (a row has at most n < 2^31 entries)
+        for (int64_t a = ta; a < tb; a += BATCH_MAX_TILE_ROWS) {
+            const int64_t b = a + BATCH_MAX_TILE_ROWS < tb ? a + BATCH_MAX_TILE_ROWS : tb;
+            if ((int64_t)rp[b] - (int64_t)rp[a] <= (int64_t)INT32_MAX) {
+                process_tile<V, I>(d, a, b, alpha, beta, mode, sh);
+            } else {
+                for (int64_t r = a; r < b; ++r) process_tile<V, I>(d, r, r + 1, alpha, beta, mode, sh);
+            }
+        }
     }
 }
 

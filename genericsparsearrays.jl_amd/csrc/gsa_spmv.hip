@@ -461,7 +461,8 @@ __global__ void k_win_bucket(int64_t m, int64_t n, int64_t nnz, const I* __restr
 // ------------------------------------------------------------ SpMM kernel --
 constexpr int SPMM_LONG = 64;  // staged rows up to this length are summed by one group
 constexpr int SPMM_RP = 1024;  // row offsets staged in LDS per workgroup
-constexpr int64_t SPMM_VIA_ROWS_MIN = 4096;  // rows from which column-major C goes via scratch
+// outputs (m·k) from which a column-major C goes through row-major scratch
+constexpr int64_t SPMM_VIA_ROWS_MIN = int64_t(1) << 21;
 
 // Dense operand read as BT[col*ldbt + kk] (row-major).  LPR lanes per row.
 template <typename T, typename I, typename CI, int LPR, typename V = T, typename PI = void,
@@ -486,6 +487,9 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
     __shared__ int s_off[SPMM_RP];
     // per group: the row whose head part it left in red, -1 for none, -2 for an empty share
     __shared__ int64_t s_hrow[WG / LPR];
+    // share q: first entry, first row at or after it, the row it starts inside (or -1)
+    __shared__ int s_bnd[WG / LPR + 1];
+    __shared__ int64_t s_first[WG / LPR + 1], s_head[WG / LPR + 1];
 
     const int tid = threadIdx.x;
     const Window w = load_window(blockIdx.x, m, nnz, blk_row, blk_pos);
@@ -540,43 +544,51 @@ __global__ __launch_bounds__(WG) void k_csr_spmm(
         return rp_in_lds ? s_off[i]
                          : (int)(clamp64((int64_t)rowptr[r0 + i] - 1, P0, Ps) - P0);
     };
-    // the row holding entry p (off(i) <= p < off(i + 1)), for begin <= p < end
-    auto row_of = [&](int p) -> int64_t {
-        int64_t lo = 0, hi = nrows - 1;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi + 1) >> 1;
-            if (off(mid) <= p) lo = mid;
-            else hi = mid - 1;
-        }
-        return lo;
-    };
     const int begin = nrows > 0 ? off(0) : 0, end = nrows > 0 ? off(nrows) : 0;
-    // share boundaries: the even split of [begin, end), moved back to the start of
-    // the row it falls in when that row has at most SPMM_LONG entries, so such a
-    // row is summed by one group, left to right, as the reference loop sums it
-    auto boundary = [&](int q) -> int {
-        const int p = begin + (int)((int64_t)(end - begin) * q / GROUPS);
-        if (q == 0 || p >= end) return q == 0 ? begin : end;
-        const int64_t i = row_of(p);
-        const int s = off(i);
-        return off(i + 1) - s <= SPMM_LONG ? s : p;
-    };
-    const int js = boundary(g), je = boundary(g + 1);
+    // Share boundaries: the even split of [begin, end) at p_q = begin + D*q/GROUPS,
+    // moved back to the start of the row it falls in when that row has at most
+    // SPMM_LONG entries, so such a row is summed by one group, left to right, as
+    // the reference loop sums it.  Each row writes the boundaries that fall in it
+    // (q with off(i) <= p_q < off(i+1), found by division), with the first row at
+    // or after the boundary and the row it cuts (whose "head" part the share
+    // takes over), so no group searches for them.
+    const int D = end - begin;
+    if (tid <= GROUPS && (tid == 0 || tid == GROUPS || D == 0)) {
+        s_bnd[tid] = tid == GROUPS ? end : begin;
+        s_first[tid] = 0;
+        s_head[tid] = -1;
+    }
+    if (D > 0) {
+        for (int64_t i = tid; i < nrows; i += WG) {
+            const int a = off(i), b = off(i + 1);
+            if (b <= a) continue;  // an empty row holds no position
+            int q = (int)(((int64_t)(a - begin) * GROUPS + D - 1) / D);
+            const int qe = (int)(((int64_t)(b - begin) * GROUPS + D - 1) / D);  // q < qe
+            q = q > 1 ? q : 1;
+            for (; q < qe && q < GROUPS; ++q) {
+                const int p = begin + (int)((int64_t)D * q / GROUPS);
+                if (b - a <= SPMM_LONG || p == a) {
+                    int64_t j = i;  // empty rows just before i start there too
+                    while (j > 0 && off(j - 1) == a) --j;
+                    s_bnd[q] = a;
+                    s_first[q] = j;
+                    s_head[q] = -1;
+                } else {
+                    s_bnd[q] = p;
+                    s_first[q] = i + 1;
+                    s_head[q] = i;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    const int js = s_bnd[g], je = s_bnd[g + 1];
     // rows are walked in order; the first row of a share may have started in an
     // earlier share (its "head" part is left in LDS for the row's owner), the last
     // may run on into later shares (its owner adds their head parts after a barrier)
-    int64_t first = 0;  // the first row starting at or after js
-    {
-        int64_t lo = 0, hi = nrows;
-        while (lo < hi) {
-            const int64_t mid = (lo + hi) >> 1;
-            if (off(mid) < js) lo = mid + 1;
-            else hi = mid;
-        }
-        first = lo;
-    }
-    const bool has_head = js < je && first > 0 && off(first) > js;
-    const int64_t head_row = has_head ? first - 1 : -1;
+    const int64_t first = s_first[g];
+    const bool has_head = js < je && s_head[g] >= 0;
+    const int64_t head_row = has_head ? s_head[g] : -1;
     for (int64_t k0 = 0; k0 < k; k0 += LPR) {
         const int64_t kk = k0 + lane;
         const bool kin = kk < k;
@@ -1055,11 +1067,11 @@ static int gather_csr(hipStream_t s, int64_t m, int64_t n, int64_t k, const I* r
         BT = bt.as<T>();
         ldbt = k;
     }
-    // a column-major C of more than a few rows is written through row-major scratch:
+    // a column-major C of 2^21 outputs or more is written through row-major scratch:
     // the kernel then stores each row's k sums as one contiguous segment, and
     // k_from_rowmajor turns the tiles into whole column segments of C (one 8-byte
     // store per (row, column) into k different lines costs more than the extra pass)
-    const bool via_rows = !c_rowmajor && k > 1 && m >= SPMM_VIA_ROWS_MIN;
+    const bool via_rows = !c_rowmajor && k > 1 && m * k >= SPMM_VIA_ROWS_MIN;
     Scratch ct(s);
     T* Cw = C;
     int kmode = mode;
