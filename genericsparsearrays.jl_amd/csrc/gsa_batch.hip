@@ -2,12 +2,15 @@
 // the BF16-with-row-scale value format, and HIP-graph capture of a batch.
 //
 // Work unit: a tile = (instance, [r0, r1)) from a device table built once per
-// batch.  A persistent grid of workgroups takes tiles from an atomic counter
-// (reset by a one-thread kernel launched just before, so the pair is
-// graph-capturable: no memset).  The counter is scratch of the call, so a plan
-// is read-only and may serve concurrent launches on several streams.  Inside a tile each row gets
-// LPR = next_pow2(k) lanes (one (row, kk) output per lane, summed left to
-// right); rows over BATCH_LONG entries are reduced by the whole workgroup.
+// batch, in 8 segments, one per XCD (an instance's tiles stay on one XCD, so
+// its dense operand stays in that XCD's L2).  A persistent grid of workgroups
+// takes tiles through one atomic counter per segment (reset by a small kernel
+// launched just before, so the pair is graph-capturable: no memset).  The
+// counters are scratch of the call, so a plan is read-only and may serve
+// concurrent launches on several streams.  Inside a tile each row gets LPR
+// lanes, each summing W consecutive output columns left to right (W elements
+// of a row-major B per vector load, up to 16 bytes; W = 1 otherwise); rows over
+// BATCH_LONG entries are reduced by the whole workgroup.
 //
 // Precision: Float32 and BF16 storage accumulate in Float32; rows over 256
 // entries accumulate with TwoSum (an error-free sum and a running
@@ -15,6 +18,7 @@
 // Float64.  BF16 values carry one Float32 scale per row (a power of two, so
 // scaling is exact): a_ij = scale_i * bf16_ij.
 #include "gsa_common.h"
+#include <algorithm>
 #include <vector>
 
 namespace gsa {
@@ -25,12 +29,24 @@ constexpr int BATCH_MAX_TILE_ROWS = 1024;
 constexpr int BATCH_TCAP = 3072;   // entries of a tile staged in LDS at a time
 constexpr int BATCH_CHUNK_LONG = BATCH_TCAP / (BATCH_LONG + 1) + 1;
 
+// The tile table in segments: workgroup b starts on segment b % n, whose
+// tiles are those of the instances given to XCD b % 8 (workgroups go to the
+// XCDs round-robin), so each XCD's L2 holds the dense operands of its own
+// instances only.  A workgroup that finds its segment used up takes tiles
+// from the next ones.
+constexpr int BATCH_SEGS = 8;
+struct BatchSegs {
+    int64_t start[BATCH_SEGS + 1];
+    int n;
+};
+
 struct BatchPlan {
     int vtype = 0, itype = 0;
     int64_t count = 0;
     gsa_batch_desc* d_desc = nullptr;
     int32_t* d_tiles = nullptr;   // 3 per tile: instance, r0, r1
     int64_t ntiles = 0;
+    BatchSegs segs{};
     int grid = 0;
 };
 
@@ -67,7 +83,19 @@ __device__ __forceinline__ void two_sum(T a, T b, T& s, T& e) {
     e = (a - ap) + (b - bp);
 }
 
-__global__ void k_reset_counter(int* c) { *c = 0; }
+__global__ void k_reset_counter(int* c, int n) {
+    if ((int)threadIdx.x < n) c[threadIdx.x] = 0;
+}
+
+// Stored entries of every instance (rowptr[m] - 1), for the plan's balance.
+template <typename I>
+__global__ void k_inst_nnz(const gsa_batch_desc* __restrict__ desc, int64_t count,
+                           int64_t* __restrict__ out) {
+    const int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const gsa_batch_desc d = desc[i];
+    out[i] = d.m > 0 ? (int64_t)((const I*)d.rowptr)[d.m] - 1 : 0;
+}
 
 // Shared-memory workspace of one workgroup processing tiles.
 template <typename T>
@@ -81,6 +109,28 @@ struct TileLds {
     T cmp[BWG];
 };
 
+template <typename T, int W> struct VecT { typedef T type __attribute__((ext_vector_type(W))); };
+
+// Elements of B one lane loads per stored entry: a 16-byte (or 8-byte) vector
+// of a row-major B whose k, leading dimension and base allow it, else 1.  A
+// row-major B row of k elements then takes k / W lanes, so one load
+// instruction covers W times as many stored entries.
+__host__ __device__ inline int batch_vw(const gsa_batch_desc& d, int esize) {
+    if (!(d.flags & GSA_BATCH_B_ROWMAJOR) || d.k < 2) return 1;
+    for (int w = 16 / esize; w >= 2; w >>= 1)
+        if (d.k % w == 0 && d.ldb % w == 0 && ((uintptr_t)d.B) % (uintptr_t)(w * esize) == 0)
+            return w;
+    return 1;
+}
+
+// Lanes per row: next_pow2(k / W), at most 64.
+__host__ __device__ inline int batch_lpr(const gsa_batch_desc& d, int esize) {
+    const int64_t slots = (d.k + batch_vw(d, esize) - 1) / batch_vw(d, esize);
+    int lpr = 1;
+    while (lpr < slots && lpr < 64) lpr <<= 1;
+    return lpr;
+}
+
 // One tile: rows [r0, r1) of instance d.  All BWG lanes call it together.
 // The tile's row offsets are staged in LDS; its entries go through LDS in
 // chunks of whole rows of at most BATCH_TCAP entries (coalesced loads).  In a
@@ -88,7 +138,7 @@ struct TileLds {
 // (one (row, kk) output per lane, summed left to right); longer rows are
 // reduced by the whole workgroup with TwoSum, from LDS, or from HBM for a row
 // longer than a chunk.
-template <typename V, typename I>
+template <typename V, typename I, int VW>
 __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0, int64_t r1,
                                              typename Acc<V>::T alpha, typename Acc<V>::T beta,
                                              int mode, TileLds<typename Acc<V>::T>& sh) {
@@ -112,8 +162,7 @@ __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0
     const int64_t P0 = (int64_t)rowptr[r0] - 1;
     for (int i = tid; i <= nrows; i += BWG) sh.off[i] = (int)((int64_t)rowptr[r0 + i] - 1 - P0);
     __syncthreads();
-    int lpr = 1;
-    while (lpr < k && lpr < 64) lpr <<= 1;
+    const int lpr = batch_lpr(d, (int)sizeof(T));  // VW == batch_vw(d, sizeof(T))
     const int groups = BWG / lpr;
     const int g = tid / lpr, lane = tid % lpr;
     auto emit = [&](int64_t r, int kk, T acc) {
@@ -210,21 +259,50 @@ __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0
                 }
                 continue;
             }
-            for (int kk = lane; kk < k; kk += lpr) {
-                // 8 gathers in flight per lane; the sum stays left to right
-                T acc = T(0);
-                for (int j = s; j < e; j += 8) {
-                    T pv[8];
+            if constexpr (VW == 1) {
+                for (int kk = lane; kk < k; kk += lpr) {
+                    // 8 gathers in flight per lane; the sum stays left to right
+                    T acc = T(0);
+                    for (int j = s; j < e; j += 8) {
+                        T pv[8];
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const int jj = j + u < e ? j + u : e - 1;
-                        pv[u] = sh.val[jj] * B[(int64_t)sh.col[jj] * bsr + kk * bsc];
+                        for (int u = 0; u < 8; ++u) {
+                            const int jj = j + u < e ? j + u : e - 1;
+                            pv[u] = sh.val[jj] * B[(int64_t)sh.col[jj] * bsr + kk * bsc];
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (j + u < e) acc = acc + pv[u];
+                    }
+                    emit(r, kk, acc);
+                }
+            } else {
+                // row-major B: VW consecutive columns kk0.. per lane, one vector load
+                // per stored entry, 8 in flight; each column's sum left to right
+                using VT = typename VecT<T, VW>::type;
+                for (int kk0 = lane * VW; kk0 < k; kk0 += lpr * VW) {
+                    T acc[VW];
+#pragma unroll
+                    for (int c = 0; c < VW; ++c) acc[c] = T(0);
+                    for (int j = s; j < e; j += 8) {
+                        VT bv[8];
+                        T av[8];
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) {
+                            const int jj = j + u < e ? j + u : e - 1;
+                            av[u] = sh.val[jj];
+                            bv[u] = *reinterpret_cast<const VT*>(B + (int64_t)sh.col[jj] * bsr + kk0);
+                        }
+#pragma unroll
+                        for (int u = 0; u < 8; ++u)
+                            if (j + u < e) {
+#pragma unroll
+                                for (int c = 0; c < VW; ++c) acc[c] = acc[c] + av[u] * bv[u][c];
+                            }
                     }
 #pragma unroll
-                    for (int u = 0; u < 8; ++u)
-                        if (j + u < e) acc = acc + pv[u];
+                    for (int c = 0; c < VW; ++c) emit(r, kk0 + c, acc[c]);
                 }
-                emit(r, kk, acc);
             }
         }
         __syncthreads();
@@ -243,22 +321,30 @@ __device__ __forceinline__ void process_tile(const gsa_batch_desc& d, int64_t r0
     }
 }
 
-// Persistent: tiles from the table, taken through the atomic counter.
+// Persistent: tiles from the table's segments, each taken through its atomic
+// counter (one per segment); a workgroup leaves when every segment is used up.
 template <typename V, typename I>
 __global__ __launch_bounds__(BWG) void k_batch_spmm(const gsa_batch_desc* __restrict__ desc,
                                                     const int32_t* __restrict__ tiles,
-                                                    int64_t ntiles, int* __restrict__ counter,
+                                                    BatchSegs segs, int* __restrict__ counter,
                                                     typename Acc<V>::T alpha,
                                                     typename Acc<V>::T beta, int mode) {
     using T = typename Acc<V>::T;
     __shared__ int s_tile;
     __shared__ TileLds<T> sh;
+    const int nseg = segs.n > 1 ? (segs.n < BATCH_SEGS ? segs.n : BATCH_SEGS) : 1;
+    int seg = (int)(blockIdx.x % (unsigned)nseg), tried = 0;
     for (;;) {
-        if (threadIdx.x == 0) s_tile = atomicAdd(counter, 1);
+        if (threadIdx.x == 0) s_tile = atomicAdd(counter + seg, 1);
         __syncthreads();
-        const int64_t t = s_tile;
+        int64_t t = s_tile;
         __syncthreads();  // every lane has read s_tile before lane 0 takes the next
-        if (t >= ntiles) break;
+        if (t >= segs.start[seg + 1] - segs.start[seg]) {  // this segment is used up
+            if (++tried >= nseg) break;
+            seg = seg + 1 < nseg ? seg + 1 : 0;
+            continue;
+        }
+        t += segs.start[seg];
         const gsa_batch_desc d = desc[tiles[3 * t]];
         const int64_t ta = tiles[3 * t + 1] > 0 ? tiles[3 * t + 1] : 0;
         const int64_t tb = tiles[3 * t + 2] < d.m ? tiles[3 * t + 2] : d.m;
@@ -267,10 +353,16 @@ __global__ __launch_bounds__(BWG) void k_batch_spmm(const gsa_batch_desc* __rest
         // failing that, one row at a time (a row has at most n < 2^31 entries)
         for (int64_t a = ta; a < tb; a += BATCH_MAX_TILE_ROWS) {
             const int64_t b = a + BATCH_MAX_TILE_ROWS < tb ? a + BATCH_MAX_TILE_ROWS : tb;
+            const int vw = batch_vw(d, (int)sizeof(T));
+            auto run = [&](int64_t x, int64_t y) {
+                if (vw == 4) process_tile<V, I, 4>(d, x, y, alpha, beta, mode, sh);
+                else if (vw == 2) process_tile<V, I, 2>(d, x, y, alpha, beta, mode, sh);
+                else process_tile<V, I, 1>(d, x, y, alpha, beta, mode, sh);
+            };
             if ((int64_t)rp[b] - (int64_t)rp[a] <= (int64_t)INT32_MAX) {
-                process_tile<V, I>(d, a, b, alpha, beta, mode, sh);
+                run(a, b);
             } else {
-                for (int64_t r = a; r < b; ++r) process_tile<V, I>(d, r, r + 1, alpha, beta, mode, sh);
+                for (int64_t r = a; r < b; ++r) run(r, r + 1);
             }
         }
     }
@@ -300,18 +392,19 @@ __global__ void k_row_bf16(int64_t m, const I* __restrict__ rowptr, const void* 
     }
 }
 
+// d_counter: segs.n ints of scratch
 static int launch_batch(hipStream_t s, int vtype, int itype, const gsa_batch_desc* d_desc,
-                        const int32_t* d_tiles, int64_t ntiles, int* d_counter, double alpha,
-                        double beta, int grid) {
+                        const int32_t* d_tiles, const BatchSegs& segs, int* d_counter,
+                        double alpha, double beta, int grid) {
     const int mode = beta == 0.0 ? 0 : (beta == 1.0 ? 1 : 2);
-    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, s, d_counter);
-    if (ntiles == 0) return GSA_OK;
+    hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(64), 0, s, d_counter, segs.n);
+    if (segs.start[segs.n] == 0) return GSA_OK;
     if (grid <= 0) grid = 256 * 8;
     auto go = [&](auto vz, auto iz) {
         using V = decltype(vz);
         using I = decltype(iz);
         using T = typename Acc<V>::T;
-        hipLaunchKernelGGL((k_batch_spmm<V, I>), dim3(grid), dim3(BWG), 0, s, d_desc, d_tiles, ntiles,
+        hipLaunchKernelGGL((k_batch_spmm<V, I>), dim3(grid), dim3(BWG), 0, s, d_desc, d_tiles, segs,
                            d_counter, (T)alpha, (T)beta, mode);
         return (int)GSA_OK;
     };
@@ -323,21 +416,47 @@ static int launch_batch(hipStream_t s, int vtype, int itype, const gsa_batch_des
     return GSA_OK;
 }
 
-// Tiles for `count` instances: tile rows = max(16, rows_per_tile / next_pow2(k)).
-static void make_tiles(const gsa_batch_desc* h, int64_t count, int rows_per_tile,
-                       std::vector<int32_t>& tiles) {
+// Tiles for `count` instances: tile rows = max(16, rows_per_tile / lanes per
+// row).  The instances are dealt to the BATCH_SEGS segments largest first,
+// each to the least loaded (work ~ entries weighted by k, plus rows), and keep
+// their order inside a segment.
+static void make_tiles(const gsa_batch_desc* h, const int64_t* nnz, int64_t count,
+                       int rows_per_tile, int esize, std::vector<int32_t>& tiles,
+                       BatchSegs& segs) {
+    std::vector<int64_t> idx((size_t)count);
+    std::vector<double> work((size_t)count);
     for (int64_t i = 0; i < count; ++i) {
-        int lpr = 1;
-        while (lpr < h[i].k && lpr < 64) lpr <<= 1;
-        int64_t rows = rows_per_tile / lpr;
-        if (rows < 16) rows = 16;
-        if (rows > BATCH_MAX_TILE_ROWS) rows = BATCH_MAX_TILE_ROWS;
-        for (int64_t r = 0; r < h[i].m; r += rows) {
-            tiles.push_back((int32_t)i);
-            tiles.push_back((int32_t)r);
-            tiles.push_back((int32_t)(r + rows < h[i].m ? r + rows : h[i].m));
+        idx[(size_t)i] = i;
+        work[(size_t)i] = (double)nnz[i] * (8.0 + h[i].k) / 8.0 + (double)h[i].m;
+    }
+    std::stable_sort(idx.begin(), idx.end(),
+                     [&](int64_t a, int64_t b) { return work[(size_t)a] > work[(size_t)b]; });
+    std::vector<int> bin((size_t)count, 0);
+    double load[BATCH_SEGS] = {0};
+    for (int64_t i : idx) {
+        int x = 0;
+        for (int y = 1; y < BATCH_SEGS; ++y)
+            if (load[y] < load[x]) x = y;
+        bin[(size_t)i] = x;
+        load[x] += work[(size_t)i];
+    }
+    segs.n = BATCH_SEGS;
+    for (int x = 0; x < BATCH_SEGS; ++x) {
+        segs.start[x] = (int64_t)tiles.size() / 3;
+        for (int64_t i = 0; i < count; ++i) {
+            if (bin[(size_t)i] != x) continue;
+            const int lpr = batch_lpr(h[i], esize);
+            int64_t rows = rows_per_tile / lpr;
+            if (rows < 16) rows = 16;
+            if (rows > BATCH_MAX_TILE_ROWS) rows = BATCH_MAX_TILE_ROWS;
+            for (int64_t r = 0; r < h[i].m; r += rows) {
+                tiles.push_back((int32_t)i);
+                tiles.push_back((int32_t)r);
+                tiles.push_back((int32_t)(r + rows < h[i].m ? r + rows : h[i].m));
+            }
         }
     }
+    segs.start[BATCH_SEGS] = (int64_t)tiles.size() / 3;
 }
 
 void batch_plan_release(gsa_plan_s* plan) {
@@ -381,19 +500,38 @@ int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
     }
     hipStream_t s = as_stream(stream);
     std::vector<int32_t> tiles;
-    make_tiles(host_desc, count, rows_per_tile, tiles);
     auto* bp = new BatchPlan();
     bp->vtype = vtype; bp->itype = itype; bp->count = count;
-    bp->ntiles = (int64_t)tiles.size() / 3;
     int dev = 0, cus = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     bp->grid = cus * 8;
+    std::vector<int64_t> nnz((size_t)count);
+    int64_t* d_nnz = nullptr;
     hipError_t e = hipMalloc(&bp->d_desc, sizeof(gsa_batch_desc) * (count > 0 ? count : 1));
-    if (e == hipSuccess) e = hipMalloc(&bp->d_tiles, sizeof(int32_t) * (tiles.size() + 3));
     if (e == hipSuccess && count > 0)
         e = hipMemcpyAsync(bp->d_desc, host_desc, sizeof(gsa_batch_desc) * count,
                            hipMemcpyHostToDevice, s);
+    if (e == hipSuccess && count > 0) e = hipMalloc(&d_nnz, sizeof(int64_t) * count);
+    if (e == hipSuccess && count > 0) {
+        if (itype == GSA_I32)
+            hipLaunchKernelGGL(k_inst_nnz<int32_t>, dim3((unsigned)cdiv(count, 256)), dim3(256), 0, s,
+                               bp->d_desc, count, d_nnz);
+        else
+            hipLaunchKernelGGL(k_inst_nnz<int64_t>, dim3((unsigned)cdiv(count, 256)), dim3(256), 0, s,
+                               bp->d_desc, count, d_nnz);
+        e = hipGetLastError();
+        if (e == hipSuccess)
+            e = hipMemcpyAsync(nnz.data(), d_nnz, sizeof(int64_t) * count, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+    }
+    if (d_nnz) (void)hipFree(d_nnz);
+    if (e == hipSuccess) {
+        make_tiles(host_desc, nnz.data(), count, rows_per_tile, vtype == GSA_F64 ? 8 : 4, tiles,
+                   bp->segs);
+        bp->ntiles = (int64_t)tiles.size() / 3;
+        e = hipMalloc(&bp->d_tiles, sizeof(int32_t) * (tiles.size() + 3));
+    }
     if (e == hipSuccess && !tiles.empty())
         e = hipMemcpyAsync(bp->d_tiles, tiles.data(), sizeof(int32_t) * tiles.size(),
                            hipMemcpyHostToDevice, s);
@@ -414,9 +552,9 @@ int gsa_batch_spmm(void* stream, gsa_plan plan, float alpha, float beta) {
     GSA_CHECK(plan && plan->kind == 1 && plan->batch, GSA_ERR_INVALID, "gsa_batch_spmm: not a batch plan");
     auto* bp = (BatchPlan*)plan->batch;
     hipStream_t s = as_stream(stream);
-    Scratch counter(s);  // this launch's tile counter
-    GSA_TRY(counter.alloc(sizeof(int)));
-    return launch_batch(s, bp->vtype, bp->itype, bp->d_desc, bp->d_tiles, bp->ntiles,
+    Scratch counter(s);  // this launch's tile counters, one per segment
+    GSA_TRY(counter.alloc(sizeof(int) * BATCH_SEGS));
+    return launch_batch(s, bp->vtype, bp->itype, bp->d_desc, bp->d_tiles, bp->segs,
                         counter.as<int>(), alpha, beta, bp->grid);
 }
 
@@ -425,7 +563,11 @@ int gsa_batch_spmm_tables(void* stream, int vtype, int itype, const gsa_batch_de
                           float beta, int grid) {
     GSA_CHECK(ntiles >= 0 && d_counter && (ntiles == 0 || (d_desc && d_tiles)), GSA_ERR_INVALID,
               "gsa_batch_spmm_tables: bad arguments");
-    return launch_batch(as_stream(stream), vtype, itype, d_desc, d_tiles, ntiles, d_counter, alpha,
+    BatchSegs one{};
+    one.n = 1;
+    one.start[0] = 0;
+    one.start[1] = ntiles;
+    return launch_batch(as_stream(stream), vtype, itype, d_desc, d_tiles, one, d_counter, alpha,
                         beta, grid);
 }
 

@@ -500,7 +500,11 @@ typedef struct {
 #define GSA_BATCH_C_ROWMAJOR 2  /* C[i,j] at C[i*ldc + j]                        */
 
 /* Tiles: instance i is cut into row tiles of max(16, min(1024,
- * rows_per_tile / next_pow2(k_i))) rows.  vtype GSA_BF16 needs row_scale. */
+ * rows_per_tile / lanes_i)) rows, lanes_i = next_pow2(k_i / w_i), w_i the
+ * elements one lane loads per stored entry (up to 16 bytes of a row-major B,
+ * else 1).  The instances are dealt to 8 tile lists, one per XCD, balanced by
+ * their stored entries and k (synchronises once).  vtype GSA_BF16 needs
+ * row_scale. */
 int gsa_batch_plan_create(void* stream, int vtype, int itype, int64_t count,
                           const gsa_batch_desc* host_desc, int rows_per_tile,
                           gsa_plan* plan);
