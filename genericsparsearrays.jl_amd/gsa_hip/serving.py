@@ -58,7 +58,7 @@ class MatrixRegistry:
         self.max_k = max_k
         self.big_rows = big_rows   # matrices with at least this many rows use the dedicated kernels
         self.big = []              # per matrix: its contiguous input column 0, or None
-        self._slot_plans = {}      # (matrix, slot) -> plan of a big matrix owned by one slot
+        self._plans = {}           # matrix -> plan of a big matrix (read-only, shared by slots)
         self.mats = []
         self.inputs = []     # per matrix: n x max_k column-major Float32 input block
         self.vtype = None
@@ -101,17 +101,17 @@ class MatrixRegistry:
         self._desc.append(d[0])
         return mid
 
-    def slot_plan(self, mid: int, slot: int):
-        """Plan handle of big matrix `mid` owned by serving slot `slot`, or None
-        for an empty matrix.  A plan carries the partial sums of the call in
-        flight, so calls that can overlap (different slots) use different plans."""
+    def plan(self, mid: int):
+        """Plan handle of big matrix `mid`, or None for an empty matrix.  A plan
+        is read-only once built (each call takes its partial sums from scratch
+        on its own stream), so one plan serves every slot, overlapping or not."""
         A = self.mats[mid]
         if A.nnz() == 0:
             return None
-        pc = self._slot_plans.get((mid, slot))
+        pc = self._plans.get(mid)
         if pc is None:
             pc = _PlanCache()
-            self._slot_plans[(mid, slot)] = pc
+            self._plans[mid] = pc
         return pc.get(A.rowptr, A.shape[0], A.nnz(), A.colval, A.n)
 
     def desc_table(self) -> np.ndarray:
@@ -184,8 +184,9 @@ class Server:
         self._m, self._n = m, n
         self.in_ptr = self.desc["B"].copy()
         self.is_big = np.array([b is not None for b in registry.big], dtype=bool)
-        # one plan per (big matrix, slot), each run once so its partial-sum
-        # buffer exists before the loop (no allocation while serving)
+        # every big matrix run once on every slot's stream, so its plan exists and
+        # each stream's scratch cache holds the blocks its calls take (no
+        # allocation while serving)
         for si, slot in enumerate(self.slots):
             slot.index = si
             for mid in np.flatnonzero(self.is_big).tolist():
@@ -205,7 +206,7 @@ class Server:
         m = A.shape[0]
         if m > slot.arena.numel():
             raise ArgumentError("a request's result is larger than a slot's share of the budget")
-        plan = reg.slot_plan(mid, slot.index)
+        plan = reg.plan(mid)
         st = ctypes.c_void_p(slot.stream.cuda_stream)
         if k == 1:
             bptr, ldb, opB = reg.big[mid].data_ptr(), max(A.n, 1), L.GSA_OP_N
