@@ -217,7 +217,27 @@ def bench_structural(H, dev):
     t_host = time.perf_counter() - t0
     out["ingest_csr_from_scipy_csc"] = {"ms": round(t_gpu * 1e3, 1),
                                         "host_transpose_ms": round(t_host * 1e3, 1)}
-    del S
+    # host egress: SparseMatrixCSC(::GenericSparseMatrixCSR) of the same matrix on the
+    # device (transposed there, then copied), against the reference's route: the CSR
+    # arrays collected as they are, then transposed on the host (scipy's tocsc)
+    # (the first call also allocates the pinned host buffers the copies go through;
+    # later calls of like sizes reuse them from torch's caching host allocator)
+    Ad = G.convert.csr_from_scipy(S.tocsr(), device=dev)
+    G.convert.csr_to_csc(Ad)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    G.SparseMatrixCSC(Ad)
+    t_first = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    G.SparseMatrixCSC(Ad)
+    t_gpu = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    G.convert.to_scipy(Ad).tocsc()
+    t_ref = time.perf_counter() - t0
+    out["egress_sparsematrixcsc_of_csr"] = {"ms": round(t_gpu * 1e3, 1),
+                                            "first_call_ms": round(t_first * 1e3, 1),
+                                            "collect_then_host_transpose_ms": round(t_ref * 1e3, 1)}
+    del S, Ad
     L = synth.laplacian2d(1000)
     Lp = synth.drop_entries(L, 0.1, seed=3)
     R = synth.fixed_row_csr(200_000, 200_000, 8, seed=3)

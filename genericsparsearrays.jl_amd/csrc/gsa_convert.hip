@@ -296,11 +296,64 @@ int build_row_view(hipStream_t s, int itype, int64_t m, int64_t nnz, const void*
     });
 }
 
+// out[i] = in[i] + delta in the output index type (host ingest and egress:
+// zero-based host arrays to one-based device arrays and back, widened or
+// narrowed between Int32 and Int64).  A value outside the output type's range
+// raises *bad (left untouched otherwise).  in and out may be the same array.
+template <typename A, typename B>
+__global__ void k_index_shift(int64_t n, const A* in, B* out, int64_t delta, int* bad) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+        const int64_t v = (int64_t)in[i] + delta;
+        if constexpr (sizeof(B) < sizeof(int64_t))
+            if (v < (int64_t)INT32_MIN || v > (int64_t)INT32_MAX) *bad = 1;
+        out[i] = (B)v;
+    }
+}
+
 }  // namespace gsa
 
 using namespace gsa;
 
 extern "C" {
+
+int gsa_index_shift(void* stream, int itype_in, int itype_out, int64_t n, const void* in,
+                    void* out, int64_t delta) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(n >= 0 && (n == 0 || (in && out)), GSA_ERR_INVALID, "gsa_index_shift: bad arguments");
+    GSA_CHECK((itype_in == GSA_I32 || itype_in == GSA_I64) &&
+              (itype_out == GSA_I32 || itype_out == GSA_I64), GSA_ERR_UNSUPPORTED,
+              "gsa_index_shift: bad index types %d, %d", itype_in, itype_out);
+    if (n == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    const unsigned grid = grid_for(n, 256);
+    if (itype_out == GSA_I64) {
+        if (itype_in == GSA_I32)
+            hipLaunchKernelGGL((k_index_shift<int32_t, int64_t>), dim3(grid), dim3(256), 0, s, n,
+                               (const int32_t*)in, (int64_t*)out, delta, (int*)nullptr);
+        else
+            hipLaunchKernelGGL((k_index_shift<int64_t, int64_t>), dim3(grid), dim3(256), 0, s, n,
+                               (const int64_t*)in, (int64_t*)out, delta, (int*)nullptr);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    }
+    // Int32 output: range-checked, one read-back
+    Scratch flag(s);
+    GSA_TRY(flag.alloc(sizeof(int)));
+    GSA_TRY(zero_fill(s, flag.p, sizeof(int)));
+    if (itype_in == GSA_I32)
+        hipLaunchKernelGGL((k_index_shift<int32_t, int32_t>), dim3(grid), dim3(256), 0, s, n,
+                           (const int32_t*)in, (int32_t*)out, delta, flag.as<int>());
+    else
+        hipLaunchKernelGGL((k_index_shift<int64_t, int32_t>), dim3(grid), dim3(256), 0, s, n,
+                           (const int64_t*)in, (int32_t*)out, delta, flag.as<int>());
+    GSA_HIP(hipGetLastError());
+    int bad = 0;
+    GSA_HIP(hipMemcpyAsync(&bad, flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    GSA_HIP(hipStreamSynchronize(s));
+    GSA_CHECK(!bad, GSA_ERR_INVALID, "gsa_index_shift: an index does not fit Int32");
+    return GSA_OK;
+}
 
 int gsa_csr_to_coo(void* stream, int vtype, int itype, int64_t m, int64_t nnz, const void* rowptr,
                    const void* colval, const void* nzval, void* rowind, void* colind,

@@ -8,6 +8,8 @@ Operators:  mul_ (mul!), A @ x / A * x, A * B (SpGEMM), A + B, A - B, kron,
             dropzeros, dropzeros_, conversions through the constructors,
             tr, sum, dot(x, A, y), issymmetric, ishermitian, A ± λI (UniformScaling, I),
             kron with a Diagonal.
+Host interop: convert.csr_from_scipy / csc_from_scipy / coo_from_scipy (ingest),
+            SparseMatrixCSC(A) and convert.to_scipy(A) (egress).
 Every operator runs in libgsa_hip.so; there is no CPU fallback.
 """
 from ._lib import ArgumentError, DimensionMismatch, GSAError, load as load_library
@@ -20,6 +22,7 @@ from .ops import dot as _dot3, sum_ as _sum_matrix
 from .vector import GenericSparseVector, norm, normalize, normalize_
 from . import vector as _vector
 from . import convert
+from .convert import sparse_matrix_csc as SparseMatrixCSC
 
 
 def dot(*args):
@@ -48,5 +51,5 @@ __all__ = [
     "mul_", "times", "add", "kron", "dropzeros", "dropzeros_", "spgemm", "scale", "negate",
     "convert", "UniformScaling", "I", "Diagonal", "tr", "sum_", "dot", "issymmetric",
     "ishermitian", "add_uniform_scaling", "GenericSparseVector", "norm", "normalize",
-    "normalize_",
+    "normalize_", "SparseMatrixCSC",
 ]

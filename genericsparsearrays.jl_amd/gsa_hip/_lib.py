@@ -88,6 +88,7 @@ _SIGS = {
                           _vp, _i64, _i32, _vp, _i64, _i32, _f32, _f32, _vp],
     "gsa_spmm_csr_ex": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64,
                         _vp, _i64, _i32, _vp, _i64, _i32, _f64, _f64, _vp],
+    "gsa_index_shift": [_vp, _i32, _i32, _i64, _vp, _vp, _i64],
     "gsa_csr_to_coo": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "gsa_coo_to_csr": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "gsa_csr_to_csc": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],

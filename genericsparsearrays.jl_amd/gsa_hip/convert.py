@@ -29,20 +29,62 @@ def _dev(device):
     return torch.device(device if device is not None else "cuda")
 
 
-def _t(a, dtype, device):
-    return torch.as_tensor(np.ascontiguousarray(a)).to(dtype=dtype, device=device)
-
-
 def _conj(v: torch.Tensor, conj: bool) -> torch.Tensor:
     return v.conj().resolve_conj() if conj and v.is_complex() else v
 
 
+def _index_up(a, index_dtype, dev):
+    """A zero-based host index array as a one-based device vector of index_dtype:
+    copied as it is (Int32 or Int64), then shifted and widened on the device
+    (gsa_index_shift), so the host makes no shifted copy."""
+    a = np.ascontiguousarray(a)
+    if a.dtype not in (np.int32, np.int64):
+        a = a.astype(np.int64)
+    raw = torch.from_numpy(a).to(dev)
+    out = raw if raw.dtype == index_dtype else torch.empty(a.size, dtype=index_dtype, device=dev)
+    L.call("gsa_index_shift", stream_of(dev), itype_of(raw), itype_of(out), a.size, ptr(raw),
+           ptr(out), 1)
+    return out
+
+
+def _down(t: torch.Tensor) -> torch.Tensor:
+    """A device tensor copied into pinned host memory (torch's caching host
+    allocator, so repeated egress of like sizes reuses it) on the current stream;
+    the caller synchronises before reading it."""
+    h = torch.empty(t.shape, dtype=t.dtype, pin_memory=True)
+    h.copy_(t, non_blocking=True)
+    return h
+
+
+def _index_down(t: torch.Tensor, small: bool) -> torch.Tensor:
+    """A one-based device index vector as a zero-based host vector, Int32 when
+    `small` (the width scipy keeps for matrices whose sizes fit it, so it makes no
+    narrowing copy of its own): shifted and narrowed on the device, then copied."""
+    out = torch.empty(t.numel(), dtype=torch.int32 if small else torch.int64, device=t.device)
+    L.call("gsa_index_shift", stream_of(t.device), itype_of(t), itype_of(out), t.numel(), ptr(t),
+           ptr(out), -1)
+    return _down(out)
+
+
+def _host(dev, *ts):
+    """The host tensors' numpy arrays, once the copies into them are done."""
+    torch.cuda.current_stream(dev).synchronize()
+    return tuple(t.numpy() for t in ts)
+
+
+def _small(A) -> bool:
+    return max(A.m, A.n, A.nnz() + 1) < 2**31
+
+
+def _values_up(v, dtype, dev):
+    vals = v if dtype is None else v.astype(dtype)
+    return torch.from_numpy(np.ascontiguousarray(vals)).to(dev)
+
+
 def _upload(A, dev, index_dtype, dtype):
     """The compressed arrays of a scipy CSR or CSC matrix as one-based device vectors."""
-    vals = A.data if dtype is None else A.data.astype(dtype)
-    return (_t(A.indptr.astype(np.int64) + 1, index_dtype, dev),
-            _t(A.indices.astype(np.int64) + 1, index_dtype, dev),
-            torch.as_tensor(np.ascontiguousarray(vals)).to(dev))
+    return (_index_up(A.indptr, index_dtype, dev), _index_up(A.indices, index_dtype, dev),
+            _values_up(A.data, dtype, dev))
 
 
 # ------------------------------------------------------------------ to CSR --
@@ -61,12 +103,7 @@ def csr_from_scipy(A, device=None, index_dtype=torch.int64, dtype=None):
     B = A.tocsr(copy=True)
     B.sum_duplicates()
     B.sort_indices()
-    dev = _dev(device)
-    vals = B.data if dtype is None else B.data.astype(dtype)
-    return GenericSparseMatrixCSR(B.shape[0], B.shape[1],
-                                  _t(B.indptr.astype(np.int64) + 1, index_dtype, dev),
-                                  _t(B.indices.astype(np.int64) + 1, index_dtype, dev),
-                                  torch.as_tensor(np.ascontiguousarray(vals)).to(dev),
+    return GenericSparseMatrixCSR(B.shape[0], B.shape[1], *_upload(B, dev, index_dtype, dtype),
                                   copy=False)
 
 
@@ -141,13 +178,8 @@ def csc_from_scipy(A, device=None, index_dtype=torch.int64, dtype=None):
     B = A.tocsc(copy=True)
     B.sum_duplicates()
     B.sort_indices()
-    dev = _dev(device)
-    vals = B.data if dtype is None else B.data.astype(dtype)
     return GenericSparseMatrixCSC(B.shape[0], B.shape[1],
-                                  _t(B.indptr.astype(np.int64) + 1, index_dtype, dev),
-                                  _t(B.indices.astype(np.int64) + 1, index_dtype, dev),
-                                  torch.as_tensor(np.ascontiguousarray(vals)).to(dev),
-                                  copy=False)
+                                  *_upload(B, _dev(device), index_dtype, dtype), copy=False)
 
 
 def coo_to_csc(A: GenericSparseMatrixCOO) -> GenericSparseMatrixCSC:
@@ -216,12 +248,9 @@ def coo_from_scipy(A, device=None, index_dtype=torch.int64, dtype=None):
     C = B.tocoo()
     order = np.lexsort((C.row, C.col))
     dev = _dev(device)
-    vals = C.data[order] if dtype is None else C.data[order].astype(dtype)
-    return GenericSparseMatrixCOO(B.shape[0], B.shape[1],
-                                  _t(C.row[order].astype(np.int64) + 1, index_dtype, dev),
-                                  _t(C.col[order].astype(np.int64) + 1, index_dtype, dev),
-                                  torch.as_tensor(np.ascontiguousarray(vals)).to(dev),
-                                  copy=False)
+    return GenericSparseMatrixCOO(B.shape[0], B.shape[1], _index_up(C.row[order], index_dtype, dev),
+                                  _index_up(C.col[order], index_dtype, dev),
+                                  _values_up(C.data[order], dtype, dev), copy=False)
 
 
 def to_coo(src, device=None, index_dtype=torch.int64) -> GenericSparseMatrixCOO:
@@ -240,3 +269,63 @@ def to_coo(src, device=None, index_dtype=torch.int64) -> GenericSparseMatrixCOO:
     if _is_scipy(src):
         return coo_from_scipy(src, device=device, index_dtype=index_dtype)
     raise ArgumentError(f"cannot convert {type(src).__name__} to GenericSparseMatrixCOO")
+
+
+# ------------------------------------------------------------ host egress --
+def to_scipy(A):
+    """The host form of a device matrix as it is stored: CSR -> scipy csr_matrix,
+    CSC -> csc_matrix, COO -> coo_matrix (entries in storage order, duplicates
+    kept), index arrays zero-based (shifted on the device, then copied), values
+    copied.  The collect() of the reference's conversions.jl:10-79 before any
+    host-side reordering."""
+    import scipy.sparse as sp
+    if isinstance(A, GenericSparseMatrixCSR):
+        k = _small(A)
+        v, i, p = _host(A.device, _down(A.nzval), _index_down(A.colval, k),
+                        _index_down(A.rowptr, k))
+        return sp.csr_matrix((v, i, p), shape=(A.m, A.n), copy=False)
+    if isinstance(A, GenericSparseMatrixCSC):
+        k = _small(A)
+        v, i, p = _host(A.device, _down(A.nzval), _index_down(A.rowval, k),
+                        _index_down(A.colptr, k))
+        return sp.csc_matrix((v, i, p), shape=(A.m, A.n), copy=False)
+    if isinstance(A, GenericSparseMatrixCOO):
+        k = _small(A)
+        v, r, c = _host(A.device, _down(A.nzval), _index_down(A.rowind, k),
+                        _index_down(A.colind, k))
+        return sp.coo_matrix((v, (r, c)), shape=(A.m, A.n), copy=False)
+    raise ArgumentError(f"cannot convert {type(A).__name__} to a host sparse matrix")
+
+
+def sparse_matrix_csc(src):
+    """SparseMatrixCSC(A) (conversions.jl:10-79) as a scipy csc_matrix.
+
+    CSC: its arrays.  CSR: transposed on the device (the stable column sort of
+    GenericSparseMatrixCSC(::CSR)), then copied; the reference collects the CSR
+    arrays and transposes them on the host, with the same result.
+    transpose(CSR) / adjoint(CSR): the CSR arrays are the CSC arrays of the
+    wrapper (values conjugated for the adjoint), as in the reference.  COO (and
+    its wrappers): sparse(rowind, colind, nzval, m, n), duplicates summed
+    (scipy's tocsc, which adds them in storage order as Julia's sparse does)."""
+    import scipy.sparse as sp
+    if isinstance(src, GenericSparseMatrixCSC):
+        return to_scipy(src)
+    if isinstance(src, GenericSparseMatrixCSR):
+        return to_scipy(csr_to_csc(src))
+    if isinstance(src, GenericSparseMatrixCOO):
+        c = to_scipy(src).tocsc()
+        c.sort_indices()
+        return c
+    if isinstance(src, Transpose):
+        p = src.parent
+        if isinstance(p, GenericSparseMatrixCSR):
+            k = _small(p)
+            v, i, q = _host(p.device, _down(_conj(p.nzval, src.conj)), _index_down(p.colval, k),
+                            _index_down(p.rowptr, k))
+            return sp.csc_matrix((v, i, q), shape=(p.n, p.m), copy=False)
+        if isinstance(p, GenericSparseMatrixCSC):  # conversions.jl:17-22: through a CSR
+            return sparse_matrix_csc(to_csr(src))
+        if isinstance(p, GenericSparseMatrixCOO):
+            return sparse_matrix_csc(to_coo(src))
+    raise ArgumentError(f"cannot convert {type(src).__name__} to SparseMatrixCSC")
+

@@ -265,6 +265,18 @@ int gsa_spmm_csr_bf16(void* stream, int itype, int64_t m, int64_t n, int64_t k,
                       float alpha, float beta, gsa_plan plan);
 
 /* ========================== format conversion =============================== */
+/* out[i] = in[i] + delta, i < n, from index type itype_in to itype_out (Int32
+ * or Int64); an Int32 output is range-checked (GSA_ERR_INVALID if a value does
+ * not fit; synchronises once); in == out is allowed when the types match.
+ * Host ingest and egress: zero-based host index
+ * arrays copied as they are, then made one-based on the device (and back), so
+ * the host neither shifts nor widens them.  Replaces the index handling of
+ * GenericSparseMatrixCSR/CSC(::SparseMatrixCSC) and SparseMatrixCSC(::Generic…),
+ * src/conversions/conversions.jl:7-79, where the arrays are one-based on both
+ * sides and the host transposes. */
+int gsa_index_shift(void* stream, int itype_in, int itype_out, int64_t n, const void* in,
+                    void* out, int64_t delta);
+
 /* CSR -> COO (row-major entry order).  Replaces GenericSparseMatrixCOO(::CSR),
  * src/conversions/conversions.jl:264-280, kernel_csr_to_coo!
  * src/conversions/conversion_kernels.jl:20-35.  For CSC -> COO

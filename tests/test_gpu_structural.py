@@ -448,6 +448,88 @@ def test_scipy_ingest_transposes_on_the_device(dev, itype):
     same(csc_tuple(Acr), O.csr_to_csc(250, 180, rp, cv, nzr))
 
 
+def test_index_shift_widens_narrows_and_checks(dev):
+    """gsa_index_shift: zero-based host arrays made one-based on the device and
+    back, Int32 <-> Int64, in place when the types match; an Int32 output that
+    cannot hold a value is an ArgumentError before anything is returned."""
+    from gsa_hip import _lib as L
+    from gsa_hip.matrix import itype_of, ptr, stream_of
+    rng = np.random.default_rng(5)
+    a = rng.integers(0, 2**31 - 2, 100_003).astype(np.int64)
+
+    def shift(t, out_dtype, delta, out=None):
+        out = torch.empty(t.numel(), dtype=out_dtype, device=dev) if out is None else out
+        L.call("gsa_index_shift", stream_of(dev), itype_of(t), itype_of(out), t.numel(), ptr(t),
+               ptr(out), delta)
+        return out
+
+    t64 = torch.as_tensor(a, device=dev)
+    t32 = shift(t64, torch.int32, 1)                 # narrowed, in range
+    assert np.array_equal(host(t32), (a + 1).astype(np.int32))
+    w = shift(t32, torch.int64, -1)                  # widened back
+    assert np.array_equal(host(w), a)
+    shift(w, torch.int64, 7, out=w)                  # in place
+    assert np.array_equal(host(w), a + 7)
+    big = torch.as_tensor(np.array([1, 2**31 - 1], dtype=np.int64), device=dev)
+    with pytest.raises(G.ArgumentError):
+        shift(big, torch.int32, 1)                   # 2^31 does not fit Int32
+    shift(torch.empty(0, dtype=torch.int64, device=dev), torch.int32, 1)  # empty: no-op
+
+
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+@pytest.mark.parametrize("vdtype", [np.float64, np.float32, np.complex128, np.int32])
+def test_host_egress_and_ingest_round_trip(dev, itype, vdtype):
+    """SparseMatrixCSC(A) (conversions.jl:10-79) for every format and wrapper
+    equals scipy's canonical CSC of the same matrix bit for bit; to_scipy gives
+    back the stored arrays; ingest from scipy CSR/CSC/COO reproduces them."""
+    import scipy.sparse as sp
+    rng = np.random.default_rng(17)
+    m, n = 700, 450
+    S = sp.random(m, n, density=0.02, random_state=rng, format="csr")
+    if np.issubdtype(vdtype, np.complexfloating):
+        S.data = (S.data + 1j * rng.standard_normal(S.nnz)).astype(vdtype)
+    elif np.issubdtype(vdtype, np.integer):
+        S.data = rng.integers(-9, 10, S.nnz).astype(vdtype)
+    else:
+        S.data = S.data.astype(vdtype)
+    S.sort_indices()
+    ref = S.tocsc()
+    ref.sort_indices()
+
+    def same_csc(C, R):
+        assert C.shape == R.shape and C.format == "csc"
+        assert np.array_equal(C.indptr, R.indptr) and np.array_equal(C.indices, R.indices)
+        assert np.array_equal(C.data, R.data) and C.data.dtype == R.data.dtype
+
+    A = G.convert.csr_from_scipy(S, device=dev, index_dtype=itype)
+    assert A.rowptr.dtype == itype and A.colval.dtype == itype
+    back = G.convert.to_scipy(A)
+    assert np.array_equal(back.indptr, S.indptr) and np.array_equal(back.indices, S.indices)
+    assert np.array_equal(back.data, S.data)
+    same_csc(G.SparseMatrixCSC(A), ref)                               # CSR: device transpose
+    Ac = G.convert.csc_from_scipy(ref, device=dev, index_dtype=itype)
+    same_csc(G.SparseMatrixCSC(Ac), ref)                              # CSC: its arrays
+    Ao = G.convert.coo_from_scipy(S, device=dev, index_dtype=itype)
+    same_csc(G.SparseMatrixCSC(Ao), ref)                              # COO: sparse(...)
+    Tt = S.T.tocsc()
+    Tt.sort_indices()
+    same_csc(G.SparseMatrixCSC(G.transpose(A)), Tt)
+    Ta = S.conj().T.tocsc()
+    Ta.sort_indices()
+    same_csc(G.SparseMatrixCSC(G.adjoint(A)), Ta)
+    same_csc(G.SparseMatrixCSC(G.transpose(Ac)), Tt)
+    same_csc(G.SparseMatrixCSC(G.adjoint(Ao)), Ta)
+    # COO with duplicates: sparse() adds them (in storage order)
+    r = np.array([0, 2, 0, 1, 0]); c = np.array([1, 0, 1, 1, 1])
+    v = np.array([1, 2, 3, 4, 5]).astype(vdtype)
+    D = G.GenericSparseMatrixCOO(3, 2, torch.as_tensor(r + 1).to(dev, itype),
+                                 torch.as_tensor(c + 1).to(dev, itype), torch.as_tensor(v).to(dev))
+    exp = sp.coo_matrix((v, (r, c)), shape=(3, 2)).tocsc()
+    exp.sort_indices()
+    same_csc(G.SparseMatrixCSC(D), exp)
+    assert np.array_equal(G.convert.to_scipy(D).row, r)               # storage order kept
+
+
 @pytest.mark.parametrize("itype", [torch.int64, torch.int32])
 def test_coo_add_merge_and_sort_paths(dev, itype):
     """COO + COO (matrix_coo.jl:300-384): operands already in column-major order take
