@@ -214,7 +214,10 @@ int counts_to_ptr(hipStream_t s, I* ptr, int64_t m, int64_t* total) {
         if (total && m > 0) {
             GSA_TRY(wide.alloc(sizeof(unsigned long long)));
             GSA_TRY(zero_fill(s, wide.p, sizeof(unsigned long long)));
-            hipLaunchKernelGGL(k_sum_counts32, dim3(grid_for(m, 256)), dim3(256), 0, s,
+            // at most 256 workgroups: each adds its part with one device-wide
+            // atomic on the same address, and those serialise
+            hipLaunchKernelGGL(k_sum_counts32, dim3((unsigned)(cdiv(m, 256) < 256 ? cdiv(m, 256) : 256)),
+                               dim3(256), 0, s,
                                (const int32_t*)(ptr + 1), m, wide.as<unsigned long long>());
             GSA_HIP(hipGetLastError());
         }

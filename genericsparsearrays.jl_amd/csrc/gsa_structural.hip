@@ -1086,23 +1086,52 @@ __global__ void k_sg_count_flags(int64_t m, const I* __restrict__ pA, const I* _
     }
     __syncthreads();
     const int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+    int64_t P = 0;
     if (r < m) {
-        int64_t P = 0;
         for (int64_t a = (int64_t)pA[r] - 1; a < (int64_t)pA[r + 1] - 1; ++a) {
             const int64_t k = dclamp((int64_t)cA[a] - 1, 0, kdim - 1);
             P += (int64_t)pB[k + 1] - pB[k];
         }
         cnt[r + 1] = P;
         f[r] = P > SG_WAVE_P;
-        atomicMax(&bmax, (unsigned long long)P);
-        const unsigned cls = P <= 0 ? 0u : (P <= 16 ? 1u : (P <= 32 ? 2u : (P <= 64 ? 4u : 0u)));
-        if (cls) atomicOr(&bcls, cls);
+    }
+    // the wavefront's largest count and size classes first (every lane takes
+    // part), then one LDS atomic per wavefront: 64 lanes contending for one LDS
+    // word serialise
+    uint64_t pm = (uint64_t)P;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint64_t o = lane_xor(pm, off);
+        pm = o > pm ? o : pm;
+    }
+    const unsigned cls = P <= 0 ? 0u : (P <= 16 ? 1u : (P <= 32 ? 2u : (P <= 64 ? 4u : 0u)));
+    const unsigned wcls = (__ballot(cls & 1u) ? 1u : 0u) | (__ballot(cls & 2u) ? 2u : 0u) |
+                          (__ballot(cls & 4u) ? 4u : 0u);
+    if ((threadIdx.x & 63) == 0) {
+        atomicMax(&bmax, (unsigned long long)pm);
+        if (wcls) atomicOr(&bcls, wcls);
     }
     __syncthreads();
+    // device-wide atomics on one address serialise across the whole grid: a block
+    // adds its figures only when they raise what is already there (a stale read
+    // is never above the true value, so it can only cost an unneeded atomic)
     if (threadIdx.x == 0) {
-        atomicMax(out, bmax);
-        if (bcls) atomicOr(out + 1, (unsigned long long)bcls);
+        if (bmax > *(volatile unsigned long long*)out) atomicMax(out, bmax);
+        if (bcls & ~*(volatile unsigned long long*)(out + 1))
+            atomicOr(out + 1, (unsigned long long)bcls);
     }
+}
+
+// The four figures the begin phase reads back, side by side for one copy:
+// largest count, long rows, size classes, product-pointer end.
+__global__ void k_sg_figures(const unsigned long long* __restrict__ mx,
+                             const int64_t* __restrict__ long_incl, int64_t m,
+                             const int64_t* __restrict__ pp, int64_t* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    out[0] = (int64_t)mx[0];
+    out[1] = m > 0 ? long_incl[m - 1] : 0;
+    out[2] = (int64_t)mx[1];
+    out[3] = pp[m];
 }
 
 __global__ void k_sg_list(int64_t m, const int64_t* __restrict__ pp,
@@ -1619,33 +1648,31 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
             *ctx = c;
             return (int)GSA_OK;
         }
-        Scratch pptr(s);
-        if ((r = pptr.alloc(sizeof(int64_t) * (m + 1))) != GSA_OK) return fail(r);
+        // the product pointer: kept in the context for the numeric phase or finish
+        if ((r = ws_alloc(&c->buf[3], sizeof(int64_t) * (m + 1), s)) != GSA_OK) return fail(r);
+        int64_t* pp = (int64_t*)c->buf[3];
         // per-row products with the long-row flags, largest count and size classes,
         // both scans, then one read-back of the four figures
-        Scratch fl(s), mx(s);
+        Scratch fl(s), mx(s), fig(s);
         if ((r = fl.alloc(sizeof(int64_t) * (m > 0 ? m : 1))) != GSA_OK) return fail(r);
         if ((r = mx.alloc(2 * sizeof(unsigned long long))) != GSA_OK) return fail(r);
+        if ((r = fig.alloc(4 * sizeof(int64_t))) != GSA_OK) return fail(r);
         if ((r = zero_fill(s, mx.p, 2 * sizeof(unsigned long long))) != GSA_OK) return fail(r);
         if (m > 0)
             hipLaunchKernelGGL((k_sg_count_flags<I>), dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s,
                                m, (const I*)rowptrA, (const I*)colA, (const I*)rowptrB, kdim,
-                               pptr.as<int64_t>(), fl.as<int64_t>(), (unsigned long long*)mx.p);
-        if ((r = counts_to_ptr<int64_t>(s, pptr.as<int64_t>(), m, nullptr)) != GSA_OK) return fail(r);
+                               pp, fl.as<int64_t>(), (unsigned long long*)mx.p);
+        if ((r = counts_to_ptr<int64_t>(s, pp, m, nullptr)) != GSA_OK) return fail(r);
         if ((r = inclusive_scan<int64_t>(s, fl.as<int64_t>(), fl.as<int64_t>(), m, 0)) != GSA_OK)
             return fail(r);
         int64_t h[4] = {0, 0, 0, 1};  // largest count, long rows, classes, pointer end
         {
-            hipError_t e = hipMemcpyAsync(&h[0], mx.p, sizeof(int64_t), hipMemcpyDeviceToHost, s);
+            hipLaunchKernelGGL(k_sg_figures, dim3(1), dim3(64), 0, s,
+                               (const unsigned long long*)mx.p, (const int64_t*)fl.as<int64_t>(), m,
+                               (const int64_t*)pp, fig.as<int64_t>());
+            hipError_t e = hipGetLastError();
             if (e == hipSuccess)
-                e = hipMemcpyAsync(&h[2], (const int64_t*)mx.p + 1, sizeof(int64_t),
-                                   hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess && m > 0)
-                e = hipMemcpyAsync(&h[1], fl.as<int64_t>() + (m - 1), sizeof(int64_t),
-                                   hipMemcpyDeviceToHost, s);
-            if (e == hipSuccess)
-                e = hipMemcpyAsync(&h[3], pptr.as<int64_t>() + m, sizeof(int64_t),
-                                   hipMemcpyDeviceToHost, s);
+                e = hipMemcpyAsync(h, fig.p, sizeof h, hipMemcpyDeviceToHost, s);
             if (e == hipSuccess) e = hipStreamSynchronize(s);
             if (e != hipSuccess) return fail(hip_fail(e, "gsa_spgemm_csr_begin"));
         }
@@ -1658,16 +1685,12 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
             const int64_t maxP = h[0], nbig = h[1];
             const unsigned classes = (unsigned)h[2];
             if (maxP <= SG_BLOCK_P && n < (int64_t)1 << 40) {
-                // keep the product pointer and the long-row list for the numeric phase
-                if ((r = ws_alloc(&c->buf[3], sizeof(int64_t) * (m + 1), s)) != GSA_OK) return fail(r);
-                e = hipMemcpyAsync(c->buf[3], pptr.p, sizeof(int64_t) * (m + 1),
-                                   hipMemcpyDeviceToDevice, s);
-                if (e != hipSuccess) return fail(hip_fail(e, "gsa_spgemm_csr_begin"));
+                // keep the long-row list for the numeric phase
                 if (nbig > 0) {
                     if ((r = ws_alloc(&c->buf[4], sizeof(int64_t) * nbig, s)) != GSA_OK) return fail(r);
                     hipLaunchKernelGGL(k_sg_list, dim3((unsigned)cdiv(m, 256)), dim3(256), 0, s, m,
-                                       (const int64_t*)pptr.as<int64_t>(),
-                                       (const int64_t*)fl.as<int64_t>(), (int64_t*)c->buf[4]);
+                                       (const int64_t*)pp, (const int64_t*)fl.as<int64_t>(),
+                                       (int64_t*)c->buf[4]);
                 }
                 c->aux[1] = 1;  // row-local path
                 c->aux[2] = nbig;
@@ -1734,7 +1757,7 @@ int gsa_spgemm_csr_begin(void* stream, int vtype, int itype, int64_t m, int64_t 
         hipLaunchKernelGGL((k_sg_expand<T, I, uint64_t>), dim3(grid_for(m, 256)), dim3(256), 0, s, m, n,
                            kdim, (const I*)rowptrA, (const I*)colA, (const T*)valA,
                            (const I*)rowptrB, (const I*)colB, (const T*)valB,
-                           (const int64_t*)pptr.as<int64_t>(), keys.as<uint64_t>(), prods.as<T>(),
+                           (const int64_t*)pp, keys.as<uint64_t>(), prods.as<T>(),
                            perm.as<uint64_t>());
         const int bits = bits_for((uint64_t)(m - 1) * (uint64_t)n + (uint64_t)(n - 1));
         if ((r = radix_sort_pairs_u64(s, keys.as<uint64_t>(), perm.as<uint64_t>(), P, bits)) != GSA_OK)
