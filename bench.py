@@ -275,6 +275,21 @@ def bench_structural(H, dev):
     t = timed(lambda: G.kron(dK1, dK2), out[kn])
     out[kn]["GBs"] = round((Kc.nnz() * 16 + (Kc.m + 1) * 8) / t / 1e9, 1)
     del Kc
+    # sparse + dense at the reference benchmark's shape (matrix_benchmarks.jl:189-229:
+    # 10000^2 at density 0.01, Float64): copy(B), then the entries added
+    S = synth.sprand(10_000, 10_000, 0.01, seed=33)
+    dS = dcsr(S)
+    Dd = torch.randn((10_000, 10_000), dtype=torch.float64, device=dev).t()  # column-major
+    sd = {"nnz": S.nnz}
+    for fname, M in (("csr", dS), ("csc", G.GenericSparseMatrixCSC(dS)),
+                     ("coo", G.GenericSparseMatrixCOO(dS))):
+        sd[fname + "_ms"] = round(_sync_time(lambda: M + Dd) * 1e3, 3)
+    # bytes: the dense operand read and written once, the entries read, their
+    # elements read and written again
+    sd["csr_GBs"] = round((2 * Dd.numel() * 8 + S.nnz * (16 + 16) + (S.m + 1) * 8)
+                          / (sd["csr_ms"] * 1e-3) / 1e9, 1)
+    out["sparse_plus_dense_10000_d0.01"] = sd
+    del Dd, dS
     torch.cuda.empty_cache()
     return out
 

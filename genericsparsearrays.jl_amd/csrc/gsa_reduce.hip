@@ -441,6 +441,46 @@ __global__ __launch_bounds__(ROW_WG) void k_pow_sum(int64_t nnz, const T* __rest
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
 
+// ---- sparse + dense: C[i, j] += a_ij into a column-major dense C -----------
+// (kernel_add_sparse_to_dense_{csr,csc,coo}!, src/matrix_csr/matrix_csr_kernels.jl:134-145,
+// src/matrix_csc/matrix_csc_kernels.jl:134-145, src/matrix_coo/matrix_coo_kernels.jl:132-142).
+// Every stored entry adds into its element with an atomic add: without
+// duplicate indices each element gets at most one add, so the result is
+// C = B + a_ij rounded once, whatever the order; duplicates add in an order the
+// hardware picks (the reference's COO kernel races on them unprotected).
+// Compressed formats: `cm` says whether the major index is C's column (CSC,
+// or transpose(CSR)); entries are visited entry-parallel per row block.
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_add_dense_ptr(int64_t nmaj, int64_t nmin,
+                                                          int64_t nnz, const I* __restrict__ ptr,
+                                                          const I* __restrict__ idx,
+                                                          const T* __restrict__ val, int cm,
+                                                          int cj, const int64_t* __restrict__ blk_row,
+                                                          T* __restrict__ C, int64_t ldc) {
+    auto add = [&](int64_t r, int64_t j) {
+        const int64_t q = dclamp((int64_t)idx[j] - 1, 0, nmin - 1);
+        const int64_t at = cm ? q + r * ldc : r + q * ldc;
+        atomic_add_v(C + at, conj_if(val[j], cj != 0));
+    };
+    visit_owned_entries<I>(
+        nmaj, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {}, add,
+        [&](int64_t r, int64_t s, int64_t e) {
+            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) add(r, j);
+        });
+}
+
+template <typename T, typename I>
+__global__ void k_add_dense_coo(int64_t m, int64_t n, int64_t nnz, const I* __restrict__ row,
+                                const I* __restrict__ col, const T* __restrict__ val, int cj,
+                                T* __restrict__ C, int64_t ldc) {
+    for (int64_t j = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; j < nnz;
+         j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = dclamp((int64_t)row[j] - 1, 0, m - 1);
+        const int64_t k = dclamp((int64_t)col[j] - 1, 0, n - 1);
+        atomic_add_v(C + i + k * ldc, conj_if(val[j], cj != 0));
+    }
+}
+
 // Row-block plan over `ptr` for the visit_owned_rows kernels: the caller's
 // plan when it was built on this pointer array, else one built into `tmp`.
 static int row_blocks(hipStream_t s, gsa_plan plan, int itype, int64_t n, int64_t nnz,
@@ -680,6 +720,39 @@ int gsa_add_diag_fill(void* stream, int vtype, int itype, int64_t n, int64_t nnz
         hipLaunchKernelGGL((k_add_diag<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, n, nnz,
                            (const I*)ptr, (const I*)idx, (const T*)val, lam, br,
                            (const I*)ptr_out, (I*)idx_out, (T*)val_out);
+        GSA_HIP(hipGetLastError());
+        return GSA_OK;
+    });
+}
+
+int gsa_add_to_dense(void* stream, int vtype, int itype, int format, int conj, int64_t m,
+                     int64_t n, int64_t nnz, const void* a, const void* b, const void* val,
+                     void* C, int64_t ldc) {
+    GSA_TRY(ensure_init());
+    GSA_CHECK(m >= 0 && n >= 0 && nnz >= 0 && C && ldc >= (m > 0 ? m : 1) &&
+              (format == GSA_FMT_CSR || format == GSA_FMT_CSC || format == GSA_FMT_COO) &&
+              (nnz == 0 || (a && b && val)) && (format == GSA_FMT_COO || a),
+              GSA_ERR_INVALID, "gsa_add_to_dense: bad arguments");
+    if (m == 0 || n == 0 || nnz == 0) return GSA_OK;
+    hipStream_t s = as_stream(stream);
+    return dispatch_vai(vtype, itype, [&](auto tv, auto ti) -> int {
+        using T = decltype(tv);
+        using I = decltype(ti);
+        if (format == GSA_FMT_COO) {
+            hipLaunchKernelGGL((k_add_dense_coo<T, I>), dim3(grid_for(nnz, 256)), dim3(256), 0, s, m,
+                               n, nnz, (const I*)a, (const I*)b, (const T*)val, conj, (T*)C, ldc);
+            GSA_HIP(hipGetLastError());
+            return GSA_OK;
+        }
+        const bool cm = format == GSA_FMT_CSC;
+        const int64_t nmaj = cm ? n : m, nmin = cm ? m : n;
+        Scratch tmp(s);
+        const int64_t* br = nullptr;
+        int64_t nb = 0;
+        GSA_TRY(row_blocks(s, nullptr, itype, nmaj, nnz, a, tmp, &br, &nb));
+        hipLaunchKernelGGL((k_add_dense_ptr<T, I>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, nmaj,
+                           nmin, nnz, (const I*)a, (const I*)b, (const T*)val, cm ? 1 : 0, conj, br,
+                           (T*)C, ldc);
         GSA_HIP(hipGetLastError());
         return GSA_OK;
     });

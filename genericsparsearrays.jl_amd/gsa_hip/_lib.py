@@ -121,6 +121,7 @@ _SIGS = {
     "gsa_add_diag_count": [_vp, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _pi64],
     "gsa_add_diag_fill": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "gsa_diag_coo": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp],
+    "gsa_add_to_dense": [_vp, _i32, _i32, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _i64],
     "gsa_spvec_dot": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp],
     "gsa_norm_values": [_vp, _i32, _i64, _vp, _f64, ctypes.POINTER(_f64)],
     "gsa_batch_plan_create": [_vp, _i32, _i32, _i64, ctypes.POINTER(BatchDesc), _i32,

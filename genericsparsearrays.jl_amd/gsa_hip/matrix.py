@@ -192,6 +192,14 @@ class AbstractGenericSparseMatrix:
         from . import ops
         return ops.add(self, ops.negate(other))
 
+    def __radd__(self, other):  # dense + sparse (src/core.jl:89)
+        from . import ops
+        return ops.add(other, self)
+
+    def __rsub__(self, other):  # dense - sparse (src/core.jl:93)
+        from . import ops
+        return ops.add(other, ops.negate(self))
+
     def __neg__(self):
         from . import ops
         return ops.negate(self)
@@ -552,6 +560,14 @@ class Transpose:
     def __sub__(self, other):
         from . import ops
         return ops.add(self, ops.negate(other))
+
+    def __radd__(self, other):
+        from . import ops
+        return ops.add(other, self)
+
+    def __rsub__(self, other):
+        from . import ops
+        return ops.add(other, ops.negate(self))
 
     def __neg__(self):
         from . import ops

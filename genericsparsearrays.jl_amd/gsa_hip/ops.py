@@ -176,8 +176,11 @@ def scale(A, alpha):
 
 
 def negate(A):
-    """Base.:-(A) (src/matrix_csr/matrix_csr.jl:88-90)."""
+    """Base.:-(A) (src/matrix_csr/matrix_csr.jl:88-90); a dense operand (the B
+    of A - B, src/core.jl:91-93) is negated as the broadcast it is in Julia."""
     if isinstance(A, UniformScaling):
+        return -A
+    if isinstance(A, torch.Tensor):
         return -A
     if isinstance(A, Transpose):
         return type(A)(negate(A.parent))
@@ -291,6 +294,10 @@ def add(A, B):
         A, B = B, A
     if isinstance(B, UniformScaling):
         return add_uniform_scaling(A, B)
+    if isinstance(A, torch.Tensor):
+        A, B = B, A  # B + A = A + B (src/core.jl:89)
+    if isinstance(B, torch.Tensor):
+        return add_dense(A, B)
     if tuple(A.shape) != tuple(B.shape):
         raise DimensionMismatch(f"dimensions must match: A has dims {tuple(A.shape)}, "
                                 f"B has dims {tuple(B.shape)}")
@@ -314,6 +321,57 @@ def add(A, B):
                                  Bp.colptr, Bp.rowval)
         return convert.to_csc(add(convert.to_csr(A), convert.to_csr(B)))
     raise ArgumentError(f"+: unsupported operand {type(A).__name__}")
+
+
+def add_dense(A, B: torch.Tensor):
+    """Base.:+(A, B::DenseMatrix) and B + A (src/core.jl:67-89): a copy of the
+    dense B, then every stored entry of the sparse A added into it
+    (gsa_add_to_dense).  The result keeps B's element type and layout (column-
+    or row-major); A's values are converted to it when that is exact under
+    promote_type (a wider A than B is an ArgumentError here).  Wrappers go
+    through their parent's arrays: transpose(CSR) reads as a CSC of the
+    transpose, and so on; adjoint conjugates the values."""
+    if B.dim() != 2 or tuple(A.shape) != tuple(B.shape):
+        raise DimensionMismatch(f"dimensions must match: A has dims {tuple(A.shape)}, "
+                                f"B has dims {tuple(B.shape)}")
+    Ap, opA = _unwrap(A)
+    vals = Ap.nzval
+    if vals.device != B.device:
+        raise ArgumentError("Both arrays must have the same backend")
+    dev = require_device(vals, B)
+    if vals.dtype != B.dtype:
+        if promote_dtype(vals.dtype, B.dtype) != B.dtype:
+            raise ArgumentError(f"+: the sparse values ({vals.dtype}) do not convert exactly "
+                                f"to the dense element type ({B.dtype})")
+        vals = vals.to(B.dtype)
+    C = B.resolve_conj() if B.is_conj() else B.clone()  # copy(B), same strides
+    m, n = B.shape
+    if m == 0 or n == 0 or Ap.nnz() == 0:
+        return C
+    if isinstance(Ap, GenericSparseMatrixCOO):
+        fmt, a, b = L.GSA_FMT_COO, Ap.rowind, Ap.colind
+        if opA != L.GSA_OP_N:
+            a, b = b, a
+    elif isinstance(Ap, GenericSparseMatrixCSR):
+        fmt, a, b = (L.GSA_FMT_CSR if opA == L.GSA_OP_N else L.GSA_FMT_CSC), Ap.rowptr, Ap.colval
+    else:
+        fmt, a, b = (L.GSA_FMT_CSC if opA == L.GSA_OP_N else L.GSA_FMT_CSR), Ap.colptr, Ap.rowval
+    if C.stride(0) == 1 or m == 1:      # column-major: element (i, j) at i + j*ld
+        ld = C.stride(1) if n > 1 else max(m, 1)
+    elif C.stride(1) == 1:              # row-major: the column-major transpose
+        ld = C.stride(0)
+        m, n = n, m
+        if fmt == L.GSA_FMT_COO:
+            a, b = b, a
+        else:
+            fmt = L.GSA_FMT_CSC if fmt == L.GSA_FMT_CSR else L.GSA_FMT_CSR
+    else:
+        C = B.t().contiguous().t()  # a strided B: work on a column-major copy
+        ld = C.stride(1)
+    L.call("gsa_add_to_dense", stream_of(dev), vtype_of(vals), itype_of(a), fmt,
+           1 if opA == L.GSA_OP_C else 0, m, n, Ap.nnz(), ptr(a), ptr(b), ptr(vals), ptr(C),
+           max(ld, m, 1))
+    return C
 
 
 # ------------------------------------------------------------ kron ---------

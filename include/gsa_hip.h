@@ -472,6 +472,21 @@ int gsa_add_diag_fill(void* stream, int vtype, int itype, int64_t n, int64_t nnz
 int gsa_diag_coo(void* stream, int vtype, int itype, int64_t n, const void* dvals,
                  const void* lambda, void* row, void* col, void* val);
 
+/* Sparse + dense: C[i, j] += a_ij for every stored entry, into the column-major
+ * m x n dense C (leading dimension ldc) that already holds a copy of the dense
+ * operand.  format GSA_FMT_CSR: a = rowptr (m+1), b = colval; GSA_FMT_CSC: a =
+ * colptr (n+1), b = rowval (also transpose/adjoint(CSR) with the CSR arrays);
+ * GSA_FMT_COO: a = rowind, b = colind.  conj != 0 adds conj(a_ij) (an
+ * adjoint).  Without duplicate indices every element gets at most one add, so
+ * C = B + a_ij rounded once in any order; duplicates add in hardware order.
+ * Replaces Base.:+(::AbstractGenericSparseMatrix, ::DenseMatrix) src/core.jl:67-87
+ * (copy(B), then _add_sparse_to_dense!: kernel_add_sparse_to_dense_csr!
+ * src/matrix_csr/matrix_csr_kernels.jl:134-145, _csc! matrix_csc_kernels.jl:134-145,
+ * _coo! matrix_coo_kernels.jl:132-142). */
+int gsa_add_to_dense(void* stream, int vtype, int itype, int format, int conj, int64_t m,
+                     int64_t n, int64_t nnz, const void* a, const void* b, const void* val,
+                     void* C, int64_t ldc);
+
 /* ========================== GenericSparseVector ============================ */
 /* dot(x, y) for x a sparse vector (length n, nnz stored entries, one-based
  * nzind) and y a dense device vector: sum of conj(nzval[i]) * y[nzind[i]] in a

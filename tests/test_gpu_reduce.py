@@ -342,3 +342,58 @@ def test_kron_diagonal_formats_and_wrappers(dev, dtype):
     exp = O.kron_coo_diag((host(Mo.rowind), host(Mo.colind), host(Mo.nzval)),
                           np.full(4, 2.0, NPT[dtype]))
     _same(_ptr_tuple(got), O.coo_to_csr(200 * 4, 300 * 4, *exp))
+
+
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32, np.complex128, np.int64])
+def test_sparse_plus_dense(dev, dtype, itype):
+    """A + B and B + A for a dense B (src/core.jl:67-89): copy(B) with every stored
+    entry added.  Without duplicates each element gets one add, so the result
+    equals numpy's B + dense(A) bit for bit, for CSR, CSC, COO, their transpose
+    and adjoint, with B column- or row-major; A - B and B - A go through -B / -A."""
+    rng = np.random.default_rng(23)
+    m, n = 300, 220
+    S = sp.random(m, n, density=0.03, random_state=rng, format="csr")
+    if np.issubdtype(dtype, np.complexfloating):
+        S.data = (S.data + 1j * rng.standard_normal(S.nnz)).astype(dtype)
+        Bh = (rng.standard_normal((m, n)) + 1j * rng.standard_normal((m, n))).astype(dtype)
+    elif np.issubdtype(dtype, np.integer):
+        S.data = rng.integers(-50, 50, S.nnz).astype(dtype)
+        Bh = rng.integers(-1000, 1000, (m, n)).astype(dtype)
+    else:
+        S.data = rng.standard_normal(S.nnz).astype(dtype)
+        Bh = rng.standard_normal((m, n)).astype(dtype)
+    A = G.convert.csr_from_scipy(S, device=dev, index_dtype=itype)
+    Dn = S.toarray()
+    forms = {"csr": A, "csc": G.GenericSparseMatrixCSC(A), "coo": G.GenericSparseMatrixCOO(A)}
+    for layout in ("col", "row"):
+        Bd = torch.as_tensor(Bh, device=dev)
+        if layout == "col":
+            Bd = Bd.t().contiguous().t()
+        for name, M in forms.items():
+            got = M + Bd
+            assert got.dtype == Bd.dtype and got.stride() == Bd.stride(), (name, layout)
+            assert np.array_equal(host(got), Bh + Dn), (name, layout)
+            assert np.array_equal(host(Bd + M), Bh + Dn), (name, layout)
+            assert np.array_equal(host(Bd), Bh)  # B untouched
+            Bt = torch.as_tensor(np.ascontiguousarray(Bh.T), device=dev)
+            assert np.array_equal(host(G.transpose(M) + Bt), Bh.T + Dn.T), name
+            assert np.array_equal(host(G.adjoint(M) + Bt), Bh.T + np.conj(Dn.T)), name
+            if not np.issubdtype(dtype, np.integer):
+                assert np.array_equal(host(M - Bd), Dn + (-Bh)), name
+                assert np.array_equal(host(Bd - M), Bh + (-Dn)), name
+    # COO duplicates add up (in any order: integer values make it exact)
+    if np.issubdtype(dtype, np.integer):
+        r = np.array([1, 3, 1, 1]); c = np.array([2, 2, 2, 5]); v = np.array([4, 5, 6, 7])
+        Dup = G.GenericSparseMatrixCOO(m, n, torch.as_tensor(r).to(dev, itype),
+                                       torch.as_tensor(c).to(dev, itype),
+                                       torch.as_tensor(v.astype(dtype)).to(dev))
+        exp = Bh.copy()
+        np.add.at(exp, (r - 1, c - 1), v.astype(dtype))
+        assert np.array_equal(host(Dup + torch.as_tensor(Bh, device=dev)), exp)
+    # the reference's checks
+    with pytest.raises(G.DimensionMismatch):
+        A + torch.zeros((m, n + 1), dtype=torch.as_tensor(Bh).dtype, device=dev)
+    if dtype == np.float64:
+        with pytest.raises(G.ArgumentError):  # Float64 values into a Float32 dense copy
+            A + torch.zeros((m, n), dtype=torch.float32, device=dev)
