@@ -207,6 +207,63 @@ class AbstractGenericSparseMatrix:
     def __pos__(self):
         return self.copy()
 
+    # ---- maps over the stored values (src/matrix_csr/matrix_csr.jl:80-106,
+    # src/core.jl:26-40; CSC and COO alike): the structure is copied, the values
+    # mapped on the device as the reference's broadcasts are ----
+    def _index_arrays(self):
+        raise NotImplementedError
+
+    def _with_values(self, vals: torch.Tensor):
+        return type(self)(self.m, self.n, *(t.clone() for t in self._index_arrays()), vals,
+                          copy=False)
+
+    def zero(self):
+        """Base.zero(A): the same size, no stored entries (a pointer of ones)."""
+        a, b = self._index_arrays()
+        empty_i = torch.empty(0, dtype=b.dtype, device=b.device)
+        empty_v = torch.empty(0, dtype=self.nzval.dtype, device=self.nzval.device)
+        if isinstance(self, GenericSparseMatrixCOO):
+            return type(self)(self.m, self.n, empty_i, empty_i.clone(), empty_v, copy=False)
+        ones = torch.ones(a.numel(), dtype=a.dtype, device=a.device)
+        return type(self)(self.m, self.n, ones, empty_i, empty_v, copy=False)
+
+    def conj(self):
+        """Base.conj(A): A itself for real values, else conjugated values."""
+        if not self.nzval.is_complex():
+            return self
+        return self._with_values(self.nzval.conj().resolve_conj())
+
+    def real(self):
+        """Base.real(A): A itself for real values, else the real parts."""
+        if not self.nzval.is_complex():
+            return self
+        return self._with_values(self.nzval.real.contiguous())
+
+    def imag(self):
+        """Base.imag(A): zero(A) for real values, else the imaginary parts (the
+        structure kept, zeros included)."""
+        if not self.nzval.is_complex():
+            return self.zero()
+        return self._with_values(self.nzval.imag.contiguous())
+
+    def rmul_(self, x):
+        """LinearAlgebra.rmul!(A, x): the stored values times x, in place."""
+        self.nzval.mul_(x)
+        return self
+
+    def lmul_(self, x):
+        """LinearAlgebra.lmul!(x, A)."""
+        self.nzval.mul_(x)
+        return self
+
+    def rdiv_(self, x):
+        """LinearAlgebra.rdiv!(A, x) = rmul!(A, inv(Tv(x)))."""
+        if self.nzval.is_floating_point() or self.nzval.is_complex():
+            inv = 1.0 / torch.tensor(x, dtype=self.nzval.dtype).item()
+            self.nzval.mul_(inv)
+            return self
+        raise ArgumentError("rdiv!: integer values have no inverse")
+
     def to_scipy(self):
         raise NotImplementedError
 
@@ -381,6 +438,9 @@ class GenericSparseMatrixCSR(AbstractGenericSparseMatrix):
     def copy(self):
         return GenericSparseMatrixCSR(self.m, self.n, self.rowptr, self.colval, self.nzval)
 
+    def _index_arrays(self):
+        return self.rowptr, self.colval
+
     def to_scipy(self):
         import scipy.sparse as sp
         rp = self.rowptr.cpu().numpy().astype(np.int64) - 1
@@ -444,6 +504,9 @@ class GenericSparseMatrixCSC(AbstractGenericSparseMatrix):
 
     def copy(self):
         return GenericSparseMatrixCSC(self.m, self.n, self.colptr, self.rowval, self.nzval)
+
+    def _index_arrays(self):
+        return self.colptr, self.rowval
 
     def to_scipy(self):
         import scipy.sparse as sp
@@ -512,6 +575,9 @@ class GenericSparseMatrixCOO(AbstractGenericSparseMatrix):
 
     def copy(self):
         return GenericSparseMatrixCOO(self.m, self.n, self.rowind, self.colind, self.nzval)
+
+    def _index_arrays(self):
+        return self.rowind, self.colind
 
     def to_scipy(self):
         """SparseMatrixCSC(::COO) sums duplicates (conversions.jl:59-66)."""

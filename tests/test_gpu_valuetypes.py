@@ -383,3 +383,35 @@ def test_sparse_vector(dev, dtype, itype):
         G.GenericSparseVector(-1, V.nzind, V.nzval)
     with pytest.raises(G.ArgumentError):
         G.GenericSparseVector(n, V.nzind, V.nzval[:-1])
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.float64, np.int64])
+def test_value_maps_every_format(dev, dtype):
+    """zero, conj, real, imag, rmul!, lmul!, rdiv! (src/matrix_csr/matrix_csr.jl:80-106,
+    src/core.jl:26-40, and their CSC / COO twins): the structure kept (copied),
+    the stored values mapped, real inputs returned as they are by conj / real and
+    as zero(A) by imag."""
+    m, n = 90, 70
+    rp, cv, v = host_csr(m, n, 0.05, 5, dtype)
+    A = dev_mat(G.GenericSparseMatrixCSR, m, n, rp, cv, v, dev)
+    for M in (A, G.GenericSparseMatrixCSC(A), G.GenericSparseMatrixCOO(A)):
+        D = M.collect()
+        Z = M.zero()
+        assert type(Z) is type(M) and Z.shape == M.shape and Z.nnz() == 0
+        assert np.array_equal(Z.collect(), np.zeros((m, n), dtype=D.dtype))
+        if np.issubdtype(dtype, np.complexfloating):
+            assert np.array_equal(M.conj().collect(), np.conj(D))
+            assert np.array_equal(M.real().collect(), D.real)
+            assert np.array_equal(M.imag().collect(), D.imag)
+            assert M.imag().nnz() == M.nnz()  # the structure kept, zeros included
+        else:
+            assert M.conj() is M and M.real() is M and M.imag().nnz() == 0
+        B = M.copy().rmul_(3)
+        assert np.array_equal(B.collect(), D * 3)
+        assert np.array_equal(M.copy().lmul_(3).collect(), D * 3)
+        if not np.issubdtype(dtype, np.integer):
+            np.testing.assert_allclose(M.copy().rdiv_(4).collect(), D / 4, rtol=1e-15)
+        else:
+            with pytest.raises(G.ArgumentError):
+                M.copy().rdiv_(4)
+        assert np.array_equal(M.collect(), D)  # the source untouched
