@@ -201,7 +201,15 @@ def bench_structural(H, dev):
         out[name] = {}
         t = timed(fn, out[name])
         out[name]["GBs"] = round(byts / t / 1e9, 1)
-    del Ac, Ao, Aoc, A
+    # A + transpose(A) on the same matrix (rows of up to 5000 entries): the CSR of
+    # transpose(A) is the CSC's arrays
+    At = G.GenericSparseMatrixCSR(n, m, Ac.colptr, Ac.rowval, Ac.nzval, copy=False)
+    Cat = A + At
+    out["add_powerlaw_A_At"] = {"nnzC": Cat.nnz()}
+    t = timed(lambda: A + At, out["add_powerlaw_A_At"])
+    out["add_powerlaw_A_At"]["GBs"] = round((2 * nnz * 16 + 2 * (m + 1) * 8 + Cat.nnz() * 16
+                                             + (m + 1) * 8) / t / 1e9, 1)
+    del Cat, At, Ac, Ao, Aoc, A
     # host ingest: GenericSparseMatrixCSR(::SparseMatrixCSC) of the same matrix held
     # as a scipy CSC (the arrays copied as they are, transposed on the device),
     # against the host transposition the reference performs (scipy's tocsr)
@@ -312,6 +320,9 @@ def cpu_structural(H):
     out["csr_to_csc"] = (t, nnz * 16 + (m + 1) * 8 + nnz * 16 + (n + 1) * 8)
     _, t = tm(lambda: O.csc_to_csr(m, n, c[0], c[1], c[2]))
     out["csc_to_csr"] = (t, nnz * 16 + (n + 1) * 8 + nnz * 16 + (m + 1) * 8)
+    C, t = tm(lambda: O.csr_add(m, (H.rowptr, H.colval, H.nzval), c))
+    out["add_powerlaw_A_At"] = (t, 2 * nnz * 16 + 2 * (m + 1) * 8 + C[2].size * 16 + (m + 1) * 8)
+    del C
     co, t = tm(lambda: O.csr_to_coo(m, H.rowptr, H.colval, H.nzval))
     out["csr_to_coo"] = (t, nnz * 16 + (m + 1) * 8 + nnz * 24)
     _, t = tm(lambda: O.coo_to_csr(m, n, co[0], co[1], co[2]))

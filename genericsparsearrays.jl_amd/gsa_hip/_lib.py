@@ -92,8 +92,10 @@ _SIGS = {
     "gsa_csr_to_coo": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "gsa_coo_to_csr": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
     "gsa_csr_to_csc": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _vp, _vp, _vp],
-    "gsa_csr_add_count": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _pi64],
-    "gsa_csr_add_fill": [_vp, _i32, _i32, _i64, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "gsa_csr_add_count": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
+                          _pi64],
+    "gsa_csr_add_fill": [_vp, _i32, _i32, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp, _vp,
+                         _vp, _vp],
     "gsa_coo_add_begin": [_vp, _i32, _i32, _i64, _i64, _i64, _vp, _vp, _vp, _i64, _vp, _vp, _vp,
                           ctypes.POINTER(_vp), _pi64],
     "gsa_coo_add_finish": [_vp, _vp, _vp, _vp],

@@ -254,12 +254,13 @@ def _csr_like_add(A, B, cls, major, ptrA, ixA, ptrB, ixB):
     vt, it = vtype_of(vA), itype_of(ptrA)
     ptrC = torch.empty(major + 1, dtype=ptrA.dtype, device=dev)
     cnt = ctypes.c_int64()
-    L.call("gsa_csr_add_count", s, vt, it, major, ptr(ptrA), ptr(ixA), ptr(vA), ptr(ptrB), ptr(ixB),
-           ptr(vB), ptr(ptrC), ctypes.byref(cnt))
+    nA, nB = min(ixA.numel(), vA.numel()), min(ixB.numel(), vB.numel())
+    L.call("gsa_csr_add_count", s, vt, it, major, nA, ptr(ptrA), ptr(ixA), ptr(vA), nB, ptr(ptrB),
+           ptr(ixB), ptr(vB), ptr(ptrC), ctypes.byref(cnt))
     ixC = torch.empty(cnt.value, dtype=ptrA.dtype, device=dev)
     vC = torch.empty(cnt.value, dtype=T, device=dev)
-    L.call("gsa_csr_add_fill", s, vt, it, major, ptr(ptrA), ptr(ixA), ptr(vA), ptr(ptrB), ptr(ixB),
-           ptr(vB), ptr(ptrC), ptr(ixC), ptr(vC))
+    L.call("gsa_csr_add_fill", s, vt, it, major, nA, ptr(ptrA), ptr(ixA), ptr(vA), nB, ptr(ptrB),
+           ptr(ixB), ptr(vB), ptr(ptrC), ptr(ixC), ptr(vC))
     return cls(A.m, A.n, ptrC, ixC, vC, copy=False)
 
 

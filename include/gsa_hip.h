@@ -312,14 +312,16 @@ int gsa_csr_to_csc(void* stream, int vtype, int itype, int64_t m, int64_t n, int
  * Replaces Base.:+(::CSR, ::CSR) src/matrix_csr/matrix_csr.jl:290-355 with
  * kernels matrix_csr_kernels.jl:148-248 and dropzeros matrix_csr.jl:723-745.
  * CSC + CSC (src/matrix_csc/matrix_csc.jl:292-357) is the same call on the
- * column pointers. */
+ * column pointers.  nnzA and nnzB are the operands' stored entries (the lengths
+ * of colA/valA and colB/valB, nnz(A) and nnz(B) in Julia); every read is
+ * clamped into them. */
 int gsa_csr_add_count(void* stream, int vtype, int itype, int64_t m,
-                      const void* rowptrA, const void* colA, const void* valA,
-                      const void* rowptrB, const void* colB, const void* valB,
+                      int64_t nnzA, const void* rowptrA, const void* colA, const void* valA,
+                      int64_t nnzB, const void* rowptrB, const void* colB, const void* valB,
                       void* rowptrC, int64_t* nnzC);
 int gsa_csr_add_fill(void* stream, int vtype, int itype, int64_t m,
-                     const void* rowptrA, const void* colA, const void* valA,
-                     const void* rowptrB, const void* colB, const void* valB,
+                     int64_t nnzA, const void* rowptrA, const void* colA, const void* valA,
+                     int64_t nnzB, const void* rowptrB, const void* colB, const void* valB,
                      const void* rowptrC, void* colC, void* valC);
 
 /* COO + COO: concatenate, stable sort by column-major key, sum duplicate runs in

@@ -530,6 +530,114 @@ def test_host_egress_and_ingest_round_trip(dev, itype, vdtype):
     assert np.array_equal(G.convert.to_scipy(D).row, r)               # storage order kept
 
 
+def _rows_to_csr(rows):
+    """CSR arrays (one-based) from a list of (columns, values) per row."""
+    lens = np.array([len(c) for c, _ in rows], dtype=np.int64)
+    rp = np.r_[0, np.cumsum(lens)] + 1
+    cv = np.concatenate([np.asarray(c, np.int64) for c, _ in rows] + [np.zeros(0, np.int64)])
+    nz = np.concatenate([np.asarray(v, np.float64) for _, v in rows] + [np.zeros(0)])
+    return rp.astype(np.int64), cv, nz
+
+
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+def test_add_tiles_chunks_and_fallbacks(dev, itype):
+    """The row merge cuts the (row, column)-ordered entries into tiles of 1792
+    entries-plus-twice-the-rows, merges a tile's rows of fewer than 1792 entries
+    in sub-tiles of at most 2048 staged entries, and longer rows in chunks of 2048
+    output-order positions.  Rows just under and over those sizes, runs of rows
+    that need several sub-tiles, long rows whose equal-column pairs straddle
+    chunk boundaries (some cancelling), runs of empty rows longer than a tile,
+    and rows with repeated or unordered columns (merged one per thread, in a
+    staged sub-tile and as a long row) all equal the oracle's two-pointer merge
+    (CSR) and run sums (row-major COO) bit for bit."""
+    rng = np.random.default_rng(77)
+    n = 30_000
+    A_rows, B_rows = [], []
+
+    def pair(la, lb, overlap):
+        ca = np.sort(rng.choice(np.arange(2, n + 1), la, replace=False))
+        keep = rng.random(la) < overlap
+        extra = np.setdiff1d(rng.choice(np.arange(2, n + 1), lb, replace=False), ca)
+        cb = np.unique(np.r_[ca[keep][:lb], extra[:max(0, lb - int(keep[:lb].sum()))]])
+        va = rng.standard_normal(la)
+        vb = rng.standard_normal(cb.size)
+        shared = np.isin(cb, ca)
+        idx = np.searchsorted(ca, cb[shared])
+        cancel = rng.random(idx.size) < 0.3
+        vb_sh = vb[shared]
+        vb_sh[cancel] = -va[idx[cancel]]
+        vb[shared] = vb_sh
+        A_rows.append((ca, va))
+        B_rows.append((cb, vb))
+
+    def empty(k):
+        for _ in range(k):
+            A_rows.append(([], []))
+            B_rows.append(([], []))
+
+    for _ in range(300):  # short rows
+        pair(int(rng.integers(0, 12)), int(rng.integers(0, 12)), 0.5)
+    for la, lb in ((600, 423), (512, 512), (1023, 0), (0, 1024), (1024, 1023), (1100, 1000),
+                   (895, 896), (896, 896), (1791, 0), (0, 1792), (1000, 1049), (1024, 1025)):
+        pair(la, lb, 0.6)
+    for _ in range(12):  # rows of 300-900 entries: several sub-tiles per tile
+        pair(int(rng.integers(150, 450)), int(rng.integers(150, 450)), 0.5)
+    empty(2500)
+    for _ in range(50):
+        pair(int(rng.integers(0, 6)), int(rng.integers(0, 6)), 0.5)
+    # long rows whose every A column is in B, B starting one column lower: the
+    # output order alternates B, A, B, A, ... so the pairs straddle chunk edges
+    for la in (2047, 3000, 5000):
+        ca = np.sort(rng.choice(np.arange(3, n + 1), la, replace=False))
+        cb = np.r_[2, ca]
+        va = rng.standard_normal(la)
+        vb = rng.standard_normal(cb.size)
+        vb[1::7] = -va[::7]  # cancels at some of the straddling pairs
+        A_rows.append((ca, va))
+        B_rows.append((cb, vb))
+        pair(int(rng.integers(0, 9)), int(rng.integers(0, 9)), 0.5)
+    pair(9000, 7000, 0.4)  # many chunks in one row
+    empty(3)
+    A_t, B_t = _rows_to_csr(A_rows), _rows_to_csr(B_rows)
+    m = len(A_rows)
+    A = from_arrays(G.GenericSparseMatrixCSR, m, n, *A_t, dev, itype)
+    B = from_arrays(G.GenericSparseMatrixCSR, m, n, *B_t, dev, itype)
+    same(csr_tuple(A + B), O.csr_add(m, A_t, B_t))
+    same(csr_tuple(B + A), O.csr_add(m, B_t, A_t))
+    same(csr_tuple(A - A), O.csr_add(m, A_t, (A_t[0], A_t[1], -A_t[2])))
+    coo = lambda T: (np.repeat(np.arange(1, T[0].size), np.diff(T[0])), T[1], T[2])
+    Ao = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo(A_t), dev, itype)
+    Bo = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo(B_t), dev, itype)
+    same(coo_tuple(Ao + Bo), O.coo_add(m, coo(A_t), coo(B_t)))
+    # one short row with an unordered pair and one long row with a repeated column:
+    # their tiles (and that long row) are merged one row per thread
+    C_rows = [(c.copy(), v.copy()) for c, v in A_rows]
+    i_short = next(i for i, (c, _) in enumerate(C_rows) if 4 <= len(c) <= 11)
+    c = C_rows[i_short][0]
+    c[1], c[2] = c[2], c[1]
+    i_long = next(i for i, (c, _) in enumerate(C_rows) if len(c) >= 3000)
+    c = C_rows[i_long][0]
+    c[100] = c[99]
+    C_t = _rows_to_csr(C_rows)
+    Cm = from_arrays(G.GenericSparseMatrixCSR, m, n, *C_t, dev, itype)
+    same(csr_tuple(Cm + B), O.csr_add(m, C_t, B_t))
+    same(csr_tuple(B + Cm), O.csr_add(m, B_t, C_t))
+
+
+def test_add_powerlaw_full_size(dev):
+    """A + transpose(A) on the 1M-row power-law matrix (rows up to 5000 entries,
+    Float64/Int64), bit for bit against the oracle's two-pointer merge."""
+    H = synth.powerlaw_csr(1_000_000, 1_000_000, alpha=1.846, seed=2)
+    A = dev_csr(H)
+    Ac = G.GenericSparseMatrixCSC(A)
+    At = G.GenericSparseMatrixCSR(H.n, H.m, Ac.colptr, Ac.rowval, Ac.nzval, copy=False)
+    T_t = csc_tuple(Ac)
+    A_t = (H.rowptr, H.colval, H.nzval)
+    C = A + At
+    same(csr_tuple(C), O.csr_add(H.m, A_t, T_t))
+    assert C.nnz() > H.nnz
+
+
 @pytest.mark.parametrize("itype", [torch.int64, torch.int32])
 def test_coo_add_merge_and_sort_paths(dev, itype):
     """COO + COO (matrix_coo.jl:300-384): operands already in column-major order take
