@@ -29,6 +29,7 @@
 //    expand-sort-compress path (keyed (row, col), stable radix sort, run sums).
 #include "gsa_scan.h"
 #include "gsa_rows.h"
+#include "gsa_merge.h"
 #include "gsa_radix_pairs.h"
 
 namespace gsa {
@@ -88,655 +89,6 @@ __global__ __launch_bounds__(ROW_WG) void k_dz_fill(int64_t m, int64_t nnz,
             }
         });
 }
-
-// Row merge of A + B.
-//   RUNS = false: CSR + (kernel_add_csr!, matrix_csr_kernels.jl:188-248): a
-//     two-pointer merge, equal columns summed as vA + vB;
-//   RUNS = true: COO + of operands stored in (row, col) order: equal (row, col)
-//     entries are summed left to right, A's in storage order then B's -- the
-//     order of the reference's stable sort of the concatenation and its run
-//     sums (matrix_coo.jl:300-384, matrix_coo_kernels.jl:260-310); columns are
-//     clamped into range as the sort keys are.
-// Exact zeros are dropped.  Without FILL each row's count goes to ptrC[r + 1];
-// with FILL row r's entries go to positions ptrC[r] - 1 onwards.
-//
-// The work is the entry stream of A and of B in (row, column) order, cut into
-// tiles by g(r) = (entries of A and B before row r) + 2r: tile t holds the
-// rows r with t*MT_TILE <= g(r) < (t+1)*MT_TILE.  A tile thus has at most
-// MT_ROWS = MT_TILE/2 rows, and a row of MT_TILE or more entries (A's and B's
-// together: a long row) can only be its last row.  The other rows, the short
-// ones, are merged in LDS in sub-tiles: as many whole rows as fit MT_CAP staged
-// entries (one sub-tile for most tiles; a short row always fits).  A sub-tile
-// is read into LDS together (coalesced); each thread gives its MT_CH
-// consecutive staged entries their rows (a search among the staged row starts,
-// then a step) and one 64-bit key, row << MT_KB | column key; then every lane
-// merges MT_CH consecutive positions of the output order, found by a
-// merge-path search on those keys in which A's entry goes first between equal
-// keys.  A long row is merged the same way in chunks of MT_CAP positions,
-// split by merge-path searches over the row in global memory; the entry just
-// before a chunk's part of A and the one just after its part of B are staged
-// too, so an equal pair that a chunk boundary separates is still summed once.
-// A sub-tile whose rows, or a long row whose columns, do not strictly increase
-// (or reach 2^MT_KB) is merged row by row by one thread (merge_row_serial),
-// which keeps the two-pointer semantics for such input.  Every read is clamped
-// into the operands' arrays.
-constexpr int MT_TILE = 1792;
-constexpr int MT_ROWS = MT_TILE / 2;
-constexpr int MT_CAP = 2048;
-constexpr int MT_WG = 512;
-constexpr int MT_CH = MT_CAP / MT_WG;  // output-order positions per lane
-static_assert(MT_TILE <= MT_CAP, "a short row fits one sub-tile");
-constexpr int MT_KB = 40;              // bits of the column key in a packed key
-constexpr uint64_t MT_KMASK = (uint64_t(1) << MT_KB) - 1;
-constexpr uint64_t MT_NONE = ~uint64_t(0);  // a key that no staged entry has
-constexpr int MT_REC = 5;  // per tile boundary: row r, pa(r), pb(r), pa(r - 1), pb(r - 1)
-// The other merge: row blocks of MR_WG rows, one row per thread, used when every
-// block's entries of A and B fit MR_CAP (k_merge_rows).  For rows of a few
-// entries this beats the tiles' lane-parallel merge; the tiles take any other
-// input.  The bounds pass decides, in flag words: flag[0] is set when a row
-// pointer decreases; a block that does not fit sets one of MT_NCROWD "crowded"
-// words (block b: word b % MT_NCROWD), which lie a cache line apart, so the
-// many blocks that set them do not all write one address.
-constexpr int MR_WG = 256;
-constexpr int MR_CAP = 3072;  // 48 KB of Float64/Int64 entries: three workgroups per CU
-constexpr int MT_NCROWD = 64, MT_CSTRIDE = 32;  // crowded words, 128 bytes apart
-constexpr int MT_FLAGS = MT_CSTRIDE * (1 + MT_NCROWD);  // flag words in all
-__device__ __forceinline__ int* mt_crowd_word(int* flag, int64_t b) {
-    return flag + MT_CSTRIDE * (1 + (int)(b % MT_NCROWD));
-}
-// Is any block crowded?  (Lane j of each wavefront reads word j.)
-__device__ __forceinline__ bool mt_crowded(const int* flag) {
-    const int lane = threadIdx.x & 63;
-    return __any(flag[MT_CSTRIDE * (1 + lane % MT_NCROWD)] != 0);
-}
-
-// The serial merge of one row; colA(k) etc. read entry k of A / B.  With FILL
-// the row's entries go to positions w onwards; without, its count to ptrC[r + 1].
-template <typename T, typename I, bool RUNS, bool FILL, typename CA, typename VA, typename CB,
-          typename VB>
-__device__ __forceinline__ void merge_row_serial(int64_t r, int64_t n, int64_t a, int64_t ea,
-                                                 int64_t b, int64_t eb, CA colA, VA valA,
-                                                 CB colB, VB valB, I* __restrict__ ptrC,
-                                                 int64_t w, I* __restrict__ cC,
-                                                 T* __restrict__ vC) {
-    int64_t cnt = 0;
-    auto emit = [&](I c, T v) {
-        if (v == T(0)) return;
-        if (FILL) {
-            cC[w] = c;
-            vC[w] = v;
-            ++w;
-        }
-        ++cnt;
-    };
-    while (a < ea || b < eb) {
-        if constexpr (RUNS) {
-            const int64_t ca = a < ea ? dclamp((int64_t)colA(a) - 1, 0, n - 1) : n;
-            const int64_t cb = b < eb ? dclamp((int64_t)colB(b) - 1, 0, n - 1) : n;
-            const int64_t c = ca < cb ? ca : cb;
-            T sum = T(0);
-            bool first = true;
-            while (a < ea && dclamp((int64_t)colA(a) - 1, 0, n - 1) == c) {
-                sum = first ? valA(a) : sum + valA(a);
-                first = false;
-                ++a;
-            }
-            while (b < eb && dclamp((int64_t)colB(b) - 1, 0, n - 1) == c) {
-                sum = first ? valB(b) : sum + valB(b);
-                first = false;
-                ++b;
-            }
-            emit((I)(c + 1), sum);
-        } else {
-            if (a < ea && (b >= eb || colA(a) < colB(b))) {
-                emit(colA(a), valA(a));
-                ++a;
-            } else if (b < eb && (a >= ea || colA(a) > colB(b))) {
-                emit(colB(b), valB(b));
-                ++b;
-            } else {
-                emit(colA(a), valA(a) + valB(b));
-                ++a;
-                ++b;
-            }
-        }
-    }
-    if (!FILL) ptrC[r + 1] = (I)cnt;
-}
-
-// The staged part of a tile or chunk: A's entries in slots [1, 1 + nA), B's in
-// [1 + nA, 1 + nA + nB); slot 0 holds the entry before A's part and slot
-// 1 + nA + nB the entry after B's part (MT_NONE when there is none).  Keys are
-// row << MT_KB | column key, the row counted within the tile and the column key
-// being the column for CSR and the clamped zero-based column for RUNS.
-//
-// One lane merges the output-order positions [d0, d1) (d1 - d0 <= MT_CH): a
-// merge-path search finds how many of A's entries come before d0, then each
-// position takes A's entry when its key is not greater than B's.  An A entry
-// whose key equals the B entry it is compared with carries vA + vB; that B
-// entry then comes next and is skipped.  Bit u of the result is set when
-// position d0 + u yields an entry (nonzero value); its column and value are in
-// oc[u], ov[u].
-template <typename T, typename I, bool RUNS>
-__device__ __forceinline__ unsigned mt_lane_merge(int nA, int nB, int d0, int d1,
-                                                  const uint64_t* s_k, const T* s_v, I* oc,
-                                                  T* ov) {
-    auto out_col = [&](uint64_t k) -> I { return (I)((int64_t)(k & MT_KMASK) + (RUNS ? 1 : 0)); };
-    const int sB = 1 + nA;
-    int lo = d0 - nB > 0 ? d0 - nB : 0, hi = d0 < nA ? d0 : nA;
-    while (lo < hi) {  // A(mid) comes before B(d0 - 1 - mid) unless B's key is smaller
-        const int mid = (lo + hi) >> 1;
-        if (s_k[1 + mid] <= s_k[sB + d0 - 1 - mid]) lo = mid + 1;
-        else hi = mid;
-    }
-    int i = lo, j = d0 - lo;
-    unsigned keep = 0;
-#pragma unroll
-    for (int u = 0; u < MT_CH; ++u) {
-        if (d0 + u < d1) {
-            // slot 1 + nA is B's first entry when i == nA; sB + nB the slot after
-            // B's part when j == nB
-            const uint64_t ka = s_k[1 + i], kb = s_k[sB + j];
-            if (j >= nB || (i < nA && ka <= kb)) {
-                T v = s_v[1 + i];
-                if (ka == kb) v = v + s_v[sB + j];
-                oc[u] = out_col(ka);
-                ov[u] = v;
-                if (!(v == T(0))) keep |= 1u << u;
-                ++i;
-            } else {  // slot i holds A's entry i - 1 (the slot before A's part when i == 0)
-                const T v = s_v[sB + j];
-                oc[u] = out_col(kb);
-                ov[u] = v;
-                if (s_k[i] != kb && !(v == T(0))) keep |= 1u << u;
-                ++j;
-            }
-        }
-    }
-    return keep;
-}
-
-// The row (0-based within the tile) of staged entry k: the last of the rows
-// [lo, nrows) whose start rs[row] is at or before k (rs[lo] <= k).
-__device__ __forceinline__ int mt_row_of(const uint16_t* rs, int lo, int nrows, int k) {
-    int hi = nrows - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (rs[mid] <= k) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Exclusive prefix sum over the MT_WG lanes of a workgroup; *total gets the sum.
-__device__ __forceinline__ int mt_exclusive_sum(int v, int* lds8, int* total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    int x = v;
-#pragma unroll
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) lds8[wave] = x;
-    __syncthreads();
-    int pre = 0, all = 0;
-#pragma unroll
-    for (int w = 0; w < MT_WG / 64; ++w) {
-        const int s = lds8[w];
-        if (w < wave) pre += s;
-        all += s;
-    }
-    __syncthreads();
-    *total = all;
-    return pre + x - v;
-}
-
-// Every tile boundary record (MT_REC values per boundary t = 0 .. ntiles) set to
-// row m, and the MT_FLAGS flag words to 0 (launched with at least that many
-// threads).
-template <typename I>
-__global__ void k_mt_init(int64_t m, int64_t nnzA, const I* __restrict__ pA, int64_t nnzB,
-                          const I* __restrict__ pB, int64_t ntiles, int64_t* __restrict__ rec,
-                          int* __restrict__ flag) {
-    const int64_t am = dclamp((int64_t)pA[m] - 1, 0, nnzA), bm = dclamp((int64_t)pB[m] - 1, 0, nnzB);
-    const int64_t am1 = dclamp((int64_t)pA[m - 1] - 1, 0, nnzA);
-    const int64_t bm1 = dclamp((int64_t)pB[m - 1] - 1, 0, nnzB);
-    const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
-    for (int64_t t = t0; t <= ntiles; t += (int64_t)gridDim.x * blockDim.x) {
-        int64_t* p = rec + MT_REC * t;
-        p[0] = m;
-        p[1] = am;
-        p[2] = bm;
-        p[3] = am1;
-        p[4] = bm1;
-    }
-    if (t0 < MT_FLAGS) flag[t0] = 0;
-}
-
-// Boundary t (t < ntiles) is the first row r with g(r) >= t * MT_TILE; its
-// record holds r and the clamped entry offsets pa(r), pb(r), pa(r-1), pb(r-1)
-// (the records were set to row m beforehand, which stays the record of every
-// boundary past g(m), and of boundary ntiles).  Row r writes the boundaries
-// whose t * MT_TILE lie in (g(r-1), g(r)].  A row pointer that decreases (after
-// clamping into the arrays) sets flag[0]: the tiles' kernel then merges every
-// row by one thread; a row block of k_merge_rows whose entries exceed MR_CAP
-// sets its crowded word: the tiles take the input.  (Plain stores of 1: every
-// writer stores the same value, so no atomics are needed.)
-template <typename I>
-__global__ void k_mt_bounds(int64_t m, int64_t nnzA, const I* __restrict__ pA, int64_t nnzB,
-                            const I* __restrict__ pB, int64_t ntiles, int64_t* __restrict__ rec,
-                            int* __restrict__ flag) {
-    auto pa = [&](int64_t r) { return dclamp((int64_t)pA[r] - 1, 0, nnzA); };
-    auto pb = [&](int64_t r) { return dclamp((int64_t)pB[r] - 1, 0, nnzB); };
-    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r <= m;
-         r += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t ar = pa(r), br = pb(r);
-        const int64_t g = ar + br + 2 * r;
-        int64_t tlo = 0, ap = ar, bp = br;
-        if (r > 0) {
-            ap = pa(r - 1);
-            bp = pb(r - 1);
-            if (ar < ap || br < bp) *(volatile int*)flag = 1;
-            tlo = (ap + bp + 2 * (r - 1)) / MT_TILE + 1;
-        }
-        if (r < m && r % MR_WG == 0) {  // row block [r, re) of k_merge_rows
-            const int64_t re = r + MR_WG < m ? r + MR_WG : m;
-            if ((pa(re) - ar) + (pb(re) - br) > MR_CAP)
-                *(volatile int*)mt_crowd_word(flag, r / MR_WG) = 1;
-        }
-        int64_t thi = g / MT_TILE;
-        thi = thi < ntiles - 1 ? thi : ntiles - 1;
-        for (int64_t t = tlo; t <= thi; ++t) {
-            int64_t* p = rec + MT_REC * t;
-            p[0] = r;
-            p[1] = ar;
-            p[2] = br;
-            p[3] = ap;
-            p[4] = bp;
-        }
-    }
-}
-
-// Row block blockIdx of MR_WG rows (when the flag is clear, so the block's
-// entries fit MR_CAP): the block read into LDS together (coalesced), then
-// each row merged there by one thread.
-template <typename T, typename I, bool RUNS, bool FILL>
-__global__ __launch_bounds__(MR_WG) void k_merge_rows(
-    int64_t m, int64_t n, const int* __restrict__ flag, int64_t nnzA, const I* __restrict__ pA,
-    const I* __restrict__ cA, const T* __restrict__ vA, int64_t nnzB, const I* __restrict__ pB,
-    const I* __restrict__ cB, const T* __restrict__ vB, I* __restrict__ ptrC,
-    I* __restrict__ cC, T* __restrict__ vC) {
-    if (flag[0] || mt_crowded(flag)) return;  // k_merge_tiles takes the input
-    __shared__ I s_c[MR_CAP];
-    __shared__ T s_v[MR_CAP];
-    const int tid = threadIdx.x;
-    const int64_t r0 = (int64_t)blockIdx.x * MR_WG;
-    if (r0 >= m) return;
-    const int64_t r1 = r0 + MR_WG < m ? r0 + MR_WG : m;
-    auto pa = [&](int64_t r) { return dclamp((int64_t)pA[r] - 1, 0, nnzA); };
-    auto pb = [&](int64_t r) { return dclamp((int64_t)pB[r] - 1, 0, nnzB); };
-    const int64_t a0 = pa(r0), b0 = pb(r0);
-    const int na = (int)(pa(r1) - a0), nb = (int)(pb(r1) - b0);  // na + nb <= MR_CAP
-    for (int k = tid; k < na; k += MR_WG) {
-        s_c[k] = cA[a0 + k];
-        s_v[k] = vA[a0 + k];
-    }
-    for (int k = tid; k < nb; k += MR_WG) {
-        s_c[na + k] = cB[b0 + k];
-        s_v[na + k] = vB[b0 + k];
-    }
-    __syncthreads();
-    const int64_t r = r0 + tid;
-    if (r < r1)
-        merge_row_serial<T, I, RUNS, FILL>(
-            r, n, pa(r) - a0, pa(r + 1) - a0, pb(r) - b0, pb(r + 1) - b0,
-            [&](int64_t k) { return s_c[k]; }, [&](int64_t k) { return s_v[k]; },
-            [&](int64_t k) { return s_c[na + k]; }, [&](int64_t k) { return s_v[na + k]; }, ptrC,
-            FILL ? (int64_t)ptrC[r] - 1 : 0, cC, vC);
-}
-
-// Tile t of the row merge (rows from boundary t to boundary t + 1),
-// XCD-contiguous tile order.
-constexpr int MT_AUX = 2 * (MT_ROWS + 1) * 2 > MT_WG * 8 ? 2 * (MT_ROWS + 1) * 2 : MT_WG * 8;
-constexpr int MT_RS = (MT_ROWS + 1 + MT_WG - 1) / MT_WG;  // row pointers per thread
-template <typename T, typename I, bool RUNS, bool FILL>
-__global__ __launch_bounds__(MT_WG) void k_merge_tiles(
-    int64_t m, int64_t n, int64_t ntiles, const int64_t* __restrict__ rec,
-    const int* __restrict__ flag, int64_t nnzA, const I* __restrict__ pA,
-    const I* __restrict__ cA, const T* __restrict__ vA, int64_t nnzB, const I* __restrict__ pB,
-    const I* __restrict__ cB, const T* __restrict__ vB, I* __restrict__ ptrC,
-    I* __restrict__ cC, T* __restrict__ vC) {
-    __shared__ uint64_t s_k[MT_CAP + 2];
-    __shared__ T s_v[MT_CAP + 2];
-    // the staged row starts (A's, then B's), or a long row's chunk splits
-    __shared__ __align__(8) unsigned char s_aux[MT_AUX];
-    __shared__ int lds8[MT_WG / 64];
-    uint16_t* s_rs = reinterpret_cast<uint16_t*>(s_aux);
-    int64_t* s_split = reinterpret_cast<int64_t*>(s_aux);
-    const int tid = threadIdx.x;
-    auto pa = [&](int64_t r) { return dclamp((int64_t)pA[r] - 1, 0, nnzA); };
-    auto pb = [&](int64_t r) { return dclamp((int64_t)pB[r] - 1, 0, nnzB); };
-    // column key, as an unsigned value: a column below 1 or of 2^MT_KB and more
-    // exceeds MT_KMASK
-    auto key = [&](I c) -> uint64_t {
-        return RUNS ? (uint64_t)dclamp((int64_t)c - 1, 0, n - 1) : (uint64_t)(int64_t)c;
-    };
-    // row r from global memory by this thread, entry ranges clamped
-    auto serial_global = [&](int64_t r) {
-        const int64_t a = pa(r), b = pb(r);
-        const int64_t ea = pa(r + 1) > a ? pa(r + 1) : a, eb = pb(r + 1) > b ? pb(r + 1) : b;
-        merge_row_serial<T, I, RUNS, FILL>(
-            r, n, a, ea, b, eb, [&](int64_t k) { return cA[k]; },
-            [&](int64_t k) { return vA[k]; }, [&](int64_t k) { return cB[k]; },
-            [&](int64_t k) { return vB[k]; }, ptrC, FILL ? (int64_t)ptrC[r] - 1 : 0, cC, vC);
-    };
-    if (flag[0]) {  // pointer arrays that decrease: one row per thread
-        for (int64_t r = blockIdx.x * (int64_t)MT_WG + tid; r < m; r += (int64_t)gridDim.x * MT_WG)
-            serial_global(r);
-        return;
-    }
-    if (!mt_crowded(flag)) return;  // every row block fits the stage: k_merge_rows merges them
-
-    // One tile; every thread of the workgroup calls it with the same t.
-    auto tile = [&](const int64_t t) {
-    // boundaries t and t + 1: rows [lo, hi); the entries of row hi - 1, the only
-    // row that may be long, start at (qa, qb) and end at (ae, be)
-    const int64_t* b0 = rec + MT_REC * t;
-    const int64_t* b1 = b0 + MT_REC;
-    const int64_t lo = dclamp(b0[0], 0, m), hi = dclamp(b1[0], lo, m);
-    if (lo >= hi) return;
-    const int64_t a_s = b0[1], b_s = b0[2], ae = b1[1], be = b1[2], qa = b1[3], qb = b1[4];
-    const int64_t q = hi - 1;
-    const bool has_long = (ae - qa) + (be - qb) >= MT_TILE;
-    const int64_t qe = has_long ? q : hi;
-    // where the tile's output starts (read now, used after the merge)
-    const int64_t wlo = FILL ? (int64_t)ptrC[lo] - 1 : 0;
-
-    // A sub-tile's (or a chunk's) entries: nA of A from position ga and nB of B
-    // from gb, staged in slots 1 .. nA + nB.  The count variant reads them all
-    // in one batch (thread tid takes entries tid + u * MT_WG of A's part followed
-    // by B's: load, then put); the fill variant, which needs more registers for
-    // its outputs, stores each as it arrives (stage), which keeps its occupancy.
-    I pc[MT_CH];
-    T pv[MT_CH];
-    auto stage = [&](int64_t ga, int nA, int64_t gb, int nB) {
-        for (int k = tid; k < nA; k += MT_WG) {
-            s_k[1 + k] = key(cA[ga + k]);
-            s_v[1 + k] = vA[ga + k];
-        }
-        for (int k = tid; k < nB; k += MT_WG) {
-            s_k[1 + nA + k] = key(cB[gb + k]);
-            s_v[1 + nA + k] = vB[gb + k];
-        }
-    };
-    auto load = [&](int64_t ga, int nA, int64_t gb, int nB) {
-#pragma unroll
-        for (int u = 0; u < MT_CH; ++u) {
-            const int x = tid + u * MT_WG;
-            if (x < nA) {
-                pc[u] = cA[ga + x];
-                pv[u] = vA[ga + x];
-            } else if (x < nA + nB) {
-                pc[u] = cB[gb + (x - nA)];
-                pv[u] = vB[gb + (x - nA)];
-            }
-        }
-    };
-    auto put = [&](int nA, int nB) {
-#pragma unroll
-        for (int u = 0; u < MT_CH; ++u) {
-            const int x = tid + u * MT_WG;
-            if (x < nA + nB) {
-                s_k[1 + x] = key(pc[u]);
-                s_v[1 + x] = pv[u];
-            }
-        }
-    };
-
-    if (qe > lo) {  // the short rows [lo, qe), in sub-tiles
-        const int nrows = (int)(qe - lo);
-        uint16_t* rsA = s_rs;
-        uint16_t* rsB = s_rs + MT_ROWS + 1;
-        // when one sub-tile holds all the short rows (most tiles), their entries
-        // are read in the same batch as the row pointers
-        const int64_t nA1 = (has_long ? qa : ae) - a_s, nB1 = (has_long ? qb : be) - b_s;
-        const bool one = nA1 + nB1 <= MT_CAP;
-        I pra[MT_RS], prb[MT_RS];
-#pragma unroll
-        for (int u = 0; u < MT_RS; ++u) {
-            const int k = tid + u * MT_WG;
-            if (k <= nrows) {
-                pra[u] = pA[lo + k];
-                prb[u] = pB[lo + k];
-            }
-        }
-        if (!FILL && one) load(a_s, (int)nA1, b_s, (int)nB1);
-#pragma unroll
-        for (int u = 0; u < MT_RS; ++u) {
-            const int k = tid + u * MT_WG;
-            if (k <= nrows) {
-                rsA[k] = (uint16_t)(dclamp((int64_t)pra[u] - 1, 0, nnzA) - a_s);
-                rsB[k] = (uint16_t)(dclamp((int64_t)prb[u] - 1, 0, nnzB) - b_s);
-            }
-        }
-        __syncthreads();
-        for (int r0 = 0; r0 < nrows;) {
-            // rows [r0, r1): the most whole rows from r0 whose entries fit MT_CAP
-            const int f0 = rsA[r0] + rsB[r0];
-            int r1 = r0 + 1, rh = nrows;
-            while (r1 < rh) {
-                const int mid = (r1 + rh + 1) >> 1;
-                if (rsA[mid] + rsB[mid] - f0 <= MT_CAP) r1 = mid;
-                else rh = mid - 1;
-            }
-            const int aoff = rsA[r0], boff = rsB[r0];
-            const int nA = rsA[r1] - aoff, nB = rsB[r1] - boff, tot = nA + nB, sB = 1 + nA;
-            const int64_t w0 = r0 == 0 ? wlo : (FILL ? (int64_t)ptrC[lo + r0] - 1 : 0);
-            if (FILL) {
-                stage(a_s + aoff, nA, b_s + boff, nB);
-            } else {
-                if (!(one && r0 == 0)) load(a_s + aoff, nA, b_s + boff, nB);
-                put(nA, nB);
-            }
-            if (tid == 0) {  // no entries before A's part or after B's
-                s_k[0] = MT_NONE;
-                s_v[0] = T(0);
-                s_k[sB + nB] = MT_NONE;
-                s_v[sB + nB] = T(0);
-            }
-            __syncthreads();
-            // Each thread's MT_CH consecutive entries (A's part, then B's): the
-            // row by a search for the first and a step or a search for the next
-            // ones, the packed key, and the canonical check (columns strictly
-            // increase inside every row of A and of B).  Slot 1 + x holds entry x
-            // of the concatenation; its left neighbour may already hold a packed
-            // key, whose low bits are the column key all the same.  Entry k of a
-            // part is entry off + k of the tile's part (the sub-tile starts a row).
-            int bad = 0;
-            {
-                int row = r0, part = -1;
-                uint64_t prev = 0;
-                for (int u = 0; u < MT_CH; ++u) {
-                    const int x = tid * MT_CH + u;
-                    if (x >= tot) break;
-                    const int inA = x < nA;
-                    const int k = inA ? x : x - nA;
-                    const int kt = k + (inA ? aoff : boff);
-                    const uint16_t* rs = inA ? rsA : rsB;
-                    if (inA != part) {
-                        row = mt_row_of(rs, r0, r1, kt);
-                        part = inA;
-                        prev = k > 0 ? (s_k[x] & MT_KMASK) : 0;  // slot x = entry x - 1
-                    } else if (row + 1 < r1 && rs[row + 1] <= kt) {
-                        row = mt_row_of(rs, row + 1, r1, kt);
-                    }
-                    const uint64_t kc = s_k[1 + x];
-                    bad |= kc > MT_KMASK;
-                    bad |= k > 0 && rs[row] < kt && prev >= kc;  // entry k - 1 in the same row
-                    prev = kc;
-                    s_k[1 + x] = ((uint64_t)row << MT_KB) | (kc & MT_KMASK);
-                }
-            }
-            bad = __syncthreads_or(bad);
-            if (bad) {  // every row of the sub-tile by one thread, from the operands
-                for (int64_t r = lo + r0 + tid; r < lo + r1; r += MT_WG) serial_global(r);
-            } else {
-                const int d0 = tid * MT_CH < tot ? tid * MT_CH : tot;
-                const int d1 = d0 + MT_CH < tot ? d0 + MT_CH : tot;
-                I oc[MT_CH];
-                T ov[MT_CH];
-                const unsigned keep = mt_lane_merge<T, I, RUNS>(nA, nB, d0, d1, s_k, s_v, oc, ov);
-                int total = 0;
-                const int base = mt_exclusive_sum(__popc(keep), lds8, &total);
-                // (every lane has finished its merge: the staged arrays may be reused)
-                if (!FILL) {
-                    // kept entries before each output-order position, then each
-                    // row's count as the difference at its start and end
-                    int* s_pos = reinterpret_cast<int*>(s_k);
-                    for (int u = 0; u < MT_CH; ++u)
-                        if (d0 + u < d1) s_pos[d0 + u] = base + __popc(keep & ((1u << u) - 1));
-                    if (tid == 0) s_pos[tot] = total;
-                    __syncthreads();
-                    for (int k = r0 + tid; k < r1; k += MT_WG) {
-                        const int g0 = rsA[k] + rsB[k] - f0, g1 = rsA[k + 1] + rsB[k + 1] - f0;
-                        ptrC[lo + k + 1] = (I)(s_pos[g1] - s_pos[g0]);
-                    }
-                } else {
-                    I* s_oc = reinterpret_cast<I*>(s_k);
-                    int w = base;
-                    for (int u = 0; u < MT_CH; ++u)
-                        if ((keep >> u) & 1) {
-                            s_oc[w] = oc[u];
-                            s_v[w] = ov[u];
-                            ++w;
-                        }
-                    __syncthreads();
-                    for (int k = tid; k < total; k += MT_WG) {
-                        cC[w0 + k] = s_oc[k];
-                        vC[w0 + k] = s_v[k];
-                    }
-                }
-            }
-            __syncthreads();  // before the next sub-tile (or the long row) reuses the LDS
-            r0 = r1;
-        }
-    }
-
-    if (has_long) {  // row q, in chunks of MT_CAP output-order positions
-        const int64_t a = qa, b = qb, na = ae - qa, nb = be - qb;
-        int bad = 0;
-        for (int64_t k = tid; k < na; k += MT_WG) {
-            const uint64_t kc = key(cA[a + k]);
-            bad |= kc > MT_KMASK || (k > 0 && key(cA[a + k - 1]) >= kc);
-        }
-        for (int64_t k = tid; k < nb; k += MT_WG) {
-            const uint64_t kc = key(cB[b + k]);
-            bad |= kc > MT_KMASK || (k > 0 && key(cB[b + k - 1]) >= kc);
-        }
-        bad = __syncthreads_or(bad);
-        if (bad) {
-            if (tid == 0) serial_global(q);
-            return;
-        }
-        const int64_t L = na + nb, nch = (L + MT_CAP - 1) / MT_CAP;
-        const int64_t w0 = FILL ? (int64_t)ptrC[q] - 1 : 0;
-        int64_t run = 0;
-        // chunk boundaries c0 .. c0 + MT_WG - 1, then the MT_WG - 1 chunks between them
-        for (int64_t c0 = 0; c0 < nch; c0 += MT_WG - 1) {
-            const int64_t c = c0 + tid;
-            if (c <= nch) {
-                const int64_t d = c * MT_CAP < L ? c * MT_CAP : L;
-                int64_t lo2 = d - nb > 0 ? d - nb : 0, hi2 = d < na ? d : na;
-                while (lo2 < hi2) {
-                    const int64_t mid = (lo2 + hi2) >> 1;
-                    if (key(cA[a + mid]) <= key(cB[b + d - 1 - mid])) lo2 = mid + 1;
-                    else hi2 = mid;
-                }
-                s_split[tid] = lo2;
-            }
-            __syncthreads();
-            const int64_t cend = c0 + MT_WG - 1 < nch ? c0 + MT_WG - 1 : nch;
-            for (int64_t cc = c0; cc < cend; ++cc) {
-                const int64_t d0 = cc * MT_CAP, d1 = d0 + MT_CAP < L ? d0 + MT_CAP : L;
-                const int64_t i0 = s_split[cc - c0], i1 = s_split[cc - c0 + 1];
-                const int64_t j0 = d0 - i0, j1 = d1 - i1;
-                const int nAc = (int)(i1 - i0), nBc = (int)(j1 - j0), sB = 1 + nAc;
-                if (FILL) {  // row 0: the key is the column key
-                    stage(a + i0, nAc, b + j0, nBc);
-                } else {
-                    load(a + i0, nAc, b + j0, nBc);
-                    put(nAc, nBc);
-                }
-                if (tid == 0) {  // the row's entries just outside this chunk
-                    const bool pre = i0 > 0, post = j1 < nb;
-                    s_k[0] = pre ? key(cA[a + i0 - 1]) : MT_NONE;
-                    s_v[0] = pre ? vA[a + i0 - 1] : T(0);
-                    s_k[sB + nBc] = post ? key(cB[b + j1]) : MT_NONE;
-                    s_v[sB + nBc] = post ? vB[b + j1] : T(0);
-                }
-                __syncthreads();
-                const int tot = nAc + nBc;
-                const int e0 = tid * MT_CH < tot ? tid * MT_CH : tot;
-                const int e1 = e0 + MT_CH < tot ? e0 + MT_CH : tot;
-                I oc[MT_CH];
-                T ov[MT_CH];
-                const unsigned keep = mt_lane_merge<T, I, RUNS>(nAc, nBc, e0, e1, s_k, s_v, oc, ov);
-                int total = 0;
-                const int base = mt_exclusive_sum(__popc(keep), lds8, &total);
-                if (FILL) {
-                    I* s_oc = reinterpret_cast<I*>(s_k);
-                    int w = base;
-                    for (int u = 0; u < MT_CH; ++u)
-                        if ((keep >> u) & 1) {
-                            s_oc[w] = oc[u];
-                            s_v[w] = ov[u];
-                            ++w;
-                        }
-                    __syncthreads();
-                    for (int k = tid; k < total; k += MT_WG) {
-                        cC[w0 + run + k] = s_oc[k];
-                        vC[w0 + run + k] = s_v[k];
-                    }
-                }
-                run += total;
-                __syncthreads();  // before the next chunk (or split batch) reuses the LDS
-            }
-        }
-        if (!FILL && tid == 0) ptrC[q + 1] = (I)run;
-    }
-    };
-
-    const int64_t t = xcd_tile(blockIdx.x, ntiles);
-    if (t < ntiles) tile(t);
-}
-
-// The row merge of A + B over m rows (n: the column count, for RUNS' clamp).
-template <typename T, typename I, bool RUNS, bool FILL>
-int merge_rows(hipStream_t s, int64_t m, int64_t n, int64_t nnzA, const I* pA, const I* cA,
-               const T* vA, int64_t nnzB, const I* pB, const I* cB, const T* vB, I* ptrC,
-               I* cC, T* vC) {
-    if (m <= 0) return GSA_OK;
-    const int64_t ntiles = (nnzA + nnzB + 2 * m) / MT_TILE + 1;
-    Scratch R(s);
-    GSA_TRY(R.alloc(sizeof(int64_t) * MT_REC * (ntiles + 1) + sizeof(int) * MT_FLAGS));
-    int64_t* rec = R.as<int64_t>();
-    int* flag = reinterpret_cast<int*>(rec + MT_REC * (ntiles + 1));
-    const int64_t init_threads = ntiles + 1 > MT_FLAGS ? ntiles + 1 : MT_FLAGS;
-    hipLaunchKernelGGL((k_mt_init<I>), dim3(grid_for(init_threads, 256)), dim3(256), 0, s, m,
-                       nnzA, pA, nnzB, pB, ntiles, rec, flag);
-    hipLaunchKernelGGL((k_mt_bounds<I>), dim3(grid_for(m + 1, 256)), dim3(256), 0, s, m, nnzA,
-                       pA, nnzB, pB, ntiles, rec, flag);
-    // one of the two merges runs, as the flag says; the other's workgroups return
-    hipLaunchKernelGGL((k_merge_rows<T, I, RUNS, FILL>), dim3((unsigned)cdiv(m, MR_WG)),
-                       dim3(MR_WG), 0, s, m, n, (const int*)flag, nnzA, pA, cA, vA, nnzB, pB, cB,
-                       vB, ptrC, cC, vC);
-    hipLaunchKernelGGL((k_merge_tiles<T, I, RUNS, FILL>), dim3((unsigned)xcd_tile_grid(ntiles)),
-                       dim3(MT_WG), 0, s, m, n, ntiles, (const int64_t*)rec, (const int*)flag,
-                       nnzA, pA, cA, vA, nnzB, pB, cB, vB, ptrC, cC, vC);
-    GSA_HIP(hipGetLastError());
-    return GSA_OK;
-}
-
 
 // ------------------------------------------------------------ COO helpers --
 template <typename T>

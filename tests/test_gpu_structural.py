@@ -541,14 +541,15 @@ def _rows_to_csr(rows):
 
 @pytest.mark.parametrize("itype", [torch.int64, torch.int32])
 def test_add_tiles_chunks_and_fallbacks(dev, itype):
-    """The row merge cuts the (row, column)-ordered entries into tiles of 1792
-    entries-plus-twice-the-rows, merges a tile's rows of fewer than 1792 entries
-    in sub-tiles of at most 2048 staged entries, and longer rows in chunks of 2048
-    output-order positions.  Rows just under and over those sizes, runs of rows
-    that need several sub-tiles, long rows whose equal-column pairs straddle
-    chunk boundaries (some cancelling), runs of empty rows longer than a tile,
-    and rows with repeated or unordered columns (merged one per thread, in a
-    staged sub-tile and as a long row) all equal the oracle's two-pointer merge
+    """The row merge works in row blocks (mean rows of up to 12 entries) or in
+    even tiles of 1792 entries-plus-twice-the-rows (longer mean rows); either
+    way it merges runs of whole rows of at most 2048 staged entries, longer rows
+    in chunks of 2048 output-order positions, and small blocks of short rows one
+    row per thread.  Rows just under and over those sizes, rows that need
+    several runs, long rows whose equal-column pairs straddle chunk boundaries
+    (some cancelling), runs of empty rows longer than a tile, and rows with
+    repeated or unordered columns (merged one per thread, in a staged run and as
+    a long row), through both kernels, all equal the oracle's two-pointer merge
     (CSR) and run sums (row-major COO) bit for bit."""
     rng = np.random.default_rng(77)
     n = 30_000
@@ -598,19 +599,8 @@ def test_add_tiles_chunks_and_fallbacks(dev, itype):
         pair(int(rng.integers(0, 9)), int(rng.integers(0, 9)), 0.5)
     pair(9000, 7000, 0.4)  # many chunks in one row
     empty(3)
-    A_t, B_t = _rows_to_csr(A_rows), _rows_to_csr(B_rows)
-    m = len(A_rows)
-    A = from_arrays(G.GenericSparseMatrixCSR, m, n, *A_t, dev, itype)
-    B = from_arrays(G.GenericSparseMatrixCSR, m, n, *B_t, dev, itype)
-    same(csr_tuple(A + B), O.csr_add(m, A_t, B_t))
-    same(csr_tuple(B + A), O.csr_add(m, B_t, A_t))
-    same(csr_tuple(A - A), O.csr_add(m, A_t, (A_t[0], A_t[1], -A_t[2])))
-    coo = lambda T: (np.repeat(np.arange(1, T[0].size), np.diff(T[0])), T[1], T[2])
-    Ao = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo(A_t), dev, itype)
-    Bo = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo(B_t), dev, itype)
-    same(coo_tuple(Ao + Bo), O.coo_add(m, coo(A_t), coo(B_t)))
     # one short row with an unordered pair and one long row with a repeated column:
-    # their tiles (and that long row) are merged one row per thread
+    # their runs (and that long row) are merged one row per thread
     C_rows = [(c.copy(), v.copy()) for c, v in A_rows]
     i_short = next(i for i, (c, _) in enumerate(C_rows) if 4 <= len(c) <= 11)
     c = C_rows[i_short][0]
@@ -618,10 +608,24 @@ def test_add_tiles_chunks_and_fallbacks(dev, itype):
     i_long = next(i for i, (c, _) in enumerate(C_rows) if len(c) >= 3000)
     c = C_rows[i_long][0]
     c[100] = c[99]
-    C_t = _rows_to_csr(C_rows)
-    Cm = from_arrays(G.GenericSparseMatrixCSR, m, n, *C_t, dev, itype)
-    same(csr_tuple(Cm + B), O.csr_add(m, C_t, B_t))
-    same(csr_tuple(B + Cm), O.csr_add(m, B_t, C_t))
+    coo = lambda T: (np.repeat(np.arange(1, T[0].size), np.diff(T[0])), T[1], T[2])
+    # as built (a mean of about 20 entries per row: the even tiles) and with
+    # 10000 empty rows after them (a mean of about 5: the row blocks)
+    for extra in (0, 10_000):
+        pad = [([], [])] * extra
+        A_t, B_t, C_t = (_rows_to_csr(R + pad) for R in (A_rows, B_rows, C_rows))
+        m = len(A_rows) + extra
+        A = from_arrays(G.GenericSparseMatrixCSR, m, n, *A_t, dev, itype)
+        B = from_arrays(G.GenericSparseMatrixCSR, m, n, *B_t, dev, itype)
+        same(csr_tuple(A + B), O.csr_add(m, A_t, B_t))
+        same(csr_tuple(B + A), O.csr_add(m, B_t, A_t))
+        same(csr_tuple(A - A), O.csr_add(m, A_t, (A_t[0], A_t[1], -A_t[2])))
+        Ao = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo(A_t), dev, itype)
+        Bo = from_arrays(G.GenericSparseMatrixCOO, m, n, *coo(B_t), dev, itype)
+        same(coo_tuple(Ao + Bo), O.coo_add(m, coo(A_t), coo(B_t)))
+        Cm = from_arrays(G.GenericSparseMatrixCSR, m, n, *C_t, dev, itype)
+        same(csr_tuple(Cm + B), O.csr_add(m, C_t, B_t))
+        same(csr_tuple(B + Cm), O.csr_add(m, B_t, C_t))
 
 
 def test_add_powerlaw_full_size(dev):
