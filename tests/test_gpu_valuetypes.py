@@ -148,6 +148,52 @@ def test_complex_f64_rounding_and_short_rows(dev):
 
 
 @pytest.mark.parametrize("dtype", TYPES)
+def test_add_merge_paths_every_type(dev, dtype):
+    """CSR + and row-major COO + with rows of about 100 entries (merge-path runs),
+    one row of 4500 entries (chunks, with equal pairs across chunk edges) and
+    cancellations, as built (a mean of about 200 entries per row: the even tiles)
+    and after 5000 empty rows (a mean under 12: the row blocks), bit for bit
+    against the oracle for every value type."""
+    rng = np.random.default_rng(57)
+    n = 6000
+    lensA = np.r_[rng.integers(60, 140, 40), [2500], rng.integers(0, 9, 10)]
+    lensB = np.r_[rng.integers(60, 140, 40), [2000], rng.integers(0, 9, 10)]
+    rowsA = [np.sort(rng.choice(np.arange(1, n + 1), k, replace=False)) for k in lensA]
+    # B shares about half of A's columns in every row (the long row: every other one)
+    rowsB = []
+    for ca, k in zip(rowsA, lensB):
+        own = rng.choice(np.arange(1, n + 1), k, replace=False)
+        rowsB.append(np.unique(np.r_[ca[::2][: k // 2], own[: k - k // 2]]))
+    vA = [int_valued(rng, c.size, dtype) for c in rowsA]
+    vB = [int_valued(rng, c.size, dtype) for c in rowsB]
+    for ca, va, cb, vb in zip(rowsA, vA, rowsB, vB):  # a third of the shared places cancel
+        shared = np.flatnonzero(np.isin(cb, ca))[::3]
+        vb[shared] = -va[np.searchsorted(ca, cb[shared])]
+    for extra in (0, 5000):
+        lA = [c.size for c in rowsA] + [0] * extra
+        lB = [c.size for c in rowsB] + [0] * extra
+        m = len(lA)
+        rpA = np.r_[0, np.cumsum(lA)].astype(np.int64) + 1
+        rpB = np.r_[0, np.cumsum(lB)].astype(np.int64) + 1
+        cvA, cvB = np.concatenate(rowsA).astype(np.int64), np.concatenate(rowsB).astype(np.int64)
+        nzA, nzB = np.concatenate(vA), np.concatenate(vB)
+        A = dev_mat(G.GenericSparseMatrixCSR, m, n, rpA, cvA, nzA, dev)
+        B = dev_mat(G.GenericSparseMatrixCSR, m, n, rpB, cvB, nzB, dev)
+        want = O.csr_add(m, (rpA, cvA, nzA), (rpB, cvB, nzB))
+        C = A + B
+        for got, w in zip((host(C.rowptr), host(C.colval), host(C.nzval)), want):
+            assert np.array_equal(got, w)
+        rA = np.repeat(np.arange(1, m + 1), np.diff(rpA)).astype(np.int64)
+        rB = np.repeat(np.arange(1, m + 1), np.diff(rpB)).astype(np.int64)
+        Ao = dev_mat(G.GenericSparseMatrixCOO, m, n, rA, cvA, nzA, dev)
+        Bo = dev_mat(G.GenericSparseMatrixCOO, m, n, rB, cvB, nzB, dev)
+        wantc = O.coo_add(m, (rA, cvA, nzA), (rB, cvB, nzB))
+        Co = Ao + Bo
+        for got, w in zip((host(Co.rowind), host(Co.colind), host(Co.nzval)), wantc):
+            assert np.array_equal(got, w)
+
+
+@pytest.mark.parametrize("dtype", TYPES)
 def test_structural_ops_bit_exact(dev, dtype):
     """+ (CSR, CSC, COO, with cancellation), dropzeros, kron (COO, CSR), SpGEMM
     (CSR, with a cancellation zero kept) and the conversions, bit for bit."""
