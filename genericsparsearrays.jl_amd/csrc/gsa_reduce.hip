@@ -16,7 +16,9 @@
 #include "gsa_rows.h"
 #include "gsa_scan.h"
 
+#include <algorithm>
 #include <cmath>
+#include <limits>
 
 namespace gsa {
 
@@ -73,15 +75,20 @@ __global__ __launch_bounds__(ROW_WG) void k_trace_ptr(int64_t n, int64_t nnz,
     using A = typename RAcc<T>::A;
     __shared__ A sh[ROW_WG / 64];
     A acc = A(0);
-    visit_owned_entries<I>(
-        n, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {},
-        [&](int64_t r, int64_t j) {
-            if ((int64_t)idx[j] - 1 == r) acc = acc + to_acc(val[j]);
-        },
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG)
-                if ((int64_t)idx[j] - 1 == r) acc = acc + to_acc(val[j]);
-        });
+    // the index and the value, read with the plan (the value of every entry:
+    // a diagonal entry's value line is nearly always fetched anyway)
+    struct E {
+        I c;
+        T v;
+    };
+    auto pre = [&](int64_t j) { return E{idx[j], val[j]}; };
+    auto load = [](int64_t, int64_t, const E& x) { return x; };
+    auto none = [](int64_t, int64_t, E&) {};
+    auto use = [&](int64_t r, int64_t, const E& x) {
+        if ((int64_t)x.c - 1 == r) acc = acc + to_acc(x.v);
+    };
+    visit_owned_entries<I>(n, nnz, ptr, blk_row, [](int64_t, int64_t, int64_t) {}, pre,
+                           lane_rounds<ENT_PER_LANE>(load, none, use));
     const A t = wg_sum_fixed(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -123,19 +130,29 @@ __global__ __launch_bounds__(ROW_WG) void k_dot3_ptr(int64_t nmaj, int64_t nidx,
     using A = typename RAcc<T>::A;
     __shared__ A sh[ROW_WG / 64];
     A acc = A(0);
-    auto term = [&](int64_t r, int64_t j) {
-        const int64_t i = dclamp((int64_t)idx[j] - 1, 0, nidx - 1);
-        const T xv = conj_v(major_left ? x[r] : x[i]);
-        const T yv = major_left ? y[i] : y[r];
-        const T t = (xv * conj_if(val[j], cja != 0)) * yv;
+    // the index and the value, read with the plan; then both vectors' entries
+    // (the major's, and the other's gathered through the index) in one round
+    struct P {
+        I c;
+        T a;
+    };
+    struct E {
+        T a, wr, wi;
+    };
+    auto pre = [&](int64_t j) { return P{idx[j], val[j]}; };
+    auto load = [&](int64_t r, int64_t, const P& p) {
+        const int64_t i = dclamp((int64_t)p.c - 1, 0, nidx - 1);
+        return E{p.a, major_left ? x[r] : y[r], major_left ? y[i] : x[i]};
+    };
+    auto none = [](int64_t, int64_t, E&) {};
+    auto use = [&](int64_t, int64_t, const E& e) {
+        const T xv = conj_v(major_left ? e.wr : e.wi);
+        const T yv = major_left ? e.wi : e.wr;
+        const T t = (xv * conj_if(e.a, cja != 0)) * yv;
         acc = acc + to_acc(t);
     };
-    visit_owned_entries<I>(
-        nmaj, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {},
-        [&](int64_t r, int64_t j) { term(r, j); },
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) term(r, j);
-        });
+    visit_owned_entries<I>(nmaj, nnz, ptr, blk_row, [](int64_t, int64_t, int64_t) {}, pre,
+                           lane_rounds<ENT_PER_LANE / 2>(load, none, use));
     const A t = wg_sum_fixed(acc, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
@@ -195,75 +212,124 @@ __global__ __launch_bounds__(ROW_WG) void k_symmetric_ptr(int64_t n, int64_t nnz
                                                           int hermitian,
                                                           const int64_t* __restrict__ blk_row,
                                                           int* __restrict__ flag) {
-    auto check = [&](int64_t r, int64_t j) {
-        const int64_t c = (int64_t)idx[j] - 1;
-        if (c == r) {
-            if (hermitian && val[j] != conj_v(val[j])) *flag = 0;
-            return;
-        }
-        bool ok = false;
-        if (c >= 0 && c < n) {
-            int64_t lo = (int64_t)ptr[c], hi = (int64_t)ptr[c + 1] - 1;  // one-based, inclusive
-            while (lo <= hi) {
-                const int64_t mid = lo + (hi - lo) / 2;
-                const int64_t mc = (int64_t)idx[dclamp(mid - 1, 0, nnz - 1)] - 1;
-                if (mc == r) {
-                    const T tv = val[dclamp(mid - 1, 0, nnz - 1)];
-                    ok = val[j] == (hermitian ? conj_v(tv) : tv);
-                    break;
+    // G entries per lane searched in lock step: every round of the binary
+    // searches issues its G index loads together.  Each search probes the
+    // same positions as a search of its own would, so it finds the same entry.
+    constexpr int G = ENT_PER_LANE / 2;
+    struct E {
+        I c;
+        T v;
+    };
+    auto pre = [&](int64_t j) { return E{idx[j], val[j]}; };
+    auto batch = [&](const int64_t* rr, const int64_t*, const bool* okk, const E* pe, int64_t) {
+        bool fail = false;
+#pragma unroll
+        for (int g = 0; g < ENT_PER_LANE; g += G) {
+            const int64_t* r = rr + g;
+            const bool* ok = okk + g;
+            const E* p = pe + g;
+            int64_t lo[G], hi[G], pos[G];
+            bool act[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int64_t cc = (int64_t)p[u].c - 1;
+                if (ok[u] && cc == r[u] && hermitian && p[u].v != conj_v(p[u].v)) fail = true;
+                const bool off = ok[u] && cc != r[u];
+                act[u] = off && cc >= 0 && cc < n;
+                if (off && !act[u]) fail = true;  // an index outside the matrix: no mirror
+                const int64_t cs = act[u] ? cc : 0;
+                lo[u] = (int64_t)ptr[cs];  // one-based, inclusive
+                hi[u] = (int64_t)ptr[cs + 1] - 1;
+                pos[u] = -1;
+            }
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                act[u] = act[u] && lo[u] <= hi[u];
+                any |= act[u];
+            }
+            for (int round = 0; any && round < 64; ++round) {
+                int64_t mid[G];
+                I mc[G];
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    mid[u] = lo[u] + (hi[u] - lo[u]) / 2;
+                    mc[u] = idx[act[u] ? dclamp(mid[u] - 1, 0, nnz - 1) : 0];
                 }
-                if (mc < r) lo = mid + 1;
-                else hi = mid - 1;
+                any = false;
+#pragma unroll
+                for (int u = 0; u < G; ++u) {
+                    if (!act[u]) continue;
+                    const int64_t m1 = (int64_t)mc[u] - 1;
+                    if (m1 == r[u]) {
+                        pos[u] = dclamp(mid[u] - 1, 0, nnz - 1);
+                        act[u] = false;
+                    } else {
+                        if (m1 < r[u]) lo[u] = mid[u] + 1;
+                        else hi[u] = mid[u] - 1;
+                        act[u] = lo[u] <= hi[u];
+                    }
+                    any |= act[u];
+                }
+            }
+            T tv[G];
+#pragma unroll
+            for (int u = 0; u < G; ++u) tv[u] = val[pos[u] >= 0 ? pos[u] : 0];
+#pragma unroll
+            for (int u = 0; u < G; ++u) {
+                const int64_t cc = (int64_t)p[u].c - 1;
+                if (!ok[u] || cc == r[u] || cc < 0 || cc >= n) continue;
+                if (pos[u] < 0 || p[u].v != (hermitian ? conj_v(tv[u]) : tv[u])) fail = true;
             }
         }
-        if (!ok) *flag = 0;
+        if (fail) *flag = 0;
     };
-    visit_owned_entries<I>(
-        n, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {},
-        [&](int64_t r, int64_t j) { check(r, j); },
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) check(r, j);
-        });
+    visit_owned_entries<I>(n, nnz, ptr, blk_row, [](int64_t, int64_t, int64_t) {}, pre, batch);
 }
 
 // ---- A + λI ------------------------------------------------------------------
 // Per major: one more entry when none of its entries sits on the diagonal
 // (kernel_count_diag_entries_csr!, src/matrix_csr/matrix_csr_kernels.jl:445-463).
-// Majors of up to ROW_SHORT entries: length + 1 by the major's lane, then
-// length by any diagonal entry (entry-parallel); longer ones by the workgroup.
+// cnt[r + 1] starts at the largest value (k_count_fill); the major's lane then
+// takes it down to length + 1 and any diagonal entry to length, with atomic
+// minima: the entries of a major can fall in other workgroups than its own,
+// and the result does not depend on the order the writes arrive in.
+template <typename I>
+__global__ void k_count_fill(int64_t n, I* __restrict__ cnt) {
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < n;
+         r += (int64_t)gridDim.x * blockDim.x)
+        cnt[r + 1] = std::numeric_limits<I>::max();
+}
+
+template <typename I>
+__device__ __forceinline__ void atomic_min_i(I* p, I v) {
+    if constexpr (sizeof(I) == 8) atomicMin(reinterpret_cast<long long*>(p), (long long)v);
+    else atomicMin(reinterpret_cast<int*>(p), (int)v);
+}
+
 template <typename I>
 __global__ __launch_bounds__(ROW_WG) void k_diag_missing(int64_t n, int64_t nnz,
                                                          const I* __restrict__ ptr,
                                                          const I* __restrict__ idx,
                                                          const int64_t* __restrict__ blk_row,
                                                          I* __restrict__ cnt) {
+    // a diagonal entry reads its major's length (a second round, taken by the
+    // diagonal entries alone)
+    struct E {
+        I c, len;
+    };
+    auto pre = [&](int64_t j) { return E{idx[j], I(0)}; };
+    auto load = [](int64_t, int64_t, const E& x) { return x; };
+    auto fetch = [&](int64_t r, int64_t, E& x) {
+        if ((int64_t)x.c - 1 == r) x.len = (I)((int64_t)ptr[r + 1] - (int64_t)ptr[r]);
+    };
+    auto use = [&](int64_t r, int64_t, const E& x) {
+        if ((int64_t)x.c - 1 == r) atomic_min_i(cnt + r + 1, x.len);
+    };
     visit_owned_entries<I>(
         n, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) { cnt[r + 1] = (I)((e - s) + 1); },
-        [&](int64_t r, int64_t j) {  // a diagonal entry: nothing to insert
-            if ((int64_t)idx[j] - 1 == r) cnt[r + 1] = (I)((int64_t)ptr[r + 1] - (int64_t)ptr[r]);
-        },
-        [&](int64_t r, int64_t s, int64_t e) {
-            int f = 0;
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) f |= (int64_t)idx[j] - 1 == r;
-            f = __syncthreads_or(f);
-            if (threadIdx.x == 0) cnt[r + 1] = (I)((e - s) + (f ? 0 : 1));
-        });
-}
-
-// The smallest value over the workgroup (every lane calls it).
-__device__ __forceinline__ int64_t wg_min(int64_t v, int64_t* sh) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        const int64_t u = __shfl_xor(v, o, 64);
-        v = u < v ? u : v;
-    }
-    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
-    __syncthreads();
-    int64_t t = sh[0];
-    for (int w = 1; w < ROW_WG / 64; ++w) t = sh[w] < t ? sh[w] : t;
-    __syncthreads();
-    return t;
+        [&](int64_t r, int64_t s, int64_t e) { atomic_min_i(cnt + r + 1, (I)((e - s) + 1)); }, pre,
+        lane_rounds<ENT_PER_LANE>(load, fetch, use));
 }
 
 // The merge of kernel_add_uniformscaling_csr! (matrix_csr_kernels.jl:466-509)
@@ -283,12 +349,42 @@ __global__ __launch_bounds__(ROW_WG) void k_add_diag(int64_t n, int64_t nnz,
                                                      const I* __restrict__ ptr_out,
                                                      I* __restrict__ idx_out,
                                                      T* __restrict__ val_out) {
-    __shared__ int64_t sh[ROW_WG / 64];
     auto put = [&](int64_t w, int64_t wend, I c, T v) {
         if (w < wend) {
             idx_out[w] = c;
             val_out[w] = v;
         }
+    };
+    // the entry read with the plan; then its major's pointers (old and new);
+    // then the previous index, for an entry past the diagonal of a major
+    // missing one
+    struct P {
+        I c;
+        T v;
+    };
+    struct E {
+        I c, ps, pe, os, oe, prev;
+        T v;
+    };
+    auto pre = [&](int64_t j) { return P{idx[j], val[j]}; };
+    auto load = [&](int64_t r, int64_t, const P& p) {
+        return E{p.c, ptr[r], ptr[r + 1], ptr_out[r], ptr_out[r + 1], I(0), p.v};
+    };
+    auto fetch = [&](int64_t r, int64_t j, E& x) {
+        const int64_t s = (int64_t)x.ps - 1, e = (int64_t)x.pe - 1;
+        const bool missing = (int64_t)x.oe - (int64_t)x.os > e - s;
+        if (missing && (int64_t)x.c - 1 > r && j > s) x.prev = idx[j - 1];
+    };
+    auto use = [&](int64_t r, int64_t j, const E& x) {
+        const int64_t s = (int64_t)x.ps - 1, e = (int64_t)x.pe - 1;
+        const int64_t base = (int64_t)x.os - 1, wend = (int64_t)x.oe - 1;
+        const bool missing = wend - base > e - s;
+        const int64_t c = (int64_t)x.c - 1;
+        const bool past = missing && c > r;
+        const int64_t w = base + (j - s) + (past ? 1 : 0);
+        put(w, wend, x.c, c == r ? x.v + lambda : x.v);
+        if (past && (j == s || (int64_t)x.prev - 1 < r)) put(w - 1, wend, (I)(r + 1), lambda);
+        if (missing && !past && j == e - 1) put(w + 1, wend, (I)(r + 1), lambda);
     };
     visit_owned_entries<I>(
         n, nnz, ptr, blk_row,
@@ -296,35 +392,7 @@ __global__ __launch_bounds__(ROW_WG) void k_add_diag(int64_t n, int64_t nnz,
             if (s == e && ptr_out[r + 1] > ptr_out[r])
                 put((int64_t)ptr_out[r] - 1, (int64_t)ptr_out[r + 1] - 1, (I)(r + 1), lambda);
         },
-        [&](int64_t r, int64_t j) {
-            const int64_t s = (int64_t)ptr[r] - 1, e = (int64_t)ptr[r + 1] - 1;
-            const int64_t base = (int64_t)ptr_out[r] - 1, wend = (int64_t)ptr_out[r + 1] - 1;
-            const bool missing = wend - base > e - s;
-            const int64_t c = (int64_t)idx[j] - 1;
-            const bool past = missing && c > r;
-            const int64_t w = base + (j - s) + (past ? 1 : 0);
-            put(w, wend, idx[j], c == r ? val[j] + lambda : val[j]);
-            if (past && (j == s || (int64_t)idx[j - 1] - 1 < r)) put(w - 1, wend, (I)(r + 1), lambda);
-            if (missing && !past && j == e - 1) put(w + 1, wend, (I)(r + 1), lambda);
-        },
-        [&](int64_t r, int64_t s, int64_t e) {
-            const int64_t base = (int64_t)ptr_out[r] - 1, wend = (int64_t)ptr_out[r + 1] - 1;
-            int f = 0;
-            int64_t first_past = e;
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) {
-                const int64_t c = (int64_t)idx[j] - 1;
-                f |= c == r;
-                if (c > r && j < first_past) first_past = j;
-            }
-            f = __syncthreads_or(f);
-            const int64_t p = f ? e : wg_min(first_past, sh);
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) {
-                const int64_t c = (int64_t)idx[j] - 1;
-                put(base + (j - s) + (!f && j >= p ? 1 : 0), wend, idx[j],
-                    c == r ? val[j] + lambda : val[j]);
-            }
-            if (!f && threadIdx.x == 0) put(base + (p - s), wend, (I)(r + 1), lambda);
-        });
+        pre, lane_rounds<ENT_PER_LANE / 4>(load, fetch, use));
 }
 
 __global__ void k_set_flag(int* f) { *f = 1; }
@@ -457,16 +525,20 @@ __global__ __launch_bounds__(ROW_WG) void k_add_dense_ptr(int64_t nmaj, int64_t 
                                                           const T* __restrict__ val, int cm,
                                                           int cj, const int64_t* __restrict__ blk_row,
                                                           T* __restrict__ C, int64_t ldc) {
-    auto add = [&](int64_t r, int64_t j) {
-        const int64_t q = dclamp((int64_t)idx[j] - 1, 0, nmin - 1);
-        const int64_t at = cm ? q + r * ldc : r + q * ldc;
-        atomic_add_v(C + at, conj_if(val[j], cj != 0));
+    struct E {
+        I c;
+        T v;
     };
-    visit_owned_entries<I>(
-        nmaj, nnz, ptr, blk_row, [&](int64_t, int64_t, int64_t) {}, add,
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) add(r, j);
-        });
+    auto pre = [&](int64_t j) { return E{idx[j], val[j]}; };
+    auto load = [](int64_t, int64_t, const E& x) { return x; };
+    auto none = [](int64_t, int64_t, E&) {};
+    auto add = [&](int64_t r, int64_t, const E& x) {
+        const int64_t q = dclamp((int64_t)x.c - 1, 0, nmin - 1);
+        const int64_t at = cm ? q + r * ldc : r + q * ldc;
+        atomic_add_v(C + at, conj_if(x.v, cj != 0));
+    };
+    visit_owned_entries<I>(nmaj, nnz, ptr, blk_row, [](int64_t, int64_t, int64_t) {}, pre,
+                           lane_rounds<ENT_PER_LANE>(load, none, add));
 }
 
 template <typename T, typename I>
@@ -693,6 +765,8 @@ int gsa_add_diag_count(void* stream, int itype, int64_t n, int64_t nnz, const vo
             const int64_t* br = nullptr;
             int64_t nb = 0;
             GSA_TRY(row_blocks(s, plan, itype, n, nnz, ptr, tmp, &br, &nb));
+            hipLaunchKernelGGL(k_count_fill<I>, dim3((unsigned)std::min<int64_t>(cdiv(n, 256), 4096)),
+                               dim3(256), 0, s, n, (I*)ptr_out);
             hipLaunchKernelGGL(k_diag_missing<I>, dim3((unsigned)nb), dim3(ROW_WG), 0, s, n, nnz,
                                (const I*)ptr, (const I*)idx, br, (I*)ptr_out);
             GSA_HIP(hipGetLastError());

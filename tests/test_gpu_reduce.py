@@ -397,3 +397,91 @@ def test_sparse_plus_dense(dev, dtype, itype):
     if dtype == np.float64:
         with pytest.raises(G.ArgumentError):  # Float64 values into a Float32 dense copy
             A + torch.zeros((m, n), dtype=torch.float32, device=dev)
+
+
+# ------------------------------------------------ entry blocks across rows --
+def _block_crossing_matrix(dtype):
+    """A 12000 x 12000 CSR whose rows cross the 2048-entry blocks of the
+    entry-parallel traversals in every way: a 5000-entry row with its diagonal
+    missing (lambda goes in front), rows whose diagonal or first index past it
+    lies blocks away from the row's start, a block owning no row (inside a long
+    row), runs of empty rows (also at the end), a long row whose indices all
+    lie left of the diagonal (lambda appended), and short rows between."""
+    rng = np.random.default_rng(41)
+    n = 12000
+    rows = {}
+    rows[0] = np.arange(1, 5001)                               # diagonal missing, all past
+    rows[1] = np.concatenate([[0], np.arange(2, 3001)])       # first past at the 2nd entry
+    rows[2] = np.arange(0, 4500)                               # diagonal at offset 2
+    rows[3] = np.arange(0, 64)                                 # short, diagonal present
+    for r in range(4, 40):                                     # short rows, some with diagonals
+        k = int(rng.integers(1, 70))
+        rows[r] = np.unique(np.concatenate([rng.choice(n, k, replace=False), [r] if r % 3 else []]))
+    for r in range(40, 2040):                                  # a run of empty rows
+        rows[r] = np.zeros(0, np.int64)
+    rows[2040] = np.concatenate([np.arange(0, 2040), np.arange(2041, 6200)])  # lambda 2040 in
+    rows[2041] = np.arange(0, 2041 + 1)                        # diagonal is the last entry
+    rows[2042] = np.arange(0, 1700)                            # all left of the diagonal: appended
+    rows[2043] = np.arange(7000, 11100)                        # all past, crosses two blocks
+    for r in range(2044, 2300):
+        k = int(rng.integers(1, 12))
+        rows[r] = np.unique(np.concatenate([rng.choice(n, k, replace=False), [r] if r % 2 else []]))
+    for r in range(2300, n):                                   # trailing empty rows
+        rows[r] = np.zeros(0, np.int64)
+    lens = np.array([len(rows[r]) for r in range(n)])
+    ptr = np.concatenate([[0], np.cumsum(lens)])
+    col = np.concatenate([np.sort(rows[r]).astype(np.int64) for r in range(n)])
+    val = rng.standard_normal(len(col)).astype(dtype)
+    S = sp.csr_matrix((val, col, ptr), shape=(n, n))
+    assert S.has_sorted_indices and S.nnz > 5 * 2048
+    return S
+
+
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+def test_entry_blocks_cross_rows(dev, itype):
+    """tr, dot, A + lambda*I (count and fill), sparse + dense and issymmetric where
+    rows span several entry blocks, against the oracle (and numpy for + dense)."""
+    dtype = torch.float64
+    S = _block_crossing_matrix(np.float64)
+    n = S.shape[0]
+    p, i, v = csr_arrays(S)
+    A = to_dev(G.GenericSparseMatrixCSR, n, n, (p, i, v), dev, dtype, itype)
+    Ac = to_dev(G.GenericSparseMatrixCSC, n, n, csc_arrays(S), dev, dtype, itype)
+    for cached in (False, True):
+        if cached:
+            A.plan()
+            Ac.plan()
+        t, bound = O.trace_ptr(n, p, i, v)
+        close(G.tr(A), t, bound, dtype)
+        close(G.tr(Ac), t, bound, dtype)
+        exp = O.add_diag(n, p, i, v, 2.5)
+        _same(_ptr_tuple(A + G.UniformScaling(2.5)), exp)
+        pc, ic, vc = csc_arrays(S)
+        _same(_ptr_tuple(Ac + G.UniformScaling(2.5)), O.add_diag(n, pc, ic, vc, 2.5))
+        rng = np.random.default_rng(42)
+        x, y = rng.standard_normal(n), rng.standard_normal(n)
+        ref, b2 = O.dot3_ptr(n, p, i, v, x, y, True)
+        xd, yd = torch.as_tensor(x).to(dev), torch.as_tensor(y).to(dev)
+        close(G.dot(xd, A, yd), ref, b2, dtype)
+        close(G.dot(xd, Ac, yd), ref, b2, dtype)
+    # sparse + dense on the rows' leading block of columns (a dense 12000 x 12000
+    # would be 1.1 GB): the same rows, columns cut to the first 3000
+    Sc = sp.csr_matrix(S[:, :3000])
+    Sc.sort_indices()
+    Ad = to_dev(G.GenericSparseMatrixCSR, n, 3000, csr_arrays(Sc), dev, dtype, itype)
+    Bh = np.random.default_rng(43).standard_normal((n, 3000))
+    got = Ad + torch.as_tensor(Bh, device=dev)
+    assert np.array_equal(host(got), Bh + Sc.toarray())
+    # symmetry: S + S' is symmetric; one changed value deep in a long row is not
+    Ssym = sp.csr_matrix(S + S.T)
+    Ssym.sort_indices()
+    Sbad = Ssym.copy()
+    j = Sbad.indptr[2040] + 4000
+    Sbad.data[j] += 1.0
+    for M, expect in ((Ssym, True), (Sbad, False)):
+        pp, ii, vv = csr_arrays(M)
+        assert O.symmetric_ptr(n, pp, ii, vv) == expect
+        D = to_dev(G.GenericSparseMatrixCSR, n, n, (pp, ii, vv), dev, dtype, itype)
+        assert G.issymmetric(D) is expect
+        D.plan()
+        assert G.issymmetric(D) is expect
