@@ -1,8 +1,8 @@
 // gsa_convert.hip — CSR/CSC <-> COO and CSR <-> CSC on gfx950.
 //
 // CSR -> COO expands the row pointer (kernel_csr_to_coo!,
-// src/conversions/conversion_kernels.jl:20-35) with a row-block plan so long
-// rows are written by a whole workgroup.  COO -> CSR is the reference's
+// src/conversions/conversion_kernels.jl:20-35) over fixed blocks of entries of
+// a row-block plan, so long rows are split over workgroups.  COO -> CSR is the reference's
 // keyed stable sort (conversions.jl:282-323): 64-bit keys (row-1)*n + (col-1)
 // — the same order as the reference's Ti keys rowind*n + colind, without their
 // Int32 overflow — radix-sorted stably with each entry's (column, value) pair
@@ -16,18 +16,22 @@
 
 namespace gsa {
 
+struct NoPre {};
+
+// Each workgroup's 2048 entries get their rows from the LDS row map of
+// visit_owned_entries and store them coalesced: a long row is split over the
+// workgroups its entries fall in.
 template <typename I>
 __global__ __launch_bounds__(ROW_WG) void k_expand_rows(int64_t m, int64_t nnz,
                                                         const I* __restrict__ ptr,
                                                         const int64_t* __restrict__ blk_row,
                                                         I* __restrict__ out) {
-    visit_owned_rows<I>(
-        m, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s; j < e; ++j) out[j] = (I)(r + 1);
-        },
-        [&](int64_t r, int64_t s, int64_t e) {
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) out[j] = (I)(r + 1);
+    visit_owned_entries<I>(
+        m, nnz, ptr, blk_row, [](int64_t, int64_t, int64_t) {}, [](int64_t) { return NoPre{}; },
+        [&](const int64_t* r, const int64_t* j, const bool* ok, const NoPre*, int64_t) {
+#pragma unroll
+            for (int u = 0; u < ENT_PER_LANE; ++u)
+                if (ok[u]) out[j[u]] = (I)(r[u] + 1);
         });
 }
 
