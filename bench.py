@@ -209,7 +209,17 @@ def bench_structural(H, dev):
     t = timed(lambda: A + At, out["add_powerlaw_A_At"])
     out["add_powerlaw_A_At"]["GBs"] = round((2 * nnz * 16 + 2 * (m + 1) * 8 + Cat.nnz() * 16
                                              + (m + 1) * 8) / t / 1e9, 1)
-    del Cat, At, Ac, Ao, Aoc, A
+    # dropzeros on the same matrix with every tenth stored value set to zero
+    # (count, allocation, fill: rows spanning many 2048-entry blocks included)
+    Z = A.nzval.clone()
+    Z[::10] = 0
+    Az = G.GenericSparseMatrixCSR(m, n, A.rowptr, A.colval, Z, copy=False)
+    Dz = G.dropzeros(Az)
+    out["dropzeros_powerlaw_tenth"] = {"nnzC": Dz.nnz()}
+    t = timed(lambda: G.dropzeros(Az), out["dropzeros_powerlaw_tenth"])
+    out["dropzeros_powerlaw_tenth"]["GBs"] = round((nnz * 16 + (m + 1) * 8 + Dz.nnz() * 16
+                                                    + (m + 1) * 8) / t / 1e9, 1)
+    del Cat, At, Ac, Ao, Aoc, A, Z, Az, Dz
     # host ingest: GenericSparseMatrixCSR(::SparseMatrixCSC) of the same matrix held
     # as a scipy CSC (the arrays copied as they are, transposed on the device),
     # against the host transposition the reference performs (scipy's tocsr)
@@ -323,6 +333,11 @@ def cpu_structural(H):
     C, t = tm(lambda: O.csr_add(m, (H.rowptr, H.colval, H.nzval), c))
     out["add_powerlaw_A_At"] = (t, 2 * nnz * 16 + 2 * (m + 1) * 8 + C[2].size * 16 + (m + 1) * 8)
     del C
+    Z = H.nzval.copy()
+    Z[::10] = 0
+    C, t = tm(lambda: O.csr_dropzeros(m, H.rowptr, H.colval, Z))
+    out["dropzeros_powerlaw_tenth"] = (t, nnz * 16 + (m + 1) * 8 + C[2].size * 16 + (m + 1) * 8)
+    del C, Z
     co, t = tm(lambda: O.csr_to_coo(m, H.rowptr, H.colval, H.nzval))
     out["csr_to_coo"] = (t, nnz * 16 + (m + 1) * 8 + nnz * 24)
     _, t = tm(lambda: O.coo_to_csr(m, n, co[0], co[1], co[2]))

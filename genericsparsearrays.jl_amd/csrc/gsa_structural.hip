@@ -37,57 +37,130 @@ namespace gsa {
 int expand_rows(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr, void* out);
 
 // ------------------------------------------------------------ dropzeros ----
-template <typename T, typename I>
-__global__ __launch_bounds__(ROW_WG) void k_dz_count(int64_t m, int64_t nnz,
-                                                     const I* __restrict__ ptr,
-                                                     const T* __restrict__ val,
-                                                     const int64_t* __restrict__ blk_row,
-                                                     I* __restrict__ counts) {
-    __shared__ int64_t lds4[ROW_WG / 64];
-    visit_owned_rows<I>(
-        m, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) {
-            int64_t c = 0;
-            for (int64_t j = s; j < e; ++j) c += (val[j] != T(0));
-            counts[r] = (I)c;
-        },
-        [&](int64_t r, int64_t s, int64_t e) {
-            int64_t c = 0;
-            for (int64_t j = s + threadIdx.x; j < e; j += ROW_WG) c += (val[j] != T(0));
-            int64_t total;
-            (void)wg_exclusive_sum<int64_t>(c, lds4, &total);
-            if (threadIdx.x == 0) counts[r] = (I)total;
-        });
+// Order-preserving stream compaction over the stored entries
+// (kernel_dropzeros_count / _fill, src/matrix_csr/matrix_csr.jl:723-745): rows
+// are contiguous in storage, so a kept entry's place in the result is the
+// number of kept entries before it, and a row's new pointer is that number at
+// the row's start.  Workgroup b takes the entries [2048b, 2048b + 2048) (the
+// row-block plan's blocks), ENT_PER_LANE per lane at j = 2048b + threadIdx.x +
+// u * ROW_WG.
+
+// Each lane's ENT_PER_LANE keep flags, and each one's rank among the kept
+// entries of the workgroup's block (entry order), from wavefront ballots and
+// one barrier; *total gets the block's kept count.
+template <typename T>
+__device__ __forceinline__ void dz_block_ranks(const T* __restrict__ val, int64_t lo, int64_t hi,
+                                               bool (&keep)[ENT_PER_LANE],
+                                               int (&rank)[ENT_PER_LANE], int* total) {
+    __shared__ int wcnt[ENT_PER_LANE][ROW_WG / 64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    T v[ENT_PER_LANE];
+#pragma unroll
+    for (int u = 0; u < ENT_PER_LANE; ++u) {
+        const int64_t j = lo + threadIdx.x + (int64_t)u * ROW_WG;
+        v[u] = val[j < hi ? j : lo];
+    }
+    int below[ENT_PER_LANE];
+#pragma unroll
+    for (int u = 0; u < ENT_PER_LANE; ++u) {
+        const int64_t j = lo + threadIdx.x + (int64_t)u * ROW_WG;
+        keep[u] = j < hi && v[u] != T(0);
+        const uint64_t bal = __ballot(keep[u]);
+        below[u] = __popcll(bal & ((1ull << lane) - 1ull));
+        if (lane == 0) wcnt[u][wave] = __popcll(bal);
+    }
+    __syncthreads();
+    int run = 0;
+#pragma unroll
+    for (int u = 0; u < ENT_PER_LANE; ++u) {
+        int before = 0;
+#pragma unroll
+        for (int w = 0; w < ROW_WG / 64; ++w) before += w < wave ? wcnt[u][w] : 0;
+        rank[u] = run + before + below[u];
+#pragma unroll
+        for (int w = 0; w < ROW_WG / 64; ++w) run += wcnt[u][w];
+    }
+    *total = run;
 }
 
+// Kept entries of each block.
+template <typename T>
+__global__ __launch_bounds__(ROW_WG) void k_dz_block_sums(int64_t nnz, const T* __restrict__ val,
+                                                          int64_t* __restrict__ bsum) {
+    const int64_t lo = blockIdx.x * (int64_t)ENT_CAP;
+    const int64_t hi = lo + ENT_CAP < nnz ? lo + ENT_CAP : nnz;
+    bool keep[ENT_PER_LANE];
+    int rank[ENT_PER_LANE], total = 0;
+    if (lo < hi) dz_block_ranks<T>(val, lo, hi, keep, rank, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// The new row pointer of every row starting in the block (rows at the end of
+// the entries, s = nnz, belong to the last block): 1 + the kept entries before
+// the row's start.  incl is the inclusive scan of the block sums.
 template <typename T, typename I>
-__global__ __launch_bounds__(ROW_WG) void k_dz_fill(int64_t m, int64_t nnz,
-                                                    const I* __restrict__ ptr,
-                                                    const I* __restrict__ idx,
-                                                    const T* __restrict__ val,
-                                                    const int64_t* __restrict__ blk_row,
-                                                    const I* __restrict__ ptr_out,
-                                                    I* __restrict__ idx_out,
-                                                    T* __restrict__ val_out) {
-    __shared__ int64_t lds4[ROW_WG / 64];
-    visit_owned_rows<I>(
-        m, nnz, ptr, blk_row,
-        [&](int64_t r, int64_t s, int64_t e) {
-            int64_t w = (int64_t)ptr_out[r] - 1;
-            for (int64_t j = s; j < e; ++j)
-                if (val[j] != T(0)) { idx_out[w] = idx[j]; val_out[w] = val[j]; ++w; }
-        },
-        [&](int64_t r, int64_t s, int64_t e) {
-            int64_t w0 = (int64_t)ptr_out[r] - 1;
-            for (int64_t base = s; base < e; base += ROW_WG) {
-                int64_t j = base + threadIdx.x;
-                bool keep = j < e && val[j] != T(0);
-                int64_t total;
-                int64_t off = wg_exclusive_sum<int64_t>(keep ? 1 : 0, lds4, &total);
-                if (keep) { idx_out[w0 + off] = idx[j]; val_out[w0 + off] = val[j]; }
-                w0 += total;
-            }
-        });
+__global__ __launch_bounds__(ROW_WG) void k_dz_rowptr(int64_t m, int64_t nnz,
+                                                      const I* __restrict__ ptr,
+                                                      const T* __restrict__ val,
+                                                      const int64_t* __restrict__ blk_row,
+                                                      const int64_t* __restrict__ incl,
+                                                      I* __restrict__ ptr_out) {
+    __shared__ int pos[ENT_CAP + 1];
+    const int64_t b = blockIdx.x;
+    const int64_t lo = b * (int64_t)ENT_CAP;
+    const int64_t hi = lo + ENT_CAP < nnz ? lo + ENT_CAP : nnz;
+    const int64_t base = b > 0 ? incl[b - 1] : 0;
+    bool keep[ENT_PER_LANE];
+    int rank[ENT_PER_LANE], total = 0;
+    if (lo < hi) dz_block_ranks<T>(val, lo, hi, keep, rank, &total);
+#pragma unroll
+    for (int u = 0; u < ENT_PER_LANE; ++u) {
+        // the kept entries of the block before entry k: the rank of a kept
+        // entry, or of the next kept one
+        const int k = threadIdx.x + u * ROW_WG;
+        pos[k] = lo < hi ? rank[u] : 0;
+    }
+    if (threadIdx.x == 0) pos[ENT_CAP] = total;
+    __syncthreads();
+    const int64_t r0 = dclamp(blk_row[b], 0, m);
+    const int64_t r1 = dclamp(blk_row[b + 1], r0, m);
+    for (int64_t r = r0 + threadIdx.x; r < r1; r += ROW_WG) {
+        const int64_t k = dclamp(dclamp((int64_t)ptr[r] - 1, 0, nnz) - lo, 0, ENT_CAP);
+        ptr_out[r] = (I)(1 + base + pos[k]);
+    }
+    if (b == (int64_t)gridDim.x - 1 && threadIdx.x == 0) ptr_out[m] = (I)(1 + base + total);
+}
+
+// The kept entries, each to its block's offset plus its rank in the block.
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_dz_compact(int64_t nnz, const I* __restrict__ idx,
+                                                       const T* __restrict__ val,
+                                                       const int64_t* __restrict__ incl,
+                                                       I* __restrict__ idx_out,
+                                                       T* __restrict__ val_out) {
+    const int64_t b = blockIdx.x;
+    const int64_t lo = b * (int64_t)ENT_CAP;
+    const int64_t hi = lo + ENT_CAP < nnz ? lo + ENT_CAP : nnz;
+    if (lo >= hi) return;
+    const int64_t base = b > 0 ? incl[b - 1] : 0;
+    bool keep[ENT_PER_LANE];
+    int rank[ENT_PER_LANE], total = 0;
+    dz_block_ranks<T>(val, lo, hi, keep, rank, &total);
+#pragma unroll
+    for (int u = 0; u < ENT_PER_LANE; ++u) {
+        if (!keep[u]) continue;
+        const int64_t j = lo + threadIdx.x + (int64_t)u * ROW_WG;
+        idx_out[base + rank[u]] = idx[j];
+        val_out[base + rank[u]] = val[j];
+    }
+}
+
+// Kept entries per 2048-entry block, inclusively scanned, into incl[0 .. nb).
+template <typename T>
+static int dz_block_prefix(hipStream_t s, int64_t nnz, const T* val, int64_t nb, int64_t* incl) {
+    hipLaunchKernelGGL((k_dz_block_sums<T>), dim3((unsigned)nb), dim3(ROW_WG), 0, s, nnz, val, incl);
+    GSA_HIP(hipGetLastError());
+    return inclusive_scan<int64_t>(s, incl, incl, nb, 0);
 }
 
 // ------------------------------------------------------------ COO helpers --
@@ -922,18 +995,26 @@ int gsa_csr_dropzeros_count(void* stream, int vtype, int itype, int64_t m, const
             GSA_HIP(hipStreamSynchronize(s));
         }
         const int64_t nnz = (int64_t)last - 1;
-        GSA_TRY(zero_fill(s, rowptr_out, sizeof(I) * (m + 1)));
-        if (m > 0 && nnz > 0) {
-            const int64_t nblocks = nnz / SPMV_NB + 1;
-            Scratch plan(s);
-            GSA_TRY(plan.alloc(sizeof(int64_t) * (nblocks + 1)));
-            GSA_TRY(build_csr_plan(s, itype, m, nnz, rowptr, plan.as<int64_t>(), nblocks));
-            hipLaunchKernelGGL((k_dz_count<T, I>), dim3((unsigned)nblocks), dim3(ROW_WG), 0, s, m, nnz,
-                               (const I*)rowptr, (const T*)nzval,
-                               (const int64_t*)plan.as<int64_t>(), (I*)rowptr_out + 1);
-            GSA_HIP(hipGetLastError());
+        if (m == 0 || nnz <= 0) {
+            GSA_TRY(zero_fill(s, rowptr_out, sizeof(I) * (m + 1)));
+            return counts_to_ptr<I>(s, (I*)rowptr_out, m, nnz_out);
         }
-        return counts_to_ptr<I>(s, (I*)rowptr_out, m, nnz_out);
+        const int64_t nblocks = nnz / SPMV_NB + 1;
+        Scratch plan(s), incl(s);
+        GSA_TRY(plan.alloc(sizeof(int64_t) * (nblocks + 1)));
+        GSA_TRY(incl.alloc(sizeof(int64_t) * nblocks));
+        GSA_TRY(build_csr_plan(s, itype, m, nnz, rowptr, plan.as<int64_t>(), nblocks));
+        GSA_TRY(dz_block_prefix<T>(s, nnz, (const T*)nzval, nblocks, incl.as<int64_t>()));
+        hipLaunchKernelGGL((k_dz_rowptr<T, I>), dim3((unsigned)nblocks), dim3(ROW_WG), 0, s, m, nnz,
+                           (const I*)rowptr, (const T*)nzval, (const int64_t*)plan.as<int64_t>(),
+                           (const int64_t*)incl.as<int64_t>(), (I*)rowptr_out);
+        GSA_HIP(hipGetLastError());
+        int64_t kept = 0;
+        GSA_HIP(hipMemcpyAsync(&kept, incl.as<int64_t>() + nblocks - 1, sizeof kept,
+                               hipMemcpyDeviceToHost, s));
+        GSA_HIP(hipStreamSynchronize(s));
+        *nnz_out = kept;
+        return (int)GSA_OK;
     });
 }
 
@@ -942,6 +1023,7 @@ int gsa_csr_dropzeros_fill(void* stream, int vtype, int itype, int64_t m, const 
                            void* colval_out, void* nzval_out) {
     GSA_TRY(ensure_init());
     GSA_CHECK(m >= 0, GSA_ERR_INVALID, "gsa_csr_dropzeros_fill: bad arguments");
+    (void)rowptr_out;  // each kept entry's place follows from the entries before it
     hipStream_t s = as_stream(stream);
     return dispatch_vai(vtype, itype, [&](auto tv, auto ti) {
         using T = decltype(tv);
@@ -953,13 +1035,12 @@ int gsa_csr_dropzeros_fill(void* stream, int vtype, int itype, int64_t m, const 
         const int64_t nnz = (int64_t)last - 1;
         if (nnz <= 0) return (int)GSA_OK;
         const int64_t nblocks = nnz / SPMV_NB + 1;
-        Scratch plan(s);
-        GSA_TRY(plan.alloc(sizeof(int64_t) * (nblocks + 1)));
-        GSA_TRY(build_csr_plan(s, itype, m, nnz, rowptr, plan.as<int64_t>(), nblocks));
-        hipLaunchKernelGGL((k_dz_fill<T, I>), dim3((unsigned)nblocks), dim3(ROW_WG), 0, s, m, nnz,
-                           (const I*)rowptr, (const I*)colval, (const T*)nzval,
-                           (const int64_t*)plan.as<int64_t>(), (const I*)rowptr_out, (I*)colval_out,
-                           (T*)nzval_out);
+        Scratch incl(s);
+        GSA_TRY(incl.alloc(sizeof(int64_t) * nblocks));
+        GSA_TRY(dz_block_prefix<T>(s, nnz, (const T*)nzval, nblocks, incl.as<int64_t>()));
+        hipLaunchKernelGGL((k_dz_compact<T, I>), dim3((unsigned)nblocks), dim3(ROW_WG), 0, s, nnz,
+                           (const I*)colval, (const T*)nzval, (const int64_t*)incl.as<int64_t>(),
+                           (I*)colval_out, (T*)nzval_out);
         GSA_HIP(hipGetLastError());
         return (int)GSA_OK;
     });

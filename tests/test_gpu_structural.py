@@ -277,6 +277,40 @@ def test_dropzeros_csr_csc_coo(dev):
     assert G.dropzeros(Z).nnz() == 0
 
 
+@pytest.mark.parametrize("itype", [torch.int64, torch.int32])
+@pytest.mark.parametrize("dtype", [np.float64, np.complex128])
+def test_dropzeros_block_edges(dev, itype, dtype):
+    """dropzeros runs over blocks of 2048 stored entries: here the entries are
+    exactly four blocks (so the plan has an empty fifth), the first block is all
+    zeros, a row starts on a block boundary, zero runs cross block edges inside
+    a 4096-entry row, and empty rows sit between and after, in CSR and CSC."""
+    rng = np.random.default_rng(25)
+    lens = np.array([2048, 1, 2047, 0, 0, 4096, 0, 0])
+    assert lens.sum() == 4 * 2048
+    ptr = np.concatenate([[1], 1 + np.cumsum(lens)]).astype(np.int64)
+    n = 4200
+    col = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in lens]).astype(np.int64) + 1
+    val = rng.standard_normal(col.size).astype(dtype)
+    if np.issubdtype(dtype, np.complexfloating):
+        val = val + 1j * rng.standard_normal(col.size)
+    val[:2048] = 0                                    # a block of zeros (row 1)
+    val[2048 + 1 + 100:2048 + 1 + 2047] = 0           # row 3 keeps its first 100
+    z = 4096 + np.arange(4096)
+    val[z[(z % 2048 > 2000) | (z % 2048 < 50)]] = 0   # runs across block edges in row 6
+    val[4096 + 7::13] = 0
+    for M in (from_arrays(G.GenericSparseMatrixCSR, lens.size, n, ptr, col, val, dev, itype),):
+        exp = O.csr_dropzeros(lens.size, ptr, col, val)
+        D = G.dropzeros(M)
+        same(csr_tuple(D), exp)
+        assert D.rowptr.dtype == itype
+        Mc = G.GenericSparseMatrixCSC(M)
+        Dc = G.dropzeros(Mc)
+        dense = np.zeros((lens.size, n), dtype=val.dtype)
+        dense[np.repeat(np.arange(lens.size), lens), col - 1] = val
+        np.testing.assert_array_equal(Dc.collect(), dense)
+        assert Dc.nnz() == exp[1].size
+
+
 # ------------------------------------------------------------ kron ---------
 def test_kron_golden(dev):
     d = g("kron_random.npz")
