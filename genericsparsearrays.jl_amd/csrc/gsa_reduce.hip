@@ -553,7 +553,7 @@ __global__ void k_add_dense_coo(int64_t m, int64_t n, int64_t nnz, const I* __re
     }
 }
 
-// Row-block plan over `ptr` for the visit_owned_rows kernels: the caller's
+// Row-block plan over `ptr` for the visit_owned_entries kernels: the caller's
 // plan when it was built on this pointer array, else one built into `tmp`.
 static int row_blocks(hipStream_t s, gsa_plan plan, int itype, int64_t n, int64_t nnz,
                       const void* ptr, Scratch& tmp, const int64_t** blk_row, int64_t* nblocks) {

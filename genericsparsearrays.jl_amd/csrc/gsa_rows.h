@@ -56,45 +56,6 @@ __device__ __forceinline__ T wg_exclusive_sum(T v, T* lds4, T* total) {
     return pre + x - v;
 }
 
-// Visit every row owned by this workgroup under a row-block plan
-// (blk_row, built by build_csr_plan over `ptr`).  Rows of length <= ROW_SHORT
-// go to short_fn(r, s, e) on one lane; longer rows go to long_fn(r, s, e),
-// called by ALL lanes of the workgroup together (it may synchronise).
-// s, e are 0-based entry positions.
-template <typename I, typename ShortFn, typename LongFn>
-__device__ __forceinline__ void visit_owned_rows(int64_t m, int64_t nnz, const I* __restrict__ ptr,
-                                                 const int64_t* __restrict__ blk_row,
-                                                 ShortFn short_fn, LongFn long_fn) {
-    __shared__ int64_t longs[ROW_MAX_LONG];
-    __shared__ int nlong;
-    const int64_t b = blockIdx.x;
-    const int64_t r0 = dclamp(blk_row[b], 0, m);
-    const int64_t r1 = dclamp(blk_row[b + 1], r0, m);
-    if (r0 >= r1) return;
-    if (threadIdx.x == 0) nlong = 0;
-    __syncthreads();
-    for (int64_t r = r0 + threadIdx.x; r < r1; r += ROW_WG) {
-        int64_t s = dclamp((int64_t)ptr[r] - 1, 0, nnz);
-        int64_t e = dclamp((int64_t)ptr[r + 1] - 1, s, nnz);
-        if (e - s <= ROW_SHORT) {
-            short_fn(r, s, e);
-        } else {
-            int slot = atomicAdd(&nlong, 1);
-            if (slot < ROW_MAX_LONG) longs[slot] = r;
-            else short_fn(r, s, e);  // unreachable for a valid plan
-        }
-    }
-    __syncthreads();
-    const int nl = nlong < ROW_MAX_LONG ? nlong : ROW_MAX_LONG;
-    for (int q = 0; q < nl; ++q) {
-        const int64_t r = longs[q];
-        int64_t s = dclamp((int64_t)ptr[r] - 1, 0, nnz);
-        int64_t e = dclamp((int64_t)ptr[r + 1] - 1, s, nnz);
-        long_fn(r, s, e);
-        __syncthreads();
-    }
-}
-
 // Entries per workgroup of visit_owned_entries: the row-block plan's block size.
 constexpr int ENT_CAP = SPMV_NB;
 constexpr int ENT_PER_LANE = ENT_CAP / ROW_WG;
