@@ -5,6 +5,9 @@ bench.py times is checked here on the same seeded inputs at the same scale.
   Float32 SpMV (Int32 and Int64 indices), SpMM k = 32 with column-major B and
   C, CSC and both COO storage orders as SpMV, transpose(CSR) and transpose(COO)
   SpMV, and the k = 32 SpMM of CSC, COO and transpose(CSR) (gather views).
+- C2 with every third diagonal entry added: tr, sum, dot(x, A, y) and A + 2I;
+  issymmetric on the 1000 x 1000 grid Laplacian; dropzeros of C2 with every
+  tenth stored value zero.
 - C3: R*R with R 200,000 x 200,000 at 8 entries per row (seed 3).
 - C4: the seed-4 batch of 4096 uneven Float32 instances, graph-replayed.
 - C5: the 520-matrix registry (512 uneven + 8 1M-row power-law), Float32 and
@@ -133,6 +136,46 @@ def test_c2_scatter_shaped_spmm_k32_full_size(dev, C2):
     refT = O.spmm_csr_T(H.m, H.n, H.rowptr, H.colval, H.nzval, Bt)
     bT = O.spmm_csr_T(H.m, H.n, H.rowptr, H.colval, np.abs(H.nzval), np.abs(Bt))
     assert_normwise(host(G.transpose(A) @ G.dense(Bt, device=dev)), refT, bT, 1e-12)
+
+
+def test_c2_reductions_and_dropzeros_full_size(dev, C2):
+    """The reduction and dropzeros workloads of bench.py at their sizes, through
+    the matrices' cached plans as timed there: sums within 1e-12 of the sum of
+    |term|, A + 2I and dropzeros index for index and bit for bit."""
+    import scipy.sparse as sp
+    S = C2.to_scipy()
+    d = np.zeros(C2.m)
+    d[::3] = 1.0
+    Hd = synth.HostCSR.from_scipy(sp.csr_matrix(S + sp.diags(d)))
+    A = dev_csr(Hd)
+    A.plan()
+    t, bound = O.trace_ptr(Hd.m, Hd.rowptr, Hd.colval, Hd.nzval)
+    assert abs(G.tr(A) - t) <= 1e-12 * bound
+    assert abs(G.sum(A) - Hd.nzval.sum()) <= 1e-12 * np.abs(Hd.nzval).sum()
+    rng = np.random.default_rng(5)
+    x, y = rng.standard_normal(Hd.m), rng.standard_normal(Hd.n)
+    ref, b2 = O.dot3_ptr(Hd.m, Hd.rowptr, Hd.colval, Hd.nzval, x, y, True)
+    got = G.dot(torch.as_tensor(x).to(dev), A, torch.as_tensor(y).to(dev))
+    assert abs(got - ref) <= 1e-12 * b2
+    rp, cv, nz = O.add_diag(Hd.m, Hd.rowptr, Hd.colval, Hd.nzval, 2.0)
+    B = A + 2.0 * G.I
+    assert np.array_equal(host(B.rowptr), rp) and np.array_equal(host(B.colval), cv)
+    assert np.array_equal(host(B.nzval), nz)
+    del A, B
+    L = synth.laplacian2d(1000)
+    dL = dev_csr(L)
+    dL.plan()
+    assert O.symmetric_ptr(L.m, L.rowptr, L.colval, L.nzval)
+    assert G.issymmetric(dL)
+    Z = C2.nzval.copy()
+    Z[::10] = 0
+    rp, cv, nz = O.csr_dropzeros(C2.m, C2.rowptr, C2.colval, Z)
+    Az = G.GenericSparseMatrixCSR(C2.m, C2.n, torch.as_tensor(C2.rowptr).to(dev),
+                                  torch.as_tensor(C2.colval).to(dev),
+                                  torch.as_tensor(Z).to(dev), copy=False)
+    D = G.dropzeros(Az)
+    assert np.array_equal(host(D.rowptr), rp) and np.array_equal(host(D.colval), cv)
+    assert np.array_equal(host(D.nzval), nz)
 
 
 def test_c3_spgemm_random_full_size(dev):
