@@ -35,6 +35,36 @@ __global__ __launch_bounds__(ROW_WG) void k_expand_rows(int64_t m, int64_t nnz,
         });
 }
 
+// CSR -> COO in one pass: each workgroup's 2048 entries read their column and
+// value with the plan's row bounds and store row, column and value together.
+template <typename T, typename I>
+struct ColVal {
+    I c;
+    T v;
+};
+template <typename T, typename I>
+__global__ __launch_bounds__(ROW_WG) void k_csr_to_coo(int64_t m, int64_t nnz,
+                                                       const I* __restrict__ ptr,
+                                                       const int64_t* __restrict__ blk_row,
+                                                       const I* __restrict__ colval,
+                                                       const T* __restrict__ nzval,
+                                                       I* __restrict__ rowind,
+                                                       I* __restrict__ colind,
+                                                       T* __restrict__ nzval_out) {
+    visit_owned_entries<I>(
+        m, nnz, ptr, blk_row, [](int64_t, int64_t, int64_t) {},
+        [&](int64_t j) { return ColVal<T, I>{colval[j], nzval[j]}; },
+        [&](const int64_t* r, const int64_t* j, const bool* ok, const ColVal<T, I>* p, int64_t) {
+#pragma unroll
+            for (int u = 0; u < ENT_PER_LANE; ++u) {
+                if (j[u] >= nnz) continue;
+                if (ok[u]) rowind[j[u]] = (I)(r[u] + 1);
+                colind[j[u]] = p[u].c;
+                nzval_out[j[u]] = p[u].v;
+            }
+        });
+}
+
 int expand_rows(hipStream_t s, int itype, int64_t m, int64_t nnz, const void* ptr, void* out) {
     if (m == 0 || nnz == 0) return GSA_OK;
     const int64_t nblocks = nnz / SPMV_NB + 1;
@@ -369,9 +399,19 @@ int gsa_csr_to_coo(void* stream, int vtype, int itype, int64_t m, int64_t nnz, c
         using T = decltype(tv);
         using I = decltype(ti);
         if (nnz == 0) return (int)GSA_OK;
-        GSA_TRY(expand_rows(s, itype, m, nnz, rowptr, rowind));
-        GSA_HIP(hipMemcpyAsync(colind, colval, sizeof(I) * nnz, hipMemcpyDeviceToDevice, s));
-        GSA_HIP(hipMemcpyAsync(nzval_out, nzval, sizeof(T) * nnz, hipMemcpyDeviceToDevice, s));
+        if (m == 0) {  // no row claims an entry: the arrays are copied as they are
+            GSA_HIP(hipMemcpyAsync(colind, colval, sizeof(I) * nnz, hipMemcpyDeviceToDevice, s));
+            GSA_HIP(hipMemcpyAsync(nzval_out, nzval, sizeof(T) * nnz, hipMemcpyDeviceToDevice, s));
+            return (int)GSA_OK;
+        }
+        const int64_t nblocks = nnz / SPMV_NB + 1;
+        Scratch plan(s);
+        GSA_TRY(plan.alloc(sizeof(int64_t) * (nblocks + 1)));
+        GSA_TRY(build_csr_plan(s, itype, m, nnz, rowptr, plan.as<int64_t>(), nblocks));
+        hipLaunchKernelGGL((k_csr_to_coo<T, I>), dim3((unsigned)nblocks), dim3(ROW_WG), 0, s, m, nnz,
+                           (const I*)rowptr, (const int64_t*)plan.as<int64_t>(), (const I*)colval,
+                           (const T*)nzval, (I*)rowind, (I*)colind, (T*)nzval_out);
+        GSA_HIP(hipGetLastError());
         return (int)GSA_OK;
     });
 }
