@@ -339,7 +339,11 @@ int gsa_coo_add_finish(gsa_ctx ctx, void* rowC, void* colC, void* valC);
 
 /* ========================== dropzeros ====================================== */
 /* Replaces SparseArrays.dropzeros(::CSR) src/matrix_csr/matrix_csr.jl:650-683,723-745
- * (kernels matrix_csr_kernels.jl:410-443); CSC is the same call on colptr. */
+ * (kernels matrix_csr_kernels.jl:410-443); CSC is the same call on colptr.
+ * Both phases read nnz from rowptr[m] (synchronises).  Phase 1 writes the result
+ * row pointer (m+1) and returns the kept count; phase 2 writes each kept entry
+ * at the number of kept entries before it (storage order), so rowptr_out, the
+ * pointer phase 1 wrote, is taken for the reference's signature and not read. */
 int gsa_csr_dropzeros_count(void* stream, int vtype, int itype, int64_t m,
                             const void* rowptr, const void* nzval,
                             void* rowptr_out, int64_t* nnz_out);
