@@ -215,7 +215,7 @@ __global__ __launch_bounds__(ROW_WG) void k_symmetric_ptr(int64_t n, int64_t nnz
     // G entries per lane searched in lock step: every round of the binary
     // searches issues its G index loads together.  Each search probes the
     // same positions as a search of its own would, so it finds the same entry.
-    constexpr int G = ENT_PER_LANE / 2;
+    constexpr int G = 2;  // 53 us on the 1M-row Laplacian; 54 with 4, 59 with 1, 68 with 8
     struct E {
         I c;
         T v;
@@ -392,7 +392,7 @@ __global__ __launch_bounds__(ROW_WG) void k_add_diag(int64_t n, int64_t nnz,
             if (s == e && ptr_out[r + 1] > ptr_out[r])
                 put((int64_t)ptr_out[r] - 1, (int64_t)ptr_out[r + 1] - 1, (I)(r + 1), lambda);
         },
-        pre, lane_rounds<ENT_PER_LANE / 4>(load, fetch, use));
+        pre, lane_rounds<1>(load, fetch, use));  // one entry per round: 75 us, 80 with two
 }
 
 __global__ void k_set_flag(int* f) { *f = 1; }
